@@ -1,0 +1,180 @@
+"""ctypes binding of libcdbg_gpu.so (include/cdbg_gpu.h) for tests and bench.
+
+The product is the C ABI; this module only moves numpy arrays across it.
+Loading fails loudly when the HIP library has not been built: there is no
+CPU fallback on the product path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcdbg_gpu.so")
+
+U8P = C.POINTER(C.c_uint8)
+U32P = C.POINTER(C.c_uint32)
+U64P = C.POINTER(C.c_uint64)
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "num_bases", "num_supers", "num_occ", "num_tuples", "num_nodes", "num_kmers",
+        "num_color_sets", "num_frags", "frag_bytes", "cs_bits", "total_integers")] + [
+        ("ms_phase1", C.c_double), ("ms_phase2", C.c_double)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class SynthParams(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("num_colors", C.c_uint32), ("genome_len", C.c_uint64),
+                ("snp_rate", C.c_double), ("indel_rate", C.c_double), ("indel_max", C.c_uint32),
+                ("pav_frac", C.c_double), ("pav_min", C.c_uint32), ("pav_max", C.c_uint32),
+                ("n_inversions", C.c_uint32), ("inv_min", C.c_uint32), ("inv_max", C.c_uint32)]
+
+
+# Exported symbols declared in include/cdbg_gpu.h (checked by tests/test_abi.py).
+_SIGS = {
+    "cdbg_gpu_last_error": (C.c_char_p, []),
+    "cdbg_gpu_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "cdbg_gpu_ctx_destroy": (None, [C.c_void_p]),
+    "cdbg_gpu_job_create": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint32, C.POINTER(C.c_void_p)]),
+    "cdbg_gpu_job_destroy": (None, [C.c_void_p]),
+    "cdbg_gpu_job_phase1": (C.c_int, [C.c_void_p]),
+    "cdbg_gpu_job_phase2": (C.c_int, [C.c_void_p]),
+    "cdbg_gpu_job_run": (C.c_int, [C.c_void_p]),
+    "cdbg_gpu_job_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
+    "cdbg_gpu_job_supers": (C.c_int, [C.c_void_p, C.c_int, U64P, U32P, U8P, U32P, U32P]),
+    "cdbg_gpu_job_bucket_offsets": (C.c_int, [C.c_void_p, U64P]),
+    "cdbg_gpu_job_frags": (C.c_int, [C.c_void_p, U8P]),
+    "cdbg_gpu_job_color_sets": (C.c_int, [C.c_void_p, U64P, U64P]),
+    "cdbg_synth_create": (C.c_void_p, [C.POINTER(SynthParams), C.c_uint32]),
+    "cdbg_synth_total_bytes": (C.c_uint64, [C.c_void_p]),
+    "cdbg_synth_bytes": (U8P, [C.c_void_p]),
+    "cdbg_synth_offsets": (U64P, [C.c_void_p]),
+    "cdbg_synth_free": (None, [C.c_void_p]),
+    "cdbg_synth_write_fasta": (C.c_int, [C.c_void_p, C.c_uint32, C.c_char_p, C.c_int]),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+
+
+def synth(seed, num_colors, genome_len, snp_rate=0.01, indel_rate=0.001, indel_max=10,
+          pav_frac=0.0, pav_min=1000, pav_max=5000, n_inversions=0, inv_min=10000,
+          inv_max=100000, threads=8):
+    """Generate a synthetic pangenome; returns (bytes u8 array, color_off u64 array)."""
+    p = SynthParams(seed, num_colors, genome_len, snp_rate, indel_rate, indel_max, pav_frac,
+                    pav_min, pav_max, n_inversions, inv_min, inv_max)
+    h = lib().cdbg_synth_create(C.byref(p), threads)
+    if not h:
+        raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+    try:
+        n = lib().cdbg_synth_total_bytes(h)
+        data = np.ctypeslib.as_array(lib().cdbg_synth_bytes(h), shape=(n,)).copy()
+        off = np.ctypeslib.as_array(lib().cdbg_synth_offsets(h), shape=(num_colors + 1,)).copy()
+    finally:
+        lib().cdbg_synth_free(h)
+    return data, off
+
+
+class Context:
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        _check(lib().cdbg_gpu_ctx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().cdbg_gpu_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def job(self, data, color_off, k, m, num_buckets):
+        return Job(self, data, color_off, k, m, num_buckets)
+
+
+class Job:
+    def __init__(self, ctx, data, color_off, k, m, num_buckets):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        off = np.ascontiguousarray(color_off, dtype=np.uint64)
+        self.num_buckets = num_buckets
+        self.h = C.c_void_p()
+        _check(lib().cdbg_gpu_job_create(ctx.h, _ptr(data, U8P), _ptr(off, U64P), len(off) - 1,
+                                         k, m, num_buckets, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().cdbg_gpu_job_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def phase1(self):
+        _check(lib().cdbg_gpu_job_phase1(self.h))
+
+    def phase2(self):
+        _check(lib().cdbg_gpu_job_phase2(self.h))
+
+    def run(self):
+        _check(lib().cdbg_gpu_job_run(self.h))
+
+    def stats(self):
+        s = Stats()
+        _check(lib().cdbg_gpu_job_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def supers(self, which=0):
+        n = self.stats()["num_supers"]
+        start = np.zeros(n, np.uint64)
+        ln = np.zeros(n, np.uint32)
+        fl = np.zeros(n, np.uint8)
+        bk = np.zeros(n, np.uint32)
+        col = np.zeros(n, np.uint32)
+        _check(lib().cdbg_gpu_job_supers(self.h, which, _ptr(start, U64P), _ptr(ln, U32P),
+                                         _ptr(fl, U8P), _ptr(bk, U32P), _ptr(col, U32P)))
+        return {"start": start, "len": ln, "flags": fl, "bucket": bk, "color": col}
+
+    def bucket_offsets(self):
+        out = np.zeros(self.num_buckets + 1, np.uint64)
+        _check(lib().cdbg_gpu_job_bucket_offsets(self.h, _ptr(out, U64P)))
+        return out
+
+    def frags(self):
+        n = self.stats()["frag_bytes"]
+        out = np.zeros(n, np.uint8)
+        _check(lib().cdbg_gpu_job_frags(self.h, _ptr(out, U8P)))
+        return out
+
+    def color_sets(self):
+        s = self.stats()
+        words = np.zeros((s["cs_bits"] + 63) // 64, np.uint64)
+        offs = np.zeros(s["num_color_sets"], np.uint64)
+        _check(lib().cdbg_gpu_job_color_sets(self.h, _ptr(words, U64P), _ptr(offs, U64P)))
+        return words, offs

@@ -1,0 +1,170 @@
+// C ABI entry points (include/cdbg_gpu.h).  Exceptions never cross this
+// boundary: each entry point converts them into a status code plus the
+// thread-local message returned by cdbg_gpu_last_error().
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "cdbg_gpu.h"
+#include "common.h"
+#include "job.h"
+
+namespace cdbg_amd {
+
+static thread_local std::string g_last_error;
+
+int set_error(const std::string& msg) {
+    g_last_error = msg;
+    return -1;
+}
+const char* last_error() { return g_last_error.c_str(); }
+
+Job::~Job() {
+    free_phase2(*this);
+    if (st) (void)hipStreamDestroy(st);
+}
+
+}  // namespace cdbg_amd
+
+using namespace cdbg_amd;
+
+#define CDBG_TRY(body)                          \
+    try {                                       \
+        body;                                   \
+        return 0;                               \
+    } catch (std::exception & e) {              \
+        return set_error(e.what());             \
+    } catch (...) {                             \
+        return set_error("unknown C++ exception"); \
+    }
+
+extern "C" {
+
+CDBG_API const char* cdbg_gpu_last_error(void) { return last_error(); }
+
+CDBG_API int cdbg_gpu_ctx_create(int device, cdbg_gpu_ctx** out) {
+    CDBG_TRY({
+        if (!out) throw Error("null output pointer");
+        int n = 0;
+        CDBG_HIP(hipGetDeviceCount(&n));
+        if (device < 0 || device >= n) throw Error("no HIP device " + std::to_string(device));
+        CDBG_HIP(hipSetDevice(device));
+        auto* c = new cdbg_gpu_ctx();
+        c->device = device;
+        *out = c;
+    })
+}
+
+CDBG_API void cdbg_gpu_ctx_destroy(cdbg_gpu_ctx* ctx) { delete ctx; }
+
+CDBG_API int cdbg_gpu_job_create(cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off,
+                                 uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets,
+                                 cdbg_gpu_job** out) {
+    CDBG_TRY({
+        if (!ctx || !out || !color_off || (!bytes && num_colors)) throw Error("null argument");
+        if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
+        if (m < 2 || m >= k) throw Error("m must satisfy 2 <= m < k");
+        if (num_buckets == 0) throw Error("num_buckets must be >= 1");
+        if (num_colors == 0) throw Error("no colours");
+        for (uint32_t c = 0; c < num_colors; ++c)
+            if (color_off[c + 1] < color_off[c]) throw Error("colour offsets must be non-decreasing");
+        CDBG_HIP(hipSetDevice(ctx->device));
+        auto* h = new cdbg_gpu_job();
+        Job& J = h->J;
+        try {
+            J.ctx = ctx;
+            J.k = k;
+            J.m = m;
+            J.B = num_buckets;
+            J.num_colors = num_colors;
+            J.n_bytes = color_off[num_colors] - color_off[0];
+            // Pad with a non-ACGT byte so tiles and 32-byte groups need no tail case.
+            J.n_pad = ((J.n_bytes + 64) / 64) * 64 + 64;
+            CDBG_HIP(hipStreamCreateWithFlags(&J.st, hipStreamNonBlocking));
+            uint8_t* d = J.bytes.ensure<uint8_t>(J.n_pad);
+            CDBG_HIP(hipMemsetAsync(d + J.n_bytes, '\n', J.n_pad - J.n_bytes, J.st));
+            CDBG_HIP(hipMemcpyAsync(d, bytes + color_off[0], J.n_bytes, hipMemcpyHostToDevice, J.st));
+            std::vector<uint64_t> off(num_colors + 1);
+            for (uint32_t c = 0; c <= num_colors; ++c) off[c] = color_off[c] - color_off[0];
+            uint64_t* doff = J.color_off.ensure<uint64_t>(num_colors + 1);
+            CDBG_HIP(hipMemcpyAsync(doff, off.data(), off.size() * 8, hipMemcpyHostToDevice, J.st));
+            CDBG_HIP(hipStreamSynchronize(J.st));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    })
+}
+
+CDBG_API void cdbg_gpu_job_destroy(cdbg_gpu_job* job) { delete job; }
+
+CDBG_API int cdbg_gpu_job_phase1(cdbg_gpu_job* job) {
+    CDBG_TRY({
+        if (!job) throw Error("null job");
+        CDBG_HIP(hipSetDevice(job->J.ctx->device));
+        job->J.phase2_done = false;
+        run_phase1(job->J);
+    })
+}
+
+CDBG_API int cdbg_gpu_job_phase2(cdbg_gpu_job* job) {
+    CDBG_TRY({
+        if (!job) throw Error("null job");
+        if (!job->J.phase1_done) throw Error("phase 2 requires phase 1");
+        CDBG_HIP(hipSetDevice(job->J.ctx->device));
+        run_phase2(job->J);
+    })
+}
+
+CDBG_API int cdbg_gpu_job_run(cdbg_gpu_job* job) {
+    int rc = cdbg_gpu_job_phase1(job);
+    if (rc) return rc;
+    return cdbg_gpu_job_phase2(job);
+}
+
+CDBG_API int cdbg_gpu_job_stats(cdbg_gpu_job* job, cdbg_gpu_stats* out) {
+    CDBG_TRY({
+        if (!job || !out) throw Error("null argument");
+        *out = job->J.stats;
+    })
+}
+
+CDBG_API int cdbg_gpu_job_supers(cdbg_gpu_job* job, int which, uint64_t* start, uint32_t* len, uint8_t* flags,
+                                 uint32_t* bucket, uint32_t* color) {
+    CDBG_TRY({
+        if (!job) throw Error("null job");
+        Job& J = job->J;
+        if (!J.phase1_done) throw Error("phase 1 has not run");
+        const uint64_t n = J.n_supers;
+        Phase1Bufs& P = J.p1;
+        auto cp = [&](void* dst, const DevBuf& src, size_t elem) {
+            if (dst && n) CDBG_HIP(hipMemcpyAsync(dst, src.p, n * elem, hipMemcpyDeviceToHost, J.st));
+        };
+        if (which == 0) {
+            cp(start, P.sk_start, 8);
+            cp(len, P.sk_len, 4);
+            cp(flags, P.sk_flags, 1);
+            cp(bucket, P.sk_bucket, 4);
+            cp(color, P.sk_color, 4);
+        } else {
+            cp(start, P.b_start, 8);
+            cp(len, P.b_len, 4);
+            cp(flags, P.b_flags, 1);
+            cp(bucket, P.key, 4);
+            cp(color, P.b_color, 4);
+        }
+        CDBG_HIP(hipStreamSynchronize(J.st));
+    })
+}
+
+CDBG_API int cdbg_gpu_job_bucket_offsets(cdbg_gpu_job* job, uint64_t* out) {
+    CDBG_TRY({
+        if (!job || !out) throw Error("null argument");
+        if (!job->J.phase1_done) throw Error("phase 1 has not run");
+        std::memcpy(out, job->J.h_bucket_off.data(), job->J.h_bucket_off.size() * 8);
+    })
+}
+
+}  // extern "C"

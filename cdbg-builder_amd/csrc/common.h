@@ -1,0 +1,26 @@
+// Shared host-side helpers: thread-local error reporting and HIP checks.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace cdbg_amd {
+
+// Records `msg` as this thread's last error; returns the generic failure code.
+int set_error(const std::string& msg);
+const char* last_error();
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+}  // namespace cdbg_amd
+
+#define CDBG_HIP(call)                                                                      \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::cdbg_amd::Error(std::string(#call) + ": " + hipGetErrorString(e_) + " (" + \
+                                    __FILE__ + ":" + std::to_string(__LINE__) + ")");      \
+    } while (0)

@@ -1,0 +1,57 @@
+// One build job resident in HBM: inputs, phase-1 records, phase-2 results.
+// Buffers only grow, so repeated runs of the same job allocate nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "cdbg_gpu.h"
+#include "prims.h"
+
+namespace cdbg_amd {
+
+struct Phase1Bufs {
+    DevBuf packed, valid;              // 2-bit bases (16 per u32), ACGT bitmap (32 per u32)
+    DevBuf tile_cnt, tile_off, totals; // per-tile start/end counts and their scans
+    DevBuf s_pos, s_bucket, s_flags;   // super starts (position order)
+    DevBuf e_end, e_flags;             // super ends (position order)
+    DevBuf sk_start, sk_len, sk_flags, sk_bucket, sk_color;  // supers, position order
+    DevBuf key, val, key_t, val_t;     // bucket sort
+    DevBuf b_start, b_len, b_flags, b_color;                // supers, bucket-major
+    DevBuf bucket_off;                 // B+1
+};
+
+struct Phase2Bufs;  // defined in phase2.hip
+
+struct Job {
+    cdbg_gpu_ctx* ctx = nullptr;
+    hipStream_t st = nullptr;
+    uint32_t k = 0, m = 0, B = 0, num_colors = 0;
+    uint64_t n_bytes = 0;        // input bytes (unpadded)
+    uint64_t n_pad = 0;          // padded to a multiple of 64
+    DevBuf bytes, color_off;
+    Phase1Bufs p1;
+    RadixScratch rs;
+    Phase2Bufs* p2 = nullptr;
+    bool phase1_done = false, phase2_done = false;
+    uint64_t n_supers = 0;
+    std::vector<uint64_t> h_bucket_off;  // host copy (B+1)
+    cdbg_gpu_stats stats{};
+    ~Job();
+};
+
+void run_phase1(Job& J);
+void run_phase2(Job& J);
+void free_phase2(Job& J);
+
+}  // namespace cdbg_amd
+
+struct cdbg_gpu_ctx {
+    int device = 0;
+};
+
+struct cdbg_gpu_job {
+    cdbg_amd::Job J;
+};

@@ -1,0 +1,397 @@
+// Phase 1 (bucket-write) on the device.
+//
+//   k_pack2bit      ASCII -> 2-bit codes (16 per u32, base i at bits 2(i%16),
+//                   i.e. the byte packing of super_kmer.hpp:43-49 read as LE
+//                   words) + an ACGT bitmap; nuc_to_2bit semantics
+//                   (kmer.hpp:35-52).
+//   k_superkmers    per tile of 4096 (k-1)-mer positions: canonical ntHash of
+//                   every m-mer (minimizer.hpp:40-62), window minimum over the
+//                   W = k-m m-mers of every (k-1)-mer (minimizer.hpp:94-157,
+//                   bucket_ingester.hpp:52), and the super-k-mer starts/ends
+//                   with their BEGIN/END/OWNS flags (bucket_ingester.hpp:80-126).
+//                   Run twice: count, then write at scanned offsets.
+//   k_zip_supers    pairs the i-th start with the i-th end (they alternate
+//                   along every ACGT run), assigns the colour.
+//   bucket scatter  stable radix sort of super indices by bucket
+//                   (h >> 1) % B (bucket_ingester.hpp:67) + gather.
+//
+// A super-k-mer is described by [start, start+len) in the job's base
+// coordinates, its 4 flag bits, its bucket and its colour.  Inside a bucket
+// the supers stay in position order, which is colour-major: this is the
+// record stream phase 2 consumes.
+#include <hip/hip_runtime.h>
+
+#include "job.h"
+
+namespace cdbg_amd {
+
+namespace {
+
+constexpr int SK_THREADS = 256;
+constexpr int SK_PER_THREAD = 16;
+constexpr int SK_TILE = SK_THREADS * SK_PER_THREAD;  // (k-1)-mer positions per tile
+constexpr int SK_NPK = SK_TILE / 16 + 16;            // packed words staged per tile
+constexpr int SK_NVD = SK_TILE / 32 + 16;            // validity words staged per tile
+
+// ntHash per-base seeds: the published ntHash table (minimizer.hpp:21-26).
+constexpr uint64_t NT_SEED0 = 0x3c8bfbb395c60474ULL;
+constexpr uint64_t NT_SEED1 = 0x3193c18562a02b4cULL;
+constexpr uint64_t NT_SEED2 = 0x20323ed082572324ULL;
+constexpr uint64_t NT_SEED3 = 0x295549f54be24456ULL;
+
+__device__ __forceinline__ uint64_t rotl(uint64_t x, uint32_t r) {
+    r &= 63u;
+    return r ? (x << r) | (x >> (64u - r)) : x;
+}
+
+__device__ __forceinline__ uint64_t sel4(uint32_t b, uint64_t a0, uint64_t a1, uint64_t a2, uint64_t a3) {
+    uint64_t lo = (b & 1u) ? a1 : a0;
+    uint64_t hi = (b & 1u) ? a3 : a2;
+    return (b & 2u) ? hi : lo;
+}
+
+struct HashTabs {
+    uint64_t s[4];    // S[b]
+    uint64_t sm[4];   // rol(S[b], m)          (leaving base, forward strand)
+    uint64_t rc1[4];  // ror(S[b^3], 1)        (leaving base, reverse strand)
+    uint64_t rcm[4];  // rol(S[b^3], m-1)      (entering base, reverse strand)
+};
+
+__global__ __launch_bounds__(256) void k_pack2bit(const uint8_t* __restrict__ bytes, uint64_t ngroups,
+                                                  uint32_t* __restrict__ packed, uint32_t* __restrict__ valid) {
+    const uint64_t g = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint4* src = reinterpret_cast<const uint4*>(bytes + g * 32);
+    const uint4 a = src[0], b = src[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t p0 = 0, p1 = 0, v = 0;
+#pragma unroll
+    for (int i = 0; i < 32; ++i) {
+        const uint32_t c = ((w[i >> 2] >> (8 * (i & 3))) & 0xffu) | 0x20u;  // fold case
+        const bool ok = (c == 'a') | (c == 'c') | (c == 'g') | (c == 't');
+        // a->0, c->1, g->2, t->3 from the ASCII bits of the lower-case letter
+        const uint32_t code = ok ? (((c >> 1) & 3u) ^ ((c >> 2) & 1u)) : 0u;
+        if (i < 16)
+            p0 |= code << (2 * i);
+        else
+            p1 |= code << (2 * (i - 16));
+        v |= uint32_t(ok) << i;
+    }
+    packed[2 * g] = p0;
+    packed[2 * g + 1] = p1;
+    valid[g] = v;
+}
+
+struct SkArgs {
+    const uint32_t* packed;
+    const uint32_t* valid;
+    uint64_t n_bases;
+    uint64_t n_pk_words, n_vd_words;
+    uint32_t k, m, W, B;
+    HashTabs T;
+    uint32_t* cnt_s;     // per-tile start counts (count pass)
+    uint32_t* cnt_e;     // per-tile end counts
+    const uint64_t* off_s;  // scanned (write pass)
+    const uint64_t* off_e;
+    uint64_t* s_pos;
+    uint32_t* s_bucket;
+    uint8_t* s_flags;
+    uint64_t* e_end;
+    uint8_t* e_flags;
+};
+
+template <bool WRITE>
+__global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
+    __shared__ uint64_t s_h[SK_TILE + 72];
+    __shared__ uint64_t s_mn[SK_TILE + 2];
+    __shared__ uint32_t s_pk[SK_NPK];
+    __shared__ uint32_t s_vd[SK_NVD];
+    __shared__ uint64_t s_tmp[SK_THREADS / 64 + 1];
+
+    const int64_t p0 = int64_t(blockIdx.x) * SK_TILE;
+    const int64_t pk_w0 = p0 / 16 - 1;  // first staged packed word
+    const int64_t vd_w0 = p0 / 32 - 1;  // first staged validity word
+    for (int i = threadIdx.x; i < SK_NPK; i += SK_THREADS) {
+        const int64_t w = pk_w0 + i;
+        s_pk[i] = (w >= 0 && uint64_t(w) < a.n_pk_words) ? a.packed[w] : 0u;
+    }
+    for (int i = threadIdx.x; i < SK_NVD; i += SK_THREADS) {
+        const int64_t w = vd_w0 + i;
+        s_vd[i] = (w >= 0 && uint64_t(w) < a.n_vd_words) ? a.valid[w] : 0u;
+    }
+    __syncthreads();
+
+    const uint32_t m = a.m, W = a.W;
+    auto base_at = [&](int64_t p) -> uint32_t {  // p >= pk_w0*16
+        const int64_t lp = p - pk_w0 * 16;
+        return (s_pk[lp >> 4] >> (2 * (lp & 15))) & 3u;
+    };
+
+    // 1. canonical m-mer hashes for positions p0-1 .. p0-1+SK_TILE+W
+    const int nq = SK_TILE + int(W) + 1;
+    for (int q0 = threadIdx.x * 16; q0 < nq; q0 += SK_THREADS * 16) {
+        const int cnt = min(16, nq - q0);
+        const int64_t pos = p0 - 1 + q0;
+        uint64_t f = 0, r = 0;
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t b = base_at(pos + i);
+            f = rotl(f, 1) ^ sel4(b, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]);
+            r ^= rotl(sel4(b ^ 3u, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]), i);
+        }
+        s_h[q0] = f <= r ? f : r;
+        for (int i = 1; i < cnt; ++i) {
+            const uint32_t bo = base_at(pos + i - 1);
+            const uint32_t bi = base_at(pos + i + m - 1);
+            f = rotl(f, 1) ^ sel4(bo, a.T.sm[0], a.T.sm[1], a.T.sm[2], a.T.sm[3]) ^
+                sel4(bi, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]);
+            r = rotl(r, 63) ^ sel4(bo, a.T.rc1[0], a.T.rc1[1], a.T.rc1[2], a.T.rc1[3]) ^
+                sel4(bi, a.T.rcm[0], a.T.rcm[1], a.T.rcm[2], a.T.rcm[3]);
+            s_h[q0 + i] = f <= r ? f : r;
+        }
+    }
+    __syncthreads();
+    // 2. window minimum of every (k-1)-mer p0-1 .. p0+SK_TILE
+    for (int q = threadIdx.x; q < SK_TILE + 2; q += SK_THREADS) {
+        uint64_t best = ~0ull;
+        for (uint32_t i = 0; i < W; ++i) {
+            const uint64_t h = s_h[q + i];
+            best = h < best ? h : best;
+        }
+        s_mn[q] = best;
+    }
+    __syncthreads();
+
+    // 3. (k-1)-mer validity and super-k-mer boundaries
+    const uint64_t kmask = (uint64_t(1) << (a.k - 1)) - 1;
+    auto valid_k1 = [&](int64_t j) -> bool {
+        if (j < 0) return false;
+        const int64_t lb = j - vd_w0 * 32;
+        const int w = int(lb >> 5), o = int(lb & 31);
+        const uint64_t lo = uint64_t(s_vd[w]) | (uint64_t(s_vd[w + 1]) << 32);
+        uint64_t win = lo >> o;
+        if (o) win |= uint64_t(s_vd[w + 2]) << (64 - o);
+        return (win & kmask) == kmask;
+    };
+    const int64_t j0 = p0 + int64_t(threadIdx.x) * SK_PER_THREAD;
+    uint32_t starts = 0, ends = 0;  // bit i: position j0+i starts / ends a super
+    bool vprev = valid_k1(j0 - 1);
+    bool vcur = valid_k1(j0);
+#pragma unroll
+    for (int i = 0; i < SK_PER_THREAD; ++i) {
+        const int64_t j = j0 + i;
+        const bool vnext = valid_k1(j + 1);
+        if (vcur && (vprev || vnext) && uint64_t(j) < a.n_bases) {
+            const int q = int(j - (p0 - 1));
+            if (!vprev || s_mn[q] != s_mn[q - 1]) starts |= 1u << i;
+            if (!vnext || s_mn[q] != s_mn[q + 1]) ends |= 1u << i;
+        }
+        vprev = vcur;
+        vcur = vnext;
+    }
+    const uint32_t ns = __popc(starts), ne = __popc(ends);
+    if (!WRITE) {
+        uint64_t ts, te;
+        (void)block_exclusive_scan<uint64_t>(uint64_t(ns), s_tmp, ts);
+        (void)block_exclusive_scan<uint64_t>(uint64_t(ne), s_tmp, te);
+        if (threadIdx.x == 0) {
+            a.cnt_s[blockIdx.x] = uint32_t(ts);
+            a.cnt_e[blockIdx.x] = uint32_t(te);
+        }
+        return;
+    }
+    uint64_t ts, te;
+    uint64_t os = block_exclusive_scan<uint64_t>(uint64_t(ns), s_tmp, ts) + a.off_s[blockIdx.x];
+    uint64_t oe = block_exclusive_scan<uint64_t>(uint64_t(ne), s_tmp, te) + a.off_e[blockIdx.x];
+    for (int i = 0; i < SK_PER_THREAD; ++i) {
+        const int64_t j = j0 + i;
+        const int q = int(j - (p0 - 1));
+        if (starts & (1u << i)) {
+            const bool begin = !valid_k1(j - 1);
+            const bool own_first = begin || s_mn[q] < s_mn[q - 1];
+            a.s_pos[os] = uint64_t(begin ? j : j - 1);
+            a.s_bucket[os] = uint32_t((s_mn[q] >> 1) % a.B);
+            a.s_flags[os] = uint8_t((begin ? 1u : 0u) | (own_first ? 4u : 0u));
+            ++os;
+        }
+        if (ends & (1u << i)) {
+            const bool end = !valid_k1(j + 1);
+            const bool own_last = end || s_mn[q] < s_mn[q + 1];
+            a.e_end[oe] = uint64_t(end ? j - 1 : j) + a.k;
+            a.e_flags[oe] = uint8_t((end ? 2u : 0u) | (own_last ? 8u : 0u));
+            ++oe;
+        }
+    }
+}
+
+__global__ void k_zip_supers(uint64_t n, const uint64_t* __restrict__ s_pos, const uint32_t* __restrict__ s_bucket,
+                             const uint8_t* __restrict__ s_flags, const uint64_t* __restrict__ e_end,
+                             const uint8_t* __restrict__ e_flags, const uint64_t* __restrict__ color_off,
+                             uint32_t num_colors, uint64_t* __restrict__ sk_start, uint32_t* __restrict__ sk_len,
+                             uint8_t* __restrict__ sk_flags, uint32_t* __restrict__ sk_bucket,
+                             uint32_t* __restrict__ sk_color) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t s = s_pos[i];
+    // colour = last c with color_off[c] <= s
+    uint32_t lo = 0, hi = num_colors;  // invariant: color_off[lo] <= s < color_off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (color_off[mid] <= s)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    sk_start[i] = s;
+    sk_len[i] = uint32_t(e_end[i] - s);
+    sk_flags[i] = uint8_t(s_flags[i] | e_flags[i]);
+    sk_bucket[i] = s_bucket[i];
+    sk_color[i] = lo;
+}
+
+__global__ void k_iota_copy(uint64_t n, const uint32_t* __restrict__ bucket, uint32_t* __restrict__ key,
+                            uint32_t* __restrict__ val) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = bucket[i];
+    val[i] = uint32_t(i);
+}
+
+__global__ void k_gather_supers(uint64_t n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ sk_start,
+                                const uint32_t* __restrict__ sk_len, const uint8_t* __restrict__ sk_flags,
+                                const uint32_t* __restrict__ sk_color, uint64_t* __restrict__ b_start,
+                                uint32_t* __restrict__ b_len, uint8_t* __restrict__ b_flags,
+                                uint32_t* __restrict__ b_color) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t s = perm[i];
+    b_start[i] = sk_start[s];
+    b_len[i] = sk_len[s];
+    b_flags[i] = sk_flags[s];
+    b_color[i] = sk_color[s];
+}
+
+// off[b] = first index i with key[i] >= b, for b in [0, B].
+__global__ void k_bucket_offsets(uint64_t n, const uint32_t* __restrict__ key, uint32_t B,
+                                 uint64_t* __restrict__ off) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n == 0) {
+        if (i <= B) off[i] = 0;
+        return;
+    }
+    if (i >= n) return;
+    const int64_t prev = i == 0 ? -1 : int64_t(key[i - 1]);
+    const int64_t cur = key[i];
+    for (int64_t b = prev + 1; b <= cur; ++b) off[b] = i;
+    if (i == n - 1)
+        for (int64_t b = cur + 1; b <= int64_t(B); ++b) off[b] = n;
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+
+}  // namespace
+
+void run_phase1(Job& J) {
+    hipStream_t st = J.st;
+    Phase1Bufs& P = J.p1;
+    hipEvent_t e0, e1;
+    CDBG_HIP(hipEventCreate(&e0));
+    CDBG_HIP(hipEventCreate(&e1));
+    CDBG_HIP(hipEventRecord(e0, st));
+
+    const uint64_t ngroups = J.n_pad / 32;
+    uint32_t* packed = P.packed.ensure<uint32_t>(ngroups * 2);
+    uint32_t* valid = P.valid.ensure<uint32_t>(ngroups);
+    k_pack2bit<<<blocks_for(ngroups, 256), 256, 0, st>>>(J.bytes.as<uint8_t>(), ngroups, packed, valid);
+
+    SkArgs a{};
+    a.packed = packed;
+    a.valid = valid;
+    a.n_bases = J.n_bytes;
+    a.n_pk_words = ngroups * 2;
+    a.n_vd_words = ngroups;
+    a.k = J.k;
+    a.m = J.m;
+    a.W = J.k - J.m;
+    a.B = J.B;
+    const uint64_t S[4] = {NT_SEED0, NT_SEED1, NT_SEED2, NT_SEED3};
+    auto hrot = [](uint64_t x, uint32_t r) { r &= 63; return r ? (x << r) | (x >> (64 - r)) : x; };
+    for (int b = 0; b < 4; ++b) {
+        a.T.s[b] = S[b];
+        a.T.sm[b] = hrot(S[b], J.m);
+        a.T.rc1[b] = hrot(S[b ^ 3], 63);
+        a.T.rcm[b] = hrot(S[b ^ 3], J.m - 1);
+    }
+    const uint64_t ntiles = (J.n_bytes + SK_TILE - 1) / SK_TILE;
+    if (ntiles >= (1ull << 31)) throw Error("input too large for one job");
+    uint32_t* cnt = P.tile_cnt.ensure<uint32_t>(2 * ntiles);
+    uint64_t* off = P.tile_off.ensure<uint64_t>(2 * ntiles);
+    uint64_t* tot = P.totals.ensure<uint64_t>(2);
+    a.cnt_s = cnt;
+    a.cnt_e = cnt + ntiles;
+    k_superkmers<false><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a);
+    exclusive_scan<uint32_t, uint64_t>(cnt, off, ntiles, tot, J.rs.scan_tmp, st);
+    exclusive_scan<uint32_t, uint64_t>(cnt + ntiles, off + ntiles, ntiles, tot + 1, J.rs.scan_tmp, st);
+    uint64_t h_tot[2];
+    CDBG_HIP(hipMemcpyAsync(h_tot, tot, sizeof h_tot, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    if (h_tot[0] != h_tot[1]) throw Error("phase 1: super-k-mer starts and ends do not pair up");
+    const uint64_t n = h_tot[0];
+    J.n_supers = n;
+    if (n >= (1ull << 32)) throw Error("phase 1: more than 2^32 super-k-mers in one job");
+    a.off_s = off;
+    a.off_e = off + ntiles;
+    a.s_pos = P.s_pos.ensure<uint64_t>(n);
+    a.s_bucket = P.s_bucket.ensure<uint32_t>(n);
+    a.s_flags = P.s_flags.ensure<uint8_t>(n);
+    a.e_end = P.e_end.ensure<uint64_t>(n);
+    a.e_flags = P.e_flags.ensure<uint8_t>(n);
+    k_superkmers<true><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a);
+
+    uint64_t* sk_start = P.sk_start.ensure<uint64_t>(n);
+    uint32_t* sk_len = P.sk_len.ensure<uint32_t>(n);
+    uint8_t* sk_flags = P.sk_flags.ensure<uint8_t>(n);
+    uint32_t* sk_bucket = P.sk_bucket.ensure<uint32_t>(n);
+    uint32_t* sk_color = P.sk_color.ensure<uint32_t>(n);
+    if (n) {
+        k_zip_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, a.s_pos, a.s_bucket, a.s_flags, a.e_end, a.e_flags,
+                                                         J.color_off.as<uint64_t>(), J.num_colors, sk_start,
+                                                         sk_len, sk_flags, sk_bucket, sk_color);
+    }
+    // Bucket scatter: stable sort of super indices by bucket, then gather.
+    uint32_t* key = P.key.ensure<uint32_t>(n);
+    uint32_t* val = P.val.ensure<uint32_t>(n);
+    uint32_t* key_t = P.key_t.ensure<uint32_t>(n);
+    uint32_t* val_t = P.val_t.ensure<uint32_t>(n);
+    if (n) k_iota_copy<<<blocks_for(n, 256), 256, 0, st>>>(n, sk_bucket, key, val);
+    int bbits = 0;
+    while ((uint64_t(1) << bbits) < J.B) ++bbits;
+    SortPlan plan;
+    plan.build({0, n});
+    segmented_radix_sort<uint32_t, uint32_t>(key, val, key_t, val_t, plan, 0, bbits, bbits > 11 ? (bbits + 1) / 2 : bbits,
+                                             J.rs, st);
+    uint64_t* b_start = P.b_start.ensure<uint64_t>(n);
+    uint32_t* b_len = P.b_len.ensure<uint32_t>(n);
+    uint8_t* b_flags = P.b_flags.ensure<uint8_t>(n);
+    uint32_t* b_color = P.b_color.ensure<uint32_t>(n);
+    uint64_t* boff = P.bucket_off.ensure<uint64_t>(J.B + 1);
+    if (n) {
+        k_gather_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, val, sk_start, sk_len, sk_flags, sk_color, b_start,
+                                                            b_len, b_flags, b_color);
+    }
+    k_bucket_offsets<<<blocks_for(std::max<uint64_t>(n, uint64_t(J.B) + 1), 256), 256, 0, st>>>(n, key, J.B, boff);
+    CDBG_HIP(hipGetLastError());
+    J.h_bucket_off.resize(J.B + 1);
+    CDBG_HIP(hipMemcpyAsync(J.h_bucket_off.data(), boff, (J.B + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipEventRecord(e1, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    CDBG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    J.stats.ms_phase1 = ms;
+    J.stats.num_bases = J.n_bytes;
+    J.stats.num_supers = n;
+    J.phase1_done = true;
+}
+
+}  // namespace cdbg_amd

@@ -1,0 +1,338 @@
+// Device primitives shared by the phase kernels: device buffers, exclusive
+// scans, and a stable segmented LSD radix sort.
+//
+// Radix sort layout.  The input is a list of segments (buckets) laid out one
+// after another.  Every pass splits each segment into tiles of RS_TILE keys
+// that never cross a segment boundary; tile t of segment s writes its digit
+// histogram at  base_s*R + digit*ntiles_s + local_t  (R = 2^bits), so one
+// exclusive scan over the whole histogram array yields, for every
+// (segment, digit, tile), the first output slot: segments stay in place and
+// each is sorted on its own.  Ranks inside a tile are stable: the tile is
+// consumed in rounds of 256 keys, each wave finds the lanes holding its digit
+// with `bits` ballots, and per-wave LDS histograms order the waves.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "common.h"
+
+namespace cdbg_amd {
+
+// ---------------------------------------------------------------------------
+// Device buffer that only grows (contents are not preserved on growth).
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T>
+    T* ensure(size_t n) {
+        size_t bytes = n * sizeof(T);
+        if (bytes < 256) bytes = 256;
+        if (bytes > cap) {
+            release();
+            CDBG_HIP(hipMalloc(&p, bytes));
+            cap = bytes;
+        }
+        return (T*)p;
+    }
+    template <class T>
+    T* as() const {
+        return (T*)p;
+    }
+};
+
+struct Key128 {
+    uint64_t lo, hi;
+};
+
+__host__ __device__ inline bool operator==(const Key128& a, const Key128& b) {
+    return a.lo == b.lo && a.hi == b.hi;
+}
+__host__ __device__ inline bool operator!=(const Key128& a, const Key128& b) { return !(a == b); }
+__host__ __device__ inline bool operator<(const Key128& a, const Key128& b) {
+    return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo);
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint32_t k, int shift, uint32_t mask) {
+    return (k >> shift) & mask;
+}
+__device__ __forceinline__ uint32_t digit_of(uint64_t k, int shift, uint32_t mask) {
+    return uint32_t(k >> shift) & mask;
+}
+__device__ __forceinline__ uint32_t digit_of(const Key128& k, int shift, uint32_t mask) {
+    if (shift >= 64) return uint32_t(k.hi >> (shift - 64)) & mask;
+    uint64_t v = k.lo >> shift;
+    if (shift > 0) v |= k.hi << (64 - shift);
+    return uint32_t(v) & mask;
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive scan (reduce, scan of block sums, down-sweep).
+// ---------------------------------------------------------------------------
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_CHUNK = SCAN_THREADS * SCAN_ITEMS;
+
+template <class T>
+__device__ inline T block_exclusive_scan(T v, T* s_tmp, T& total) {
+    // s_tmp holds >= blockDim/64 entries.  Wave-level inclusive scan first.
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        T y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        T run = 0;
+        for (int w = 0; w < nw; ++w) {
+            T t = s_tmp[w];
+            s_tmp[w] = run;
+            run += t;
+        }
+        s_tmp[nw] = run;
+    }
+    __syncthreads();
+    T res = s_tmp[wave] + x - v;
+    total = s_tmp[nw];
+    __syncthreads();
+    return res;
+}
+
+template <class Tin, class Tout>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const Tin* __restrict__ in, uint64_t n,
+                                                              Tout* __restrict__ sums) {
+    __shared__ Tout s_tmp[SCAN_THREADS / 64 + 1];
+    const uint64_t base = uint64_t(blockIdx.x) * SCAN_CHUNK;
+    Tout acc = 0;
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t idx = base + uint64_t(i) * SCAN_THREADS + threadIdx.x;
+        if (idx < n) acc += Tout(in[idx]);
+    }
+    Tout total;
+    (void)block_exclusive_scan<Tout>(acc, s_tmp, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_single(T* __restrict__ a, uint64_t n, T* __restrict__ total_out) {
+    __shared__ T s_tmp[1024 / 64 + 1];
+    T carry = 0;
+    for (uint64_t base = 0; base < n; base += 1024) {
+        uint64_t idx = base + threadIdx.x;
+        T v = idx < n ? a[idx] : T(0);
+        T tot;
+        T ex = block_exclusive_scan<T>(v, s_tmp, tot);
+        if (idx < n) a[idx] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+}
+
+template <class Tin, class Tout>
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const Tin* __restrict__ in, uint64_t n,
+                                                            const Tout* __restrict__ sums,
+                                                            Tout* __restrict__ out) {
+    __shared__ Tout s_tmp[SCAN_THREADS / 64 + 1];
+    // Blocked arrangement: thread t owns items [t*ITEMS, t*ITEMS+ITEMS).
+    const uint64_t base = uint64_t(blockIdx.x) * SCAN_CHUNK + uint64_t(threadIdx.x) * SCAN_ITEMS;
+    Tout v[SCAN_ITEMS];
+    Tout acc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t idx = base + i;
+        v[i] = idx < n ? Tout(in[idx]) : Tout(0);
+        acc += v[i];
+    }
+    Tout tot;
+    Tout ex = block_exclusive_scan<Tout>(acc, s_tmp, tot) + sums[blockIdx.x];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        uint64_t idx = base + i;
+        if (idx < n) out[idx] = ex;
+        ex += v[i];
+    }
+}
+
+// out[i] = sum(in[0..i)); *d_total (device) = sum(in).  `tmp` is scratch.
+template <class Tin, class Tout>
+inline void exclusive_scan(const Tin* in, Tout* out, uint64_t n, Tout* d_total, DevBuf& tmp,
+                           hipStream_t st) {
+    uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+    if (nb == 0) nb = 1;
+    Tout* sums = tmp.ensure<Tout>(nb + 1);
+    k_scan_reduce<Tin, Tout><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, sums);
+    k_scan_single<Tout><<<1, 1024, 0, st>>>(sums, nb, d_total);
+    k_scan_down<Tin, Tout><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, sums, out);
+    CDBG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Segmented stable LSD radix sort.
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256;
+constexpr int RS_TILE = 4096;
+
+struct TileDesc {
+    uint64_t start;       // first element of the tile
+    uint32_t count;       // elements in the tile
+    uint32_t ntiles_seg;  // tiles of the tile's segment
+    uint64_t hist_base;   // first tile index of the segment (times R gives the region)
+    uint32_t local;       // tile index inside the segment
+    uint32_t pad;
+};
+
+template <class K>
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K* __restrict__ keys,
+                                                           const TileDesc* __restrict__ tiles, int shift,
+                                                           int bits, uint32_t* __restrict__ hist) {
+    extern __shared__ uint32_t s_hist[];
+    const uint32_t R = 1u << bits, mask = R - 1;
+    const TileDesc d = tiles[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < R; i += RS_THREADS) s_hist[i] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < d.count; i += RS_THREADS)
+        atomicAdd(&s_hist[digit_of(keys[d.start + i], shift, mask)], 1u);
+    __syncthreads();
+    const uint64_t region = d.hist_base * R;
+    for (uint32_t b = threadIdx.x; b < R; b += RS_THREADS)
+        hist[region + uint64_t(b) * d.ntiles_seg + d.local] = s_hist[b];
+}
+
+template <class K, class V>
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const K* __restrict__ keys,
+                                                              const V* __restrict__ vals,
+                                                              K* __restrict__ okeys, V* __restrict__ ovals,
+                                                              const TileDesc* __restrict__ tiles, int shift,
+                                                              int bits, const uint32_t* __restrict__ offs) {
+    extern __shared__ uint32_t sm[];
+    const uint32_t R = 1u << bits, mask = R - 1;
+    uint32_t* running = sm;
+    uint32_t* whist = sm + R;  // [4][R]
+    const TileDesc d = tiles[blockIdx.x];
+    const uint64_t region = d.hist_base * R;
+    for (uint32_t b = threadIdx.x; b < R; b += RS_THREADS) {
+        running[b] = offs[region + uint64_t(b) * d.ntiles_seg + d.local];
+        whist[b] = whist[R + b] = whist[2 * R + b] = whist[3 * R + b] = 0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (uint32_t base = 0; base < d.count; base += RS_THREADS) {
+        const uint32_t i = base + threadIdx.x;
+        const bool active = i < d.count;
+        K key{};
+        V val{};
+        uint32_t dig = 0;
+        if (active) {
+            key = keys[d.start + i];
+            val = vals[d.start + i];
+            dig = digit_of(key, shift, mask);
+        }
+        uint64_t peers = __ballot(active);
+        for (int b = 0; b < bits; ++b) {
+            const bool bit = (dig >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+        if (active && rank == 0) whist[wave * R + dig] = __popcll(peers);
+        __syncthreads();
+        if (active) {
+            uint32_t pre = 0;
+            for (int w = 0; w < wave; ++w) pre += whist[w * R + dig];
+            const uint32_t pos = running[dig] + pre + rank;
+            okeys[pos] = key;
+            ovals[pos] = val;
+        }
+        __syncthreads();
+        for (uint32_t b = threadIdx.x; b < R; b += RS_THREADS) {
+            running[b] += whist[b] + whist[R + b] + whist[2 * R + b] + whist[3 * R + b];
+            whist[b] = whist[R + b] = whist[2 * R + b] = whist[3 * R + b] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+// Host-side tile plan for a segmented sort.  seg_off has nseg+1 entries.
+struct SortPlan {
+    std::vector<TileDesc> tiles;
+    uint64_t n = 0;
+    void build(const std::vector<uint64_t>& seg_off) {
+        tiles.clear();
+        n = seg_off.empty() ? 0 : seg_off.back();
+        uint64_t tile_base = 0;
+        for (size_t s = 0; s + 1 < seg_off.size(); ++s) {
+            uint64_t len = seg_off[s + 1] - seg_off[s];
+            uint32_t nt = uint32_t((len + RS_TILE - 1) / RS_TILE);
+            for (uint32_t t = 0; t < nt; ++t) {
+                TileDesc d{};
+                d.start = seg_off[s] + uint64_t(t) * RS_TILE;
+                uint64_t rem = len - uint64_t(t) * RS_TILE;
+                d.count = uint32_t(rem < RS_TILE ? rem : RS_TILE);
+                d.ntiles_seg = nt;
+                d.hist_base = tile_base;
+                d.local = t;
+                tiles.push_back(d);
+            }
+            tile_base += nt;
+        }
+    }
+};
+
+// Sorts (keys, vals) in place by key bits [begin_bit, end_bit) inside every
+// segment of `plan`, `digit_bits` bits per pass.  kt/vt are ping-pong
+// buffers of the same length; the result always ends in keys/vals.
+struct RadixScratch {
+    DevBuf tiles, hist, offs, scan_tmp, total;
+};
+
+template <class K, class V>
+inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan& plan, int begin_bit,
+                                 int end_bit, int digit_bits, RadixScratch& S, hipStream_t st) {
+    if (plan.n == 0 || begin_bit >= end_bit) return;
+    if (plan.n >= (uint64_t(1) << 32)) throw Error("segmented_radix_sort: more than 2^32 elements");
+    const uint32_t nt = uint32_t(plan.tiles.size());
+    TileDesc* d_tiles = S.tiles.ensure<TileDesc>(nt);
+    CDBG_HIP(hipMemcpyAsync(d_tiles, plan.tiles.data(), nt * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+    K* src_k = keys;
+    V* src_v = vals;
+    K* dst_k = kt;
+    V* dst_v = vt;
+    int passes = 0;
+    for (int sh = begin_bit; sh < end_bit; sh += digit_bits) {
+        const int bits = (end_bit - sh) < digit_bits ? (end_bit - sh) : digit_bits;
+        const uint32_t R = 1u << bits;
+        const uint64_t hl = uint64_t(nt) * R;
+        uint32_t* hist = S.hist.ensure<uint32_t>(hl);
+        uint32_t* offs = S.offs.ensure<uint32_t>(hl);
+        k_radix_hist<K><<<nt, RS_THREADS, R * sizeof(uint32_t), st>>>(src_k, d_tiles, sh, bits, hist);
+        exclusive_scan<uint32_t, uint32_t>(hist, offs, hl, S.total.ensure<uint32_t>(1), S.scan_tmp, st);
+        k_radix_scatter<K, V><<<nt, RS_THREADS, 5 * R * sizeof(uint32_t), st>>>(src_k, src_v, dst_k, dst_v,
+                                                                              d_tiles, sh, bits, offs);
+        CDBG_HIP(hipGetLastError());
+        std::swap(src_k, dst_k);
+        std::swap(src_v, dst_v);
+        ++passes;
+    }
+    if (passes & 1) {
+        CDBG_HIP(hipMemcpyAsync(keys, src_k, plan.n * sizeof(K), hipMemcpyDeviceToDevice, st));
+        CDBG_HIP(hipMemcpyAsync(vals, src_v, plan.n * sizeof(V), hipMemcpyDeviceToDevice, st));
+    }
+}
+
+}  // namespace cdbg_amd

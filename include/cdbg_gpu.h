@@ -1,0 +1,126 @@
+/*
+ * cdbg_gpu.h — C ABI of the MI355X-native bucket-write (phase 1) and
+ * bucket-process (phase 2) of cdbg-builder.
+ *
+ * Plain pointers and sizes only; no C++ types cross this boundary.  Every
+ * call returns 0 on success and a negative status on failure, with the
+ * message available from cdbg_gpu_last_error() (thread-local).  The reference
+ * reports failures as C++ exceptions caught in main (src/main.cpp:82-88);
+ * nothing here throws.
+ *
+ * Reference interfaces replaced (file:line in the reference tree):
+ *   cdbg_gpu_job_create + cdbg_gpu_job_phase1
+ *       -> ingest_bucketed(files, k, m, num_buckets, writer, T, ...)
+ *          include/phase1_bucket_write/bucket_ingester.hpp:183-215,
+ *          called from include/builder.hpp:180-181
+ *   cdbg_gpu_job_phase2
+ *       -> process_buckets(writer, k, num_colors, T, sink, global_dict, ...)
+ *          include/phase2_bucket_process/bucket_walker.hpp:607-779,
+ *          called from include/builder.hpp:269-271
+ *   cdbg_gpu_bucket_write / cdbg_gpu_bucket_process
+ *       -> the same two phases with their on-disk artifacts: bucket_<b>.bin
+ *          (bucket_io.hpp:19-40), frag_unitigs.bin (unitig_spill.hpp:491-522),
+ *          <out>.color_sets + .tmp_offsets (streaming_color_set_dict.hpp:86-182)
+ */
+#ifndef CDBG_GPU_H
+#define CDBG_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CDBG_API __attribute__((visibility("default")))
+
+typedef struct cdbg_gpu_ctx cdbg_gpu_ctx;
+typedef struct cdbg_gpu_job cdbg_gpu_job;
+
+/* Statistics of one job.  num_kmers equals the reference's metadata
+ * num_kmers (builder.hpp:405-406): every distinct canonical k-mer is primary
+ * in exactly one bucket. */
+typedef struct {
+    uint64_t num_bases;      /* input bytes scanned by phase 1 */
+    uint64_t num_supers;     /* super-k-mer occurrences = bucket records */
+    uint64_t num_occ;        /* k-mer occurrences over all records */
+    uint64_t num_tuples;     /* (k-mer, colour) tuples sorted in phase 2 */
+    uint64_t num_nodes;      /* per-bucket nodes, foreign copies included */
+    uint64_t num_kmers;      /* distinct canonical k-mers */
+    uint64_t num_color_sets; /* distinct colour sets */
+    uint64_t num_frags;      /* unitig fragments */
+    uint64_t frag_bytes;     /* bytes of the frag_unitigs.bin stream */
+    uint64_t cs_bits;        /* bits of the colour-set stream */
+    uint64_t total_integers; /* sum of colour-set sizes */
+    double ms_phase1;        /* device wall time of the last phase-1 run */
+    double ms_phase2;        /* device wall time of the last phase-2 run */
+} cdbg_gpu_stats;
+
+/* Error string of the last failing call on this thread. */
+CDBG_API const char* cdbg_gpu_last_error(void);
+
+/* One context per device.  `device` is a HIP device ordinal. */
+CDBG_API int cdbg_gpu_ctx_create(int device, cdbg_gpu_ctx** out);
+CDBG_API void cdbg_gpu_ctx_destroy(cdbg_gpu_ctx* ctx);
+
+/* Stage one build job in HBM.  `bytes` holds the decoded sequences of all
+ * colours; colour c is bytes[color_off[c] .. color_off[c+1]), its records
+ * separated by any non-ACGT byte (the layout cdbg_gpu_load_fasta produces).
+ * k in [2, 63] (k <= 31 uses 64-bit k-mers, 32..63 uses 128-bit), m in
+ * [2, k-1], num_buckets >= 1.  Mirrors the inputs of ingest_bucketed
+ * (bucket_ingester.hpp:183-187): colour = index of the input file. */
+CDBG_API int cdbg_gpu_job_create(cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off,
+                                 uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets,
+                                 cdbg_gpu_job** out);
+CDBG_API void cdbg_gpu_job_destroy(cdbg_gpu_job* job);
+
+/* Phase 1 on the device: 2-bit packing, canonical ntHash minimizers,
+ * super-k-mers with BEGIN/END/OWNS flags, bucket scatter
+ * (bucket_ingester.hpp:36-127). */
+CDBG_API int cdbg_gpu_job_phase1(cdbg_gpu_job* job);
+/* Phase 2 on the device over the phase-1 records: per-bucket canonical
+ * k-mers, colour sets, global colour-set ids, unitig fragments
+ * (bucket_walker.hpp:166-566,713-761).  Requires phase 1. */
+CDBG_API int cdbg_gpu_job_phase2(cdbg_gpu_job* job);
+/* Both phases back to back. */
+CDBG_API int cdbg_gpu_job_run(cdbg_gpu_job* job);
+
+CDBG_API int cdbg_gpu_job_stats(cdbg_gpu_job* job, cdbg_gpu_stats* out);
+
+/* Phase-1 records.  which = 0: position order; which = 1: bucket-major
+ * (position order inside each bucket).  Arrays hold num_supers entries.
+ * bucket_offsets (num_buckets + 1 entries) delimits the buckets of which=1. */
+CDBG_API int cdbg_gpu_job_supers(cdbg_gpu_job* job, int which, uint64_t* start, uint32_t* len,
+                                 uint8_t* flags, uint32_t* bucket, uint32_t* color);
+CDBG_API int cdbg_gpu_job_bucket_offsets(cdbg_gpu_job* job, uint64_t* out);
+
+/* Phase-2 artifacts: the frag_unitigs.bin byte stream (frag_bytes bytes),
+ * the colour-set bit stream (ceil(cs_bits/64) words) and the bit offset of
+ * every class (num_color_sets words). */
+CDBG_API int cdbg_gpu_job_frags(cdbg_gpu_job* job, uint8_t* dst);
+CDBG_API int cdbg_gpu_job_color_sets(cdbg_gpu_job* job, uint64_t* words, uint64_t* offsets);
+
+/* Synthetic pangenome generator (seeded std::mt19937_64, explicit modulo). */
+typedef struct {
+    uint64_t seed;
+    uint32_t num_colors;
+    uint64_t genome_len;
+    double snp_rate;
+    double indel_rate;
+    uint32_t indel_max;
+    double pav_frac;
+    uint32_t pav_min, pav_max;
+    uint32_t n_inversions, inv_min, inv_max;
+} cdbg_synth_params;
+
+CDBG_API void* cdbg_synth_create(const cdbg_synth_params* p, uint32_t threads);
+CDBG_API uint64_t cdbg_synth_total_bytes(void* h);
+CDBG_API const uint8_t* cdbg_synth_bytes(void* h);
+CDBG_API const uint64_t* cdbg_synth_offsets(void* h);
+CDBG_API void cdbg_synth_free(void* h);
+CDBG_API int cdbg_synth_write_fasta(void* h, uint32_t color, const char* path, int gz);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDBG_GPU_H */

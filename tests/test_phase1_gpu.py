@@ -1,0 +1,74 @@
+"""Phase 1 on the GPU, bit-exact against the oracle: every super-k-mer
+(start, length, flags, bucket, colour) in position order, the bucket-major
+order and the bucket offsets."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import cdbg_gpu as G
+
+pytestmark = pytest.mark.gpu
+ACGT = np.frombuffer(b"ACGT", np.uint8)
+
+
+def _edge_case_job(rng):
+    """Colours that exercise tile borders, non-ACGT bytes, case, empty colours."""
+    parts = [
+        ACGT[rng.integers(0, 4, 9000)].tobytes(),                 # > 2 tiles
+        b"N" * 5000 + ACGT[rng.integers(0, 4, 50)].tobytes(),
+        b"",                                                        # empty colour
+        b"A" * 12000 + b"C" * 10,                                   # long homopolymer
+        ACGT[rng.integers(0, 4, 3000)].tobytes().lower() + b"\r\n" + ACGT[rng.integers(0, 4, 20)].tobytes(),
+        b"ACGTNACGTTGCA" * 500,
+    ]
+    data = b"".join(p + b"\n" for p in parts)
+    off = np.zeros(len(parts) + 1, np.uint64)
+    off[1:] = np.cumsum([len(p) + 1 for p in parts])
+    return np.frombuffer(data, np.uint8).copy(), off
+
+
+def _compare(job, ob, B):
+    for which in (0, 1):
+        g = job.supers(which)
+        o = ob.supers(which)
+        assert len(g["start"]) == len(o["start"])
+        for key in ("start", "len", "flags", "bucket", "color"):
+            if not np.array_equal(g[key], o[key]):
+                bad = np.nonzero(g[key] != o[key])[0][:5]
+                raise AssertionError(f"which={which} field={key} first mismatches at {bad}: "
+                                     f"gpu={g[key][bad]} oracle={o[key][bad]}")
+    o1 = ob.supers(1)
+    off = job.bucket_offsets()
+    counts = np.bincount(o1["bucket"], minlength=B)
+    assert np.array_equal(np.diff(off), counts)
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 19, 64), (31, 21, 1024), (63, 27, 4096), (15, 9, 37),
+                                   (5, 3, 1), (31, 12, 16384)])
+def test_phase1_synthetic(gpu_ctx, k, m, B):
+    data, off = G.synth(seed=1, num_colors=6, genome_len=30000, snp_rate=0.01, indel_rate=0.001)
+    ob = O.Build(data, off, k, m, B, threads=4)
+    with gpu_ctx.job(data, off, k, m, B) as job:
+        job.phase1()
+        _compare(job, ob, B)
+        # a second run of the same job reuses its buffers and must agree
+        job.phase1()
+        _compare(job, ob, B)
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 19, 64), (63, 27, 97), (9, 4, 8)])
+def test_phase1_edge_cases(gpu_ctx, k, m, B):
+    data, off = _edge_case_job(np.random.default_rng(5))
+    ob = O.Build(data, off, k, m, B)
+    with gpu_ctx.job(data, off, k, m, B) as job:
+        job.phase1()
+        _compare(job, ob, B)
+
+
+def test_phase1_no_valid_runs(gpu_ctx):
+    data = np.frombuffer(b"NNNNACGTNNNN\n", np.uint8).copy()
+    off = np.array([0, len(data)], np.uint64)
+    with gpu_ctx.job(data, off, 31, 19, 64) as job:
+        job.phase1()
+        assert job.stats()["num_supers"] == 0
+        assert np.all(job.bucket_offsets() == 0)
