@@ -1,16 +1,1075 @@
-// Phase 2 (bucket-process) on the device.  (Kernels follow in this file.)
+// Phase 2 (bucket-process) on the device, over the phase-1 record stream.
+//
+// Semantics are those of process_buckets/process_bucket at -t 1
+// (bucket_walker.hpp:166-566,607-779), reproduced exactly:
+//   * a node is every canonical k-mer of a bucket's records; flags = OR of
+//     the per-occurrence boundary bits, primary = any owned occurrence
+//     (bucket_walker.hpp:212-250);
+//   * a node's colour set is the union of the colours of its owned
+//     occurrences; its rank is its first occurrence in the bucket stream,
+//     which is the reference's ankerl insertion order (:160,387,563);
+//   * global colour-set ids are assigned in order of first appearance over
+//     (bucket, rank) -- buckets merged 0..B-1, local classes in first-intern
+//     order (:713-757, streaming_color_set_dict.hpp:148-182);
+//   * fragments are the walks of emit_from_seed with seeds taken in rank
+//     order (:431-566), emitted in (bucket, seed rank) order.
+//
+// Kernel pipeline (tuple = one k-mer occurrence):
+//   k_count_kmers / scan      -> tuple offsets of every record
+//   k_expand<K>               -> (canonical k-mer, rank|colour|owned|contrib)
+//   segmented radix sort      -> tuples sorted by k-mer inside each bucket
+//   k_heads / scan / k_node_* -> nodes: flags, primary, min rank, owned colours
+//   k_set_*                   -> colour-set interning (singleton table + hash
+//                                table with exact list verification), global
+//                                ids by sorting (bucket<<32|rank) keys, hybrid
+//                                encoding (hybrid_color_sets.hpp:41-96)
+//   k_table_insert<K>         -> per-bucket open-addressing k-mer table
+//   k_masks<K> / k_links      -> 8-neighbour ext masks (local_ext_mask
+//                                :260-271) and clean links (walk_step :317-336)
+//   k_walk_paths/cycles       -> one thread per component emulates the
+//                                rank-ordered seed loop
+//   fragment sort + k_frag_*  -> frag_unitigs.bin records (unitig_spill.hpp:491-522)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+
 #include "job.h"
+#include "kmer_ops.h"
 
 namespace cdbg_amd {
 
-struct Phase2Bufs {
-    DevBuf frag_bytes;
-    DevBuf cs_words, cs_offsets;
+namespace {
+
+constexpr uint32_t NONE32 = 0xFFFFFFFFu;
+constexpr uint32_t PORT_L = 0, PORT_R = 1;
+constexpr uint8_t BOUND_LEFT = 1, BOUND_RIGHT = 2;
+constexpr int TPB = 256;
+
+inline uint32_t blocks_for(uint64_t n, uint32_t t = TPB) {
+    uint64_t b = (n + t - 1) / t;
+    return uint32_t(b ? b : 1);
+}
+
+// Tuple value: rank (32) | colour (29) | owned (1) | contrib (2).
+__device__ __forceinline__ uint32_t v_rank(uint64_t v) { return uint32_t(v >> 32); }
+__device__ __forceinline__ uint32_t v_color(uint64_t v) { return uint32_t(v >> 3) & 0x1FFFFFFFu; }
+__device__ __forceinline__ bool v_owned(uint64_t v) { return (v >> 2) & 1u; }
+__device__ __forceinline__ uint32_t v_contrib(uint64_t v) { return uint32_t(v) & 3u; }
+
+template <class K>
+struct alignas(16) Slot;
+template <>
+struct alignas(16) Slot<uint64_t> {
+    uint64_t key;
+    uint32_t node;
+    uint32_t pad;
+};
+template <>
+struct alignas(16) Slot<Key128> {
+    Key128 key;
+    uint32_t node;
+    uint32_t pad[3];
 };
 
-void run_phase2(Job& J) { throw Error("phase 2 not built yet"); }
+__device__ __forceinline__ uint32_t upper_bucket(const uint64_t* off, uint32_t B, uint64_t i) {
+    // bucket b with off[b] <= i < off[b+1] (first such b for empty buckets skipped)
+    uint32_t lo = 0, hi = B;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_iota(uint32_t* p, uint64_t n) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = uint32_t(i);
+}
+
+template <class T>
+__global__ void k_fill(T* p, uint64_t n, T v) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// block-wide sum added atomically to *dst
+__device__ __forceinline__ void block_add(unsigned long long* dst, uint64_t v) {
+    __shared__ uint64_t s_part[TPB / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (int w = 0; w < int(blockDim.x >> 6); ++w) t += s_part[w];
+        if (t) atomicAdd(dst, (unsigned long long)t);
+    }
+    __syncthreads();  // s_part is reused by the next call
+}
+
+// ---------------------------------------------------------------------------
+// expand
+// ---------------------------------------------------------------------------
+__global__ void k_count_kmers(uint64_t n, const uint32_t* __restrict__ len, uint32_t k, uint32_t* __restrict__ cnt) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = len[i] - k + 1;
+}
+
+__global__ void k_bucket_tuple_off(uint32_t B, const uint64_t* __restrict__ boff, const uint64_t* __restrict__ tup_off,
+                                   uint64_t n_supers, uint64_t n_tuples, uint64_t* __restrict__ btup) {
+    const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b > B) return;
+    const uint64_t s = boff[b];
+    btup[b] = s < n_supers ? tup_off[s] : n_tuples;
+}
+
+template <class K>
+__global__ void k_expand(uint64_t n, const uint64_t* __restrict__ start, const uint32_t* __restrict__ len,
+                         const uint8_t* __restrict__ flags, const uint32_t* __restrict__ color,
+                         const uint32_t* __restrict__ bucket, const uint64_t* __restrict__ tup_off,
+                         const uint64_t* __restrict__ btup, const uint32_t* __restrict__ packed, uint32_t k,
+                         K* __restrict__ keys, uint64_t* __restrict__ vals) {
+    using O = KOps<K>;
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t s = start[i];
+    const uint32_t L = len[i];
+    const uint32_t fl = flags[i];
+    const uint64_t t0 = tup_off[i];
+    const uint32_t rank0 = uint32_t(t0 - btup[bucket[i]]);
+    const uint64_t cbits = uint64_t(color[i]) << 3;
+    const uint32_t nk = L - k + 1;
+    const bool begin_incl = fl & 1u, end_incl = fl & 2u, own_first = fl & 4u, own_last = fl & 8u;
+    K f = O::zero(), r = O::zero();
+    auto base = [&](uint64_t p) -> uint32_t { return (packed[p >> 4] >> (2 * (p & 15))) & 3u; };
+    for (uint32_t j = 0; j + 1 < k; ++j) O::roll(f, r, base(s + j), k);
+    for (uint32_t ix = 0; ix < nk; ++ix) {
+        O::roll(f, r, base(s + k - 1 + ix), k);
+        const bool fwd = O::le(f, r);
+        uint32_t contrib = 0;
+        if (!begin_incl && ix == 0) contrib |= fwd ? BOUND_LEFT : BOUND_RIGHT;
+        if (!end_incl && ix == nk - 1) contrib |= fwd ? BOUND_RIGHT : BOUND_LEFT;
+        const bool owned = !((ix == 0 && !own_first) || (ix == nk - 1 && !own_last));
+        keys[t0 + ix] = fwd ? f : r;
+        vals[t0 + ix] = (uint64_t(rank0 + ix) << 32) | cbits | (uint64_t(owned) << 2) | contrib;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// nodes
+// ---------------------------------------------------------------------------
+template <class K>
+__global__ void k_heads(uint64_t n, const K* __restrict__ keys, uint32_t* __restrict__ heads) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    heads[i] = (i == 0 || !KOps<K>::eq(keys[i], keys[i - 1])) ? 1u : 0u;
+}
+
+__global__ void k_mark_bucket_starts(uint32_t B, const uint64_t* __restrict__ btup, uint32_t* __restrict__ heads) {
+    const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b < B && btup[b] < btup[b + 1]) heads[btup[b]] = 1u;
+}
+
+__global__ void k_node_start(uint64_t n, const uint32_t* __restrict__ heads, const uint32_t* __restrict__ hscan,
+                             uint32_t n_nodes, uint32_t* __restrict__ node_start) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && heads[i]) node_start[hscan[i]] = uint32_t(i);
+    if (i == 0) node_start[n_nodes] = uint32_t(n);
+}
+
+__global__ void k_node_off(uint32_t B, const uint64_t* __restrict__ btup, uint64_t n_tuples,
+                           const uint32_t* __restrict__ hscan, uint32_t n_nodes, uint64_t* __restrict__ node_off) {
+    const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b > B) return;
+    const uint64_t t = btup[b];
+    node_off[b] = t < n_tuples ? hscan[t] : n_nodes;
+}
+
+template <class K>
+__global__ void k_node_reduce(uint32_t n_nodes, const uint32_t* __restrict__ node_start, const K* __restrict__ keys,
+                              const uint64_t* __restrict__ vals, K* __restrict__ node_key, uint8_t* __restrict__ node_meta,
+                              uint32_t* __restrict__ node_rank, uint32_t* __restrict__ node_ncol,
+                              unsigned long long* __restrict__ counters) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t prim = 0, multi = 0;
+    if (n < n_nodes) {
+        const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
+        uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
+        bool primary = false;
+        for (uint32_t t = t0; t < t1; ++t) {
+            const uint64_t v = vals[t];
+            fl |= v_contrib(v);
+            const uint32_t r = v_rank(v);
+            rank = r < rank ? r : rank;
+            if (v_owned(v)) {
+                primary = true;
+                const uint32_t c = v_color(v);
+                if (c != last) {
+                    ++ncol;
+                    last = c;
+                }
+            }
+        }
+        node_key[n] = keys[t0];
+        node_meta[n] = uint8_t(fl | (primary ? 4u : 0u));
+        node_rank[n] = rank;
+        node_ncol[n] = ncol;
+        prim = primary;
+        multi = ncol >= 2;
+    }
+    block_add(&counters[0], prim);
+    block_add(&counters[1], multi);
+}
+
+__global__ void k_node_colors(uint32_t n_nodes, const uint32_t* __restrict__ node_start, const uint64_t* __restrict__ vals,
+                              const uint64_t* __restrict__ node_coff, uint32_t* __restrict__ colors) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
+    uint64_t o = node_coff[n];
+    uint32_t last = 0xFFFFFFFFu;
+    for (uint32_t t = t0; t < t1; ++t) {
+        const uint64_t v = vals[t];
+        if (!v_owned(v)) continue;
+        const uint32_t c = v_color(v);
+        if (c != last) {
+            colors[o++] = c;
+            last = c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// colour sets
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t list_hash(const uint32_t* c, uint32_t n) {
+    uint64_t h = 0x243F6A8885A308D3ULL ^ n;
+    for (uint32_t i = 0; i < n; ++i) h = mix64(h ^ (uint64_t(c[i]) + 0x9E3779B97F4A7C15ULL * (i + 1)));
+    return h | 1ull;  // 0 marks an empty slot
+}
+
+__global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
+                             const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                             const uint32_t* __restrict__ rank, const uint64_t* __restrict__ node_off, uint32_t B,
+                             unsigned long long* __restrict__ single_min, unsigned long long* __restrict__ mt_hash,
+                             unsigned long long* __restrict__ mt_key, uint64_t mt_cap, uint32_t* __restrict__ node_slot) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes || !(meta[n] & 4u)) return;
+    const uint32_t b = upper_bucket(node_off, B, n);
+    const unsigned long long key = (uint64_t(b) << 32) | rank[n];
+    const uint32_t nc = ncol[n];
+    const uint32_t* c = colors + coff[n];
+    if (nc == 1) {
+        atomicMin(&single_min[c[0]], key);
+        return;
+    }
+    const unsigned long long h = list_hash(c, nc);
+    uint64_t s = h & (mt_cap - 1);
+    for (uint64_t probe = 0; probe < mt_cap; ++probe) {
+        const unsigned long long prev = atomicCAS(&mt_hash[s], 0ull, h);
+        if (prev == 0ull || prev == h) {
+            atomicMin(&mt_key[s], key);
+            node_slot[n] = uint32_t(s);
+            return;
+        }
+        s = (s + 1) & (mt_cap - 1);
+    }
+}
+
+__global__ void k_set_rep(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
+                          const uint32_t* __restrict__ rank, const uint64_t* __restrict__ node_off, uint32_t B,
+                          const uint32_t* __restrict__ node_slot, const unsigned long long* __restrict__ mt_key,
+                          uint32_t* __restrict__ mt_rep) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2) return;
+    const uint32_t b = upper_bucket(node_off, B, n);
+    const uint64_t key = (uint64_t(b) << 32) | rank[n];
+    const uint32_t s = node_slot[n];
+    if (mt_key[s] == key) mt_rep[s] = uint32_t(n);
+}
+
+__global__ void k_set_verify(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
+                             const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                             const uint32_t* __restrict__ node_slot, const uint32_t* __restrict__ mt_rep,
+                             unsigned long long* __restrict__ err) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2) return;
+    const uint32_t r = mt_rep[node_slot[n]];
+    bool same = ncol[r] == ncol[n];
+    for (uint32_t i = 0; same && i < ncol[n]; ++i) same = colors[coff[n] + i] == colors[coff[r] + i];
+    if (!same) atomicAdd(err, 1ull);
+}
+
+// flags over [0, num_colors) singletons then [0, cap) hash slots
+__global__ void k_set_flags(uint32_t nc, uint64_t cap, const unsigned long long* __restrict__ single_min,
+                            const unsigned long long* __restrict__ mt_hash, uint32_t* __restrict__ fl) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nc)
+        fl[i] = single_min[i] != ~0ull;
+    else if (i < nc + cap)
+        fl[i] = mt_hash[i - nc] != 0ull;
+}
+
+__global__ void k_set_emit(uint32_t nc, uint64_t cap, const uint32_t* __restrict__ fl, const uint64_t* __restrict__ pos,
+                           const unsigned long long* __restrict__ single_min, const unsigned long long* __restrict__ mt_key,
+                           uint64_t* __restrict__ set_key, uint64_t* __restrict__ set_desc) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nc + cap || !fl[i]) return;
+    const uint64_t p = pos[i];
+    if (i < nc) {
+        set_key[p] = single_min[i];
+        set_desc[p] = i;
+    } else {
+        set_key[p] = mt_key[i - nc];
+        set_desc[p] = (uint64_t(1) << 63) | (i - nc);
+    }
+}
+
+__device__ __forceinline__ uint32_t msb64d(uint64_t x) { return 63u - uint32_t(__clzll(x)); }
+__device__ __forceinline__ uint64_t delta_len(uint64_t x) {
+    const uint64_t b = msb64d(x + 1);
+    return 2ull * msb64d(b + 1) + 1 + b;
+}
+
+// Resolves a set's colour list: singletons point into `single_val`.
+__device__ __forceinline__ void set_list(uint64_t desc, const uint32_t* __restrict__ mt_rep,
+                                         const uint64_t* __restrict__ coff, const uint32_t* __restrict__ ncol,
+                                         const uint32_t* __restrict__ colors, const uint32_t*& c, uint32_t& n,
+                                         uint32_t& single) {
+    if (desc >> 63) {
+        const uint32_t r = mt_rep[desc & 0xFFFFFFFFull];
+        c = colors + coff[r];
+        n = ncol[r];
+    } else {
+        single = uint32_t(desc);
+        c = &single;
+        n = 1;
+    }
+}
+
+__global__ void k_set_cid_len(uint64_t n_sets, const uint64_t* __restrict__ set_desc, const uint32_t* __restrict__ mt_rep,
+                              const uint64_t* __restrict__ coff, const uint32_t* __restrict__ ncol,
+                              const uint32_t* __restrict__ colors, uint32_t num_colors, uint32_t* __restrict__ single_cid,
+                              uint32_t* __restrict__ mt_cid, uint64_t* __restrict__ bits,
+                              unsigned long long* __restrict__ counters) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    uint64_t ints = 0;
+    if (i < n_sets) {
+        const uint64_t d = set_desc[i];
+        if (d >> 63)
+            mt_cid[d & 0xFFFFFFFFull] = uint32_t(i);
+        else
+            single_cid[uint32_t(d)] = uint32_t(i);
+        const uint32_t* c;
+        uint32_t n, single;
+        set_list(d, mt_rep, coff, ncol, colors, c, n, single);
+        const uint32_t sparse = num_colors / 4, dense = uint32_t(uint64_t(num_colors) * 3 / 4);
+        uint64_t L = delta_len(n);
+        if (n < sparse) {
+            L += delta_len(c[0]);
+            for (uint32_t j = 1; j < n; ++j) L += delta_len(uint64_t(c[j] - (c[j - 1] + 1)));
+        } else if (n < dense) {
+            L += num_colors;
+        } else {
+            // delta gaps over the absent colours (hybrid_color_sets.hpp:67-94)
+            uint32_t j = 0, prev = 0xFFFFFFFFu;
+            for (uint32_t v = 0; v < num_colors; ++v) {
+                if (j < n && c[j] == v) {
+                    ++j;
+                    continue;
+                }
+                L += delta_len(uint64_t(v - (prev + 1)));
+                prev = v;
+            }
+        }
+        bits[i] = L;
+        ints = n;
+    }
+    block_add(&counters[2], ints);
+}
+
+struct BitWriter {
+    unsigned long long* words;
+    uint64_t pos;
+    __device__ void put(uint64_t v, uint32_t len) {  // low `len` bits of v, LSB first
+        if (len == 0) return;
+        if (len < 64) v &= (uint64_t(1) << len) - 1;
+        const uint64_t w = pos >> 6;
+        const uint32_t o = uint32_t(pos & 63);
+        atomicOr(&words[w], (unsigned long long)(v << o));
+        if (o && o + len > 64) atomicOr(&words[w + 1], (unsigned long long)(v >> (64 - o)));
+        pos += len;
+    }
+    __device__ void gamma(uint64_t x) {
+        const uint64_t xx = x + 1;
+        const uint32_t b = msb64d(xx);
+        put(0, b);
+        put(((xx & ((uint64_t(1) << b) - 1)) << 1) | 1u, b + 1);
+    }
+    __device__ void delta(uint64_t x) {
+        const uint64_t xx = x + 1;
+        const uint32_t b = msb64d(xx);
+        gamma(b);
+        put(xx & ((uint64_t(1) << b) - 1), b);
+    }
+};
+
+__global__ void k_set_encode(uint64_t n_sets, const uint64_t* __restrict__ set_desc, const uint64_t* __restrict__ boff,
+                             const uint32_t* __restrict__ mt_rep, const uint64_t* __restrict__ coff,
+                             const uint32_t* __restrict__ ncol, const uint32_t* __restrict__ colors, uint32_t num_colors,
+                             unsigned long long* __restrict__ words) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_sets) return;
+    const uint32_t* c;
+    uint32_t n, single;
+    set_list(set_desc[i], mt_rep, coff, ncol, colors, c, n, single);
+    const uint32_t sparse = num_colors / 4, dense = uint32_t(uint64_t(num_colors) * 3 / 4);
+    BitWriter w{words, boff[i]};
+    w.delta(n);
+    if (n < sparse) {
+        w.delta(c[0]);
+        for (uint32_t j = 1; j < n; ++j) w.delta(uint64_t(c[j] - (c[j - 1] + 1)));
+    } else if (n < dense) {
+        uint64_t acc = 0;
+        uint32_t filled = 0, j = 0;
+        for (uint32_t v = 0; v < num_colors; ++v) {
+            const bool bit = j < n && c[j] == v;
+            if (bit) ++j;
+            acc |= uint64_t(bit) << filled;
+            if (++filled == 64) {
+                w.put(acc, 64);
+                acc = 0;
+                filled = 0;
+            }
+        }
+        w.put(acc, filled);
+    } else {
+        uint32_t j = 0, prev = 0xFFFFFFFFu;
+        for (uint32_t v = 0; v < num_colors; ++v) {
+            if (j < n && c[j] == v) {
+                ++j;
+                continue;
+            }
+            w.delta(uint64_t(v - (prev + 1)));
+            prev = v;
+        }
+    }
+}
+
+__global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
+                           const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                           const uint32_t* __restrict__ node_slot, const uint32_t* __restrict__ single_cid,
+                           const uint32_t* __restrict__ mt_cid, uint32_t* __restrict__ node_cid) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    uint32_t cid = NONE32;
+    if (meta[n] & 4u) cid = ncol[n] == 1 ? single_cid[colors[coff[n]]] : mt_cid[node_slot[n]];
+    node_cid[n] = cid;
+}
+
+// ---------------------------------------------------------------------------
+// per-bucket k-mer table, masks, links
+// ---------------------------------------------------------------------------
+template <class K>
+__global__ void k_table_insert(uint32_t n_nodes, const K* __restrict__ node_key, const uint64_t* __restrict__ node_off,
+                               uint32_t B, const uint64_t* __restrict__ tab_off, Slot<K>* __restrict__ tab) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const uint32_t b = upper_bucket(node_off, B, n);
+    const uint64_t base = tab_off[b], cap = tab_off[b + 1] - base;
+    const K key = node_key[n];
+    uint64_t s = KOps<K>::hash(key) & (cap - 1);
+    for (uint64_t probe = 0; probe < cap; ++probe) {
+        if (atomicCAS(&tab[base + s].node, NONE32, uint32_t(n)) == NONE32) {
+            tab[base + s].key = key;
+            return;
+        }
+        s = (s + 1) & (cap - 1);
+    }
+}
+
+template <class K>
+__device__ __forceinline__ uint32_t tab_find(const Slot<K>* __restrict__ tab, uint64_t base, uint64_t cap, const K& key) {
+    uint64_t s = KOps<K>::hash(key) & (cap - 1);
+    for (uint64_t probe = 0; probe < cap; ++probe) {
+        const uint32_t nd = tab[base + s].node;
+        if (nd == NONE32) return NONE32;
+        if (KOps<K>::eq(tab[base + s].key, key)) return nd;
+        s = (s + 1) & (cap - 1);
+    }
+    return NONE32;
+}
+
+// mask bits 0..3: successors of the canonical k-mer, 4..7: successors of its
+// reverse complement (local_ext_mask, bucket_walker.hpp:260-271).  For a side
+// with exactly one present extension, nb[2n+port] = (neighbour << 1) | entry
+// port: the neighbour is entered at its L side iff the extension is canonical.
+template <class K>
+__global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint64_t* __restrict__ node_off,
+                        uint32_t B, const uint64_t* __restrict__ tab_off, const Slot<K>* __restrict__ tab, uint32_t k,
+                        uint8_t* __restrict__ mask_out, uint32_t* __restrict__ nb) {
+    using O = KOps<K>;
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const uint32_t b = upper_bucket(node_off, B, n);
+    const uint64_t base = tab_off[b], cap = tab_off[b + 1] - base;
+    const K can = node_key[n];
+    const K rv = O::rc(can, k);
+    uint32_t mask = 0, yR = NONE32, yL = NONE32, qR = 0, qL = 0;
+    for (uint32_t nt = 0; nt < 4; ++nt) {
+        K e = O::shift_append(can, nt, k);
+        K er = O::rc(e, k);
+        bool ec = O::le(e, er);
+        uint32_t y = tab_find<K>(tab, base, cap, ec ? e : er);
+        if (y != NONE32) {
+            mask |= 1u << nt;
+            yR = y;
+            qR = ec ? PORT_L : PORT_R;
+        }
+        e = O::shift_append(rv, nt, k);
+        er = O::rc(e, k);
+        ec = O::le(e, er);
+        y = tab_find<K>(tab, base, cap, ec ? e : er);
+        if (y != NONE32) {
+            mask |= 1u << (4 + nt);
+            yL = y;
+            qL = ec ? PORT_L : PORT_R;
+        }
+    }
+    mask_out[n] = uint8_t(mask);
+    nb[2 * n + PORT_R] = __popc(mask & 0xfu) == 1 ? (yR << 1) | qR : NONE32;
+    nb[2 * n + PORT_L] = __popc(mask >> 4) == 1 ? (yL << 1) | qL : NONE32;
+}
+
+__device__ __forceinline__ uint32_t nib(uint32_t m, uint32_t port) { return port == PORT_R ? (m & 0xfu) : (m >> 4); }
+
+__global__ void k_links(uint32_t n_nodes, const uint8_t* __restrict__ mask, const uint32_t* __restrict__ nb,
+                        uint32_t* __restrict__ link) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    for (uint32_t p = 0; p < 2; ++p) {
+        uint32_t l = nb[2 * n + p];
+        if (l != NONE32 && __popc(nib(mask[l >> 1], l & 1u)) != 1) l = NONE32;
+        link[2 * n + p] = l;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// walk: one thread per component
+// ---------------------------------------------------------------------------
+struct WalkArgs {
+    uint32_t n_nodes;
+    const uint32_t* link;
+    const uint8_t* meta;
+    const uint8_t* mask;
+    const uint32_t* rank;
+    const uint64_t* node_off;
+    uint32_t B;
+    uint8_t* visited;
+    uint32_t* frag_nodes;
+    uint8_t* frag_rc;
+    uint64_t* frag_off;
+    uint32_t* frag_n;
+    uint8_t* frag_open;
+    uint64_t* frag_key;
+    unsigned long long* counters;  // [3] nodes placed, [4] fragments, [5] errors
+};
+
+__device__ __forceinline__ bool is_break(uint8_t meta) {
+    const uint32_t f = meta & 3u;
+    return f == BOUND_LEFT || f == BOUND_RIGHT;
+}
+
+// emit_from_seed (bucket_walker.hpp:451-542) over the link graph.
+__device__ void emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
+    a.visited[s] = 1;
+    const uint32_t sflags = a.meta[s] & 3u;
+    const uint32_t ms = a.mask[s];
+    const uint32_t limit = a.n_nodes + 1;
+    uint32_t nbk = 0, nfw = 0;
+    bool open_l = false, open_r = false;
+    if (sflags == BOUND_LEFT) {
+        open_l = __popc(ms >> 4) <= 1;
+    } else {
+        uint32_t cur = s, ex = PORT_L;
+        for (uint32_t h = 0; h < limit; ++h) {
+            const uint32_t l = a.link[2 * cur + ex];
+            if (l == NONE32) break;
+            const uint32_t y = l >> 1, q = l & 1u;
+            if (a.visited[y]) break;
+            a.visited[y] = 1;
+            ++nbk;
+            cur = y;
+            ex = 1u - q;
+            if (is_break(a.meta[y])) {
+                open_l = __popc(nib(a.mask[y], ex)) <= 1;
+                break;
+            }
+        }
+    }
+    if (sflags == BOUND_RIGHT) {
+        open_r = __popc(ms & 0xfu) <= 1;
+    } else {
+        uint32_t cur = s, ex = PORT_R;
+        for (uint32_t h = 0; h < limit; ++h) {
+            const uint32_t l = a.link[2 * cur + ex];
+            if (l == NONE32) break;
+            const uint32_t y = l >> 1, q = l & 1u;
+            if (a.visited[y]) break;
+            a.visited[y] = 1;
+            ++nfw;
+            cur = y;
+            ex = 1u - q;
+            if (is_break(a.meta[y])) {
+                open_r = __popc(nib(a.mask[y], ex)) <= 1;
+                break;
+            }
+        }
+    }
+    const uint32_t total = nbk + 1 + nfw;
+    const uint64_t off = atomicAdd(&a.counters[3], (unsigned long long)total);
+    const uint64_t fi = atomicAdd(&a.counters[4], 1ull);
+    if (off + total > a.n_nodes) {
+        atomicAdd(&a.counters[5], 1ull);
+        return;
+    }
+    uint32_t cur = s, ex = PORT_L;
+    for (uint32_t i = 0; i < nbk; ++i) {  // walking backward: entered at L <=> reverse orientation
+        const uint32_t l = a.link[2 * cur + ex];
+        const uint32_t y = l >> 1, q = l & 1u;
+        a.frag_nodes[off + nbk - 1 - i] = y;
+        a.frag_rc[off + nbk - 1 - i] = q == PORT_L;
+        cur = y;
+        ex = 1u - q;
+    }
+    a.frag_nodes[off + nbk] = s;
+    a.frag_rc[off + nbk] = 0;
+    cur = s;
+    ex = PORT_R;
+    for (uint32_t i = 0; i < nfw; ++i) {  // walking forward: entered at R <=> reverse orientation
+        const uint32_t l = a.link[2 * cur + ex];
+        const uint32_t y = l >> 1, q = l & 1u;
+        a.frag_nodes[off + nbk + 1 + i] = y;
+        a.frag_rc[off + nbk + 1 + i] = q == PORT_R;
+        cur = y;
+        ex = 1u - q;
+    }
+    a.frag_off[fi] = off;
+    a.frag_n[fi] = total;
+    a.frag_open[fi] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
+    a.frag_key[fi] = (uint64_t(b) << 32) | a.rank[s];
+}
+
+// Runs the rank-ordered seed loop over the component that contains `x`
+// (a path end, or the smallest node of a cycle).
+__device__ void simulate_component(const WalkArgs& a, uint32_t x, bool cycle) {
+    const uint32_t b = upper_bucket(a.node_off, a.B, x);
+    const uint32_t limit = a.n_nodes + 1;
+    uint32_t ex0 = PORT_R;
+    if (!cycle) ex0 = a.link[2 * x + PORT_R] == NONE32 ? PORT_L : PORT_R;
+    for (uint32_t guard = 0; guard < limit; ++guard) {
+        uint32_t best = NONE32, best_rank = 0xFFFFFFFFu;
+        uint32_t cur = x, ex = ex0;
+        for (uint32_t h = 0; h < limit; ++h) {
+            if (!a.visited[cur] && a.rank[cur] <= best_rank) {
+                if (a.rank[cur] < best_rank || best == NONE32) {
+                    best = cur;
+                    best_rank = a.rank[cur];
+                }
+            }
+            const uint32_t l = a.link[2 * cur + ex];
+            if (l == NONE32) break;
+            const uint32_t y = l >> 1;
+            if (cycle && y == x) break;
+            cur = y;
+            ex = 1u - (l & 1u);
+        }
+        if (best == NONE32) return;
+        emit_fragment(a, best, b);
+    }
+    atomicAdd(&a.counters[5], 1ull);
+}
+
+__global__ void k_walk_paths(WalkArgs a) {
+    const uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (x >= a.n_nodes) return;
+    const uint32_t lL = a.link[2 * x + PORT_L], lR = a.link[2 * x + PORT_R];
+    if (lL != NONE32 && lR != NONE32) return;  // not a path end
+    // walk to the other end; the end with the smaller index owns the path
+    uint32_t cur = uint32_t(x), ex = lR == NONE32 ? PORT_L : PORT_R;
+    const uint32_t limit = a.n_nodes + 1;
+    for (uint32_t h = 0; h < limit; ++h) {
+        const uint32_t l = a.link[2 * cur + ex];
+        if (l == NONE32) break;
+        cur = l >> 1;
+        ex = 1u - (l & 1u);
+    }
+    if (cur < x) return;
+    simulate_component(a, uint32_t(x), false);
+}
+
+__global__ void k_walk_cycles(WalkArgs a) {
+    const uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (x >= a.n_nodes || a.visited[x]) return;
+    uint32_t cur = uint32_t(x), ex = PORT_R;
+    const uint32_t limit = a.n_nodes + 1;
+    for (uint32_t h = 0; h < limit; ++h) {
+        const uint32_t l = a.link[2 * cur + ex];
+        if (l == NONE32) {  // not a cycle: should have been a path
+            atomicAdd(&a.counters[5], 1ull);
+            return;
+        }
+        cur = l >> 1;
+        ex = 1u - (l & 1u);
+        if (cur == x) break;
+        if (cur < x) return;
+    }
+    simulate_component(a, uint32_t(x), true);
+}
+
+// ---------------------------------------------------------------------------
+// fragment records
+// ---------------------------------------------------------------------------
+__global__ void k_frag_size(uint64_t n_frags, const uint32_t* __restrict__ order, const uint64_t* __restrict__ frag_off,
+                            const uint32_t* __restrict__ frag_n, const uint32_t* __restrict__ frag_nodes,
+                            const uint32_t* __restrict__ node_cid, uint32_t k, uint64_t* __restrict__ size) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_frags) return;
+    const uint32_t f = order[i];
+    const uint64_t o = frag_off[f];
+    const uint32_t n = frag_n[f];
+    uint32_t runs = 0, last = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t c = node_cid[frag_nodes[o + j]];
+        if (j == 0 || c != last) ++runs;
+        last = c;
+    }
+    size[i] = 1 + 4 + 12ull * runs + 4 + (k + n - 1);
+}
+
+__device__ __forceinline__ void put_bytes(uint8_t* p, uint64_t v, int n) {
+    for (int i = 0; i < n; ++i) p[i] = uint8_t(v >> (8 * i));
+}
+
+template <class K>
+__global__ void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ order, const uint64_t* __restrict__ frag_off,
+                             const uint32_t* __restrict__ frag_n, const uint8_t* __restrict__ frag_open,
+                             const uint32_t* __restrict__ frag_nodes, const uint8_t* __restrict__ frag_rc,
+                             const uint32_t* __restrict__ node_cid, const K* __restrict__ node_key, uint32_t k,
+                             const uint64_t* __restrict__ boff, uint8_t* __restrict__ out) {
+    using O = KOps<K>;
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_frags) return;
+    const uint32_t f = order[i];
+    const uint64_t o = frag_off[f];
+    const uint32_t n = frag_n[f];
+    uint8_t* p = out + boff[i];
+    p[0] = frag_open[f];
+    uint8_t* pr = p + 5;
+    uint32_t runs = 0, last = 0, cnt = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t c = node_cid[frag_nodes[o + j]];
+        if (j > 0 && c != last) {
+            put_bytes(pr, last == NONE32 ? ~0ull : uint64_t(last), 8);
+            put_bytes(pr + 8, cnt, 4);
+            pr += 12;
+            ++runs;
+            cnt = 0;
+        }
+        last = c;
+        ++cnt;
+    }
+    put_bytes(pr, last == NONE32 ? ~0ull : uint64_t(last), 8);
+    put_bytes(pr + 8, cnt, 4);
+    pr += 12;
+    ++runs;
+    put_bytes(p + 1, runs, 4);
+    const uint32_t seq_len = k + n - 1;
+    put_bytes(pr, seq_len, 4);
+    uint8_t* ps = pr + 4;
+    const char ACGT[4] = {'A', 'C', 'G', 'T'};
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t nd = frag_nodes[o + j];
+        const K key = node_key[nd];
+        const K km = frag_rc[o + j] ? O::rc(key, k) : key;
+        if (j == 0) {
+            for (uint32_t t = 0; t < k; ++t) ps[t] = ACGT[O::base(km, t, k)];
+        } else {
+            ps[k + j - 1] = ACGT[O::base(km, k - 1, k)];
+        }
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+struct Phase2Bufs {
+    DevBuf kcnt, tup_off, btup, total;
+    DevBuf keys, vals, keys_t, vals_t;
+    DevBuf heads, hscan, node_start, node_off;
+    DevBuf node_key, node_meta, node_rank, node_ncol, node_coff, colors;
+    DevBuf counters;
+    DevBuf single_min, single_cid, mt_hash, mt_key, mt_rep, mt_cid, node_slot;
+    DevBuf set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
+    DevBuf cs_words;
+    DevBuf node_cid, tab_off, tab, node_mask, nb, link, visited;
+    DevBuf frag_nodes, frag_rc, frag_off, frag_n, frag_open, frag_key;
+    DevBuf fkey_t, forder, forder_t, fsize, fboff, frag_bytes;
+    RadixScratch rs2;
+    uint64_t n_tuples = 0, n_nodes = 0, n_sets = 0, n_frags = 0, frag_total = 0, cs_bits = 0;
+};
+
+template <class T>
+static T d2h(const void* d, hipStream_t st) {
+    T v;
+    CDBG_HIP(hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    return v;
+}
+
+static int bits_for(uint64_t n) {
+    int b = 0;
+    while ((uint64_t(1) << b) < n) ++b;
+    return b;
+}
+
+template <class K>
+static void phase2_impl(Job& J) {
+    if (!J.p2) J.p2 = new Phase2Bufs();
+    Phase2Bufs& P = *J.p2;
+    Phase1Bufs& Q = J.p1;
+    hipStream_t st = J.st;
+    const uint32_t k = J.k, B = J.B;
+    const uint64_t ns = J.n_supers;
+    if (J.num_colors >= (1u << 29)) throw Error("more than 2^29 colours");
+
+    hipEvent_t e0, e1;
+    CDBG_HIP(hipEventCreate(&e0));
+    CDBG_HIP(hipEventCreate(&e1));
+    CDBG_HIP(hipEventRecord(e0, st));
+
+    // --- expand
+    uint32_t* kcnt = P.kcnt.ensure<uint32_t>(ns);
+    uint64_t* tup_off = P.tup_off.ensure<uint64_t>(ns + 1);
+    uint64_t* dtot = P.total.ensure<uint64_t>(4);
+    if (ns) k_count_kmers<<<blocks_for(ns), TPB, 0, st>>>(ns, Q.b_len.as<uint32_t>(), k, kcnt);
+    exclusive_scan<uint32_t, uint64_t>(kcnt, tup_off, ns, dtot, J.rs.scan_tmp, st);
+    const uint64_t nt = d2h<uint64_t>(dtot, st);
+    P.n_tuples = nt;
+    if (nt >= (1ull << 32)) throw Error("phase 2: more than 2^32 k-mer occurrences in one job");
+    uint64_t* btup = P.btup.ensure<uint64_t>(B + 1);
+    k_bucket_tuple_off<<<blocks_for(B + 1), TPB, 0, st>>>(B, Q.bucket_off.as<uint64_t>(), tup_off, ns, nt, btup);
+    std::vector<uint64_t> h_btup(B + 1);
+    CDBG_HIP(hipMemcpyAsync(h_btup.data(), btup, (B + 1) * 8, hipMemcpyDeviceToHost, st));
+    K* keys = P.keys.ensure<K>(nt);
+    uint64_t* vals = P.vals.ensure<uint64_t>(nt);
+    K* keys_t = P.keys_t.ensure<K>(nt);
+    uint64_t* vals_t = P.vals_t.ensure<uint64_t>(nt);
+    if (ns)
+        k_expand<K><<<blocks_for(ns), TPB, 0, st>>>(ns, Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(),
+                                                    Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(),
+                                                    Q.key.as<uint32_t>(), tup_off, btup, Q.packed.as<uint32_t>(), k,
+                                                    keys, vals);
+    CDBG_HIP(hipStreamSynchronize(st));
+
+    // --- sort tuples by k-mer inside every bucket
+    SortPlan plan;
+    plan.build(h_btup);
+    segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, 0, 2 * int(k), 8, P.rs2, st);
+
+    // --- nodes
+    uint32_t* heads = P.heads.ensure<uint32_t>(nt);
+    uint32_t* hscan = P.hscan.ensure<uint32_t>(nt + 1);
+    if (nt) k_heads<K><<<blocks_for(nt), TPB, 0, st>>>(nt, keys, heads);
+    k_mark_bucket_starts<<<blocks_for(B), TPB, 0, st>>>(B, btup, heads);
+    uint32_t* dtot32 = (uint32_t*)(dtot + 1);
+    exclusive_scan<uint32_t, uint32_t>(heads, hscan, nt, dtot32, J.rs.scan_tmp, st);
+    const uint32_t nn = d2h<uint32_t>(dtot32, st);
+    P.n_nodes = nn;
+    if (nn >= (1u << 31)) throw Error("phase 2: more than 2^31 nodes in one job");
+    uint32_t* node_start = P.node_start.ensure<uint32_t>(uint64_t(nn) + 1);
+    uint64_t* node_off = P.node_off.ensure<uint64_t>(B + 1);
+    k_node_start<<<blocks_for(nt), TPB, 0, st>>>(nt, heads, hscan, nn, node_start);
+    k_node_off<<<blocks_for(B + 1), TPB, 0, st>>>(B, btup, nt, hscan, nn, node_off);
+    K* node_key = P.node_key.ensure<K>(nn);
+    uint8_t* meta = P.node_meta.ensure<uint8_t>(nn);
+    uint32_t* rank = P.node_rank.ensure<uint32_t>(nn);
+    uint32_t* ncol = P.node_ncol.ensure<uint32_t>(nn);
+    uint64_t* coff = P.node_coff.ensure<uint64_t>(uint64_t(nn) + 1);
+    unsigned long long* cnt = P.counters.ensure<unsigned long long>(8);
+    k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, 8, 0ull);
+    k_node_reduce<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, keys, vals, node_key, meta, rank, ncol, cnt);
+    exclusive_scan<uint32_t, uint64_t>(ncol, coff, nn, coff + nn, J.rs.scan_tmp, st);
+    const uint64_t n_colors_total = d2h<uint64_t>(coff + nn, st);
+    uint32_t* colors = P.colors.ensure<uint32_t>(n_colors_total);
+    k_node_colors<<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, vals, coff, colors);
+    unsigned long long h_cnt[8];
+    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1];
+
+    // --- colour sets
+    const uint32_t NC = J.num_colors;
+    uint64_t mt_cap = 64;
+    while (mt_cap < 2 * n_multi) mt_cap <<= 1;
+    auto* single_min = P.single_min.ensure<unsigned long long>(NC);
+    uint32_t* single_cid = P.single_cid.ensure<uint32_t>(NC);
+    auto* mt_hash = P.mt_hash.ensure<unsigned long long>(mt_cap);
+    auto* mt_key = P.mt_key.ensure<unsigned long long>(mt_cap);
+    uint32_t* mt_rep = P.mt_rep.ensure<uint32_t>(mt_cap);
+    uint32_t* mt_cid = P.mt_cid.ensure<uint32_t>(mt_cap);
+    uint32_t* node_slot = P.node_slot.ensure<uint32_t>(nn);
+    k_fill<unsigned long long><<<blocks_for(NC), TPB, 0, st>>>(single_min, NC, ~0ull);
+    k_fill<unsigned long long><<<blocks_for(mt_cap), TPB, 0, st>>>(mt_hash, mt_cap, 0ull);
+    k_fill<unsigned long long><<<blocks_for(mt_cap), TPB, 0, st>>>(mt_key, mt_cap, ~0ull);
+    k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, rank, node_off, B, single_min, mt_hash,
+                                                 mt_key, mt_cap, node_slot);
+    k_set_rep<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, rank, node_off, B, node_slot, mt_key, mt_rep);
+    k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, mt_rep, cnt + 6);
+    const uint64_t nflag = NC + mt_cap;
+    uint32_t* set_fl = P.set_fl.ensure<uint32_t>(nflag);
+    uint64_t* set_pos = P.set_pos.ensure<uint64_t>(nflag);
+    k_set_flags<<<blocks_for(nflag), TPB, 0, st>>>(NC, mt_cap, single_min, mt_hash, set_fl);
+    exclusive_scan<uint32_t, uint64_t>(set_fl, set_pos, nflag, dtot + 2, J.rs.scan_tmp, st);
+    const uint64_t n_sets = d2h<uint64_t>(dtot + 2, st);
+    P.n_sets = n_sets;
+    uint64_t* set_key = P.set_key.ensure<uint64_t>(n_sets);
+    uint64_t* set_desc = P.set_desc.ensure<uint64_t>(n_sets);
+    uint64_t* set_key_t = P.set_key_t.ensure<uint64_t>(n_sets);
+    uint64_t* set_desc_t = P.set_desc_t.ensure<uint64_t>(n_sets);
+    k_set_emit<<<blocks_for(nflag), TPB, 0, st>>>(NC, mt_cap, set_fl, set_pos, single_min, mt_key, set_key, set_desc);
+    SortPlan splan;
+    splan.build({0, n_sets});
+    const int bbits = bits_for(B);
+    segmented_radix_sort<uint64_t, uint64_t>(set_key, set_desc, set_key_t, set_desc_t, splan, 0, 32 + bbits, 8, P.rs2,
+                                             st);
+    uint64_t* set_bits = P.set_bits.ensure<uint64_t>(n_sets);
+    uint64_t* set_boff = P.set_boff.ensure<uint64_t>(n_sets + 1);
+    if (n_sets)
+        k_set_cid_len<<<blocks_for(n_sets), TPB, 0, st>>>(n_sets, set_desc, mt_rep, coff, ncol, colors, NC, single_cid,
+                                                         mt_cid, set_bits, cnt);
+    exclusive_scan<uint64_t, uint64_t>(set_bits, set_boff, n_sets, set_boff + n_sets, J.rs.scan_tmp, st);
+    const uint64_t cs_bits = d2h<uint64_t>(set_boff + n_sets, st);
+    P.cs_bits = cs_bits;
+    const uint64_t n_words = (cs_bits + 63) / 64;
+    auto* words = P.cs_words.ensure<unsigned long long>(n_words + 1);
+    k_fill<unsigned long long><<<blocks_for(n_words + 1), TPB, 0, st>>>(words, n_words + 1, 0ull);
+    if (n_sets)
+        k_set_encode<<<blocks_for(n_sets), TPB, 0, st>>>(n_sets, set_desc, set_boff, mt_rep, coff, ncol, colors, NC,
+                                                        words);
+    uint32_t* node_cid = P.node_cid.ensure<uint32_t>(nn);
+    k_node_cid<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, single_cid, mt_cid, node_cid);
+
+    // --- per-bucket k-mer table
+    std::vector<uint64_t> h_node_off(B + 1);
+    CDBG_HIP(hipMemcpyAsync(h_node_off.data(), node_off, (B + 1) * 8, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> h_tab_off(B + 1, 0);
+    for (uint32_t b = 0; b < B; ++b) {
+        const uint64_t nb_ = h_node_off[b + 1] - h_node_off[b];
+        uint64_t cap = 8;
+        while (cap * 5 < nb_ * 8) cap <<= 1;  // load factor <= 0.625
+        h_tab_off[b + 1] = h_tab_off[b] + (nb_ ? cap : 0);
+    }
+    uint64_t* tab_off = P.tab_off.ensure<uint64_t>(B + 1);
+    CDBG_HIP(hipMemcpyAsync(tab_off, h_tab_off.data(), (B + 1) * 8, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipStreamSynchronize(st));  // h_tab_off is pageable and local
+    const uint64_t tab_n = h_tab_off[B];
+    Slot<K>* tab = P.tab.ensure<Slot<K>>(tab_n);
+    Slot<K> empty{};
+    empty.node = NONE32;
+    k_fill<Slot<K>><<<blocks_for(tab_n), TPB, 0, st>>>(tab, tab_n, empty);
+    k_table_insert<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, B, tab_off, tab);
+    uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
+    uint32_t* nb = P.nb.ensure<uint32_t>(2ull * nn);
+    uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
+    k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, B, tab_off, tab, k, node_mask, nb);
+    k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nb, link);
+
+    // --- walk
+    uint8_t* visited = P.visited.ensure<uint8_t>(nn);
+    k_fill<uint8_t><<<blocks_for(nn), TPB, 0, st>>>(visited, nn, uint8_t(0));
+    WalkArgs wa{};
+    wa.n_nodes = nn;
+    wa.link = link;
+    wa.meta = meta;
+    wa.mask = node_mask;
+    wa.rank = rank;
+    wa.node_off = node_off;
+    wa.B = B;
+    wa.visited = visited;
+    wa.frag_nodes = P.frag_nodes.ensure<uint32_t>(nn);
+    wa.frag_rc = P.frag_rc.ensure<uint8_t>(nn);
+    wa.frag_off = P.frag_off.ensure<uint64_t>(nn);
+    wa.frag_n = P.frag_n.ensure<uint32_t>(nn);
+    wa.frag_open = P.frag_open.ensure<uint8_t>(nn);
+    wa.frag_key = P.frag_key.ensure<uint64_t>(nn);
+    wa.counters = cnt;
+    k_walk_paths<<<blocks_for(nn), TPB, 0, st>>>(wa);
+    k_walk_cycles<<<blocks_for(nn), TPB, 0, st>>>(wa);
+    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    if (h_cnt[6]) throw Error("phase 2: colour-set content hash collision (" + std::to_string(h_cnt[6]) + " nodes)");
+    if (h_cnt[5] || h_cnt[3] != nn)
+        throw Error("phase 2: walk covered " + std::to_string(h_cnt[3]) + " of " + std::to_string(nn) +
+                    " nodes (" + std::to_string(h_cnt[5]) + " errors)");
+    const uint64_t nf = h_cnt[4];
+    P.n_frags = nf;
+
+    // --- fragments in (bucket, seed rank) order
+    uint64_t* fkey_t = P.fkey_t.ensure<uint64_t>(nf);
+    uint32_t* forder = P.forder.ensure<uint32_t>(nf);
+    uint32_t* forder_t = P.forder_t.ensure<uint32_t>(nf);
+    k_iota<<<blocks_for(nf), TPB, 0, st>>>(forder, nf);
+    SortPlan fplan;
+    fplan.build({0, nf});
+    segmented_radix_sort<uint64_t, uint32_t>(wa.frag_key, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st);
+    uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
+    uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
+    if (nf)
+        k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_nodes, node_cid, k,
+                                                    fsize);
+    exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
+    const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
+    P.frag_total = fbytes;
+    uint8_t* fout = P.frag_bytes.ensure<uint8_t>(fbytes);
+    if (nf)
+        k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_nodes,
+                                                        wa.frag_rc, node_cid, node_key, k, fboff, fout);
+    CDBG_HIP(hipGetLastError());
+    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipEventRecord(e1, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    float ms = 0;
+    CDBG_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+
+    J.stats.num_occ = nt;
+    J.stats.num_tuples = nt;
+    J.stats.num_nodes = nn;
+    J.stats.num_kmers = n_primary;
+    J.stats.num_color_sets = n_sets;
+    J.stats.num_frags = nf;
+    J.stats.frag_bytes = fbytes;
+    J.stats.cs_bits = cs_bits;
+    J.stats.total_integers = h_cnt[2];
+    J.stats.ms_phase2 = ms;
+    J.phase2_done = true;
+}
+
+void run_phase2(Job& J) {
+    if (J.k <= 31)
+        phase2_impl<uint64_t>(J);
+    else
+        phase2_impl<Key128>(J);
+}
 
 void free_phase2(Job& J) {
     delete J.p2;
@@ -19,11 +1078,34 @@ void free_phase2(Job& J) {
 
 }  // namespace cdbg_amd
 
+using namespace cdbg_amd;
+
 extern "C" {
+
 CDBG_API int cdbg_gpu_job_frags(cdbg_gpu_job* job, uint8_t* dst) {
-    return cdbg_amd::set_error("phase 2 not built yet");
+    try {
+        Job& J = job->J;
+        if (!J.phase2_done) throw Error("phase 2 has not run");
+        if (J.p2->frag_total)
+            CDBG_HIP(hipMemcpy(dst, J.p2->frag_bytes.p, J.p2->frag_total, hipMemcpyDeviceToHost));
+        return 0;
+    } catch (std::exception& e) {
+        return set_error(e.what());
+    }
 }
+
 CDBG_API int cdbg_gpu_job_color_sets(cdbg_gpu_job* job, uint64_t* words, uint64_t* offsets) {
-    return cdbg_amd::set_error("phase 2 not built yet");
+    try {
+        Job& J = job->J;
+        if (!J.phase2_done) throw Error("phase 2 has not run");
+        const uint64_t nw = (J.p2->cs_bits + 63) / 64;
+        if (nw) CDBG_HIP(hipMemcpy(words, J.p2->cs_words.p, nw * 8, hipMemcpyDeviceToHost));
+        if (J.p2->n_sets)
+            CDBG_HIP(hipMemcpy(offsets, J.p2->set_boff.p, J.p2->n_sets * 8, hipMemcpyDeviceToHost));
+        return 0;
+    } catch (std::exception& e) {
+        return set_error(e.what());
+    }
 }
-}
+
+}  // extern "C"

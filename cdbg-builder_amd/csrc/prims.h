@@ -308,7 +308,10 @@ inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan&
     if (plan.n >= (uint64_t(1) << 32)) throw Error("segmented_radix_sort: more than 2^32 elements");
     const uint32_t nt = uint32_t(plan.tiles.size());
     TileDesc* d_tiles = S.tiles.ensure<TileDesc>(nt);
+    // The previous sort may still read d_tiles: order the upload on the stream
+    // and wait for it, since plan.tiles is pageable host memory.
     CDBG_HIP(hipMemcpyAsync(d_tiles, plan.tiles.data(), nt * sizeof(TileDesc), hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipStreamSynchronize(st));
     K* src_k = keys;
     V* src_v = vals;
     K* dst_k = kt;

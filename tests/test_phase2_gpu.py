@@ -1,0 +1,111 @@
+"""Phase 2 on the GPU, bit-exact against the oracle (the reference's
+process_buckets semantics at -t 1 over the same record stream):
+
+* the frag_unitigs.bin byte stream (every fragment's open flags, colour runs
+  with global cids, sequence, in (bucket, seed rank) order);
+* the colour-set bit stream and per-class bit offsets;
+* num_kmers, node, fragment and colour-set counts.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import cdbg_gpu as G
+
+pytestmark = pytest.mark.gpu
+ACGT = np.frombuffer(b"ACGT", np.uint8)
+
+
+def _run_both(ctx, data, off, k, m, B):
+    ob = O.Build(data, off, k, m, B, threads=4)
+    with ctx.job(data, off, k, m, B) as job:
+        job.run()
+        s = job.stats()
+        gf = job.frags()
+        gw, go = job.color_sets()
+    return ob, s, gf, gw, go
+
+
+def _assert_same(ob, s, gf, gw, go):
+    assert s["num_kmers"] == ob.stat("num_kmers")
+    assert s["num_nodes"] == ob.stat("num_nodes")
+    assert s["num_frags"] == ob.stat("num_frags")
+    assert s["num_color_sets"] == ob.stat("num_sets")
+    assert s["cs_bits"] == ob.stat("cs_bits")
+    assert s["total_integers"] == ob.stat("total_integers")
+    of = ob.frags()
+    if not np.array_equal(gf, of):
+        n = min(len(gf), len(of))
+        first = int(np.argmax(gf[:n] != of[:n])) if np.any(gf[:n] != of[:n]) else n
+        raise AssertionError(f"frag stream differs (gpu {len(gf)} B, oracle {len(of)} B), first byte {first}")
+    ow, oo = ob.color_sets()
+    assert np.array_equal(go, oo), "colour-set offsets differ"
+    assert np.array_equal(gw, ow), "colour-set words differ"
+
+
+@pytest.mark.parametrize("k,m,B,colors,glen", [
+    (31, 19, 64, 10, 20000),     # the small plain configuration, shrunk
+    (31, 21, 1024, 6, 30000),
+    (31, 12, 16, 3, 15000),
+    (63, 27, 256, 5, 20000),     # 128-bit k-mers
+    (15, 9, 37, 4, 8000),
+    (21, 11, 1, 2, 5000),        # one bucket: no cross-bucket boundaries
+])
+def test_phase2_matches_oracle(gpu_ctx, k, m, B, colors, glen):
+    data, off = G.synth(seed=3, num_colors=colors, genome_len=glen, snp_rate=0.01, indel_rate=0.001)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
+
+
+@pytest.mark.parametrize("num_colors", [1, 3, 8, 40])
+def test_phase2_codec_branches(gpu_ctx, num_colors):
+    """Colour counts that drive the sparse, bitmap and complement branches."""
+    data, off = G.synth(seed=11, num_colors=num_colors, genome_len=6000, snp_rate=0.02, indel_rate=0.002)
+    _assert_same(*_run_both(gpu_ctx, data, off, 31, 19, 32))
+
+
+def test_phase2_repeats_and_cycles(gpu_ctx):
+    """Tandem repeats, homopolymers and reverse-complement palindromes make
+    branching nodes, cycles and contig breaks inside paths."""
+    rng = np.random.default_rng(9)
+    unit = ACGT[rng.integers(0, 4, 37)].tobytes()
+    pal = ACGT[rng.integers(0, 4, 40)].tobytes()
+    rc = pal[::-1].translate(bytes.maketrans(b"ACGT", b"TGCA"))
+    parts = [
+        unit * 30 + ACGT[rng.integers(0, 4, 500)].tobytes(),
+        b"A" * 300 + b"C" * 50 + b"AC" * 100,
+        pal + rc + pal + rc,
+        ACGT[rng.integers(0, 4, 2000)].tobytes() + unit * 5,
+    ]
+    data = b"".join(p + b"\n" for p in parts)
+    off = np.zeros(len(parts) + 1, np.uint64)
+    off[1:] = np.cumsum([len(p) + 1 for p in parts])
+    data = np.frombuffer(data, np.uint8).copy()
+    for (k, m, B) in [(31, 19, 8), (15, 7, 3), (9, 5, 1), (63, 27, 5)]:
+        _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
+
+
+def test_phase2_rerun_same_job(gpu_ctx):
+    data, off = G.synth(seed=4, num_colors=4, genome_len=10000)
+    ob = O.Build(data, off, 31, 19, 64)
+    with gpu_ctx.job(data, off, 31, 19, 64) as job:
+        for _ in range(2):
+            job.run()
+            s = job.stats()
+            _assert_same(ob, s, job.frags(), *job.color_sets())
+
+
+def test_phase2_distinct_kmers_bruteforce(gpu_ctx):
+    """num_kmers equals the number of distinct canonical k-mers (verify.py:103-131)."""
+    data, off = G.synth(seed=5, num_colors=3, genome_len=3000)
+    k = 21
+    kms = set()
+    for c in range(3):
+        seq = bytes(data[off[c]:off[c + 1]])
+        for run in seq.replace(b"\n", b"N").split(b"N"):
+            for i in range(len(run) - k + 1):
+                s = run[i:i + k]
+                r = s[::-1].translate(bytes.maketrans(b"ACGT", b"TGCA"))
+                kms.add(min(s, r))
+    with gpu_ctx.job(data, off, k, 11, 16) as job:
+        job.run()
+        assert job.stats()["num_kmers"] == len(kms)
