@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Phases 1+2 (bucket-write + bucket-process) throughput on one MI355X.
+
+Workload: the bacterial-pangenome-shaped build -- 500 colours derived from a
+5 Mbp uniform-random base genome, each with 2% SNPs, 0.2% indels (1-10 bp),
+1% of bases removed as 1-5 kbp presence/absence deletions and 50 inversions
+of 10-100 kbp; k=31 (64-bit k-mers), m=21, 2^10 buckets.  Sequences are
+generated with the seeded generator and staged in HBM before timing (the
+input files' gzip decode is host work outside the timed region).
+
+A step is one full pass of phase 1 and phase 2 over the resident input,
+producing the frag_unitigs.bin stream and the colour-set stream on the
+device.  value = distinct k-mers (metadata num_kmers) x steps / wall time.
+
+Printed JSON also carries the roofline of the dominant kernel (algorithmic
+bytes per launch / its average HIP-event duration on the launch stream) and
+a CPU baseline: the oracle (a multi-threaded CPU restatement of the same
+phases) timed on a bounded sample of the same workload.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location(
+        "cdbg_gpu", os.path.join(ROOT, "cdbg-builder_amd", "cdbg_gpu.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["cdbg_gpu"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+WORKLOAD = dict(seed=2, snp_rate=0.02, indel_rate=0.002, indel_max=10, pav_frac=0.01,
+                pav_min=1000, pav_max=5000, n_inversions=50, inv_min=10000, inv_max=100000)
+
+
+def alg_bytes(name, s, k):
+    """Algorithmic (compulsory) HBM bytes of ONE launch of `name`."""
+    wk = 8 if k <= 31 else 16
+    n, nn, ns, nb = s["num_tuples"], s["num_nodes"], s["num_supers"], s["num_bases"]
+    table = {
+        # read key+value, write key+value
+        "k_radix_scatter:tuples": 2 * n * (wk + 8),
+        # read keys
+        "k_radix_hist:tuples": n * wk,
+        # read the super descriptors + 2-bit bases, write one tuple per k-mer
+        "k_expand": ns * 29 + nb // 4 + n * (wk + 8),
+        # ASCII in, 2-bit codes + ACGT bitmap out
+        "k_pack2bit": nb + nb // 4 + nb // 8,
+        # 2-bit codes + bitmap in, super starts/ends out
+        "k_superkmers": nb // 4 + nb // 8 + 22 * ns,
+        # tuples in, node records out
+        "k_node_reduce": n * (wk + 8) + nn * (wk + 13),
+        # node keys in, 8 table probes of one 16/32-byte slot, mask + links out
+        "k_masks": nn * wk + 8 * nn * (16 if wk == 8 else 32) + nn * 9,
+    }
+    return table.get(name)
+
+
+def cpu_baseline(G, sample_colors, genome, k, m, B, cores):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    data, off = G.synth(num_colors=sample_colors, genome_len=genome, threads=cores, **WORKLOAD)
+    ob = O.Build(data, off, k, m, B, threads=cores)
+    t1, t2 = ob.times()
+    nk = ob.stat("num_kmers")
+    return {"value": nk / (t1 + t2), "unit": "distinct k-mers/s", "cores": cores, "kind": "port",
+            "sample": f"first {sample_colors} of the workload's colours ({len(data)/1e6:.1f} Mbp), "
+                      f"phases 1+2 of the oracle, {nk} distinct k-mers in {t1 + t2:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--colors", type=int, default=500)
+    ap.add_argument("--genome", type=int, default=5_000_000)
+    ap.add_argument("--k", type=int, default=31)
+    ap.add_argument("--m", type=int, default=21)
+    ap.add_argument("--buckets", type=int, default=1024)
+    ap.add_argument("--cpu-sample-colors", type=int, default=12)
+    ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-times", action="store_true", help="print per-kernel totals to stderr")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU sharding is design-only "
+              "(see DESIGN.md) and no N>1 figure is produced.", file=sys.stderr)
+        sys.exit(2)
+
+    G = load_pkg()
+    t_gen = time.perf_counter()
+    data, off = G.synth(num_colors=args.colors, genome_len=args.genome, threads=args.cpu_cores,
+                        **WORKLOAD)
+    print(f"bench: generated {args.colors} colours, {len(data)/1e9:.2f} GB in "
+          f"{time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    ctx = G.Context(0)
+    job = ctx.job(data, off, args.k, args.m, args.buckets)
+    del data
+    for i in range(args.warmup):
+        job.run()
+        print(f"bench: warmup {i} done ({G.lib().cdbg_gpu_device_bytes() / 2**30:.1f} GiB on device)",
+              file=sys.stderr, flush=True)
+    job.set_profiling(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        job.run()
+    elapsed = time.perf_counter() - t0
+    s = job.stats()
+    kt = job.kernel_times()
+    ms_step = 1e3 * elapsed / args.steps
+    value = s["num_kmers"] * args.steps / elapsed
+    print(f"bench: {args.steps} steps in {elapsed:.2f} s, phase1 {s['ms_phase1']:.1f} ms, "
+          f"phase2 {s['ms_phase2']:.1f} ms (last step)", file=sys.stderr, flush=True)
+    if args.kernel_times:
+        for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
+            print(f"  {name:32s} {cnt:6d} {ms:10.2f} ms", file=sys.stderr)
+
+    # dominant kernel with a known algorithmic-byte count
+    roof = None
+    for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
+        b = alg_bytes(name, s, args.k)
+        if b is None or cnt == 0:
+            continue
+        avg_s = ms / cnt / 1e3
+        ach = b / avg_s / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": b, "avg_launch_ms": round(ms / cnt, 3),
+                "share_of_step": round(ms / args.steps / ms_step, 3)}
+        break
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(G, args.cpu_sample_colors, args.genome, args.k, args.m, args.buckets,
+                           args.cpu_cores)
+    out = {
+        "metric": "distinct k-mers/s (phases 1+2)",
+        "value": round(value, 1),
+        "unit": "distinct k-mers/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64" if args.k <= 31 else "u128",
+        "data": "synthetic",
+        "config": {"workload": f"pangenome {args.colors} colours x {args.genome/1e6:g} Mbp, 2% SNP, "
+                               f"0.2% indel, 1% PAV, 50 inversions; k={args.k} m={args.m} "
+                               f"B={args.buckets}; plain single-job path, input resident in HBM",
+                   "num_kmers": s["num_kmers"], "num_occ": s["num_occ"],
+                   "num_color_sets": s["num_color_sets"], "num_frags": s["num_frags"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    job.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -48,6 +48,11 @@ _SIGS = {
     "cdbg_gpu_job_stats": (C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     "cdbg_gpu_job_supers": (C.c_int, [C.c_void_p, C.c_int, U64P, U32P, U8P, U32P, U32P]),
     "cdbg_gpu_job_bucket_offsets": (C.c_int, [C.c_void_p, U64P]),
+    "cdbg_gpu_job_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "cdbg_gpu_job_kernel_times": (C.c_int64, [C.c_void_p, C.c_char_p, C.c_uint64]),
+    "cdbg_gpu_device_bytes": (C.c_uint64, []),
+    "cdbg_gpu_job_set_chunk_tuples": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "cdbg_gpu_job_num_chunks": (C.c_int64, [C.c_void_p]),
     "cdbg_gpu_job_frags": (C.c_int, [C.c_void_p, U8P]),
     "cdbg_gpu_job_color_sets": (C.c_int, [C.c_void_p, U64P, U64P]),
     "cdbg_synth_create": (C.c_void_p, [C.POINTER(SynthParams), C.c_uint32]),
@@ -160,6 +165,26 @@ class Job:
         _check(lib().cdbg_gpu_job_supers(self.h, which, _ptr(start, U64P), _ptr(ln, U32P),
                                          _ptr(fl, U8P), _ptr(bk, U32P), _ptr(col, U32P)))
         return {"start": start, "len": ln, "flags": fl, "bucket": bk, "color": col}
+
+    def set_chunk_tuples(self, n):
+        _check(lib().cdbg_gpu_job_set_chunk_tuples(self.h, n))
+
+    def num_chunks(self):
+        return lib().cdbg_gpu_job_num_chunks(self.h)
+
+    def set_profiling(self, on=True):
+        _check(lib().cdbg_gpu_job_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self):
+        """{kernel: (launches, total_ms)} accumulated since profiling was enabled."""
+        n = lib().cdbg_gpu_job_kernel_times(self.h, None, 0)
+        buf = C.create_string_buffer(int(n) + 1)
+        lib().cdbg_gpu_job_kernel_times(self.h, buf, n + 1)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms = line.split()
+            out[name] = (int(cnt), float(ms))
+        return out
 
     def bucket_offsets(self):
         out = np.zeros(self.num_buckets + 1, np.uint64)

@@ -3,6 +3,7 @@
 // thread-local message returned by cdbg_gpu_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -105,7 +106,10 @@ CDBG_API int cdbg_gpu_job_phase1(cdbg_gpu_job* job) {
         if (!job) throw Error("null job");
         CDBG_HIP(hipSetDevice(job->J.ctx->device));
         job->J.phase2_done = false;
+        g_prof = &job->J.prof;
         run_phase1(job->J);
+        job->J.prof.collect();
+        g_prof = nullptr;
     })
 }
 
@@ -114,7 +118,10 @@ CDBG_API int cdbg_gpu_job_phase2(cdbg_gpu_job* job) {
         if (!job) throw Error("null job");
         if (!job->J.phase1_done) throw Error("phase 2 requires phase 1");
         CDBG_HIP(hipSetDevice(job->J.ctx->device));
+        g_prof = &job->J.prof;
         run_phase2(job->J);
+        job->J.prof.collect();
+        g_prof = nullptr;
     })
 }
 
@@ -158,6 +165,29 @@ CDBG_API int cdbg_gpu_job_supers(cdbg_gpu_job* job, int which, uint64_t* start, 
         CDBG_HIP(hipStreamSynchronize(J.st));
     })
 }
+
+CDBG_API int cdbg_gpu_job_set_profiling(cdbg_gpu_job* job, int on) {
+    CDBG_TRY({
+        if (!job) throw Error("null job");
+        job->J.prof.on = on != 0;
+        job->J.prof.stats.clear();
+    })
+}
+
+CDBG_API int64_t cdbg_gpu_job_kernel_times(cdbg_gpu_job* job, char* buf, uint64_t cap) {
+    if (!job) return set_error("null job");
+    std::string txt;
+    for (auto& kv : job->J.prof.stats)
+        txt += kv.first + " " + std::to_string(kv.second.launches) + " " + std::to_string(kv.second.ms) + "\n";
+    if (buf && cap) {
+        size_t n = std::min<size_t>(txt.size(), cap - 1);
+        std::memcpy(buf, txt.data(), n);
+        buf[n] = 0;
+    }
+    return int64_t(txt.size());
+}
+
+CDBG_API uint64_t cdbg_gpu_device_bytes(void) { return g_device_bytes.load(); }
 
 CDBG_API int cdbg_gpu_job_bucket_offsets(cdbg_gpu_job* job, uint64_t* out) {
     CDBG_TRY({

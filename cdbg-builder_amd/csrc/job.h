@@ -39,6 +39,10 @@ struct Job {
     uint64_t n_supers = 0;
     std::vector<uint64_t> h_bucket_off;  // host copy (B+1)
     cdbg_gpu_stats stats{};
+    Profiler prof;  // per-kernel event timing, off unless enabled
+    uint64_t chunk_tuples = 0;  // phase-2 chunk budget in k-mer occurrences (0 = from free HBM)
+    uint64_t auto_chunk = 0;    // budget derived from free HBM on the first run, then kept
+    uint32_t n_chunks = 0;      // chunks used by the last phase-2 run
     ~Job();
 };
 

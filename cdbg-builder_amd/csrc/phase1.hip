@@ -301,7 +301,7 @@ void run_phase1(Job& J) {
     const uint64_t ngroups = J.n_pad / 32;
     uint32_t* packed = P.packed.ensure<uint32_t>(ngroups * 2);
     uint32_t* valid = P.valid.ensure<uint32_t>(ngroups);
-    k_pack2bit<<<blocks_for(ngroups, 256), 256, 0, st>>>(J.bytes.as<uint8_t>(), ngroups, packed, valid);
+    { KSCOPE("k_pack2bit", st); k_pack2bit<<<blocks_for(ngroups, 256), 256, 0, st>>>(J.bytes.as<uint8_t>(), ngroups, packed, valid); }
 
     SkArgs a{};
     a.packed = packed;
@@ -328,7 +328,7 @@ void run_phase1(Job& J) {
     uint64_t* tot = P.totals.ensure<uint64_t>(2);
     a.cnt_s = cnt;
     a.cnt_e = cnt + ntiles;
-    k_superkmers<false><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a);
+    { KSCOPE("k_superkmers", st); k_superkmers<false><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
     exclusive_scan<uint32_t, uint64_t>(cnt, off, ntiles, tot, J.rs.scan_tmp, st);
     exclusive_scan<uint32_t, uint64_t>(cnt + ntiles, off + ntiles, ntiles, tot + 1, J.rs.scan_tmp, st);
     uint64_t h_tot[2];
@@ -345,7 +345,7 @@ void run_phase1(Job& J) {
     a.s_flags = P.s_flags.ensure<uint8_t>(n);
     a.e_end = P.e_end.ensure<uint64_t>(n);
     a.e_flags = P.e_flags.ensure<uint8_t>(n);
-    k_superkmers<true><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a);
+    { KSCOPE("k_superkmers", st); k_superkmers<true><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
 
     uint64_t* sk_start = P.sk_start.ensure<uint64_t>(n);
     uint32_t* sk_len = P.sk_len.ensure<uint32_t>(n);
@@ -353,32 +353,32 @@ void run_phase1(Job& J) {
     uint32_t* sk_bucket = P.sk_bucket.ensure<uint32_t>(n);
     uint32_t* sk_color = P.sk_color.ensure<uint32_t>(n);
     if (n) {
-        k_zip_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, a.s_pos, a.s_bucket, a.s_flags, a.e_end, a.e_flags,
+        { KSCOPE("k_zip_supers", st); k_zip_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, a.s_pos, a.s_bucket, a.s_flags, a.e_end, a.e_flags,
                                                          J.color_off.as<uint64_t>(), J.num_colors, sk_start,
-                                                         sk_len, sk_flags, sk_bucket, sk_color);
+                                                         sk_len, sk_flags, sk_bucket, sk_color); }
     }
     // Bucket scatter: stable sort of super indices by bucket, then gather.
     uint32_t* key = P.key.ensure<uint32_t>(n);
     uint32_t* val = P.val.ensure<uint32_t>(n);
     uint32_t* key_t = P.key_t.ensure<uint32_t>(n);
     uint32_t* val_t = P.val_t.ensure<uint32_t>(n);
-    if (n) k_iota_copy<<<blocks_for(n, 256), 256, 0, st>>>(n, sk_bucket, key, val);
+    if (n) { KSCOPE("k_iota_copy", st); k_iota_copy<<<blocks_for(n, 256), 256, 0, st>>>(n, sk_bucket, key, val); }
     int bbits = 0;
     while ((uint64_t(1) << bbits) < J.B) ++bbits;
     SortPlan plan;
     plan.build({0, n});
     segmented_radix_sort<uint32_t, uint32_t>(key, val, key_t, val_t, plan, 0, bbits, bbits > 11 ? (bbits + 1) / 2 : bbits,
-                                             J.rs, st);
+                                             J.rs, st, "buckets");
     uint64_t* b_start = P.b_start.ensure<uint64_t>(n);
     uint32_t* b_len = P.b_len.ensure<uint32_t>(n);
     uint8_t* b_flags = P.b_flags.ensure<uint8_t>(n);
     uint32_t* b_color = P.b_color.ensure<uint32_t>(n);
     uint64_t* boff = P.bucket_off.ensure<uint64_t>(J.B + 1);
     if (n) {
-        k_gather_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, val, sk_start, sk_len, sk_flags, sk_color, b_start,
-                                                            b_len, b_flags, b_color);
+        { KSCOPE("k_gather_supers", st); k_gather_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, val, sk_start, sk_len, sk_flags, sk_color, b_start,
+                                                            b_len, b_flags, b_color); }
     }
-    k_bucket_offsets<<<blocks_for(std::max<uint64_t>(n, uint64_t(J.B) + 1), 256), 256, 0, st>>>(n, key, J.B, boff);
+    { KSCOPE("k_bucket_offsets", st); k_bucket_offsets<<<blocks_for(std::max<uint64_t>(n, uint64_t(J.B) + 1), 256), 256, 0, st>>>(n, key, J.B, boff); }
     CDBG_HIP(hipGetLastError());
     J.h_bucket_off.resize(J.B + 1);
     CDBG_HIP(hipMemcpyAsync(J.h_bucket_off.data(), boff, (J.B + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));

@@ -14,21 +14,31 @@
 //   * fragments are the walks of emit_from_seed with seeds taken in rank
 //     order (:431-566), emitted in (bucket, seed rank) order.
 //
-// Kernel pipeline (tuple = one k-mer occurrence):
-//   k_count_kmers / scan      -> tuple offsets of every record
-//   k_expand<K>               -> (canonical k-mer, rank|colour|owned|contrib)
-//   segmented radix sort      -> tuples sorted by k-mer inside each bucket
-//   k_heads / scan / k_node_* -> nodes: flags, primary, min rank, owned colours
-//   k_set_*                   -> colour-set interning (singleton table + hash
-//                                table with exact list verification), global
-//                                ids by sorting (bucket<<32|rank) keys, hybrid
-//                                encoding (hybrid_color_sets.hpp:41-96)
-//   k_table_insert<K>         -> per-bucket open-addressing k-mer table
-//   k_masks<K> / k_links      -> 8-neighbour ext masks (local_ext_mask
-//                                :260-271) and clean links (walk_step :317-336)
-//   k_walk_paths/cycles       -> one thread per component emulates the
-//                                rank-ordered seed loop
-//   fragment sort + k_frag_*  -> frag_unitigs.bin records (unitig_spill.hpp:491-522)
+// Buckets are processed in chunks of consecutive buckets whose k-mer
+// occurrences fit the chunk budget, so HBM use is bounded by the chunk, not
+// the job.  Because colour-set keys are (bucket << 32 | rank), every set first
+// seen in a chunk sorts after all sets of earlier chunks: its global id and
+// its place in the colour-set bit stream are final when the chunk ends, as in
+// the reference's streaming dictionary.  The singleton table, the multi-set
+// hash table with its colour-list store, the colour-set stream and the
+// fragment stream persist across chunks.
+//
+// Kernels per chunk (tuple = one k-mer occurrence):
+//   k_expand<K>               thread per tuple: canonical k-mer straight from
+//                             the 2-bit window, rank|colour|owned|contrib
+//   segmented radix sort      tuples by k-mer inside each bucket
+//   k_heads / scan / k_node_* nodes: flags, primary, min rank, owned colours
+//   k_set_*                   colour-set interning: singleton table keyed by
+//                             colour, content-hash table for larger sets with
+//                             an exact list compare against the stored list;
+//                             new sets sorted by key, ids assigned, hybrid
+//                             encoded (hybrid_color_sets.hpp:41-96)
+//   k_table_insert<K>         per-bucket open-addressing k-mer table
+//   k_masks<K> / k_links      8-neighbour ext masks (local_ext_mask :260-271)
+//                             and clean links (walk_step :317-336)
+//   k_walk_paths/cycles       one thread per component emulates the
+//                             rank-ordered seed loop
+//   fragment sort + k_frag_*  frag_unitigs.bin records (unitig_spill.hpp:491-522)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,8 +82,8 @@ struct alignas(16) Slot<Key128> {
     uint32_t pad[3];
 };
 
-__device__ __forceinline__ uint32_t upper_bucket(const uint64_t* off, uint32_t B, uint64_t i) {
-    // bucket b with off[b] <= i < off[b+1] (first such b for empty buckets skipped)
+// last b in [0, B) with off[b] <= i (off has B+1 entries, off[B] > i)
+__device__ __forceinline__ uint32_t bucket_of(const uint64_t* off, uint32_t B, uint64_t i) {
     uint32_t lo = 0, hi = B;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
@@ -120,43 +130,108 @@ __global__ void k_count_kmers(uint64_t n, const uint32_t* __restrict__ len, uint
 }
 
 __global__ void k_bucket_tuple_off(uint32_t B, const uint64_t* __restrict__ boff, const uint64_t* __restrict__ tup_off,
-                                   uint64_t n_supers, uint64_t n_tuples, uint64_t* __restrict__ btup) {
+                                   uint64_t* __restrict__ btup) {
     const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b > B) return;
-    const uint64_t s = boff[b];
-    btup[b] = s < n_supers ? tup_off[s] : n_tuples;
+    if (b <= B) btup[b] = tup_off[boff[b]];  // tup_off has n_supers+1 entries
 }
 
+// bases p..p+31 of the 2-bit array, base p in bits 0-1
+__device__ __forceinline__ uint64_t window64(const uint32_t* __restrict__ packed, uint64_t p) {
+    const uint64_t w = p >> 4;
+    const uint32_t sh = 2 * uint32_t(p & 15);
+    const uint64_t lo = uint64_t(packed[w]) | (uint64_t(packed[w + 1]) << 32);
+    uint64_t v = lo >> sh;
+    if (sh) v |= uint64_t(packed[w + 2]) << (64 - sh);
+    return v;
+}
+
+// forward and reverse-complement words of the k-mer starting at base p
+__device__ __forceinline__ void kmer_at(const uint32_t* packed, uint64_t p, uint32_t k, uint64_t& f, uint64_t& r) {
+    const uint64_t w = window64(packed, p);
+    f = rc_word64(~w) >> (64 - 2 * k);        // bases in reading order
+    r = ~w & ((uint64_t(1) << (2 * k)) - 1);  // complemented, reversed order
+}
+__device__ __forceinline__ void kmer_at(const uint32_t* packed, uint64_t p, uint32_t k, Key128& f, Key128& r) {
+    const uint64_t w0 = window64(packed, p), w1 = window64(packed, p + 32);
+    f = KOps<Key128>::shr(Key128{rc_word64(~w1), rc_word64(~w0)}, 128 - 2 * k);
+    const Key128 m = KOps<Key128>::mask(k);
+    r = Key128{~w0 & m.lo, ~w1 & m.hi};
+}
+
+struct ExpArgs {
+    uint64_t n;       // tuples in the chunk
+    uint64_t t0;      // global index of the chunk's first tuple
+    uint64_t s0, s1;  // super range of the chunk
+    const uint64_t* tup_off;
+    const uint64_t* btup;
+    const uint32_t* bucket;
+    const uint64_t* start;
+    const uint32_t* len;
+    const uint8_t* flags;
+    const uint32_t* color;
+    const uint32_t* packed;
+    uint32_t k;
+};
+
 template <class K>
-__global__ void k_expand(uint64_t n, const uint64_t* __restrict__ start, const uint32_t* __restrict__ len,
-                         const uint8_t* __restrict__ flags, const uint32_t* __restrict__ color,
-                         const uint32_t* __restrict__ bucket, const uint64_t* __restrict__ tup_off,
-                         const uint64_t* __restrict__ btup, const uint32_t* __restrict__ packed, uint32_t k,
-                         K* __restrict__ keys, uint64_t* __restrict__ vals) {
+__global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, K* __restrict__ keys, uint64_t* __restrict__ vals) {
     using O = KOps<K>;
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t s = start[i];
-    const uint32_t L = len[i];
-    const uint32_t fl = flags[i];
-    const uint64_t t0 = tup_off[i];
-    const uint32_t rank0 = uint32_t(t0 - btup[bucket[i]]);
-    const uint64_t cbits = uint64_t(color[i]) << 3;
-    const uint32_t nk = L - k + 1;
-    const bool begin_incl = fl & 1u, end_incl = fl & 2u, own_first = fl & 4u, own_last = fl & 8u;
-    K f = O::zero(), r = O::zero();
-    auto base = [&](uint64_t p) -> uint32_t { return (packed[p >> 4] >> (2 * (p & 15))) & 3u; };
-    for (uint32_t j = 0; j + 1 < k; ++j) O::roll(f, r, base(s + j), k);
-    for (uint32_t ix = 0; ix < nk; ++ix) {
-        O::roll(f, r, base(s + k - 1 + ix), k);
-        const bool fwd = O::le(f, r);
-        uint32_t contrib = 0;
-        if (!begin_incl && ix == 0) contrib |= fwd ? BOUND_LEFT : BOUND_RIGHT;
-        if (!end_incl && ix == nk - 1) contrib |= fwd ? BOUND_RIGHT : BOUND_LEFT;
-        const bool owned = !((ix == 0 && !own_first) || (ix == nk - 1 && !own_last));
-        keys[t0 + ix] = fwd ? f : r;
-        vals[t0 + ix] = (uint64_t(rank0 + ix) << 32) | cbits | (uint64_t(owned) << 2) | contrib;
+    __shared__ uint64_t s_off[TPB + 1];
+    __shared__ uint64_t s_first;
+    __shared__ uint32_t s_cnt;
+    const uint64_t T0 = a.t0 + uint64_t(blockIdx.x) * TPB;
+    const uint64_t TL = min(T0 + TPB, a.t0 + a.n) - 1;  // last tuple of the block
+    if (threadIdx.x == 0) {
+        // supers holding T0 and TL: last s with tup_off[s] <= T
+        uint64_t lo = a.s0, hi = a.s1;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (a.tup_off[mid] <= T0)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint64_t sa = lo;
+        hi = a.s1;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (a.tup_off[mid] <= TL)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        s_first = sa;
+        s_cnt = uint32_t(lo - sa + 1);
     }
+    __syncthreads();
+    const uint64_t sa = s_first;
+    const uint32_t ns = s_cnt;  // <= TPB: every super has >= 1 k-mer
+    for (uint32_t i = threadIdx.x; i <= ns; i += TPB) s_off[i] = a.tup_off[sa + i];
+    __syncthreads();
+    const uint64_t t = T0 + threadIdx.x;
+    if (t > TL) return;
+    uint32_t lo = 0, hi = ns;  // s_off[lo] <= t < s_off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint64_t s = sa + lo;
+    const uint32_t ix = uint32_t(t - s_off[lo]);
+    const uint32_t nk = a.len[s] - a.k + 1;
+    const uint32_t fl = a.flags[s];
+    K f, r;
+    kmer_at(a.packed, a.start[s] + ix, a.k, f, r);
+    const bool fwd = O::le(f, r);
+    uint32_t contrib = 0;
+    if (!(fl & 1u) && ix == 0) contrib |= fwd ? BOUND_LEFT : BOUND_RIGHT;
+    if (!(fl & 2u) && ix == nk - 1) contrib |= fwd ? BOUND_RIGHT : BOUND_LEFT;
+    const bool owned = !((ix == 0 && !(fl & 4u)) || (ix == nk - 1 && !(fl & 8u)));
+    const uint32_t rank = uint32_t(t - a.btup[a.bucket[s]]);
+    keys[t - a.t0] = fwd ? f : r;
+    vals[t - a.t0] = (uint64_t(rank) << 32) | (uint64_t(a.color[s]) << 3) | (uint64_t(owned) << 2) | contrib;
 }
 
 // ---------------------------------------------------------------------------
@@ -169,9 +244,9 @@ __global__ void k_heads(uint64_t n, const K* __restrict__ keys, uint32_t* __rest
     heads[i] = (i == 0 || !KOps<K>::eq(keys[i], keys[i - 1])) ? 1u : 0u;
 }
 
-__global__ void k_mark_bucket_starts(uint32_t B, const uint64_t* __restrict__ btup, uint32_t* __restrict__ heads) {
+__global__ void k_mark_bucket_starts(uint32_t nb, const uint64_t* __restrict__ lbtup, uint32_t* __restrict__ heads) {
     const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b < B && btup[b] < btup[b + 1]) heads[btup[b]] = 1u;
+    if (b < nb && lbtup[b] < lbtup[b + 1]) heads[lbtup[b]] = 1u;
 }
 
 __global__ void k_node_start(uint64_t n, const uint32_t* __restrict__ heads, const uint32_t* __restrict__ hscan,
@@ -181,11 +256,11 @@ __global__ void k_node_start(uint64_t n, const uint32_t* __restrict__ heads, con
     if (i == 0) node_start[n_nodes] = uint32_t(n);
 }
 
-__global__ void k_node_off(uint32_t B, const uint64_t* __restrict__ btup, uint64_t n_tuples,
+__global__ void k_node_off(uint32_t nb, const uint64_t* __restrict__ lbtup, uint64_t n_tuples,
                            const uint32_t* __restrict__ hscan, uint32_t n_nodes, uint64_t* __restrict__ node_off) {
     const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b > B) return;
-    const uint64_t t = btup[b];
+    if (b > nb) return;
+    const uint64_t t = lbtup[b];
     node_off[b] = t < n_tuples ? hscan[t] : n_nodes;
 }
 
@@ -195,7 +270,7 @@ __global__ void k_node_reduce(uint32_t n_nodes, const uint32_t* __restrict__ nod
                               uint32_t* __restrict__ node_rank, uint32_t* __restrict__ node_ncol,
                               unsigned long long* __restrict__ counters) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    uint64_t prim = 0, multi = 0;
+    uint64_t prim = 0, multi = 0, multi_cols = 0;
     if (n < n_nodes) {
         const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
         uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
@@ -220,9 +295,11 @@ __global__ void k_node_reduce(uint32_t n_nodes, const uint32_t* __restrict__ nod
         node_ncol[n] = ncol;
         prim = primary;
         multi = ncol >= 2;
+        multi_cols = ncol >= 2 ? ncol : 0;
     }
     block_add(&counters[0], prim);
     block_add(&counters[1], multi);
+    block_add(&counters[7], multi_cols);
 }
 
 __global__ void k_node_colors(uint32_t n_nodes, const uint32_t* __restrict__ node_start, const uint64_t* __restrict__ vals,
@@ -252,79 +329,107 @@ __device__ __forceinline__ uint64_t list_hash(const uint32_t* c, uint32_t n) {
     return h | 1ull;  // 0 marks an empty slot
 }
 
+struct SetTables {
+    unsigned long long* single_min;  // [num_colors] min key of the set {c}
+    uint32_t* single_cid;            // [num_colors]
+    unsigned long long* mt_hash;     // [cap] 0 = empty
+    unsigned long long* mt_key;      // [cap] min key
+    uint32_t* mt_cid;                // [cap]
+    uint64_t* mt_loff;               // [cap] colour list in `store`
+    uint32_t* mt_llen;               // [cap]
+    uint64_t cap;
+    uint32_t* store;
+    unsigned long long* store_used;
+};
+
 __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
                              const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
-                             const uint32_t* __restrict__ rank, const uint64_t* __restrict__ node_off, uint32_t B,
-                             unsigned long long* __restrict__ single_min, unsigned long long* __restrict__ mt_hash,
-                             unsigned long long* __restrict__ mt_key, uint64_t mt_cap, uint32_t* __restrict__ node_slot) {
+                             const uint32_t* __restrict__ rank, const uint64_t* __restrict__ node_off, uint32_t nb,
+                             uint32_t b0, SetTables T, uint32_t* __restrict__ node_slot) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes || !(meta[n] & 4u)) return;
-    const uint32_t b = upper_bucket(node_off, B, n);
+    const uint32_t b = b0 + bucket_of(node_off, nb, n);
     const unsigned long long key = (uint64_t(b) << 32) | rank[n];
     const uint32_t nc = ncol[n];
     const uint32_t* c = colors + coff[n];
     if (nc == 1) {
-        atomicMin(&single_min[c[0]], key);
+        // keys only decrease: a plain read that is already <= key skips the atomic
+        if (T.single_min[c[0]] > key) atomicMin(&T.single_min[c[0]], key);
         return;
     }
     const unsigned long long h = list_hash(c, nc);
-    uint64_t s = h & (mt_cap - 1);
-    for (uint64_t probe = 0; probe < mt_cap; ++probe) {
-        const unsigned long long prev = atomicCAS(&mt_hash[s], 0ull, h);
-        if (prev == 0ull || prev == h) {
-            atomicMin(&mt_key[s], key);
+    uint64_t s = h & (T.cap - 1);
+    for (uint64_t probe = 0; probe < T.cap; ++probe) {
+        unsigned long long cur = T.mt_hash[s];
+        if (cur == 0ull) {
+            cur = atomicCAS(&T.mt_hash[s], 0ull, h);
+            if (cur == 0ull) {  // this thread created the slot: store its colour list
+                const uint64_t lo = atomicAdd(T.store_used, (unsigned long long)nc);
+                for (uint32_t i = 0; i < nc; ++i) T.store[lo + i] = c[i];
+                T.mt_loff[s] = lo;
+                T.mt_llen[s] = nc;
+                cur = h;
+            }
+        }
+        if (cur == h) {
+            if (T.mt_key[s] > key) atomicMin(&T.mt_key[s], key);
             node_slot[n] = uint32_t(s);
             return;
         }
-        s = (s + 1) & (mt_cap - 1);
+        s = (s + 1) & (T.cap - 1);
     }
-}
-
-__global__ void k_set_rep(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
-                          const uint32_t* __restrict__ rank, const uint64_t* __restrict__ node_off, uint32_t B,
-                          const uint32_t* __restrict__ node_slot, const unsigned long long* __restrict__ mt_key,
-                          uint32_t* __restrict__ mt_rep) {
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2) return;
-    const uint32_t b = upper_bucket(node_off, B, n);
-    const uint64_t key = (uint64_t(b) << 32) | rank[n];
-    const uint32_t s = node_slot[n];
-    if (mt_key[s] == key) mt_rep[s] = uint32_t(n);
 }
 
 __global__ void k_set_verify(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
                              const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
-                             const uint32_t* __restrict__ node_slot, const uint32_t* __restrict__ mt_rep,
-                             unsigned long long* __restrict__ err) {
+                             const uint32_t* __restrict__ node_slot, SetTables T, unsigned long long* __restrict__ err) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2) return;
-    const uint32_t r = mt_rep[node_slot[n]];
-    bool same = ncol[r] == ncol[n];
-    for (uint32_t i = 0; same && i < ncol[n]; ++i) same = colors[coff[n] + i] == colors[coff[r] + i];
+    const uint32_t s = node_slot[n];
+    const uint32_t* ref = T.store + T.mt_loff[s];
+    bool same = T.mt_llen[s] == ncol[n];
+    for (uint32_t i = 0; same && i < ncol[n]; ++i) same = colors[coff[n] + i] == ref[i];
     if (!same) atomicAdd(err, 1ull);
 }
 
-// flags over [0, num_colors) singletons then [0, cap) hash slots
-__global__ void k_set_flags(uint32_t nc, uint64_t cap, const unsigned long long* __restrict__ single_min,
-                            const unsigned long long* __restrict__ mt_hash, uint32_t* __restrict__ fl) {
+__global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restrict__ oh,
+                            const unsigned long long* __restrict__ ok, const uint32_t* __restrict__ oc,
+                            const uint64_t* __restrict__ oo, const uint32_t* __restrict__ ol, SetTables T) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < nc)
-        fl[i] = single_min[i] != ~0ull;
-    else if (i < nc + cap)
-        fl[i] = mt_hash[i - nc] != 0ull;
+    if (i >= old_cap || oh[i] == 0ull) return;
+    const unsigned long long h = oh[i];
+    uint64_t s = h & (T.cap - 1);
+    for (uint64_t probe = 0; probe < T.cap; ++probe) {
+        if (atomicCAS(&T.mt_hash[s], 0ull, h) == 0ull) {
+            T.mt_key[s] = ok[i];
+            T.mt_cid[s] = oc[i];
+            T.mt_loff[s] = oo[i];
+            T.mt_llen[s] = ol[i];
+            return;
+        }
+        s = (s + 1) & (T.cap - 1);
+    }
 }
 
-__global__ void k_set_emit(uint32_t nc, uint64_t cap, const uint32_t* __restrict__ fl, const uint64_t* __restrict__ pos,
-                           const unsigned long long* __restrict__ single_min, const unsigned long long* __restrict__ mt_key,
-                           uint64_t* __restrict__ set_key, uint64_t* __restrict__ set_desc) {
+// sets with a key but no id yet: singletons [0, nc) then hash slots [0, cap)
+__global__ void k_newset_flags(uint32_t nc, SetTables T, uint32_t* __restrict__ fl) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nc + cap || !fl[i]) return;
+    if (i < nc)
+        fl[i] = T.single_min[i] != ~0ull && T.single_cid[i] == NONE32;
+    else if (i < nc + T.cap)
+        fl[i] = T.mt_hash[i - nc] != 0ull && T.mt_cid[i - nc] == NONE32;
+}
+
+__global__ void k_newset_emit(uint32_t nc, SetTables T, const uint32_t* __restrict__ fl, const uint64_t* __restrict__ pos,
+                              uint64_t* __restrict__ set_key, uint64_t* __restrict__ set_desc) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nc + T.cap || !fl[i]) return;
     const uint64_t p = pos[i];
     if (i < nc) {
-        set_key[p] = single_min[i];
+        set_key[p] = T.single_min[i];
         set_desc[p] = i;
     } else {
-        set_key[p] = mt_key[i - nc];
+        set_key[p] = T.mt_key[i - nc];
         set_desc[p] = (uint64_t(1) << 63) | (i - nc);
     }
 }
@@ -335,15 +440,12 @@ __device__ __forceinline__ uint64_t delta_len(uint64_t x) {
     return 2ull * msb64d(b + 1) + 1 + b;
 }
 
-// Resolves a set's colour list: singletons point into `single_val`.
-__device__ __forceinline__ void set_list(uint64_t desc, const uint32_t* __restrict__ mt_rep,
-                                         const uint64_t* __restrict__ coff, const uint32_t* __restrict__ ncol,
-                                         const uint32_t* __restrict__ colors, const uint32_t*& c, uint32_t& n,
+__device__ __forceinline__ void set_list(uint64_t desc, const SetTables& T, const uint32_t*& c, uint32_t& n,
                                          uint32_t& single) {
     if (desc >> 63) {
-        const uint32_t r = mt_rep[desc & 0xFFFFFFFFull];
-        c = colors + coff[r];
-        n = ncol[r];
+        const uint64_t s = desc & 0xFFFFFFFFFFFFull;
+        c = T.store + T.mt_loff[s];
+        n = T.mt_llen[s];
     } else {
         single = uint32_t(desc);
         c = &single;
@@ -351,22 +453,19 @@ __device__ __forceinline__ void set_list(uint64_t desc, const uint32_t* __restri
     }
 }
 
-__global__ void k_set_cid_len(uint64_t n_sets, const uint64_t* __restrict__ set_desc, const uint32_t* __restrict__ mt_rep,
-                              const uint64_t* __restrict__ coff, const uint32_t* __restrict__ ncol,
-                              const uint32_t* __restrict__ colors, uint32_t num_colors, uint32_t* __restrict__ single_cid,
-                              uint32_t* __restrict__ mt_cid, uint64_t* __restrict__ bits,
-                              unsigned long long* __restrict__ counters) {
+__global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t* __restrict__ set_desc, SetTables T,
+                              uint32_t num_colors, uint64_t* __restrict__ bits, unsigned long long* __restrict__ counters) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     uint64_t ints = 0;
-    if (i < n_sets) {
+    if (i < n_new) {
         const uint64_t d = set_desc[i];
         if (d >> 63)
-            mt_cid[d & 0xFFFFFFFFull] = uint32_t(i);
+            T.mt_cid[d & 0xFFFFFFFFFFFFull] = uint32_t(cid_base + i);
         else
-            single_cid[uint32_t(d)] = uint32_t(i);
+            T.single_cid[uint32_t(d)] = uint32_t(cid_base + i);
         const uint32_t* c;
         uint32_t n, single;
-        set_list(d, mt_rep, coff, ncol, colors, c, n, single);
+        set_list(d, T, c, n, single);
         const uint32_t sparse = num_colors / 4, dense = uint32_t(uint64_t(num_colors) * 3 / 4);
         uint64_t L = delta_len(n);
         if (n < sparse) {
@@ -418,17 +517,18 @@ struct BitWriter {
     }
 };
 
-__global__ void k_set_encode(uint64_t n_sets, const uint64_t* __restrict__ set_desc, const uint64_t* __restrict__ boff,
-                             const uint32_t* __restrict__ mt_rep, const uint64_t* __restrict__ coff,
-                             const uint32_t* __restrict__ ncol, const uint32_t* __restrict__ colors, uint32_t num_colors,
-                             unsigned long long* __restrict__ words) {
+__global__ void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_base, const uint64_t* __restrict__ set_desc,
+                             const uint64_t* __restrict__ boff, SetTables T, uint32_t num_colors,
+                             unsigned long long* __restrict__ words, uint64_t* __restrict__ set_off) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_sets) return;
+    if (i >= n_new) return;
     const uint32_t* c;
     uint32_t n, single;
-    set_list(set_desc[i], mt_rep, coff, ncol, colors, c, n, single);
+    set_list(set_desc[i], T, c, n, single);
     const uint32_t sparse = num_colors / 4, dense = uint32_t(uint64_t(num_colors) * 3 / 4);
-    BitWriter w{words, boff[i]};
+    const uint64_t at = bit_base + boff[i];
+    set_off[cid_base + i] = at;
+    BitWriter w{words, at};
     w.delta(n);
     if (n < sparse) {
         w.delta(c[0]);
@@ -462,12 +562,11 @@ __global__ void k_set_encode(uint64_t n_sets, const uint64_t* __restrict__ set_d
 
 __global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
                            const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
-                           const uint32_t* __restrict__ node_slot, const uint32_t* __restrict__ single_cid,
-                           const uint32_t* __restrict__ mt_cid, uint32_t* __restrict__ node_cid) {
+                           const uint32_t* __restrict__ node_slot, SetTables T, uint32_t* __restrict__ node_cid) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     uint32_t cid = NONE32;
-    if (meta[n] & 4u) cid = ncol[n] == 1 ? single_cid[colors[coff[n]]] : mt_cid[node_slot[n]];
+    if (meta[n] & 4u) cid = ncol[n] == 1 ? T.single_cid[colors[coff[n]]] : T.mt_cid[node_slot[n]];
     node_cid[n] = cid;
 }
 
@@ -476,10 +575,10 @@ __global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, c
 // ---------------------------------------------------------------------------
 template <class K>
 __global__ void k_table_insert(uint32_t n_nodes, const K* __restrict__ node_key, const uint64_t* __restrict__ node_off,
-                               uint32_t B, const uint64_t* __restrict__ tab_off, Slot<K>* __restrict__ tab) {
+                               uint32_t nb, const uint64_t* __restrict__ tab_off, Slot<K>* __restrict__ tab) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
-    const uint32_t b = upper_bucket(node_off, B, n);
+    const uint32_t b = bucket_of(node_off, nb, n);
     const uint64_t base = tab_off[b], cap = tab_off[b + 1] - base;
     const K key = node_key[n];
     uint64_t s = KOps<K>::hash(key) & (cap - 1);
@@ -510,12 +609,12 @@ __device__ __forceinline__ uint32_t tab_find(const Slot<K>* __restrict__ tab, ui
 // port: the neighbour is entered at its L side iff the extension is canonical.
 template <class K>
 __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint64_t* __restrict__ node_off,
-                        uint32_t B, const uint64_t* __restrict__ tab_off, const Slot<K>* __restrict__ tab, uint32_t k,
+                        uint32_t nbk, const uint64_t* __restrict__ tab_off, const Slot<K>* __restrict__ tab, uint32_t k,
                         uint8_t* __restrict__ mask_out, uint32_t* __restrict__ nb) {
     using O = KOps<K>;
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
-    const uint32_t b = upper_bucket(node_off, B, n);
+    const uint32_t b = bucket_of(node_off, nbk, n);
     const uint64_t base = tab_off[b], cap = tab_off[b + 1] - base;
     const K can = node_key[n];
     const K rv = O::rc(can, k);
@@ -568,7 +667,8 @@ struct WalkArgs {
     const uint8_t* mask;
     const uint32_t* rank;
     const uint64_t* node_off;
-    uint32_t B;
+    uint32_t nb;
+    uint32_t b0;
     uint8_t* visited;
     uint32_t* frag_nodes;
     uint8_t* frag_rc;
@@ -584,12 +684,13 @@ __device__ __forceinline__ bool is_break(uint8_t meta) {
     return f == BOUND_LEFT || f == BOUND_RIGHT;
 }
 
-// emit_from_seed (bucket_walker.hpp:451-542) over the link graph.
-__device__ void emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
+// emit_from_seed (bucket_walker.hpp:451-542) over the link graph; returns the
+// number of nodes placed in the fragment.
+__device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
     a.visited[s] = 1;
     const uint32_t sflags = a.meta[s] & 3u;
     const uint32_t ms = a.mask[s];
-    const uint32_t limit = a.n_nodes + 1;
+    const uint32_t limit = 2 * a.n_nodes + 2;
     uint32_t nbk = 0, nfw = 0;
     bool open_l = false, open_r = false;
     if (sflags == BOUND_LEFT) {
@@ -635,7 +736,7 @@ __device__ void emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
     const uint64_t fi = atomicAdd(&a.counters[4], 1ull);
     if (off + total > a.n_nodes) {
         atomicAdd(&a.counters[5], 1ull);
-        return;
+        return total;
     }
     uint32_t cur = s, ex = PORT_L;
     for (uint32_t i = 0; i < nbk; ++i) {  // walking backward: entered at L <=> reverse orientation
@@ -662,34 +763,41 @@ __device__ void emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
     a.frag_n[fi] = total;
     a.frag_open[fi] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
     a.frag_key[fi] = (uint64_t(b) << 32) | a.rank[s];
+    return total;
 }
 
-// Runs the rank-ordered seed loop over the component that contains `x`
-// (a path end, or the smallest node of a cycle).
-__device__ void simulate_component(const WalkArgs& a, uint32_t x, bool cycle) {
-    const uint32_t b = upper_bucket(a.node_off, a.B, x);
-    const uint32_t limit = a.n_nodes + 1;
-    uint32_t ex0 = PORT_R;
-    if (!cycle) ex0 = a.link[2 * x + PORT_R] == NONE32 ? PORT_L : PORT_R;
+// Seed loop of one component (path from end x, or cycle through x): seeds are
+// the unvisited nodes in rank order.  `len` is the component size when the
+// first traversal saw every node once (0 when it bounced off a node linked to
+// itself, where nodes are passed twice), `seed` the first seed.
+//
+// A cycle is traversed from x leaving through R until it re-enters x through
+// L; a re-entry through R is a bounce (a hairpin node sends the walk back) and
+// the traversal continues through x's other side.
+__device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, bool cycle, uint32_t len,
+                                   uint32_t seed) {
+    const uint32_t b = a.b0 + bucket_of(a.node_off, a.nb, x);
+    const uint32_t limit = 2 * a.n_nodes + 2;
+    uint32_t placed = 0;
     for (uint32_t guard = 0; guard < limit; ++guard) {
+        placed += emit_fragment(a, seed, b);
+        if (len && placed >= len) return;
         uint32_t best = NONE32, best_rank = 0xFFFFFFFFu;
         uint32_t cur = x, ex = ex0;
         for (uint32_t h = 0; h < limit; ++h) {
-            if (!a.visited[cur] && a.rank[cur] <= best_rank) {
-                if (a.rank[cur] < best_rank || best == NONE32) {
-                    best = cur;
-                    best_rank = a.rank[cur];
-                }
+            if (!a.visited[cur] && a.rank[cur] < best_rank) {
+                best = cur;
+                best_rank = a.rank[cur];
             }
             const uint32_t l = a.link[2 * cur + ex];
             if (l == NONE32) break;
-            const uint32_t y = l >> 1;
-            if (cycle && y == x) break;
+            const uint32_t y = l >> 1, q = l & 1u;
+            if (cycle && y == x && q == PORT_L) break;
             cur = y;
-            ex = 1u - (l & 1u);
+            ex = 1u - q;
         }
-        if (best == NONE32) return;
-        emit_fragment(a, best, b);
+        if (best == NONE32) return;  // every node of the component is placed
+        seed = best;
     }
     atomicAdd(&a.counters[5], 1ull);
 }
@@ -699,36 +807,56 @@ __global__ void k_walk_paths(WalkArgs a) {
     if (x >= a.n_nodes) return;
     const uint32_t lL = a.link[2 * x + PORT_L], lR = a.link[2 * x + PORT_R];
     if (lL != NONE32 && lR != NONE32) return;  // not a path end
-    // walk to the other end; the end with the smaller index owns the path
-    uint32_t cur = uint32_t(x), ex = lR == NONE32 ? PORT_L : PORT_R;
-    const uint32_t limit = a.n_nodes + 1;
+    // walk to the other end (the end with the smaller index owns the path),
+    // counting nodes and tracking the first seed
+    const uint32_t ex0 = lR == NONE32 ? PORT_L : PORT_R;
+    uint32_t cur = uint32_t(x), ex = ex0, len = 1, seed = uint32_t(x), seed_rank = a.rank[x];
+    bool simple = true;
+    const uint32_t limit = 2 * a.n_nodes + 2;
     for (uint32_t h = 0; h < limit; ++h) {
         const uint32_t l = a.link[2 * cur + ex];
         if (l == NONE32) break;
-        cur = l >> 1;
+        const uint32_t y = l >> 1;
+        if (y == cur) simple = false;  // hairpin: the walk comes back
+        cur = y;
         ex = 1u - (l & 1u);
+        ++len;
+        const uint32_t r = a.rank[cur];
+        if (r < seed_rank) {
+            seed = cur;
+            seed_rank = r;
+        }
     }
     if (cur < x) return;
-    simulate_component(a, uint32_t(x), false);
+    simulate_component(a, uint32_t(x), ex0, false, simple ? len : 0, seed);
 }
 
 __global__ void k_walk_cycles(WalkArgs a) {
     const uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (x >= a.n_nodes || a.visited[x]) return;
-    uint32_t cur = uint32_t(x), ex = PORT_R;
-    const uint32_t limit = a.n_nodes + 1;
+    uint32_t cur = uint32_t(x), ex = PORT_R, len = 1, seed = uint32_t(x), seed_rank = a.rank[x];
+    bool simple = true;
+    const uint32_t limit = 2 * a.n_nodes + 2;
     for (uint32_t h = 0; h < limit; ++h) {
         const uint32_t l = a.link[2 * cur + ex];
         if (l == NONE32) {  // not a cycle: should have been a path
             atomicAdd(&a.counters[5], 1ull);
             return;
         }
-        cur = l >> 1;
-        ex = 1u - (l & 1u);
-        if (cur == x) break;
+        const uint32_t y = l >> 1, q = l & 1u;
+        if (y == x && q == PORT_L) break;
+        if (y == cur || y == x) simple = false;  // hairpin or bounce through x
+        cur = y;
+        ex = 1u - q;
         if (cur < x) return;
+        ++len;
+        const uint32_t r = a.rank[cur];
+        if (r < seed_rank) {
+            seed = cur;
+            seed_rank = r;
+        }
     }
-    simulate_component(a, uint32_t(x), true);
+    simulate_component(a, uint32_t(x), PORT_R, true, simple ? len : 0, seed);
 }
 
 // ---------------------------------------------------------------------------
@@ -810,19 +938,23 @@ __global__ void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ orde
 // host orchestration
 // ---------------------------------------------------------------------------
 struct Phase2Bufs {
-    DevBuf kcnt, tup_off, btup, total;
+    // job-wide
+    DevBuf kcnt, tup_off, btup, total, lbtup;
+    DevBuf single_min, single_cid, mt_hash, mt_key, mt_cid, mt_loff, mt_llen, store, store_used;
+    DevBuf set_off, cs_words, frag_bytes;
+    uint64_t mt_cap = 0, mt_count = 0;
+    // per chunk (reused)
     DevBuf keys, vals, keys_t, vals_t;
     DevBuf heads, hscan, node_start, node_off;
-    DevBuf node_key, node_meta, node_rank, node_ncol, node_coff, colors;
-    DevBuf counters;
-    DevBuf single_min, single_cid, mt_hash, mt_key, mt_rep, mt_cid, node_slot;
-    DevBuf set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
-    DevBuf cs_words;
+    DevBuf node_key, node_meta, node_rank, node_ncol, node_coff, colors, counters;
+    DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
     DevBuf node_cid, tab_off, tab, node_mask, nb, link, visited;
     DevBuf frag_nodes, frag_rc, frag_off, frag_n, frag_open, frag_key;
-    DevBuf fkey_t, forder, forder_t, fsize, fboff, frag_bytes;
+    DevBuf fkey_t, forder, forder_t, fsize, fboff;
     RadixScratch rs2;
+    // results
     uint64_t n_tuples = 0, n_nodes = 0, n_sets = 0, n_frags = 0, frag_total = 0, cs_bits = 0;
+    uint64_t n_primary = 0, total_integers = 0, words_zeroed = 0;
 };
 
 template <class T>
@@ -837,6 +969,244 @@ static int bits_for(uint64_t n) {
     int b = 0;
     while ((uint64_t(1) << b) < n) ++b;
     return b;
+}
+
+static SetTables set_tables(Phase2Bufs& P) {
+    SetTables T;
+    T.single_min = P.single_min.as<unsigned long long>();
+    T.single_cid = P.single_cid.as<uint32_t>();
+    T.mt_hash = P.mt_hash.as<unsigned long long>();
+    T.mt_key = P.mt_key.as<unsigned long long>();
+    T.mt_cid = P.mt_cid.as<uint32_t>();
+    T.mt_loff = P.mt_loff.as<uint64_t>();
+    T.mt_llen = P.mt_llen.as<uint32_t>();
+    T.cap = P.mt_cap;
+    T.store = P.store.as<uint32_t>();
+    T.store_used = P.store_used.as<unsigned long long>();
+    return T;
+}
+
+// (Re)allocates the colour-set hash table with `cap` slots, keeping entries.
+static void mt_resize(Phase2Bufs& P, uint64_t cap, hipStream_t st) {
+    DevBuf oh, ok, oc, oo, ol;
+    const uint64_t old_cap = P.mt_cap;
+    oh.swap(P.mt_hash);
+    ok.swap(P.mt_key);
+    oc.swap(P.mt_cid);
+    oo.swap(P.mt_loff);
+    ol.swap(P.mt_llen);
+    P.mt_cap = cap;
+    P.mt_hash.ensure<unsigned long long>(cap);
+    P.mt_key.ensure<unsigned long long>(cap);
+    P.mt_cid.ensure<uint32_t>(cap);
+    P.mt_loff.ensure<uint64_t>(cap);
+    P.mt_llen.ensure<uint32_t>(cap);
+    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(cap), TPB, 0, st>>>(P.mt_hash.as<unsigned long long>(), cap, 0ull); }
+    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(cap), TPB, 0, st>>>(P.mt_key.as<unsigned long long>(), cap, ~0ull); }
+    { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(cap), TPB, 0, st>>>(P.mt_cid.as<uint32_t>(), cap, NONE32); }
+    if (old_cap) {
+        KSCOPE("k_mt_rehash", st);
+        k_mt_rehash<<<blocks_for(old_cap), TPB, 0, st>>>(old_cap, oh.as<unsigned long long>(), ok.as<unsigned long long>(),
+                                                          oc.as<uint32_t>(), oo.as<uint64_t>(), ol.as<uint32_t>(),
+                                                          set_tables(P));
+    }
+    CDBG_HIP(hipStreamSynchronize(st));
+}
+
+template <class K>
+static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std::vector<uint64_t>& h_btup) {
+    hipStream_t st = J.st;
+    Phase1Bufs& Q = J.p1;
+    const uint32_t k = J.k, nb = b1 - b0;
+    const uint64_t t0 = h_btup[b0], nt = h_btup[b1] - t0;
+    const uint64_t s0 = J.h_bucket_off[b0], s1 = J.h_bucket_off[b1];
+    if (nt == 0) return;
+    if (nt >= (1ull << 32)) throw Error("phase 2: a chunk holds more than 2^32 k-mer occurrences");
+    const int bbits = bits_for(J.B);
+
+    // local bucket tuple offsets
+    std::vector<uint64_t> lb(nb + 1);
+    for (uint32_t b = 0; b <= nb; ++b) lb[b] = h_btup[b0 + b] - t0;
+    uint64_t* lbtup = P.lbtup.ensure<uint64_t>(nb + 1);
+    CDBG_HIP(hipMemcpyAsync(lbtup, lb.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+
+    // --- expand
+    K* keys = P.keys.ensure<K>(nt);
+    uint64_t* vals = P.vals.ensure<uint64_t>(nt);
+    K* keys_t = P.keys_t.ensure<K>(nt);
+    uint64_t* vals_t = P.vals_t.ensure<uint64_t>(nt);
+    ExpArgs ea{nt, t0, s0, s1, P.tup_off.as<uint64_t>(), P.btup.as<uint64_t>(), Q.key.as<uint32_t>(),
+               Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(),
+               Q.packed.as<uint32_t>(), k};
+    { KSCOPE("k_expand", st); k_expand<K><<<blocks_for(nt), TPB, 0, st>>>(ea, keys, vals); }
+
+    // --- sort tuples by k-mer inside every bucket
+    SortPlan plan;
+    plan.build(lb);
+    segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, 0, 2 * int(k), 8, P.rs2, st, "tuples");
+
+    // --- nodes
+    uint32_t* heads = P.heads.ensure<uint32_t>(nt);
+    uint32_t* hscan = P.hscan.ensure<uint32_t>(nt + 1);
+    uint64_t* dtot = P.total.ensure<uint64_t>(8);
+    { KSCOPE("k_heads", st); k_heads<K><<<blocks_for(nt), TPB, 0, st>>>(nt, keys, heads); }
+    { KSCOPE("k_mark_bucket_starts", st); k_mark_bucket_starts<<<blocks_for(nb), TPB, 0, st>>>(nb, lbtup, heads); }
+    uint32_t* dtot32 = (uint32_t*)(dtot + 1);
+    exclusive_scan<uint32_t, uint32_t>(heads, hscan, nt, dtot32, J.rs.scan_tmp, st);
+    const uint32_t nn = d2h<uint32_t>(dtot32, st);
+    if (nn >= (1u << 31)) throw Error("phase 2: more than 2^31 nodes in one chunk");
+    uint32_t* node_start = P.node_start.ensure<uint32_t>(uint64_t(nn) + 1);
+    uint64_t* node_off = P.node_off.ensure<uint64_t>(nb + 1);
+    { KSCOPE("k_node_start", st); k_node_start<<<blocks_for(nt), TPB, 0, st>>>(nt, heads, hscan, nn, node_start); }
+    { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, nt, hscan, nn, node_off); }
+    K* node_key = P.node_key.ensure<K>(nn);
+    uint8_t* meta = P.node_meta.ensure<uint8_t>(nn);
+    uint32_t* rank = P.node_rank.ensure<uint32_t>(nn);
+    uint32_t* ncol = P.node_ncol.ensure<uint32_t>(nn);
+    uint64_t* coff = P.node_coff.ensure<uint64_t>(uint64_t(nn) + 1);
+    unsigned long long* cnt = P.counters.ensure<unsigned long long>(8);
+    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, 8, 0ull); }
+    { KSCOPE("k_node_reduce", st); k_node_reduce<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, keys, vals, node_key, meta, rank, ncol, cnt); }
+    exclusive_scan<uint32_t, uint64_t>(ncol, coff, nn, coff + nn, J.rs.scan_tmp, st);
+    const uint64_t n_cols = d2h<uint64_t>(coff + nn, st);
+    uint32_t* colors = P.colors.ensure<uint32_t>(n_cols);
+    { KSCOPE("k_node_colors", st); k_node_colors<<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, vals, coff, colors); }
+    unsigned long long h_cnt[8];
+    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1], multi_cols = h_cnt[7];
+
+    // --- colour sets: intern into the job-wide tables
+    uint64_t cap = P.mt_cap;
+    while (cap < 2 * (P.mt_count + n_multi)) cap <<= 1;
+    if (cap != P.mt_cap) mt_resize(P, cap, st);
+    const uint64_t store_used = d2h<uint64_t>(P.store_used.p, st);
+    P.store.ensure_keep<uint32_t>(store_used + multi_cols + 1, store_used * 4, st);
+    SetTables T = set_tables(P);
+    uint32_t* node_slot = P.node_slot.ensure<uint32_t>(nn);
+    { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, rank, node_off, nb, b0, T, node_slot); }
+    { KSCOPE("k_set_verify", st); k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
+    const uint32_t NC = J.num_colors;
+    const uint64_t nflag = NC + P.mt_cap;
+    uint32_t* set_fl = P.set_fl.ensure<uint32_t>(nflag);
+    uint64_t* set_pos = P.set_pos.ensure<uint64_t>(nflag);
+    { KSCOPE("k_newset_flags", st); k_newset_flags<<<blocks_for(nflag), TPB, 0, st>>>(NC, T, set_fl); }
+    exclusive_scan<uint32_t, uint64_t>(set_fl, set_pos, nflag, dtot + 2, J.rs.scan_tmp, st);
+    const uint64_t n_new = d2h<uint64_t>(dtot + 2, st);
+    uint64_t* set_key = P.set_key.ensure<uint64_t>(n_new);
+    uint64_t* set_desc = P.set_desc.ensure<uint64_t>(n_new);
+    uint64_t* set_key_t = P.set_key_t.ensure<uint64_t>(n_new);
+    uint64_t* set_desc_t = P.set_desc_t.ensure<uint64_t>(n_new);
+    { KSCOPE("k_newset_emit", st); k_newset_emit<<<blocks_for(nflag), TPB, 0, st>>>(NC, T, set_fl, set_pos, set_key, set_desc); }
+    SortPlan splan;
+    splan.build({0, n_new});
+    segmented_radix_sort<uint64_t, uint64_t>(set_key, set_desc, set_key_t, set_desc_t, splan, 0, 32 + bbits, 8, P.rs2,
+                                             st, "sets");
+    const uint64_t cid_base = P.n_sets;
+    uint64_t* set_bits = P.set_bits.ensure<uint64_t>(n_new);
+    uint64_t* set_boff = P.set_boff.ensure<uint64_t>(n_new + 1);
+    if (n_new) { KSCOPE("k_set_cid_len", st); k_set_cid_len<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, set_desc, T, NC, set_bits, cnt); }
+    exclusive_scan<uint64_t, uint64_t>(set_bits, set_boff, n_new, set_boff + n_new, J.rs.scan_tmp, st);
+    const uint64_t new_bits = d2h<uint64_t>(set_boff + n_new, st);
+    const uint64_t bit_base = P.cs_bits;
+    const uint64_t words_needed = (bit_base + new_bits + 63) / 64 + 1;
+    auto* words = P.cs_words.ensure_keep<unsigned long long>(words_needed, P.words_zeroed * 8, st);
+    if (words_needed > P.words_zeroed) {
+        KSCOPE("k_fill", st);
+        k_fill<unsigned long long><<<blocks_for(words_needed - P.words_zeroed), TPB, 0, st>>>(
+            words + P.words_zeroed, words_needed - P.words_zeroed, 0ull);
+        P.words_zeroed = words_needed;
+    }
+    uint64_t* set_off = P.set_off.ensure_keep<uint64_t>(cid_base + n_new, cid_base * 8, st);
+    if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
+    uint32_t* node_cid = P.node_cid.ensure<uint32_t>(nn);
+    { KSCOPE("k_node_cid", st); k_node_cid<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, node_cid); }
+    P.n_sets += n_new;
+    P.cs_bits += new_bits;
+    P.mt_count += n_new;  // upper bound of occupied slots (singletons included): sizes the table
+
+    // --- per-bucket k-mer table
+    std::vector<uint64_t> h_node_off(nb + 1);
+    CDBG_HIP(hipMemcpyAsync(h_node_off.data(), node_off, (nb + 1) * 8, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    std::vector<uint64_t> h_tab_off(nb + 1, 0);
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint64_t n_b = h_node_off[b + 1] - h_node_off[b];
+        uint64_t c = 8;
+        while (c * 5 < n_b * 8) c <<= 1;  // load factor <= 0.625
+        h_tab_off[b + 1] = h_tab_off[b] + (n_b ? c : 0);
+    }
+    uint64_t* tab_off = P.tab_off.ensure<uint64_t>(nb + 1);
+    CDBG_HIP(hipMemcpyAsync(tab_off, h_tab_off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipStreamSynchronize(st));  // h_tab_off is pageable and local
+    const uint64_t tab_n = h_tab_off[nb];
+    Slot<K>* tab = P.tab.ensure<Slot<K>>(tab_n);
+    Slot<K> empty{};
+    empty.node = NONE32;
+    { KSCOPE("k_fill", st); k_fill<Slot<K>><<<blocks_for(tab_n), TPB, 0, st>>>(tab, tab_n, empty); }
+    { KSCOPE("k_table_insert", st); k_table_insert<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, nb, tab_off, tab); }
+    uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
+    uint32_t* nbr = P.nb.ensure<uint32_t>(2ull * nn);
+    uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
+    { KSCOPE("k_masks", st); k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, nb, tab_off, tab, k, node_mask, nbr); }
+    { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link); }
+
+    // --- walk
+    uint8_t* visited = P.visited.ensure<uint8_t>(nn);
+    { KSCOPE("k_fill", st); k_fill<uint8_t><<<blocks_for(nn), TPB, 0, st>>>(visited, nn, uint8_t(0)); }
+    WalkArgs wa{};
+    wa.n_nodes = nn;
+    wa.link = link;
+    wa.meta = meta;
+    wa.mask = node_mask;
+    wa.rank = rank;
+    wa.node_off = node_off;
+    wa.nb = nb;
+    wa.b0 = b0;
+    wa.visited = visited;
+    wa.frag_nodes = P.frag_nodes.ensure<uint32_t>(nn);
+    wa.frag_rc = P.frag_rc.ensure<uint8_t>(nn);
+    wa.frag_off = P.frag_off.ensure<uint64_t>(nn);
+    wa.frag_n = P.frag_n.ensure<uint32_t>(nn);
+    wa.frag_open = P.frag_open.ensure<uint8_t>(nn);
+    wa.frag_key = P.frag_key.ensure<uint64_t>(nn);
+    wa.counters = cnt;
+    { KSCOPE("k_walk_paths", st); k_walk_paths<<<blocks_for(nn), TPB, 0, st>>>(wa); }
+    { KSCOPE("k_walk_cycles", st); k_walk_cycles<<<blocks_for(nn), TPB, 0, st>>>(wa); }
+    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    if (h_cnt[6]) throw Error("phase 2: colour-set content hash collision (" + std::to_string(h_cnt[6]) + " nodes)");
+    if (h_cnt[5] || h_cnt[3] != nn)
+        throw Error("phase 2: walk covered " + std::to_string(h_cnt[3]) + " of " + std::to_string(nn) +
+                    " nodes (" + std::to_string(h_cnt[5]) + " errors)");
+    const uint64_t nf = h_cnt[4];
+
+    // --- fragments in (bucket, seed rank) order, appended to the job stream
+    uint64_t* fkey_t = P.fkey_t.ensure<uint64_t>(nf);
+    uint32_t* forder = P.forder.ensure<uint32_t>(nf);
+    uint32_t* forder_t = P.forder_t.ensure<uint32_t>(nf);
+    { KSCOPE("k_iota", st); k_iota<<<blocks_for(nf), TPB, 0, st>>>(forder, nf); }
+    SortPlan fplan;
+    fplan.build({0, nf});
+    segmented_radix_sort<uint64_t, uint32_t>(wa.frag_key, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
+                                             "frags");
+    uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
+    uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
+    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_nodes, node_cid, k, fsize); }
+    exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
+    const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
+    uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st);
+    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_nodes, wa.frag_rc, node_cid, node_key, k, fboff, fout + P.frag_total); }
+    CDBG_HIP(hipGetLastError());
+    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    P.frag_total += fbytes;
+    P.n_frags += nf;
+    P.n_nodes += nn;
+    P.n_tuples += nt;
+    P.n_primary += n_primary;
+    P.total_integers += h_cnt[2];
 }
 
 template <class K>
@@ -854,212 +1224,76 @@ static void phase2_impl(Job& J) {
     CDBG_HIP(hipEventCreate(&e1));
     CDBG_HIP(hipEventRecord(e0, st));
 
-    // --- expand
+    // job-wide state
+    P.n_tuples = P.n_nodes = P.n_sets = P.n_frags = P.frag_total = P.cs_bits = 0;
+    P.n_primary = P.total_integers = P.words_zeroed = 0;
+    P.mt_count = 0;
+    const uint32_t NC = J.num_colors;
+    auto* single_min = P.single_min.ensure<unsigned long long>(NC);
+    auto* single_cid = P.single_cid.ensure<uint32_t>(NC);
+    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(NC), TPB, 0, st>>>(single_min, NC, ~0ull); }
+    { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(NC), TPB, 0, st>>>(single_cid, NC, NONE32); }
+    P.store_used.ensure<unsigned long long>(1);
+    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(P.store_used.as<unsigned long long>(), 1, 0ull); }
+    if (P.mt_cap == 0) {
+        mt_resize(P, 1024, st);
+    } else {  // clear, keeping the capacity of the previous run
+        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(P.mt_cap), TPB, 0, st>>>(P.mt_hash.as<unsigned long long>(), P.mt_cap, 0ull); }
+        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(P.mt_cap), TPB, 0, st>>>(P.mt_key.as<unsigned long long>(), P.mt_cap, ~0ull); }
+        { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(P.mt_cap), TPB, 0, st>>>(P.mt_cid.as<uint32_t>(), P.mt_cap, NONE32); }
+    }
+
+    // k-mer occurrences of every record, per bucket
     uint32_t* kcnt = P.kcnt.ensure<uint32_t>(ns);
     uint64_t* tup_off = P.tup_off.ensure<uint64_t>(ns + 1);
-    uint64_t* dtot = P.total.ensure<uint64_t>(4);
-    if (ns) k_count_kmers<<<blocks_for(ns), TPB, 0, st>>>(ns, Q.b_len.as<uint32_t>(), k, kcnt);
-    exclusive_scan<uint32_t, uint64_t>(kcnt, tup_off, ns, dtot, J.rs.scan_tmp, st);
-    const uint64_t nt = d2h<uint64_t>(dtot, st);
-    P.n_tuples = nt;
-    if (nt >= (1ull << 32)) throw Error("phase 2: more than 2^32 k-mer occurrences in one job");
+    if (ns) { KSCOPE("k_count_kmers", st); k_count_kmers<<<blocks_for(ns), TPB, 0, st>>>(ns, Q.b_len.as<uint32_t>(), k, kcnt); }
+    exclusive_scan<uint32_t, uint64_t>(kcnt, tup_off, ns, tup_off + ns, J.rs.scan_tmp, st);
     uint64_t* btup = P.btup.ensure<uint64_t>(B + 1);
-    k_bucket_tuple_off<<<blocks_for(B + 1), TPB, 0, st>>>(B, Q.bucket_off.as<uint64_t>(), tup_off, ns, nt, btup);
+    { KSCOPE("k_bucket_tuple_off", st); k_bucket_tuple_off<<<blocks_for(B + 1), TPB, 0, st>>>(B, Q.bucket_off.as<uint64_t>(), tup_off, btup); }
     std::vector<uint64_t> h_btup(B + 1);
     CDBG_HIP(hipMemcpyAsync(h_btup.data(), btup, (B + 1) * 8, hipMemcpyDeviceToHost, st));
-    K* keys = P.keys.ensure<K>(nt);
-    uint64_t* vals = P.vals.ensure<uint64_t>(nt);
-    K* keys_t = P.keys_t.ensure<K>(nt);
-    uint64_t* vals_t = P.vals_t.ensure<uint64_t>(nt);
-    if (ns)
-        k_expand<K><<<blocks_for(ns), TPB, 0, st>>>(ns, Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(),
-                                                    Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(),
-                                                    Q.key.as<uint32_t>(), tup_off, btup, Q.packed.as<uint32_t>(), k,
-                                                    keys, vals);
     CDBG_HIP(hipStreamSynchronize(st));
 
-    // --- sort tuples by k-mer inside every bucket
-    SortPlan plan;
-    plan.build(h_btup);
-    segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, 0, 2 * int(k), 8, P.rs2, st);
-
-    // --- nodes
-    uint32_t* heads = P.heads.ensure<uint32_t>(nt);
-    uint32_t* hscan = P.hscan.ensure<uint32_t>(nt + 1);
-    if (nt) k_heads<K><<<blocks_for(nt), TPB, 0, st>>>(nt, keys, heads);
-    k_mark_bucket_starts<<<blocks_for(B), TPB, 0, st>>>(B, btup, heads);
-    uint32_t* dtot32 = (uint32_t*)(dtot + 1);
-    exclusive_scan<uint32_t, uint32_t>(heads, hscan, nt, dtot32, J.rs.scan_tmp, st);
-    const uint32_t nn = d2h<uint32_t>(dtot32, st);
-    P.n_nodes = nn;
-    if (nn >= (1u << 31)) throw Error("phase 2: more than 2^31 nodes in one job");
-    uint32_t* node_start = P.node_start.ensure<uint32_t>(uint64_t(nn) + 1);
-    uint64_t* node_off = P.node_off.ensure<uint64_t>(B + 1);
-    k_node_start<<<blocks_for(nt), TPB, 0, st>>>(nt, heads, hscan, nn, node_start);
-    k_node_off<<<blocks_for(B + 1), TPB, 0, st>>>(B, btup, nt, hscan, nn, node_off);
-    K* node_key = P.node_key.ensure<K>(nn);
-    uint8_t* meta = P.node_meta.ensure<uint8_t>(nn);
-    uint32_t* rank = P.node_rank.ensure<uint32_t>(nn);
-    uint32_t* ncol = P.node_ncol.ensure<uint32_t>(nn);
-    uint64_t* coff = P.node_coff.ensure<uint64_t>(uint64_t(nn) + 1);
-    unsigned long long* cnt = P.counters.ensure<unsigned long long>(8);
-    k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, 8, 0ull);
-    k_node_reduce<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, keys, vals, node_key, meta, rank, ncol, cnt);
-    exclusive_scan<uint32_t, uint64_t>(ncol, coff, nn, coff + nn, J.rs.scan_tmp, st);
-    const uint64_t n_colors_total = d2h<uint64_t>(coff + nn, st);
-    uint32_t* colors = P.colors.ensure<uint32_t>(n_colors_total);
-    k_node_colors<<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, vals, coff, colors);
-    unsigned long long h_cnt[8];
-    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipStreamSynchronize(st));
-    const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1];
-
-    // --- colour sets
-    const uint32_t NC = J.num_colors;
-    uint64_t mt_cap = 64;
-    while (mt_cap < 2 * n_multi) mt_cap <<= 1;
-    auto* single_min = P.single_min.ensure<unsigned long long>(NC);
-    uint32_t* single_cid = P.single_cid.ensure<uint32_t>(NC);
-    auto* mt_hash = P.mt_hash.ensure<unsigned long long>(mt_cap);
-    auto* mt_key = P.mt_key.ensure<unsigned long long>(mt_cap);
-    uint32_t* mt_rep = P.mt_rep.ensure<uint32_t>(mt_cap);
-    uint32_t* mt_cid = P.mt_cid.ensure<uint32_t>(mt_cap);
-    uint32_t* node_slot = P.node_slot.ensure<uint32_t>(nn);
-    k_fill<unsigned long long><<<blocks_for(NC), TPB, 0, st>>>(single_min, NC, ~0ull);
-    k_fill<unsigned long long><<<blocks_for(mt_cap), TPB, 0, st>>>(mt_hash, mt_cap, 0ull);
-    k_fill<unsigned long long><<<blocks_for(mt_cap), TPB, 0, st>>>(mt_key, mt_cap, ~0ull);
-    k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, rank, node_off, B, single_min, mt_hash,
-                                                 mt_key, mt_cap, node_slot);
-    k_set_rep<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, rank, node_off, B, node_slot, mt_key, mt_rep);
-    k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, mt_rep, cnt + 6);
-    const uint64_t nflag = NC + mt_cap;
-    uint32_t* set_fl = P.set_fl.ensure<uint32_t>(nflag);
-    uint64_t* set_pos = P.set_pos.ensure<uint64_t>(nflag);
-    k_set_flags<<<blocks_for(nflag), TPB, 0, st>>>(NC, mt_cap, single_min, mt_hash, set_fl);
-    exclusive_scan<uint32_t, uint64_t>(set_fl, set_pos, nflag, dtot + 2, J.rs.scan_tmp, st);
-    const uint64_t n_sets = d2h<uint64_t>(dtot + 2, st);
-    P.n_sets = n_sets;
-    uint64_t* set_key = P.set_key.ensure<uint64_t>(n_sets);
-    uint64_t* set_desc = P.set_desc.ensure<uint64_t>(n_sets);
-    uint64_t* set_key_t = P.set_key_t.ensure<uint64_t>(n_sets);
-    uint64_t* set_desc_t = P.set_desc_t.ensure<uint64_t>(n_sets);
-    k_set_emit<<<blocks_for(nflag), TPB, 0, st>>>(NC, mt_cap, set_fl, set_pos, single_min, mt_key, set_key, set_desc);
-    SortPlan splan;
-    splan.build({0, n_sets});
-    const int bbits = bits_for(B);
-    segmented_radix_sort<uint64_t, uint64_t>(set_key, set_desc, set_key_t, set_desc_t, splan, 0, 32 + bbits, 8, P.rs2,
-                                             st);
-    uint64_t* set_bits = P.set_bits.ensure<uint64_t>(n_sets);
-    uint64_t* set_boff = P.set_boff.ensure<uint64_t>(n_sets + 1);
-    if (n_sets)
-        k_set_cid_len<<<blocks_for(n_sets), TPB, 0, st>>>(n_sets, set_desc, mt_rep, coff, ncol, colors, NC, single_cid,
-                                                         mt_cid, set_bits, cnt);
-    exclusive_scan<uint64_t, uint64_t>(set_bits, set_boff, n_sets, set_boff + n_sets, J.rs.scan_tmp, st);
-    const uint64_t cs_bits = d2h<uint64_t>(set_boff + n_sets, st);
-    P.cs_bits = cs_bits;
-    const uint64_t n_words = (cs_bits + 63) / 64;
-    auto* words = P.cs_words.ensure<unsigned long long>(n_words + 1);
-    k_fill<unsigned long long><<<blocks_for(n_words + 1), TPB, 0, st>>>(words, n_words + 1, 0ull);
-    if (n_sets)
-        k_set_encode<<<blocks_for(n_sets), TPB, 0, st>>>(n_sets, set_desc, set_boff, mt_rep, coff, ncol, colors, NC,
-                                                        words);
-    uint32_t* node_cid = P.node_cid.ensure<uint32_t>(nn);
-    k_node_cid<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, single_cid, mt_cid, node_cid);
-
-    // --- per-bucket k-mer table
-    std::vector<uint64_t> h_node_off(B + 1);
-    CDBG_HIP(hipMemcpyAsync(h_node_off.data(), node_off, (B + 1) * 8, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipStreamSynchronize(st));
-    std::vector<uint64_t> h_tab_off(B + 1, 0);
-    for (uint32_t b = 0; b < B; ++b) {
-        const uint64_t nb_ = h_node_off[b + 1] - h_node_off[b];
-        uint64_t cap = 8;
-        while (cap * 5 < nb_ * 8) cap <<= 1;  // load factor <= 0.625
-        h_tab_off[b + 1] = h_tab_off[b] + (nb_ ? cap : 0);
+    // chunk budget: k-mer occurrences whose working set fits free HBM
+    uint64_t budget = J.chunk_tuples ? J.chunk_tuples : J.auto_chunk;
+    if (budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
+        // ~130 B per occurrence covers tuples, sort buffers, nodes, table,
+        // walk and fragment arrays; buffers this phase already holds from an
+        // earlier run count as available; 40% is kept for the growing output
+        // streams.
+        const uint64_t held = g_device_bytes.load();
+        const uint64_t usable = uint64_t(free_b) + std::min<uint64_t>(held, uint64_t(free_b)) / 4;
+        budget = std::max<uint64_t>(1u << 20, uint64_t(0.6 * double(usable)) / 130);
+        budget = std::min<uint64_t>(budget, (1ull << 31) - 1);
+        J.auto_chunk = budget;  // later runs of this job plan the same chunks
     }
-    uint64_t* tab_off = P.tab_off.ensure<uint64_t>(B + 1);
-    CDBG_HIP(hipMemcpyAsync(tab_off, h_tab_off.data(), (B + 1) * 8, hipMemcpyHostToDevice, st));
-    CDBG_HIP(hipStreamSynchronize(st));  // h_tab_off is pageable and local
-    const uint64_t tab_n = h_tab_off[B];
-    Slot<K>* tab = P.tab.ensure<Slot<K>>(tab_n);
-    Slot<K> empty{};
-    empty.node = NONE32;
-    k_fill<Slot<K>><<<blocks_for(tab_n), TPB, 0, st>>>(tab, tab_n, empty);
-    k_table_insert<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, B, tab_off, tab);
-    uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
-    uint32_t* nb = P.nb.ensure<uint32_t>(2ull * nn);
-    uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
-    k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, B, tab_off, tab, k, node_mask, nb);
-    k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nb, link);
+    uint32_t nchunks = 0;
+    for (uint32_t b0 = 0; b0 < B;) {
+        uint32_t b1 = b0 + 1;
+        while (b1 < B && h_btup[b1 + 1] - h_btup[b0] <= budget) ++b1;
+        run_chunk<K>(J, P, b0, b1, h_btup);
+        ++nchunks;
+        b0 = b1;
+    }
+    J.n_chunks = nchunks;
 
-    // --- walk
-    uint8_t* visited = P.visited.ensure<uint8_t>(nn);
-    k_fill<uint8_t><<<blocks_for(nn), TPB, 0, st>>>(visited, nn, uint8_t(0));
-    WalkArgs wa{};
-    wa.n_nodes = nn;
-    wa.link = link;
-    wa.meta = meta;
-    wa.mask = node_mask;
-    wa.rank = rank;
-    wa.node_off = node_off;
-    wa.B = B;
-    wa.visited = visited;
-    wa.frag_nodes = P.frag_nodes.ensure<uint32_t>(nn);
-    wa.frag_rc = P.frag_rc.ensure<uint8_t>(nn);
-    wa.frag_off = P.frag_off.ensure<uint64_t>(nn);
-    wa.frag_n = P.frag_n.ensure<uint32_t>(nn);
-    wa.frag_open = P.frag_open.ensure<uint8_t>(nn);
-    wa.frag_key = P.frag_key.ensure<uint64_t>(nn);
-    wa.counters = cnt;
-    k_walk_paths<<<blocks_for(nn), TPB, 0, st>>>(wa);
-    k_walk_cycles<<<blocks_for(nn), TPB, 0, st>>>(wa);
-    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipStreamSynchronize(st));
-    if (h_cnt[6]) throw Error("phase 2: colour-set content hash collision (" + std::to_string(h_cnt[6]) + " nodes)");
-    if (h_cnt[5] || h_cnt[3] != nn)
-        throw Error("phase 2: walk covered " + std::to_string(h_cnt[3]) + " of " + std::to_string(nn) +
-                    " nodes (" + std::to_string(h_cnt[5]) + " errors)");
-    const uint64_t nf = h_cnt[4];
-    P.n_frags = nf;
-
-    // --- fragments in (bucket, seed rank) order
-    uint64_t* fkey_t = P.fkey_t.ensure<uint64_t>(nf);
-    uint32_t* forder = P.forder.ensure<uint32_t>(nf);
-    uint32_t* forder_t = P.forder_t.ensure<uint32_t>(nf);
-    k_iota<<<blocks_for(nf), TPB, 0, st>>>(forder, nf);
-    SortPlan fplan;
-    fplan.build({0, nf});
-    segmented_radix_sort<uint64_t, uint32_t>(wa.frag_key, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st);
-    uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
-    uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
-    if (nf)
-        k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_nodes, node_cid, k,
-                                                    fsize);
-    exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
-    const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
-    P.frag_total = fbytes;
-    uint8_t* fout = P.frag_bytes.ensure<uint8_t>(fbytes);
-    if (nf)
-        k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_nodes,
-                                                        wa.frag_rc, node_cid, node_key, k, fboff, fout);
-    CDBG_HIP(hipGetLastError());
-    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipEventRecord(e1, st));
     CDBG_HIP(hipStreamSynchronize(st));
     float ms = 0;
     CDBG_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-
-    J.stats.num_occ = nt;
-    J.stats.num_tuples = nt;
-    J.stats.num_nodes = nn;
-    J.stats.num_kmers = n_primary;
-    J.stats.num_color_sets = n_sets;
-    J.stats.num_frags = nf;
-    J.stats.frag_bytes = fbytes;
-    J.stats.cs_bits = cs_bits;
-    J.stats.total_integers = h_cnt[2];
+    J.stats.num_occ = P.n_tuples;
+    J.stats.num_tuples = P.n_tuples;
+    J.stats.num_nodes = P.n_nodes;
+    J.stats.num_kmers = P.n_primary;
+    J.stats.num_color_sets = P.n_sets;
+    J.stats.num_frags = P.n_frags;
+    J.stats.frag_bytes = P.frag_total;
+    J.stats.cs_bits = P.cs_bits;
+    J.stats.total_integers = P.total_integers;
     J.stats.ms_phase2 = ms;
     J.phase2_done = true;
 }
@@ -1100,12 +1334,22 @@ CDBG_API int cdbg_gpu_job_color_sets(cdbg_gpu_job* job, uint64_t* words, uint64_
         if (!J.phase2_done) throw Error("phase 2 has not run");
         const uint64_t nw = (J.p2->cs_bits + 63) / 64;
         if (nw) CDBG_HIP(hipMemcpy(words, J.p2->cs_words.p, nw * 8, hipMemcpyDeviceToHost));
-        if (J.p2->n_sets)
-            CDBG_HIP(hipMemcpy(offsets, J.p2->set_boff.p, J.p2->n_sets * 8, hipMemcpyDeviceToHost));
+        if (J.p2->n_sets) CDBG_HIP(hipMemcpy(offsets, J.p2->set_off.p, J.p2->n_sets * 8, hipMemcpyDeviceToHost));
         return 0;
     } catch (std::exception& e) {
         return set_error(e.what());
     }
+}
+
+CDBG_API int cdbg_gpu_job_set_chunk_tuples(cdbg_gpu_job* job, uint64_t max_tuples) {
+    if (!job) return set_error("null job");
+    job->J.chunk_tuples = max_tuples;
+    return 0;
+}
+
+CDBG_API int64_t cdbg_gpu_job_num_chunks(cdbg_gpu_job* job) {
+    if (!job) return set_error("null job");
+    return job->J.n_chunks;
 }
 
 }  // extern "C"

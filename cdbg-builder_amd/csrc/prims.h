@@ -14,10 +14,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <vector>
 
 #include "common.h"
+#include "prof.h"
 
 namespace cdbg_amd {
 
@@ -32,7 +34,10 @@ struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            g_device_bytes -= cap;
+        }
         p = nullptr;
         cap = 0;
     }
@@ -44,11 +49,29 @@ struct DevBuf {
             release();
             CDBG_HIP(hipMalloc(&p, bytes));
             cap = bytes;
+            g_device_bytes += cap;
         }
         return (T*)p;
     }
     template <class T>
     T* as() const {
+        return (T*)p;
+    }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+    }
+    // Grows to hold n elements keeping the first `keep_bytes` bytes
+    // (stream-ordered copy; waits for it before freeing the old block).
+    template <class T>
+    T* ensure_keep(size_t n, size_t keep_bytes, hipStream_t st) {
+        size_t bytes = n * sizeof(T);
+        if (bytes <= cap) return (T*)p;
+        DevBuf nb;
+        nb.ensure<uint8_t>(std::max(bytes, cap + cap / 2));
+        if (keep_bytes && p) CDBG_HIP(hipMemcpyAsync(nb.p, p, std::min(keep_bytes, cap), hipMemcpyDeviceToDevice, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+        swap(nb);
         return (T*)p;
     }
 };
@@ -175,9 +198,9 @@ inline void exclusive_scan(const Tin* in, Tout* out, uint64_t n, Tout* d_total, 
     uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
     if (nb == 0) nb = 1;
     Tout* sums = tmp.ensure<Tout>(nb + 1);
-    k_scan_reduce<Tin, Tout><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, sums);
-    k_scan_single<Tout><<<1, 1024, 0, st>>>(sums, nb, d_total);
-    k_scan_down<Tin, Tout><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, sums, out);
+    { KSCOPE("k_scan_reduce", st); k_scan_reduce<Tin, Tout><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, sums); }
+    { KSCOPE("k_scan_single", st); k_scan_single<Tout><<<1, 1024, 0, st>>>(sums, nb, d_total); }
+    { KSCOPE("k_scan_down", st); k_scan_down<Tin, Tout><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, sums, out); }
     CDBG_HIP(hipGetLastError());
 }
 
@@ -303,7 +326,8 @@ struct RadixScratch {
 
 template <class K, class V>
 inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan& plan, int begin_bit,
-                                 int end_bit, int digit_bits, RadixScratch& S, hipStream_t st) {
+                                 int end_bit, int digit_bits, RadixScratch& S, hipStream_t st,
+                                 const char* tag = "") {
     if (plan.n == 0 || begin_bit >= end_bit) return;
     if (plan.n >= (uint64_t(1) << 32)) throw Error("segmented_radix_sort: more than 2^32 elements");
     const uint32_t nt = uint32_t(plan.tiles.size());
@@ -323,10 +347,10 @@ inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan&
         const uint64_t hl = uint64_t(nt) * R;
         uint32_t* hist = S.hist.ensure<uint32_t>(hl);
         uint32_t* offs = S.offs.ensure<uint32_t>(hl);
-        k_radix_hist<K><<<nt, RS_THREADS, R * sizeof(uint32_t), st>>>(src_k, d_tiles, sh, bits, hist);
+        { KSCOPE(std::string("k_radix_hist:") + tag, st); k_radix_hist<K><<<nt, RS_THREADS, R * sizeof(uint32_t), st>>>(src_k, d_tiles, sh, bits, hist); }
         exclusive_scan<uint32_t, uint32_t>(hist, offs, hl, S.total.ensure<uint32_t>(1), S.scan_tmp, st);
-        k_radix_scatter<K, V><<<nt, RS_THREADS, 5 * R * sizeof(uint32_t), st>>>(src_k, src_v, dst_k, dst_v,
-                                                                              d_tiles, sh, bits, offs);
+        { KSCOPE(std::string("k_radix_scatter:") + tag, st); k_radix_scatter<K, V><<<nt, RS_THREADS, 5 * R * sizeof(uint32_t), st>>>(src_k, src_v, dst_k, dst_v,
+                                                                              d_tiles, sh, bits, offs); }
         CDBG_HIP(hipGetLastError());
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);

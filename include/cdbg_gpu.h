@@ -93,6 +93,27 @@ CDBG_API int cdbg_gpu_job_supers(cdbg_gpu_job* job, int which, uint64_t* start, 
                                  uint8_t* flags, uint32_t* bucket, uint32_t* color);
 CDBG_API int cdbg_gpu_job_bucket_offsets(cdbg_gpu_job* job, uint64_t* out);
 
+/* Per-kernel timing with HIP events on the job's stream (off by default).
+ * Enabling resets the totals.  cdbg_gpu_job_kernel_times writes lines
+ * "<kernel> <launches> <total ms>" into buf (NUL-terminated, truncated to cap)
+ * and returns the full text length.  No reference counterpart: it replaces
+ * the reference's in-process phase profilers (util.hpp:433-585). */
+CDBG_API int cdbg_gpu_job_set_profiling(cdbg_gpu_job* job, int on);
+CDBG_API int64_t cdbg_gpu_job_kernel_times(cdbg_gpu_job* job, char* buf, uint64_t cap);
+
+/* Device bytes currently held by all jobs of this process (the HBM gauge an
+ * admission policy reads; the reference's analogue is the live RSS probe,
+ * util.hpp:113-215). */
+CDBG_API uint64_t cdbg_gpu_device_bytes(void);
+
+/* Phase 2 works on chunks of consecutive buckets whose k-mer occurrences fit
+ * a budget, so HBM use is bounded by the chunk.  0 (default) derives the
+ * budget from free HBM on the first run.  Results do not depend on it.  The
+ * reference bounds host RAM per in-flight bucket instead
+ * (bucket_walker.hpp:623-671). */
+CDBG_API int cdbg_gpu_job_set_chunk_tuples(cdbg_gpu_job* job, uint64_t max_tuples);
+CDBG_API int64_t cdbg_gpu_job_num_chunks(cdbg_gpu_job* job);
+
 /* Phase-2 artifacts: the frag_unitigs.bin byte stream (frag_bytes bytes),
  * the colour-set bit stream (ceil(cs_bits/64) words) and the bit offset of
  * every class (num_color_sets words). */

@@ -16,10 +16,14 @@ pytestmark = pytest.mark.gpu
 ACGT = np.frombuffer(b"ACGT", np.uint8)
 
 
-def _run_both(ctx, data, off, k, m, B):
+def _run_both(ctx, data, off, k, m, B, chunk=0):
     ob = O.Build(data, off, k, m, B, threads=4)
     with ctx.job(data, off, k, m, B) as job:
+        if chunk:
+            job.set_chunk_tuples(chunk)
         job.run()
+        if chunk:
+            assert job.num_chunks() > 1
         s = job.stats()
         gf = job.frags()
         gw, go = job.color_sets()
@@ -109,3 +113,12 @@ def test_phase2_distinct_kmers_bruteforce(gpu_ctx):
     with gpu_ctx.job(data, off, k, 11, 16) as job:
         job.run()
         assert job.stats()["num_kmers"] == len(kms)
+
+
+@pytest.mark.parametrize("k,m,B,chunk", [(31, 19, 64, 20000), (31, 19, 64, 1), (63, 27, 97, 30000),
+                                         (31, 21, 1024, 50000)])
+def test_phase2_chunked(gpu_ctx, k, m, B, chunk):
+    """Buckets processed in several chunks: colour-set ids, the colour-set
+    stream and the fragment stream continue across chunks."""
+    data, off = G.synth(seed=8, num_colors=7, genome_len=25000, snp_rate=0.015, indel_rate=0.001)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B, chunk=chunk))
