@@ -41,24 +41,24 @@ WORKLOAD = dict(seed=2, snp_rate=0.02, indel_rate=0.002, indel_max=10, pav_frac=
                 pav_min=1000, pav_max=5000, n_inversions=50, inv_min=10000, inv_max=100000)
 
 
-def alg_bytes(name, s, k):
-    """Algorithmic (compulsory) HBM bytes of ONE launch of `name`."""
+def step_bytes(name, s, k):
+    """Algorithmic (compulsory) HBM bytes moved by all launches of `name` in
+    one step: each buffer touched once, from the job's sizes."""
     wk = 8 if k <= 31 else 16
+    passes = -(-2 * k // 8)  # 8-bit digits over the 2k key bits
     n, nn, ns, nb = s["num_tuples"], s["num_nodes"], s["num_supers"], s["num_bases"]
     table = {
-        # read key+value, write key+value
-        "k_radix_scatter:tuples": 2 * n * (wk + 8),
-        # read keys
-        "k_radix_hist:tuples": n * wk,
-        # read the super descriptors + 2-bit bases, write one tuple per k-mer
+        # read key+value, write key+value, every pass
+        "k_radix_scatter:tuples": passes * 2 * n * (wk + 8),
+        # read keys, every pass
+        "k_radix_hist:tuples": passes * n * wk,
+        # super records + 2-bit bases in, one (k-mer, value) tuple out per k-mer
         "k_expand": ns * 29 + nb // 4 + n * (wk + 8),
         # ASCII in, 2-bit codes + ACGT bitmap out
         "k_pack2bit": nb + nb // 4 + nb // 8,
-        # 2-bit codes + bitmap in, super starts/ends out
-        "k_superkmers": nb // 4 + nb // 8 + 22 * ns,
-        # tuples in, node records out
-        "k_node_reduce": n * (wk + 8) + nn * (wk + 13),
-        # node keys in, 8 table probes of one 16/32-byte slot, mask + links out
+        # tuple values in, node records out
+        "k_node_reduce": n * 8 + nn * (4 + wk) + nn * (wk + 9),
+        # node keys in, 8 probes of one table slot, mask + neighbours out
         "k_masks": nn * wk + 8 * nn * (16 if wk == 8 else 32) + nn * 9,
     }
     return table.get(name)
@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel totals to stderr")
+    ap.add_argument("--copy-probe-only", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel over BYTES (counter calibration)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -99,6 +101,11 @@ def main():
         sys.exit(2)
 
     G = load_pkg()
+    if args.copy_probe_only:
+        ctx = G.Context(0)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe_only,
+                          "gbps": ctx.copy_probe(args.copy_probe_only, 3)}))
+        return
     t_gen = time.perf_counter()
     data, off = G.synth(num_colors=args.colors, genome_len=args.genome, threads=args.cpu_cores,
                         **WORKLOAD)
@@ -107,6 +114,7 @@ def main():
     ctx = G.Context(0)
     job = ctx.job(data, off, args.k, args.m, args.buckets)
     del data
+    copy_gbps = ctx.copy_probe(4 << 30, 3)
     for i in range(args.warmup):
         job.run()
         print(f"bench: warmup {i} done ({G.lib().cdbg_gpu_device_bytes() / 2**30:.1f} GiB on device)",
@@ -126,19 +134,24 @@ def main():
         for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
             print(f"  {name:32s} {cnt:6d} {ms:10.2f} ms", file=sys.stderr)
 
-    # dominant kernel with a known algorithmic-byte count
+    # roofline of the dominant kernel that has an algorithmic-byte count
     roof = None
     for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
-        b = alg_bytes(name, s, args.k)
+        b = step_bytes(name, s, args.k)
         if b is None or cnt == 0:
             continue
-        avg_s = ms / cnt / 1e3
-        ach = b / avg_s / 1e9
+        per_launch = b * args.steps / cnt
+        avg_ms = ms / cnt
+        ach = per_launch / (avg_ms / 1e3) / 1e9
         roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": b, "avg_launch_ms": round(ms / cnt, 3),
+                "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 3),
+                "launches_per_step": cnt // args.steps,
+                "copy_probe_gbps": round(copy_gbps, 1),
                 "share_of_step": round(ms / args.steps / ms_step, 3)}
         break
+    top = sorted(kt.items(), key=lambda x: -x[1][1])[:6]
+    kernels = {n: round(ms / args.steps, 2) for n, (c, ms) in top}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -163,6 +176,8 @@ def main():
                    "num_kmers": s["num_kmers"], "num_occ": s["num_occ"],
                    "num_color_sets": s["num_color_sets"], "num_frags": s["num_frags"]},
         "roofline": roof,
+        "top_kernels_ms_per_step": kernels,
+        "chunks": job.num_chunks(),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

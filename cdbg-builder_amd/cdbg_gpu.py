@@ -51,6 +51,7 @@ _SIGS = {
     "cdbg_gpu_job_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "cdbg_gpu_job_kernel_times": (C.c_int64, [C.c_void_p, C.c_char_p, C.c_uint64]),
     "cdbg_gpu_device_bytes": (C.c_uint64, []),
+    "cdbg_gpu_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "cdbg_gpu_job_set_chunk_tuples": (C.c_int, [C.c_void_p, C.c_uint64]),
     "cdbg_gpu_job_num_chunks": (C.c_int64, [C.c_void_p]),
     "cdbg_gpu_job_frags": (C.c_int, [C.c_void_p, U8P]),
@@ -116,6 +117,12 @@ class Context:
         if self.h:
             lib().cdbg_gpu_ctx_destroy(self.h)
             self.h = C.c_void_p()
+
+    def copy_probe(self, nbytes, iters=5):
+        """Measured read+write GB/s of a plain device copy of nbytes."""
+        g = C.c_double()
+        _check(lib().cdbg_gpu_copy_probe(self.h, nbytes, iters, C.byref(g)))
+        return g.value
 
     def job(self, data, color_off, k, m, num_buckets):
         return Job(self, data, color_off, k, m, num_buckets)

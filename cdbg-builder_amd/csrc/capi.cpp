@@ -26,13 +26,18 @@ Job::~Job() {
     if (st) (void)hipStreamDestroy(st);
 }
 
+__global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace cdbg_amd
 
 using namespace cdbg_amd;
 
-#define CDBG_TRY(body)                          \
+#define CDBG_TRY(...)                           \
     try {                                       \
-        body;                                   \
+        __VA_ARGS__;                            \
         return 0;                               \
     } catch (std::exception & e) {              \
         return set_error(e.what());             \
@@ -188,6 +193,31 @@ CDBG_API int64_t cdbg_gpu_job_kernel_times(cdbg_gpu_job* job, char* buf, uint64_
 }
 
 CDBG_API uint64_t cdbg_gpu_device_bytes(void) { return g_device_bytes.load(); }
+
+CDBG_API int cdbg_gpu_copy_probe(cdbg_gpu_ctx* ctx, uint64_t bytes, int iters, double* gbps) {
+    CDBG_TRY({
+        if (!ctx || !gbps || iters < 1) throw Error("bad argument");
+        CDBG_HIP(hipSetDevice(ctx->device));
+        const uint64_t n = bytes / 16;
+        DevBuf a, b;
+        a.ensure<uint4>(n);
+        b.ensure<uint4>(n);
+        CDBG_HIP(hipMemset(a.p, 1, n * 16));
+        hipEvent_t e0, e1;
+        CDBG_HIP(hipEventCreate(&e0));
+        CDBG_HIP(hipEventCreate(&e1));
+        k_copy_probe<<<4096, 256>>>(a.as<uint4>(), b.as<uint4>(), n);
+        CDBG_HIP(hipEventRecord(e0, nullptr));
+        for (int i = 0; i < iters; ++i) k_copy_probe<<<4096, 256>>>(a.as<uint4>(), b.as<uint4>(), n);
+        CDBG_HIP(hipEventRecord(e1, nullptr));
+        CDBG_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        CDBG_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *gbps = 2.0 * double(n) * 16.0 * iters / (ms * 1e-3) / 1e9;
+    })
+}
 
 CDBG_API int cdbg_gpu_job_bucket_offsets(cdbg_gpu_job* job, uint64_t* out) {
     CDBG_TRY({

@@ -106,6 +106,11 @@ CDBG_API int64_t cdbg_gpu_job_kernel_times(cdbg_gpu_job* job, char* buf, uint64_
  * util.hpp:113-215). */
 CDBG_API uint64_t cdbg_gpu_device_bytes(void);
 
+/* Device copy of `bytes` bytes repeated `iters` times; *gbps = read+write
+ * GB/s.  Measurement aid (achievable HBM bandwidth, counter calibration); no
+ * reference counterpart. */
+CDBG_API int cdbg_gpu_copy_probe(cdbg_gpu_ctx* ctx, uint64_t bytes, int iters, double* gbps);
+
 /* Phase 2 works on chunks of consecutive buckets whose k-mer occurrences fit
  * a budget, so HBM use is bounded by the chunk.  0 (default) derives the
  * budget from free HBM on the first run.  Results do not depend on it.  The

@@ -35,3 +35,12 @@ def test_errors_are_reported_not_thrown():
     rc = G.lib().cdbg_gpu_ctx_create(10_000, ctypes.byref(ctx))
     assert rc != 0
     assert G.lib().cdbg_gpu_last_error()
+
+
+def test_library_carries_gfx950_code_only():
+    """Every device code object in the library targets gfx950 (a HIP source
+    built for another target has no kernel the MI355X can load)."""
+    import re as _re
+    data = open(G.LIB_PATH, "rb").read()
+    targets = set(_re.findall(rb"amdhsa--(gfx[0-9a-z]+)", data))
+    assert targets == {b"gfx950"}, targets
