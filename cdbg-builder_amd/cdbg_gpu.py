@@ -27,6 +27,17 @@ class Stats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class ColorState(C.Structure):
+    _fields_ = [("num_colors", C.c_uint32), ("sparse_threshold", C.c_uint32),
+                ("dense_threshold", C.c_uint32), ("num_color_sets", C.c_uint64),
+                ("num_bits", C.c_uint64), ("total_integers", C.c_uint64),
+                ("trailing_word", C.c_uint64), ("num_kmers", C.c_uint64),
+                ("num_fragments", C.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class SynthParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("num_colors", C.c_uint32), ("genome_len", C.c_uint64),
                 ("snp_rate", C.c_double), ("indel_rate", C.c_double), ("indel_max", C.c_uint32),
@@ -56,6 +67,24 @@ _SIGS = {
     "cdbg_gpu_job_num_chunks": (C.c_int64, [C.c_void_p]),
     "cdbg_gpu_job_frags": (C.c_int, [C.c_void_p, U8P]),
     "cdbg_gpu_job_color_sets": (C.c_int, [C.c_void_p, U64P, U64P]),
+    "cdbg_gpu_bucket_write": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32, U64P]),
+    "cdbg_gpu_bucket_process": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_char_p, C.c_char_p, C.POINTER(ColorState)]),
+    "cdbg_gpu_job_write_bucket_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, U64P]),
+    "cdbg_gpu_job_from_bucket_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32,
+                                                 C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "cdbg_gpu_job_write_phase2_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p,
+                                                  C.POINTER(ColorState)]),
+    "cdbg_load_fasta": (C.c_void_p, [C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32]),
+    "cdbg_load_fasta_list": (C.c_void_p, [C.c_char_p, C.c_uint32]),
+    "cdbg_seqs_total_bytes": (C.c_uint64, [C.c_void_p]),
+    "cdbg_seqs_num_colors": (C.c_uint32, [C.c_void_p]),
+    "cdbg_seqs_bytes": (U8P, [C.c_void_p]),
+    "cdbg_seqs_offsets": (U64P, [C.c_void_p]),
+    "cdbg_seqs_free": (None, [C.c_void_p]),
+    "cdbg_lz4_compress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
+    "cdbg_lz4_decompress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
     "cdbg_synth_create": (C.c_void_p, [C.POINTER(SynthParams), C.c_uint32]),
     "cdbg_synth_total_bytes": (C.c_uint64, [C.c_void_p]),
     "cdbg_synth_bytes": (U8P, [C.c_void_p]),
@@ -108,6 +137,46 @@ def synth(seed, num_colors, genome_len, snp_rate=0.01, indel_rate=0.001, indel_m
     return data, off
 
 
+def _cstrs(items):
+    arr = (C.c_char_p * len(items))(*[os.fsencode(x) for x in items])
+    return arr
+
+
+def load_fasta(files, threads=8):
+    """Read FASTA/FASTQ files (kseq semantics); returns (bytes, color_off)."""
+    h = lib().cdbg_load_fasta(_cstrs(files), len(files), threads)
+    if not h:
+        raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+    try:
+        n = lib().cdbg_seqs_total_bytes(h)
+        nc = lib().cdbg_seqs_num_colors(h)
+        data = (np.ctypeslib.as_array(lib().cdbg_seqs_bytes(h), shape=(n,)).copy() if n
+                else np.zeros(0, np.uint8))
+        off = np.ctypeslib.as_array(lib().cdbg_seqs_offsets(h), shape=(nc + 1,)).copy()
+    finally:
+        lib().cdbg_seqs_free(h)
+    return data, off
+
+
+def lz4_compress(src):
+    src = np.ascontiguousarray(np.frombuffer(bytes(src), np.uint8))
+    cap = len(src) + len(src) // 255 + 64
+    out = np.zeros(cap, np.uint8)
+    n = lib().cdbg_lz4_compress(_ptr(src, U8P), len(src), _ptr(out, U8P), cap)
+    if n < 0:
+        raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+    return out[:n].tobytes()
+
+
+def lz4_decompress(src, size):
+    src = np.ascontiguousarray(np.frombuffer(bytes(src), np.uint8))
+    out = np.zeros(max(size, 1), np.uint8)
+    n = lib().cdbg_lz4_decompress(_ptr(src, U8P), len(src), _ptr(out, U8P), size)
+    if n < 0:
+        raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+    return out[:n].tobytes()
+
+
 class Context:
     def __init__(self, device=0):
         self.h = C.c_void_p()
@@ -126,6 +195,27 @@ class Context:
 
     def job(self, data, color_off, k, m, num_buckets):
         return Job(self, data, color_off, k, m, num_buckets)
+
+    def job_from_bucket_files(self, d, num_buckets, k, num_colors, threads=8):
+        j = Job.__new__(Job)
+        j.num_buckets = num_buckets
+        j.h = C.c_void_p()
+        _check(lib().cdbg_gpu_job_from_bucket_files(self.h, os.fsencode(d), num_buckets, k, num_colors,
+                                                    threads, C.byref(j.h)))
+        return j
+
+    def bucket_write(self, files, k, m, num_buckets, tmp_dir, threads=8):
+        unc = np.zeros(num_buckets, np.uint64)
+        _check(lib().cdbg_gpu_bucket_write(self.h, _cstrs(files), len(files), k, m, num_buckets,
+                                           os.fsencode(tmp_dir), threads, _ptr(unc, U64P)))
+        return unc
+
+    def bucket_process(self, tmp_dir, num_buckets, k, num_colors, frag_path, cs_path, threads=8):
+        st = ColorState()
+        _check(lib().cdbg_gpu_bucket_process(self.h, os.fsencode(tmp_dir), num_buckets, k, num_colors,
+                                             threads, os.fsencode(frag_path), os.fsencode(cs_path),
+                                             C.byref(st)))
+        return st.as_dict()
 
 
 class Job:
@@ -172,6 +262,17 @@ class Job:
         _check(lib().cdbg_gpu_job_supers(self.h, which, _ptr(start, U64P), _ptr(ln, U32P),
                                          _ptr(fl, U8P), _ptr(bk, U32P), _ptr(col, U32P)))
         return {"start": start, "len": ln, "flags": fl, "bucket": bk, "color": col}
+
+    def write_bucket_files(self, d, threads=8):
+        unc = np.zeros(self.num_buckets, np.uint64)
+        _check(lib().cdbg_gpu_job_write_bucket_files(self.h, os.fsencode(d), threads, _ptr(unc, U64P)))
+        return unc
+
+    def write_phase2_files(self, frag_path, cs_path):
+        st = ColorState()
+        _check(lib().cdbg_gpu_job_write_phase2_files(self.h, os.fsencode(frag_path), os.fsencode(cs_path),
+                                                     C.byref(st)))
+        return st.as_dict()
 
     def set_chunk_tuples(self, n):
         _check(lib().cdbg_gpu_job_set_chunk_tuples(self.h, n))

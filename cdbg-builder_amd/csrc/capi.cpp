@@ -109,6 +109,7 @@ CDBG_API void cdbg_gpu_job_destroy(cdbg_gpu_job* job) { delete job; }
 CDBG_API int cdbg_gpu_job_phase1(cdbg_gpu_job* job) {
     CDBG_TRY({
         if (!job) throw Error("null job");
+        if (job->J.from_files) throw Error("this job was loaded from bucket files: it has no sequences for phase 1");
         CDBG_HIP(hipSetDevice(job->J.ctx->device));
         job->J.phase2_done = false;
         g_prof = &job->J.prof;
@@ -131,6 +132,7 @@ CDBG_API int cdbg_gpu_job_phase2(cdbg_gpu_job* job) {
 }
 
 CDBG_API int cdbg_gpu_job_run(cdbg_gpu_job* job) {
+    if (job && job->J.from_files) return cdbg_gpu_job_phase2(job);
     int rc = cdbg_gpu_job_phase1(job);
     if (rc) return rc;
     return cdbg_gpu_job_phase2(job);
@@ -190,6 +192,73 @@ CDBG_API int64_t cdbg_gpu_job_kernel_times(cdbg_gpu_job* job, char* buf, uint64_
         buf[n] = 0;
     }
     return int64_t(txt.size());
+}
+
+CDBG_API int cdbg_gpu_job_write_bucket_files(cdbg_gpu_job* job, const char* dir, uint32_t threads,
+                                             uint64_t* bucket_unc_bytes) {
+    CDBG_TRY({
+        if (!job || !dir) throw Error("null argument");
+        CDBG_HIP(hipSetDevice(job->J.ctx->device));
+        write_bucket_files(job->J, dir, threads, bucket_unc_bytes);
+    })
+}
+
+CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, uint32_t num_buckets, uint32_t k,
+                                            uint32_t num_colors, uint32_t threads, cdbg_gpu_job** out) {
+    CDBG_TRY({
+        if (!ctx || !dir || !out) throw Error("null argument");
+        if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
+        if (num_buckets == 0 || num_colors == 0) throw Error("num_buckets and num_colors must be >= 1");
+        CDBG_HIP(hipSetDevice(ctx->device));
+        auto* h = new cdbg_gpu_job();
+        try {
+            Job& J = h->J;
+            J.ctx = ctx;
+            J.k = k;
+            J.B = num_buckets;
+            J.num_colors = num_colors;
+            CDBG_HIP(hipStreamCreateWithFlags(&J.st, hipStreamNonBlocking));
+            load_bucket_files(J, dir, threads);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    })
+}
+
+CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag_path, const char* color_sets_path,
+                                             cdbg_gpu_color_state* state) {
+    CDBG_TRY({
+        if (!job || !frag_path || !color_sets_path) throw Error("null argument");
+        CDBG_HIP(hipSetDevice(job->J.ctx->device));
+        write_phase2_files(job->J, frag_path, color_sets_path, state);
+    })
+}
+
+CDBG_API int cdbg_gpu_bucket_write(cdbg_gpu_ctx* ctx, const char* const* files, uint32_t n_files, uint32_t k,
+                                   uint32_t m, uint32_t num_buckets, const char* tmp_dir, uint32_t threads,
+                                   uint64_t* bucket_unc_bytes) {
+    void* seqs = cdbg_load_fasta(files, n_files, threads);
+    if (!seqs) return -1;
+    cdbg_gpu_job* job = nullptr;
+    int rc = cdbg_gpu_job_create(ctx, cdbg_seqs_bytes(seqs), cdbg_seqs_offsets(seqs), n_files, k, m, num_buckets, &job);
+    cdbg_seqs_free(seqs);
+    if (rc == 0) rc = cdbg_gpu_job_phase1(job);
+    if (rc == 0) rc = cdbg_gpu_job_write_bucket_files(job, tmp_dir, threads, bucket_unc_bytes);
+    if (job) cdbg_gpu_job_destroy(job);
+    return rc;
+}
+
+CDBG_API int cdbg_gpu_bucket_process(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
+                                     uint32_t num_colors, uint32_t threads, const char* frag_path,
+                                     const char* color_sets_path, cdbg_gpu_color_state* state) {
+    cdbg_gpu_job* job = nullptr;
+    int rc = cdbg_gpu_job_from_bucket_files(ctx, tmp_dir, num_buckets, k, num_colors, threads, &job);
+    if (rc == 0) rc = cdbg_gpu_job_phase2(job);
+    if (rc == 0) rc = cdbg_gpu_job_write_phase2_files(job, frag_path, color_sets_path, state);
+    if (job) cdbg_gpu_job_destroy(job);
+    return rc;
 }
 
 CDBG_API uint64_t cdbg_gpu_device_bytes(void) { return g_device_bytes.load(); }

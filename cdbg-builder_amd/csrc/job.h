@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "cdbg_gpu.h"
@@ -21,6 +22,7 @@ struct Phase1Bufs {
     DevBuf key, val, key_t, val_t;     // bucket sort
     DevBuf b_start, b_len, b_flags, b_color;                // supers, bucket-major
     DevBuf bucket_off;                 // B+1
+    DevBuf rank0;                      // per record: stream rank of its first k-mer (bucket files only)
 };
 
 struct Phase2Bufs;  // defined in phase2.hip
@@ -36,6 +38,7 @@ struct Job {
     RadixScratch rs;
     Phase2Bufs* p2 = nullptr;
     bool phase1_done = false, phase2_done = false;
+    bool from_files = false;  // phase-1 records were read from bucket files
     uint64_t n_supers = 0;
     std::vector<uint64_t> h_bucket_off;  // host copy (B+1)
     cdbg_gpu_stats stats{};
@@ -49,6 +52,13 @@ struct Job {
 void run_phase1(Job& J);
 void run_phase2(Job& J);
 void free_phase2(Job& J);
+// Host copies of the phase-2 results (any pointer may be null).
+void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offsets);
+// bucket_io.hip
+void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64_t* unc_bytes);
+void load_bucket_files(Job& J, const std::string& dir, uint32_t threads);
+void write_phase2_files(Job& J, const std::string& frag_path, const std::string& cs_path,
+                        cdbg_gpu_color_state* state);
 
 }  // namespace cdbg_amd
 

@@ -171,6 +171,7 @@ struct ExpArgs {
     const uint32_t* color;
     const uint32_t* packed;
     uint32_t k;
+    const uint32_t* rank0;  // per record: stream rank of its first k-mer (null: records are the stream)
 };
 
 template <class K>
@@ -229,7 +230,7 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, K* __restrict__ keys,
     if (!(fl & 1u) && ix == 0) contrib |= fwd ? BOUND_LEFT : BOUND_RIGHT;
     if (!(fl & 2u) && ix == nk - 1) contrib |= fwd ? BOUND_RIGHT : BOUND_LEFT;
     const bool owned = !((ix == 0 && !(fl & 4u)) || (ix == nk - 1 && !(fl & 8u)));
-    const uint32_t rank = uint32_t(t - a.btup[a.bucket[s]]);
+    const uint32_t rank = a.rank0 ? a.rank0[s] + ix : uint32_t(t - a.btup[a.bucket[s]]);
     keys[t - a.t0] = fwd ? f : r;
     vals[t - a.t0] = (uint64_t(rank) << 32) | (uint64_t(a.color[s]) << 3) | (uint64_t(owned) << 2) | contrib;
 }
@@ -1038,7 +1039,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint64_t* vals_t = P.vals_t.ensure<uint64_t>(nt);
     ExpArgs ea{nt, t0, s0, s1, P.tup_off.as<uint64_t>(), P.btup.as<uint64_t>(), Q.key.as<uint32_t>(),
                Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(),
-               Q.packed.as<uint32_t>(), k};
+               Q.packed.as<uint32_t>(), k, J.from_files ? Q.rank0.as<uint32_t>() : nullptr};
     { KSCOPE("k_expand", st); k_expand<K><<<blocks_for(nt), TPB, 0, st>>>(ea, keys, vals); }
 
     // --- sort tuples by k-mer inside every bucket
@@ -1310,6 +1311,15 @@ void free_phase2(Job& J) {
     J.p2 = nullptr;
 }
 
+void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offsets) {
+    if (!J.phase2_done) throw Error("phase 2 has not run");
+    Phase2Bufs& P = *J.p2;
+    if (frags && P.frag_total) CDBG_HIP(hipMemcpy(frags, P.frag_bytes.p, P.frag_total, hipMemcpyDeviceToHost));
+    const uint64_t nw = (P.cs_bits + 63) / 64;
+    if (words && nw) CDBG_HIP(hipMemcpy(words, P.cs_words.p, nw * 8, hipMemcpyDeviceToHost));
+    if (offsets && P.n_sets) CDBG_HIP(hipMemcpy(offsets, P.set_off.p, P.n_sets * 8, hipMemcpyDeviceToHost));
+}
+
 }  // namespace cdbg_amd
 
 using namespace cdbg_amd;
@@ -1318,10 +1328,7 @@ extern "C" {
 
 CDBG_API int cdbg_gpu_job_frags(cdbg_gpu_job* job, uint8_t* dst) {
     try {
-        Job& J = job->J;
-        if (!J.phase2_done) throw Error("phase 2 has not run");
-        if (J.p2->frag_total)
-            CDBG_HIP(hipMemcpy(dst, J.p2->frag_bytes.p, J.p2->frag_total, hipMemcpyDeviceToHost));
+        copy_phase2_results(job->J, dst, nullptr, nullptr);
         return 0;
     } catch (std::exception& e) {
         return set_error(e.what());
@@ -1330,11 +1337,7 @@ CDBG_API int cdbg_gpu_job_frags(cdbg_gpu_job* job, uint8_t* dst) {
 
 CDBG_API int cdbg_gpu_job_color_sets(cdbg_gpu_job* job, uint64_t* words, uint64_t* offsets) {
     try {
-        Job& J = job->J;
-        if (!J.phase2_done) throw Error("phase 2 has not run");
-        const uint64_t nw = (J.p2->cs_bits + 63) / 64;
-        if (nw) CDBG_HIP(hipMemcpy(words, J.p2->cs_words.p, nw * 8, hipMemcpyDeviceToHost));
-        if (J.p2->n_sets) CDBG_HIP(hipMemcpy(offsets, J.p2->set_off.p, J.p2->n_sets * 8, hipMemcpyDeviceToHost));
+        copy_phase2_results(job->J, nullptr, words, offsets);
         return 0;
     } catch (std::exception& e) {
         return set_error(e.what());

@@ -125,6 +125,61 @@ CDBG_API int64_t cdbg_gpu_job_num_chunks(cdbg_gpu_job* job);
 CDBG_API int cdbg_gpu_job_frags(cdbg_gpu_job* job, uint8_t* dst);
 CDBG_API int cdbg_gpu_job_color_sets(cdbg_gpu_job* job, uint64_t* words, uint64_t* offsets);
 
+/* State of the colour-set stream at the end of phase 2: what the host's
+ * finalize step (streaming_color_set_dict.hpp:221-294) needs besides the
+ * files: the trailing partial word (num_bits % 64 bits) and the totals. */
+typedef struct {
+    uint32_t num_colors, sparse_threshold, dense_threshold;
+    uint64_t num_color_sets;
+    uint64_t num_bits;
+    uint64_t total_integers;
+    uint64_t trailing_word;
+    uint64_t num_kmers;
+    uint64_t num_fragments;
+} cdbg_gpu_color_state;
+
+/* Drop-in phase 1: reads the FASTA/FASTQ files (colour = index in `files`),
+ * runs phase 1 on the device and writes <tmp_dir>/bucket_<b>.bin for every
+ * bucket (bucket_io.hpp:19-40, records super_kmer.hpp:105-117).
+ * bucket_unc_bytes (num_buckets entries, may be NULL) receives each bucket's
+ * uncompressed size, as bucket_writer::close records it (bucket_io.hpp:389-415).
+ * Replaces ingest_bucketed (bucket_ingester.hpp:183-215). */
+CDBG_API int cdbg_gpu_bucket_write(cdbg_gpu_ctx* ctx, const char* const* files, uint32_t n_files, uint32_t k,
+                                   uint32_t m, uint32_t num_buckets, const char* tmp_dir, uint32_t threads,
+                                   uint64_t* bucket_unc_bytes);
+/* Drop-in phase 2: reads <tmp_dir>/bucket_<b>.bin, runs phase 2 on the device
+ * and writes frag_path (frag_unitigs.bin), color_sets_path (28-byte zero
+ * header + complete words) and color_sets_path.tmp_offsets.  Replaces
+ * process_buckets (bucket_walker.hpp:607-779) with a frag_unitig_writer sink
+ * and a streaming_color_set_dict. */
+CDBG_API int cdbg_gpu_bucket_process(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
+                                     uint32_t num_colors, uint32_t threads, const char* frag_path,
+                                     const char* color_sets_path, cdbg_gpu_color_state* state);
+
+/* Job-level pieces of the two calls above. */
+CDBG_API int cdbg_gpu_job_write_bucket_files(cdbg_gpu_job* job, const char* dir, uint32_t threads,
+                                             uint64_t* bucket_unc_bytes);
+CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, uint32_t num_buckets, uint32_t k,
+                                            uint32_t num_colors, uint32_t threads, cdbg_gpu_job** out);
+CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag_path, const char* color_sets_path,
+                                             cdbg_gpu_color_state* state);
+
+/* Host input: FASTA/FASTQ (plain or gzip) with kseq record semantics
+ * (external/kseq.h:161-201).  Returns a handle holding the job layout
+ * (bytes + colour offsets) or NULL. */
+CDBG_API void* cdbg_load_fasta(const char* const* files, uint32_t n_files, uint32_t threads);
+CDBG_API void* cdbg_load_fasta_list(const char* list_path, uint32_t threads);
+CDBG_API uint64_t cdbg_seqs_total_bytes(void* h);
+CDBG_API uint32_t cdbg_seqs_num_colors(void* h);
+CDBG_API const uint8_t* cdbg_seqs_bytes(void* h);
+CDBG_API const uint64_t* cdbg_seqs_offsets(void* h);
+CDBG_API void cdbg_seqs_free(void* h);
+
+/* LZ4 block codec of the bucket-file frames (public block format).  Return
+ * the output length, or a negative status. */
+CDBG_API int64_t cdbg_lz4_compress(const uint8_t* src, uint64_t n, uint8_t* dst, uint64_t cap);
+CDBG_API int64_t cdbg_lz4_decompress(const uint8_t* src, uint64_t n, uint8_t* dst, uint64_t cap);
+
 /* Synthetic pangenome generator (seeded std::mt19937_64, explicit modulo). */
 typedef struct {
     uint64_t seed;

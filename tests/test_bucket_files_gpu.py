@@ -1,0 +1,212 @@
+"""The drop-in file boundary on the GPU.
+
+* cdbg_gpu_bucket_write: FASTA files -> bucket_<b>.bin.  Files are decoded
+  with an independent reader (system liblz4 when present + a varint parser
+  written from super_kmer.hpp:121-143) and must hold exactly the oracle's
+  super-k-mer records, bucket by bucket, in stream order.
+* cdbg_gpu_bucket_process: bucket files -> frag_unitigs.bin, .color_sets,
+  .color_sets.tmp_offsets, byte-identical to the oracle's phase 2 over the
+  same files.  Also on bucket files whose records carry several colours and
+  arrive in another order (what the reference's compacting writer produces).
+"""
+import ctypes
+import os
+import struct
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import cdbg_gpu as G
+from test_host_io import SYS
+
+pytestmark = pytest.mark.gpu
+ACGT = b"ACGT"
+
+
+def _varint(buf, pos):
+    v = sh = 0
+    while True:
+        b = buf[pos]
+        pos += 1
+        v |= (b & 0x7F) << sh
+        if not b & 0x80:
+            return v, pos
+        sh += 7
+
+
+def _frames(path):
+    raw = open(path, "rb").read()
+    out, p = bytearray(), 0
+    while p + 4 <= len(raw):
+        u, = struct.unpack_from("<I", raw, p)
+        p += 4
+        if u == 0:
+            break
+        c, = struct.unpack_from("<I", raw, p)
+        p += 4
+        blk = raw[p:p + c]
+        p += c
+        if SYS is not None:
+            buf = ctypes.create_string_buffer(u)
+            assert SYS.LZ4_decompress_safe(blk, buf, c, u) == u
+            out += buf.raw
+        else:
+            out += G.lz4_decompress(blk, u)
+    return bytes(out)
+
+
+def _records(path):
+    d, p, recs = _frames(path), 0, []
+    while p < len(d):
+        nc, p = _varint(d, p)
+        cols, prev = [], 0
+        for _ in range(nc):
+            x, p = _varint(d, p)
+            prev += x
+            cols.append(prev)
+        lf, p = _varint(d, p)
+        ln, fl = lf >> 4, lf & 15
+        nb = (ln + 3) // 4
+        bases = bytes(ACGT[(d[p + i // 4] >> (2 * (i % 4))) & 3] for i in range(ln))
+        assert all((d[p + nb - 1] >> (2 * j)) & 3 == 0 for j in range(ln - 4 * (nb - 1), 4)), "padding bits"
+        p += nb
+        recs.append((tuple(cols), fl, bases))
+    return recs
+
+
+def _fasta_job(tmp_path, colors=5, glen=20000, seed=6):
+    data, off = G.synth(seed=seed, num_colors=colors, genome_len=glen, snp_rate=0.01, indel_rate=0.001)
+    paths = []
+    for c in range(colors):
+        seq = bytes(data[off[c]:off[c + 1] - 1])
+        p = str(tmp_path / f"c{c}.fa")
+        with open(p, "wb") as f:
+            f.write(b">g%d\n" % c + b"\n".join(seq[i:i + 70] for i in range(0, len(seq), 70)) + b"\n")
+        paths.append(p)
+    return paths
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 19, 64), (63, 27, 32), (21, 11, 7)])
+def test_bucket_write_records(gpu_ctx, tmp_path, k, m, B):
+    files = _fasta_job(tmp_path)
+    d = str(tmp_path / "buckets")
+    unc = gpu_ctx.bucket_write(files, k, m, B, d)
+    data, off = G.load_fasta(files)
+    ob = O.Build(data, off, k, m, B)
+    sup = ob.supers(1)
+    boff = np.concatenate([[0], np.cumsum(np.bincount(sup["bucket"], minlength=B))])
+    for b in range(B):
+        recs = _records(os.path.join(d, f"bucket_{b}.bin"))
+        want = []
+        for i in range(boff[b], boff[b + 1]):
+            s, ln = int(sup["start"][i]), int(sup["len"][i])
+            want.append(((int(sup["color"][i]),), int(sup["flags"][i]), bytes(data[s:s + ln]).upper()))
+        assert recs == want, f"bucket {b}"
+        payload = len(_frames(os.path.join(d, f"bucket_{b}.bin")))
+        assert unc[b] == payload
+
+
+def _compare_phase2(gpu_ctx, tmp_path, d, B, k, num_colors):
+    frag, cs = str(tmp_path / "frag_unitigs.bin"), str(tmp_path / "out.color_sets")
+    st = gpu_ctx.bucket_process(d, B, k, num_colors, frag, cs)
+    ob = O.Build.from_bucket_files(d, B, k, num_colors)
+    assert open(frag, "rb").read() == ob.frags().tobytes()
+    words, offs = ob.color_sets()
+    full = ob.stat("cs_bits") // 64
+    assert open(cs, "rb").read() == bytes(28) + words[:full].tobytes()
+    assert open(cs + ".tmp_offsets", "rb").read() == offs.tobytes()
+    assert st["num_color_sets"] == ob.stat("num_sets")
+    assert st["num_bits"] == ob.stat("cs_bits")
+    assert st["num_kmers"] == ob.stat("num_kmers")
+    assert st["trailing_word"] == (int(words[full]) if ob.stat("cs_bits") % 64 else 0)
+    return st
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 19, 64), (63, 27, 32)])
+def test_bucket_process_matches_oracle(gpu_ctx, tmp_path, k, m, B):
+    files = _fasta_job(tmp_path, colors=6)
+    d = str(tmp_path / "buckets")
+    gpu_ctx.bucket_write(files, k, m, B, d)
+    _compare_phase2(gpu_ctx, tmp_path, d, B, k, len(files))
+
+
+def test_file_path_equals_resident_path(gpu_ctx, tmp_path):
+    files = _fasta_job(tmp_path, colors=4)
+    data, off = G.load_fasta(files)
+    with gpu_ctx.job(data, off, 31, 19, 64) as job:
+        job.phase1()
+        d = str(tmp_path / "b")
+        job.write_bucket_files(d)
+        job.phase2()
+        resident = job.frags()
+        rw, ro = job.color_sets()
+    with gpu_ctx.job_from_bucket_files(d, 64, 31, 4) as fj:
+        fj.run()
+        assert np.array_equal(fj.frags(), resident)
+        fw, fo = fj.color_sets()
+        assert np.array_equal(fw, rw) and np.array_equal(fo, ro)
+
+
+def _literal_block(b):
+    n = len(b)
+    tok = bytearray([min(n, 15) << 4])
+    if n >= 15:
+        r = n - 15
+        while r >= 255:
+            tok.append(255)
+            r -= 255
+        tok.append(r)
+    return bytes(tok) + b
+
+
+def _enc_varint(v):
+    out = bytearray()
+    while v >= 0x80:
+        out.append((v & 0x7F) | 0x80)
+        v >>= 7
+    out.append(v)
+    return bytes(out)
+
+
+def _write_bucket(path, recs, frame=700):
+    stream = bytearray()
+    for cols, fl, bases in recs:
+        stream += _enc_varint(len(cols))
+        prev = 0
+        for c in cols:
+            stream += _enc_varint(c - prev)
+            prev = c
+        stream += _enc_varint((len(bases) << 4) | fl)
+        packed = bytearray((len(bases) + 3) // 4)
+        for i, ch in enumerate(bases):
+            packed[i // 4] |= ACGT.index(ch) << (2 * (i % 4))
+        stream += packed
+    with open(path, "wb") as f:
+        for o in range(0, len(stream), frame):
+            blk = bytes(stream[o:o + frame])
+            lit = _literal_block(blk)
+            f.write(struct.pack("<II", len(blk), len(lit)) + lit)
+        f.write(struct.pack("<I", 0))
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 19, 16), (21, 11, 8)])
+def test_multicolour_records(gpu_ctx, tmp_path, k, m, B):
+    """Records merged across colours (sorted unique colour lists), shuffled
+    within each bucket, split over literal-only LZ4 frames."""
+    files = _fasta_job(tmp_path, colors=7, glen=8000, seed=12)
+    data, off = G.load_fasta(files)
+    sup = O.Build(data, off, k, m, B).supers(1)
+    rng = np.random.default_rng(3)
+    d = tmp_path / "merged"
+    d.mkdir()
+    for b in range(B):
+        groups = {}
+        for i in np.nonzero(sup["bucket"] == b)[0]:
+            s, ln = int(sup["start"][i]), int(sup["len"][i])
+            key = (bytes(data[s:s + ln]).upper(), int(sup["flags"][i]))
+            groups.setdefault(key, set()).add(int(sup["color"][i]))
+        recs = [(tuple(sorted(cols)), fl, bases) for (bases, fl), cols in groups.items()]
+        rng.shuffle(recs)
+        _write_bucket(str(d / f"bucket_{b}.bin"), recs)
+    _compare_phase2(gpu_ctx, tmp_path, str(d), B, k, len(files))
