@@ -5,11 +5,12 @@ import sys
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = os.path.join(ROOT, "cdbg-builder_amd")
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
 
 def _load_pkg():
-    path = os.path.join(ROOT, "cdbg-builder_amd", "cdbg_gpu.py")
+    path = os.path.join(PKG_DIR, "cdbg_gpu.py")
     spec = importlib.util.spec_from_file_location("cdbg_gpu", path)
     mod = importlib.util.module_from_spec(spec)
     sys.modules["cdbg_gpu"] = mod

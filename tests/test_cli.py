@@ -1,0 +1,91 @@
+"""The cdbg-build host driver (command line of src/main.cpp:13-74, configuration
+rules of builder.hpp:550-649).
+
+CPU: argument and configuration errors exit with status 1 before any device
+work.  GPU: a run over FASTA files leaves the same phase-2 artifacts as the
+oracle's phase 2 over the bucket files the run kept (--keep-tmp), and the
+resident mode (no bucket files) leaves identical artifacts.
+"""
+import os
+import subprocess
+
+import pytest
+
+import oracle as O
+from conftest import PKG_DIR, cdbg_gpu as G
+
+CLI = os.path.join(PKG_DIR, "bin", "cdbg-build")
+needs_cli = pytest.mark.skipif(not os.path.exists(CLI), reason="cdbg-build not built")
+
+
+def _run(*args):
+    return subprocess.run([CLI, *map(str, args)], capture_output=True, text=True, timeout=600)
+
+
+def _list(tmp_path, colors=3, glen=15000, seed=9):
+    data, off = G.synth(seed=seed, num_colors=colors, genome_len=glen, snp_rate=0.01, indel_rate=0.001)
+    paths = []
+    for c in range(colors):
+        p = tmp_path / f"g{c}.fa"
+        seq = bytes(data[off[c]:off[c + 1] - 1])
+        p.write_bytes(b">g\n" + b"\n".join(seq[i:i + 80] for i in range(0, len(seq), 80)) + b"\n")
+        paths.append(str(p))
+    lst = tmp_path / "list.txt"
+    lst.write_text("\n".join(paths) + "\n")
+    return str(lst), paths
+
+
+@needs_cli
+@pytest.mark.parametrize("args,msg", [
+    ((), "usage"),
+    (("-i", "x", "-o", "y"), "usage"),
+    (("-i", "x", "-o", "y", "-k", "64"), "k must satisfy"),
+    (("-i", "x", "-o", "y", "-k", "31", "-m", "31"), "invalid m"),
+    (("-i", "x", "-o", "y", "-k", "31", "-b", "17"), "-b must be in"),
+    (("-i", "x", "-o", "y", "-k", "31", "-g", "2"), "-g must be at least"),
+    (("-i", "x", "-o", "y", "-k", "31", "-g", "4", "-t", "10000"), "Reduce -t"),
+    (("-i", "/nonexistent/list", "-o", "y", "-k", "31"), "cannot open filenames list"),
+    (("-i", "x", "-o", "y", "-k", "31", "--bogus"), "unknown option"),
+])
+def test_cli_rejects(args, msg):
+    r = _run(*args)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+@needs_cli
+def test_cli_rejects_nonempty_tmp_dir(tmp_path):
+    lst, _ = _list(tmp_path, colors=1, glen=200)
+    d = tmp_path / "scratch"
+    d.mkdir()
+    (d / "junk").write_text("x")
+    r = _run("-i", lst, "-o", tmp_path / "out", "-k", 31, "-d", d)
+    assert r.returncode == 1 and "is not empty" in r.stderr
+
+
+@pytest.mark.gpu
+@needs_cli
+@pytest.mark.parametrize("k,m,b", [(31, 19, 6), (63, 27, 5)])
+def test_cli_end_to_end(tmp_path, k, m, b):
+    lst, paths = _list(tmp_path)
+    d = tmp_path / "scratch"
+    out = tmp_path / "out"
+    r = _run("-i", lst, "-o", out, "-k", k, "-m", m, "-b", b, "-d", d, "-t", 4, "--keep-tmp")
+    assert r.returncode == 0, r.stderr
+    B = 1 << b
+    ob = O.Build.from_bucket_files(str(d), B, k, len(paths))
+    assert (d / "frag_unitigs.bin").read_bytes() == ob.frags().tobytes()
+    words, offs = ob.color_sets()
+    full = ob.stat("cs_bits") // 64
+    assert (tmp_path / "out.color_sets").read_bytes() == bytes(28) + words[:full].tobytes()
+    assert (tmp_path / "out.color_sets.tmp_offsets").read_bytes() == offs.tobytes()
+    meta = dict(line.split("=", 1) for line in (tmp_path / "out.phase2.txt").read_text().split())
+    assert int(meta["num_kmers"]) == ob.stat("num_kmers")
+    assert int(meta["num_color_sets"]) == ob.stat("num_sets")
+
+    d2, out2 = tmp_path / "scratch2", tmp_path / "res"
+    r = _run("-i", lst, "-o", out2, "-k", k, "-m", m, "-b", b, "-d", d2, "--resident")
+    assert r.returncode == 0, r.stderr
+    assert (d2 / "frag_unitigs.bin").read_bytes() == (d / "frag_unitigs.bin").read_bytes()
+    assert (tmp_path / "res.color_sets").read_bytes() == (tmp_path / "out.color_sets").read_bytes()
+    assert (tmp_path / "res.color_sets.tmp_offsets").read_bytes() == offs.tobytes()
