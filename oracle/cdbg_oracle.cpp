@@ -20,11 +20,19 @@
 //                                   compiled into oracle/_ref, plus brute force.
 //   super-k-mer split + flags     : pinned by brute force from the definitions
 //                                   and by invariants (tests/test_oracle_phase1.py).
-//   per-bucket load/resolve/walk  : pinned by the ground-truth ccdBG definition of
-//                                   test_data/verify.py:103-220 through a host
-//                                   stitch on small inputs (tests/test_oracle_phase2.py).
-//   Elias-delta bit order         : parity unpinned (jermp/bits is absent; the
-//                                   layout follows its published write_delta).
+//   per-bucket load/resolve/walk, : pinned by the ground-truth ccdBG definition of
+//   colour-set dictionary,          test_data/verify.py:103-220 through a test-side
+//   fragment records                stitch + monochromatic split on small inputs, and
+//                                   by the salmonella_10 known answers (86,630
+//                                   unitigs, 171 colour sets; algorithm.md:36-39)
+//                                   (tests/test_oracle_phase2.py).
+//   hybrid colour-set code        : decoded by a reader written from the reference's
+//                                   own decoder (stream.hpp:196-237); the Elias-delta
+//                                   bit order inside it is parity unpinned (jermp/bits
+//                                   is absent; the layout follows its write_delta).
+//   fragment decomposition order  : (which seed starts which fragment) follows
+//                                   bucket_walker.hpp:563-566 at -t 1; pinned only
+//                                   through the final unitig set above.
 #include <zlib.h>
 
 #include <algorithm>
