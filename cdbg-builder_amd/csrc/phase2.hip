@@ -67,19 +67,29 @@ __device__ __forceinline__ uint32_t v_color(uint64_t v) { return uint32_t(v >> 3
 __device__ __forceinline__ bool v_owned(uint64_t v) { return (v >> 2) & 1u; }
 __device__ __forceinline__ uint32_t v_contrib(uint64_t v) { return uint32_t(v) & 3u; }
 
+// Per-bucket k-mer table: open addressing over groups of 64 bytes (8 u64 or
+// 4 128-bit keys), probed a group at a time; the node of slot i is node_of[i].
+// An empty slot holds all-ones (no k-mer of 2k <= 126 bits has it).
 template <class K>
-struct alignas(16) Slot;
+struct Tab;
 template <>
-struct alignas(16) Slot<uint64_t> {
-    uint64_t key;
-    uint32_t node;
-    uint32_t pad;
+struct Tab<uint64_t> {
+    static constexpr uint32_t G = 8;
+    __device__ static bool empty(const uint64_t& x) { return x == ~0ull; }
+    __device__ static bool claim(uint64_t* slot, const uint64_t& key) {
+        return atomicCAS((unsigned long long*)slot, ~0ull, (unsigned long long)key) == ~0ull;
+    }
 };
 template <>
-struct alignas(16) Slot<Key128> {
-    Key128 key;
-    uint32_t node;
-    uint32_t pad[3];
+struct Tab<Key128> {
+    static constexpr uint32_t G = 4;
+    __device__ static bool empty(const Key128& x) { return x.hi == ~0ull; }
+    // hi is claimed atomically; lo is read only by later kernels
+    __device__ static bool claim(Key128* slot, const Key128& key) {
+        if (atomicCAS((unsigned long long*)&slot->hi, ~0ull, (unsigned long long)key.hi) != ~0ull) return false;
+        slot->lo = key.lo;
+        return true;
+    }
 };
 
 // last b in [0, B) with off[b] <= i (off has B+1 entries, off[B] > i)
@@ -93,6 +103,20 @@ __device__ __forceinline__ uint32_t bucket_of(const uint64_t* off, uint32_t B, u
             hi = mid;
     }
     return lo;
+}
+
+// node -> local bucket, by binary search over node_off staged in LDS
+constexpr uint32_t NB_LDS = 4096;
+__global__ void k_node_bucket(uint32_t n_nodes, const uint64_t* __restrict__ node_off, uint32_t nb,
+                              uint32_t* __restrict__ node_bucket) {
+    __shared__ uint64_t s_off[NB_LDS + 1];
+    const bool staged = nb <= NB_LDS;
+    if (staged)
+        for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_off[i] = node_off[i];
+    __syncthreads();
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    node_bucket[n] = staged ? bucket_of(s_off, nb, n) : bucket_of(node_off, nb, n);
 }
 
 __global__ void k_iota(uint32_t* p, uint64_t n) {
@@ -265,19 +289,65 @@ __global__ void k_node_off(uint32_t nb, const uint64_t* __restrict__ lbtup, uint
     node_off[b] = t < n_tuples ? hscan[t] : n_nodes;
 }
 
+// Nodes with more than BIG_RUN occurrences are reduced by a whole wave.
+constexpr uint32_t BIG_RUN = 32;
+// counters: 0 primary nodes, 1 multi-colour primary nodes, 2 total integers,
+// 3 nodes placed by the walk, 4 fragments, 5 walk errors, 6 colour-set
+// collisions, 7 colours of multi-colour nodes, 8 big nodes
+constexpr int N_COUNTERS = 16;
+
+// Colour-set hash: a sum of mixed terms over the sorted unique list, so a
+// wave can add its lanes' terms in any order and get the thread's value.
+__device__ __forceinline__ uint64_t set_term(uint32_t c) { return mix64(uint64_t(c) + 0x9E3779B97F4A7C15ull); }
+__device__ __forceinline__ uint64_t set_hash_fin(uint64_t sum, uint32_t n) {
+    return mix64(sum ^ (uint64_t(n) * 0xC2B2AE3D27D4EB4Full)) | 1ull;  // 0 marks an empty slot
+}
+
+// One thread per node (a run of equal keys [t0, t1) in the sorted tuples):
+// flags OR, seed rank = min, primary = any owned occurrence, the sorted unique
+// owned colours (written in place at colors[t0..]) and their set hash
+// (bucket_walker.hpp:228-249 merge of map entries).  A block takes the nodes
+// whose first occurrence lies in its tile of NT_TILE tuples and stages the
+// tile (plus the BIG_RUN tuples after it) in LDS; runs longer than BIG_RUN
+// are queued for k_nodes_big.
+constexpr uint32_t NT_TILE = 1024;
+
 template <class K>
-__global__ void k_node_reduce(uint32_t n_nodes, const uint32_t* __restrict__ node_start, const K* __restrict__ keys,
-                              const uint64_t* __restrict__ vals, K* __restrict__ node_key, uint8_t* __restrict__ node_meta,
-                              uint32_t* __restrict__ node_rank, uint32_t* __restrict__ node_ncol,
-                              unsigned long long* __restrict__ counters) {
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint32_t* __restrict__ heads,
+                                               const uint32_t* __restrict__ hscan,
+                                               const uint32_t* __restrict__ node_start, const K* __restrict__ keys,
+                                               const uint64_t* __restrict__ vals, K* __restrict__ node_key,
+                                               uint8_t* __restrict__ node_meta, uint32_t* __restrict__ node_rank,
+                                               uint32_t* __restrict__ node_ncol, uint64_t* __restrict__ node_hash,
+                                               uint32_t* __restrict__ colors, uint32_t* __restrict__ big,
+                                               unsigned long long* __restrict__ counters) {
+    __shared__ uint64_t s_val[NT_TILE + BIG_RUN];
+    __shared__ uint32_t s_list[NT_TILE];
+    __shared__ uint32_t s_n;
+    const uint64_t T0 = uint64_t(blockIdx.x) * NT_TILE;
+    const uint64_t T1 = min(T0 + NT_TILE, nt), TE = min(T1 + BIG_RUN, nt);
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < TE - T0; i += TPB) s_val[i] = vals[T0 + i];
+    for (uint32_t i = threadIdx.x; i < T1 - T0; i += TPB)
+        if (heads[T0 + i]) s_list[atomicAdd(&s_n, 1u)] = i;
+    __syncthreads();
     uint64_t prim = 0, multi = 0, multi_cols = 0;
-    if (n < n_nodes) {
-        const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
+    for (uint32_t j = threadIdx.x; j < s_n; j += TPB) {
+        const uint32_t h = s_list[j];
+        const uint32_t t0 = uint32_t(T0 + h);
+        const uint32_t n = hscan[t0];
+        const uint32_t t1 = node_start[n + 1];
+        node_key[n] = keys[t0];
+        if (t1 - t0 > BIG_RUN) {
+            big[atomicAdd(&counters[8], 1ull)] = n;
+            continue;
+        }
         uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
+        uint64_t sum = 0;
         bool primary = false;
         for (uint32_t t = t0; t < t1; ++t) {
-            const uint64_t v = vals[t];
+            const uint64_t v = s_val[t - T0];
             fl |= v_contrib(v);
             const uint32_t r = v_rank(v);
             rank = r < rank ? r : rank;
@@ -285,51 +355,109 @@ __global__ void k_node_reduce(uint32_t n_nodes, const uint32_t* __restrict__ nod
                 primary = true;
                 const uint32_t c = v_color(v);
                 if (c != last) {
+                    colors[t0 + ncol] = c;
+                    sum += set_term(c);
                     ++ncol;
                     last = c;
                 }
             }
         }
-        node_key[n] = keys[t0];
         node_meta[n] = uint8_t(fl | (primary ? 4u : 0u));
         node_rank[n] = rank;
         node_ncol[n] = ncol;
-        prim = primary;
-        multi = ncol >= 2;
-        multi_cols = ncol >= 2 ? ncol : 0;
+        node_hash[n] = set_hash_fin(sum, ncol);
+        prim += primary;
+        multi += ncol >= 2;
+        multi_cols += ncol >= 2 ? ncol : 0;
     }
     block_add(&counters[0], prim);
     block_add(&counters[1], multi);
     block_add(&counters[7], multi_cols);
 }
 
-__global__ void k_node_colors(uint32_t n_nodes, const uint32_t* __restrict__ node_start, const uint64_t* __restrict__ vals,
-                              const uint64_t* __restrict__ node_coff, uint32_t* __restrict__ colors) {
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
-    const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
-    uint64_t o = node_coff[n];
-    uint32_t last = 0xFFFFFFFFu;
-    for (uint32_t t = t0; t < t1; ++t) {
-        const uint64_t v = vals[t];
-        if (!v_owned(v)) continue;
-        const uint32_t c = v_color(v);
-        if (c != last) {
-            colors[o++] = c;
-            last = c;
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// One wave per queued node (the grid holds one wave per queued node): the
+// same reduction with the run read 256 occurrences at a time.  A lane's
+// colour is new when it differs from the colour of the nearest owned
+// occurrence before it (runs are colour-sorted).
+constexpr int BIG_UNROLL = 4;
+__global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ big, uint64_t nbig,
+                                                   const uint32_t* __restrict__ node_start,
+                                                   const uint64_t* __restrict__ vals, uint8_t* __restrict__ node_meta,
+                                                   uint32_t* __restrict__ node_rank, uint32_t* __restrict__ node_ncol,
+                                                   uint64_t* __restrict__ node_hash, uint32_t* __restrict__ colors,
+                                                   unsigned long long* __restrict__ counters) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t lt = lanes_below(lane);
+    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    uint64_t prim = 0, multi = 0, multi_cols = 0;
+    if (i < nbig) {
+        const uint32_t n = big[i];
+        const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
+        uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
+        uint64_t sum = 0;
+        bool primary = false;
+        for (uint32_t base = t0; base < t1; base += 64 * BIG_UNROLL) {
+            uint64_t v[BIG_UNROLL];
+#pragma unroll
+            for (int u = 0; u < BIG_UNROLL; ++u) {
+                const uint32_t t = base + uint32_t(64 * u + lane);
+                v[u] = t < t1 ? vals[t] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < BIG_UNROLL; ++u) {
+                const bool active = base + uint32_t(64 * u + lane) < t1;
+                const bool owned = active && v_owned(v[u]);
+                const uint32_t c = v_color(v[u]);
+                if (active) {
+                    fl |= v_contrib(v[u]);
+                    const uint32_t r = v_rank(v[u]);
+                    rank = r < rank ? r : rank;
+                }
+                const uint64_t om = __ballot(owned);
+                const uint64_t below = om & lt;
+                const int src = below ? 63 - __clzll(below) : lane;
+                const uint32_t pc = __shfl(c, src, 64);
+                const uint32_t prev = below ? pc : last;
+                const bool fresh = owned && c != prev;
+                const uint64_t fm = __ballot(fresh);
+                if (fresh) {
+                    colors[t0 + ncol + __popcll(fm & lt)] = c;
+                    sum += set_term(c);
+                }
+                ncol += __popcll(fm);
+                if (om) {
+                    last = __shfl(c, 63 - __clzll(om), 64);
+                    primary = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            fl |= __shfl_xor(fl, o, 64);
+            const uint32_t r = __shfl_xor(rank, o, 64);
+            rank = r < rank ? r : rank;
+            sum += __shfl_xor(sum, o, 64);
+        }
+        if (lane == 0) {
+            node_meta[n] = uint8_t(fl | (primary ? 4u : 0u));
+            node_rank[n] = rank;
+            node_ncol[n] = ncol;
+            node_hash[n] = set_hash_fin(sum, ncol);
+            prim = primary;
+            multi = ncol >= 2;
+            multi_cols = ncol >= 2 ? ncol : 0;
         }
     }
+    block_add(&counters[0], prim);
+    block_add(&counters[1], multi);
+    block_add(&counters[7], multi_cols);
 }
 
 // ---------------------------------------------------------------------------
 // colour sets
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t list_hash(const uint32_t* c, uint32_t n) {
-    uint64_t h = 0x243F6A8885A308D3ULL ^ n;
-    for (uint32_t i = 0; i < n; ++i) h = mix64(h ^ (uint64_t(c[i]) + 0x9E3779B97F4A7C15ULL * (i + 1)));
-    return h | 1ull;  // 0 marks an empty slot
-}
-
 struct SetTables {
     unsigned long long* single_min;  // [num_colors] min key of the set {c}
     uint32_t* single_cid;            // [num_colors]
@@ -343,13 +471,15 @@ struct SetTables {
     unsigned long long* store_used;
 };
 
+// Colour lists live at colors[coff[n]] (coff = the node's first occurrence).
 __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
-                             const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
-                             const uint32_t* __restrict__ rank, const uint64_t* __restrict__ node_off, uint32_t nb,
-                             uint32_t b0, SetTables T, uint32_t* __restrict__ node_slot) {
+                             const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                             const uint64_t* __restrict__ node_hash, const uint32_t* __restrict__ rank,
+                             const uint32_t* __restrict__ node_bucket, uint32_t b0, SetTables T,
+                             uint32_t* __restrict__ node_slot) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes || !(meta[n] & 4u)) return;
-    const uint32_t b = b0 + bucket_of(node_off, nb, n);
+    const uint32_t b = b0 + node_bucket[n];
     const unsigned long long key = (uint64_t(b) << 32) | rank[n];
     const uint32_t nc = ncol[n];
     const uint32_t* c = colors + coff[n];
@@ -358,7 +488,7 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
         if (T.single_min[c[0]] > key) atomicMin(&T.single_min[c[0]], key);
         return;
     }
-    const unsigned long long h = list_hash(c, nc);
+    const unsigned long long h = node_hash[n];
     uint64_t s = h & (T.cap - 1);
     for (uint64_t probe = 0; probe < T.cap; ++probe) {
         unsigned long long cur = T.mt_hash[s];
@@ -381,16 +511,38 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
     }
 }
 
+// Exact check of every multi-colour node's list against the list stored with
+// its hash slot; nodes with long runs are checked by k_set_verify_big.
 __global__ void k_set_verify(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
-                             const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                             const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
                              const uint32_t* __restrict__ node_slot, SetTables T, unsigned long long* __restrict__ err) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2) return;
+    if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2 || coff[n + 1] - coff[n] > BIG_RUN) return;
     const uint32_t s = node_slot[n];
     const uint32_t* ref = T.store + T.mt_loff[s];
     bool same = T.mt_llen[s] == ncol[n];
     for (uint32_t i = 0; same && i < ncol[n]; ++i) same = colors[coff[n] + i] == ref[i];
     if (!same) atomicAdd(err, 1ull);
+}
+
+__global__ __launch_bounds__(TPB) void k_set_verify_big(const uint32_t* __restrict__ big, uint64_t nbig,
+                                                        const uint8_t* __restrict__ meta,
+                                                        const uint32_t* __restrict__ ncol,
+                                                        const uint32_t* __restrict__ coff,
+                                                        const uint32_t* __restrict__ colors,
+                                                        const uint32_t* __restrict__ node_slot, SetTables T,
+                                                        unsigned long long* __restrict__ err) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (i >= nbig) return;
+    const uint32_t n = big[i];
+    if (!(meta[n] & 4u) || ncol[n] < 2) return;
+    const uint32_t s = node_slot[n], nc = ncol[n];
+    const uint32_t* ref = T.store + T.mt_loff[s];
+    const uint32_t* own = colors + coff[n];
+    bool diff = T.mt_llen[s] != nc;
+    for (uint32_t j = uint32_t(lane); !diff && j < nc; j += 64) diff = own[j] != ref[j];
+    if (__ballot(diff) && lane == 0) atomicAdd(err, 1ull);
 }
 
 __global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restrict__ oh,
@@ -562,7 +714,7 @@ __global__ void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_bas
 }
 
 __global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
-                           const uint64_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                           const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
                            const uint32_t* __restrict__ node_slot, SetTables T, uint32_t* __restrict__ node_cid) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
@@ -575,33 +727,48 @@ __global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, c
 // per-bucket k-mer table, masks, links
 // ---------------------------------------------------------------------------
 template <class K>
-__global__ void k_table_insert(uint32_t n_nodes, const K* __restrict__ node_key, const uint64_t* __restrict__ node_off,
-                               uint32_t nb, const uint64_t* __restrict__ tab_off, Slot<K>* __restrict__ tab) {
+__global__ void k_table_insert(uint32_t n_nodes, const K* __restrict__ node_key,
+                               const uint32_t* __restrict__ node_bucket, const uint64_t* __restrict__ tab_off,
+                               K* __restrict__ tab, uint32_t* __restrict__ node_of) {
+    constexpr uint32_t G = Tab<K>::G;
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
-    const uint32_t b = bucket_of(node_off, nb, n);
-    const uint64_t base = tab_off[b], cap = tab_off[b + 1] - base;
+    const uint32_t b = node_bucket[n];
+    const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base) / G;
     const K key = node_key[n];
-    uint64_t s = KOps<K>::hash(key) & (cap - 1);
-    for (uint64_t probe = 0; probe < cap; ++probe) {
-        if (atomicCAS(&tab[base + s].node, NONE32, uint32_t(n)) == NONE32) {
-            tab[base + s].key = key;
-            return;
+    uint64_t g = KOps<K>::hash(key) & (ng - 1);
+    for (uint64_t probe = 0; probe < ng; ++probe) {
+        K* grp = tab + base + g * G;
+        for (uint32_t j = 0; j < G; ++j) {
+            if (Tab<K>::empty(grp[j]) && Tab<K>::claim(&grp[j], key)) {
+                node_of[base + g * G + j] = uint32_t(n);
+                return;
+            }
         }
-        s = (s + 1) & (cap - 1);
+        g = (g + 1) & (ng - 1);
     }
 }
 
+// slot of key in bucket table [base, base + ng * G), or ~0 when absent
 template <class K>
-__device__ __forceinline__ uint32_t tab_find(const Slot<K>* __restrict__ tab, uint64_t base, uint64_t cap, const K& key) {
-    uint64_t s = KOps<K>::hash(key) & (cap - 1);
-    for (uint64_t probe = 0; probe < cap; ++probe) {
-        const uint32_t nd = tab[base + s].node;
-        if (nd == NONE32) return NONE32;
-        if (KOps<K>::eq(tab[base + s].key, key)) return nd;
-        s = (s + 1) & (cap - 1);
+__device__ __forceinline__ uint64_t tab_find(const K* __restrict__ tab, uint64_t base, uint64_t ng, const K& key) {
+    constexpr uint32_t G = Tab<K>::G;
+    uint64_t g = KOps<K>::hash(key) & (ng - 1);
+    for (uint64_t probe = 0; probe < ng; ++probe) {
+        const K* grp = tab + base + g * G;
+        K v[G];
+#pragma unroll
+        for (uint32_t j = 0; j < G; ++j) v[j] = grp[j];
+        bool any_empty = false;
+#pragma unroll
+        for (uint32_t j = 0; j < G; ++j) {
+            if (KOps<K>::eq(v[j], key)) return base + g * G + j;
+            any_empty |= Tab<K>::empty(v[j]);
+        }
+        if (any_empty) return ~0ull;
+        g = (g + 1) & (ng - 1);
     }
-    return NONE32;
+    return ~0ull;
 }
 
 // mask bits 0..3: successors of the canonical k-mer, 4..7: successors of its
@@ -609,40 +776,44 @@ __device__ __forceinline__ uint32_t tab_find(const Slot<K>* __restrict__ tab, ui
 // with exactly one present extension, nb[2n+port] = (neighbour << 1) | entry
 // port: the neighbour is entered at its L side iff the extension is canonical.
 template <class K>
-__global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint64_t* __restrict__ node_off,
-                        uint32_t nbk, const uint64_t* __restrict__ tab_off, const Slot<K>* __restrict__ tab, uint32_t k,
-                        uint8_t* __restrict__ mask_out, uint32_t* __restrict__ nb) {
+__global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
+                        const uint64_t* __restrict__ tab_off, const K* __restrict__ tab,
+                        const uint32_t* __restrict__ node_of, uint32_t k, uint8_t* __restrict__ mask_out,
+                        uint32_t* __restrict__ nb) {
     using O = KOps<K>;
+    constexpr uint32_t G = Tab<K>::G;
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
-    const uint32_t b = bucket_of(node_off, nbk, n);
-    const uint64_t base = tab_off[b], cap = tab_off[b + 1] - base;
+    const uint32_t b = node_bucket[n];
+    const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base) / G;
     const K can = node_key[n];
     const K rv = O::rc(can, k);
-    uint32_t mask = 0, yR = NONE32, yL = NONE32, qR = 0, qL = 0;
+    uint32_t mask = 0, qR = 0, qL = 0;
+    uint64_t sR = ~0ull, sL = ~0ull;
+#pragma unroll
     for (uint32_t nt = 0; nt < 4; ++nt) {
         K e = O::shift_append(can, nt, k);
         K er = O::rc(e, k);
         bool ec = O::le(e, er);
-        uint32_t y = tab_find<K>(tab, base, cap, ec ? e : er);
-        if (y != NONE32) {
+        uint64_t sl = tab_find<K>(tab, base, ng, ec ? e : er);
+        if (sl != ~0ull) {
             mask |= 1u << nt;
-            yR = y;
+            sR = sl;
             qR = ec ? PORT_L : PORT_R;
         }
         e = O::shift_append(rv, nt, k);
         er = O::rc(e, k);
         ec = O::le(e, er);
-        y = tab_find<K>(tab, base, cap, ec ? e : er);
-        if (y != NONE32) {
+        sl = tab_find<K>(tab, base, ng, ec ? e : er);
+        if (sl != ~0ull) {
             mask |= 1u << (4 + nt);
-            yL = y;
+            sL = sl;
             qL = ec ? PORT_L : PORT_R;
         }
     }
     mask_out[n] = uint8_t(mask);
-    nb[2 * n + PORT_R] = __popc(mask & 0xfu) == 1 ? (yR << 1) | qR : NONE32;
-    nb[2 * n + PORT_L] = __popc(mask >> 4) == 1 ? (yL << 1) | qL : NONE32;
+    nb[2 * n + PORT_R] = __popc(mask & 0xfu) == 1 ? (node_of[sR] << 1) | qR : NONE32;
+    nb[2 * n + PORT_L] = __popc(mask >> 4) == 1 ? (node_of[sL] << 1) | qL : NONE32;
 }
 
 __device__ __forceinline__ uint32_t nib(uint32_t m, uint32_t port) { return port == PORT_R ? (m & 0xfu) : (m >> 4); }
@@ -667,8 +838,7 @@ struct WalkArgs {
     const uint8_t* meta;
     const uint8_t* mask;
     const uint32_t* rank;
-    const uint64_t* node_off;
-    uint32_t nb;
+    const uint32_t* node_bucket;
     uint32_t b0;
     uint8_t* visited;
     uint32_t* frag_nodes;
@@ -677,7 +847,17 @@ struct WalkArgs {
     uint32_t* frag_n;
     uint8_t* frag_open;
     uint64_t* frag_key;
-    unsigned long long* counters;  // [3] nodes placed, [4] fragments, [5] errors
+    uint8_t* frag_dir;             // 1: the fragment is its node range read backwards
+    unsigned long long* counters;  // [3] nodes placed, [4] fragments, [5] errors, [9] paths, [10] other paths
+    // simple paths, recorded end to end
+    uint32_t* comp_x;              // owning end
+    uint8_t* comp_ex;              // port the path leaves it by
+    uint32_t* comp_len;
+    const uint32_t* comp_off;      // first position in frag_nodes
+    uint32_t* other_x;             // paths that pass a node twice (hairpins)
+    uint8_t* other_ex;
+    uint16_t* pinfo;               // per position: meta | mask << 8
+    uint32_t* prank;               // per position: seed rank
 };
 
 __device__ __forceinline__ bool is_break(uint8_t meta) {
@@ -764,6 +944,7 @@ __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
     a.frag_n[fi] = total;
     a.frag_open[fi] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
     a.frag_key[fi] = (uint64_t(b) << 32) | a.rank[s];
+    a.frag_dir[fi] = 0;
     return total;
 }
 
@@ -777,7 +958,7 @@ __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
 // the traversal continues through x's other side.
 __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, bool cycle, uint32_t len,
                                    uint32_t seed) {
-    const uint32_t b = a.b0 + bucket_of(a.node_off, a.nb, x);
+    const uint32_t b = a.b0 + a.node_bucket[x];
     const uint32_t limit = 2 * a.n_nodes + 2;
     uint32_t placed = 0;
     for (uint32_t guard = 0; guard < limit; ++guard) {
@@ -803,15 +984,17 @@ __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, 
     atomicAdd(&a.counters[5], 1ull);
 }
 
-__global__ void k_walk_paths(WalkArgs a) {
+// Path ends walk to the other end; the end with the smaller index owns the
+// path.  Paths that never pass a node twice are queued for recording, the
+// others (a node linked to itself turns the walk back) for the general seed
+// loop.
+__global__ void k_walk_ends(WalkArgs a) {
     const uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (x >= a.n_nodes) return;
     const uint32_t lL = a.link[2 * x + PORT_L], lR = a.link[2 * x + PORT_R];
     if (lL != NONE32 && lR != NONE32) return;  // not a path end
-    // walk to the other end (the end with the smaller index owns the path),
-    // counting nodes and tracking the first seed
     const uint32_t ex0 = lR == NONE32 ? PORT_L : PORT_R;
-    uint32_t cur = uint32_t(x), ex = ex0, len = 1, seed = uint32_t(x), seed_rank = a.rank[x];
+    uint32_t cur = uint32_t(x), ex = ex0, len = 1;
     bool simple = true;
     const uint32_t limit = 2 * a.n_nodes + 2;
     for (uint32_t h = 0; h < limit; ++h) {
@@ -822,14 +1005,133 @@ __global__ void k_walk_paths(WalkArgs a) {
         cur = y;
         ex = 1u - (l & 1u);
         ++len;
+    }
+    if (cur < x) return;
+    if (simple) {
+        const uint64_t c = atomicAdd(&a.counters[9], 1ull);
+        a.comp_x[c] = uint32_t(x);
+        a.comp_ex[c] = uint8_t(ex0);
+        a.comp_len[c] = len;
+    } else {
+        const uint64_t c = atomicAdd(&a.counters[10], 1ull);
+        a.other_x[c] = uint32_t(x);
+        a.other_ex[c] = uint8_t(ex0);
+    }
+}
+
+// Writes path c end to end at frag_nodes[comp_off[c] ...]: node, its
+// orientation along the path (1 when it is passed entering by R), its meta,
+// mask and rank.
+__global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_comp) return;
+    const uint32_t L = a.comp_len[c];
+    const uint64_t off = a.comp_off[c];
+    uint32_t cur = a.comp_x[c], ex = a.comp_ex[c];
+    uint8_t o = ex == PORT_L ? 1 : 0;
+    for (uint32_t i = 0;; ++i) {
+        a.frag_nodes[off + i] = cur;
+        a.frag_rc[off + i] = o;
+        a.pinfo[off + i] = uint16_t(a.meta[cur] | (uint32_t(a.mask[cur]) << 8));
+        a.prank[off + i] = a.rank[cur];
+        a.visited[cur] = 1;
+        if (i + 1 == L) break;
+        const uint32_t l = a.link[2 * cur + ex];
+        const uint32_t q = l & 1u;
+        cur = l >> 1;
+        ex = 1u - q;
+        o = q == PORT_R ? 1 : 0;
+    }
+}
+
+// The seed loop of bucket_walker.hpp:451-570 on a recorded path: seeds are
+// the unplaced positions in rank order; from a seed the fragment grows
+// backward then forward along the path (positions -1 / +1 when the seed lies
+// forward on the path, +1 / -1 otherwise), stopping before a placed position
+// or after a contig-break node.  Bit 1 of frag_rc marks placed positions.
+__global__ void k_walk_frags(WalkArgs a, uint64_t n_comp) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_comp) return;
+    const uint32_t L = a.comp_len[c];
+    const uint64_t off = a.comp_off[c];
+    uint8_t* rc = a.frag_rc + off;
+    const uint16_t* info = a.pinfo + off;
+    const uint32_t* rk = a.prank + off;
+    const uint32_t b = a.b0 + a.node_bucket[a.frag_nodes[off]];
+    uint32_t placed = 0;
+    while (placed < L) {
+        uint32_t p = NONE32, best = 0xFFFFFFFFu;
+        for (uint32_t i = 0; i < L; ++i)
+            if (!(rc[i] & 2u) && rk[i] < best) {
+                best = rk[i];
+                p = i;
+            }
+        const uint32_t o = rc[p] & 1u;
+        const int fwd = o ? -1 : 1;
+        const uint32_t sflags = info[p] & 3u, ms = info[p] >> 8;
+        rc[p] |= 2u;
+        int lo = int(p), hi = int(p);
+        bool open_l = false, open_r = false;
+        // backward, then forward; the port a node is left by in walking
+        // direction d is its path exit port (R when forward on the path)
+        // for d = +1 and its path entry port for d = -1
+        for (int side = 0; side < 2; ++side) {
+            const int d = side == 0 ? -fwd : fwd;
+            bool& open = side == 0 ? open_l : open_r;
+            if (sflags == (side == 0 ? BOUND_LEFT : BOUND_RIGHT)) {
+                open = __popc(side == 0 ? (ms >> 4) : (ms & 0xfu)) <= 1;
+                continue;
+            }
+            for (int i = int(p) + d; i >= 0 && i < int(L); i += d) {
+                if (rc[i] & 2u) break;
+                rc[i] |= 2u;
+                if (d < 0)
+                    lo = i;
+                else
+                    hi = i;
+                const uint32_t mi = info[i];
+                if (is_break(uint8_t(mi))) {
+                    const uint32_t oi = rc[i] & 1u;
+                    const uint32_t ex = (d > 0) == (oi == 0) ? PORT_R : PORT_L;
+                    open = __popc(nib(mi >> 8, ex)) <= 1;
+                    break;
+                }
+            }
+        }
+        const uint32_t n = uint32_t(hi - lo + 1);
+        placed += n;
+        const uint64_t fi = atomicAdd(&a.counters[4], 1ull);
+        a.frag_off[fi] = off + uint32_t(lo);
+        a.frag_n[fi] = n;
+        a.frag_dir[fi] = uint8_t(o);
+        a.frag_open[fi] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
+        a.frag_key[fi] = (uint64_t(b) << 32) | best;
+    }
+}
+
+// General seed loop for the queued paths that pass a node twice.
+__global__ void k_walk_other(WalkArgs a, uint64_t n_other) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_other) return;
+    const uint32_t x = a.other_x[c], ex0 = a.other_ex[c];
+    uint32_t cur = x, ex = ex0, seed = x, seed_rank = a.rank[x];
+    const uint32_t limit = 2 * a.n_nodes + 2;
+    for (uint32_t h = 0; h < limit; ++h) {
+        const uint32_t l = a.link[2 * cur + ex];
+        if (l == NONE32) break;
+        cur = l >> 1;
+        ex = 1u - (l & 1u);
         const uint32_t r = a.rank[cur];
         if (r < seed_rank) {
             seed = cur;
             seed_rank = r;
         }
     }
-    if (cur < x) return;
-    simulate_component(a, uint32_t(x), ex0, false, simple ? len : 0, seed);
+    simulate_component(a, x, ex0, false, 0, seed);
+}
+
+__global__ void k_set_u64(unsigned long long* p, uint64_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 __global__ void k_walk_cycles(WalkArgs a) {
@@ -864,16 +1166,18 @@ __global__ void k_walk_cycles(WalkArgs a) {
 // fragment records
 // ---------------------------------------------------------------------------
 __global__ void k_frag_size(uint64_t n_frags, const uint32_t* __restrict__ order, const uint64_t* __restrict__ frag_off,
-                            const uint32_t* __restrict__ frag_n, const uint32_t* __restrict__ frag_nodes,
-                            const uint32_t* __restrict__ node_cid, uint32_t k, uint64_t* __restrict__ size) {
+                            const uint32_t* __restrict__ frag_n, const uint8_t* __restrict__ frag_dir,
+                            const uint32_t* __restrict__ frag_nodes, const uint32_t* __restrict__ node_cid, uint32_t k,
+                            uint64_t* __restrict__ size) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_frags) return;
     const uint32_t f = order[i];
     const uint64_t o = frag_off[f];
     const uint32_t n = frag_n[f];
+    const bool rev = frag_dir[f];
     uint32_t runs = 0, last = 0;
     for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t c = node_cid[frag_nodes[o + j]];
+        const uint32_t c = node_cid[frag_nodes[o + (rev ? n - 1 - j : j)]];
         if (j == 0 || c != last) ++runs;
         last = c;
     }
@@ -887,6 +1191,7 @@ __device__ __forceinline__ void put_bytes(uint8_t* p, uint64_t v, int n) {
 template <class K>
 __global__ void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ order, const uint64_t* __restrict__ frag_off,
                              const uint32_t* __restrict__ frag_n, const uint8_t* __restrict__ frag_open,
+                             const uint8_t* __restrict__ frag_dir,
                              const uint32_t* __restrict__ frag_nodes, const uint8_t* __restrict__ frag_rc,
                              const uint32_t* __restrict__ node_cid, const K* __restrict__ node_key, uint32_t k,
                              const uint64_t* __restrict__ boff, uint8_t* __restrict__ out) {
@@ -896,12 +1201,13 @@ __global__ void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ orde
     const uint32_t f = order[i];
     const uint64_t o = frag_off[f];
     const uint32_t n = frag_n[f];
+    const uint32_t rev = frag_dir[f];
     uint8_t* p = out + boff[i];
     p[0] = frag_open[f];
     uint8_t* pr = p + 5;
     uint32_t runs = 0, last = 0, cnt = 0;
     for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t c = node_cid[frag_nodes[o + j]];
+        const uint32_t c = node_cid[frag_nodes[o + (rev ? n - 1 - j : j)]];
         if (j > 0 && c != last) {
             put_bytes(pr, last == NONE32 ? ~0ull : uint64_t(last), 8);
             put_bytes(pr + 8, cnt, 4);
@@ -922,9 +1228,10 @@ __global__ void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ orde
     uint8_t* ps = pr + 4;
     const char ACGT[4] = {'A', 'C', 'G', 'T'};
     for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t nd = frag_nodes[o + j];
+        const uint64_t at = o + (rev ? n - 1 - j : j);
+        const uint32_t nd = frag_nodes[at];
         const K key = node_key[nd];
-        const K km = frag_rc[o + j] ? O::rc(key, k) : key;
+        const K km = ((frag_rc[at] & 1u) ^ rev) ? O::rc(key, k) : key;
         if (j == 0) {
             for (uint32_t t = 0; t < k; ++t) ps[t] = ACGT[O::base(km, t, k)];
         } else {
@@ -947,10 +1254,11 @@ struct Phase2Bufs {
     // per chunk (reused)
     DevBuf keys, vals, keys_t, vals_t;
     DevBuf heads, hscan, node_start, node_off;
-    DevBuf node_key, node_meta, node_rank, node_ncol, node_coff, colors, counters;
+    DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
-    DevBuf node_cid, tab_off, tab, node_mask, nb, link, visited;
-    DevBuf frag_nodes, frag_rc, frag_off, frag_n, frag_open, frag_key;
+    DevBuf node_cid, node_bucket, tab_off, tab, node_of, node_mask, nb, link, visited;
+    DevBuf frag_nodes, frag_rc, frag_off, frag_n, frag_open, frag_key, frag_dir;
+    DevBuf comp_x, comp_ex, comp_len, comp_off, other_x, other_ex, pinfo, prank;
     DevBuf fkey_t, forder, forder_t, fsize, fboff;
     RadixScratch rs2;
     // results
@@ -1061,19 +1369,22 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint64_t* node_off = P.node_off.ensure<uint64_t>(nb + 1);
     { KSCOPE("k_node_start", st); k_node_start<<<blocks_for(nt), TPB, 0, st>>>(nt, heads, hscan, nn, node_start); }
     { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, nt, hscan, nn, node_off); }
+    uint32_t* node_bucket = P.node_bucket.ensure<uint32_t>(nn);
+    { KSCOPE("k_node_bucket", st); k_node_bucket<<<blocks_for(nn), TPB, 0, st>>>(nn, node_off, nb, node_bucket); }
     K* node_key = P.node_key.ensure<K>(nn);
     uint8_t* meta = P.node_meta.ensure<uint8_t>(nn);
     uint32_t* rank = P.node_rank.ensure<uint32_t>(nn);
     uint32_t* ncol = P.node_ncol.ensure<uint32_t>(nn);
-    uint64_t* coff = P.node_coff.ensure<uint64_t>(uint64_t(nn) + 1);
-    unsigned long long* cnt = P.counters.ensure<unsigned long long>(8);
-    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, 8, 0ull); }
-    { KSCOPE("k_node_reduce", st); k_node_reduce<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, keys, vals, node_key, meta, rank, ncol, cnt); }
-    exclusive_scan<uint32_t, uint64_t>(ncol, coff, nn, coff + nn, J.rs.scan_tmp, st);
-    const uint64_t n_cols = d2h<uint64_t>(coff + nn, st);
-    uint32_t* colors = P.colors.ensure<uint32_t>(n_cols);
-    { KSCOPE("k_node_colors", st); k_node_colors<<<blocks_for(nn), TPB, 0, st>>>(nn, node_start, vals, coff, colors); }
-    unsigned long long h_cnt[8];
+    uint64_t* nhash = P.node_hash.ensure<uint64_t>(nn);
+    uint32_t* big = P.big.ensure<uint32_t>(nn);
+    uint32_t* colors = P.colors.ensure<uint32_t>(nt);  // node lists in place at their first occurrence
+    const uint32_t* coff = node_start;
+    unsigned long long* cnt = P.counters.ensure<unsigned long long>(N_COUNTERS);
+    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, N_COUNTERS, 0ull); }
+    { KSCOPE("k_nodes", st); k_nodes<K><<<blocks_for(nt, NT_TILE), TPB, 0, st>>>(nt, heads, hscan, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big, cnt); }
+    unsigned long long h_cnt[N_COUNTERS];
+    const uint64_t nbig = d2h<unsigned long long>(cnt + 8, st);
+    if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors, cnt); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1], multi_cols = h_cnt[7];
@@ -1086,8 +1397,9 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     P.store.ensure_keep<uint32_t>(store_used + multi_cols + 1, store_used * 4, st);
     SetTables T = set_tables(P);
     uint32_t* node_slot = P.node_slot.ensure<uint32_t>(nn);
-    { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, rank, node_off, nb, b0, T, node_slot); }
+    { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, nhash, rank, node_bucket, b0, T, node_slot); }
     { KSCOPE("k_set_verify", st); k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
+    if (nbig) { KSCOPE("k_set_verify_big", st); k_set_verify_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
     const uint32_t NC = J.num_colors;
     const uint64_t nflag = NC + P.mt_cap;
     uint32_t* set_fl = P.set_fl.ensure<uint32_t>(nflag);
@@ -1134,7 +1446,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     std::vector<uint64_t> h_tab_off(nb + 1, 0);
     for (uint32_t b = 0; b < nb; ++b) {
         const uint64_t n_b = h_node_off[b + 1] - h_node_off[b];
-        uint64_t c = 8;
+        uint64_t c = Tab<K>::G;  // slots, a power-of-two number of groups
         while (c * 5 < n_b * 8) c <<= 1;  // load factor <= 0.625
         h_tab_off[b + 1] = h_tab_off[b] + (n_b ? c : 0);
     }
@@ -1142,15 +1454,14 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     CDBG_HIP(hipMemcpyAsync(tab_off, h_tab_off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
     CDBG_HIP(hipStreamSynchronize(st));  // h_tab_off is pageable and local
     const uint64_t tab_n = h_tab_off[nb];
-    Slot<K>* tab = P.tab.ensure<Slot<K>>(tab_n);
-    Slot<K> empty{};
-    empty.node = NONE32;
-    { KSCOPE("k_fill", st); k_fill<Slot<K>><<<blocks_for(tab_n), TPB, 0, st>>>(tab, tab_n, empty); }
-    { KSCOPE("k_table_insert", st); k_table_insert<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, nb, tab_off, tab); }
+    K* tab = P.tab.ensure<K>(tab_n);
+    uint32_t* node_of = P.node_of.ensure<uint32_t>(tab_n);
+    { KSCOPE("k_fill", st); k_fill<uint64_t><<<blocks_for(tab_n * sizeof(K) / 8), TPB, 0, st>>>((uint64_t*)tab, tab_n * sizeof(K) / 8, ~0ull); }
+    { KSCOPE("k_table_insert", st); k_table_insert<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of); }
     uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
     uint32_t* nbr = P.nb.ensure<uint32_t>(2ull * nn);
     uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
-    { KSCOPE("k_masks", st); k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_off, nb, tab_off, tab, k, node_mask, nbr); }
+    { KSCOPE("k_masks", st); k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, node_mask, nbr); }
     { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link); }
 
     // --- walk
@@ -1162,8 +1473,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.meta = meta;
     wa.mask = node_mask;
     wa.rank = rank;
-    wa.node_off = node_off;
-    wa.nb = nb;
+    wa.node_bucket = node_bucket;
     wa.b0 = b0;
     wa.visited = visited;
     wa.frag_nodes = P.frag_nodes.ensure<uint32_t>(nn);
@@ -1172,8 +1482,28 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.frag_n = P.frag_n.ensure<uint32_t>(nn);
     wa.frag_open = P.frag_open.ensure<uint8_t>(nn);
     wa.frag_key = P.frag_key.ensure<uint64_t>(nn);
+    wa.frag_dir = P.frag_dir.ensure<uint8_t>(nn);
     wa.counters = cnt;
-    { KSCOPE("k_walk_paths", st); k_walk_paths<<<blocks_for(nn), TPB, 0, st>>>(wa); }
+    wa.comp_x = P.comp_x.ensure<uint32_t>(nn);
+    wa.comp_ex = P.comp_ex.ensure<uint8_t>(nn);
+    wa.comp_len = P.comp_len.ensure<uint32_t>(nn);
+    uint32_t* comp_off = P.comp_off.ensure<uint32_t>(uint64_t(nn) + 1);
+    wa.comp_off = comp_off;
+    wa.other_x = P.other_x.ensure<uint32_t>(nn);
+    wa.other_ex = P.other_ex.ensure<uint8_t>(nn);
+    wa.pinfo = P.pinfo.ensure<uint16_t>(nn);
+    wa.prank = P.prank.ensure<uint32_t>(nn);
+    { KSCOPE("k_walk_ends", st); k_walk_ends<<<blocks_for(nn), TPB, 0, st>>>(wa); }
+    CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    const uint64_t n_comp = h_cnt[9], n_other = h_cnt[10];
+    exclusive_scan<uint32_t, uint32_t>(wa.comp_len, comp_off, n_comp, comp_off + n_comp, J.rs.scan_tmp, st);
+    const uint32_t path_nodes = n_comp ? d2h<uint32_t>(comp_off + n_comp, st) : 0u;
+    // the general seed loop places its nodes after the recorded paths
+    { KSCOPE("k_set_u64", st); k_set_u64<<<1, 64, 0, st>>>(cnt + 3, path_nodes); }
+    if (n_comp) { KSCOPE("k_walk_record", st); k_walk_record<<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
+    if (n_comp) { KSCOPE("k_walk_frags", st); k_walk_frags<<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
+    if (n_other) { KSCOPE("k_walk_other", st); k_walk_other<<<blocks_for(n_other), TPB, 0, st>>>(wa, n_other); }
     { KSCOPE("k_walk_cycles", st); k_walk_cycles<<<blocks_for(nn), TPB, 0, st>>>(wa); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
@@ -1194,11 +1524,11 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
                                              "frags");
     uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
     uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
-    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_nodes, node_cid, k, fsize); }
+    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_dir, wa.frag_nodes, node_cid, k, fsize); }
     exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
     const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
     uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st);
-    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_nodes, wa.frag_rc, node_cid, node_key, k, fboff, fout + P.frag_total); }
+    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_dir, wa.frag_nodes, wa.frag_rc, node_cid, node_key, k, fboff, fout + P.frag_total); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));

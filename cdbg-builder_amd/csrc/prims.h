@@ -219,13 +219,25 @@ struct TileDesc {
     uint32_t pad;
 };
 
+// Workgroups are dispatched to the 8 XCDs round-robin; tile_of() gives XCD x
+// the x-th eighth of the tiles, so the tiles of one segment (whose outputs
+// interleave) share one L2.  Grids are launched with 8 * ceil(nt / 8) blocks.
+constexpr uint32_t N_XCD = 8;
+__device__ __forceinline__ uint32_t tile_of(uint32_t block, uint32_t nt) {
+    const uint32_t per = (nt + N_XCD - 1) / N_XCD;
+    return (block % N_XCD) * per + block / N_XCD;
+}
+inline uint32_t xcd_grid(uint32_t nt) { return ((nt + N_XCD - 1) / N_XCD) * N_XCD; }
+
 template <class K>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K* __restrict__ keys,
-                                                           const TileDesc* __restrict__ tiles, int shift,
-                                                           int bits, uint32_t* __restrict__ hist) {
+                                                           const TileDesc* __restrict__ tiles, uint32_t nt,
+                                                           int shift, int bits, uint32_t* __restrict__ hist) {
     extern __shared__ uint32_t s_hist[];
+    const uint32_t tile = tile_of(blockIdx.x, nt);
+    if (tile >= nt) return;
     const uint32_t R = 1u << bits, mask = R - 1;
-    const TileDesc d = tiles[blockIdx.x];
+    const TileDesc d = tiles[tile];
     for (uint32_t i = threadIdx.x; i < R; i += RS_THREADS) s_hist[i] = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < d.count; i += RS_THREADS)
@@ -236,35 +248,54 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const K* __restrict__
         hist[region + uint64_t(b) * d.ntiles_seg + d.local] = s_hist[b];
 }
 
+// Stable scatter of one tile.  Wave w owns keys [1024w, 1024w + 1024) of
+// the tile and ranks them alone, 64 at a time: `bits` ballots give each
+// lane its peers (same digit) in the round, and a per-wave LDS count per
+// digit carries the rank across rounds.  With the per-wave counts known, the
+// tile is placed in LDS in digit order and written out digit run by digit
+// run, so consecutive lanes store consecutive addresses.  Four barriers per
+// tile.
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_ITEMS = RS_TILE / RS_THREADS;  // rounds per wave
+
 template <class K, class V>
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const K* __restrict__ keys,
                                                               const V* __restrict__ vals,
                                                               K* __restrict__ okeys, V* __restrict__ ovals,
-                                                              const TileDesc* __restrict__ tiles, int shift,
-                                                              int bits, const uint32_t* __restrict__ offs) {
-    extern __shared__ uint32_t sm[];
+                                                              const TileDesc* __restrict__ tiles, uint32_t nt,
+                                                              int shift, int bits, const uint32_t* __restrict__ hist,
+                                                              const uint32_t* __restrict__ offs) {
+    extern __shared__ uint64_t sm64[];
+    const uint32_t tile = tile_of(blockIdx.x, nt);
+    if (tile >= nt) return;
     const uint32_t R = 1u << bits, mask = R - 1;
-    uint32_t* running = sm;
-    uint32_t* whist = sm + R;  // [4][R]
-    const TileDesc d = tiles[blockIdx.x];
+    K* s_key = (K*)sm64;
+    V* s_val = (V*)(s_key + RS_TILE);
+    uint32_t* dstart = (uint32_t*)(s_val + RS_TILE);  // [R] tile-local first slot of each digit
+    uint32_t* gbase = dstart + R;                      // [R] global first slot of each digit
+    uint32_t* wcnt = gbase + R;                        // [RS_WAVES][R] per-wave counts, then bases
+    uint32_t* s_tmp = wcnt + RS_WAVES * R;             // scan scratch
+    const TileDesc d = tiles[tile];
     const uint64_t region = d.hist_base * R;
-    for (uint32_t b = threadIdx.x; b < R; b += RS_THREADS) {
-        running[b] = offs[region + uint64_t(b) * d.ntiles_seg + d.local];
-        whist[b] = whist[R + b] = whist[2 * R + b] = whist[3 * R + b] = 0;
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    for (uint32_t base = 0; base < d.count; base += RS_THREADS) {
-        const uint32_t i = base + threadIdx.x;
+    for (uint32_t b = threadIdx.x; b < RS_WAVES * R; b += RS_THREADS) wcnt[b] = 0;
+    __syncthreads();
+
+    // rank: digit and wave-local rank of every item, kept in registers
+    K key[RS_ITEMS];
+    V val[RS_ITEMS];
+    uint32_t dr[RS_ITEMS];  // digit << 16 | wave-local rank
+    uint32_t* my = wcnt + wave * R;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const uint32_t i = uint32_t(wave) * (RS_TILE / RS_WAVES) + uint32_t(r) * 64 + uint32_t(lane);
         const bool active = i < d.count;
-        K key{};
-        V val{};
         uint32_t dig = 0;
         if (active) {
-            key = keys[d.start + i];
-            val = vals[d.start + i];
-            dig = digit_of(key, shift, mask);
+            key[r] = keys[d.start + i];
+            val[r] = vals[d.start + i];
+            dig = digit_of(key[r], shift, mask);
         }
         uint64_t peers = __ballot(active);
         for (int b = 0; b < bits; ++b) {
@@ -272,23 +303,77 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const K* __restric
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
         }
-        const uint32_t rank = __popcll(peers & lt);
-        if (active && rank == 0) whist[wave * R + dig] = __popcll(peers);
-        __syncthreads();
-        if (active) {
-            uint32_t pre = 0;
-            for (int w = 0; w < wave; ++w) pre += whist[w * R + dig];
-            const uint32_t pos = running[dig] + pre + rank;
-            okeys[pos] = key;
-            ovals[pos] = val;
-        }
-        __syncthreads();
-        for (uint32_t b = threadIdx.x; b < R; b += RS_THREADS) {
-            running[b] += whist[b] + whist[R + b] + whist[2 * R + b] + whist[3 * R + b];
-            whist[b] = whist[R + b] = whist[2 * R + b] = whist[3 * R + b] = 0;
-        }
-        __syncthreads();
+        const uint32_t rk = __popcll(peers & lt);
+        uint32_t before = 0;
+        if (active) before = my[dig];
+        // leaders write after every lane has read (the write uses the read)
+        if (active && rk == 0) my[dig] = before + __popcll(peers);
+        dr[r] = (dig << 16) | (before + rk);
     }
+    __syncthreads();
+
+    // digit starts of the tile and of every wave inside it
+    {
+        const uint32_t per = R >= RS_THREADS ? R / RS_THREADS : 1u;  // <= 8 for R <= 2048
+        const uint32_t b0 = threadIdx.x * per;
+        uint32_t c[8];
+        uint32_t acc = 0;
+        for (uint32_t j = 0; j < per; ++j) {
+            c[j] = (b0 + j < R) ? hist[region + uint64_t(b0 + j) * d.ntiles_seg + d.local] : 0u;
+            acc += c[j];
+        }
+        uint32_t tot;
+        uint32_t ex = block_exclusive_scan<uint32_t>(acc, s_tmp, tot);
+        for (uint32_t j = 0; j < per; ++j) {
+            const uint32_t b = b0 + j;
+            if (b < R) {
+                dstart[b] = ex;
+                gbase[b] = offs[region + uint64_t(b) * d.ntiles_seg + d.local];
+                uint32_t run = ex;
+                for (int w = 0; w < RS_WAVES; ++w) {
+                    const uint32_t cw = wcnt[w * R + b];
+                    wcnt[w * R + b] = run;
+                    run += cw;
+                }
+            }
+            ex += c[j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; ++r) {
+        const uint32_t i = uint32_t(wave) * (RS_TILE / RS_WAVES) + uint32_t(r) * 64 + uint32_t(lane);
+        if (i < d.count) {
+            const uint32_t pos = my[dr[r] >> 16] + (dr[r] & 0xFFFFu);
+            s_key[pos] = key[r];
+            s_val[pos] = val[r];
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < d.count; j += RS_THREADS) {
+        const K k = s_key[j];
+        const uint32_t dig = digit_of(k, shift, mask);
+        const uint32_t pos = gbase[dig] + (j - dstart[dig]);
+        okeys[pos] = k;
+        ovals[pos] = s_val[j];
+    }
+}
+
+template <class K, class V>
+inline size_t radix_scatter_lds(int bits) {
+    const size_t R = size_t(1) << bits;
+    return RS_TILE * (sizeof(K) + sizeof(V)) + ((2 + RS_WAVES) * R + RS_WAVES + 1) * sizeof(uint32_t);
+}
+
+// Lets k_radix_scatter<K, V> use the dynamic LDS it needs (once per process).
+template <class K, class V>
+inline void radix_scatter_prepare() {
+    static const bool done = [] {
+        CDBG_HIP(hipFuncSetAttribute((const void*)&k_radix_scatter<K, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(radix_scatter_lds<K, V>(11))));
+        return true;
+    }();
+    (void)done;
 }
 
 // Host-side tile plan for a segmented sort.  seg_off has nseg+1 entries.
@@ -330,7 +415,9 @@ inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan&
                                  const char* tag = "") {
     if (plan.n == 0 || begin_bit >= end_bit) return;
     if (plan.n >= (uint64_t(1) << 32)) throw Error("segmented_radix_sort: more than 2^32 elements");
+    if (digit_bits < 1 || digit_bits > 11) throw Error("segmented_radix_sort: digit width must be 1..11 bits");
     const uint32_t nt = uint32_t(plan.tiles.size());
+    radix_scatter_prepare<K, V>();
     TileDesc* d_tiles = S.tiles.ensure<TileDesc>(nt);
     // The previous sort may still read d_tiles: order the upload on the stream
     // and wait for it, since plan.tiles is pageable host memory.
@@ -347,10 +434,10 @@ inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan&
         const uint64_t hl = uint64_t(nt) * R;
         uint32_t* hist = S.hist.ensure<uint32_t>(hl);
         uint32_t* offs = S.offs.ensure<uint32_t>(hl);
-        { KSCOPE(std::string("k_radix_hist:") + tag, st); k_radix_hist<K><<<nt, RS_THREADS, R * sizeof(uint32_t), st>>>(src_k, d_tiles, sh, bits, hist); }
+        { KSCOPE(std::string("k_radix_hist:") + tag, st); k_radix_hist<K><<<xcd_grid(nt), RS_THREADS, R * sizeof(uint32_t), st>>>(src_k, d_tiles, nt, sh, bits, hist); }
         exclusive_scan<uint32_t, uint32_t>(hist, offs, hl, S.total.ensure<uint32_t>(1), S.scan_tmp, st);
-        { KSCOPE(std::string("k_radix_scatter:") + tag, st); k_radix_scatter<K, V><<<nt, RS_THREADS, 5 * R * sizeof(uint32_t), st>>>(src_k, src_v, dst_k, dst_v,
-                                                                              d_tiles, sh, bits, offs); }
+        { KSCOPE(std::string("k_radix_scatter:") + tag, st); k_radix_scatter<K, V><<<xcd_grid(nt), RS_THREADS, radix_scatter_lds<K, V>(bits), st>>>(
+              src_k, src_v, dst_k, dst_v, d_tiles, nt, sh, bits, hist, offs); }
         CDBG_HIP(hipGetLastError());
         std::swap(src_k, dst_k);
         std::swap(src_v, dst_v);
