@@ -858,7 +858,18 @@ struct WalkArgs {
     uint8_t* other_ex;
     uint16_t* pinfo;               // per position: meta | mask << 8
     uint32_t* prank;               // per position: seed rank
+    // per position, for the fragment writer: the node's k-mer and colour-set id
+    const uint32_t* node_cid;
+    const void* node_key;
+    void* pkey;
+    uint32_t* pcid;
 };
+
+template <class K>
+__device__ __forceinline__ void put_position(const WalkArgs& a, uint64_t pos, uint32_t node) {
+    ((K*)a.pkey)[pos] = ((const K*)a.node_key)[node];
+    a.pcid[pos] = a.node_cid[node];
+}
 
 __device__ __forceinline__ bool is_break(uint8_t meta) {
     const uint32_t f = meta & 3u;
@@ -867,6 +878,7 @@ __device__ __forceinline__ bool is_break(uint8_t meta) {
 
 // emit_from_seed (bucket_walker.hpp:451-542) over the link graph; returns the
 // number of nodes placed in the fragment.
+template <class K>
 __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
     a.visited[s] = 1;
     const uint32_t sflags = a.meta[s] & 3u;
@@ -925,11 +937,13 @@ __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
         const uint32_t y = l >> 1, q = l & 1u;
         a.frag_nodes[off + nbk - 1 - i] = y;
         a.frag_rc[off + nbk - 1 - i] = q == PORT_L;
+        put_position<K>(a, off + nbk - 1 - i, y);
         cur = y;
         ex = 1u - q;
     }
     a.frag_nodes[off + nbk] = s;
     a.frag_rc[off + nbk] = 0;
+    put_position<K>(a, off + nbk, s);
     cur = s;
     ex = PORT_R;
     for (uint32_t i = 0; i < nfw; ++i) {  // walking forward: entered at R <=> reverse orientation
@@ -937,6 +951,7 @@ __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
         const uint32_t y = l >> 1, q = l & 1u;
         a.frag_nodes[off + nbk + 1 + i] = y;
         a.frag_rc[off + nbk + 1 + i] = q == PORT_R;
+        put_position<K>(a, off + nbk + 1 + i, y);
         cur = y;
         ex = 1u - q;
     }
@@ -956,13 +971,14 @@ __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
 // A cycle is traversed from x leaving through R until it re-enters x through
 // L; a re-entry through R is a bounce (a hairpin node sends the walk back) and
 // the traversal continues through x's other side.
+template <class K>
 __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, bool cycle, uint32_t len,
                                    uint32_t seed) {
     const uint32_t b = a.b0 + a.node_bucket[x];
     const uint32_t limit = 2 * a.n_nodes + 2;
     uint32_t placed = 0;
     for (uint32_t guard = 0; guard < limit; ++guard) {
-        placed += emit_fragment(a, seed, b);
+        placed += emit_fragment<K>(a, seed, b);
         if (len && placed >= len) return;
         uint32_t best = NONE32, best_rank = 0xFFFFFFFFu;
         uint32_t cur = x, ex = ex0;
@@ -1022,6 +1038,7 @@ __global__ void k_walk_ends(WalkArgs a) {
 // Writes path c end to end at frag_nodes[comp_off[c] ...]: node, its
 // orientation along the path (1 when it is passed entering by R), its meta,
 // mask and rank.
+template <class K>
 __global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= n_comp) return;
@@ -1034,6 +1051,7 @@ __global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
         a.frag_rc[off + i] = o;
         a.pinfo[off + i] = uint16_t(a.meta[cur] | (uint32_t(a.mask[cur]) << 8));
         a.prank[off + i] = a.rank[cur];
+        put_position<K>(a, off + i, cur);
         a.visited[cur] = 1;
         if (i + 1 == L) break;
         const uint32_t l = a.link[2 * cur + ex];
@@ -1110,6 +1128,7 @@ __global__ void k_walk_frags(WalkArgs a, uint64_t n_comp) {
 }
 
 // General seed loop for the queued paths that pass a node twice.
+template <class K>
 __global__ void k_walk_other(WalkArgs a, uint64_t n_other) {
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= n_other) return;
@@ -1127,13 +1146,14 @@ __global__ void k_walk_other(WalkArgs a, uint64_t n_other) {
             seed_rank = r;
         }
     }
-    simulate_component(a, x, ex0, false, 0, seed);
+    simulate_component<K>(a, x, ex0, false, 0, seed);
 }
 
 __global__ void k_set_u64(unsigned long long* p, uint64_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+template <class K>
 __global__ void k_walk_cycles(WalkArgs a) {
     const uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (x >= a.n_nodes || a.visited[x]) return;
@@ -1159,28 +1179,23 @@ __global__ void k_walk_cycles(WalkArgs a) {
             seed_rank = r;
         }
     }
-    simulate_component(a, uint32_t(x), PORT_R, true, simple ? len : 0, seed);
+    simulate_component<K>(a, uint32_t(x), PORT_R, true, simple ? len : 0, seed);
 }
 
 // ---------------------------------------------------------------------------
 // fragment records
 // ---------------------------------------------------------------------------
+// Record bytes of each fragment in output order (unitig_spill.hpp:491-522).
 __global__ void k_frag_size(uint64_t n_frags, const uint32_t* __restrict__ order, const uint64_t* __restrict__ frag_off,
-                            const uint32_t* __restrict__ frag_n, const uint8_t* __restrict__ frag_dir,
-                            const uint32_t* __restrict__ frag_nodes, const uint32_t* __restrict__ node_cid, uint32_t k,
+                            const uint32_t* __restrict__ frag_n, const uint32_t* __restrict__ pcid, uint32_t k,
                             uint64_t* __restrict__ size) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_frags) return;
     const uint32_t f = order[i];
     const uint64_t o = frag_off[f];
     const uint32_t n = frag_n[f];
-    const bool rev = frag_dir[f];
-    uint32_t runs = 0, last = 0;
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t c = node_cid[frag_nodes[o + (rev ? n - 1 - j : j)]];
-        if (j == 0 || c != last) ++runs;
-        last = c;
-    }
+    uint32_t runs = 1;  // run count is the same read either way
+    for (uint32_t j = 1; j < n; ++j) runs += pcid[o + j] != pcid[o + j - 1];
     size[i] = 1 + 4 + 12ull * runs + 4 + (k + n - 1);
 }
 
@@ -1188,26 +1203,18 @@ __device__ __forceinline__ void put_bytes(uint8_t* p, uint64_t v, int n) {
     for (int i = 0; i < n; ++i) p[i] = uint8_t(v >> (8 * i));
 }
 
+// `[u8 open][u32 nruns]{[u64 cid][u32 n]}[u32 seq_len][seq]`, the node range
+// [o, o + n) read forwards or (rev) backwards with orientations flipped.
 template <class K>
-__global__ void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ order, const uint64_t* __restrict__ frag_off,
-                             const uint32_t* __restrict__ frag_n, const uint8_t* __restrict__ frag_open,
-                             const uint8_t* __restrict__ frag_dir,
-                             const uint32_t* __restrict__ frag_nodes, const uint8_t* __restrict__ frag_rc,
-                             const uint32_t* __restrict__ node_cid, const K* __restrict__ node_key, uint32_t k,
-                             const uint64_t* __restrict__ boff, uint8_t* __restrict__ out) {
+__device__ void write_record(uint8_t* p, uint8_t open, uint64_t o, uint32_t n, uint32_t rev,
+                             const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
+                             const uint32_t* __restrict__ pcid, uint32_t k) {
     using O = KOps<K>;
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_frags) return;
-    const uint32_t f = order[i];
-    const uint64_t o = frag_off[f];
-    const uint32_t n = frag_n[f];
-    const uint32_t rev = frag_dir[f];
-    uint8_t* p = out + boff[i];
-    p[0] = frag_open[f];
+    p[0] = open;
     uint8_t* pr = p + 5;
     uint32_t runs = 0, last = 0, cnt = 0;
     for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t c = node_cid[frag_nodes[o + (rev ? n - 1 - j : j)]];
+        const uint32_t c = pcid[o + (rev ? n - 1 - j : j)];
         if (j > 0 && c != last) {
             put_bytes(pr, last == NONE32 ? ~0ull : uint64_t(last), 8);
             put_bytes(pr + 8, cnt, 4);
@@ -1229,15 +1236,53 @@ __global__ void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ orde
     const char ACGT[4] = {'A', 'C', 'G', 'T'};
     for (uint32_t j = 0; j < n; ++j) {
         const uint64_t at = o + (rev ? n - 1 - j : j);
-        const uint32_t nd = frag_nodes[at];
-        const K key = node_key[nd];
-        const K km = ((frag_rc[at] & 1u) ^ rev) ? O::rc(key, k) : key;
+        const K key = pkey[at];
+        const K km = ((frc[at] & 1u) ^ rev) ? O::rc(key, k) : key;
         if (j == 0) {
             for (uint32_t t = 0; t < k; ++t) ps[t] = ACGT[O::base(km, t, k)];
         } else {
             ps[k + j - 1] = ACGT[O::base(km, k - 1, k)];
         }
     }
+}
+
+// A block writes the records of TPB consecutive fragments (in output order),
+// staged in LDS when they fit and then copied out with aligned 32-bit stores.
+constexpr uint32_t FW_LDS = 32768;
+template <class K>
+__global__ __launch_bounds__(TPB) void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ order,
+                                                    const uint64_t* __restrict__ frag_off,
+                                                    const uint32_t* __restrict__ frag_n,
+                                                    const uint8_t* __restrict__ frag_open,
+                                                    const uint8_t* __restrict__ frag_dir,
+                                                    const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
+                                                    const uint32_t* __restrict__ pcid, uint32_t k,
+                                                    const uint64_t* __restrict__ boff, uint8_t* __restrict__ out) {
+    __shared__ uint8_t s_out[FW_LDS];
+    const uint64_t i0 = uint64_t(blockIdx.x) * TPB;
+    const uint64_t i1 = min(i0 + TPB, n_frags);
+    const uint64_t b0 = boff[i0], b1 = boff[i1];
+    const bool staged = b1 - b0 <= FW_LDS;
+    const uint64_t i = i0 + threadIdx.x;
+    if (i < i1) {
+        const uint32_t f = order[i];
+        uint8_t* p = staged ? s_out + (boff[i] - b0) : out + boff[i];
+        write_record<K>(p, frag_open[f], frag_off[f], frag_n[f], frag_dir[f], frc, pkey, pcid, k);
+    }
+    if (!staged) return;
+    __syncthreads();
+    const uint32_t len = uint32_t(b1 - b0);
+    const uint32_t mis = uint32_t((4 - (reinterpret_cast<uintptr_t>(out + b0) & 3)) & 3);
+    const uint32_t head = mis < len ? mis : len;
+    if (threadIdx.x < head) out[b0 + threadIdx.x] = s_out[threadIdx.x];
+    const uint32_t words = (len - head) / 4;
+    uint32_t* ow = (uint32_t*)(out + b0 + head);
+    for (uint32_t w = threadIdx.x; w < words; w += TPB) {
+        const uint8_t* q = s_out + head + 4 * w;
+        ow[w] = uint32_t(q[0]) | (uint32_t(q[1]) << 8) | (uint32_t(q[2]) << 16) | (uint32_t(q[3]) << 24);
+    }
+    const uint32_t tail = head + 4 * words;
+    if (threadIdx.x < len - tail) out[b0 + tail + threadIdx.x] = s_out[tail + threadIdx.x];
 }
 
 }  // namespace
@@ -1258,7 +1303,7 @@ struct Phase2Bufs {
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
     DevBuf node_cid, node_bucket, tab_off, tab, node_of, node_mask, nb, link, visited;
     DevBuf frag_nodes, frag_rc, frag_off, frag_n, frag_open, frag_key, frag_dir;
-    DevBuf comp_x, comp_ex, comp_len, comp_off, other_x, other_ex, pinfo, prank;
+    DevBuf comp_x, comp_ex, comp_len, comp_off, other_x, other_ex, pinfo, prank, pkey, pcid;
     DevBuf fkey_t, forder, forder_t, fsize, fboff;
     RadixScratch rs2;
     // results
@@ -1493,6 +1538,12 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.other_ex = P.other_ex.ensure<uint8_t>(nn);
     wa.pinfo = P.pinfo.ensure<uint16_t>(nn);
     wa.prank = P.prank.ensure<uint32_t>(nn);
+    wa.node_cid = node_cid;
+    wa.node_key = node_key;
+    K* pkey = P.pkey.ensure<K>(nn);
+    uint32_t* pcid = P.pcid.ensure<uint32_t>(nn);
+    wa.pkey = pkey;
+    wa.pcid = pcid;
     { KSCOPE("k_walk_ends", st); k_walk_ends<<<blocks_for(nn), TPB, 0, st>>>(wa); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
@@ -1501,10 +1552,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint32_t path_nodes = n_comp ? d2h<uint32_t>(comp_off + n_comp, st) : 0u;
     // the general seed loop places its nodes after the recorded paths
     { KSCOPE("k_set_u64", st); k_set_u64<<<1, 64, 0, st>>>(cnt + 3, path_nodes); }
-    if (n_comp) { KSCOPE("k_walk_record", st); k_walk_record<<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
+    if (n_comp) { KSCOPE("k_walk_record", st); k_walk_record<K><<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
     if (n_comp) { KSCOPE("k_walk_frags", st); k_walk_frags<<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
-    if (n_other) { KSCOPE("k_walk_other", st); k_walk_other<<<blocks_for(n_other), TPB, 0, st>>>(wa, n_other); }
-    { KSCOPE("k_walk_cycles", st); k_walk_cycles<<<blocks_for(nn), TPB, 0, st>>>(wa); }
+    if (n_other) { KSCOPE("k_walk_other", st); k_walk_other<K><<<blocks_for(n_other), TPB, 0, st>>>(wa, n_other); }
+    { KSCOPE("k_walk_cycles", st); k_walk_cycles<K><<<blocks_for(nn), TPB, 0, st>>>(wa); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     if (h_cnt[6]) throw Error("phase 2: colour-set content hash collision (" + std::to_string(h_cnt[6]) + " nodes)");
@@ -1524,11 +1575,11 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
                                              "frags");
     uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
     uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
-    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_dir, wa.frag_nodes, node_cid, k, fsize); }
+    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, pcid, k, fsize); }
     exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
     const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
     uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st);
-    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_dir, wa.frag_nodes, wa.frag_rc, node_cid, node_key, k, fboff, fout + P.frag_total); }
+    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_dir, wa.frag_rc, pkey, pcid, k, fboff, fout + P.frag_total); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));

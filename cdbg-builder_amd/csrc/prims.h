@@ -207,7 +207,7 @@ inline void exclusive_scan(const Tin* in, Tout* out, uint64_t n, Tout* d_total, 
 // ---------------------------------------------------------------------------
 // Segmented stable LSD radix sort.
 // ---------------------------------------------------------------------------
-constexpr int RS_THREADS = 256;
+constexpr int RS_THREADS = 512;
 constexpr int RS_TILE = 4096;
 
 struct TileDesc {
@@ -365,15 +365,18 @@ inline size_t radix_scatter_lds(int bits) {
     return RS_TILE * (sizeof(K) + sizeof(V)) + ((2 + RS_WAVES) * R + RS_WAVES + 1) * sizeof(uint32_t);
 }
 
-// Lets k_radix_scatter<K, V> use the dynamic LDS it needs (once per process).
+// Lets k_radix_scatter<K, V> use the dynamic LDS a pass of `bits`-bit digits
+// needs (raised once per size, up to the 160 KiB of a gfx950 workgroup).
 template <class K, class V>
-inline void radix_scatter_prepare() {
-    static const bool done = [] {
+inline void radix_scatter_prepare(int bits) {
+    static size_t granted = 0;
+    const size_t need = radix_scatter_lds<K, V>(bits);
+    if (need > 160 * 1024) throw Error("segmented_radix_sort: digit width too large for LDS");
+    if (need > granted) {
         CDBG_HIP(hipFuncSetAttribute((const void*)&k_radix_scatter<K, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     int(radix_scatter_lds<K, V>(11))));
-        return true;
-    }();
-    (void)done;
+                                     int(need)));
+        granted = need;
+    }
 }
 
 // Host-side tile plan for a segmented sort.  seg_off has nseg+1 entries.
@@ -417,7 +420,6 @@ inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan&
     if (plan.n >= (uint64_t(1) << 32)) throw Error("segmented_radix_sort: more than 2^32 elements");
     if (digit_bits < 1 || digit_bits > 11) throw Error("segmented_radix_sort: digit width must be 1..11 bits");
     const uint32_t nt = uint32_t(plan.tiles.size());
-    radix_scatter_prepare<K, V>();
     TileDesc* d_tiles = S.tiles.ensure<TileDesc>(nt);
     // The previous sort may still read d_tiles: order the upload on the stream
     // and wait for it, since plan.tiles is pageable host memory.
@@ -431,6 +433,7 @@ inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan&
     for (int sh = begin_bit; sh < end_bit; sh += digit_bits) {
         const int bits = (end_bit - sh) < digit_bits ? (end_bit - sh) : digit_bits;
         const uint32_t R = 1u << bits;
+        radix_scatter_prepare<K, V>(bits);
         const uint64_t hl = uint64_t(nt) * R;
         uint32_t* hist = S.hist.ensure<uint32_t>(hl);
         uint32_t* offs = S.offs.ensure<uint32_t>(hl);
