@@ -713,14 +713,20 @@ __global__ void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_bas
     }
 }
 
+template <class K>
+struct NodeRec;
+
+// Colour-set id of every node, stored with its k-mer and seed rank.
+template <class K>
 __global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
                            const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
-                           const uint32_t* __restrict__ node_slot, SetTables T, uint32_t* __restrict__ node_cid) {
+                           const uint32_t* __restrict__ node_slot, SetTables T, const K* __restrict__ node_key,
+                           const uint32_t* __restrict__ rank, NodeRec<K>* __restrict__ rec) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     uint32_t cid = NONE32;
     if (meta[n] & 4u) cid = ncol[n] == 1 ? T.single_cid[colors[coff[n]]] : T.mt_cid[node_slot[n]];
-    node_cid[n] = cid;
+    rec[n] = NodeRec<K>{node_key[n], rank[n], cid};
 }
 
 // ---------------------------------------------------------------------------
@@ -778,8 +784,8 @@ __device__ __forceinline__ uint64_t tab_find(const K* __restrict__ tab, uint64_t
 template <class K>
 __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
                         const uint64_t* __restrict__ tab_off, const K* __restrict__ tab,
-                        const uint32_t* __restrict__ node_of, uint32_t k, uint8_t* __restrict__ mask_out,
-                        uint32_t* __restrict__ nb) {
+                        const uint32_t* __restrict__ node_of, uint32_t k, const uint8_t* __restrict__ meta,
+                        uint8_t* __restrict__ mask_out, uint16_t* __restrict__ node_mm, uint32_t* __restrict__ nb) {
     using O = KOps<K>;
     constexpr uint32_t G = Tab<K>::G;
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -812,6 +818,7 @@ __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const 
         }
     }
     mask_out[n] = uint8_t(mask);
+    node_mm[n] = uint16_t(meta[n] | (mask << 8));
     nb[2 * n + PORT_R] = __popc(mask & 0xfu) == 1 ? (node_of[sR] << 1) | qR : NONE32;
     nb[2 * n + PORT_L] = __popc(mask >> 4) == 1 ? (node_of[sL] << 1) | qL : NONE32;
 }
@@ -841,39 +848,68 @@ struct WalkArgs {
     const uint32_t* node_bucket;
     uint32_t b0;
     uint8_t* visited;
-    uint32_t* frag_nodes;
-    uint8_t* frag_rc;
-    uint64_t* frag_off;
+    unsigned long long* counters;  // [3] positions placed by the general loop, [5] errors, [9] paths, [10] other
+    // fragment descriptors, indexed by the fragment's first position
     uint32_t* frag_n;
     uint8_t* frag_open;
     uint64_t* frag_key;
-    uint8_t* frag_dir;             // 1: the fragment is its node range read backwards
-    unsigned long long* counters;  // [3] nodes placed, [4] fragments, [5] errors, [9] paths, [10] other paths
+    uint8_t* frag_dir;             // 1: the fragment is its position range read backwards
+    uint32_t* fflag;               // 1 at the first position of a fragment
     // simple paths, recorded end to end
     uint32_t* comp_x;              // owning end
     uint8_t* comp_ex;              // port the path leaves it by
     uint32_t* comp_len;
-    const uint32_t* comp_off;      // first position in frag_nodes
+    uint32_t* comp_nodes;          // [PATH_BUF] per path: node | orientation << 31 (short paths)
+    const uint32_t* comp_off;      // first position
     uint32_t* other_x;             // paths that pass a node twice (hairpins)
     uint8_t* other_ex;
-    uint16_t* pinfo;               // per position: meta | mask << 8
-    uint32_t* prank;               // per position: seed rank
-    // per position, for the fragment writer: the node's k-mer and colour-set id
-    const uint32_t* node_cid;
-    const void* node_key;
-    void* pkey;
-    uint32_t* pcid;
+    // per position
+    uint8_t* prc;                  // bit 0: orientation, bit 1: placed
+    uint16_t* pinfo;               // meta | mask << 8
+    uint32_t* prank;
+    void* pkey;                    // the node's k-mer
+    uint32_t* pcid;                // the node's colour-set id
+    const void* node_rec;          // NodeRec<K> per node
+    const uint16_t* node_mm;       // meta | mask << 8 per node
 };
 
 template <class K>
-__device__ __forceinline__ void put_position(const WalkArgs& a, uint64_t pos, uint32_t node) {
-    ((K*)a.pkey)[pos] = ((const K*)a.node_key)[node];
-    a.pcid[pos] = a.node_cid[node];
+struct NodeRec {
+    K key;
+    uint32_t rank;
+    uint32_t cid;
+};
+
+constexpr uint32_t PATH_BUF = 8;
+
+template <class K>
+__device__ __forceinline__ void put_position(const WalkArgs& a, uint64_t pos, uint32_t node, uint8_t o) {
+    const NodeRec<K> r = ((const NodeRec<K>*)a.node_rec)[node];
+    ((K*)a.pkey)[pos] = r.key;
+    a.pcid[pos] = r.cid;
+    a.prank[pos] = r.rank;
+    a.pinfo[pos] = a.node_mm[node];
+    a.prc[pos] = o;
+    a.fflag[pos] = 0;
 }
 
 __device__ __forceinline__ bool is_break(uint8_t meta) {
     const uint32_t f = meta & 3u;
     return f == BOUND_LEFT || f == BOUND_RIGHT;
+}
+
+// Index of this lane's item among the wave's requesting lanes, added to a
+// counter once per wave.  Every lane of the wave that is still running must
+// call it.
+__device__ __forceinline__ uint64_t wave_alloc(unsigned long long* ctr, bool want) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = __ballot(want);
+    if (!m) return 0;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(m & lanes_below(lane));
 }
 
 // emit_from_seed (bucket_walker.hpp:451-542) over the link graph; returns the
@@ -926,7 +962,6 @@ __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
     }
     const uint32_t total = nbk + 1 + nfw;
     const uint64_t off = atomicAdd(&a.counters[3], (unsigned long long)total);
-    const uint64_t fi = atomicAdd(&a.counters[4], 1ull);
     if (off + total > a.n_nodes) {
         atomicAdd(&a.counters[5], 1ull);
         return total;
@@ -935,31 +970,25 @@ __device__ uint32_t emit_fragment(const WalkArgs& a, uint32_t s, uint32_t b) {
     for (uint32_t i = 0; i < nbk; ++i) {  // walking backward: entered at L <=> reverse orientation
         const uint32_t l = a.link[2 * cur + ex];
         const uint32_t y = l >> 1, q = l & 1u;
-        a.frag_nodes[off + nbk - 1 - i] = y;
-        a.frag_rc[off + nbk - 1 - i] = q == PORT_L;
-        put_position<K>(a, off + nbk - 1 - i, y);
+        put_position<K>(a, off + nbk - 1 - i, y, q == PORT_L);
         cur = y;
         ex = 1u - q;
     }
-    a.frag_nodes[off + nbk] = s;
-    a.frag_rc[off + nbk] = 0;
-    put_position<K>(a, off + nbk, s);
+    put_position<K>(a, off + nbk, s, 0);
     cur = s;
     ex = PORT_R;
     for (uint32_t i = 0; i < nfw; ++i) {  // walking forward: entered at R <=> reverse orientation
         const uint32_t l = a.link[2 * cur + ex];
         const uint32_t y = l >> 1, q = l & 1u;
-        a.frag_nodes[off + nbk + 1 + i] = y;
-        a.frag_rc[off + nbk + 1 + i] = q == PORT_R;
-        put_position<K>(a, off + nbk + 1 + i, y);
+        put_position<K>(a, off + nbk + 1 + i, y, q == PORT_R);
         cur = y;
         ex = 1u - q;
     }
-    a.frag_off[fi] = off;
-    a.frag_n[fi] = total;
-    a.frag_open[fi] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
-    a.frag_key[fi] = (uint64_t(b) << 32) | a.rank[s];
-    a.frag_dir[fi] = 0;
+    a.fflag[off] = 1;
+    a.frag_n[off] = total;
+    a.frag_open[off] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
+    a.frag_key[off] = (uint64_t(b) << 32) | a.rank[s];
+    a.frag_dir[off] = 0;
     return total;
 }
 
@@ -1001,57 +1030,74 @@ __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, 
 }
 
 // Path ends walk to the other end; the end with the smaller index owns the
-// path.  Paths that never pass a node twice are queued for recording, the
-// others (a node linked to itself turns the walk back) for the general seed
-// loop.
-__global__ void k_walk_ends(WalkArgs a) {
+// path.  Paths that never pass a node twice are queued for recording (with
+// their nodes when they have at most PATH_BUF), the others (a node linked to
+// itself turns the walk back) for the general seed loop.
+__global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a) {
     const uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (x >= a.n_nodes) return;
-    const uint32_t lL = a.link[2 * x + PORT_L], lR = a.link[2 * x + PORT_R];
-    if (lL != NONE32 && lR != NONE32) return;  // not a path end
-    const uint32_t ex0 = lR == NONE32 ? PORT_L : PORT_R;
-    uint32_t cur = uint32_t(x), ex = ex0, len = 1;
-    bool simple = true;
-    const uint32_t limit = 2 * a.n_nodes + 2;
-    for (uint32_t h = 0; h < limit; ++h) {
-        const uint32_t l = a.link[2 * cur + ex];
-        if (l == NONE32) break;
-        const uint32_t y = l >> 1;
-        if (y == cur) simple = false;  // hairpin: the walk comes back
-        cur = y;
-        ex = 1u - (l & 1u);
-        ++len;
+    bool owner = false, simple = true;
+    uint32_t ex0 = 0, len = 0;
+    uint32_t buf[PATH_BUF];
+    if (x < a.n_nodes) {
+        const uint32_t lL = a.link[2 * x + PORT_L], lR = a.link[2 * x + PORT_R];
+        if (lL == NONE32 || lR == NONE32) {  // a path end
+            ex0 = lR == NONE32 ? PORT_L : PORT_R;
+            uint32_t cur = uint32_t(x), ex = ex0;
+            buf[0] = cur | (ex0 == PORT_L ? 0x80000000u : 0u);
+            len = 1;
+            const uint32_t limit = 2 * a.n_nodes + 2;
+            for (uint32_t h = 0; h < limit; ++h) {
+                const uint32_t l = a.link[2 * cur + ex];
+                if (l == NONE32) break;
+                const uint32_t y = l >> 1, q = l & 1u;
+                if (y == cur) simple = false;  // hairpin: the walk comes back
+                cur = y;
+                ex = 1u - q;
+#pragma unroll
+                for (uint32_t j = 1; j < PATH_BUF; ++j)
+                    if (j == len) buf[j] = cur | (q == PORT_R ? 0x80000000u : 0u);
+                ++len;
+            }
+            owner = cur >= x;
+        }
     }
-    if (cur < x) return;
-    if (simple) {
-        const uint64_t c = atomicAdd(&a.counters[9], 1ull);
+    const uint64_t c = wave_alloc(&a.counters[9], owner && simple);
+    const uint64_t o = wave_alloc(&a.counters[10], owner && !simple);
+    if (owner && simple) {
         a.comp_x[c] = uint32_t(x);
         a.comp_ex[c] = uint8_t(ex0);
         a.comp_len[c] = len;
-    } else {
-        const uint64_t c = atomicAdd(&a.counters[10], 1ull);
-        a.other_x[c] = uint32_t(x);
-        a.other_ex[c] = uint8_t(ex0);
+        if (len <= PATH_BUF)
+#pragma unroll
+            for (uint32_t j = 0; j < PATH_BUF; ++j)
+                if (j < len) a.comp_nodes[c * PATH_BUF + j] = buf[j];
+    } else if (owner) {
+        a.other_x[o] = uint32_t(x);
+        a.other_ex[o] = uint8_t(ex0);
     }
 }
 
-// Writes path c end to end at frag_nodes[comp_off[c] ...]: node, its
-// orientation along the path (1 when it is passed entering by R), its meta,
-// mask and rank.
+// Writes path c end to end from position comp_off[c]: per position the
+// node's data and its orientation along the path (1 when it is passed
+// entering by R).
 template <class K>
 __global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= n_comp) return;
     const uint32_t L = a.comp_len[c];
     const uint64_t off = a.comp_off[c];
+    if (L <= PATH_BUF) {
+        for (uint32_t i = 0; i < L; ++i) {
+            const uint32_t v = a.comp_nodes[c * PATH_BUF + i];
+            put_position<K>(a, off + i, v & 0x7FFFFFFFu, uint8_t(v >> 31));
+            a.visited[v & 0x7FFFFFFFu] = 1;
+        }
+        return;
+    }
     uint32_t cur = a.comp_x[c], ex = a.comp_ex[c];
     uint8_t o = ex == PORT_L ? 1 : 0;
     for (uint32_t i = 0;; ++i) {
-        a.frag_nodes[off + i] = cur;
-        a.frag_rc[off + i] = o;
-        a.pinfo[off + i] = uint16_t(a.meta[cur] | (uint32_t(a.mask[cur]) << 8));
-        a.prank[off + i] = a.rank[cur];
-        put_position<K>(a, off + i, cur);
+        put_position<K>(a, off + i, cur, o);
         a.visited[cur] = 1;
         if (i + 1 == L) break;
         const uint32_t l = a.link[2 * cur + ex];
@@ -1066,16 +1112,16 @@ __global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
 // the unplaced positions in rank order; from a seed the fragment grows
 // backward then forward along the path (positions -1 / +1 when the seed lies
 // forward on the path, +1 / -1 otherwise), stopping before a placed position
-// or after a contig-break node.  Bit 1 of frag_rc marks placed positions.
+// or after a contig-break node.
 __global__ void k_walk_frags(WalkArgs a, uint64_t n_comp) {
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= n_comp) return;
     const uint32_t L = a.comp_len[c];
     const uint64_t off = a.comp_off[c];
-    uint8_t* rc = a.frag_rc + off;
+    uint8_t* rc = a.prc + off;
     const uint16_t* info = a.pinfo + off;
     const uint32_t* rk = a.prank + off;
-    const uint32_t b = a.b0 + a.node_bucket[a.frag_nodes[off]];
+    const uint32_t b = a.b0 + a.node_bucket[a.comp_x[c]];
     uint32_t placed = 0;
     while (placed < L) {
         uint32_t p = NONE32, best = 0xFFFFFFFFu;
@@ -1118,12 +1164,12 @@ __global__ void k_walk_frags(WalkArgs a, uint64_t n_comp) {
         }
         const uint32_t n = uint32_t(hi - lo + 1);
         placed += n;
-        const uint64_t fi = atomicAdd(&a.counters[4], 1ull);
-        a.frag_off[fi] = off + uint32_t(lo);
-        a.frag_n[fi] = n;
-        a.frag_dir[fi] = uint8_t(o);
-        a.frag_open[fi] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
-        a.frag_key[fi] = (uint64_t(b) << 32) | best;
+        const uint64_t f = off + uint32_t(lo);
+        a.fflag[f] = 1;
+        a.frag_n[f] = n;
+        a.frag_dir[f] = uint8_t(o);
+        a.frag_open[f] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
+        a.frag_key[f] = (uint64_t(b) << 32) | best;
     }
 }
 
@@ -1182,17 +1228,28 @@ __global__ void k_walk_cycles(WalkArgs a) {
     simulate_component<K>(a, uint32_t(x), PORT_R, true, simple ? len : 0, seed);
 }
 
+// Dense fragment list from the first-position flags: fkey[d] / forder[d] =
+// the key and first position of the d-th fragment in position order.
+__global__ void k_frag_compact(uint64_t n, const uint32_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
+                               const uint64_t* __restrict__ frag_key, uint64_t* __restrict__ fkey,
+                               uint32_t* __restrict__ forder) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || !fflag[i]) return;
+    const uint32_t d = fscan[i];
+    fkey[d] = frag_key[i];
+    forder[d] = uint32_t(i);
+}
+
 // ---------------------------------------------------------------------------
 // fragment records
 // ---------------------------------------------------------------------------
 // Record bytes of each fragment in output order (unitig_spill.hpp:491-522).
-__global__ void k_frag_size(uint64_t n_frags, const uint32_t* __restrict__ order, const uint64_t* __restrict__ frag_off,
-                            const uint32_t* __restrict__ frag_n, const uint32_t* __restrict__ pcid, uint32_t k,
-                            uint64_t* __restrict__ size) {
+__global__ void k_frag_size(uint64_t n_frags, const uint32_t* __restrict__ order, const uint32_t* __restrict__ frag_n,
+                            const uint32_t* __restrict__ pcid, uint32_t k, uint64_t* __restrict__ size) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_frags) return;
-    const uint32_t f = order[i];
-    const uint64_t o = frag_off[f];
+    const uint32_t f = order[i];  // the fragment's first position
+    const uint64_t o = f;
     const uint32_t n = frag_n[f];
     uint32_t runs = 1;  // run count is the same read either way
     for (uint32_t j = 1; j < n; ++j) runs += pcid[o + j] != pcid[o + j - 1];
@@ -1251,7 +1308,6 @@ __device__ void write_record(uint8_t* p, uint8_t open, uint64_t o, uint32_t n, u
 constexpr uint32_t FW_LDS = 32768;
 template <class K>
 __global__ __launch_bounds__(TPB) void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ order,
-                                                    const uint64_t* __restrict__ frag_off,
                                                     const uint32_t* __restrict__ frag_n,
                                                     const uint8_t* __restrict__ frag_open,
                                                     const uint8_t* __restrict__ frag_dir,
@@ -1267,7 +1323,7 @@ __global__ __launch_bounds__(TPB) void k_frag_write(uint64_t n_frags, const uint
     if (i < i1) {
         const uint32_t f = order[i];
         uint8_t* p = staged ? s_out + (boff[i] - b0) : out + boff[i];
-        write_record<K>(p, frag_open[f], frag_off[f], frag_n[f], frag_dir[f], frc, pkey, pcid, k);
+        write_record<K>(p, frag_open[f], f, frag_n[f], frag_dir[f], frc, pkey, pcid, k);
     }
     if (!staged) return;
     __syncthreads();
@@ -1301,9 +1357,10 @@ struct Phase2Bufs {
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
-    DevBuf node_cid, node_bucket, tab_off, tab, node_of, node_mask, nb, link, visited;
-    DevBuf frag_nodes, frag_rc, frag_off, frag_n, frag_open, frag_key, frag_dir;
-    DevBuf comp_x, comp_ex, comp_len, comp_off, other_x, other_ex, pinfo, prank, pkey, pcid;
+    DevBuf node_bucket, tab_off, tab, node_of, node_mask, nb, link, visited;
+    DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fscan, fkey;
+    DevBuf comp_x, comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
+    DevBuf node_rec, node_mm;
     DevBuf fkey_t, forder, forder_t, fsize, fboff;
     RadixScratch rs2;
     // results
@@ -1478,8 +1535,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     }
     uint64_t* set_off = P.set_off.ensure_keep<uint64_t>(cid_base + n_new, cid_base * 8, st);
     if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
-    uint32_t* node_cid = P.node_cid.ensure<uint32_t>(nn);
-    { KSCOPE("k_node_cid", st); k_node_cid<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, node_cid); }
+    NodeRec<K>* node_rec = P.node_rec.ensure<NodeRec<K>>(nn);
+    { KSCOPE("k_node_cid", st); k_node_cid<K><<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, node_key, rank, node_rec); }
     P.n_sets += n_new;
     P.cs_bits += new_bits;
     P.mt_count += n_new;  // upper bound of occupied slots (singletons included): sizes the table
@@ -1506,7 +1563,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
     uint32_t* nbr = P.nb.ensure<uint32_t>(2ull * nn);
     uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
-    { KSCOPE("k_masks", st); k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, node_mask, nbr); }
+    uint16_t* node_mm = P.node_mm.ensure<uint16_t>(nn);
+    { KSCOPE("k_masks", st); k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
     { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link); }
 
     // --- walk
@@ -1521,29 +1579,31 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.node_bucket = node_bucket;
     wa.b0 = b0;
     wa.visited = visited;
-    wa.frag_nodes = P.frag_nodes.ensure<uint32_t>(nn);
-    wa.frag_rc = P.frag_rc.ensure<uint8_t>(nn);
-    wa.frag_off = P.frag_off.ensure<uint64_t>(nn);
+    wa.counters = cnt;
     wa.frag_n = P.frag_n.ensure<uint32_t>(nn);
     wa.frag_open = P.frag_open.ensure<uint8_t>(nn);
     wa.frag_key = P.frag_key.ensure<uint64_t>(nn);
     wa.frag_dir = P.frag_dir.ensure<uint8_t>(nn);
-    wa.counters = cnt;
+    uint32_t* fflag = P.fflag.ensure<uint32_t>(nn);
+    wa.fflag = fflag;
     wa.comp_x = P.comp_x.ensure<uint32_t>(nn);
     wa.comp_ex = P.comp_ex.ensure<uint8_t>(nn);
     wa.comp_len = P.comp_len.ensure<uint32_t>(nn);
+    wa.comp_nodes = P.comp_nodes.ensure<uint32_t>(uint64_t(nn) * PATH_BUF);
     uint32_t* comp_off = P.comp_off.ensure<uint32_t>(uint64_t(nn) + 1);
     wa.comp_off = comp_off;
     wa.other_x = P.other_x.ensure<uint32_t>(nn);
     wa.other_ex = P.other_ex.ensure<uint8_t>(nn);
+    uint8_t* prc = P.prc.ensure<uint8_t>(nn);
+    wa.prc = prc;
     wa.pinfo = P.pinfo.ensure<uint16_t>(nn);
     wa.prank = P.prank.ensure<uint32_t>(nn);
-    wa.node_cid = node_cid;
-    wa.node_key = node_key;
     K* pkey = P.pkey.ensure<K>(nn);
     uint32_t* pcid = P.pcid.ensure<uint32_t>(nn);
     wa.pkey = pkey;
     wa.pcid = pcid;
+    wa.node_rec = node_rec;
+    wa.node_mm = node_mm;
     { KSCOPE("k_walk_ends", st); k_walk_ends<<<blocks_for(nn), TPB, 0, st>>>(wa); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
@@ -1562,24 +1622,27 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (h_cnt[5] || h_cnt[3] != nn)
         throw Error("phase 2: walk covered " + std::to_string(h_cnt[3]) + " of " + std::to_string(nn) +
                     " nodes (" + std::to_string(h_cnt[5]) + " errors)");
-    const uint64_t nf = h_cnt[4];
+    uint32_t* fscan = P.fscan.ensure<uint32_t>(uint64_t(nn) + 1);
+    exclusive_scan<uint32_t, uint32_t>(fflag, fscan, nn, fscan + nn, J.rs.scan_tmp, st);
+    const uint64_t nf = d2h<uint32_t>(fscan + nn, st);
 
     // --- fragments in (bucket, seed rank) order, appended to the job stream
+    uint64_t* fkey = P.fkey.ensure<uint64_t>(nf);
     uint64_t* fkey_t = P.fkey_t.ensure<uint64_t>(nf);
     uint32_t* forder = P.forder.ensure<uint32_t>(nf);
     uint32_t* forder_t = P.forder_t.ensure<uint32_t>(nf);
-    { KSCOPE("k_iota", st); k_iota<<<blocks_for(nf), TPB, 0, st>>>(forder, nf); }
+    { KSCOPE("k_frag_compact", st); k_frag_compact<<<blocks_for(nn), TPB, 0, st>>>(nn, fflag, fscan, wa.frag_key, fkey, forder); }
     SortPlan fplan;
     fplan.build({0, nf});
-    segmented_radix_sort<uint64_t, uint32_t>(wa.frag_key, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
+    segmented_radix_sort<uint64_t, uint32_t>(fkey, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
                                              "frags");
     uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
     uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
-    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, pcid, k, fsize); }
+    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, pcid, k, fsize); }
     exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
     const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
     uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st);
-    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_off, wa.frag_n, wa.frag_open, wa.frag_dir, wa.frag_rc, pkey, pcid, k, fboff, fout + P.frag_total); }
+    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, fboff, fout + P.frag_total); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
@@ -1641,13 +1704,13 @@ static void phase2_impl(Job& J) {
     if (budget == 0) {
         size_t free_b = 0, total_b = 0;
         CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
-        // ~130 B per occurrence covers tuples, sort buffers, nodes, table,
+        // ~160 B per occurrence covers tuples, sort buffers, nodes, table,
         // walk and fragment arrays; buffers this phase already holds from an
         // earlier run count as available; 40% is kept for the growing output
         // streams.
         const uint64_t held = g_device_bytes.load();
         const uint64_t usable = uint64_t(free_b) + std::min<uint64_t>(held, uint64_t(free_b)) / 4;
-        budget = std::max<uint64_t>(1u << 20, uint64_t(0.6 * double(usable)) / 130);
+        budget = std::max<uint64_t>(1u << 20, uint64_t(0.6 * double(usable)) / 160);
         budget = std::min<uint64_t>(budget, (1ull << 31) - 1);
         J.auto_chunk = budget;  // later runs of this job plan the same chunks
     }
