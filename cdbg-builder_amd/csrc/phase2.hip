@@ -198,39 +198,26 @@ struct ExpArgs {
     const uint32_t* rank0;  // per record: stream rank of its first k-mer (null: records are the stream)
 };
 
+// First super-k-mer of every expand block: block b starts at chunk tuple
+// b * TPB; block_super[n_blocks] = s1 - 1 closes the table.
+__global__ void k_block_super(uint64_t s0, uint64_t s1, uint64_t t0, uint64_t n_blocks,
+                              const uint64_t* __restrict__ tup_off, uint32_t* __restrict__ block_super) {
+    const uint64_t s = s0 + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= s1) return;
+    const uint64_t lo = tup_off[s] - t0, hi = tup_off[s + 1] - t0;  // local tuples [lo, hi)
+    for (uint64_t b = (lo + TPB - 1) / TPB; b * TPB < hi && b < n_blocks; ++b) block_super[b] = uint32_t(s - s0);
+    if (s == s1 - 1) block_super[n_blocks] = uint32_t(s - s0);
+}
+
 template <class K>
-__global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, K* __restrict__ keys, uint64_t* __restrict__ vals) {
+__global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __restrict__ block_super,
+                                                K* __restrict__ keys, uint64_t* __restrict__ vals) {
     using O = KOps<K>;
-    __shared__ uint64_t s_off[TPB + 1];
-    __shared__ uint64_t s_first;
-    __shared__ uint32_t s_cnt;
+    __shared__ uint64_t s_off[TPB + 2];
     const uint64_t T0 = a.t0 + uint64_t(blockIdx.x) * TPB;
     const uint64_t TL = min(T0 + TPB, a.t0 + a.n) - 1;  // last tuple of the block
-    if (threadIdx.x == 0) {
-        // supers holding T0 and TL: last s with tup_off[s] <= T
-        uint64_t lo = a.s0, hi = a.s1;
-        while (hi - lo > 1) {
-            const uint64_t mid = (lo + hi) >> 1;
-            if (a.tup_off[mid] <= T0)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        const uint64_t sa = lo;
-        hi = a.s1;
-        while (hi - lo > 1) {
-            const uint64_t mid = (lo + hi) >> 1;
-            if (a.tup_off[mid] <= TL)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        s_first = sa;
-        s_cnt = uint32_t(lo - sa + 1);
-    }
-    __syncthreads();
-    const uint64_t sa = s_first;
-    const uint32_t ns = s_cnt;  // <= TPB: every super has >= 1 k-mer
+    const uint64_t sa = a.s0 + block_super[blockIdx.x];
+    const uint32_t ns = block_super[blockIdx.x + 1] - block_super[blockIdx.x] + 1;  // <= TPB + 1
     for (uint32_t i = threadIdx.x; i <= ns; i += TPB) s_off[i] = a.tup_off[sa + i];
     __syncthreads();
     const uint64_t t = T0 + threadIdx.x;
@@ -737,8 +724,9 @@ __global__ void k_table_insert(uint32_t n_nodes, const K* __restrict__ node_key,
                                const uint32_t* __restrict__ node_bucket, const uint64_t* __restrict__ tab_off,
                                K* __restrict__ tab, uint32_t* __restrict__ node_of) {
     constexpr uint32_t G = Tab<K>::G;
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
+    const uint32_t nblk = (n_nodes + TPB - 1) / TPB, blk = tile_of(blockIdx.x, nblk);
+    const uint64_t n = uint64_t(blk) * TPB + threadIdx.x;
+    if (blk >= nblk || n >= n_nodes) return;
     const uint32_t b = node_bucket[n];
     const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base) / G;
     const K key = node_key[n];
@@ -788,8 +776,9 @@ __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const 
                         uint8_t* __restrict__ mask_out, uint16_t* __restrict__ node_mm, uint32_t* __restrict__ nb) {
     using O = KOps<K>;
     constexpr uint32_t G = Tab<K>::G;
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
+    const uint32_t nblk = (n_nodes + TPB - 1) / TPB, blk = tile_of(blockIdx.x, nblk);
+    const uint64_t n = uint64_t(blk) * TPB + threadIdx.x;
+    if (blk >= nblk || n >= n_nodes) return;
     const uint32_t b = node_bucket[n];
     const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base) / G;
     const K can = node_key[n];
@@ -1029,42 +1018,56 @@ __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, 
     atomicAdd(&a.counters[5], 1ull);
 }
 
+// Path ends (a node with at most one link) in node order.
+__global__ void k_end_flags(uint32_t n_nodes, const uint32_t* __restrict__ link, uint32_t* __restrict__ flag) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n < n_nodes) flag[n] = (link[2 * n + PORT_L] == NONE32 || link[2 * n + PORT_R] == NONE32) ? 1u : 0u;
+}
+__global__ void k_end_list(uint32_t n_nodes, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
+                           uint32_t* __restrict__ ends) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n < n_nodes && flag[n]) ends[scan[n]] = uint32_t(n);
+}
+
 // Path ends walk to the other end; the end with the smaller index owns the
 // path.  Paths that never pass a node twice are queued for recording (with
 // their nodes when they have at most PATH_BUF), the others (a node linked to
-// itself turns the walk back) for the general seed loop.
-__global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a) {
-    const uint64_t x = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// itself turns the walk back) for the general seed loop.  One thread per end
+// of the dense end list; XCD x takes the x-th eighth of it, which keeps a
+// bucket's links in one L2.
+__global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a, const uint32_t* __restrict__ ends, uint32_t n_ends) {
+    const uint32_t nblk = (n_ends + TPB - 1) / TPB;
+    const uint32_t blk = tile_of(blockIdx.x, nblk);
+    const uint64_t e = uint64_t(blk) * TPB + threadIdx.x;
     bool owner = false, simple = true;
-    uint32_t ex0 = 0, len = 0;
+    uint32_t x = 0, ex0 = 0, len = 0;
     uint32_t buf[PATH_BUF];
-    if (x < a.n_nodes) {
-        const uint32_t lL = a.link[2 * x + PORT_L], lR = a.link[2 * x + PORT_R];
-        if (lL == NONE32 || lR == NONE32) {  // a path end
-            ex0 = lR == NONE32 ? PORT_L : PORT_R;
-            uint32_t cur = uint32_t(x), ex = ex0;
-            buf[0] = cur | (ex0 == PORT_L ? 0x80000000u : 0u);
-            len = 1;
-            const uint32_t limit = 2 * a.n_nodes + 2;
-            for (uint32_t h = 0; h < limit; ++h) {
-                const uint32_t l = a.link[2 * cur + ex];
-                if (l == NONE32) break;
-                const uint32_t y = l >> 1, q = l & 1u;
-                if (y == cur) simple = false;  // hairpin: the walk comes back
-                cur = y;
-                ex = 1u - q;
+    if (blk < nblk && e < n_ends) {
+        x = ends[e];
+        const uint32_t lR = a.link[2 * x + PORT_R];
+        ex0 = lR == NONE32 ? PORT_L : PORT_R;
+        uint32_t cur = x, ex = ex0;
+        buf[0] = cur | (ex0 == PORT_L ? 0x80000000u : 0u);
+        len = 1;
+        const uint32_t limit = 2 * a.n_nodes + 2;
+        for (uint32_t h = 0; h < limit; ++h) {
+            const uint32_t l = a.link[2 * cur + ex];
+            if (l == NONE32) break;
+            const uint32_t y = l >> 1, q = l & 1u;
+            if (y == cur) simple = false;  // hairpin: the walk comes back
+            cur = y;
+            ex = 1u - q;
 #pragma unroll
-                for (uint32_t j = 1; j < PATH_BUF; ++j)
-                    if (j == len) buf[j] = cur | (q == PORT_R ? 0x80000000u : 0u);
-                ++len;
-            }
-            owner = cur >= x;
+            for (uint32_t j = 1; j < PATH_BUF; ++j)
+                if (j == len) buf[j] = cur | (q == PORT_R ? 0x80000000u : 0u);
+            ++len;
         }
+        owner = cur >= x;
     }
     const uint64_t c = wave_alloc(&a.counters[9], owner && simple);
     const uint64_t o = wave_alloc(&a.counters[10], owner && !simple);
     if (owner && simple) {
-        a.comp_x[c] = uint32_t(x);
+        a.comp_x[c] = x;
         a.comp_ex[c] = uint8_t(ex0);
         a.comp_len[c] = len;
         if (len <= PATH_BUF)
@@ -1072,7 +1075,7 @@ __global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a) {
             for (uint32_t j = 0; j < PATH_BUF; ++j)
                 if (j < len) a.comp_nodes[c * PATH_BUF + j] = buf[j];
     } else if (owner) {
-        a.other_x[o] = uint32_t(x);
+        a.other_x[o] = x;
         a.other_ex[o] = uint8_t(ex0);
     }
 }
@@ -1353,14 +1356,14 @@ struct Phase2Bufs {
     DevBuf set_off, cs_words, frag_bytes;
     uint64_t mt_cap = 0, mt_count = 0;
     // per chunk (reused)
-    DevBuf keys, vals, keys_t, vals_t;
+    DevBuf keys, vals, keys_t, vals_t, block_super;
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
     DevBuf node_bucket, tab_off, tab, node_of, node_mask, nb, link, visited;
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fscan, fkey;
     DevBuf comp_x, comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
-    DevBuf node_rec, node_mm;
+    DevBuf node_rec, node_mm, eflag, escan, ends;
     DevBuf fkey_t, forder, forder_t, fsize, fboff;
     RadixScratch rs2;
     // results
@@ -1450,7 +1453,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     ExpArgs ea{nt, t0, s0, s1, P.tup_off.as<uint64_t>(), P.btup.as<uint64_t>(), Q.key.as<uint32_t>(),
                Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(),
                Q.packed.as<uint32_t>(), k, J.from_files ? Q.rank0.as<uint32_t>() : nullptr};
-    { KSCOPE("k_expand", st); k_expand<K><<<blocks_for(nt), TPB, 0, st>>>(ea, keys, vals); }
+    const uint64_t n_eblocks = (nt + TPB - 1) / TPB;
+    uint32_t* block_super = P.block_super.ensure<uint32_t>(n_eblocks + 1);
+    { KSCOPE("k_block_super", st); k_block_super<<<blocks_for(s1 - s0), TPB, 0, st>>>(s0, s1, t0, n_eblocks, P.tup_off.as<uint64_t>(), block_super); }
+    { KSCOPE("k_expand", st); k_expand<K><<<uint32_t(n_eblocks), TPB, 0, st>>>(ea, block_super, keys, vals); }
 
     // --- sort tuples by k-mer inside every bucket
     SortPlan plan;
@@ -1559,12 +1565,12 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     K* tab = P.tab.ensure<K>(tab_n);
     uint32_t* node_of = P.node_of.ensure<uint32_t>(tab_n);
     { KSCOPE("k_fill", st); k_fill<uint64_t><<<blocks_for(tab_n * sizeof(K) / 8), TPB, 0, st>>>((uint64_t*)tab, tab_n * sizeof(K) / 8, ~0ull); }
-    { KSCOPE("k_table_insert", st); k_table_insert<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of); }
+    { KSCOPE("k_table_insert", st); k_table_insert<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of); }
     uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
     uint32_t* nbr = P.nb.ensure<uint32_t>(2ull * nn);
     uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
     uint16_t* node_mm = P.node_mm.ensure<uint16_t>(nn);
-    { KSCOPE("k_masks", st); k_masks<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
+    { KSCOPE("k_masks", st); k_masks<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
     { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link); }
 
     // --- walk
@@ -1604,7 +1610,14 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.pcid = pcid;
     wa.node_rec = node_rec;
     wa.node_mm = node_mm;
-    { KSCOPE("k_walk_ends", st); k_walk_ends<<<blocks_for(nn), TPB, 0, st>>>(wa); }
+    uint32_t* eflag = P.eflag.ensure<uint32_t>(nn);
+    uint32_t* escan = P.escan.ensure<uint32_t>(uint64_t(nn) + 1);
+    { KSCOPE("k_end_flags", st); k_end_flags<<<blocks_for(nn), TPB, 0, st>>>(nn, link, eflag); }
+    exclusive_scan<uint32_t, uint32_t>(eflag, escan, nn, escan + nn, J.rs.scan_tmp, st);
+    const uint32_t n_ends = d2h<uint32_t>(escan + nn, st);
+    uint32_t* ends = P.ends.ensure<uint32_t>(n_ends);
+    { KSCOPE("k_end_list", st); k_end_list<<<blocks_for(nn), TPB, 0, st>>>(nn, eflag, escan, ends); }
+    if (n_ends) { KSCOPE("k_walk_ends", st); k_walk_ends<<<xcd_grid(blocks_for(n_ends)), TPB, 0, st>>>(wa, ends, n_ends); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     const uint64_t n_comp = h_cnt[9], n_other = h_cnt[10];
