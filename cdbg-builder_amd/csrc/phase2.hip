@@ -130,6 +130,16 @@ __global__ void k_fill(T* p, uint64_t n, T v) {
     if (i < n) p[i] = v;
 }
 
+// p[2i] = a, p[2i+1] = b for i < n
+template <class T>
+__global__ void k_fill2(T* p, uint64_t n, T a, T b) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        p[2 * i] = a;
+        p[2 * i + 1] = b;
+    }
+}
+
 // block-wide sum added atomically to *dst
 __device__ __forceinline__ void block_add(unsigned long long* dst, uint64_t v) {
     __shared__ uint64_t s_part[TPB / 64];
@@ -244,6 +254,199 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
     const uint32_t rank = a.rank0 ? a.rank0[s] + ix : uint32_t(t - a.btup[a.bucket[s]]);
     keys[t - a.t0] = fwd ? f : r;
     vals[t - a.t0] = (uint64_t(rank) << 32) | (uint64_t(a.color[s]) << 3) | (uint64_t(owned) << 2) | contrib;
+}
+
+// ---------------------------------------------------------------------------
+// Cell sort: the last stage of a stable sort whose top bits are already
+// sorted.  A cell (a run of keys equal above end_bit) is sorted by one
+// workgroup in LDS: LSD passes of 8-bit digits, each ranking the items with
+// the per-wave scheme of k_radix_scatter and placing them in LDS, keys and
+// values held in registers between passes.  Size classes: a launch of
+// THREADS threads and capacity CAP takes the listed cells (or all cells)
+// with more than `lo` items; cells above CAP go to the next class's list.
+// ---------------------------------------------------------------------------
+template <class K, class V, uint32_t CAP, int THREADS>
+inline size_t cell_sort_lds() {
+    return size_t(CAP) * (sizeof(K) + sizeof(V)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
+}
+
+template <class K, class V, uint32_t CAP, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* __restrict__ vals,
+                                                       const uint32_t* __restrict__ cell_start,
+                                                       const uint32_t* __restrict__ list, uint32_t n_items,
+                                                       uint32_t lo, int end_bit, uint32_t* __restrict__ next,
+                                                       unsigned long long* __restrict__ n_next) {
+    constexpr int WAVES = THREADS / 64;
+    constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
+    extern __shared__ uint64_t sm64[];
+    K* s_key = (K*)sm64;
+    V* s_val = (V*)(s_key + CAP);
+    uint32_t* wcnt = (uint32_t*)(s_val + CAP);  // [WAVES][256]
+    uint32_t* s_tmp = wcnt + WAVES * 256;
+    if (blockIdx.x >= n_items) return;
+    const uint32_t cell = list ? list[blockIdx.x] : blockIdx.x;
+    const uint32_t start = cell_start[cell], count = cell_start[cell + 1] - start;
+    if (count <= 1 || count <= lo) return;
+    if (count > CAP) {
+        if (threadIdx.x == 0) next[atomicAdd(n_next, 1ull)] = cell;
+        return;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const uint32_t rounds = (count + THREADS - 1) / THREADS;
+    const uint32_t wbase = uint32_t(wave) * rounds * 64;  // wave w owns items [wbase, wbase + 64 rounds)
+    K key[RMAX];
+    V val[RMAX];
+    uint32_t dr[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+        const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+        if (uint32_t(r) < rounds && i < count) {
+            key[r] = keys[start + i];
+            val[r] = vals[start + i];
+        }
+    }
+    uint32_t* my = wcnt + wave * 256;
+    for (int sh = 0; sh < end_bit; sh += 8) {
+        const int bits = (end_bit - sh) < 8 ? (end_bit - sh) : 8;
+        const uint32_t mask = (1u << bits) - 1;
+        for (uint32_t b = threadIdx.x; b < WAVES * 256; b += THREADS) wcnt[b] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            if (uint32_t(r) >= rounds) break;
+            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+            const bool active = i < count;
+            const uint32_t dig = active ? digit_of(key[r], sh, mask) : 0u;
+            uint64_t peers = __ballot(active);
+            for (int b = 0; b < bits; ++b) {
+                const bool bit = (dig >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const uint32_t rk = __popcll(peers & lt);
+            uint32_t before = 0;
+            if (active) before = my[dig];
+            if (active && rk == 0) my[dig] = before + __popcll(peers);
+            dr[r] = (dig << 16) | (before + rk);
+        }
+        __syncthreads();
+        {
+            // digit-major, wave-minor bases; thread t < 256 owns digit t
+            uint32_t tot = 0;
+            for (uint32_t d = threadIdx.x; d < 256; d += THREADS)
+                for (int w = 0; w < WAVES; ++w) tot += wcnt[w * 256 + d];
+            uint32_t all;
+            uint32_t run = block_exclusive_scan<uint32_t>(tot, s_tmp, all);
+            // THREADS >= 256 here: one digit per thread
+            if (threadIdx.x < 256)
+                for (int w = 0; w < WAVES; ++w) {
+                    const uint32_t c = wcnt[w * 256 + threadIdx.x];
+                    wcnt[w * 256 + threadIdx.x] = run;
+                    run += c;
+                }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            if (uint32_t(r) >= rounds) break;
+            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+            if (i < count) {
+                const uint32_t pos = my[dr[r] >> 16] + (dr[r] & 0xFFFFu);
+                s_key[pos] = key[r];
+                s_val[pos] = val[r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            if (uint32_t(r) >= rounds) break;
+            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+            if (i < count) {
+                key[r] = s_key[i];
+                val[r] = s_val[i];
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+        if (uint32_t(r) >= rounds) break;
+        const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+        if (i < count) {
+            keys[start + i] = key[r];
+            vals[start + i] = val[r];
+        }
+    }
+}
+
+// Lets a k_cell_sort class use its LDS (once per process).
+template <class K, class V, uint32_t CAP, int THREADS>
+inline void cell_sort_prepare() {
+    static const bool done = [] {
+        CDBG_HIP(hipFuncSetAttribute((const void*)&k_cell_sort<K, V, CAP, THREADS>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(cell_sort_lds<K, V, CAP, THREADS>())));
+        return true;
+    }();
+    (void)done;
+}
+
+// the two LDS classes: (capacity, threads) per key width
+template <class K>
+struct CellClasses {
+    static constexpr uint32_t SMALL = 2048, LARGE = 8192;
+};
+template <>
+struct CellClasses<Key128> {
+    static constexpr uint32_t SMALL = 1024, LARGE = 4096;
+};
+
+// cell starts: positions where a segment starts or the bits above `sh` change
+template <class K>
+__global__ void k_cell_flags(uint64_t n, const K* __restrict__ keys, int sh, const uint32_t* __restrict__ seg_start_flag,
+                             uint32_t* __restrict__ flag) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    bool f = i == 0 || seg_start_flag[i];
+    if (!f) {
+        const K a = keys[i - 1], b = keys[i];
+        f = digit_of(a, sh, 0xFFFFFFFFu) != digit_of(b, sh, 0xFFFFFFFFu);
+    }
+    flag[i] = f ? 1u : 0u;
+}
+
+__global__ void k_cell_list(uint64_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
+                            uint32_t* __restrict__ cell_start) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) cell_start[scan[i]] = uint32_t(i);
+    if (i == 0) cell_start[scan[n]] = uint32_t(n);  // scan has n + 1 entries
+}
+
+// Moves the elements of listed cells between the tuple arrays and a compact
+// buffer (to_compact) or back: compact index j lies in cell c with
+// pre[c] <= j < pre[c + 1] and maps to start[c] + j - pre[c].
+template <class K, class V>
+__global__ void k_cells_move(bool to_compact, const uint32_t* __restrict__ start, const uint64_t* __restrict__ pre,
+                             uint32_t n_list, K* __restrict__ keys, V* __restrict__ vals, K* __restrict__ ck,
+                             V* __restrict__ cv) {
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= pre[n_list]) return;
+    uint32_t lo = 0, hi = n_list;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= j)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint64_t i = start[lo] + (j - pre[lo]);
+    if (to_compact) {
+        ck[j] = keys[i];
+        cv[j] = vals[i];
+    } else {
+        keys[i] = ck[j];
+        vals[i] = cv[j];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -448,8 +651,7 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
 struct SetTables {
     unsigned long long* single_min;  // [num_colors] min key of the set {c}
     uint32_t* single_cid;            // [num_colors]
-    unsigned long long* mt_hash;     // [cap] 0 = empty
-    unsigned long long* mt_key;      // [cap] min key
+    unsigned long long* mt_hk;       // [2 cap] per slot: hash (0 = empty), min key; one line per probe
     uint32_t* mt_cid;                // [cap]
     uint64_t* mt_loff;               // [cap] colour list in `store`
     uint32_t* mt_llen;               // [cap]
@@ -478,9 +680,9 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
     const unsigned long long h = node_hash[n];
     uint64_t s = h & (T.cap - 1);
     for (uint64_t probe = 0; probe < T.cap; ++probe) {
-        unsigned long long cur = T.mt_hash[s];
+        unsigned long long cur = T.mt_hk[2 * s];
         if (cur == 0ull) {
-            cur = atomicCAS(&T.mt_hash[s], 0ull, h);
+            cur = atomicCAS(&T.mt_hk[2 * s], 0ull, h);
             if (cur == 0ull) {  // this thread created the slot: store its colour list
                 const uint64_t lo = atomicAdd(T.store_used, (unsigned long long)nc);
                 for (uint32_t i = 0; i < nc; ++i) T.store[lo + i] = c[i];
@@ -490,7 +692,7 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
             }
         }
         if (cur == h) {
-            if (T.mt_key[s] > key) atomicMin(&T.mt_key[s], key);
+            if (T.mt_hk[2 * s + 1] > key) atomicMin(&T.mt_hk[2 * s + 1], key);
             node_slot[n] = uint32_t(s);
             return;
         }
@@ -532,16 +734,16 @@ __global__ __launch_bounds__(TPB) void k_set_verify_big(const uint32_t* __restri
     if (__ballot(diff) && lane == 0) atomicAdd(err, 1ull);
 }
 
-__global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restrict__ oh,
-                            const unsigned long long* __restrict__ ok, const uint32_t* __restrict__ oc,
+__global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restrict__ ohk,
+                            const uint32_t* __restrict__ oc,
                             const uint64_t* __restrict__ oo, const uint32_t* __restrict__ ol, SetTables T) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= old_cap || oh[i] == 0ull) return;
-    const unsigned long long h = oh[i];
+    if (i >= old_cap || ohk[2 * i] == 0ull) return;
+    const unsigned long long h = ohk[2 * i];
     uint64_t s = h & (T.cap - 1);
     for (uint64_t probe = 0; probe < T.cap; ++probe) {
-        if (atomicCAS(&T.mt_hash[s], 0ull, h) == 0ull) {
-            T.mt_key[s] = ok[i];
+        if (atomicCAS(&T.mt_hk[2 * s], 0ull, h) == 0ull) {
+            T.mt_hk[2 * s + 1] = ohk[2 * i + 1];
             T.mt_cid[s] = oc[i];
             T.mt_loff[s] = oo[i];
             T.mt_llen[s] = ol[i];
@@ -557,7 +759,7 @@ __global__ void k_newset_flags(uint32_t nc, SetTables T, uint32_t* __restrict__ 
     if (i < nc)
         fl[i] = T.single_min[i] != ~0ull && T.single_cid[i] == NONE32;
     else if (i < nc + T.cap)
-        fl[i] = T.mt_hash[i - nc] != 0ull && T.mt_cid[i - nc] == NONE32;
+        fl[i] = T.mt_hk[2 * (i - nc)] != 0ull && T.mt_cid[i - nc] == NONE32;
 }
 
 __global__ void k_newset_emit(uint32_t nc, SetTables T, const uint32_t* __restrict__ fl, const uint64_t* __restrict__ pos,
@@ -569,7 +771,7 @@ __global__ void k_newset_emit(uint32_t nc, SetTables T, const uint32_t* __restri
         set_key[p] = T.single_min[i];
         set_desc[p] = i;
     } else {
-        set_key[p] = T.mt_key[i - nc];
+        set_key[p] = T.mt_hk[2 * (i - nc) + 1];
         set_desc[p] = (uint64_t(1) << 63) | (i - nc);
     }
 }
@@ -1352,11 +1554,12 @@ __global__ __launch_bounds__(TPB) void k_frag_write(uint64_t n_frags, const uint
 struct Phase2Bufs {
     // job-wide
     DevBuf kcnt, tup_off, btup, total, lbtup;
-    DevBuf single_min, single_cid, mt_hash, mt_key, mt_cid, mt_loff, mt_llen, store, store_used;
+    DevBuf single_min, single_cid, mt_hk, mt_cid, mt_loff, mt_llen, store, store_used;
     DevBuf set_off, cs_words, frag_bytes;
     uint64_t mt_cap = 0, mt_count = 0;
     // per chunk (reused)
     DevBuf keys, vals, keys_t, vals_t, block_super;
+    DevBuf cflag, cscan, cell_start, cell_mid, cell_over, cell_novf, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
@@ -1389,8 +1592,7 @@ static SetTables set_tables(Phase2Bufs& P) {
     SetTables T;
     T.single_min = P.single_min.as<unsigned long long>();
     T.single_cid = P.single_cid.as<uint32_t>();
-    T.mt_hash = P.mt_hash.as<unsigned long long>();
-    T.mt_key = P.mt_key.as<unsigned long long>();
+    T.mt_hk = P.mt_hk.as<unsigned long long>();
     T.mt_cid = P.mt_cid.as<uint32_t>();
     T.mt_loff = P.mt_loff.as<uint64_t>();
     T.mt_llen = P.mt_llen.as<uint32_t>();
@@ -1402,27 +1604,23 @@ static SetTables set_tables(Phase2Bufs& P) {
 
 // (Re)allocates the colour-set hash table with `cap` slots, keeping entries.
 static void mt_resize(Phase2Bufs& P, uint64_t cap, hipStream_t st) {
-    DevBuf oh, ok, oc, oo, ol;
+    DevBuf ohk, oc, oo, ol;
     const uint64_t old_cap = P.mt_cap;
-    oh.swap(P.mt_hash);
-    ok.swap(P.mt_key);
+    ohk.swap(P.mt_hk);
     oc.swap(P.mt_cid);
     oo.swap(P.mt_loff);
     ol.swap(P.mt_llen);
     P.mt_cap = cap;
-    P.mt_hash.ensure<unsigned long long>(cap);
-    P.mt_key.ensure<unsigned long long>(cap);
+    P.mt_hk.ensure<unsigned long long>(2 * cap);
     P.mt_cid.ensure<uint32_t>(cap);
     P.mt_loff.ensure<uint64_t>(cap);
     P.mt_llen.ensure<uint32_t>(cap);
-    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(cap), TPB, 0, st>>>(P.mt_hash.as<unsigned long long>(), cap, 0ull); }
-    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(cap), TPB, 0, st>>>(P.mt_key.as<unsigned long long>(), cap, ~0ull); }
+    { KSCOPE("k_fill2", st); k_fill2<unsigned long long><<<blocks_for(cap), TPB, 0, st>>>(P.mt_hk.as<unsigned long long>(), cap, 0ull, ~0ull); }
     { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(cap), TPB, 0, st>>>(P.mt_cid.as<uint32_t>(), cap, NONE32); }
     if (old_cap) {
         KSCOPE("k_mt_rehash", st);
-        k_mt_rehash<<<blocks_for(old_cap), TPB, 0, st>>>(old_cap, oh.as<unsigned long long>(), ok.as<unsigned long long>(),
-                                                          oc.as<uint32_t>(), oo.as<uint64_t>(), ol.as<uint32_t>(),
-                                                          set_tables(P));
+        k_mt_rehash<<<blocks_for(old_cap), TPB, 0, st>>>(old_cap, ohk.as<unsigned long long>(), oc.as<uint32_t>(),
+                                                          oo.as<uint64_t>(), ol.as<uint32_t>(), set_tables(P));
     }
     CDBG_HIP(hipStreamSynchronize(st));
 }
@@ -1458,13 +1656,72 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     { KSCOPE("k_block_super", st); k_block_super<<<blocks_for(s1 - s0), TPB, 0, st>>>(s0, s1, t0, n_eblocks, P.tup_off.as<uint64_t>(), block_super); }
     { KSCOPE("k_expand", st); k_expand<K><<<uint32_t(n_eblocks), TPB, 0, st>>>(ea, block_super, keys, vals); }
 
-    // --- sort tuples by k-mer inside every bucket
+    // --- sort tuples by k-mer inside every bucket (stable): the top D bits by
+    // two global passes, which cuts every bucket into cells of ~700 keys,
+    // then the rest of each cell in LDS
+    const int kb = 2 * int(k);
+    int D = 0;
+    while (D < 20 && double(nt) / nb / double(1ull << D) > 700.0) ++D;
+    D = std::min(D, kb);
     SortPlan plan;
     plan.build(lb);
-    segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, 0, 2 * int(k), 8, P.rs2, st, "tuples");
+    if (D > 0) segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D, kb, (D + 1) / 2, P.rs2, st, "tuples");
+    uint32_t* heads = P.heads.ensure<uint32_t>(nt);
+    {
+        uint32_t* cflag = P.cflag.ensure<uint32_t>(nt);
+        uint32_t* cscan = P.cscan.ensure<uint32_t>(nt + 1);
+        { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(nt), TPB, 0, st>>>(heads, nt, 0u); }
+        { KSCOPE("k_mark_bucket_starts", st); k_mark_bucket_starts<<<blocks_for(nb), TPB, 0, st>>>(nb, lbtup, heads); }
+        { KSCOPE("k_cell_flags", st); k_cell_flags<K><<<blocks_for(nt), TPB, 0, st>>>(nt, keys, kb - D, heads, cflag); }
+        exclusive_scan<uint32_t, uint32_t>(cflag, cscan, nt, cscan + nt, J.rs.scan_tmp, st);
+        const uint32_t n_cells = d2h<uint32_t>(cscan + nt, st);
+        uint32_t* cell_start = P.cell_start.ensure<uint32_t>(uint64_t(n_cells) + 1);
+        { KSCOPE("k_cell_list", st); k_cell_list<<<blocks_for(nt), TPB, 0, st>>>(nt, cflag, cscan, cell_start); }
+        // small class over every cell, then the large class over the cells it
+        // passed on; what is left goes to the global passes
+        constexpr uint32_t C1 = CellClasses<K>::SMALL, C2 = CellClasses<K>::LARGE;
+        uint32_t* mid = P.cell_mid.ensure<uint32_t>(n_cells);
+        uint32_t* over = P.cell_over.ensure<uint32_t>(n_cells);
+        auto* novf = P.cell_novf.ensure<unsigned long long>(2);
+        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 2, 0ull); }
+        cell_sort_prepare<K, uint64_t, C1, 256>();
+        cell_sort_prepare<K, uint64_t, C2, 512>();
+        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C1, 256><<<n_cells, 256, cell_sort_lds<K, uint64_t, C1, 256>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, mid, novf); }
+        const uint64_t n_mid = d2h<unsigned long long>(novf, st);
+        if (n_mid) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C2, 512><<<uint32_t(n_mid), 512, cell_sort_lds<K, uint64_t, C2, 512>(), st>>>(keys, vals, cell_start, mid, uint32_t(n_mid), C1, kb - D, over, novf + 1); }
+        const uint64_t n_over = d2h<unsigned long long>(novf + 1, st);
+        if (n_over) {
+            // cells too large for LDS: gathered, sorted by the global passes, put back
+            std::vector<uint32_t> ids(n_over), cs(n_cells + 1);
+            CDBG_HIP(hipMemcpyAsync(ids.data(), over, n_over * 4, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipMemcpyAsync(cs.data(), cell_start, (uint64_t(n_cells) + 1) * 4, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipStreamSynchronize(st));
+            std::sort(ids.begin(), ids.end());
+            std::vector<uint32_t> st_l(n_over);
+            std::vector<uint64_t> pre(n_over + 1, 0);
+            for (uint64_t i = 0; i < n_over; ++i) {
+                st_l[i] = cs[ids[i]];
+                pre[i + 1] = pre[i] + (cs[ids[i] + 1] - cs[ids[i]]);
+            }
+            const uint64_t tot = pre[n_over];
+            uint32_t* d_st = P.fb_start.ensure<uint32_t>(n_over);
+            uint64_t* d_pre = P.fb_pre.ensure<uint64_t>(n_over + 1);
+            CDBG_HIP(hipMemcpyAsync(d_st, st_l.data(), n_over * 4, hipMemcpyHostToDevice, st));
+            CDBG_HIP(hipMemcpyAsync(d_pre, pre.data(), (n_over + 1) * 8, hipMemcpyHostToDevice, st));
+            K* ck = P.fb_k.ensure<K>(tot);
+            uint64_t* cv = P.fb_v.ensure<uint64_t>(tot);
+            K* ckt = P.fb_kt.ensure<K>(tot);
+            uint64_t* cvt = P.fb_vt.ensure<uint64_t>(tot);
+            { KSCOPE("k_cells_move", st); k_cells_move<K, uint64_t><<<blocks_for(tot), TPB, 0, st>>>(true, d_st, d_pre, uint32_t(n_over), keys, vals, ck, cv); }
+            SortPlan fplan;
+            fplan.build(pre);
+            segmented_radix_sort<K, uint64_t>(ck, cv, ckt, cvt, fplan, 0, kb - D, 8, P.rs2, st, "cells");
+            { KSCOPE("k_cells_move", st); k_cells_move<K, uint64_t><<<blocks_for(tot), TPB, 0, st>>>(false, d_st, d_pre, uint32_t(n_over), keys, vals, ck, cv); }
+            CDBG_HIP(hipStreamSynchronize(st));  // st_l / pre are pageable
+        }
+    }
 
     // --- nodes
-    uint32_t* heads = P.heads.ensure<uint32_t>(nt);
     uint32_t* hscan = P.hscan.ensure<uint32_t>(nt + 1);
     uint64_t* dtot = P.total.ensure<uint64_t>(8);
     { KSCOPE("k_heads", st); k_heads<K><<<blocks_for(nt), TPB, 0, st>>>(nt, keys, heads); }
@@ -1696,8 +1953,7 @@ static void phase2_impl(Job& J) {
     if (P.mt_cap == 0) {
         mt_resize(P, 1024, st);
     } else {  // clear, keeping the capacity of the previous run
-        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(P.mt_cap), TPB, 0, st>>>(P.mt_hash.as<unsigned long long>(), P.mt_cap, 0ull); }
-        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(P.mt_cap), TPB, 0, st>>>(P.mt_key.as<unsigned long long>(), P.mt_cap, ~0ull); }
+        { KSCOPE("k_fill2", st); k_fill2<unsigned long long><<<blocks_for(P.mt_cap), TPB, 0, st>>>(P.mt_hk.as<unsigned long long>(), P.mt_cap, 0ull, ~0ull); }
         { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(P.mt_cap), TPB, 0, st>>>(P.mt_cid.as<uint32_t>(), P.mt_cap, NONE32); }
     }
 

@@ -122,3 +122,26 @@ def test_phase2_chunked(gpu_ctx, k, m, B, chunk):
     stream and the fragment stream continue across chunks."""
     data, off = G.synth(seed=8, num_colors=7, genome_len=25000, snp_rate=0.015, indel_rate=0.001)
     _assert_same(*_run_both(gpu_ctx, data, off, k, m, B, chunk=chunk))
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 19, 4), (63, 27, 2)])
+def test_phase2_large_buckets_split_into_cells(gpu_ctx, k, m, B):
+    """Buckets of ~10^5 occurrences: the tuple sort splits them on their top
+    key bits into cells that are finished in LDS."""
+    data, off = G.synth(seed=14, num_colors=8, genome_len=120000, snp_rate=0.01, indel_rate=0.001)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 19, 8), (63, 27, 4), (21, 11, 16)])
+def test_phase2_heavy_kmers(gpu_ctx, k, m, B):
+    """One k-mer repeated far beyond what a cell holds in LDS (homopolymer
+    and short tandem repeats in every colour): those cells take the global
+    sort."""
+    rng = np.random.default_rng(8)
+    seqs = []
+    for c in range(3):
+        body = ACGT[rng.integers(0, 4, 20000)].tobytes()
+        seqs.append(b"A" * 12000 + body[:10000] + b"ACG" * 4000 + body[10000:] + b"\n")
+    data = np.frombuffer(b"".join(seqs), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.uint64)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
