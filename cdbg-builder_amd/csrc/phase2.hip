@@ -653,12 +653,18 @@ struct SetTables {
     uint32_t* single_cid;            // [num_colors]
     unsigned long long* mt_hk;       // [2 cap] per slot: hash (0 = empty), min key; one line per probe
     uint32_t* mt_cid;                // [cap]
-    uint64_t* mt_loff;               // [cap] colour list in `store`
+    uint64_t* mt_loff;               // [cap] colour list in `store`, or (IN_CHUNK | offset) in cur_colors
     uint32_t* mt_llen;               // [cap]
     uint64_t cap;
     uint32_t* store;
-    unsigned long long* store_used;
+    const uint32_t* cur_colors;      // node colour lists of the chunk being processed
 };
+constexpr uint64_t IN_CHUNK = uint64_t(1) << 63;
+
+__device__ __forceinline__ const uint32_t* slot_list(const SetTables& T, uint64_t s) {
+    const uint64_t o = T.mt_loff[s];
+    return (o & IN_CHUNK) ? T.cur_colors + (o & ~IN_CHUNK) : T.store + o;
+}
 
 // Colour lists live at colors[coff[n]] (coff = the node's first occurrence).
 __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
@@ -683,10 +689,8 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
         unsigned long long cur = T.mt_hk[2 * s];
         if (cur == 0ull) {
             cur = atomicCAS(&T.mt_hk[2 * s], 0ull, h);
-            if (cur == 0ull) {  // this thread created the slot: store its colour list
-                const uint64_t lo = atomicAdd(T.store_used, (unsigned long long)nc);
-                for (uint32_t i = 0; i < nc; ++i) T.store[lo + i] = c[i];
-                T.mt_loff[s] = lo;
+            if (cur == 0ull) {  // this thread created the slot: its list stands for the set
+                T.mt_loff[s] = IN_CHUNK | coff[n];
                 T.mt_llen[s] = nc;
                 cur = h;
             }
@@ -708,7 +712,7 @@ __global__ void k_set_verify(uint32_t n_nodes, const uint8_t* __restrict__ meta,
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2 || coff[n + 1] - coff[n] > BIG_RUN) return;
     const uint32_t s = node_slot[n];
-    const uint32_t* ref = T.store + T.mt_loff[s];
+    const uint32_t* ref = slot_list(T, s);
     bool same = T.mt_llen[s] == ncol[n];
     for (uint32_t i = 0; same && i < ncol[n]; ++i) same = colors[coff[n] + i] == ref[i];
     if (!same) atomicAdd(err, 1ull);
@@ -727,7 +731,7 @@ __global__ __launch_bounds__(TPB) void k_set_verify_big(const uint32_t* __restri
     const uint32_t n = big[i];
     if (!(meta[n] & 4u) || ncol[n] < 2) return;
     const uint32_t s = node_slot[n], nc = ncol[n];
-    const uint32_t* ref = T.store + T.mt_loff[s];
+    const uint32_t* ref = slot_list(T, s);
     const uint32_t* own = colors + coff[n];
     bool diff = T.mt_llen[s] != nc;
     for (uint32_t j = uint32_t(lane); !diff && j < nc; j += 64) diff = own[j] != ref[j];
@@ -786,7 +790,7 @@ __device__ __forceinline__ void set_list(uint64_t desc, const SetTables& T, cons
                                          uint32_t& single) {
     if (desc >> 63) {
         const uint64_t s = desc & 0xFFFFFFFFFFFFull;
-        c = T.store + T.mt_loff[s];
+        c = slot_list(T, s);
         n = T.mt_llen[s];
     } else {
         single = uint32_t(desc);
@@ -796,7 +800,8 @@ __device__ __forceinline__ void set_list(uint64_t desc, const SetTables& T, cons
 }
 
 __global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t* __restrict__ set_desc, SetTables T,
-                              uint32_t num_colors, uint64_t* __restrict__ bits, unsigned long long* __restrict__ counters) {
+                              uint32_t num_colors, uint64_t* __restrict__ bits, uint64_t* __restrict__ keep,
+                              unsigned long long* __restrict__ counters) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     uint64_t ints = 0;
     if (i < n_new) {
@@ -828,6 +833,7 @@ __global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t*
             }
         }
         bits[i] = L;
+        keep[i] = (d >> 63) ? n : 0;  // colours to keep in the store
         ints = n;
     }
     block_add(&counters[2], ints);
@@ -900,6 +906,23 @@ __global__ void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_bas
             prev = v;
         }
     }
+}
+
+// Copies the lists of the chunk's new multi-colour sets into the job-wide
+// store (one wave per set) and points their slots at the copies.
+__global__ __launch_bounds__(TPB) void k_store_new(uint64_t n_new, const uint64_t* __restrict__ set_desc,
+                                                   const uint64_t* __restrict__ keep_off, uint64_t base, SetTables T) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (i >= n_new) return;
+    const uint64_t d = set_desc[i];
+    if (!(d >> 63)) return;
+    const uint64_t s = d & 0xFFFFFFFFFFFFull;
+    const uint32_t* src = slot_list(T, s);
+    const uint32_t n = T.mt_llen[s];
+    uint32_t* dst = T.store + base + keep_off[i];
+    for (uint32_t j = uint32_t(lane); j < n; j += 64) dst[j] = src[j];
+    if (lane == 0) T.mt_loff[s] = base + keep_off[i];
 }
 
 template <class K>
@@ -1554,7 +1577,8 @@ __global__ __launch_bounds__(TPB) void k_frag_write(uint64_t n_frags, const uint
 struct Phase2Bufs {
     // job-wide
     DevBuf kcnt, tup_off, btup, total, lbtup;
-    DevBuf single_min, single_cid, mt_hk, mt_cid, mt_loff, mt_llen, store, store_used;
+    DevBuf single_min, single_cid, mt_hk, mt_cid, mt_loff, mt_llen, store, keep, keep_off;
+    uint64_t store_used = 0;  // colours held in `store`
     DevBuf set_off, cs_words, frag_bytes;
     uint64_t mt_cap = 0, mt_count = 0;
     // per chunk (reused)
@@ -1598,7 +1622,7 @@ static SetTables set_tables(Phase2Bufs& P) {
     T.mt_llen = P.mt_llen.as<uint32_t>();
     T.cap = P.mt_cap;
     T.store = P.store.as<uint32_t>();
-    T.store_used = P.store_used.as<unsigned long long>();
+    T.cur_colors = nullptr;
     return T;
 }
 
@@ -1752,15 +1776,14 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors, cnt); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
-    const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1], multi_cols = h_cnt[7];
+    const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1];
 
     // --- colour sets: intern into the job-wide tables
     uint64_t cap = P.mt_cap;
     while (cap < 2 * (P.mt_count + n_multi)) cap <<= 1;
     if (cap != P.mt_cap) mt_resize(P, cap, st);
-    const uint64_t store_used = d2h<uint64_t>(P.store_used.p, st);
-    P.store.ensure_keep<uint32_t>(store_used + multi_cols + 1, store_used * 4, st);
     SetTables T = set_tables(P);
+    T.cur_colors = colors;
     uint32_t* node_slot = P.node_slot.ensure<uint32_t>(nn);
     { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, nhash, rank, node_bucket, b0, T, node_slot); }
     { KSCOPE("k_set_verify", st); k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
@@ -1784,7 +1807,14 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint64_t cid_base = P.n_sets;
     uint64_t* set_bits = P.set_bits.ensure<uint64_t>(n_new);
     uint64_t* set_boff = P.set_boff.ensure<uint64_t>(n_new + 1);
-    if (n_new) { KSCOPE("k_set_cid_len", st); k_set_cid_len<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, set_desc, T, NC, set_bits, cnt); }
+    uint64_t* keep = P.keep.ensure<uint64_t>(n_new);
+    uint64_t* keep_off = P.keep_off.ensure<uint64_t>(n_new + 1);
+    if (n_new) { KSCOPE("k_set_cid_len", st); k_set_cid_len<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, set_desc, T, NC, set_bits, keep, cnt); }
+    exclusive_scan<uint64_t, uint64_t>(keep, keep_off, n_new, keep_off + n_new, J.rs.scan_tmp, st);
+    const uint64_t kept = d2h<uint64_t>(keep_off + n_new, st);
+    P.store.ensure_keep<uint32_t>(P.store_used + kept + 1, P.store_used * 4, st);
+    T = set_tables(P);
+    T.cur_colors = colors;
     exclusive_scan<uint64_t, uint64_t>(set_bits, set_boff, n_new, set_boff + n_new, J.rs.scan_tmp, st);
     const uint64_t new_bits = d2h<uint64_t>(set_boff + n_new, st);
     const uint64_t bit_base = P.cs_bits;
@@ -1800,6 +1830,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
     NodeRec<K>* node_rec = P.node_rec.ensure<NodeRec<K>>(nn);
     { KSCOPE("k_node_cid", st); k_node_cid<K><<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, node_key, rank, node_rec); }
+    if (n_new) { KSCOPE("k_store_new", st); k_store_new<<<blocks_for(n_new, TPB / 64), TPB, 0, st>>>(n_new, set_desc, keep_off, P.store_used, T); }
+    P.store_used += kept;
     P.n_sets += n_new;
     P.cs_bits += new_bits;
     P.mt_count += n_new;  // upper bound of occupied slots (singletons included): sizes the table
@@ -1948,8 +1980,7 @@ static void phase2_impl(Job& J) {
     auto* single_cid = P.single_cid.ensure<uint32_t>(NC);
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(NC), TPB, 0, st>>>(single_min, NC, ~0ull); }
     { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(NC), TPB, 0, st>>>(single_cid, NC, NONE32); }
-    P.store_used.ensure<unsigned long long>(1);
-    { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(P.store_used.as<unsigned long long>(), 1, 0ull); }
+    P.store_used = 0;
     if (P.mt_cap == 0) {
         mt_resize(P, 1024, st);
     } else {  // clear, keeping the capacity of the previous run
