@@ -501,6 +501,7 @@ __device__ __forceinline__ uint64_t set_hash_fin(uint64_t sum, uint32_t n) {
 // tile (plus the BIG_RUN tuples after it) in LDS; runs longer than BIG_RUN
 // are queued for k_nodes_big.
 constexpr uint32_t NT_TILE = 1024;
+__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
 template <class K>
 __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint32_t* __restrict__ heads,
@@ -509,30 +510,55 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint32_t* __re
                                                const uint64_t* __restrict__ vals, K* __restrict__ node_key,
                                                uint8_t* __restrict__ node_meta, uint32_t* __restrict__ node_rank,
                                                uint32_t* __restrict__ node_ncol, uint64_t* __restrict__ node_hash,
-                                               uint32_t* __restrict__ colors, uint32_t* __restrict__ big,
-                                               unsigned long long* __restrict__ counters) {
+                                               uint32_t* __restrict__ colors, uint32_t* __restrict__ big_flag) {
     __shared__ uint64_t s_val[NT_TILE + BIG_RUN];
     __shared__ uint32_t s_list[NT_TILE];
-    __shared__ uint32_t s_n;
+    __shared__ uint32_t s_wn[TPB / 64 + 1];
     const uint64_t T0 = uint64_t(blockIdx.x) * NT_TILE;
     const uint64_t T1 = min(T0 + NT_TILE, nt), TE = min(T1 + BIG_RUN, nt);
-    if (threadIdx.x == 0) s_n = 0;
-    __syncthreads();
     for (uint32_t i = threadIdx.x; i < TE - T0; i += TPB) s_val[i] = vals[T0 + i];
-    for (uint32_t i = threadIdx.x; i < T1 - T0; i += TPB)
-        if (heads[T0 + i]) s_list[atomicAdd(&s_n, 1u)] = i;
+    // heads of the tile, in order: wave w lists the heads of its 4 x 64 tuples
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lanes_below(lane);
+    uint32_t my = 0;
+    bool h[NT_TILE / TPB];
+#pragma unroll
+    for (int r = 0; r < int(NT_TILE / TPB); ++r) {
+        const uint32_t i = uint32_t(wave) * (NT_TILE / (TPB / 64)) + uint32_t(r) * 64 + uint32_t(lane);
+        h[r] = T0 + i < T1 && heads[T0 + i];
+        my += __popcll(__ballot(h[r]));
+    }
+    if (lane == 0) s_wn[wave] = my;
     __syncthreads();
-    uint64_t prim = 0, multi = 0, multi_cols = 0;
-    for (uint32_t j = threadIdx.x; j < s_n; j += TPB) {
-        const uint32_t h = s_list[j];
-        const uint32_t t0 = uint32_t(T0 + h);
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int w = 0; w < TPB / 64; ++w) {
+            const uint32_t c = s_wn[w];
+            s_wn[w] = run;
+            run += c;
+        }
+        s_wn[TPB / 64] = run;
+    }
+    __syncthreads();
+    uint32_t at = s_wn[wave];
+#pragma unroll
+    for (int r = 0; r < int(NT_TILE / TPB); ++r) {
+        const uint32_t i = uint32_t(wave) * (NT_TILE / (TPB / 64)) + uint32_t(r) * 64 + uint32_t(lane);
+        const uint64_t m = __ballot(h[r]);
+        if (h[r]) s_list[at + __popcll(m & lt)] = i;
+        at += __popcll(m);
+    }
+    __syncthreads();
+    const uint32_t n_heads = s_wn[TPB / 64];
+    for (uint32_t j = threadIdx.x; j < n_heads; j += TPB) {
+        const uint32_t hd = s_list[j];
+        const uint32_t t0 = uint32_t(T0 + hd);
         const uint32_t n = hscan[t0];
         const uint32_t t1 = node_start[n + 1];
         node_key[n] = keys[t0];
-        if (t1 - t0 > BIG_RUN) {
-            big[atomicAdd(&counters[8], 1ull)] = n;
-            continue;
-        }
+        const bool is_big = t1 - t0 > BIG_RUN;
+        big_flag[n] = is_big;
+        if (is_big) continue;
         uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
         uint64_t sum = 0;
         bool primary = false;
@@ -556,16 +582,8 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint32_t* __re
         node_rank[n] = rank;
         node_ncol[n] = ncol;
         node_hash[n] = set_hash_fin(sum, ncol);
-        prim += primary;
-        multi += ncol >= 2;
-        multi_cols += ncol >= 2 ? ncol : 0;
     }
-    block_add(&counters[0], prim);
-    block_add(&counters[1], multi);
-    block_add(&counters[7], multi_cols);
 }
-
-__device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
 // One wave per queued node (the grid holds one wave per queued node): the
 // same reduction with the run read 256 occurrences at a time.  A lane's
@@ -576,12 +594,10 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ node_start,
                                                    const uint64_t* __restrict__ vals, uint8_t* __restrict__ node_meta,
                                                    uint32_t* __restrict__ node_rank, uint32_t* __restrict__ node_ncol,
-                                                   uint64_t* __restrict__ node_hash, uint32_t* __restrict__ colors,
-                                                   unsigned long long* __restrict__ counters) {
+                                                   uint64_t* __restrict__ node_hash, uint32_t* __restrict__ colors) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below(lane);
     const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-    uint64_t prim = 0, multi = 0, multi_cols = 0;
     if (i < nbig) {
         const uint32_t n = big[i];
         const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
@@ -635,15 +651,31 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
             node_rank[n] = rank;
             node_ncol[n] = ncol;
             node_hash[n] = set_hash_fin(sum, ncol);
-            prim = primary;
-            multi = ncol >= 2;
-            multi_cols = ncol >= 2 ? ncol : 0;
         }
+    }
+}
+
+// Indices of flagged items: list[scan[i]] = i.
+__global__ void k_compact(uint64_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
+                          uint32_t* __restrict__ list) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n && flag[i]) list[scan[i]] = uint32_t(i);
+}
+
+// counters[0] += primary nodes, counters[1] += primary nodes of >= 2 colours
+__global__ void k_count_nodes(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
+                              unsigned long long* __restrict__ counters) {
+    uint64_t prim = 0, multi = 0;
+    for (uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < n_nodes; n += uint64_t(gridDim.x) * blockDim.x) {
+        const bool p = meta[n] & 4u;
+        prim += p;
+        multi += p && ncol[n] >= 2;
     }
     block_add(&counters[0], prim);
     block_add(&counters[1], multi);
-    block_add(&counters[7], multi_cols);
 }
+
+
 
 // ---------------------------------------------------------------------------
 // colour sets
@@ -1070,7 +1102,7 @@ struct WalkArgs {
     uint8_t* frag_dir;             // 1: the fragment is its position range read backwards
     uint32_t* fflag;               // 1 at the first position of a fragment
     // simple paths, recorded end to end
-    uint32_t* comp_x;              // owning end
+    const uint32_t* comp_x;        // owning end (the end list: a path is stored at its owner's index)
     uint8_t* comp_ex;              // port the path leaves it by
     uint32_t* comp_len;
     uint32_t* comp_nodes;          // [PATH_BUF] per path: node | orientation << 31 (short paths)
@@ -1289,17 +1321,17 @@ __global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a, const uint32_t* _
         }
         owner = cur >= x;
     }
-    const uint64_t c = wave_alloc(&a.counters[9], owner && simple);
     const uint64_t o = wave_alloc(&a.counters[10], owner && !simple);
-    if (owner && simple) {
-        a.comp_x[c] = x;
-        a.comp_ex[c] = uint8_t(ex0);
-        a.comp_len[c] = len;
-        if (len <= PATH_BUF)
+    if (blk < nblk && e < n_ends) {
+        // a path is stored at the index of its owning end (length 0: none)
+        a.comp_len[e] = owner && simple ? len : 0u;
+        a.comp_ex[e] = uint8_t(ex0);
+        if (owner && simple && len <= PATH_BUF)
 #pragma unroll
             for (uint32_t j = 0; j < PATH_BUF; ++j)
-                if (j < len) a.comp_nodes[c * PATH_BUF + j] = buf[j];
-    } else if (owner) {
+                if (j < len) a.comp_nodes[e * PATH_BUF + j] = buf[j];
+    }
+    if (owner && !simple) {
         a.other_x[o] = x;
         a.other_ex[o] = uint8_t(ex0);
     }
@@ -1313,6 +1345,7 @@ __global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= n_comp) return;
     const uint32_t L = a.comp_len[c];
+    if (L == 0) return;
     const uint64_t off = a.comp_off[c];
     if (L <= PATH_BUF) {
         for (uint32_t i = 0; i < L; ++i) {
@@ -1345,6 +1378,7 @@ __global__ void k_walk_frags(WalkArgs a, uint64_t n_comp) {
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c >= n_comp) return;
     const uint32_t L = a.comp_len[c];
+    if (L == 0) return;
     const uint64_t off = a.comp_off[c];
     uint8_t* rc = a.prc + off;
     const uint16_t* info = a.pinfo + off;
@@ -1585,11 +1619,11 @@ struct Phase2Bufs {
     DevBuf keys, vals, keys_t, vals_t, block_super;
     DevBuf cflag, cscan, cell_start, cell_mid, cell_over, cell_novf, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
     DevBuf heads, hscan, node_start, node_off;
-    DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, colors, counters;
+    DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_scan, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
     DevBuf node_bucket, tab_off, tab, node_of, node_mask, nb, link, visited;
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fscan, fkey;
-    DevBuf comp_x, comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
+    DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
     DevBuf node_rec, node_mm, eflag, escan, ends;
     DevBuf fkey_t, forder, forder_t, fsize, fboff;
     RadixScratch rs2;
@@ -1770,10 +1804,15 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint32_t* coff = node_start;
     unsigned long long* cnt = P.counters.ensure<unsigned long long>(N_COUNTERS);
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, N_COUNTERS, 0ull); }
-    { KSCOPE("k_nodes", st); k_nodes<K><<<blocks_for(nt, NT_TILE), TPB, 0, st>>>(nt, heads, hscan, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big, cnt); }
+    uint32_t* big_flag = P.big_flag.ensure<uint32_t>(nn);
+    uint32_t* big_scan = P.big_scan.ensure<uint32_t>(uint64_t(nn) + 1);
+    { KSCOPE("k_nodes", st); k_nodes<K><<<blocks_for(nt, NT_TILE), TPB, 0, st>>>(nt, heads, hscan, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
+    exclusive_scan<uint32_t, uint32_t>(big_flag, big_scan, nn, big_scan + nn, J.rs.scan_tmp, st);
+    const uint64_t nbig = d2h<uint32_t>(big_scan + nn, st);
     unsigned long long h_cnt[N_COUNTERS];
-    const uint64_t nbig = d2h<unsigned long long>(cnt + 8, st);
-    if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors, cnt); }
+    if (nbig) { KSCOPE("k_compact", st); k_compact<<<blocks_for(nn), TPB, 0, st>>>(nn, big_flag, big_scan, big); }
+    if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors); }
+    { KSCOPE("k_count_nodes", st); k_count_nodes<<<1024, TPB, 0, st>>>(nn, meta, ncol, cnt); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1];
@@ -1881,12 +1920,6 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.frag_dir = P.frag_dir.ensure<uint8_t>(nn);
     uint32_t* fflag = P.fflag.ensure<uint32_t>(nn);
     wa.fflag = fflag;
-    wa.comp_x = P.comp_x.ensure<uint32_t>(nn);
-    wa.comp_ex = P.comp_ex.ensure<uint8_t>(nn);
-    wa.comp_len = P.comp_len.ensure<uint32_t>(nn);
-    wa.comp_nodes = P.comp_nodes.ensure<uint32_t>(uint64_t(nn) * PATH_BUF);
-    uint32_t* comp_off = P.comp_off.ensure<uint32_t>(uint64_t(nn) + 1);
-    wa.comp_off = comp_off;
     wa.other_x = P.other_x.ensure<uint32_t>(nn);
     wa.other_ex = P.other_ex.ensure<uint8_t>(nn);
     uint8_t* prc = P.prc.ensure<uint8_t>(nn);
@@ -1906,10 +1939,16 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint32_t n_ends = d2h<uint32_t>(escan + nn, st);
     uint32_t* ends = P.ends.ensure<uint32_t>(n_ends);
     { KSCOPE("k_end_list", st); k_end_list<<<blocks_for(nn), TPB, 0, st>>>(nn, eflag, escan, ends); }
+    wa.comp_x = ends;
+    wa.comp_ex = P.comp_ex.ensure<uint8_t>(n_ends);
+    wa.comp_len = P.comp_len.ensure<uint32_t>(n_ends);
+    wa.comp_nodes = P.comp_nodes.ensure<uint32_t>(uint64_t(n_ends) * PATH_BUF);
+    uint32_t* comp_off = P.comp_off.ensure<uint32_t>(uint64_t(n_ends) + 1);
+    wa.comp_off = comp_off;
     if (n_ends) { KSCOPE("k_walk_ends", st); k_walk_ends<<<xcd_grid(blocks_for(n_ends)), TPB, 0, st>>>(wa, ends, n_ends); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
-    const uint64_t n_comp = h_cnt[9], n_other = h_cnt[10];
+    const uint64_t n_comp = n_ends, n_other = h_cnt[10];
     exclusive_scan<uint32_t, uint32_t>(wa.comp_len, comp_off, n_comp, comp_off + n_comp, J.rs.scan_tmp, st);
     const uint32_t path_nodes = n_comp ? d2h<uint32_t>(comp_off + n_comp, st) : 0u;
     // the general seed loop places its nodes after the recorded paths
