@@ -136,14 +136,16 @@ __device__ inline T block_exclusive_scan(T v, T* s_tmp, T& total) {
     return res;
 }
 
+// Chunk sums; loads are striped across the block (coalesced).
 template <class Tin, class Tout>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const Tin* __restrict__ in, uint64_t n,
                                                               Tout* __restrict__ sums) {
     __shared__ Tout s_tmp[SCAN_THREADS / 64 + 1];
     const uint64_t base = uint64_t(blockIdx.x) * SCAN_CHUNK;
     Tout acc = 0;
+#pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        uint64_t idx = base + uint64_t(i) * SCAN_THREADS + threadIdx.x;
+        const uint64_t idx = base + uint64_t(i) * SCAN_THREADS + threadIdx.x;
         if (idx < n) acc += Tout(in[idx]);
     }
     Tout total;
@@ -166,28 +168,45 @@ __global__ __launch_bounds__(1024) void k_scan_single(T* __restrict__ a, uint64_
     if (threadIdx.x == 0 && total_out) *total_out = carry;
 }
 
+// Down-sweep: the chunk is staged in LDS with striped (coalesced) loads,
+// each thread scans 16 consecutive items there, and the result leaves with
+// striped stores.  One pad word per 32 keeps the per-thread runs on
+// different banks.
+__device__ __forceinline__ uint32_t scan_pad(uint32_t i) { return i + (i >> 5); }
+
 template <class Tin, class Tout>
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_down(const Tin* __restrict__ in, uint64_t n,
                                                             const Tout* __restrict__ sums,
                                                             Tout* __restrict__ out) {
     __shared__ Tout s_tmp[SCAN_THREADS / 64 + 1];
-    // Blocked arrangement: thread t owns items [t*ITEMS, t*ITEMS+ITEMS).
-    const uint64_t base = uint64_t(blockIdx.x) * SCAN_CHUNK + uint64_t(threadIdx.x) * SCAN_ITEMS;
-    Tout v[SCAN_ITEMS];
-    Tout acc = 0;
+    __shared__ Tout s_v[SCAN_CHUNK + SCAN_CHUNK / 32];
+    const uint64_t base = uint64_t(blockIdx.x) * SCAN_CHUNK;
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        uint64_t idx = base + i;
-        v[i] = idx < n ? Tout(in[idx]) : Tout(0);
-        acc += v[i];
+        const uint32_t j = uint32_t(i) * SCAN_THREADS + threadIdx.x;
+        const uint64_t idx = base + j;
+        s_v[scan_pad(j)] = idx < n ? Tout(in[idx]) : Tout(0);
     }
+    __syncthreads();
+    const uint32_t first = threadIdx.x * SCAN_ITEMS;
+    Tout acc = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) acc += s_v[scan_pad(first + i)];
     Tout tot;
     Tout ex = block_exclusive_scan<Tout>(acc, s_tmp, tot) + sums[blockIdx.x];
 #pragma unroll
     for (int i = 0; i < SCAN_ITEMS; ++i) {
-        uint64_t idx = base + i;
-        if (idx < n) out[idx] = ex;
-        ex += v[i];
+        const uint32_t j = scan_pad(first + i);
+        const Tout v = s_v[j];
+        s_v[j] = ex;
+        ex += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const uint32_t j = uint32_t(i) * SCAN_THREADS + threadIdx.x;
+        const uint64_t idx = base + j;
+        if (idx < n) out[idx] = s_v[scan_pad(j)];
     }
 }
 
