@@ -157,11 +157,29 @@ CDBG_API int cdbg_gpu_job_supers(cdbg_gpu_job* job, int which, uint64_t* start, 
             if (dst && n) CDBG_HIP(hipMemcpyAsync(dst, src.p, n * elem, hipMemcpyDeviceToHost, J.st));
         };
         if (which == 0) {
-            cp(start, P.sk_start, 8);
-            cp(len, P.sk_len, 4);
-            cp(flags, P.sk_flags, 1);
-            cp(bucket, P.sk_bucket, 4);
-            cp(color, P.sk_color, 4);
+            // position order: the bucket order re-sorted by (start, length);
+            // two supers share a start only when the first is the one-(k-1)-mer
+            // run at the start of a run, and it is also the shorter one
+            std::vector<uint64_t> s(n);
+            std::vector<uint32_t> l(n), b(n), c(n);
+            std::vector<uint8_t> f(n);
+            cp(s.data(), P.b_start, 8);
+            cp(l.data(), P.b_len, 4);
+            cp(f.data(), P.b_flags, 1);
+            cp(b.data(), P.key, 4);
+            cp(c.data(), P.b_color, 4);
+            CDBG_HIP(hipStreamSynchronize(J.st));
+            std::vector<uint32_t> perm(n);
+            for (uint64_t i = 0; i < n; ++i) perm[i] = uint32_t(i);
+            std::sort(perm.begin(), perm.end(),
+                      [&](uint32_t x, uint32_t y) { return s[x] < s[y] || (s[x] == s[y] && l[x] < l[y]); });
+            for (uint64_t i = 0; i < n; ++i) {
+                if (start) start[i] = s[perm[i]];
+                if (len) len[i] = l[perm[i]];
+                if (flags) flags[i] = f[perm[i]];
+                if (bucket) bucket[i] = b[perm[i]];
+                if (color) color[i] = c[perm[i]];
+            }
         } else {
             cp(start, P.b_start, 8);
             cp(len, P.b_len, 4);

@@ -18,8 +18,8 @@ struct Phase1Bufs {
     DevBuf tile_cnt, tile_off, totals; // per-tile start/end counts and their scans
     DevBuf s_pos, s_bucket, s_flags;   // super starts (position order)
     DevBuf e_end, e_flags;             // super ends (position order)
-    DevBuf sk_start, sk_len, sk_flags, sk_bucket, sk_color;  // supers, position order
-    DevBuf key, val, key_t, val_t;     // bucket sort
+    DevBuf sk_rec, sk_rec_t;           // super records (16 B), bucket sort payload
+    DevBuf key, key_t;                 // bucket sort keys (bucket of each super)
     DevBuf b_start, b_len, b_flags, b_color;                // supers, bucket-major
     DevBuf bucket_off;                 // B+1
     DevBuf rank0;                      // per record: stream rank of its first k-mer (bucket files only)

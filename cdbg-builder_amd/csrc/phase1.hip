@@ -223,12 +223,19 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     }
 }
 
+// One super-k-mer record as the bucket sort moves it.
+struct SuperRec {
+    uint64_t start;
+    uint32_t lenfl;  // length << 4 | flags
+    uint32_t color;
+};
+
+// Pairs the i-th start with the i-th end: record, and its bucket as sort key.
 __global__ void k_zip_supers(uint64_t n, const uint64_t* __restrict__ s_pos, const uint32_t* __restrict__ s_bucket,
                              const uint8_t* __restrict__ s_flags, const uint64_t* __restrict__ e_end,
                              const uint8_t* __restrict__ e_flags, const uint64_t* __restrict__ color_off,
-                             uint32_t num_colors, uint64_t* __restrict__ sk_start, uint32_t* __restrict__ sk_len,
-                             uint8_t* __restrict__ sk_flags, uint32_t* __restrict__ sk_bucket,
-                             uint32_t* __restrict__ sk_color) {
+                             uint32_t num_colors, SuperRec* __restrict__ rec, uint32_t* __restrict__ key,
+                             unsigned long long* __restrict__ too_long) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint64_t s = s_pos[i];
@@ -241,33 +248,23 @@ __global__ void k_zip_supers(uint64_t n, const uint64_t* __restrict__ s_pos, con
         else
             hi = mid;
     }
-    sk_start[i] = s;
-    sk_len[i] = uint32_t(e_end[i] - s);
-    sk_flags[i] = uint8_t(s_flags[i] | e_flags[i]);
-    sk_bucket[i] = s_bucket[i];
-    sk_color[i] = lo;
+    const uint64_t len = e_end[i] - s;
+    if (len >= (uint64_t(1) << 28)) atomicAdd(too_long, 1ull);
+    rec[i] = SuperRec{s, uint32_t(len << 4) | uint32_t(s_flags[i] | e_flags[i]), lo};
+    key[i] = s_bucket[i];
 }
 
-__global__ void k_iota_copy(uint64_t n, const uint32_t* __restrict__ bucket, uint32_t* __restrict__ key,
-                            uint32_t* __restrict__ val) {
+// The bucket-ordered records as separate arrays.
+__global__ void k_split_supers(uint64_t n, const SuperRec* __restrict__ rec, uint64_t* __restrict__ b_start,
+                               uint32_t* __restrict__ b_len, uint8_t* __restrict__ b_flags,
+                               uint32_t* __restrict__ b_color) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    key[i] = bucket[i];
-    val[i] = uint32_t(i);
-}
-
-__global__ void k_gather_supers(uint64_t n, const uint32_t* __restrict__ perm, const uint64_t* __restrict__ sk_start,
-                                const uint32_t* __restrict__ sk_len, const uint8_t* __restrict__ sk_flags,
-                                const uint32_t* __restrict__ sk_color, uint64_t* __restrict__ b_start,
-                                uint32_t* __restrict__ b_len, uint8_t* __restrict__ b_flags,
-                                uint32_t* __restrict__ b_color) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t s = perm[i];
-    b_start[i] = sk_start[s];
-    b_len[i] = sk_len[s];
-    b_flags[i] = sk_flags[s];
-    b_color[i] = sk_color[s];
+    const SuperRec r = rec[i];
+    b_start[i] = r.start;
+    b_len[i] = r.lenfl >> 4;
+    b_flags[i] = uint8_t(r.lenfl & 15u);
+    b_color[i] = r.color;
 }
 
 // off[b] = first index i with key[i] >= b, for b in [0, B].
@@ -287,6 +284,18 @@ __global__ void k_bucket_offsets(uint64_t n, const uint32_t* __restrict__ key, u
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+inline uint64_t d2h_u64(const uint64_t* d, hipStream_t st) {
+    uint64_t v = 0;
+    CDBG_HIP(hipMemcpyAsync(&v, d, sizeof v, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    return v;
+}
 
 }  // namespace
 
@@ -347,37 +356,31 @@ void run_phase1(Job& J) {
     a.e_flags = P.e_flags.ensure<uint8_t>(n);
     { KSCOPE("k_superkmers", st); k_superkmers<true><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
 
-    uint64_t* sk_start = P.sk_start.ensure<uint64_t>(n);
-    uint32_t* sk_len = P.sk_len.ensure<uint32_t>(n);
-    uint8_t* sk_flags = P.sk_flags.ensure<uint8_t>(n);
-    uint32_t* sk_bucket = P.sk_bucket.ensure<uint32_t>(n);
-    uint32_t* sk_color = P.sk_color.ensure<uint32_t>(n);
+    // Bucket scatter: stable sort of the records by bucket.
+    SuperRec* rec = (SuperRec*)P.sk_rec.ensure<SuperRec>(n);
+    SuperRec* rec_t = (SuperRec*)P.sk_rec_t.ensure<SuperRec>(n);
+    uint32_t* key = P.key.ensure<uint32_t>(n);
+    uint32_t* key_t = P.key_t.ensure<uint32_t>(n);
+    auto* too_long = P.totals.ensure<uint64_t>(3) + 2;
+    { KSCOPE("k_fill", st); k_fill_u64<<<1, 64, 0, st>>>(too_long, 1, 0ull); }
     if (n) {
         { KSCOPE("k_zip_supers", st); k_zip_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, a.s_pos, a.s_bucket, a.s_flags, a.e_end, a.e_flags,
-                                                         J.color_off.as<uint64_t>(), J.num_colors, sk_start,
-                                                         sk_len, sk_flags, sk_bucket, sk_color); }
+                                                         J.color_off.as<uint64_t>(), J.num_colors, rec, key,
+                                                         (unsigned long long*)too_long); }
     }
-    // Bucket scatter: stable sort of super indices by bucket, then gather.
-    uint32_t* key = P.key.ensure<uint32_t>(n);
-    uint32_t* val = P.val.ensure<uint32_t>(n);
-    uint32_t* key_t = P.key_t.ensure<uint32_t>(n);
-    uint32_t* val_t = P.val_t.ensure<uint32_t>(n);
-    if (n) { KSCOPE("k_iota_copy", st); k_iota_copy<<<blocks_for(n, 256), 256, 0, st>>>(n, sk_bucket, key, val); }
     int bbits = 0;
     while ((uint64_t(1) << bbits) < J.B) ++bbits;
     SortPlan plan;
     plan.build({0, n});
-    segmented_radix_sort<uint32_t, uint32_t>(key, val, key_t, val_t, plan, 0, bbits, bbits > 11 ? (bbits + 1) / 2 : bbits,
+    segmented_radix_sort<uint32_t, SuperRec>(key, rec, key_t, rec_t, plan, 0, bbits, bbits > 10 ? (bbits + 1) / 2 : bbits,
                                              J.rs, st, "buckets");
     uint64_t* b_start = P.b_start.ensure<uint64_t>(n);
     uint32_t* b_len = P.b_len.ensure<uint32_t>(n);
     uint8_t* b_flags = P.b_flags.ensure<uint8_t>(n);
     uint32_t* b_color = P.b_color.ensure<uint32_t>(n);
     uint64_t* boff = P.bucket_off.ensure<uint64_t>(J.B + 1);
-    if (n) {
-        { KSCOPE("k_gather_supers", st); k_gather_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, val, sk_start, sk_len, sk_flags, sk_color, b_start,
-                                                            b_len, b_flags, b_color); }
-    }
+    if (n) { KSCOPE("k_split_supers", st); k_split_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, rec, b_start, b_len, b_flags, b_color); }
+    if (d2h_u64(too_long, st)) throw Error("phase 1: a super-k-mer is longer than 2^28 bases");
     { KSCOPE("k_bucket_offsets", st); k_bucket_offsets<<<blocks_for(std::max<uint64_t>(n, uint64_t(J.B) + 1), 256), 256, 0, st>>>(n, key, J.B, boff); }
     CDBG_HIP(hipGetLastError());
     J.h_bucket_off.resize(J.B + 1);
