@@ -957,8 +957,13 @@ __global__ __launch_bounds__(TPB) void k_store_new(uint64_t n_new, const uint64_
     if (lane == 0) T.mt_loff[s] = base + keep_off[i];
 }
 
+// What the walk gathers per node besides its meta and mask.
 template <class K>
-struct NodeRec;
+struct NodeRec {
+    K key;
+    uint32_t rank;
+    uint32_t cid;
+};
 
 // Colour-set id of every node, stored with its k-mer and seed rank.
 template <class K>
@@ -1117,14 +1122,9 @@ struct WalkArgs {
     uint32_t* pcid;                // the node's colour-set id
     const void* node_rec;          // NodeRec<K> per node
     const uint16_t* node_mm;       // meta | mask << 8 per node
+    bool mark_visited;             // nodes outside simple paths exist: keep `visited` exact
 };
 
-template <class K>
-struct NodeRec {
-    K key;
-    uint32_t rank;
-    uint32_t cid;
-};
 
 constexpr uint32_t PATH_BUF = 8;
 
@@ -1351,7 +1351,7 @@ __global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
         for (uint32_t i = 0; i < L; ++i) {
             const uint32_t v = a.comp_nodes[c * PATH_BUF + i];
             put_position<K>(a, off + i, v & 0x7FFFFFFFu, uint8_t(v >> 31));
-            a.visited[v & 0x7FFFFFFFu] = 1;
+            if (a.mark_visited) a.visited[v & 0x7FFFFFFFu] = 1;
         }
         return;
     }
@@ -1359,7 +1359,7 @@ __global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
     uint8_t o = ex == PORT_L ? 1 : 0;
     for (uint32_t i = 0;; ++i) {
         put_position<K>(a, off + i, cur, o);
-        a.visited[cur] = 1;
+        if (a.mark_visited) a.visited[cur] = 1;
         if (i + 1 == L) break;
         const uint32_t l = a.link[2 * cur + ex];
         const uint32_t q = l & 1u;
@@ -1903,7 +1903,6 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
 
     // --- walk
     uint8_t* visited = P.visited.ensure<uint8_t>(nn);
-    { KSCOPE("k_fill", st); k_fill<uint8_t><<<blocks_for(nn), TPB, 0, st>>>(visited, nn, uint8_t(0)); }
     WalkArgs wa{};
     wa.n_nodes = nn;
     wa.link = link;
@@ -1951,12 +1950,15 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint64_t n_comp = n_ends, n_other = h_cnt[10];
     exclusive_scan<uint32_t, uint32_t>(wa.comp_len, comp_off, n_comp, comp_off + n_comp, J.rs.scan_tmp, st);
     const uint32_t path_nodes = n_comp ? d2h<uint32_t>(comp_off + n_comp, st) : 0u;
+    // nodes outside simple paths (hairpin paths, cycles) need the visited marks
+    wa.mark_visited = path_nodes < nn;
+    if (wa.mark_visited) { KSCOPE("k_fill", st); k_fill<uint8_t><<<blocks_for(nn), TPB, 0, st>>>(visited, nn, uint8_t(0)); }
     // the general seed loop places its nodes after the recorded paths
     { KSCOPE("k_set_u64", st); k_set_u64<<<1, 64, 0, st>>>(cnt + 3, path_nodes); }
     if (n_comp) { KSCOPE("k_walk_record", st); k_walk_record<K><<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
     if (n_comp) { KSCOPE("k_walk_frags", st); k_walk_frags<<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
     if (n_other) { KSCOPE("k_walk_other", st); k_walk_other<K><<<blocks_for(n_other), TPB, 0, st>>>(wa, n_other); }
-    { KSCOPE("k_walk_cycles", st); k_walk_cycles<K><<<blocks_for(nn), TPB, 0, st>>>(wa); }
+    if (wa.mark_visited) { KSCOPE("k_walk_cycles", st); k_walk_cycles<K><<<blocks_for(nn), TPB, 0, st>>>(wa); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     if (h_cnt[6]) throw Error("phase 2: colour-set content hash collision (" + std::to_string(h_cnt[6]) + " nodes)");
