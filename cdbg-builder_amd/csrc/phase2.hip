@@ -67,6 +67,11 @@ __device__ __forceinline__ uint32_t v_color(uint64_t v) { return uint32_t(v >> 3
 __device__ __forceinline__ bool v_owned(uint64_t v) { return (v >> 2) & 1u; }
 __device__ __forceinline__ uint32_t v_contrib(uint64_t v) { return uint32_t(v) & 3u; }
 
+// Slots after each bucket's table (ordered placement may run past its end)
+// and the largest displacement the ordered table accepts.
+constexpr uint64_t TAB_SLACK = 256;
+constexpr int64_t TAB_MAX_DISP = 192;
+
 // Per-bucket k-mer table: open addressing over groups of 64 bytes (8 u64 or
 // 4 128-bit keys), probed a group at a time; the node of slot i is node_of[i].
 // An empty slot holds all-ones (no k-mer of 2k <= 126 bits has it).
@@ -990,7 +995,7 @@ __global__ void k_table_insert(uint32_t n_nodes, const K* __restrict__ node_key,
     const uint64_t n = uint64_t(blk) * TPB + threadIdx.x;
     if (blk >= nblk || n >= n_nodes) return;
     const uint32_t b = node_bucket[n];
-    const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base) / G;
+    const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base - TAB_SLACK) / G;
     const K key = node_key[n];
     uint64_t g = KOps<K>::hash(key) & (ng - 1);
     for (uint64_t probe = 0; probe < ng; ++probe) {
@@ -1027,11 +1032,97 @@ __device__ __forceinline__ uint64_t tab_find(const K* __restrict__ tab, uint64_t
     return ~0ull;
 }
 
+// Ordered per-bucket table.  Nodes of a bucket are sorted by k-mer, and a
+// key's home slot is a monotone function of the key, flattened by the
+// distribution of canonical k-mers (the smaller of a random word and its
+// reverse complement: CDF 1 - (1 - x)^2), computed in integers so that it is
+// exactly monotone.  Keys are placed in order at max(home, previous + 1), so
+// a lookup scans forward from home and stops at the first key >= the target.
+// The SLACK slots after each bucket's table absorb the last clusters.
+
+template <class K>
+__device__ __forceinline__ uint32_t top32(const K& key, uint32_t k);
+template <>
+__device__ __forceinline__ uint32_t top32<uint64_t>(const uint64_t& key, uint32_t k) {
+    const uint32_t b = 2 * k;
+    return b >= 32 ? uint32_t(key >> (b - 32)) : uint32_t(key << (32 - b));
+}
+template <>
+__device__ __forceinline__ uint32_t top32<Key128>(const Key128& key, uint32_t k) {
+    const uint32_t b = 2 * k;  // 66 .. 126 for k > 32
+    if (b <= 64) return top32<uint64_t>(key.lo, k);
+    const uint32_t sh = b - 32;  // >= 34
+    return sh >= 64 ? uint32_t(key.hi >> (sh - 64)) : uint32_t((key.lo >> sh) | (key.hi << (64 - sh)));
+}
+
+template <class K>
+__device__ __forceinline__ uint64_t home_of(const K& key, uint32_t k, uint64_t size) {
+    const uint64_t u = (uint64_t(1) << 32) - top32<K>(key, k);  // 1 .. 2^32
+    const uint64_t f = uint64_t(0) - u * u;                    // 2^64 - u^2 (0 when u = 2^32)
+    return ((f >> 32) * size) >> 32;
+}
+
+template <class K>
+__device__ __forceinline__ bool key_lt(const K& a, const K& b) {
+    return !KOps<K>::le(b, a);
+}
+
+// v[n] = bucket << 40 | (home slot - n): a running maximum over the bucket's
+// nodes then gives each node's slot.
+template <class K>
+__global__ void k_otab_home(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
+                            const uint64_t* __restrict__ tab_off, uint32_t k, int64_t* __restrict__ v) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const uint32_t b = node_bucket[n];
+    const uint64_t base = tab_off[b], size = tab_off[b + 1] - base - TAB_SLACK;
+    const uint64_t h = base + home_of<K>(node_key[n], k, size);
+    v[n] = (int64_t(b) << 40) + (int64_t(h) - int64_t(n));
+}
+
+template <class K>
+__global__ void k_otab_place(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
+                             const uint64_t* __restrict__ tab_off, uint32_t k, const int64_t* __restrict__ vmax,
+                             K* __restrict__ tab, uint32_t* __restrict__ node_of, uint32_t* __restrict__ bad) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    const uint32_t b = node_bucket[n];
+    const uint64_t base = tab_off[b], end = tab_off[b + 1], size = end - base - TAB_SLACK;
+    const uint64_t slot = uint64_t(int64_t(n) + (vmax[n] - (int64_t(b) << 40)));
+    const uint64_t h = base + home_of<K>(node_key[n], k, size);
+    if (slot >= end || int64_t(slot - h) > TAB_MAX_DISP) {
+        *bad = 1u;  // clustered keys: the caller builds the hashed table instead
+        return;
+    }
+    tab[slot] = node_key[n];
+    node_of[slot] = uint32_t(n);
+}
+
+template <class K>
+__device__ __forceinline__ uint64_t otab_find(const K* __restrict__ tab, uint64_t base, uint64_t size, uint64_t end,
+                                              const K& key, uint32_t k) {
+    constexpr uint32_t G = Tab<K>::G;
+    uint64_t s = base + home_of<K>(key, k, size);
+    while (s < end) {
+        const uint64_t g0 = s & ~uint64_t(G - 1);
+        K v[G];
+#pragma unroll
+        for (uint32_t j = 0; j < G; ++j) v[j] = tab[g0 + j];
+#pragma unroll
+        for (uint32_t j = 0; j < G; ++j) {
+            if (g0 + j < s) continue;
+            if (!key_lt<K>(v[j], key)) return KOps<K>::eq(v[j], key) ? g0 + j : ~0ull;
+        }
+        s = g0 + G;
+    }
+    return ~0ull;
+}
+
 // mask bits 0..3: successors of the canonical k-mer, 4..7: successors of its
 // reverse complement (local_ext_mask, bucket_walker.hpp:260-271).  For a side
 // with exactly one present extension, nb[2n+port] = (neighbour << 1) | entry
 // port: the neighbour is entered at its L side iff the extension is canonical.
-template <class K>
+template <class K, bool ORDERED>
 __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
                         const uint64_t* __restrict__ tab_off, const K* __restrict__ tab,
                         const uint32_t* __restrict__ node_of, uint32_t k, const uint8_t* __restrict__ meta,
@@ -1042,7 +1133,10 @@ __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const 
     const uint64_t n = uint64_t(blk) * TPB + threadIdx.x;
     if (blk >= nblk || n >= n_nodes) return;
     const uint32_t b = node_bucket[n];
-    const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base) / G;
+    const uint64_t base = tab_off[b], end = tab_off[b + 1], size = end - base - TAB_SLACK, ng = size / G;
+    auto find = [&](const K& e) -> uint64_t {
+        return ORDERED ? otab_find<K>(tab, base, size, end, e, k) : tab_find<K>(tab, base, ng, e);
+    };
     const K can = node_key[n];
     const K rv = O::rc(can, k);
     uint32_t mask = 0, qR = 0, qL = 0;
@@ -1052,7 +1146,7 @@ __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const 
         K e = O::shift_append(can, nt, k);
         K er = O::rc(e, k);
         bool ec = O::le(e, er);
-        uint64_t sl = tab_find<K>(tab, base, ng, ec ? e : er);
+        uint64_t sl = find(ec ? e : er);
         if (sl != ~0ull) {
             mask |= 1u << nt;
             sR = sl;
@@ -1061,7 +1155,7 @@ __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const 
         e = O::shift_append(rv, nt, k);
         er = O::rc(e, k);
         ec = O::le(e, er);
-        sl = tab_find<K>(tab, base, ng, ec ? e : er);
+        sl = find(ec ? e : er);
         if (sl != ~0ull) {
             mask |= 1u << (4 + nt);
             sL = sl;
@@ -1621,7 +1715,7 @@ struct Phase2Bufs {
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_scan, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
-    DevBuf node_bucket, tab_off, tab, node_of, node_mask, nb, link, visited;
+    DevBuf node_bucket, tab_off, tab, node_of, tab_v, tab_max, tab_bad, tab_scan, node_mask, nb, link, visited;
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fscan, fkey;
     DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
     DevBuf node_rec, node_mm, eflag, escan, ends;
@@ -1884,7 +1978,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         const uint64_t n_b = h_node_off[b + 1] - h_node_off[b];
         uint64_t c = Tab<K>::G;  // slots, a power-of-two number of groups
         while (c * 5 < n_b * 8) c <<= 1;  // load factor <= 0.625
-        h_tab_off[b + 1] = h_tab_off[b] + (n_b ? c : 0);
+        h_tab_off[b + 1] = h_tab_off[b] + c + TAB_SLACK;
     }
     uint64_t* tab_off = P.tab_off.ensure<uint64_t>(nb + 1);
     CDBG_HIP(hipMemcpyAsync(tab_off, h_tab_off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
@@ -1893,12 +1987,28 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     K* tab = P.tab.ensure<K>(tab_n);
     uint32_t* node_of = P.node_of.ensure<uint32_t>(tab_n);
     { KSCOPE("k_fill", st); k_fill<uint64_t><<<blocks_for(tab_n * sizeof(K) / 8), TPB, 0, st>>>((uint64_t*)tab, tab_n * sizeof(K) / 8, ~0ull); }
-    { KSCOPE("k_table_insert", st); k_table_insert<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of); }
+    // ordered table by a running maximum; hashed table when keys cluster
+    int64_t* tv = P.tab_v.ensure<int64_t>(nn);
+    int64_t* tmax = P.tab_max.ensure<int64_t>(nn);
+    uint32_t* bad = (uint32_t*)P.tab_bad.ensure<uint32_t>(1);
+    { KSCOPE("k_fill", st); k_fill<uint32_t><<<1, 64, 0, st>>>(bad, 1, 0u); }
+    { KSCOPE("k_otab_home", st); k_otab_home<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, k, tv); }
+    inclusive_max_scan(tv, tmax, nn, P.tab_scan, st);
+    { KSCOPE("k_otab_place", st); k_otab_place<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, k, tmax, tab, node_of, bad); }
+    uint32_t h_bad = 0;
+    CDBG_HIP(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    const bool ordered = h_bad == 0;
+    if (!ordered) {
+        { KSCOPE("k_fill", st); k_fill<uint64_t><<<blocks_for(tab_n * sizeof(K) / 8), TPB, 0, st>>>((uint64_t*)tab, tab_n * sizeof(K) / 8, ~0ull); }
+        { KSCOPE("k_table_insert", st); k_table_insert<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of); }
+    }
     uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
     uint32_t* nbr = P.nb.ensure<uint32_t>(2ull * nn);
     uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
     uint16_t* node_mm = P.node_mm.ensure<uint16_t>(nn);
-    { KSCOPE("k_masks", st); k_masks<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
+    if (ordered) { KSCOPE("k_masks", st); k_masks<K, true><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
+    else { KSCOPE("k_masks", st); k_masks<K, false><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
     { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link); }
 
     // --- walk
