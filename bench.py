@@ -41,25 +41,38 @@ WORKLOAD = dict(seed=2, snp_rate=0.02, indel_rate=0.002, indel_max=10, pav_frac=
                 pav_min=1000, pav_max=5000, n_inversions=50, inv_min=10000, inv_max=100000)
 
 
-def step_bytes(name, s, k):
+def step_bytes(name, s, k, launches, chunks):
     """Algorithmic (compulsory) HBM bytes moved by all launches of `name` in
-    one step: each buffer touched once, from the job's sizes."""
+    one step: every buffer touched once, from the job's sizes.  `launches` is
+    the kernel's launch count per step, `chunks` the phase-2 chunks."""
     wk = 8 if k <= 31 else 16
-    passes = -(-2 * k // 8)  # 8-bit digits over the 2k key bits
     n, nn, ns, nb = s["num_tuples"], s["num_nodes"], s["num_supers"], s["num_bases"]
+    nf, F, nk = s["num_frags"], s["frag_bytes"], s["num_kmers"]
+    passes = launches / max(chunks, 1)  # radix passes per chunk
     table = {
-        # read key+value, write key+value, every pass
+        # the tuples of every chunk: read key+value, write key+value, per global pass
         "k_radix_scatter:tuples": passes * 2 * n * (wk + 8),
-        # read keys, every pass
         "k_radix_hist:tuples": passes * n * wk,
+        # every tuple is sorted in LDS by exactly one cell class: read and written once
+        "k_cell_sort": n * (wk + 8) * 2,
         # super records + 2-bit bases in, one (k-mer, value) tuple out per k-mer
         "k_expand": ns * 29 + nb // 4 + n * (wk + 8),
-        # ASCII in, 2-bit codes + ACGT bitmap out
-        "k_pack2bit": nb + nb // 4 + nb // 8,
-        # tuple values in, node records out
-        "k_node_reduce": n * 8 + nn * (4 + wk) + nn * (wk + 9),
-        # node keys in, 8 probes of one table slot, mask + neighbours out
-        "k_masks": nn * wk + 8 * nn * (16 if wk == 8 else 32) + nn * 9,
+        # count pass and write pass over the 2-bit bases and the ACGT bitmap; starts/ends out
+        "k_superkmers": 2 * (nb // 4 + nb // 8) + ns * 22,
+        # values, head flags in; node key, meta, rank, colour count, set hash, long-run flag,
+        # and the owned colours out
+        "k_nodes": n * (8 + 4) + nn * (4 + 4 + wk) + nn * (wk + 1 + 4 + 4 + 8 + 4) + n * 4,
+        # node key and meta in, 8 neighbour probes (key + node), mask + meta/mask + links out
+        "k_masks": nn * (wk + 1) + 8 * nn * (wk + 4) + nn * (1 + 2 + 8),
+        # ordered table: key + running max in, key + node out
+        "k_otab_place": nn * (wk + 4 + 8) + nn * (wk + 4),
+        # per path position: node record + meta/mask + path entry in; key, cid, rank,
+        # meta/mask, orientation, fragment flag out
+        "k_walk_record": nn * ((wk + 8) + 2 + 4) + nn * (wk + 4 + 4 + 2 + 1 + 4),
+        # fragment order + descriptors, per position key/cid/orientation, records out
+        "k_frag_write": nf * (4 + 4 + 1 + 1) + nn * (wk + 4 + 1) + F,
+        # primary nodes: meta, bucket, rank, colour count, offset, hash, first colour in; slot out
+        "k_set_insert": nn * 1 + nk * (4 + 4 + 4 + 4 + 8 + 4 + 4),
     }
     return table.get(name)
 
@@ -86,7 +99,7 @@ def main():
     ap.add_argument("--k", type=int, default=31)
     ap.add_argument("--m", type=int, default=21)
     ap.add_argument("--buckets", type=int, default=1024)
-    ap.add_argument("--cpu-sample-colors", type=int, default=12)
+    ap.add_argument("--cpu-sample-colors", type=int, default=30)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel totals to stderr")
@@ -134,11 +147,19 @@ def main():
         for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
             print(f"  {name:32s} {cnt:6d} {ms:10.2f} ms", file=sys.stderr)
 
-    # roofline of the dominant kernel that has an algorithmic-byte count
+    # roofline of the dominant kernel that has an algorithmic-byte count, and
+    # the aggregate over every modelled phase-2 kernel
+    chunks = job.num_chunks()
     roof = None
+    agg_b = agg_ms = 0.0
     for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
-        b = step_bytes(name, s, args.k)
+        b = step_bytes(name, s, args.k, cnt / args.steps, chunks)
         if b is None or cnt == 0:
+            continue
+        if name not in ("k_superkmers",):
+            agg_b += b * args.steps
+            agg_ms += ms
+        if roof is not None:
             continue
         per_launch = b * args.steps / cnt
         avg_ms = ms / cnt
@@ -149,7 +170,10 @@ def main():
                 "launches_per_step": cnt // args.steps,
                 "copy_probe_gbps": round(copy_gbps, 1),
                 "share_of_step": round(ms / args.steps / ms_step, 3)}
-        break
+    if roof is not None and agg_ms > 0:
+        ach = agg_b / (agg_ms / 1e3) / 1e9
+        roof["phase2_modelled"] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                                   "share_of_step": round(agg_ms / args.steps / ms_step, 3)}
     top = sorted(kt.items(), key=lambda x: -x[1][1])[:6]
     kernels = {n: round(ms / args.steps, 2) for n, (c, ms) in top}
 
@@ -177,7 +201,7 @@ def main():
                    "num_color_sets": s["num_color_sets"], "num_frags": s["num_frags"]},
         "roofline": roof,
         "top_kernels_ms_per_step": kernels,
-        "chunks": job.num_chunks(),
+        "chunks": chunks,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
