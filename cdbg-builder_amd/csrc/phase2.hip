@@ -1141,27 +1141,60 @@ __global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const 
     const K rv = O::rc(can, k);
     uint32_t mask = 0, qR = 0, qL = 0;
     uint64_t sR = ~0ull, sL = ~0ull;
+    // the extensions x.A .. x.T of one side: a forward-canonical extension is
+    // a key next to x's shifted key, so in the ordered table (k >= 17: the low
+    // base is below the home bits) one forward scan finds all of them; the
+    // others are reverse complements and are looked up one by one
+    auto side = [&](const K& x, uint32_t shift, uint64_t& sl_out, uint32_t& q_out) {
+        K e[4], er[4];
+        bool ec[4];
 #pragma unroll
-    for (uint32_t nt = 0; nt < 4; ++nt) {
-        K e = O::shift_append(can, nt, k);
-        K er = O::rc(e, k);
-        bool ec = O::le(e, er);
-        uint64_t sl = find(ec ? e : er);
-        if (sl != ~0ull) {
-            mask |= 1u << nt;
-            sR = sl;
-            qR = ec ? PORT_L : PORT_R;
+        for (uint32_t nt = 0; nt < 4; ++nt) {
+            e[nt] = O::shift_append(x, nt, k);
+            er[nt] = O::rc(e[nt], k);
+            ec[nt] = O::le(e[nt], er[nt]);
         }
-        e = O::shift_append(rv, nt, k);
-        er = O::rc(e, k);
-        ec = O::le(e, er);
-        sl = find(ec ? e : er);
-        if (sl != ~0ull) {
-            mask |= 1u << (4 + nt);
-            sL = sl;
-            qL = ec ? PORT_L : PORT_R;
+        const bool grouped = ORDERED && k >= 17 && (ec[0] || ec[1] || ec[2] || ec[3]);
+        if (grouped) {
+            constexpr uint32_t G = Tab<K>::G;
+            uint64_t sl = base + home_of<K>(e[0], k, size);
+            bool done = false;
+            while (!done && sl < end) {
+                const uint64_t g0 = sl & ~uint64_t(G - 1);
+                K v[G];
+#pragma unroll
+                for (uint32_t j = 0; j < G; ++j) v[j] = tab[g0 + j];
+#pragma unroll
+                for (uint32_t j = 0; j < G; ++j) {
+                    if (done || g0 + j < sl || key_lt<K>(v[j], e[0])) continue;
+                    if (key_lt<K>(e[3], v[j])) {
+                        done = true;
+                        continue;
+                    }
+#pragma unroll
+                    for (uint32_t nt = 0; nt < 4; ++nt)
+                        if (ec[nt] && KOps<K>::eq(v[j], e[nt])) {
+                            mask |= 1u << (shift + nt);
+                            sl_out = g0 + j;
+                            q_out = PORT_L;
+                        }
+                }
+                sl = g0 + G;
+            }
         }
-    }
+#pragma unroll
+        for (uint32_t nt = 0; nt < 4; ++nt) {
+            if (grouped && ec[nt]) continue;
+            const uint64_t sl = find(ec[nt] ? e[nt] : er[nt]);
+            if (sl != ~0ull) {
+                mask |= 1u << (shift + nt);
+                sl_out = sl;
+                q_out = ec[nt] ? PORT_L : PORT_R;
+            }
+        }
+    };
+    side(can, 0, sR, qR);
+    side(rv, 4, sL, qL);
     mask_out[n] = uint8_t(mask);
     node_mm[n] = uint16_t(meta[n] | (mask << 8));
     nb[2 * n + PORT_R] = __popc(mask & 0xfu) == 1 ? (node_of[sR] << 1) | qR : NONE32;
@@ -1433,33 +1466,78 @@ __global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a, const uint32_t* _
 
 // Writes path c end to end from position comp_off[c]: per position the
 // node's data and its orientation along the path (1 when it is passed
-// entering by R).
+// entering by R).  The paths of a wave's ends occupy one contiguous range of
+// positions; when it holds at most REC_CAP positions it is assembled in LDS
+// and written out with coalesced stores.
+constexpr uint32_t REC_CAP = 512;
+
 template <class K>
-__global__ void k_walk_record(WalkArgs a, uint64_t n_comp) {
+__global__ __launch_bounds__(TPB) void k_walk_record(WalkArgs a, uint64_t n_comp) {
+    __shared__ K s_key[TPB / 64][REC_CAP];
+    __shared__ uint32_t s_cid[TPB / 64][REC_CAP];
+    __shared__ uint32_t s_rank[TPB / 64][REC_CAP];
+    __shared__ uint16_t s_info[TPB / 64][REC_CAP];
+    __shared__ uint8_t s_rc[TPB / 64][REC_CAP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= n_comp) return;
-    const uint32_t L = a.comp_len[c];
-    if (L == 0) return;
-    const uint64_t off = a.comp_off[c];
-    if (L <= PATH_BUF) {
+    const bool valid = c < n_comp;
+    const uint32_t L = valid ? a.comp_len[c] : 0u;
+    const uint64_t off = valid ? a.comp_off[c] : 0ull;
+    // lane 0 holds the wave's first position (offsets grow with the lane)
+    const uint64_t base = __shfl(off, 0, 64);
+    uint64_t last = valid ? off + L : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t y = __shfl_xor(last, o, 64);
+        last = y > last ? y : last;
+    }
+    const bool staged = last - base <= REC_CAP;  // the same for the whole wave
+    const NodeRec<K>* rec = (const NodeRec<K>*)a.node_rec;
+    auto put = [&](uint32_t i, uint32_t node, uint8_t o) {
+        if (staged) {
+            const uint32_t j = uint32_t(off - base) + i;
+            const NodeRec<K> r = rec[node];
+            s_key[wave][j] = r.key;
+            s_cid[wave][j] = r.cid;
+            s_rank[wave][j] = r.rank;
+            s_info[wave][j] = a.node_mm[node];
+            s_rc[wave][j] = o;
+        } else {
+            put_position<K>(a, off + i, node, o);
+        }
+        if (a.mark_visited) a.visited[node] = 1;
+    };
+    if (L > 0 && L <= PATH_BUF) {
         for (uint32_t i = 0; i < L; ++i) {
             const uint32_t v = a.comp_nodes[c * PATH_BUF + i];
-            put_position<K>(a, off + i, v & 0x7FFFFFFFu, uint8_t(v >> 31));
-            if (a.mark_visited) a.visited[v & 0x7FFFFFFFu] = 1;
+            put(i, v & 0x7FFFFFFFu, uint8_t(v >> 31));
         }
-        return;
+    } else if (L > 0) {
+        uint32_t cur = a.comp_x[c], ex = a.comp_ex[c];
+        uint8_t o = ex == PORT_L ? 1 : 0;
+        for (uint32_t i = 0;; ++i) {
+            put(i, cur, o);
+            if (i + 1 == L) break;
+            const uint32_t l = a.link[2 * cur + ex];
+            const uint32_t q = l & 1u;
+            cur = l >> 1;
+            ex = 1u - q;
+            o = q == PORT_R ? 1 : 0;
+        }
     }
-    uint32_t cur = a.comp_x[c], ex = a.comp_ex[c];
-    uint8_t o = ex == PORT_L ? 1 : 0;
-    for (uint32_t i = 0;; ++i) {
-        put_position<K>(a, off + i, cur, o);
-        if (a.mark_visited) a.visited[cur] = 1;
-        if (i + 1 == L) break;
-        const uint32_t l = a.link[2 * cur + ex];
-        const uint32_t q = l & 1u;
-        cur = l >> 1;
-        ex = 1u - q;
-        o = q == PORT_R ? 1 : 0;
+    if (!staged) return;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    const uint32_t total = uint32_t(last - base);
+    K* pkey = (K*)a.pkey;
+    for (uint32_t j = uint32_t(lane); j < total; j += 64) {
+        pkey[base + j] = s_key[wave][j];
+        a.pcid[base + j] = s_cid[wave][j];
+        a.prank[base + j] = s_rank[wave][j];
+        a.pinfo[base + j] = s_info[wave][j];
+        a.prc[base + j] = s_rc[wave][j];
+        a.fflag[base + j] = 0;
     }
 }
 
