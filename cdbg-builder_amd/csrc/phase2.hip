@@ -293,7 +293,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     const uint32_t start = cell_start[cell], count = cell_start[cell + 1] - start;
     if (count <= 1 || count <= lo) return;
     if (count > CAP) {
-        if (threadIdx.x == 0) next[atomicAdd(n_next, 1ull)] = cell;
+        if (next && threadIdx.x == 0) next[atomicAdd(n_next, 1ull)] = cell;
         return;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -399,12 +399,19 @@ inline void cell_sort_prepare() {
 // the two LDS classes: (capacity, threads) per key width
 template <class K>
 struct CellClasses {
-    static constexpr uint32_t SMALL = 2048, LARGE = 8192;
+    static constexpr uint32_t TINY = 1024, SMALL = 2048, LARGE = 8192;
 };
 template <>
 struct CellClasses<Key128> {
-    static constexpr uint32_t SMALL = 1024, LARGE = 4096;
+    static constexpr uint32_t TINY = 512, SMALL = 1024, LARGE = 4096;
 };
+
+// flag[c] = cell c holds more than cap keys
+__global__ void k_cell_over(uint32_t n_cells, const uint32_t* __restrict__ cell_start, uint32_t cap,
+                            uint32_t* __restrict__ flag) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c < n_cells) flag[c] = cell_start[c + 1] - cell_start[c] > cap ? 1u : 0u;
+}
 
 // cell starts: positions where a segment starts or the bits above `sh` change
 template <class K>
@@ -1789,7 +1796,7 @@ struct Phase2Bufs {
     uint64_t mt_cap = 0, mt_count = 0;
     // per chunk (reused)
     DevBuf keys, vals, keys_t, vals_t, block_super;
-    DevBuf cflag, cscan, cell_start, cell_mid, cell_over, cell_novf, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
+    DevBuf cflag, cscan, cell_start, cell_mid, cell_mid2, cell_over, cell_novf, cell_oflag, cell_oscan, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_scan, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
@@ -1907,19 +1914,32 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         const uint32_t n_cells = d2h<uint32_t>(cscan + nt, st);
         uint32_t* cell_start = P.cell_start.ensure<uint32_t>(uint64_t(n_cells) + 1);
         { KSCOPE("k_cell_list", st); k_cell_list<<<blocks_for(nt), TPB, 0, st>>>(nt, cflag, cscan, cell_start); }
-        // small class over every cell, then the large class over the cells it
-        // passed on; what is left goes to the global passes
-        constexpr uint32_t C1 = CellClasses<K>::SMALL, C2 = CellClasses<K>::LARGE;
+        // the tiny class over every cell, then each larger class over the
+        // cells the previous one passed on; what is left goes to the global
+        // passes
+        constexpr uint32_t C0 = CellClasses<K>::TINY, C1 = CellClasses<K>::SMALL, C2 = CellClasses<K>::LARGE;
         uint32_t* mid = P.cell_mid.ensure<uint32_t>(n_cells);
+        uint32_t* mid2 = P.cell_mid2.ensure<uint32_t>(n_cells);
         uint32_t* over = P.cell_over.ensure<uint32_t>(n_cells);
-        auto* novf = P.cell_novf.ensure<unsigned long long>(2);
-        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 2, 0ull); }
+        auto* novf = P.cell_novf.ensure<unsigned long long>(3);
+        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 3, 0ull); }
+        cell_sort_prepare<K, uint64_t, C0, 256>();
         cell_sort_prepare<K, uint64_t, C1, 256>();
         cell_sort_prepare<K, uint64_t, C2, 512>();
-        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C1, 256><<<n_cells, 256, cell_sort_lds<K, uint64_t, C1, 256>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, mid, novf); }
-        const uint64_t n_mid = d2h<unsigned long long>(novf, st);
-        if (n_mid) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C2, 512><<<uint32_t(n_mid), 512, cell_sort_lds<K, uint64_t, C2, 512>(), st>>>(keys, vals, cell_start, mid, uint32_t(n_mid), C1, kb - D, over, novf + 1); }
-        const uint64_t n_over = d2h<unsigned long long>(novf + 1, st);
+        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C0, 256><<<n_cells, 256, cell_sort_lds<K, uint64_t, C0, 256>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf); }
+        // cells above the tiny class, listed by a scan (many: no shared counter)
+        uint32_t* oflag = P.cell_oflag.ensure<uint32_t>(n_cells);
+        uint32_t* oscan = P.cell_oscan.ensure<uint32_t>(uint64_t(n_cells) + 1);
+        { KSCOPE("k_cell_over", st); k_cell_over<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, cell_start, C0, oflag); }
+        exclusive_scan<uint32_t, uint32_t>(oflag, oscan, n_cells, oscan + n_cells, J.rs.scan_tmp, st);
+        const uint64_t n_mid = d2h<uint32_t>(oscan + n_cells, st);
+        if (n_mid) { KSCOPE("k_compact", st); k_compact<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, oflag, oscan, mid); }
+        if (n_mid) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C1, 256><<<uint32_t(n_mid), 256, cell_sort_lds<K, uint64_t, C1, 256>(), st>>>(keys, vals, cell_start, mid, uint32_t(n_mid), C0, kb - D, mid2, novf + 1); }
+        const uint64_t n_mid2 = n_mid ? d2h<unsigned long long>(novf + 1, st) : 0;
+        if (n_mid2 > n_mid) throw Error("phase 2: cell sort listed more cells than exist");
+        if (n_mid2) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C2, 512><<<uint32_t(n_mid2), 512, cell_sort_lds<K, uint64_t, C2, 512>(), st>>>(keys, vals, cell_start, mid2, uint32_t(n_mid2), C1, kb - D, over, novf + 2); }
+        const uint64_t n_over = n_mid2 ? d2h<unsigned long long>(novf + 2, st) : 0;
+        if (n_over > n_mid2) throw Error("phase 2: cell sort listed more cells than exist");
         if (n_over) {
             // cells too large for LDS: gathered, sorted by the global passes, put back
             std::vector<uint32_t> ids(n_over), cs(n_cells + 1);
