@@ -337,19 +337,27 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
         }
         __syncthreads();
         {
-            // digit-major, wave-minor bases; thread t < 256 owns digit t
+            // digit-major, wave-minor bases; thread t owns digits DPT t .. DPT t + DPT - 1
+            constexpr uint32_t DPT = THREADS >= 256 ? 1u : 256u / THREADS;
             uint32_t tot = 0;
-            for (uint32_t d = threadIdx.x; d < 256; d += THREADS)
-                for (int w = 0; w < WAVES; ++w) tot += wcnt[w * 256 + d];
+#pragma unroll
+            for (uint32_t j = 0; j < DPT; ++j) {
+                const uint32_t d = threadIdx.x * DPT + j;
+                if (d < 256)
+                    for (int w = 0; w < WAVES; ++w) tot += wcnt[w * 256 + d];
+            }
             uint32_t all;
             uint32_t run = block_exclusive_scan<uint32_t>(tot, s_tmp, all);
-            // THREADS >= 256 here: one digit per thread
-            if (threadIdx.x < 256)
-                for (int w = 0; w < WAVES; ++w) {
-                    const uint32_t c = wcnt[w * 256 + threadIdx.x];
-                    wcnt[w * 256 + threadIdx.x] = run;
-                    run += c;
-                }
+#pragma unroll
+            for (uint32_t j = 0; j < DPT; ++j) {
+                const uint32_t d = threadIdx.x * DPT + j;
+                if (d < 256)
+                    for (int w = 0; w < WAVES; ++w) {
+                        const uint32_t c = wcnt[w * 256 + d];
+                        wcnt[w * 256 + d] = run;
+                        run += c;
+                    }
+            }
         }
         __syncthreads();
 #pragma unroll
@@ -399,7 +407,7 @@ inline void cell_sort_prepare() {
 // the two LDS classes: (capacity, threads) per key width
 template <class K>
 struct CellClasses {
-    static constexpr uint32_t TINY = 1024, SMALL = 2048, LARGE = 8192;
+    static constexpr uint32_t TINY = 512, SMALL = 2048, LARGE = 8192;
 };
 template <>
 struct CellClasses<Key128> {
@@ -1898,7 +1906,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     // then the rest of each cell in LDS
     const int kb = 2 * int(k);
     int D = 0;
-    while (D < 20 && double(nt) / nb / double(1ull << D) > 700.0) ++D;
+    while (D < 20 && double(nt) / nb / double(1ull << D) > 350.0) ++D;
     D = std::min(D, kb);
     SortPlan plan;
     plan.build(lb);
@@ -1923,10 +1931,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         uint32_t* over = P.cell_over.ensure<uint32_t>(n_cells);
         auto* novf = P.cell_novf.ensure<unsigned long long>(3);
         { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 3, 0ull); }
-        cell_sort_prepare<K, uint64_t, C0, 256>();
+        cell_sort_prepare<K, uint64_t, C0, 128>();
         cell_sort_prepare<K, uint64_t, C1, 256>();
         cell_sort_prepare<K, uint64_t, C2, 512>();
-        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C0, 256><<<n_cells, 256, cell_sort_lds<K, uint64_t, C0, 256>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf); }
+        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C0, 128><<<n_cells, 128, cell_sort_lds<K, uint64_t, C0, 128>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf); }
         // cells above the tiny class, listed by a scan (many: no shared counter)
         uint32_t* oflag = P.cell_oflag.ensure<uint32_t>(n_cells);
         uint32_t* oscan = P.cell_oscan.ensure<uint32_t>(uint64_t(n_cells) + 1);
