@@ -103,7 +103,6 @@ struct SkArgs {
 template <bool WRITE>
 __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     __shared__ uint64_t s_h[SK_TILE + 72];
-    __shared__ uint64_t s_mn[SK_TILE + 2];
     __shared__ uint32_t s_pk[SK_NPK];
     __shared__ uint32_t s_vd[SK_NVD];
     __shared__ uint64_t s_tmp[SK_THREADS / 64 + 1];
@@ -150,16 +149,19 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
         }
     }
     __syncthreads();
-    // 2. window minimum of every (k-1)-mer p0-1 .. p0+SK_TILE
-    for (int q = threadIdx.x; q < SK_TILE + 2; q += SK_THREADS) {
+    // 2. window minima of this thread's (k-1)-mers and one neighbour on each
+    //    side: mn[t] = minimizer of (k-1)-mer j0 - 1 + t, t = 0 .. 17
+    const int q0 = int(threadIdx.x) * SK_PER_THREAD;  // s_h index of (k-1)-mer j0 - 1
+    uint64_t mn[SK_PER_THREAD + 2];
+#pragma unroll
+    for (int t = 0; t < SK_PER_THREAD + 2; ++t) {
         uint64_t best = ~0ull;
         for (uint32_t i = 0; i < W; ++i) {
-            const uint64_t h = s_h[q + i];
+            const uint64_t h = s_h[q0 + t + i];
             best = h < best ? h : best;
         }
-        s_mn[q] = best;
+        mn[t] = best;
     }
-    __syncthreads();
 
     // 3. (k-1)-mer validity and super-k-mer boundaries
     const uint64_t kmask = (uint64_t(1) << (a.k - 1)) - 1;
@@ -181,9 +183,8 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
         const int64_t j = j0 + i;
         const bool vnext = valid_k1(j + 1);
         if (vcur && (vprev || vnext) && uint64_t(j) < a.n_bases) {
-            const int q = int(j - (p0 - 1));
-            if (!vprev || s_mn[q] != s_mn[q - 1]) starts |= 1u << i;
-            if (!vnext || s_mn[q] != s_mn[q + 1]) ends |= 1u << i;
+            if (!vprev || mn[i + 1] != mn[i]) starts |= 1u << i;
+            if (!vnext || mn[i + 1] != mn[i + 2]) ends |= 1u << i;
         }
         vprev = vcur;
         vcur = vnext;
@@ -202,20 +203,20 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     uint64_t ts, te;
     uint64_t os = block_exclusive_scan<uint64_t>(uint64_t(ns), s_tmp, ts) + a.off_s[blockIdx.x];
     uint64_t oe = block_exclusive_scan<uint64_t>(uint64_t(ne), s_tmp, te) + a.off_e[blockIdx.x];
+#pragma unroll
     for (int i = 0; i < SK_PER_THREAD; ++i) {
         const int64_t j = j0 + i;
-        const int q = int(j - (p0 - 1));
         if (starts & (1u << i)) {
             const bool begin = !valid_k1(j - 1);
-            const bool own_first = begin || s_mn[q] < s_mn[q - 1];
+            const bool own_first = begin || mn[i + 1] < mn[i];
             a.s_pos[os] = uint64_t(begin ? j : j - 1);
-            a.s_bucket[os] = uint32_t((s_mn[q] >> 1) % a.B);
+            a.s_bucket[os] = uint32_t((mn[i + 1] >> 1) % a.B);
             a.s_flags[os] = uint8_t((begin ? 1u : 0u) | (own_first ? 4u : 0u));
             ++os;
         }
         if (ends & (1u << i)) {
             const bool end = !valid_k1(j + 1);
-            const bool own_last = end || s_mn[q] < s_mn[q + 1];
+            const bool own_last = end || mn[i + 1] < mn[i + 2];
             a.e_end[oe] = uint64_t(end ? j - 1 : j) + a.k;
             a.e_flags[oe] = uint8_t((end ? 2u : 0u) | (own_last ? 8u : 0u));
             ++oe;
