@@ -62,10 +62,11 @@ def step_bytes(name, s, k, launches, chunks):
         # values, head flags in; node key, meta, rank, colour count, set hash, long-run flag,
         # and the owned colours out
         "k_nodes": n * (8 + 4) + nn * (4 + 4 + wk) + nn * (wk + 1 + 4 + 4 + 8 + 4) + n * 4,
-        # node key and meta in, 8 neighbour probes (key + node), mask + meta/mask + links out
-        "k_masks": nn * (wk + 1) + 8 * nn * (wk + 4) + nn * (1 + 2 + 8),
-        # ordered table: key + running max in, key + node out
-        "k_otab_place": nn * (wk + 4 + 8) + nn * (wk + 4),
+        # node key, bucket and meta in; 8 neighbour lookups (two directory words and a key
+        # each); mask, meta/mask and the two neighbour words out
+        "k_masks": nn * (wk + 4 + 1) + 8 * nn * (8 + wk) + nn * (1 + 2 + 8),
+        # node key and bucket in, one directory word out per node
+        "k_dir_build": nn * (wk + 4) + nn * 4,
         # per path position: node record + meta/mask + path entry in; key, cid, rank,
         # meta/mask, orientation, fragment flag out
         "k_walk_record": nn * ((wk + 8) + 2 + 4) + nn * (wk + 4 + 4 + 2 + 1 + 4),
@@ -143,13 +144,17 @@ def main():
     value = s["num_kmers"] * args.steps / elapsed
     print(f"bench: {args.steps} steps in {elapsed:.2f} s, phase1 {s['ms_phase1']:.1f} ms, "
           f"phase2 {s['ms_phase2']:.1f} ms (last step)", file=sys.stderr, flush=True)
+    chunks = job.num_chunks()
     if args.kernel_times:
+        print("bench: sizes " + json.dumps(s), file=sys.stderr)
         for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
-            print(f"  {name:32s} {cnt:6d} {ms:10.2f} ms", file=sys.stderr)
+            b = step_bytes(name, s, args.k, cnt / args.steps, chunks)
+            model = (f"  {b * args.steps / cnt / 1e9:9.3f} GB/launch  {b * args.steps / (ms / 1e3) / 1e9:8.1f} GB/s"
+                     if b and cnt else "")
+            print(f"  {name:32s} {cnt:6d} {ms:10.2f} ms{model}", file=sys.stderr)
 
     # roofline of the dominant kernel that has an algorithmic-byte count, and
     # the aggregate over every modelled phase-2 kernel
-    chunks = job.num_chunks()
     roof = None
     agg_b = agg_ms = 0.0
     for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):

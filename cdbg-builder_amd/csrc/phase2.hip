@@ -33,7 +33,7 @@
 //                             an exact list compare against the stored list;
 //                             new sets sorted by key, ids assigned, hybrid
 //                             encoded (hybrid_color_sets.hpp:41-96)
-//   k_table_insert<K>         per-bucket open-addressing k-mer table
+//   k_dir_build<K>            per-bucket directory over the sorted node keys
 //   k_masks<K> / k_links      8-neighbour ext masks (local_ext_mask :260-271)
 //                             and clean links (walk_step :317-336)
 //   k_walk_paths/cycles       one thread per component emulates the
@@ -66,36 +66,6 @@ __device__ __forceinline__ uint32_t v_rank(uint64_t v) { return uint32_t(v >> 32
 __device__ __forceinline__ uint32_t v_color(uint64_t v) { return uint32_t(v >> 3) & 0x1FFFFFFFu; }
 __device__ __forceinline__ bool v_owned(uint64_t v) { return (v >> 2) & 1u; }
 __device__ __forceinline__ uint32_t v_contrib(uint64_t v) { return uint32_t(v) & 3u; }
-
-// Slots after each bucket's table (ordered placement may run past its end)
-// and the largest displacement the ordered table accepts.
-constexpr uint64_t TAB_SLACK = 256;
-constexpr int64_t TAB_MAX_DISP = 192;
-
-// Per-bucket k-mer table: open addressing over groups of 64 bytes (8 u64 or
-// 4 128-bit keys), probed a group at a time; the node of slot i is node_of[i].
-// An empty slot holds all-ones (no k-mer of 2k <= 126 bits has it).
-template <class K>
-struct Tab;
-template <>
-struct Tab<uint64_t> {
-    static constexpr uint32_t G = 8;
-    __device__ static bool empty(const uint64_t& x) { return x == ~0ull; }
-    __device__ static bool claim(uint64_t* slot, const uint64_t& key) {
-        return atomicCAS((unsigned long long*)slot, ~0ull, (unsigned long long)key) == ~0ull;
-    }
-};
-template <>
-struct Tab<Key128> {
-    static constexpr uint32_t G = 4;
-    __device__ static bool empty(const Key128& x) { return x.hi == ~0ull; }
-    // hi is claimed atomically; lo is read only by later kernels
-    __device__ static bool claim(Key128* slot, const Key128& key) {
-        if (atomicCAS((unsigned long long*)&slot->hi, ~0ull, (unsigned long long)key.hi) != ~0ull) return false;
-        slot->lo = key.lo;
-        return true;
-    }
-};
 
 // last b in [0, B) with off[b] <= i (off has B+1 entries, off[B] > i)
 __device__ __forceinline__ uint32_t bucket_of(const uint64_t* off, uint32_t B, uint64_t i) {
@@ -999,61 +969,23 @@ __global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, c
 }
 
 // ---------------------------------------------------------------------------
-// per-bucket k-mer table, masks, links
+// per-bucket directory over the sorted node keys, masks, links
 // ---------------------------------------------------------------------------
-template <class K>
-__global__ void k_table_insert(uint32_t n_nodes, const K* __restrict__ node_key,
-                               const uint32_t* __restrict__ node_bucket, const uint64_t* __restrict__ tab_off,
-                               K* __restrict__ tab, uint32_t* __restrict__ node_of) {
-    constexpr uint32_t G = Tab<K>::G;
-    const uint32_t nblk = (n_nodes + TPB - 1) / TPB, blk = tile_of(blockIdx.x, nblk);
-    const uint64_t n = uint64_t(blk) * TPB + threadIdx.x;
-    if (blk >= nblk || n >= n_nodes) return;
-    const uint32_t b = node_bucket[n];
-    const uint64_t base = tab_off[b], ng = (tab_off[b + 1] - base - TAB_SLACK) / G;
-    const K key = node_key[n];
-    uint64_t g = KOps<K>::hash(key) & (ng - 1);
-    for (uint64_t probe = 0; probe < ng; ++probe) {
-        K* grp = tab + base + g * G;
-        for (uint32_t j = 0; j < G; ++j) {
-            if (Tab<K>::empty(grp[j]) && Tab<K>::claim(&grp[j], key)) {
-                node_of[base + g * G + j] = uint32_t(n);
-                return;
-            }
-        }
-        g = (g + 1) & (ng - 1);
-    }
-}
-
-// slot of key in bucket table [base, base + ng * G), or ~0 when absent
-template <class K>
-__device__ __forceinline__ uint64_t tab_find(const K* __restrict__ tab, uint64_t base, uint64_t ng, const K& key) {
-    constexpr uint32_t G = Tab<K>::G;
-    uint64_t g = KOps<K>::hash(key) & (ng - 1);
-    for (uint64_t probe = 0; probe < ng; ++probe) {
-        const K* grp = tab + base + g * G;
-        K v[G];
-#pragma unroll
-        for (uint32_t j = 0; j < G; ++j) v[j] = grp[j];
-        bool any_empty = false;
-#pragma unroll
-        for (uint32_t j = 0; j < G; ++j) {
-            if (KOps<K>::eq(v[j], key)) return base + g * G + j;
-            any_empty |= Tab<K>::empty(v[j]);
-        }
-        if (any_empty) return ~0ull;
-        g = (g + 1) & (ng - 1);
-    }
-    return ~0ull;
-}
-
-// Ordered per-bucket table.  Nodes of a bucket are sorted by k-mer, and a
-// key's home slot is a monotone function of the key, flattened by the
+// The nodes of a bucket are its distinct k-mers in increasing order.  A key's
+// cell d in [0, n_b) is a monotone function of the key, flattened by the
 // distribution of canonical k-mers (the smaller of a random word and its
-// reverse complement: CDF 1 - (1 - x)^2), computed in integers so that it is
-// exactly monotone.  Keys are placed in order at max(home, previous + 1), so
-// a lookup scans forward from home and stops at the first key >= the target.
-// The SLACK slots after each bucket's table absorb the last clusters.
+// reverse complement: CDF 1 - (1 - x)^2) and computed in integers so that it
+// is exactly monotone.  dir[d] = the first node of the bucket whose cell is
+// >= d, so the nodes of cell d are [dir[d], dir[d + 1]): a lookup reads two
+// directory words and about one key, and the node index is the position.
+// Bucket b's directory has n_b + 1 words and starts at node_off[b] + b.
+//
+// A presence filter in front of it: the bitmap of occupied fine cells, 8 per
+// node (fine cell f lies in cell f >> 3), so an absent key is rejected by one
+// bit with probability about 1 - 1/8, from a bitmap of one byte per node that
+// stays in L2 while an XCD works through a bucket.  Bucket b's bitmap has
+// ceil(n_b / 4) 32-bit words and starts at word node_off[b] / 4 + b.
+constexpr uint32_t FINE_SHIFT = 3;
 
 template <class K>
 __device__ __forceinline__ uint32_t top32(const K& key, uint32_t k);
@@ -1071,7 +1003,7 @@ __device__ __forceinline__ uint32_t top32<Key128>(const Key128& key, uint32_t k)
 }
 
 template <class K>
-__device__ __forceinline__ uint64_t home_of(const K& key, uint32_t k, uint64_t size) {
+__device__ __forceinline__ uint64_t cell_of(const K& key, uint32_t k, uint64_t size) {
     const uint64_t u = (uint64_t(1) << 32) - top32<K>(key, k);  // 1 .. 2^32
     const uint64_t f = uint64_t(0) - u * u;                    // 2^64 - u^2 (0 when u = 2^32)
     return ((f >> 32) * size) >> 32;
@@ -1082,138 +1014,164 @@ __device__ __forceinline__ bool key_lt(const K& a, const K& b) {
     return !KOps<K>::le(b, a);
 }
 
-// v[n] = bucket << 40 | (home slot - n): a running maximum over the bucket's
-// nodes then gives each node's slot.
+// Node n writes the directory words of the cells after its predecessor's
+// cell up to its own; the bucket's last node also writes the words after it.
+// Filter words: each lane's bit is ORed over the lanes of the wave that share
+// its word (words are non-decreasing along the nodes), and the first node of
+// the word stores it, after adding the nodes past the wave's end that still
+// fall in it; it also clears the empty words before it (the bucket's last
+// node: the words after it).
 template <class K>
-__global__ void k_otab_home(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
-                            const uint64_t* __restrict__ tab_off, uint32_t k, int64_t* __restrict__ v) {
+__global__ __launch_bounds__(TPB) void k_dir_build(uint32_t n_nodes, const K* __restrict__ node_key,
+                                                   const uint32_t* __restrict__ node_bucket,
+                                                   const uint64_t* __restrict__ node_off, uint32_t k,
+                                                   uint32_t* __restrict__ dir, uint32_t* __restrict__ filt) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
-    const uint32_t b = node_bucket[n];
-    const uint64_t base = tab_off[b], size = tab_off[b + 1] - base - TAB_SLACK;
-    const uint64_t h = base + home_of<K>(node_key[n], k, size);
-    v[n] = (int64_t(b) << 40) + (int64_t(h) - int64_t(n));
-}
-
-template <class K>
-__global__ void k_otab_place(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
-                             const uint64_t* __restrict__ tab_off, uint32_t k, const int64_t* __restrict__ vmax,
-                             K* __restrict__ tab, uint32_t* __restrict__ node_of, uint32_t* __restrict__ bad) {
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
-    const uint32_t b = node_bucket[n];
-    const uint64_t base = tab_off[b], end = tab_off[b + 1], size = end - base - TAB_SLACK;
-    const uint64_t slot = uint64_t(int64_t(n) + (vmax[n] - (int64_t(b) << 40)));
-    const uint64_t h = base + home_of<K>(node_key[n], k, size);
-    if (slot >= end || int64_t(slot - h) > TAB_MAX_DISP) {
-        *bad = 1u;  // clustered keys: the caller builds the hashed table instead
-        return;
+    const uint32_t lane = threadIdx.x & 63;
+    const bool live = n < n_nodes;
+    uint64_t first = 0, last = 0, size = 0, f = 0, fp = 0, gw = ~0ull;
+    uint32_t b = 0, bits = 0;
+    if (live) {
+        b = node_bucket[n];
+        first = node_off[b], last = node_off[b + 1], size = last - first;
+        const uint64_t fsize = size << FINE_SHIFT;
+        f = cell_of<K>(node_key[n], k, fsize);
+        fp = n == first ? 0 : cell_of<K>(node_key[n - 1], k, fsize);
+        uint32_t* d = dir + first + b;
+        const uint64_t c = f >> FINE_SHIFT, c0 = n == first ? 0 : (fp >> FINE_SHIFT) + 1;
+        for (uint64_t j = c0; j <= c; ++j) d[j] = uint32_t(n);
+        if (n + 1 == last)
+            for (uint64_t j = c + 1; j <= size; ++j) d[j] = uint32_t(last);
+        gw = first / 4 + b + (f >> 5);
+        bits = 1u << (f & 31);
     }
-    tab[slot] = node_key[n];
-    node_of[slot] = uint32_t(n);
-}
-
-template <class K>
-__device__ __forceinline__ uint64_t otab_find(const K* __restrict__ tab, uint64_t base, uint64_t size, uint64_t end,
-                                              const K& key, uint32_t k) {
-    constexpr uint32_t G = Tab<K>::G;
-    uint64_t s = base + home_of<K>(key, k, size);
-    while (s < end) {
-        const uint64_t g0 = s & ~uint64_t(G - 1);
-        K v[G];
 #pragma unroll
-        for (uint32_t j = 0; j < G; ++j) v[j] = tab[g0 + j];
-#pragma unroll
-        for (uint32_t j = 0; j < G; ++j) {
-            if (g0 + j < s) continue;
-            if (!key_lt<K>(v[j], key)) return KOps<K>::eq(v[j], key) ? g0 + j : ~0ull;
+    for (uint32_t sft = 1; sft < 64; sft <<= 1) {
+        const uint32_t ob = __shfl_down(bits, sft, 64);
+        const uint64_t ow = __shfl_down(gw, sft, 64);
+        if (lane + sft < 64 && ow == gw) bits |= ob;
+    }
+    const uint64_t gw_end = __shfl(gw, 63, 64);
+    if (!live) return;
+    uint32_t* fw = filt + first / 4 + b;
+    const uint64_t w = f >> 5, nw = (size + 3) / 4;
+    if (n == first || (fp >> 5) < w) {
+        const uint64_t w0 = n == first ? 0 : (fp >> 5) + 1;
+        for (uint64_t j = w0; j < w; ++j) fw[j] = 0u;
+        if (gw_end == gw) {  // the word continues past this wave
+            const uint64_t fsize = size << FINE_SHIFT;
+            for (uint64_t m = n - lane + 64; m < last; ++m) {
+                const uint64_t g = cell_of<K>(node_key[m], k, fsize);
+                if ((g >> 5) != w) break;
+                bits |= 1u << (g & 31);
+            }
         }
-        s = g0 + G;
+        fw[w] = bits;
     }
-    return ~0ull;
+    if (n + 1 == last)
+        for (uint64_t j = w + 1; j < nw; ++j) fw[j] = 0u;
 }
 
 // mask bits 0..3: successors of the canonical k-mer, 4..7: successors of its
 // reverse complement (local_ext_mask, bucket_walker.hpp:260-271).  For a side
 // with exactly one present extension, nb[2n+port] = (neighbour << 1) | entry
 // port: the neighbour is entered at its L side iff the extension is canonical.
-template <class K, bool ORDERED>
-__global__ void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
-                        const uint64_t* __restrict__ tab_off, const K* __restrict__ tab,
-                        const uint32_t* __restrict__ node_of, uint32_t k, const uint8_t* __restrict__ meta,
-                        uint8_t* __restrict__ mask_out, uint16_t* __restrict__ node_mm, uint32_t* __restrict__ nb) {
+// The four lookups of a side are independent: their filter words, then their
+// directory words, then the first key of each cell are loaded together;
+// cells holding more keys continue by binary search.  The two sides run one after
+// the other (register pressure: eight in flight halve the waves per SIMD).
+template <class K>
+__device__ __forceinline__ uint32_t side_lookup(const K& x, uint32_t k, const K* __restrict__ node_key,
+                                                const uint32_t* __restrict__ d, const uint32_t* __restrict__ fwords,
+                                                uint64_t size, uint32_t& found, uint32_t& port) {
     using O = KOps<K>;
-    constexpr uint32_t G = Tab<K>::G;
+    K q[4];
+    bool fw[4];
+#pragma unroll
+    for (uint32_t nt = 0; nt < 4; ++nt) {
+        const K e = O::shift_append(x, nt, k);
+        const K er = O::rc(e, k);
+        fw[nt] = O::le(e, er);
+        q[nt] = fw[nt] ? e : er;
+    }
+    uint64_t c[4];
+    uint32_t word[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        c[i] = cell_of<K>(q[i], k, size << FINE_SHIFT);
+        word[i] = fwords[c[i] >> 5];
+    }
+    uint32_t lo[4], hi[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        lo[i] = hi[i] = 0;
+        if ((word[i] >> (c[i] & 31)) & 1u) {  // the key's fine cell is occupied
+            lo[i] = d[c[i] >> FINE_SHIFT];
+            hi[i] = d[(c[i] >> FINE_SHIFT) + 1];
+        }
+    }
+    K v0[4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i)
+        if (hi[i] > lo[i]) v0[i] = node_key[lo[i]];
+    uint32_t mask = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+        uint32_t r = NONE32;
+        if (hi[i] > lo[i]) {
+            if (O::eq(v0[i], q[i])) {
+                r = lo[i];
+            } else if (hi[i] > lo[i] + 1 && key_lt<K>(v0[i], q[i])) {
+                uint32_t a = lo[i] + 1, z = hi[i];  // first key >= q in [a, z)
+                while (a < z) {
+                    const uint32_t mid = (a + z) >> 1;
+                    if (key_lt<K>(node_key[mid], q[i]))
+                        a = mid + 1;
+                    else
+                        z = mid;
+                }
+                if (a < hi[i] && O::eq(node_key[a], q[i])) r = a;
+            }
+        }
+        if (r != NONE32) {
+            mask |= 1u << i;
+            found = r;
+            port = fw[i] ? PORT_L : PORT_R;
+        }
+    }
+    return mask;
+}
+
+template <class K>
+__global__ __attribute__((amdgpu_waves_per_eu(sizeof(K) == 8 ? 8 : 6))) void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
+                        const uint64_t* __restrict__ node_off, const uint32_t* __restrict__ dir,
+                        const uint32_t* __restrict__ filt, uint32_t k,
+                        const uint8_t* __restrict__ meta, uint8_t* __restrict__ mask_out,
+                        uint16_t* __restrict__ node_mm, uint32_t* __restrict__ nb) {
     const uint32_t nblk = (n_nodes + TPB - 1) / TPB, blk = tile_of(blockIdx.x, nblk);
     const uint64_t n = uint64_t(blk) * TPB + threadIdx.x;
     if (blk >= nblk || n >= n_nodes) return;
     const uint32_t b = node_bucket[n];
-    const uint64_t base = tab_off[b], end = tab_off[b + 1], size = end - base - TAB_SLACK, ng = size / G;
-    auto find = [&](const K& e) -> uint64_t {
-        return ORDERED ? otab_find<K>(tab, base, size, end, e, k) : tab_find<K>(tab, base, ng, e);
-    };
+    const uint64_t first = node_off[b], size = node_off[b + 1] - first;
+    const uint32_t* d = dir + first + b;
+    const uint32_t* fwords = filt + first / 4 + b;
     const K can = node_key[n];
-    const K rv = O::rc(can, k);
-    uint32_t mask = 0, qR = 0, qL = 0;
-    uint64_t sR = ~0ull, sL = ~0ull;
-    // the extensions x.A .. x.T of one side: a forward-canonical extension is
-    // a key next to x's shifted key, so in the ordered table (k >= 17: the low
-    // base is below the home bits) one forward scan finds all of them; the
-    // others are reverse complements and are looked up one by one
-    auto side = [&](const K& x, uint32_t shift, uint64_t& sl_out, uint32_t& q_out) {
-        K e[4], er[4];
-        bool ec[4];
-#pragma unroll
-        for (uint32_t nt = 0; nt < 4; ++nt) {
-            e[nt] = O::shift_append(x, nt, k);
-            er[nt] = O::rc(e[nt], k);
-            ec[nt] = O::le(e[nt], er[nt]);
+    uint32_t sR = NONE32, pR = 0, sL = NONE32, pL = 0, mR = 0, mL = 0;
+#pragma unroll 1
+    for (uint32_t side = 0; side < 2; ++side) {
+        uint32_t s = NONE32, p = 0;
+        const uint32_t m = side_lookup<K>(side ? KOps<K>::rc(can, k) : can, k, node_key, d, fwords, size, s, p);
+        if (side) {
+            mL = m, sL = s, pL = p;
+        } else {
+            mR = m, sR = s, pR = p;
         }
-        const bool grouped = ORDERED && k >= 17 && (ec[0] || ec[1] || ec[2] || ec[3]);
-        if (grouped) {
-            constexpr uint32_t G = Tab<K>::G;
-            uint64_t sl = base + home_of<K>(e[0], k, size);
-            bool done = false;
-            while (!done && sl < end) {
-                const uint64_t g0 = sl & ~uint64_t(G - 1);
-                K v[G];
-#pragma unroll
-                for (uint32_t j = 0; j < G; ++j) v[j] = tab[g0 + j];
-#pragma unroll
-                for (uint32_t j = 0; j < G; ++j) {
-                    if (done || g0 + j < sl || key_lt<K>(v[j], e[0])) continue;
-                    if (key_lt<K>(e[3], v[j])) {
-                        done = true;
-                        continue;
-                    }
-#pragma unroll
-                    for (uint32_t nt = 0; nt < 4; ++nt)
-                        if (ec[nt] && KOps<K>::eq(v[j], e[nt])) {
-                            mask |= 1u << (shift + nt);
-                            sl_out = g0 + j;
-                            q_out = PORT_L;
-                        }
-                }
-                sl = g0 + G;
-            }
-        }
-#pragma unroll
-        for (uint32_t nt = 0; nt < 4; ++nt) {
-            if (grouped && ec[nt]) continue;
-            const uint64_t sl = find(ec[nt] ? e[nt] : er[nt]);
-            if (sl != ~0ull) {
-                mask |= 1u << (shift + nt);
-                sl_out = sl;
-                q_out = ec[nt] ? PORT_L : PORT_R;
-            }
-        }
-    };
-    side(can, 0, sR, qR);
-    side(rv, 4, sL, qL);
+    }
+    const uint32_t mask = mR | (mL << 4);
     mask_out[n] = uint8_t(mask);
     node_mm[n] = uint16_t(meta[n] | (mask << 8));
-    nb[2 * n + PORT_R] = __popc(mask & 0xfu) == 1 ? (node_of[sR] << 1) | qR : NONE32;
-    nb[2 * n + PORT_L] = __popc(mask >> 4) == 1 ? (node_of[sL] << 1) | qL : NONE32;
+    nb[2 * n + PORT_R] = __popc(mR) == 1 ? (sR << 1) | pR : NONE32;
+    nb[2 * n + PORT_L] = __popc(mL) == 1 ? (sL << 1) | pL : NONE32;
 }
 
 __device__ __forceinline__ uint32_t nib(uint32_t m, uint32_t port) { return port == PORT_R ? (m & 0xfu) : (m >> 4); }
@@ -1808,7 +1766,7 @@ struct Phase2Bufs {
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_scan, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
-    DevBuf node_bucket, tab_off, tab, node_of, tab_v, tab_max, tab_bad, tab_scan, node_mask, nb, link, visited;
+    DevBuf node_bucket, dir, filt, node_mask, nb, link, visited;
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fscan, fkey;
     DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
     DevBuf node_rec, node_mm, eflag, escan, ends;
@@ -2075,46 +2033,15 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     P.cs_bits += new_bits;
     P.mt_count += n_new;  // upper bound of occupied slots (singletons included): sizes the table
 
-    // --- per-bucket k-mer table
-    std::vector<uint64_t> h_node_off(nb + 1);
-    CDBG_HIP(hipMemcpyAsync(h_node_off.data(), node_off, (nb + 1) * 8, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipStreamSynchronize(st));
-    std::vector<uint64_t> h_tab_off(nb + 1, 0);
-    for (uint32_t b = 0; b < nb; ++b) {
-        const uint64_t n_b = h_node_off[b + 1] - h_node_off[b];
-        uint64_t c = Tab<K>::G;  // slots, a power-of-two number of groups
-        while (c * 5 < n_b * 8) c <<= 1;  // load factor <= 0.625
-        h_tab_off[b + 1] = h_tab_off[b] + c + TAB_SLACK;
-    }
-    uint64_t* tab_off = P.tab_off.ensure<uint64_t>(nb + 1);
-    CDBG_HIP(hipMemcpyAsync(tab_off, h_tab_off.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
-    CDBG_HIP(hipStreamSynchronize(st));  // h_tab_off is pageable and local
-    const uint64_t tab_n = h_tab_off[nb];
-    K* tab = P.tab.ensure<K>(tab_n);
-    uint32_t* node_of = P.node_of.ensure<uint32_t>(tab_n);
-    { KSCOPE("k_fill", st); k_fill<uint64_t><<<blocks_for(tab_n * sizeof(K) / 8), TPB, 0, st>>>((uint64_t*)tab, tab_n * sizeof(K) / 8, ~0ull); }
-    // ordered table by a running maximum; hashed table when keys cluster
-    int64_t* tv = P.tab_v.ensure<int64_t>(nn);
-    int64_t* tmax = P.tab_max.ensure<int64_t>(nn);
-    uint32_t* bad = (uint32_t*)P.tab_bad.ensure<uint32_t>(1);
-    { KSCOPE("k_fill", st); k_fill<uint32_t><<<1, 64, 0, st>>>(bad, 1, 0u); }
-    { KSCOPE("k_otab_home", st); k_otab_home<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, k, tv); }
-    inclusive_max_scan(tv, tmax, nn, P.tab_scan, st);
-    { KSCOPE("k_otab_place", st); k_otab_place<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, k, tmax, tab, node_of, bad); }
-    uint32_t h_bad = 0;
-    CDBG_HIP(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipStreamSynchronize(st));
-    const bool ordered = h_bad == 0;
-    if (!ordered) {
-        { KSCOPE("k_fill", st); k_fill<uint64_t><<<blocks_for(tab_n * sizeof(K) / 8), TPB, 0, st>>>((uint64_t*)tab, tab_n * sizeof(K) / 8, ~0ull); }
-        { KSCOPE("k_table_insert", st); k_table_insert<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of); }
-    }
+    // --- per-bucket directory over the sorted keys, masks, links
+    uint32_t* dir = P.dir.ensure<uint32_t>(uint64_t(nn) + nb);
+    uint32_t* filt = P.filt.ensure<uint32_t>(uint64_t(nn) / 4 + nb + 1);
+    { KSCOPE("k_dir_build", st); k_dir_build<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, node_off, k, dir, filt); }
     uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
     uint32_t* nbr = P.nb.ensure<uint32_t>(2ull * nn);
     uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
     uint16_t* node_mm = P.node_mm.ensure<uint16_t>(nn);
-    if (ordered) { KSCOPE("k_masks", st); k_masks<K, true><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
-    else { KSCOPE("k_masks", st); k_masks<K, false><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, tab_off, tab, node_of, k, meta, node_mask, node_mm, nbr); }
+    { KSCOPE("k_masks", st); k_masks<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, node_off, dir, filt, k, meta, node_mask, node_mm, nbr); }
     { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link); }
 
     // --- walk

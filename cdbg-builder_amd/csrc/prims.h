@@ -222,124 +222,6 @@ inline void exclusive_scan(const Tin* in, Tout* out, uint64_t n, Tout* d_total, 
     { KSCOPE("k_scan_down", st); k_scan_down<Tin, Tout><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, sums, out); }
     CDBG_HIP(hipGetLastError());
 }
-
-// ---------------------------------------------------------------------------
-// Inclusive max-scan of int64 (same three-kernel shape as the sum scan).
-// ---------------------------------------------------------------------------
-__device__ inline int64_t block_inclusive_max(int64_t v, int64_t* s_tmp, int64_t& total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    int64_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int64_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x = y > x ? y : x;
-    }
-    if (lane == 63) s_tmp[wave] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int64_t run = INT64_MIN;
-        for (int w = 0; w < nw; ++w) {
-            const int64_t t = s_tmp[w];
-            s_tmp[w] = run;  // max of the waves before w
-            run = t > run ? t : run;
-        }
-        s_tmp[nw] = run;
-    }
-    __syncthreads();
-    const int64_t before = s_tmp[wave];
-    total = s_tmp[nw];
-    __syncthreads();
-    return before > x ? before : x;
-}
-
-static __global__ __launch_bounds__(SCAN_THREADS) void k_maxscan_reduce(const int64_t* __restrict__ in, uint64_t n,
-                                                                 int64_t* __restrict__ part) {
-    __shared__ int64_t s_tmp[SCAN_THREADS / 64 + 1];
-    const uint64_t base = uint64_t(blockIdx.x) * SCAN_CHUNK;
-    int64_t acc = INT64_MIN;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        const uint64_t idx = base + uint64_t(i) * SCAN_THREADS + threadIdx.x;
-        if (idx < n) acc = in[idx] > acc ? in[idx] : acc;
-    }
-    int64_t total;
-    (void)block_inclusive_max(acc, s_tmp, total);
-    if (threadIdx.x == 0) part[blockIdx.x] = total;
-}
-
-// part[i] = max of chunks 0..i
-static __global__ __launch_bounds__(1024) void k_maxscan_single(int64_t* __restrict__ part, uint64_t n) {
-    __shared__ int64_t s_tmp[1024 / 64 + 1];
-    int64_t carry = INT64_MIN;
-    for (uint64_t base = 0; base < n; base += 1024) {
-        const uint64_t idx = base + threadIdx.x;
-        const int64_t v = idx < n ? part[idx] : INT64_MIN;
-        int64_t tot;
-        const int64_t inc = block_inclusive_max(v, s_tmp, tot);
-        if (idx < n) part[idx] = inc > carry ? inc : carry;
-        carry = tot > carry ? tot : carry;
-    }
-}
-
-static __global__ __launch_bounds__(SCAN_THREADS) void k_maxscan_down(const int64_t* __restrict__ in, uint64_t n,
-                                                               const int64_t* __restrict__ part_incl,
-                                                               int64_t* __restrict__ out) {
-    __shared__ int64_t s_tmp[SCAN_THREADS / 64 + 1];
-    __shared__ int64_t s_v[SCAN_CHUNK + SCAN_CHUNK / 32];
-    const uint64_t base = uint64_t(blockIdx.x) * SCAN_CHUNK;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        const uint32_t j = uint32_t(i) * SCAN_THREADS + threadIdx.x;
-        const uint64_t idx = base + j;
-        s_v[scan_pad(j)] = idx < n ? in[idx] : INT64_MIN;
-    }
-    __syncthreads();
-    const uint32_t first = threadIdx.x * SCAN_ITEMS;
-    int64_t acc = INT64_MIN;
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        const int64_t v = s_v[scan_pad(first + i)];
-        acc = v > acc ? v : acc;
-    }
-    int64_t tot;
-    const int64_t inc = block_inclusive_max(acc, s_tmp, tot);
-    // carry into this thread: max of the earlier chunks and the earlier threads
-    int64_t carry = blockIdx.x ? part_incl[blockIdx.x - 1] : INT64_MIN;
-    const int64_t prev_thr = __shfl_up(inc, 1, 64);
-    int64_t before_thr;
-    if ((threadIdx.x & 63) == 0)
-        before_thr = s_tmp[threadIdx.x >> 6];  // max of the waves before (s_tmp holds exclusive wave maxima)
-    else
-        before_thr = prev_thr;
-    carry = before_thr > carry ? before_thr : carry;
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        const uint32_t j = scan_pad(first + i);
-        const int64_t v = s_v[j];
-        carry = v > carry ? v : carry;
-        s_v[j] = carry;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < SCAN_ITEMS; ++i) {
-        const uint32_t j = uint32_t(i) * SCAN_THREADS + threadIdx.x;
-        const uint64_t idx = base + j;
-        if (idx < n) out[idx] = s_v[scan_pad(j)];
-    }
-}
-
-// out[i] = max(in[0..i]).
-inline void inclusive_max_scan(const int64_t* in, int64_t* out, uint64_t n, DevBuf& tmp, hipStream_t st) {
-    uint64_t nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
-    if (nb == 0) return;
-    int64_t* part = tmp.ensure<int64_t>(nb);
-    { KSCOPE("k_maxscan_reduce", st); k_maxscan_reduce<<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, part); }
-    { KSCOPE("k_maxscan_single", st); k_maxscan_single<<<1, 1024, 0, st>>>(part, nb); }
-    { KSCOPE("k_maxscan_down", st); k_maxscan_down<<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(in, n, part, out); }
-    CDBG_HIP(hipGetLastError());
-}
-
 // ---------------------------------------------------------------------------
 // Segmented stable LSD radix sort.
 // ---------------------------------------------------------------------------
