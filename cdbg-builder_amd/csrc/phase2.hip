@@ -391,25 +391,30 @@ __global__ void k_cell_over(uint32_t n_cells, const uint32_t* __restrict__ cell_
     if (c < n_cells) flag[c] = cell_start[c + 1] - cell_start[c] > cap ? 1u : 0u;
 }
 
-// cell starts: positions where a segment starts or the bits above `sh` change
+// Cells are dense: cell c = (local bucket c >> D, top-D-bit value c & (2^D - 1))
+// and cell_start[c] is the first tuple of the bucket whose top D bits are >= the
+// value (binary search over the bucket's sorted keys); empty cells have equal
+// starts.  cell_start[n_cells] = n_tuples.
 template <class K>
-__global__ void k_cell_flags(uint64_t n, const K* __restrict__ keys, int sh, const uint32_t* __restrict__ seg_start_flag,
-                             uint32_t* __restrict__ flag) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    bool f = i == 0 || seg_start_flag[i];
-    if (!f) {
-        const K a = keys[i - 1], b = keys[i];
-        f = digit_of(a, sh, 0xFFFFFFFFu) != digit_of(b, sh, 0xFFFFFFFFu);
+__global__ void k_cell_starts(uint32_t n_cells, uint32_t D, const uint64_t* __restrict__ lbtup, uint64_t n_tuples,
+                              const K* __restrict__ keys, int sh, uint32_t* __restrict__ cell_start) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c > n_cells) return;
+    if (c == n_cells) {
+        cell_start[c] = uint32_t(n_tuples);
+        return;
     }
-    flag[i] = f ? 1u : 0u;
-}
-
-__global__ void k_cell_list(uint64_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
-                            uint32_t* __restrict__ cell_start) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n && flag[i]) cell_start[scan[i]] = uint32_t(i);
-    if (i == 0) cell_start[scan[n]] = uint32_t(n);  // scan has n + 1 entries
+    const uint32_t lb = uint32_t(c >> D), v = uint32_t(c) & ((1u << D) - 1u);
+    uint64_t lo = lbtup[lb], hi = lbtup[lb + 1];
+    if (v == 0) hi = lo;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (digit_of(keys[mid], sh, 0xFFFFFFFFu) < v)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    cell_start[c] = uint32_t(lo);
 }
 
 // Moves the elements of listed cells between the tuple arrays and a compact
@@ -1762,7 +1767,7 @@ struct Phase2Bufs {
     uint64_t mt_cap = 0, mt_count = 0;
     // per chunk (reused)
     DevBuf keys, vals, keys_t, vals_t, block_super;
-    DevBuf cflag, cscan, cell_start, cell_mid, cell_mid2, cell_over, cell_novf, cell_oflag, cell_oscan, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
+    DevBuf cell_start, cell_mid, cell_mid2, cell_over, cell_novf, cell_oflag, cell_oscan, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_scan, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
@@ -1871,15 +1876,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (D > 0) segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D, kb, (D + 1) / 2, P.rs2, st, "tuples");
     uint32_t* heads = P.heads.ensure<uint32_t>(nt);
     {
-        uint32_t* cflag = P.cflag.ensure<uint32_t>(nt);
-        uint32_t* cscan = P.cscan.ensure<uint32_t>(nt + 1);
-        { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(nt), TPB, 0, st>>>(heads, nt, 0u); }
-        { KSCOPE("k_mark_bucket_starts", st); k_mark_bucket_starts<<<blocks_for(nb), TPB, 0, st>>>(nb, lbtup, heads); }
-        { KSCOPE("k_cell_flags", st); k_cell_flags<K><<<blocks_for(nt), TPB, 0, st>>>(nt, keys, kb - D, heads, cflag); }
-        exclusive_scan<uint32_t, uint32_t>(cflag, cscan, nt, cscan + nt, J.rs.scan_tmp, st);
-        const uint32_t n_cells = d2h<uint32_t>(cscan + nt, st);
+        if (uint64_t(nb) << D >= (1ull << 32)) throw Error("phase 2: too many sort cells in one chunk");
+        const uint32_t n_cells = nb << D;
         uint32_t* cell_start = P.cell_start.ensure<uint32_t>(uint64_t(n_cells) + 1);
-        { KSCOPE("k_cell_list", st); k_cell_list<<<blocks_for(nt), TPB, 0, st>>>(nt, cflag, cscan, cell_start); }
+        { KSCOPE("k_cell_starts", st); k_cell_starts<K><<<blocks_for(uint64_t(n_cells) + 1), TPB, 0, st>>>(n_cells, uint32_t(D), lbtup, nt, keys, kb - D, cell_start); }
         // the tiny class over every cell, then each larger class over the
         // cells the previous one passed on; what is left goes to the global
         // passes
