@@ -57,8 +57,8 @@ def step_bytes(name, s, k, launches, chunks):
         "k_cell_sort": n * (wk + 8) * 2,
         # super records + 2-bit bases in, one (k-mer, value) tuple out per k-mer
         "k_expand": ns * 29 + nb // 4 + n * (wk + 8),
-        # count pass and write pass over the 2-bit bases and the ACGT bitmap; starts/ends out
-        "k_superkmers": 2 * (nb // 4 + nb // 8) + ns * 22,
+        # the 2-bit bases and the ACGT bitmap in; 4-byte start and 2-byte end entries out
+        "k_superkmers": nb // 4 + nb // 8 + ns * 6,
         # values, head flags in; node key, meta, rank, colour count, set hash, long-run flag,
         # and the owned colours out
         "k_nodes": n * (8 + 4) + nn * (4 + 4 + wk) + nn * (wk + 1 + 4 + 4 + 8 + 4) + n * 4,

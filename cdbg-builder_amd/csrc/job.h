@@ -16,8 +16,7 @@ namespace cdbg_amd {
 struct Phase1Bufs {
     DevBuf packed, valid;              // 2-bit bases (16 per u32), ACGT bitmap (32 per u32)
     DevBuf tile_cnt, tile_off, totals; // per-tile start/end counts and their scans
-    DevBuf s_pos, s_bucket, s_flags;   // super starts (position order)
-    DevBuf e_end, e_flags;             // super ends (position order)
+    DevBuf st_s, st_e;                 // per-tile super starts (4 B) and ends (2 B)
     DevBuf sk_rec, sk_rec_t;           // super records (16 B), bucket sort payload
     DevBuf key, key_t;                 // bucket sort keys (bucket of each super)
     DevBuf b_start, b_len, b_flags, b_color;                // supers, bucket-major

@@ -9,9 +9,12 @@
 //                   W = k-m m-mers of every (k-1)-mer (minimizer.hpp:94-157,
 //                   bucket_ingester.hpp:52), and the super-k-mer starts/ends
 //                   with their BEGIN/END/OWNS flags (bucket_ingester.hpp:80-126).
-//                   Run twice: count, then write at scanned offsets.
-//   k_zip_supers    pairs the i-th start with the i-th end (they alternate
-//                   along every ACGT run), assigns the colour.
+//                   One pass: each tile stores its starts (4 bytes: offset in
+//                   the tile, flags, bucket) and ends (2 bytes) in its own
+//                   slot range and its two counts.
+//   k_zip_supers    after a scan of the tile counts: pairs the i-th start
+//                   with the i-th end (they alternate along every ACGT run),
+//                   assigns the colour.
 //   bucket scatter  stable radix sort of super indices by bucket
 //                   (h >> 1) % B (bucket_ingester.hpp:67) + gather.
 //
@@ -89,18 +92,12 @@ struct SkArgs {
     uint64_t n_pk_words, n_vd_words;
     uint32_t k, m, W, B;
     HashTabs T;
-    uint32_t* cnt_s;     // per-tile start counts (count pass)
-    uint32_t* cnt_e;     // per-tile end counts
-    const uint64_t* off_s;  // scanned (write pass)
-    const uint64_t* off_e;
-    uint64_t* s_pos;
-    uint32_t* s_bucket;
-    uint8_t* s_flags;
-    uint64_t* e_end;
-    uint8_t* e_flags;
+    uint32_t* cnt_s;  // per-tile start counts
+    uint32_t* cnt_e;  // per-tile end counts
+    uint32_t* st_s;   // per tile SK_TILE slots: bucket << 16 | flags << 12 | offset in tile
+    uint16_t* st_e;   // per tile SK_TILE slots: flags << 12 | offset in tile
 };
 
-template <bool WRITE>
 __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     __shared__ uint64_t s_h[SK_TILE + 72];
     __shared__ uint32_t s_pk[SK_NPK];
@@ -190,36 +187,30 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
         vcur = vnext;
     }
     const uint32_t ns = __popc(starts), ne = __popc(ends);
-    if (!WRITE) {
-        uint64_t ts, te;
-        (void)block_exclusive_scan<uint64_t>(uint64_t(ns), s_tmp, ts);
-        (void)block_exclusive_scan<uint64_t>(uint64_t(ne), s_tmp, te);
-        if (threadIdx.x == 0) {
-            a.cnt_s[blockIdx.x] = uint32_t(ts);
-            a.cnt_e[blockIdx.x] = uint32_t(te);
-        }
-        return;
-    }
     uint64_t ts, te;
-    uint64_t os = block_exclusive_scan<uint64_t>(uint64_t(ns), s_tmp, ts) + a.off_s[blockIdx.x];
-    uint64_t oe = block_exclusive_scan<uint64_t>(uint64_t(ne), s_tmp, te) + a.off_e[blockIdx.x];
+    uint32_t os = uint32_t(block_exclusive_scan<uint64_t>(uint64_t(ns), s_tmp, ts));
+    uint32_t oe = uint32_t(block_exclusive_scan<uint64_t>(uint64_t(ne), s_tmp, te));
+    if (threadIdx.x == 0) {
+        a.cnt_s[blockIdx.x] = uint32_t(ts);
+        a.cnt_e[blockIdx.x] = uint32_t(te);
+    }
+    uint32_t* st_s = a.st_s + uint64_t(blockIdx.x) * SK_TILE;
+    uint16_t* st_e = a.st_e + uint64_t(blockIdx.x) * SK_TILE;
 #pragma unroll
     for (int i = 0; i < SK_PER_THREAD; ++i) {
         const int64_t j = j0 + i;
+        const uint32_t lo = uint32_t(threadIdx.x) * SK_PER_THREAD + uint32_t(i);  // j - p0
         if (starts & (1u << i)) {
             const bool begin = !valid_k1(j - 1);
             const bool own_first = begin || mn[i + 1] < mn[i];
-            a.s_pos[os] = uint64_t(begin ? j : j - 1);
-            a.s_bucket[os] = uint32_t((mn[i + 1] >> 1) % a.B);
-            a.s_flags[os] = uint8_t((begin ? 1u : 0u) | (own_first ? 4u : 0u));
-            ++os;
+            const uint32_t fl = (begin ? 1u : 0u) | (own_first ? 4u : 0u);
+            st_s[os++] = (uint32_t((mn[i + 1] >> 1) % a.B) << 16) | (fl << 12) | lo;
         }
         if (ends & (1u << i)) {
             const bool end = !valid_k1(j + 1);
             const bool own_last = end || mn[i + 1] < mn[i + 2];
-            a.e_end[oe] = uint64_t(end ? j - 1 : j) + a.k;
-            a.e_flags[oe] = uint8_t((end ? 2u : 0u) | (own_last ? 8u : 0u));
-            ++oe;
+            const uint32_t fl = (end ? 2u : 0u) | (own_last ? 8u : 0u);
+            st_e[oe++] = uint16_t((fl << 12) | lo);
         }
     }
 }
@@ -232,27 +223,46 @@ struct SuperRec {
 };
 
 // Pairs the i-th start with the i-th end: record, and its bucket as sort key.
-__global__ void k_zip_supers(uint64_t n, const uint64_t* __restrict__ s_pos, const uint32_t* __restrict__ s_bucket,
-                             const uint8_t* __restrict__ s_flags, const uint64_t* __restrict__ e_end,
-                             const uint8_t* __restrict__ e_flags, const uint64_t* __restrict__ color_off,
-                             uint32_t num_colors, SuperRec* __restrict__ rec, uint32_t* __restrict__ key,
-                             unsigned long long* __restrict__ too_long) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t s = s_pos[i];
-    // colour = last c with color_off[c] <= s
-    uint32_t lo = 0, hi = num_colors;  // invariant: color_off[lo] <= s < color_off[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (color_off[mid] <= s)
-            lo = mid;
-        else
-            hi = mid;
+// One block per tile; its starts are global supers off_s[t] .. off_s[t] +
+// cnt_s[t] - 1, and the i-th end lies in the first tile whose ends reach i.
+__global__ __launch_bounds__(256) void k_zip_supers(uint64_t ntiles, const uint32_t* __restrict__ cnt_s,
+                                                    const uint32_t* __restrict__ cnt_e,
+                                                    const uint64_t* __restrict__ off_s,
+                                                    const uint64_t* __restrict__ off_e,
+                                                    const uint32_t* __restrict__ st_s,
+                                                    const uint16_t* __restrict__ st_e, uint32_t k,
+                                                    const uint64_t* __restrict__ color_off, uint32_t num_colors,
+                                                    SuperRec* __restrict__ rec, uint32_t* __restrict__ key,
+                                                    unsigned long long* __restrict__ too_long) {
+    const uint64_t t = blockIdx.x;
+    const uint32_t ns = cnt_s[t];
+    for (uint32_t j = threadIdx.x; j < ns; j += blockDim.x) {
+        const uint64_t i = off_s[t] + j;
+        const uint32_t se = st_s[t * SK_TILE + j];
+        const uint64_t sj = t * SK_TILE + (se & 0xFFFu);
+        const uint32_t sfl = (se >> 12) & 15u;
+        const uint64_t s = (sfl & 1u) ? sj : sj - 1;
+        uint64_t te = t;
+        while (te > 0 && off_e[te] > i) --te;
+        while (te + 1 < ntiles && off_e[te] + cnt_e[te] <= i) ++te;
+        const uint32_t ee = st_e[te * SK_TILE + (i - off_e[te])];
+        const uint64_t ej = te * SK_TILE + (ee & 0xFFFu);
+        const uint32_t efl = (ee >> 12) & 15u;
+        const uint64_t e = ((efl & 2u) ? ej - 1 : ej) + k;
+        // colour = last c with color_off[c] <= s
+        uint32_t lo = 0, hi = num_colors;  // invariant: color_off[lo] <= s < color_off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (color_off[mid] <= s)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const uint64_t len = e - s;
+        if (len >= (uint64_t(1) << 28)) atomicAdd(too_long, 1ull);
+        rec[i] = SuperRec{s, uint32_t(len << 4) | sfl | efl, lo};
+        key[i] = se >> 16;
     }
-    const uint64_t len = e_end[i] - s;
-    if (len >= (uint64_t(1) << 28)) atomicAdd(too_long, 1ull);
-    rec[i] = SuperRec{s, uint32_t(len << 4) | uint32_t(s_flags[i] | e_flags[i]), lo};
-    key[i] = s_bucket[i];
 }
 
 // The bucket-ordered records as separate arrays.
@@ -338,7 +348,10 @@ void run_phase1(Job& J) {
     uint64_t* tot = P.totals.ensure<uint64_t>(2);
     a.cnt_s = cnt;
     a.cnt_e = cnt + ntiles;
-    { KSCOPE("k_superkmers", st); k_superkmers<false><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
+    a.st_s = P.st_s.ensure<uint32_t>(ntiles * SK_TILE);
+    a.st_e = P.st_e.ensure<uint16_t>(ntiles * SK_TILE);
+    if (J.B > 65536) throw Error("phase 1: more than 2^16 buckets");
+    { KSCOPE("k_superkmers", st); k_superkmers<<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
     exclusive_scan<uint32_t, uint64_t>(cnt, off, ntiles, tot, J.rs.scan_tmp, st);
     exclusive_scan<uint32_t, uint64_t>(cnt + ntiles, off + ntiles, ntiles, tot + 1, J.rs.scan_tmp, st);
     uint64_t h_tot[2];
@@ -348,14 +361,6 @@ void run_phase1(Job& J) {
     const uint64_t n = h_tot[0];
     J.n_supers = n;
     if (n >= (1ull << 32)) throw Error("phase 1: more than 2^32 super-k-mers in one job");
-    a.off_s = off;
-    a.off_e = off + ntiles;
-    a.s_pos = P.s_pos.ensure<uint64_t>(n);
-    a.s_bucket = P.s_bucket.ensure<uint32_t>(n);
-    a.s_flags = P.s_flags.ensure<uint8_t>(n);
-    a.e_end = P.e_end.ensure<uint64_t>(n);
-    a.e_flags = P.e_flags.ensure<uint8_t>(n);
-    { KSCOPE("k_superkmers", st); k_superkmers<true><<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
 
     // Bucket scatter: stable sort of the records by bucket.
     SuperRec* rec = (SuperRec*)P.sk_rec.ensure<SuperRec>(n);
@@ -365,9 +370,10 @@ void run_phase1(Job& J) {
     auto* too_long = P.totals.ensure<uint64_t>(3) + 2;
     { KSCOPE("k_fill", st); k_fill_u64<<<1, 64, 0, st>>>(too_long, 1, 0ull); }
     if (n) {
-        { KSCOPE("k_zip_supers", st); k_zip_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, a.s_pos, a.s_bucket, a.s_flags, a.e_end, a.e_flags,
-                                                         J.color_off.as<uint64_t>(), J.num_colors, rec, key,
-                                                         (unsigned long long*)too_long); }
+        KSCOPE("k_zip_supers", st);
+        k_zip_supers<<<uint32_t(ntiles), 256, 0, st>>>(ntiles, cnt, cnt + ntiles, off, off + ntiles, a.st_s, a.st_e,
+                                                      J.k, J.color_off.as<uint64_t>(), J.num_colors, rec, key,
+                                                      (unsigned long long*)too_long);
     }
     int bbits = 0;
     while ((uint64_t(1) << bbits) < J.B) ++bbits;
