@@ -1865,15 +1865,17 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     { KSCOPE("k_expand", st); k_expand<K><<<uint32_t(n_eblocks), TPB, 0, st>>>(ea, block_super, keys, vals); }
 
     // --- sort tuples by k-mer inside every bucket (stable): the top D bits by
-    // two global passes, which cuts every bucket into cells of ~700 keys,
-    // then the rest of each cell in LDS
+    // two global passes, which cuts every bucket into cells of 700 - 1400
+    // keys (measured best on MI355X: larger cells amortise the per-cell
+    // passes, an 11-bit single global pass scatters too slowly), then the
+    // rest of each cell in LDS
     const int kb = 2 * int(k);
     int D = 0;
-    while (D < 20 && double(nt) / nb / double(1ull << D) > 350.0) ++D;
+    while (D < 20 && double(nt) / nb / double(1ull << D) > 1400.0) ++D;
     D = std::min(D, kb);
     SortPlan plan;
     plan.build(lb);
-    if (D > 0) segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D, kb, (D + 1) / 2, P.rs2, st, "tuples");
+    if (D > 0) segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D, kb, D <= 11 ? D : (D + 1) / 2, P.rs2, st, "tuples");
     uint32_t* heads = P.heads.ensure<uint32_t>(nt);
     {
         if (uint64_t(nb) << D >= (1ull << 32)) throw Error("phase 2: too many sort cells in one chunk");
