@@ -1875,7 +1875,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     D = std::min(D, kb);
     SortPlan plan;
     plan.build(lb);
-    if (D > 0) segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D, kb, D <= 11 ? D : (D + 1) / 2, P.rs2, st, "tuples");
+    if (D > 0) segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D, kb, D <= 8 ? D : (D + 1) / 2, P.rs2, st, "tuples");
     uint32_t* heads = P.heads.ensure<uint32_t>(nt);
     {
         if (uint64_t(nb) << D >= (1ull << 32)) throw Error("phase 2: too many sort cells in one chunk");
