@@ -36,6 +36,8 @@ constexpr int SK_TILE = SK_THREADS * SK_PER_THREAD;  // (k-1)-mer positions per 
 constexpr int SK_NPK = SK_TILE / 16 + 16;            // packed words staged per tile
 constexpr int SK_NVD = SK_TILE / 32 + 16;            // validity words staged per tile
 
+__host__ __device__ constexpr int sk_hidx(int p) { return p + (p >> 4); }
+
 // ntHash per-base seeds: the published ntHash table (minimizer.hpp:21-26).
 constexpr uint64_t NT_SEED0 = 0x3c8bfbb395c60474ULL;
 constexpr uint64_t NT_SEED1 = 0x3193c18562a02b4cULL;
@@ -99,7 +101,9 @@ struct SkArgs {
 };
 
 __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
-    __shared__ uint64_t s_h[SK_TILE + 72];
+    // one pad slot per 16 hashes: threads own 16 consecutive hashes, so an
+    // unpadded stride of 128 bytes would put a wave's accesses in 2 banks
+    __shared__ uint64_t s_h[sk_hidx(SK_TILE + 72)];
     __shared__ uint32_t s_pk[SK_NPK];
     __shared__ uint32_t s_vd[SK_NVD];
     __shared__ uint64_t s_tmp[SK_THREADS / 64 + 1];
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
             f = rotl(f, 1) ^ sel4(b, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]);
             r ^= rotl(sel4(b ^ 3u, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]), i);
         }
-        s_h[q0] = f <= r ? f : r;
+        s_h[sk_hidx(q0)] = f <= r ? f : r;
         for (int i = 1; i < cnt; ++i) {
             const uint32_t bo = base_at(pos + i - 1);
             const uint32_t bi = base_at(pos + i + m - 1);
@@ -142,7 +146,7 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
                 sel4(bi, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]);
             r = rotl(r, 63) ^ sel4(bo, a.T.rc1[0], a.T.rc1[1], a.T.rc1[2], a.T.rc1[3]) ^
                 sel4(bi, a.T.rcm[0], a.T.rcm[1], a.T.rcm[2], a.T.rcm[3]);
-            s_h[q0 + i] = f <= r ? f : r;
+            s_h[sk_hidx(q0 + i)] = f <= r ? f : r;
         }
     }
     __syncthreads();
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     for (int t = 0; t < SK_PER_THREAD + 2; ++t) {
         uint64_t best = ~0ull;
         for (uint32_t i = 0; i < W; ++i) {
-            const uint64_t h = s_h[q0 + t + i];
+            const uint64_t h = s_h[sk_hidx(q0 + t + i)];
             best = h < best ? h : best;
         }
         mn[t] = best;
