@@ -183,14 +183,20 @@ struct ExpArgs {
     const uint32_t* rank0;  // per record: stream rank of its first k-mer (null: records are the stream)
 };
 
-// First super-k-mer of every expand block: block b starts at chunk tuple
-// b * TPB; block_super[n_blocks] = s1 - 1 closes the table.
+// Expand tiles: EXP_PER tuples per thread, strided by the block size so that
+// the stores stay coalesced and each thread has EXP_PER independent lookups
+// in flight.
+constexpr uint32_t EXP_PER = 4, EXP_TILE = TPB * EXP_PER;
+
+// First super-k-mer of every expand tile: tile b starts at chunk tuple
+// b * EXP_TILE; block_super[n_blocks] = s1 - 1 closes the table.
 __global__ void k_block_super(uint64_t s0, uint64_t s1, uint64_t t0, uint64_t n_blocks,
                               const uint64_t* __restrict__ tup_off, uint32_t* __restrict__ block_super) {
     const uint64_t s = s0 + uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (s >= s1) return;
     const uint64_t lo = tup_off[s] - t0, hi = tup_off[s + 1] - t0;  // local tuples [lo, hi)
-    for (uint64_t b = (lo + TPB - 1) / TPB; b * TPB < hi && b < n_blocks; ++b) block_super[b] = uint32_t(s - s0);
+    for (uint64_t b = (lo + EXP_TILE - 1) / EXP_TILE; b * EXP_TILE < hi && b < n_blocks; ++b)
+        block_super[b] = uint32_t(s - s0);
     if (s == s1 - 1) block_super[n_blocks] = uint32_t(s - s0);
 }
 
@@ -198,37 +204,57 @@ template <class K>
 __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __restrict__ block_super,
                                                 K* __restrict__ keys, uint64_t* __restrict__ vals) {
     using O = KOps<K>;
-    __shared__ uint64_t s_off[TPB + 2];
-    const uint64_t T0 = a.t0 + uint64_t(blockIdx.x) * TPB;
-    const uint64_t TL = min(T0 + TPB, a.t0 + a.n) - 1;  // last tuple of the block
+    __shared__ uint64_t s_off[EXP_TILE + 2];
+    const uint64_t T0 = a.t0 + uint64_t(blockIdx.x) * EXP_TILE;
+    const uint64_t TE = min(T0 + EXP_TILE, a.t0 + a.n);  // tiles end
     const uint64_t sa = a.s0 + block_super[blockIdx.x];
-    const uint32_t ns = block_super[blockIdx.x + 1] - block_super[blockIdx.x] + 1;  // <= TPB + 1
+    const uint32_t ns = block_super[blockIdx.x + 1] - block_super[blockIdx.x] + 1;  // <= EXP_TILE + 1
     for (uint32_t i = threadIdx.x; i <= ns; i += TPB) s_off[i] = a.tup_off[sa + i];
     __syncthreads();
-    const uint64_t t = T0 + threadIdx.x;
-    if (t > TL) return;
-    uint32_t lo = 0, hi = ns;  // s_off[lo] <= t < s_off[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_off[mid] <= t)
-            lo = mid;
-        else
-            hi = mid;
+    uint64_t sup[EXP_PER];
+    uint32_t ix[EXP_PER];
+#pragma unroll
+    for (uint32_t j = 0; j < EXP_PER; ++j) {
+        const uint64_t t = T0 + threadIdx.x + j * TPB;
+        uint32_t lo = 0, hi = ns;  // s_off[lo] <= t < s_off[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_off[mid] <= t)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        sup[j] = sa + lo;
+        ix[j] = uint32_t(t - s_off[lo]);
     }
-    const uint64_t s = sa + lo;
-    const uint32_t ix = uint32_t(t - s_off[lo]);
-    const uint32_t nk = a.len[s] - a.k + 1;
-    const uint32_t fl = a.flags[s];
-    K f, r;
-    kmer_at(a.packed, a.start[s] + ix, a.k, f, r);
-    const bool fwd = O::le(f, r);
-    uint32_t contrib = 0;
-    if (!(fl & 1u) && ix == 0) contrib |= fwd ? BOUND_LEFT : BOUND_RIGHT;
-    if (!(fl & 2u) && ix == nk - 1) contrib |= fwd ? BOUND_RIGHT : BOUND_LEFT;
-    const bool owned = !((ix == 0 && !(fl & 4u)) || (ix == nk - 1 && !(fl & 8u)));
-    const uint32_t rank = a.rank0 ? a.rank0[s] + ix : uint32_t(t - a.btup[a.bucket[s]]);
-    keys[t - a.t0] = fwd ? f : r;
-    vals[t - a.t0] = (uint64_t(rank) << 32) | (uint64_t(a.color[s]) << 3) | (uint64_t(owned) << 2) | contrib;
+    uint32_t len[EXP_PER], fl[EXP_PER], col[EXP_PER];
+    uint64_t st[EXP_PER], rk[EXP_PER];
+#pragma unroll
+    for (uint32_t j = 0; j < EXP_PER; ++j) {
+        if (T0 + threadIdx.x + j * TPB >= TE) continue;
+        const uint64_t s = sup[j];
+        len[j] = a.len[s];
+        fl[j] = a.flags[s];
+        col[j] = a.color[s];
+        st[j] = a.start[s];
+        rk[j] = a.rank0 ? uint64_t(a.rank0[s]) : a.btup[a.bucket[s]];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < EXP_PER; ++j) {
+        const uint64_t t = T0 + threadIdx.x + j * TPB;
+        if (t >= TE) continue;
+        const uint32_t nk = len[j] - a.k + 1;
+        K f, r;
+        kmer_at(a.packed, st[j] + ix[j], a.k, f, r);
+        const bool fwd = O::le(f, r);
+        uint32_t contrib = 0;
+        if (!(fl[j] & 1u) && ix[j] == 0) contrib |= fwd ? BOUND_LEFT : BOUND_RIGHT;
+        if (!(fl[j] & 2u) && ix[j] == nk - 1) contrib |= fwd ? BOUND_RIGHT : BOUND_LEFT;
+        const bool owned = !((ix[j] == 0 && !(fl[j] & 4u)) || (ix[j] == nk - 1 && !(fl[j] & 8u)));
+        const uint32_t rank = a.rank0 ? uint32_t(rk[j]) + ix[j] : uint32_t(t - rk[j]);
+        keys[t - a.t0] = fwd ? f : r;
+        vals[t - a.t0] = (uint64_t(rank) << 32) | (uint64_t(col[j]) << 3) | (uint64_t(owned) << 2) | contrib;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1859,7 +1885,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     ExpArgs ea{nt, t0, s0, s1, P.tup_off.as<uint64_t>(), P.btup.as<uint64_t>(), Q.key.as<uint32_t>(),
                Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(),
                Q.packed.as<uint32_t>(), k, J.from_files ? Q.rank0.as<uint32_t>() : nullptr};
-    const uint64_t n_eblocks = (nt + TPB - 1) / TPB;
+    const uint64_t n_eblocks = (nt + EXP_TILE - 1) / EXP_TILE;
     uint32_t* block_super = P.block_super.ensure<uint32_t>(n_eblocks + 1);
     { KSCOPE("k_block_super", st); k_block_super<<<blocks_for(s1 - s0), TPB, 0, st>>>(s0, s1, t0, n_eblocks, P.tup_off.as<uint64_t>(), block_super); }
     { KSCOPE("k_expand", st); k_expand<K><<<uint32_t(n_eblocks), TPB, 0, st>>>(ea, block_super, keys, vals); }
