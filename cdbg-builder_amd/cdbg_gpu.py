@@ -45,6 +45,16 @@ class SynthParams(C.Structure):
                 ("n_inversions", C.c_uint32), ("inv_min", C.c_uint32), ("inv_max", C.c_uint32)]
 
 
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("keep_results", C.c_uint32)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("hbm_estimate", C.c_uint64), ("chunk_occurrences", C.c_uint64),
+                ("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double),
+                ("stats", Stats)]
+
+
 # Exported symbols declared in include/cdbg_gpu.h (checked by tests/test_abi.py).
 _SIGS = {
     "cdbg_gpu_last_error": (C.c_char_p, []),
@@ -76,6 +86,14 @@ _SIGS = {
                                                  C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "cdbg_gpu_job_write_phase2_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p,
                                                   C.POINTER(ColorState)]),
+    "cdbg_gpu_hbm_estimate": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "cdbg_gpu_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerParams), C.POINTER(C.c_void_p)]),
+    "cdbg_gpu_server_submit": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                         C.c_uint32, U64P]),
+    "cdbg_gpu_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(JobStatus)]),
+    "cdbg_gpu_server_result": (C.c_int, [C.c_void_p, C.c_uint64, U8P, U64P, U64P]),
+    "cdbg_gpu_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "cdbg_gpu_server_destroy": (None, [C.c_void_p]),
     "cdbg_load_fasta": (C.c_void_p, [C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32]),
     "cdbg_load_fasta_list": (C.c_void_p, [C.c_char_p, C.c_uint32]),
     "cdbg_seqs_total_bytes": (C.c_uint64, [C.c_void_p]),
@@ -204,6 +222,9 @@ class Context:
                                                     threads, C.byref(j.h)))
         return j
 
+    def server(self, slots=4, hbm_budget=0, keep_results=True):
+        return Server(self, slots, hbm_budget, keep_results)
+
     def bucket_write(self, files, k, m, num_buckets, tmp_dir, threads=8):
         unc = np.zeros(num_buckets, np.uint64)
         _check(lib().cdbg_gpu_bucket_write(self.h, _cstrs(files), len(files), k, m, num_buckets,
@@ -311,3 +332,54 @@ class Job:
         offs = np.zeros(s["num_color_sets"], np.uint64)
         _check(lib().cdbg_gpu_job_color_sets(self.h, _ptr(words, U64P), _ptr(offs, U64P)))
         return words, offs
+
+
+def hbm_estimate(bases, chunk_occurrences=0):
+    return lib().cdbg_gpu_hbm_estimate(bases, chunk_occurrences)
+
+
+class Server:
+    """Served mode (include/cdbg_gpu.h): submit jobs, poll, fetch results."""
+
+    def __init__(self, ctx, slots=4, hbm_budget=0, keep_results=True):
+        p = ServerParams(hbm_budget, slots, 1 if keep_results else 0)
+        self.h = C.c_void_p()
+        _check(lib().cdbg_gpu_server_create(ctx.h, C.byref(p), C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().cdbg_gpu_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, data, color_off, k, m, num_buckets):
+        data = np.ascontiguousarray(data, dtype=np.uint8)
+        off = np.ascontiguousarray(color_off, dtype=np.uint64)
+        jid = C.c_uint64()
+        _check(lib().cdbg_gpu_server_submit(self.h, _ptr(data, U8P), _ptr(off, U64P), len(off) - 1, k, m,
+                                            num_buckets, C.byref(jid)))
+        return jid.value
+
+    def poll(self, jid, wait=False):
+        st = JobStatus()
+        _check(lib().cdbg_gpu_server_poll(self.h, jid, 1 if wait else 0, C.byref(st)))
+        return {"state": st.state, "hbm_estimate": st.hbm_estimate, "chunk": st.chunk_occurrences,
+                "submit_s": st.submit_s, "start_s": st.start_s, "done_s": st.done_s,
+                "stats": st.stats.as_dict()}
+
+    def result(self, jid):
+        """(frags u8, colour-set words u64, class offsets u64) of a finished job."""
+        s = self.poll(jid)["stats"]
+        frags = np.zeros(s["frag_bytes"], np.uint8)
+        words = np.zeros((s["cs_bits"] + 63) // 64, np.uint64)
+        offs = np.zeros(s["num_color_sets"], np.uint64)
+        _check(lib().cdbg_gpu_server_result(self.h, jid, _ptr(frags, U8P), _ptr(words, U64P), _ptr(offs, U64P)))
+        return frags, words, offs
+
+    def release(self, jid):
+        _check(lib().cdbg_gpu_server_release(self.h, jid))

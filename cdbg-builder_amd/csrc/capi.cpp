@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "cdbg_gpu.h"
 #include "common.h"
@@ -24,6 +25,40 @@ const char* last_error() { return g_last_error.c_str(); }
 Job::~Job() {
     free_phase2(*this);
     if (st) (void)hipStreamDestroy(st);
+}
+
+void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off, uint32_t num_colors,
+                   uint32_t k, uint32_t m, uint32_t num_buckets) {
+    if (!ctx || !color_off || (!bytes && num_colors)) throw Error("null argument");
+    if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
+    if (m < 2 || m >= k) throw Error("m must satisfy 2 <= m < k");
+    if (num_buckets == 0) throw Error("num_buckets must be >= 1");
+    if (num_colors == 0) throw Error("no colours");
+    for (uint32_t c = 0; c < num_colors; ++c)
+        if (color_off[c + 1] < color_off[c]) throw Error("colour offsets must be non-decreasing");
+    J.ctx = ctx;
+    J.k = k;
+    J.m = m;
+    J.B = num_buckets;
+    J.num_colors = num_colors;
+    J.n_bytes = color_off[num_colors] - color_off[0];
+    // Pad with a non-ACGT byte so tiles and 32-byte groups need no tail case.
+    J.n_pad = ((J.n_bytes + 64) / 64) * 64 + 64;
+    J.phase1_done = J.phase2_done = J.from_files = false;
+    J.n_supers = 0;
+    J.auto_chunk = 0;
+    J.n_chunks = 0;
+    J.h_bucket_off.clear();
+    J.stats = cdbg_gpu_stats{};
+    if (!J.st) CDBG_HIP(hipStreamCreateWithFlags(&J.st, hipStreamNonBlocking));
+    uint8_t* d = J.bytes.ensure<uint8_t>(J.n_pad);
+    CDBG_HIP(hipMemsetAsync(d + J.n_bytes, '\n', J.n_pad - J.n_bytes, J.st));
+    CDBG_HIP(hipMemcpyAsync(d, bytes + color_off[0], J.n_bytes, hipMemcpyHostToDevice, J.st));
+    std::vector<uint64_t> off(num_colors + 1);
+    for (uint32_t c = 0; c <= num_colors; ++c) off[c] = color_off[c] - color_off[0];
+    uint64_t* doff = J.color_off.ensure<uint64_t>(num_colors + 1);
+    CDBG_HIP(hipMemcpyAsync(doff, off.data(), off.size() * 8, hipMemcpyHostToDevice, J.st));
+    CDBG_HIP(hipStreamSynchronize(J.st));  // the host arrays are pageable and the caller's
 }
 
 __global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
@@ -68,34 +103,11 @@ CDBG_API int cdbg_gpu_job_create(cdbg_gpu_ctx* ctx, const uint8_t* bytes, const 
                                  uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets,
                                  cdbg_gpu_job** out) {
     CDBG_TRY({
-        if (!ctx || !out || !color_off || (!bytes && num_colors)) throw Error("null argument");
-        if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
-        if (m < 2 || m >= k) throw Error("m must satisfy 2 <= m < k");
-        if (num_buckets == 0) throw Error("num_buckets must be >= 1");
-        if (num_colors == 0) throw Error("no colours");
-        for (uint32_t c = 0; c < num_colors; ++c)
-            if (color_off[c + 1] < color_off[c]) throw Error("colour offsets must be non-decreasing");
+        if (!ctx || !out) throw Error("null argument");
         CDBG_HIP(hipSetDevice(ctx->device));
         auto* h = new cdbg_gpu_job();
-        Job& J = h->J;
         try {
-            J.ctx = ctx;
-            J.k = k;
-            J.m = m;
-            J.B = num_buckets;
-            J.num_colors = num_colors;
-            J.n_bytes = color_off[num_colors] - color_off[0];
-            // Pad with a non-ACGT byte so tiles and 32-byte groups need no tail case.
-            J.n_pad = ((J.n_bytes + 64) / 64) * 64 + 64;
-            CDBG_HIP(hipStreamCreateWithFlags(&J.st, hipStreamNonBlocking));
-            uint8_t* d = J.bytes.ensure<uint8_t>(J.n_pad);
-            CDBG_HIP(hipMemsetAsync(d + J.n_bytes, '\n', J.n_pad - J.n_bytes, J.st));
-            CDBG_HIP(hipMemcpyAsync(d, bytes + color_off[0], J.n_bytes, hipMemcpyHostToDevice, J.st));
-            std::vector<uint64_t> off(num_colors + 1);
-            for (uint32_t c = 0; c <= num_colors; ++c) off[c] = color_off[c] - color_off[0];
-            uint64_t* doff = J.color_off.ensure<uint64_t>(num_colors + 1);
-            CDBG_HIP(hipMemcpyAsync(doff, off.data(), off.size() * 8, hipMemcpyHostToDevice, J.st));
-            CDBG_HIP(hipStreamSynchronize(J.st));
+            job_set_input(h->J, ctx, bytes, color_off, num_colors, k, m, num_buckets);
         } catch (...) {
             delete h;
             throw;

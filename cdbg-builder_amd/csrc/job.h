@@ -48,6 +48,10 @@ struct Job {
     ~Job();
 };
 
+// (Re)loads a job's input: validates the arguments, uploads the sequences and
+// colour offsets on the job's stream, keeps the device buffers it already has.
+void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off, uint32_t num_colors,
+                   uint32_t k, uint32_t m, uint32_t num_buckets);
 void run_phase1(Job& J);
 void run_phase2(Job& J);
 void free_phase2(Job& J);

@@ -37,6 +37,7 @@ struct DevBuf {
         if (p) {
             (void)hipFree(p);
             g_device_bytes -= cap;
+            t_device_bytes -= int64_t(cap);
         }
         p = nullptr;
         cap = 0;
@@ -50,6 +51,7 @@ struct DevBuf {
             CDBG_HIP(hipMalloc(&p, bytes));
             cap = bytes;
             g_device_bytes += cap;
+            t_device_bytes += int64_t(cap);
         }
         return (T*)p;
     }
@@ -383,18 +385,19 @@ inline size_t radix_scatter_lds(int bits) {
     return RS_TILE * (sizeof(K) + sizeof(V)) + ((2 + RS_WAVES) * R + RS_WAVES + 1) * sizeof(uint32_t);
 }
 
-// Lets k_radix_scatter<K, V> use the dynamic LDS a pass of `bits`-bit digits
-// needs (raised once per size, up to the 160 KiB of a gfx950 workgroup).
+// Lets k_radix_scatter<K, V> use up to the 160 KiB of LDS of a gfx950
+// workgroup (set once per instantiation, thread-safe: jobs of a server run
+// on several host threads); a pass of `bits`-bit digits must fit it.
 template <class K, class V>
 inline void radix_scatter_prepare(int bits) {
-    static size_t granted = 0;
-    const size_t need = radix_scatter_lds<K, V>(bits);
-    if (need > 160 * 1024) throw Error("segmented_radix_sort: digit width too large for LDS");
-    if (need > granted) {
+    constexpr size_t LDS_MAX = 160 * 1024;
+    if (radix_scatter_lds<K, V>(bits) > LDS_MAX) throw Error("segmented_radix_sort: digit width too large for LDS");
+    static const bool done = [] {
         CDBG_HIP(hipFuncSetAttribute((const void*)&k_radix_scatter<K, V>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     int(need)));
-        granted = need;
-    }
+                                     int(LDS_MAX)));
+        return true;
+    }();
+    (void)done;
 }
 
 // Host-side tile plan for a segmented sort.  seg_off has nseg+1 entries.

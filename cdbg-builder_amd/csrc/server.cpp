@@ -1,0 +1,339 @@
+// Served mode: a request loop over build jobs (sequences + k) on one GPU.
+//
+// The reference builds one genome list per process run; its phase-2
+// scheduler admits buckets against a RAM budget (24 x the bucket's
+// uncompressed bytes, live-RSS ceiling: bucket_walker.hpp:623-671).  Here the
+// unit of admission is a whole job and the budget is HBM:
+//
+//   * submit() copies the request to the host queue and returns an id;
+//   * `slots` worker threads each own a Job (device buffers, one HIP stream)
+//     that stays resident between requests, so a slot's next job reuses its
+//     buffers; jobs on different slots overlap on the device;
+//   * the queue head is admitted when a slot is free and the HBM the slots
+//     would hold with it (each running slot: the larger of what it holds and
+//     its job's estimate; each idle slot: what it holds) fits the budget.
+//     A slot that holds more than its share after a job releases its
+//     buffers;
+//   * phase 2 of a job is chunked so that its estimate fits one slot's share
+//     (results do not depend on chunking);
+//   * poll() reports the state, the stats and the submit / start / done times
+//     (steady clock, seconds), result() copies the phase-2 artifacts.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "cdbg_gpu.h"
+#include "common.h"
+#include "job.h"
+
+namespace cdbg_amd {
+
+int set_error(const std::string& msg);
+
+namespace {
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// HBM model of one job, from the sizes of its device buffers: phase 1 holds
+// the input (1 B/base), 2-bit bases and ACGT bitmap (3/8 B/base), the tile
+// staging of super-k-mer starts and ends (6 B/base) and the records with
+// their sort buffers (~57 B per super-k-mer, about 0.2 per base); phase 2
+// holds ~160 B per k-mer occurrence of its chunk plus the output streams
+// (~6 B per occurrence).  Occurrences are at most the bases.
+constexpr double P1_PER_BASE = 1.0 + 0.375 + 6.0 + 57.0 * 0.2;
+constexpr double P2_PER_OCC = 166.0;
+constexpr uint64_t FIXED_BYTES = 64ull << 20;
+
+}  // namespace
+
+uint64_t hbm_estimate(uint64_t bases, uint64_t chunk_occ) {
+    const uint64_t occ = chunk_occ ? std::min(bases, chunk_occ) : bases;
+    return FIXED_BYTES + uint64_t(P1_PER_BASE * double(bases)) + uint64_t(P2_PER_OCC * double(occ));
+}
+
+// phase-2 chunk (k-mer occurrences) that keeps a job's estimate within cap
+uint64_t chunk_for_cap(uint64_t bases, uint64_t cap) {
+    const double p1 = double(FIXED_BYTES) + P1_PER_BASE * double(bases);
+    const double left = double(cap) - p1;
+    const uint64_t occ = left > 0 ? uint64_t(left / P2_PER_OCC) : 0;
+    return std::max<uint64_t>(occ, 1u << 20);
+}
+
+struct SJob {
+    uint64_t id = 0;
+    std::vector<uint8_t> bytes;
+    std::vector<uint64_t> off;
+    uint32_t nc = 0, k = 0, m = 0, B = 0;
+    uint64_t est = 0, chunk = 0;
+    int state = 0;  // 0 queued, 1 running, 2 done, -1 failed
+    std::string error;
+    double t_submit = 0, t_start = 0, t_done = 0;
+    cdbg_gpu_stats stats{};
+    std::vector<uint8_t> frags;
+    std::vector<uint64_t> words, offsets;
+};
+
+struct Slot {
+    std::unique_ptr<Job> job;
+    int64_t held = 0;      // device bytes the slot's thread holds
+    uint64_t reserve = 0;  // while running: max(held, estimate)
+    bool busy = false;
+};
+
+struct Server {
+    cdbg_gpu_ctx* ctx = nullptr;
+    uint64_t budget = 0;
+    uint32_t n_slots = 1;
+    bool keep_results = true;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<SJob*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<SJob>> jobs;
+    std::vector<Slot> slots;
+    std::vector<std::thread> threads;
+    uint64_t next_id = 1;
+    bool stop = false;
+
+    uint64_t share() const { return budget / n_slots; }
+
+    // the queue head fits next to what the other slots hold or may grow to
+    bool admissible(uint32_t me, const SJob& j) const {
+        uint64_t total = 0;
+        for (uint32_t s = 0; s < n_slots; ++s) {
+            if (s == me) continue;
+            total += slots[s].busy ? slots[s].reserve : uint64_t(std::max<int64_t>(slots[s].held, 0));
+        }
+        const uint64_t mine = std::max<uint64_t>(uint64_t(std::max<int64_t>(slots[me].held, 0)), j.est);
+        // a job larger than the whole budget still runs once it is alone
+        return total + mine <= budget || total == 0;
+    }
+
+    void worker(uint32_t me) {
+        (void)hipSetDevice(ctx->device);
+        for (;;) {
+            SJob* j = nullptr;
+            bool release = false;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+                if (stop) break;
+                if (!admissible(me, *queue.front())) {
+                    // this idle slot's resident buffers may be what keeps the head out
+                    if (slots[me].job) {
+                        release = true;
+                    } else {
+                        cv_work.wait(lk);  // until a job ends or arrives
+                        continue;
+                    }
+                } else {
+                    j = queue.front();
+                    queue.pop_front();
+                    j->state = 1;
+                    j->t_start = now_s();
+                    slots[me].busy = true;
+                    slots[me].reserve = std::max<uint64_t>(uint64_t(std::max<int64_t>(slots[me].held, 0)), j->est);
+                }
+            }
+            if (release) {
+                slots[me].job.reset();
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    slots[me].held = t_device_bytes;
+                }
+                cv_work.notify_all();
+                continue;
+            }
+            cv_work.notify_all();  // the next head may fit another slot
+            std::string err;
+            try {
+                if (!slots[me].job) slots[me].job = std::make_unique<Job>();
+                Job& J = *slots[me].job;
+                job_set_input(J, ctx, j->bytes.data(), j->off.data(), j->nc, j->k, j->m, j->B);
+                J.chunk_tuples = j->chunk;
+                run_phase1(J);
+                run_phase2(J);
+                J.phase2_done = true;
+                j->stats = J.stats;
+                if (keep_results) {
+                    j->frags.resize(J.stats.frag_bytes);
+                    j->words.resize((J.stats.cs_bits + 63) / 64);
+                    j->offsets.resize(J.stats.num_color_sets);
+                    copy_phase2_results(J, j->frags.data(), j->words.data(), j->offsets.data());
+                }
+            } catch (std::exception& e) {
+                err = e.what();
+            } catch (...) {
+                err = "unknown C++ exception";
+            }
+            if (!err.empty() || t_device_bytes > int64_t(share())) slots[me].job.reset();  // give the HBM back
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                slots[me].held = t_device_bytes;
+                slots[me].busy = false;
+                slots[me].reserve = 0;
+                j->t_done = now_s();
+                j->state = err.empty() ? 2 : -1;
+                j->error = err;
+                j->bytes = std::vector<uint8_t>();  // the input is no longer needed
+                j->off = std::vector<uint64_t>();
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+        slots[me].job.reset();
+    }
+};
+
+}  // namespace cdbg_amd
+
+using namespace cdbg_amd;
+
+struct cdbg_gpu_server {
+    Server S;
+};
+
+#define CDBG_TRY(...)                              \
+    try {                                          \
+        __VA_ARGS__;                               \
+        return 0;                                  \
+    } catch (std::exception & e) {                 \
+        return set_error(e.what());                \
+    } catch (...) {                                \
+        return set_error("unknown C++ exception"); \
+    }
+
+extern "C" {
+
+CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint64_t chunk_occurrences) {
+    return hbm_estimate(bases, chunk_occurrences);
+}
+
+CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params, cdbg_gpu_server** out) {
+    CDBG_TRY({
+        if (!ctx || !params || !out) throw Error("null argument");
+        if (params->slots == 0 || params->slots > 16) throw Error("slots must be in [1, 16]");
+        CDBG_HIP(hipSetDevice(ctx->device));
+        uint64_t budget = params->hbm_budget;
+        if (budget == 0) {
+            size_t free_b = 0, total_b = 0;
+            CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
+            budget = uint64_t(0.85 * double(free_b));
+        }
+        auto* h = new cdbg_gpu_server();
+        Server& S = h->S;
+        S.ctx = ctx;
+        S.budget = budget;
+        S.n_slots = params->slots;
+        S.keep_results = params->keep_results != 0;
+        S.slots.resize(S.n_slots);
+        for (uint32_t s = 0; s < S.n_slots; ++s) S.threads.emplace_back([&S, s] { S.worker(s); });
+        *out = h;
+    })
+}
+
+CDBG_API int cdbg_gpu_server_submit(cdbg_gpu_server* srv, const uint8_t* bytes, const uint64_t* color_off,
+                                    uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets,
+                                    uint64_t* job_id) {
+    CDBG_TRY({
+        if (!srv || !color_off || !job_id || (!bytes && num_colors)) throw Error("null argument");
+        if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
+        if (m < 2 || m >= k) throw Error("m must satisfy 2 <= m < k");
+        if (num_buckets == 0) throw Error("num_buckets must be >= 1");
+        if (num_colors == 0) throw Error("no colours");
+        Server& S = srv->S;
+        auto j = std::make_unique<SJob>();
+        const uint64_t n = color_off[num_colors] - color_off[0];
+        j->bytes.assign(bytes + color_off[0], bytes + color_off[0] + n);
+        j->off.resize(num_colors + 1);
+        for (uint32_t c = 0; c <= num_colors; ++c) j->off[c] = color_off[c] - color_off[0];
+        j->nc = num_colors;
+        j->k = k;
+        j->m = m;
+        j->B = num_buckets;
+        j->chunk = chunk_for_cap(n, S.share());
+        j->est = hbm_estimate(n, j->chunk);
+        j->t_submit = now_s();
+        {
+            std::lock_guard<std::mutex> lk(S.mu);
+            j->id = S.next_id++;
+            *job_id = j->id;
+            S.queue.push_back(j.get());
+            S.jobs.emplace(j->id, std::move(j));
+        }
+        S.cv_work.notify_all();
+    })
+}
+
+CDBG_API int cdbg_gpu_server_poll(cdbg_gpu_server* srv, uint64_t job_id, int wait, cdbg_gpu_job_status* out) {
+    CDBG_TRY({
+        if (!srv || !out) throw Error("null argument");
+        Server& S = srv->S;
+        std::unique_lock<std::mutex> lk(S.mu);
+        auto it = S.jobs.find(job_id);
+        if (it == S.jobs.end()) throw Error("unknown job id " + std::to_string(job_id));
+        SJob& j = *it->second;
+        if (wait) S.cv_done.wait(lk, [&] { return j.state == 2 || j.state == -1; });
+        out->state = j.state;
+        out->hbm_estimate = j.est;
+        out->chunk_occurrences = j.chunk;
+        out->submit_s = j.t_submit;
+        out->start_s = j.t_start;
+        out->done_s = j.t_done;
+        out->stats = j.stats;
+        if (j.state == -1) throw Error("job " + std::to_string(job_id) + " failed: " + j.error);
+    })
+}
+
+CDBG_API int cdbg_gpu_server_result(cdbg_gpu_server* srv, uint64_t job_id, uint8_t* frags, uint64_t* words,
+                                    uint64_t* offsets) {
+    CDBG_TRY({
+        if (!srv) throw Error("null argument");
+        Server& S = srv->S;
+        std::lock_guard<std::mutex> lk(S.mu);
+        auto it = S.jobs.find(job_id);
+        if (it == S.jobs.end()) throw Error("unknown job id " + std::to_string(job_id));
+        SJob& j = *it->second;
+        if (j.state != 2) throw Error("job " + std::to_string(job_id) + " has not finished");
+        if (!S.keep_results) throw Error("the server does not keep results");
+        if (frags && !j.frags.empty()) std::copy(j.frags.begin(), j.frags.end(), frags);
+        if (words && !j.words.empty()) std::copy(j.words.begin(), j.words.end(), words);
+        if (offsets && !j.offsets.empty()) std::copy(j.offsets.begin(), j.offsets.end(), offsets);
+    })
+}
+
+CDBG_API int cdbg_gpu_server_release(cdbg_gpu_server* srv, uint64_t job_id) {
+    CDBG_TRY({
+        if (!srv) throw Error("null argument");
+        Server& S = srv->S;
+        std::lock_guard<std::mutex> lk(S.mu);
+        auto it = S.jobs.find(job_id);
+        if (it == S.jobs.end()) throw Error("unknown job id " + std::to_string(job_id));
+        if (it->second->state != 2 && it->second->state != -1) throw Error("job is still queued or running");
+        S.jobs.erase(it);
+    })
+}
+
+CDBG_API void cdbg_gpu_server_destroy(cdbg_gpu_server* srv) {
+    if (!srv) return;
+    Server& S = srv->S;
+    {
+        std::lock_guard<std::mutex> lk(S.mu);
+        S.stop = true;
+    }
+    S.cv_work.notify_all();
+    for (auto& t : S.threads) t.join();
+    delete srv;
+}
+
+}  // extern "C"

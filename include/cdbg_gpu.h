@@ -164,6 +164,49 @@ CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, 
 CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag_path, const char* color_sets_path,
                                              cdbg_gpu_color_state* state);
 
+/* Served mode: a request loop over build jobs (sequences + k).  The reference
+ * runs one genome list per process; its phase-2 scheduler admits buckets
+ * against a RAM budget (bucket_walker.hpp:623-671, cost 24 x uncompressed
+ * bucket bytes).  Here `slots` worker threads each keep a job's device
+ * buffers and HIP stream resident between requests; the queue head is
+ * admitted (FIFO) when a slot is free and the HBM the slots would hold with
+ * it fits hbm_budget, by the estimate of cdbg_gpu_hbm_estimate; jobs on
+ * different slots overlap on the device.  A job's results are the same as
+ * those of cdbg_gpu_job_run on the same input.  Thread-safe. */
+typedef struct cdbg_gpu_server cdbg_gpu_server;
+typedef struct {
+    uint64_t hbm_budget;   /* device bytes the server may hold (0: 85% of free HBM at creation) */
+    uint32_t slots;        /* jobs on the device at once, 1..16 */
+    uint32_t keep_results; /* 1: keep each job's artifacts on the host for cdbg_gpu_server_result */
+} cdbg_gpu_server_params;
+typedef struct {
+    int32_t state;              /* 0 queued, 1 running, 2 done, -1 failed */
+    uint64_t hbm_estimate;      /* bytes admitted for the job */
+    uint64_t chunk_occurrences; /* phase-2 chunk that keeps the estimate within one slot's share */
+    double submit_s, start_s, done_s; /* steady-clock seconds: submitted, admitted, artifacts on the host */
+    cdbg_gpu_stats stats;
+} cdbg_gpu_job_status;
+/* HBM model: device bytes of a job over `bases` input bytes whose phase 2 runs
+ * in chunks of `chunk_occurrences` k-mer occurrences (0: one chunk). */
+CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint64_t chunk_occurrences);
+CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params,
+                                    cdbg_gpu_server** out);
+/* Copies the request (layout of cdbg_gpu_job_create) and queues it. */
+CDBG_API int cdbg_gpu_server_submit(cdbg_gpu_server* srv, const uint8_t* bytes, const uint64_t* color_off,
+                                    uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets,
+                                    uint64_t* job_id);
+/* State of a job; wait != 0 blocks until it is done or failed.  Returns -1
+ * (message in cdbg_gpu_last_error) for an unknown or failed job. */
+CDBG_API int cdbg_gpu_server_poll(cdbg_gpu_server* srv, uint64_t job_id, int wait, cdbg_gpu_job_status* out);
+/* Artifacts of a finished job: stats.frag_bytes bytes, ceil(stats.cs_bits/64)
+ * words, stats.num_color_sets offsets (any pointer may be NULL). */
+CDBG_API int cdbg_gpu_server_result(cdbg_gpu_server* srv, uint64_t job_id, uint8_t* frags, uint64_t* words,
+                                    uint64_t* offsets);
+/* Forgets a finished or failed job and its host copies. */
+CDBG_API int cdbg_gpu_server_release(cdbg_gpu_server* srv, uint64_t job_id);
+/* Stops the workers after their running jobs; queued jobs are dropped. */
+CDBG_API void cdbg_gpu_server_destroy(cdbg_gpu_server* srv);
+
 /* Host input: FASTA/FASTQ (plain or gzip) with kseq record semantics
  * (external/kseq.h:161-201).  Returns a handle holding the job layout
  * (bytes + colour offsets) or NULL. */
