@@ -20,6 +20,7 @@ phases) timed on a bounded sample of the same workload.
 import argparse
 import importlib.util
 import json
+import math
 import os
 import sys
 import time
@@ -78,6 +79,108 @@ def step_bytes(name, s, k, launches, chunks):
     return table.get(name)
 
 
+def serve_workload(n, rate, seed):
+    """Build requests of the serving configuration: colours log-uniform in
+    [2, 200], genomes U[0.5, 5] Mbp at U[0.5, 3]% divergence (SNPs; indels at a
+    tenth of it), k drawn from {31, 63}; Poisson arrivals at `rate` jobs/s."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n):
+        nc = int(round(math.exp(rng.uniform(math.log(2), math.log(200)))))
+        glen = int(rng.uniform(0.5e6, 5e6))
+        div = float(rng.uniform(0.005, 0.03))
+        k = 31 if rng.random() < 0.5 else 63
+        jobs.append(dict(seed=10_000 + i, num_colors=nc, genome_len=glen, snp_rate=div, indel_rate=div / 10,
+                         k=k, m=21 if k == 31 else 27, B=512))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    return jobs, arrivals
+
+
+def served_leg(G, ctx, n_jobs, slots, rate, threads):
+    """Plain (a new job per request, one at a time) against served (resident
+    per-slot buffers; one slot, then `slots` jobs overlapping on the device)
+    over the same requests: aggregate distinct k-mers/s when every request is
+    queued at once, and p50/p99 job latency (submit -> artifacts complete in
+    device memory) under Poisson arrivals; the plain latencies replay the arrivals through a
+    FIFO queue with the measured plain service times."""
+    import numpy as np
+    jobs, arrivals = serve_workload(n_jobs, rate, seed=5)
+    t_gen = time.perf_counter()
+    inputs = []
+    for j in jobs:
+        data, off = G.synth(seed=j["seed"], num_colors=j["num_colors"], genome_len=j["genome_len"],
+                            snp_rate=j["snp_rate"], indel_rate=j["indel_rate"], threads=threads)
+        inputs.append((data, off, j["k"], j["m"], j["B"]))
+    bases = sum(len(x[0]) for x in inputs)
+    print(f"bench: serving workload {n_jobs} jobs, {bases / 1e9:.2f} Gbp, generated in "
+          f"{time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+
+    def plain(x):  # the artifacts stay in device memory, as the served jobs' do
+        with ctx.job(*x) as jb:
+            jb.run()
+            return jb.stats()["num_kmers"]
+
+    for x in inputs[:2]:  # both key widths: load the kernels once
+        plain(x)
+    service, kmers = [], []
+    for x in inputs:
+        t0 = time.perf_counter()
+        kmers.append(plain(x))
+        service.append(time.perf_counter() - t0)
+    plain_tput = sum(kmers) / sum(service)
+    fin, lat_plain = 0.0, []
+    for a, sv in zip(arrivals, service):
+        fin = max(fin, a) + sv
+        lat_plain.append(fin - a)
+    print(f"bench: plain {sum(service):.2f} s for {n_jobs} jobs", file=sys.stderr, flush=True)
+
+    def closed_batch(srv, n_slots):
+        warm = [srv.submit(*x) for x in inputs[:n_slots]]  # every slot loads its buffers once
+        for jid in warm:
+            srv.poll(jid, wait=True)
+            srv.release(jid)
+        t0 = time.perf_counter()
+        ids = [srv.submit(*x) for x in inputs]
+        got = []
+        for jid in ids:
+            got.append(srv.poll(jid, wait=True)["stats"]["num_kmers"])
+            srv.release(jid)
+        wall = time.perf_counter() - t0
+        if got != kmers:
+            raise RuntimeError("served results differ from the plain path")
+        print(f"bench: served, {n_slots} slot(s), all queued: {wall:.2f} s", file=sys.stderr, flush=True)
+        return sum(got) / wall
+
+    with ctx.server(slots=1) as srv:
+        resident_tput = closed_batch(srv, 1)
+    with ctx.server(slots=slots) as srv:
+        served_tput = closed_batch(srv, slots)
+        t0 = time.perf_counter()
+        ids = []
+        for a, x in zip(arrivals, inputs):
+            dt = t0 + a - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(*x))
+        lat_served = []
+        for jid in ids:
+            st = srv.poll(jid, wait=True)
+            lat_served.append(st["done_s"] - st["submit_s"])
+            srv.release(jid)
+    pct = lambda v, q: round(1e3 * float(np.percentile(v, q)), 1)
+    return {"jobs": n_jobs, "slots": slots, "bases": bases, "distinct_kmers": sum(kmers),
+            "plain": {"value": round(plain_tput, 1), "p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99)},
+            "served_1_slot": {"value": round(resident_tput, 1)},
+            "served": {"value": round(served_tput, 1), "p50_ms": pct(lat_served, 50),
+                       "p99_ms": pct(lat_served, 99)},
+            "served_over_plain": round(served_tput / plain_tput, 3),
+            "arrival_rate": rate,
+            "workload": f"{n_jobs} build requests: colours log-uniform [2,200], genomes U[0.5,5] Mbp, "
+                        f"divergence U[0.5,3]%, k in {{31,63}} (m 21/27), B=512; throughput with every "
+                        f"request queued at once, latency under Poisson arrivals at {rate} jobs/s"}
+
+
 def cpu_baseline(G, sample_colors, genome, k, m, B, cores):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
@@ -103,6 +206,9 @@ def main():
     ap.add_argument("--cpu-sample-colors", type=int, default=30)
     ap.add_argument("--cpu-cores", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=40, help="requests of the served leg (0: skip it)")
+    ap.add_argument("--serve-slots", type=int, default=2)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel totals to stderr")
     ap.add_argument("--copy-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration)")
@@ -182,6 +288,10 @@ def main():
     top = sorted(kt.items(), key=lambda x: -x[1][1])[:6]
     kernels = {n: round(ms / args.steps, 2) for n, (c, ms) in top}
 
+    job.close()
+    served = None
+    if args.serve_jobs:
+        served = served_leg(G, ctx, args.serve_jobs, args.serve_slots, args.serve_rate, args.cpu_cores)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(G, args.cpu_sample_colors, args.genome, args.k, args.m, args.buckets,
@@ -207,10 +317,10 @@ def main():
         "roofline": roof,
         "top_kernels_ms_per_step": kernels,
         "chunks": chunks,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
-    job.close()
     ctx.close()
 
 

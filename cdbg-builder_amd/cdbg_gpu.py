@@ -86,7 +86,7 @@ _SIGS = {
                                                  C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "cdbg_gpu_job_write_phase2_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p,
                                                   C.POINTER(ColorState)]),
-    "cdbg_gpu_hbm_estimate": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+    "cdbg_gpu_hbm_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint64]),
     "cdbg_gpu_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerParams), C.POINTER(C.c_void_p)]),
     "cdbg_gpu_server_submit": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, U64P]),
@@ -334,8 +334,8 @@ class Job:
         return words, offs
 
 
-def hbm_estimate(bases, chunk_occurrences=0):
-    return lib().cdbg_gpu_hbm_estimate(bases, chunk_occurrences)
+def hbm_estimate(bases, k=31, chunk_occurrences=0):
+    return lib().cdbg_gpu_hbm_estimate(bases, k, chunk_occurrences)
 
 
 class Server:

@@ -250,6 +250,7 @@ void parallel_for(uint32_t n, uint32_t threads, F&& f) {
 }  // namespace
 
 void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64_t* unc_bytes) {
+    AllocStream alloc_on_job_stream(J.st);
     if (!J.phase1_done || !J.bytes.p) throw Error("bucket files need phase 1 of a sequence job");
     hipStream_t st = J.st;
     Phase1Bufs& Q = J.p1;
@@ -280,6 +281,7 @@ void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64
 }
 
 void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
+    AllocStream alloc_on_job_stream(J.st);
     const uint32_t B = J.B, k = J.k;
     std::vector<ParsedBucket> pb(B);
     parallel_for(B, threads, [&](uint32_t b) { parse_bucket(bucket_path(dir, b), k, pb[b]); });
@@ -341,6 +343,7 @@ void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
 
 void write_phase2_files(Job& J, const std::string& frag_path, const std::string& cs_path,
                         cdbg_gpu_color_state* state) {
+    AllocStream alloc_on_job_stream(J.st);
     if (!J.phase2_done) throw Error("phase 2 has not run");
     const cdbg_gpu_stats& s = J.stats;
     std::vector<uint8_t> frags(s.frag_bytes);

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,10 +23,36 @@ int set_error(const std::string& msg) {
 }
 const char* last_error() { return g_last_error.c_str(); }
 
-Job::~Job() {
-    free_phase2(*this);
-    if (st) (void)hipStreamDestroy(st);
+namespace {
+std::mutex g_stream_mu;
+std::vector<std::pair<int, hipStream_t>> g_idle_streams;  // live until the process ends
+}  // namespace
+
+hipStream_t acquire_stream(int device) {
+    {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
+        for (size_t i = 0; i < g_idle_streams.size(); ++i)
+            if (g_idle_streams[i].first == device) {
+                hipStream_t s = g_idle_streams[i].second;
+                g_idle_streams.erase(g_idle_streams.begin() + i);
+                return s;
+            }
+    }
+    hipStream_t s = nullptr;
+    CDBG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
 }
+
+void release_stream(int device, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    g_idle_streams.emplace_back(device, s);
+}
+
+StreamOwner::~StreamOwner() {
+    if (s) release_stream(device, s);
+}
+
+Job::~Job() { free_phase2(*this); }  // the stream goes last, after every buffer freed on it
 
 void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off, uint32_t num_colors,
                    uint32_t k, uint32_t m, uint32_t num_buckets) {
@@ -50,7 +77,12 @@ void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64
     J.n_chunks = 0;
     J.h_bucket_off.clear();
     J.stats = cdbg_gpu_stats{};
-    if (!J.st) CDBG_HIP(hipStreamCreateWithFlags(&J.st, hipStreamNonBlocking));
+    if (!J.st) {
+        J.st = acquire_stream(ctx->device);
+        J.stream_owner.s = J.st;
+        J.stream_owner.device = ctx->device;
+    }
+    AllocStream alloc_on_job_stream(J.st);
     uint8_t* d = J.bytes.ensure<uint8_t>(J.n_pad);
     CDBG_HIP(hipMemsetAsync(d + J.n_bytes, '\n', J.n_pad - J.n_bytes, J.st));
     CDBG_HIP(hipMemcpyAsync(d, bytes + color_off[0], J.n_bytes, hipMemcpyHostToDevice, J.st));
@@ -247,7 +279,9 @@ CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, 
             J.k = k;
             J.B = num_buckets;
             J.num_colors = num_colors;
-            CDBG_HIP(hipStreamCreateWithFlags(&J.st, hipStreamNonBlocking));
+            J.st = acquire_stream(ctx->device);
+            J.stream_owner.s = J.st;
+            J.stream_owner.device = ctx->device;
             load_bucket_files(J, dir, threads);
         } catch (...) {
             delete h;

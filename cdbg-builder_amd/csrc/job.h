@@ -26,7 +26,25 @@ struct Phase1Bufs {
 
 struct Phase2Bufs;  // defined in phase2.hip
 
+// Streams are created once per device and handed from finished jobs to new
+// ones: creating a stream costs tens of milliseconds here, and a new job on
+// a recycled stream reuses the blocks the previous one freed on it, in order.
+hipStream_t acquire_stream(int device);
+void release_stream(int device, hipStream_t s);
+
+// Holds a job's HIP stream; a Job declares it first, so it is given back
+// after every device buffer of the job has been freed in order on it.
+struct StreamOwner {
+    hipStream_t s = nullptr;
+    int device = 0;
+    StreamOwner() = default;
+    StreamOwner(const StreamOwner&) = delete;
+    StreamOwner& operator=(const StreamOwner&) = delete;
+    ~StreamOwner();
+};
+
 struct Job {
+    StreamOwner stream_owner;
     cdbg_gpu_ctx* ctx = nullptr;
     hipStream_t st = nullptr;
     uint32_t k = 0, m = 0, B = 0, num_colors = 0;
@@ -57,6 +75,8 @@ void run_phase2(Job& J);
 void free_phase2(Job& J);
 // Host copies of the phase-2 results (any pointer may be null).
 void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offsets);
+// Device copies of the same results on stream st (waits for the job first).
+void copy_phase2_results_device(Job& J, void* frags, void* words, void* offsets, hipStream_t st);
 // bucket_io.hip
 void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64_t* unc_bytes);
 void load_bucket_files(Job& J, const std::string& dir, uint32_t threads);

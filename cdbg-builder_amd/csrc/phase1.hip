@@ -315,6 +315,7 @@ inline uint64_t d2h_u64(const uint64_t* d, hipStream_t st) {
 }  // namespace
 
 void run_phase1(Job& J) {
+    AllocStream alloc_on_job_stream(J.st);
     hipStream_t st = J.st;
     Phase1Bufs& P = J.p1;
     hipEvent_t e0, e1;

@@ -2216,6 +2216,7 @@ static void phase2_impl(Job& J) {
     if (budget == 0) {
         size_t free_b = 0, total_b = 0;
         CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
+        free_b += size_t(g_block_cache.cached.load());  // cached blocks are free for this job
         // ~160 B per occurrence covers tuples, sort buffers, nodes, table,
         // walk and fragment arrays; buffers this phase already holds from an
         // earlier run count as available; 40% is kept for the growing output
@@ -2256,6 +2257,7 @@ static void phase2_impl(Job& J) {
 }
 
 void run_phase2(Job& J) {
+    AllocStream alloc_on_job_stream(J.st);
     if (J.k <= 31)
         phase2_impl<uint64_t>(J);
     else
@@ -2267,7 +2269,19 @@ void free_phase2(Job& J) {
     J.p2 = nullptr;
 }
 
+void copy_phase2_results_device(Job& J, void* frags, void* words, void* offsets, hipStream_t st) {
+    if (!J.phase2_done) throw Error("phase 2 has not run");
+    Phase2Bufs& P = *J.p2;
+    CDBG_HIP(hipStreamSynchronize(J.st));
+    if (frags && P.frag_total) CDBG_HIP(hipMemcpyAsync(frags, P.frag_bytes.p, P.frag_total, hipMemcpyDeviceToDevice, st));
+    const uint64_t nw = (P.cs_bits + 63) / 64;
+    if (words && nw) CDBG_HIP(hipMemcpyAsync(words, P.cs_words.p, nw * 8, hipMemcpyDeviceToDevice, st));
+    if (offsets && P.n_sets) CDBG_HIP(hipMemcpyAsync(offsets, P.set_off.p, P.n_sets * 8, hipMemcpyDeviceToDevice, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+}
+
 void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offsets) {
+    AllocStream alloc_on_job_stream(J.st);
     if (!J.phase2_done) throw Error("phase 2 has not run");
     Phase2Bufs& P = *J.p2;
     if (frags && P.frag_total) CDBG_HIP(hipMemcpy(frags, P.frag_bytes.p, P.frag_total, hipMemcpyDeviceToHost));

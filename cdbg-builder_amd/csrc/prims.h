@@ -16,6 +16,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -24,31 +26,122 @@
 namespace cdbg_amd {
 
 // ---------------------------------------------------------------------------
+// Block cache.  hipFree waits for the whole device and hipMalloc maps fresh
+// pages, so buffers released on a job stream are kept: a block goes back to
+// the cache with an event recorded on its stream, and a later request takes
+// the smallest cached block of at least its size (and at most twice it) that
+// is on the same stream or whose event has completed.  When hipMalloc runs
+// out of memory the cache is emptied (device-wide wait) and it is tried
+// again.  Process-wide and thread-safe.
+// ---------------------------------------------------------------------------
+struct BlockCache {
+    struct Block {
+        void* p;
+        hipStream_t st;
+        hipEvent_t ev;
+    };
+    std::mutex mu;
+    std::multimap<size_t, Block> blocks;
+    std::atomic<uint64_t> cached{0};
+
+    void* get(size_t bytes, hipStream_t st, size_t& got) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto it = blocks.lower_bound(bytes); it != blocks.end() && it->first <= 2 * bytes; ++it) {
+                const Block& b = it->second;
+                if (b.st == st || hipEventQuery(b.ev) == hipSuccess) {
+                    void* p = b.p;
+                    got = it->first;
+                    (void)hipEventDestroy(b.ev);
+                    cached -= got;
+                    blocks.erase(it);
+                    return p;
+                }
+            }
+        }
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            trim(2 * bytes);  // the largest blocks first, about what is asked and as much again
+            if (hipMalloc(&p, bytes) != hipSuccess) {
+                (void)hipGetLastError();
+                trim();
+                CDBG_HIP(hipMalloc(&p, bytes));
+            }
+        }
+        got = bytes;
+        return p;
+    }
+    void put(void* p, size_t size, hipStream_t st) {
+        hipEvent_t ev = nullptr;
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess || hipEventRecord(ev, st) != hipSuccess) {
+            if (ev) (void)hipEventDestroy(ev);
+            (void)hipStreamSynchronize(st);
+            (void)hipFree(p);
+            return;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        blocks.emplace(size, Block{p, st, ev});
+        cached += size;
+    }
+    // hands cached blocks back to the driver, the largest first, until at
+    // least `want` bytes are released (all of them by default)
+    void trim(uint64_t want = ~0ull) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (blocks.empty()) return;
+        (void)hipDeviceSynchronize();
+        uint64_t freed = 0;
+        while (!blocks.empty() && freed < want) {
+            auto it = std::prev(blocks.end());
+            (void)hipEventDestroy(it->second.ev);
+            (void)hipFree(it->second.p);
+            freed += it->first;
+            cached -= it->first;
+            blocks.erase(it);
+        }
+    }
+};
+inline BlockCache g_block_cache;
+
+// ---------------------------------------------------------------------------
 // Device buffer that only grows (contents are not preserved on growth).
+// With a t_alloc_stream set it comes from, and goes back to, the block
+// cache; otherwise hipMalloc / hipFree.
 // ---------------------------------------------------------------------------
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    hipStream_t st = nullptr;  // stream the buffer is used on (null: not cached)
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
         if (p) {
-            (void)hipFree(p);
+            if (st)
+                g_block_cache.put(p, cap, st);
+            else
+                (void)hipFree(p);
             g_device_bytes -= cap;
             t_device_bytes -= int64_t(cap);
         }
         p = nullptr;
         cap = 0;
+        st = nullptr;
     }
     template <class T>
     T* ensure(size_t n) {
         size_t bytes = n * sizeof(T);
         if (bytes < 256) bytes = 256;
         if (bytes > cap) {
+            if (t_grow_headroom && cap) bytes = std::max(bytes, cap + cap / 2);
             release();
-            CDBG_HIP(hipMalloc(&p, bytes));
+            if (t_alloc_stream) {
+                p = g_block_cache.get(bytes, t_alloc_stream, bytes);
+            } else {
+                CDBG_HIP(hipMalloc(&p, bytes));
+            }
+            st = t_alloc_stream;
             cap = bytes;
             g_device_bytes += cap;
             t_device_bytes += int64_t(cap);
@@ -62,6 +155,7 @@ struct DevBuf {
     void swap(DevBuf& o) {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
+        std::swap(st, o.st);
     }
     // Grows to hold n elements keeping the first `keep_bytes` bytes
     // (stream-ordered copy; waits for it before freeing the old block).

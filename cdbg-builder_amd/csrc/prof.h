@@ -69,6 +69,23 @@ inline std::atomic<uint64_t> g_device_bytes{0};
 // Device bytes allocated (net) by the calling thread: a server slot's jobs
 // are only touched by that slot's thread, so this is the slot's footprint.
 inline thread_local int64_t t_device_bytes = 0;
+// Stream that the calling thread's device buffers are used on: released
+// buffers go to the block cache tagged with it (no hipFree, which would wait
+// for the whole device).  Null: plain hipMalloc / hipFree.
+inline thread_local hipStream_t t_alloc_stream = nullptr;
+
+// A growing buffer takes 1.5 times its old size at least (server slots,
+// whose buffers follow the largest job seen so far).
+inline thread_local bool t_grow_headroom = false;
+
+// Sets t_alloc_stream for a scope.
+struct AllocStream {
+    hipStream_t prev;
+    explicit AllocStream(hipStream_t s) : prev(t_alloc_stream) { t_alloc_stream = s; }
+    ~AllocStream() { t_alloc_stream = prev; }
+    AllocStream(const AllocStream&) = delete;
+    AllocStream& operator=(const AllocStream&) = delete;
+};
 
 }  // namespace cdbg_amd
 

@@ -12,12 +12,16 @@
 //   * the queue head is admitted when a slot is free and the HBM the slots
 //     would hold with it (each running slot: the larger of what it holds and
 //     its job's estimate; each idle slot: what it holds) fits the budget.
-//     A slot that holds more than its share after a job releases its
-//     buffers;
+//     An idle slot whose buffers keep the head out releases them;
 //   * phase 2 of a job is chunked so that its estimate fits one slot's share
 //     (results do not depend on chunking);
-//   * poll() reports the state, the stats and the submit / start / done times
-//     (steady clock, seconds), result() copies the phase-2 artifacts.
+//   * a finished job's phase-2 artifacts are copied into device buffers of
+//     the job (they count against the budget until release()); poll()
+//     reports the state, the stats and the submit / start / done times
+//     (steady clock, seconds), result() copies the artifacts to the host.
+//   * released device buffers go to the block cache (no device-wide wait for
+//     a free); a job that runs out of device memory is run again once after
+//     the cache hands its blocks back to the driver.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,24 +53,25 @@ double now_s() {
 // the input (1 B/base), 2-bit bases and ACGT bitmap (3/8 B/base), the tile
 // staging of super-k-mer starts and ends (6 B/base) and the records with
 // their sort buffers (~57 B per super-k-mer, about 0.2 per base); phase 2
-// holds ~160 B per k-mer occurrence of its chunk plus the output streams
-// (~6 B per occurrence).  Occurrences are at most the bases.
+// holds ~160 B per k-mer occurrence of its chunk with 64-bit k-mers (~230 B
+// with 128-bit ones: wider tuples, sort buffers and node keys) plus the
+// output streams (~6 B per occurrence).  Occurrences are at most the bases.
 constexpr double P1_PER_BASE = 1.0 + 0.375 + 6.0 + 57.0 * 0.2;
-constexpr double P2_PER_OCC = 166.0;
 constexpr uint64_t FIXED_BYTES = 64ull << 20;
+double p2_per_occ(uint32_t k) { return k <= 31 ? 166.0 : 236.0; }
 
 }  // namespace
 
-uint64_t hbm_estimate(uint64_t bases, uint64_t chunk_occ) {
+uint64_t hbm_estimate(uint64_t bases, uint32_t k, uint64_t chunk_occ) {
     const uint64_t occ = chunk_occ ? std::min(bases, chunk_occ) : bases;
-    return FIXED_BYTES + uint64_t(P1_PER_BASE * double(bases)) + uint64_t(P2_PER_OCC * double(occ));
+    return FIXED_BYTES + uint64_t(P1_PER_BASE * double(bases)) + uint64_t(p2_per_occ(k) * double(occ));
 }
 
 // phase-2 chunk (k-mer occurrences) that keeps a job's estimate within cap
-uint64_t chunk_for_cap(uint64_t bases, uint64_t cap) {
+uint64_t chunk_for_cap(uint64_t bases, uint32_t k, uint64_t cap) {
     const double p1 = double(FIXED_BYTES) + P1_PER_BASE * double(bases);
     const double left = double(cap) - p1;
-    const uint64_t occ = left > 0 ? uint64_t(left / P2_PER_OCC) : 0;
+    const uint64_t occ = left > 0 ? uint64_t(left / p2_per_occ(k)) : 0;
     return std::max<uint64_t>(occ, 1u << 20);
 }
 
@@ -80,8 +85,8 @@ struct SJob {
     std::string error;
     double t_submit = 0, t_start = 0, t_done = 0;
     cdbg_gpu_stats stats{};
-    std::vector<uint8_t> frags;
-    std::vector<uint64_t> words, offsets;
+    DevBuf frags, words, offsets;  // phase-2 artifacts, kept on the device until released
+    uint64_t res_bytes = 0;
 };
 
 struct Slot {
@@ -102,6 +107,8 @@ struct Server {
     std::unordered_map<uint64_t, std::unique_ptr<SJob>> jobs;
     std::vector<Slot> slots;
     std::vector<std::thread> threads;
+    hipStream_t res_st = nullptr;  // stream of the result copies (lives as long as the server)
+    uint64_t result_bytes = 0;     // device bytes of finished jobs' results not yet released
     uint64_t next_id = 1;
     bool stop = false;
 
@@ -109,7 +116,7 @@ struct Server {
 
     // the queue head fits next to what the other slots hold or may grow to
     bool admissible(uint32_t me, const SJob& j) const {
-        uint64_t total = 0;
+        uint64_t total = result_bytes;
         for (uint32_t s = 0; s < n_slots; ++s) {
             if (s == me) continue;
             total += slots[s].busy ? slots[s].reserve : uint64_t(std::max<int64_t>(slots[s].held, 0));
@@ -121,6 +128,7 @@ struct Server {
 
     void worker(uint32_t me) {
         (void)hipSetDevice(ctx->device);
+        t_grow_headroom = true;  // the slot's buffers follow the largest job seen
         for (;;) {
             SJob* j = nullptr;
             bool release = false;
@@ -156,27 +164,39 @@ struct Server {
             }
             cv_work.notify_all();  // the next head may fit another slot
             std::string err;
-            try {
-                if (!slots[me].job) slots[me].job = std::make_unique<Job>();
-                Job& J = *slots[me].job;
-                job_set_input(J, ctx, j->bytes.data(), j->off.data(), j->nc, j->k, j->m, j->B);
-                J.chunk_tuples = j->chunk;
-                run_phase1(J);
-                run_phase2(J);
-                J.phase2_done = true;
-                j->stats = J.stats;
-                if (keep_results) {
-                    j->frags.resize(J.stats.frag_bytes);
-                    j->words.resize((J.stats.cs_bits + 63) / 64);
-                    j->offsets.resize(J.stats.num_color_sets);
-                    copy_phase2_results(J, j->frags.data(), j->words.data(), j->offsets.data());
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                err.clear();
+                try {
+                    // cached blocks are reused by this job or handed back
+                    // when an allocation runs short; a kernel launch that
+                    // runs short fails the attempt, and the second one
+                    // starts from an empty cache
+                    if (attempt > 0) g_block_cache.trim();
+                    if (!slots[me].job) slots[me].job = std::make_unique<Job>();
+                    Job& J = *slots[me].job;
+                    job_set_input(J, ctx, j->bytes.data(), j->off.data(), j->nc, j->k, j->m, j->B);
+                    J.chunk_tuples = j->chunk;
+                    run_phase1(J);
+                    run_phase2(J);
+                    J.phase2_done = true;
+                    j->stats = J.stats;
+                    if (keep_results) {
+                        AllocStream on_result_stream(res_st);
+                        uint8_t* f = j->frags.ensure<uint8_t>(J.stats.frag_bytes);
+                        uint64_t* w = j->words.ensure<uint64_t>((J.stats.cs_bits + 63) / 64);
+                        uint64_t* o = j->offsets.ensure<uint64_t>(J.stats.num_color_sets);
+                        copy_phase2_results_device(J, f, w, o, res_st);
+                        j->res_bytes = j->frags.cap + j->words.cap + j->offsets.cap;
+                        t_device_bytes -= int64_t(j->res_bytes);  // they belong to the server, not the slot
+                    }
+                } catch (std::exception& e) {
+                    err = e.what();
+                } catch (...) {
+                    err = "unknown C++ exception";
                 }
-            } catch (std::exception& e) {
-                err = e.what();
-            } catch (...) {
-                err = "unknown C++ exception";
+                if (err.find("out of memory") == std::string::npos) break;
             }
-            if (!err.empty() || t_device_bytes > int64_t(share())) slots[me].job.reset();  // give the HBM back
+            if (!err.empty()) slots[me].job.reset();  // a failed job leaves no state behind
             {
                 std::lock_guard<std::mutex> lk(mu);
                 slots[me].held = t_device_bytes;
@@ -185,6 +205,7 @@ struct Server {
                 j->t_done = now_s();
                 j->state = err.empty() ? 2 : -1;
                 j->error = err;
+                result_bytes += j->res_bytes;
                 j->bytes = std::vector<uint8_t>();  // the input is no longer needed
                 j->off = std::vector<uint64_t>();
             }
@@ -215,8 +236,8 @@ struct cdbg_gpu_server {
 
 extern "C" {
 
-CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint64_t chunk_occurrences) {
-    return hbm_estimate(bases, chunk_occurrences);
+CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint64_t chunk_occurrences) {
+    return hbm_estimate(bases, k, chunk_occurrences);
 }
 
 CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params, cdbg_gpu_server** out) {
@@ -228,7 +249,8 @@ CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_par
         if (budget == 0) {
             size_t free_b = 0, total_b = 0;
             CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
-            budget = uint64_t(0.85 * double(free_b));
+            free_b += size_t(g_block_cache.cached.load());  // cached blocks are free for the server
+            budget = uint64_t(0.75 * double(free_b));
         }
         auto* h = new cdbg_gpu_server();
         Server& S = h->S;
@@ -237,6 +259,7 @@ CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_par
         S.n_slots = params->slots;
         S.keep_results = params->keep_results != 0;
         S.slots.resize(S.n_slots);
+        S.res_st = acquire_stream(ctx->device);
         for (uint32_t s = 0; s < S.n_slots; ++s) S.threads.emplace_back([&S, s] { S.worker(s); });
         *out = h;
     })
@@ -261,8 +284,8 @@ CDBG_API int cdbg_gpu_server_submit(cdbg_gpu_server* srv, const uint8_t* bytes, 
         j->k = k;
         j->m = m;
         j->B = num_buckets;
-        j->chunk = chunk_for_cap(n, S.share());
-        j->est = hbm_estimate(n, j->chunk);
+        j->chunk = chunk_for_cap(n, k, S.share());
+        j->est = hbm_estimate(n, k, j->chunk);
         j->t_submit = now_s();
         {
             std::lock_guard<std::mutex> lk(S.mu);
@@ -306,9 +329,12 @@ CDBG_API int cdbg_gpu_server_result(cdbg_gpu_server* srv, uint64_t job_id, uint8
         SJob& j = *it->second;
         if (j.state != 2) throw Error("job " + std::to_string(job_id) + " has not finished");
         if (!S.keep_results) throw Error("the server does not keep results");
-        if (frags && !j.frags.empty()) std::copy(j.frags.begin(), j.frags.end(), frags);
-        if (words && !j.words.empty()) std::copy(j.words.begin(), j.words.end(), words);
-        if (offsets && !j.offsets.empty()) std::copy(j.offsets.begin(), j.offsets.end(), offsets);
+        const cdbg_gpu_stats& st = j.stats;
+        if (frags && st.frag_bytes) CDBG_HIP(hipMemcpy(frags, j.frags.p, st.frag_bytes, hipMemcpyDeviceToHost));
+        if (words && st.cs_bits)
+            CDBG_HIP(hipMemcpy(words, j.words.p, (st.cs_bits + 63) / 64 * 8, hipMemcpyDeviceToHost));
+        if (offsets && st.num_color_sets)
+            CDBG_HIP(hipMemcpy(offsets, j.offsets.p, st.num_color_sets * 8, hipMemcpyDeviceToHost));
     })
 }
 
@@ -320,7 +346,10 @@ CDBG_API int cdbg_gpu_server_release(cdbg_gpu_server* srv, uint64_t job_id) {
         auto it = S.jobs.find(job_id);
         if (it == S.jobs.end()) throw Error("unknown job id " + std::to_string(job_id));
         if (it->second->state != 2 && it->second->state != -1) throw Error("job is still queued or running");
+        S.result_bytes -= it->second->res_bytes;
+        t_device_bytes += int64_t(it->second->res_bytes);  // freed on this thread's count
         S.jobs.erase(it);
+        S.cv_work.notify_all();  // the freed results may let the queue head in
     })
 }
 
@@ -333,7 +362,12 @@ CDBG_API void cdbg_gpu_server_destroy(cdbg_gpu_server* srv) {
     }
     S.cv_work.notify_all();
     for (auto& t : S.threads) t.join();
+    for (auto& kv : S.jobs) t_device_bytes += int64_t(kv.second->res_bytes);
+    S.jobs.clear();  // results free in order on res_st, which goes last
+    hipStream_t rs = S.res_st;
+    const int dev = S.ctx->device;
     delete srv;
+    if (rs) release_stream(dev, rs);
 }
 
 }  // extern "C"

@@ -175,20 +175,22 @@ CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag
  * those of cdbg_gpu_job_run on the same input.  Thread-safe. */
 typedef struct cdbg_gpu_server cdbg_gpu_server;
 typedef struct {
-    uint64_t hbm_budget;   /* device bytes the server may hold (0: 85% of free HBM at creation) */
+    uint64_t hbm_budget;   /* device bytes the server may hold (0: 75% of free HBM at creation) */
     uint32_t slots;        /* jobs on the device at once, 1..16 */
-    uint32_t keep_results; /* 1: keep each job's artifacts on the host for cdbg_gpu_server_result */
+    uint32_t keep_results; /* 1: keep each finished job's artifacts in device memory (counted against
+                              hbm_budget) until cdbg_gpu_server_release */
 } cdbg_gpu_server_params;
 typedef struct {
     int32_t state;              /* 0 queued, 1 running, 2 done, -1 failed */
     uint64_t hbm_estimate;      /* bytes admitted for the job */
     uint64_t chunk_occurrences; /* phase-2 chunk that keeps the estimate within one slot's share */
-    double submit_s, start_s, done_s; /* steady-clock seconds: submitted, admitted, artifacts on the host */
+    double submit_s, start_s, done_s; /* steady-clock seconds: submitted, admitted, artifacts complete */
     cdbg_gpu_stats stats;
 } cdbg_gpu_job_status;
-/* HBM model: device bytes of a job over `bases` input bytes whose phase 2 runs
- * in chunks of `chunk_occurrences` k-mer occurrences (0: one chunk). */
-CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint64_t chunk_occurrences);
+/* HBM model: device bytes of a job over `bases` input bytes with k-mer
+ * length k whose phase 2 runs in chunks of `chunk_occurrences` k-mer
+ * occurrences (0: one chunk). */
+CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint64_t chunk_occurrences);
 CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params,
                                     cdbg_gpu_server** out);
 /* Copies the request (layout of cdbg_gpu_job_create) and queues it. */
@@ -198,8 +200,9 @@ CDBG_API int cdbg_gpu_server_submit(cdbg_gpu_server* srv, const uint8_t* bytes, 
 /* State of a job; wait != 0 blocks until it is done or failed.  Returns -1
  * (message in cdbg_gpu_last_error) for an unknown or failed job. */
 CDBG_API int cdbg_gpu_server_poll(cdbg_gpu_server* srv, uint64_t job_id, int wait, cdbg_gpu_job_status* out);
-/* Artifacts of a finished job: stats.frag_bytes bytes, ceil(stats.cs_bits/64)
- * words, stats.num_color_sets offsets (any pointer may be NULL). */
+/* Copies a finished job's artifacts to the host: stats.frag_bytes bytes,
+ * ceil(stats.cs_bits/64) words, stats.num_color_sets offsets (any pointer may
+ * be NULL). */
 CDBG_API int cdbg_gpu_server_result(cdbg_gpu_server* srv, uint64_t job_id, uint8_t* frags, uint64_t* words,
                                     uint64_t* offsets);
 /* Forgets a finished or failed job and its host copies. */

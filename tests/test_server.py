@@ -13,9 +13,11 @@ from conftest import cdbg_gpu as G
 def test_hbm_estimate_is_monotone_and_chunk_bounded():
     e = [G.hbm_estimate(n) for n in (10**6, 10**7, 10**8)]
     assert e[0] < e[1] < e[2]
-    assert G.hbm_estimate(10**8, 10**6) < G.hbm_estimate(10**8)
+    assert G.hbm_estimate(10**8, 31, 10**6) < G.hbm_estimate(10**8)
     # phase 1 is not chunked: the estimate keeps growing with the input
-    assert G.hbm_estimate(10**9, 10**6) > G.hbm_estimate(10**8, 10**6)
+    assert G.hbm_estimate(10**9, 31, 10**6) > G.hbm_estimate(10**8, 31, 10**6)
+    # 128-bit k-mers need more than 64-bit ones
+    assert G.hbm_estimate(10**8, 63) > G.hbm_estimate(10**8, 31)
 
 
 JOBS = [  # (seed, colours, genome, k, m, buckets)
@@ -50,7 +52,7 @@ def test_served_results_equal_alone(gpu_ctx, slots, budget):
     inputs = _inputs()
     want = [_alone(gpu_ctx, *x) for x in inputs]
     if budget == 1:  # room for the largest job only: admission runs them one at a time
-        budget = max(G.hbm_estimate(len(x[0])) for x in inputs) + (1 << 20)
+        budget = max(G.hbm_estimate(len(x[0]), x[2]) for x in inputs) + (1 << 20)
     with gpu_ctx.server(slots=slots, hbm_budget=budget) as srv:
         ids = [srv.submit(*x) for x in inputs]
         for jid, (st, frags, words, offs) in zip(ids, want):
