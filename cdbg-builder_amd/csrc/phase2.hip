@@ -27,7 +27,8 @@
 //   k_expand<K>               thread per tuple: canonical k-mer straight from
 //                             the 2-bit window, rank|colour|owned|contrib
 //   segmented radix sort      tuples by k-mer inside each bucket
-//   k_heads / scan / k_node_* nodes: flags, primary, min rank, owned colours
+//   heads (from the cell sort) / scan / k_node_* nodes: flags, primary,
+//                             min rank, owned colours
 //   k_set_*                   colour-set interning: singleton table keyed by
 //                             colour, content-hash table for larger sets with
 //                             an exact list compare against the stored list;
@@ -78,20 +79,6 @@ __device__ __forceinline__ uint32_t bucket_of(const uint64_t* off, uint32_t B, u
             hi = mid;
     }
     return lo;
-}
-
-// node -> local bucket, by binary search over node_off staged in LDS
-constexpr uint32_t NB_LDS = 4096;
-__global__ void k_node_bucket(uint32_t n_nodes, const uint64_t* __restrict__ node_off, uint32_t nb,
-                              uint32_t* __restrict__ node_bucket) {
-    __shared__ uint64_t s_off[NB_LDS + 1];
-    const bool staged = nb <= NB_LDS;
-    if (staged)
-        for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_off[i] = node_off[i];
-    __syncthreads();
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
-    node_bucket[n] = staged ? bucket_of(s_off, nb, n) : bucket_of(node_off, nb, n);
 }
 
 __global__ void k_iota(uint32_t* p, uint64_t n) {
@@ -276,7 +263,8 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
                                                        const uint32_t* __restrict__ cell_start,
                                                        const uint32_t* __restrict__ list, uint32_t n_items,
                                                        uint32_t lo, int end_bit, uint32_t* __restrict__ next,
-                                                       unsigned long long* __restrict__ n_next) {
+                                                       unsigned long long* __restrict__ n_next,
+                                                       uint32_t* __restrict__ heads) {
     constexpr int WAVES = THREADS / 64;
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
     extern __shared__ uint64_t sm64[];
@@ -287,6 +275,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     if (blockIdx.x >= n_items) return;
     const uint32_t cell = list ? list[blockIdx.x] : blockIdx.x;
     const uint32_t start = cell_start[cell], count = cell_start[cell + 1] - start;
+    if (count == 1 && lo == 0 && threadIdx.x == 0) heads[start] = 1u;  // a one-key cell is one node
     if (count <= 1 || count <= lo) return;
     if (count > CAP) {
         if (next && threadIdx.x == 0) next[atomicAdd(n_next, 1ull)] = cell;
@@ -386,6 +375,12 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
             vals[start + i] = val[r];
         }
     }
+    // node heads of the cell: its first key, and every key unlike the one
+    // before it (the last pass left the sorted keys in LDS; a cell is one
+    // bucket, so a bucket's first tuple is a head)
+    const bool sorted_in_lds = end_bit > 0;
+    for (uint32_t i = threadIdx.x; i < count; i += THREADS)
+        heads[start + i] = (i == 0 || (sorted_in_lds && !KOps<K>::eq(s_key[i], s_key[i - 1]))) ? 1u : 0u;
 }
 
 // Lets a k_cell_sort class use its LDS (once per process).
@@ -443,6 +438,24 @@ __global__ void k_cell_starts(uint32_t n_cells, uint32_t D, const uint64_t* __re
     cell_start[c] = uint32_t(lo);
 }
 
+// Node heads of listed cells from their sorted compact copy: compact index j
+// of cell c maps to start[c] + j - pre[c].
+template <class K>
+__global__ void k_cells_heads(const uint32_t* __restrict__ start, const uint64_t* __restrict__ pre, uint32_t n_list,
+                              const K* __restrict__ ck, uint32_t* __restrict__ heads) {
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= pre[n_list]) return;
+    uint32_t lo = 0, hi = n_list;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pre[mid] <= j)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    heads[start[lo] + (j - pre[lo])] = (j == pre[lo] || !KOps<K>::eq(ck[j], ck[j - 1])) ? 1u : 0u;
+}
+
 // Moves the elements of listed cells between the tuple arrays and a compact
 // buffer (to_compact) or back: compact index j lies in cell c with
 // pre[c] <= j < pre[c + 1] and maps to start[c] + j - pre[c].
@@ -473,23 +486,25 @@ __global__ void k_cells_move(bool to_compact, const uint32_t* __restrict__ start
 // ---------------------------------------------------------------------------
 // nodes
 // ---------------------------------------------------------------------------
-template <class K>
-__global__ void k_heads(uint64_t n, const K* __restrict__ keys, uint32_t* __restrict__ heads) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    heads[i] = (i == 0 || !KOps<K>::eq(keys[i], keys[i - 1])) ? 1u : 0u;
-}
-
-__global__ void k_mark_bucket_starts(uint32_t nb, const uint64_t* __restrict__ lbtup, uint32_t* __restrict__ heads) {
-    const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b < nb && lbtup[b] < lbtup[b + 1]) heads[lbtup[b]] = 1u;
-}
-
 __global__ void k_node_start(uint64_t n, const uint32_t* __restrict__ heads, const uint32_t* __restrict__ hscan,
                              uint32_t n_nodes, uint32_t* __restrict__ node_start) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n && heads[i]) node_start[hscan[i]] = uint32_t(i);
     if (i == 0) node_start[n_nodes] = uint32_t(n);
+}
+
+// node -> local bucket, by binary search over node_off staged in LDS
+constexpr uint32_t NB_LDS = 4096;
+__global__ void k_node_bucket(uint32_t n_nodes, const uint64_t* __restrict__ node_off, uint32_t nb,
+                              uint32_t* __restrict__ node_bucket) {
+    __shared__ uint64_t s_off[NB_LDS + 1];
+    const bool staged = nb <= NB_LDS;
+    if (staged)
+        for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_off[i] = node_off[i];
+    __syncthreads();
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n >= n_nodes) return;
+    node_bucket[n] = staged ? bucket_of(s_off, nb, n) : bucket_of(node_off, nb, n);
 }
 
 __global__ void k_node_off(uint32_t nb, const uint64_t* __restrict__ lbtup, uint64_t n_tuples,
@@ -1920,7 +1935,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         cell_sort_prepare<K, uint64_t, C0, 128>();
         cell_sort_prepare<K, uint64_t, C1, 256>();
         cell_sort_prepare<K, uint64_t, C2, 512>();
-        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C0, 128><<<n_cells, 128, cell_sort_lds<K, uint64_t, C0, 128>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf); }
+        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C0, 128><<<n_cells, 128, cell_sort_lds<K, uint64_t, C0, 128>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf, heads); }
         // cells above the tiny class, listed by a scan (many: no shared counter)
         uint32_t* oflag = P.cell_oflag.ensure<uint32_t>(n_cells);
         uint32_t* oscan = P.cell_oscan.ensure<uint32_t>(uint64_t(n_cells) + 1);
@@ -1928,10 +1943,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         exclusive_scan<uint32_t, uint32_t>(oflag, oscan, n_cells, oscan + n_cells, J.rs.scan_tmp, st);
         const uint64_t n_mid = d2h<uint32_t>(oscan + n_cells, st);
         if (n_mid) { KSCOPE("k_compact", st); k_compact<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, oflag, oscan, mid); }
-        if (n_mid) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C1, 256><<<uint32_t(n_mid), 256, cell_sort_lds<K, uint64_t, C1, 256>(), st>>>(keys, vals, cell_start, mid, uint32_t(n_mid), C0, kb - D, mid2, novf + 1); }
+        if (n_mid) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C1, 256><<<uint32_t(n_mid), 256, cell_sort_lds<K, uint64_t, C1, 256>(), st>>>(keys, vals, cell_start, mid, uint32_t(n_mid), C0, kb - D, mid2, novf + 1, heads); }
         const uint64_t n_mid2 = n_mid ? d2h<unsigned long long>(novf + 1, st) : 0;
         if (n_mid2 > n_mid) throw Error("phase 2: cell sort listed more cells than exist");
-        if (n_mid2) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C2, 512><<<uint32_t(n_mid2), 512, cell_sort_lds<K, uint64_t, C2, 512>(), st>>>(keys, vals, cell_start, mid2, uint32_t(n_mid2), C1, kb - D, over, novf + 2); }
+        if (n_mid2) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C2, 512><<<uint32_t(n_mid2), 512, cell_sort_lds<K, uint64_t, C2, 512>(), st>>>(keys, vals, cell_start, mid2, uint32_t(n_mid2), C1, kb - D, over, novf + 2, heads); }
         const uint64_t n_over = n_mid2 ? d2h<unsigned long long>(novf + 2, st) : 0;
         if (n_over > n_mid2) throw Error("phase 2: cell sort listed more cells than exist");
         if (n_over) {
@@ -1961,6 +1976,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
             fplan.build(pre);
             segmented_radix_sort<K, uint64_t>(ck, cv, ckt, cvt, fplan, 0, kb - D, 8, P.rs2, st, "cells");
             { KSCOPE("k_cells_move", st); k_cells_move<K, uint64_t><<<blocks_for(tot), TPB, 0, st>>>(false, d_st, d_pre, uint32_t(n_over), keys, vals, ck, cv); }
+            { KSCOPE("k_cells_heads", st); k_cells_heads<K><<<blocks_for(tot), TPB, 0, st>>>(d_st, d_pre, uint32_t(n_over), ck, heads); }
             CDBG_HIP(hipStreamSynchronize(st));  // st_l / pre are pageable
         }
     }
@@ -1968,8 +1984,6 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     // --- nodes
     uint32_t* hscan = P.hscan.ensure<uint32_t>(nt + 1);
     uint64_t* dtot = P.total.ensure<uint64_t>(8);
-    { KSCOPE("k_heads", st); k_heads<K><<<blocks_for(nt), TPB, 0, st>>>(nt, keys, heads); }
-    { KSCOPE("k_mark_bucket_starts", st); k_mark_bucket_starts<<<blocks_for(nb), TPB, 0, st>>>(nb, lbtup, heads); }
     uint32_t* dtot32 = (uint32_t*)(dtot + 1);
     exclusive_scan<uint32_t, uint32_t>(heads, hscan, nt, dtot32, J.rs.scan_tmp, st);
     const uint32_t nn = d2h<uint32_t>(dtot32, st);
