@@ -249,13 +249,14 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
 // sorted.  A cell (a run of keys equal above end_bit) is sorted by one
 // workgroup in LDS: LSD passes of 8-bit digits, each ranking the items with
 // the per-wave scheme of k_radix_scatter and placing them in LDS, keys and
-// values held in registers between passes.  Size classes: a launch of
+// their original positions held in registers between passes; the values
+// follow once, at the end.  Size classes: a launch of
 // THREADS threads and capacity CAP takes the listed cells (or all cells)
 // with more than `lo` items; cells above CAP go to the next class's list.
 // ---------------------------------------------------------------------------
 template <class K, class V, uint32_t CAP, int THREADS>
 inline size_t cell_sort_lds() {
-    return size_t(CAP) * (sizeof(K) + sizeof(V)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
+    return size_t(CAP) * (sizeof(K) + sizeof(uint16_t)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
 }
 
 template <class K, class V, uint32_t CAP, int THREADS>
@@ -268,9 +269,12 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     constexpr int WAVES = THREADS / 64;
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
     extern __shared__ uint64_t sm64[];
+    static_assert(CAP <= 65536 && CAP % 2 == 0, "item indices are 16-bit");
+    // keys and their positions in the cell move through LDS; each value is
+    // fetched once at the end from its original position
     K* s_key = (K*)sm64;
-    V* s_val = (V*)(s_key + CAP);
-    uint32_t* wcnt = (uint32_t*)(s_val + CAP);  // [WAVES][256]
+    uint16_t* s_idx = (uint16_t*)(s_key + CAP);
+    uint32_t* wcnt = (uint32_t*)(s_idx + CAP);  // [WAVES][256]
     uint32_t* s_tmp = wcnt + WAVES * 256;
     if (blockIdx.x >= n_items) return;
     const uint32_t cell = list ? list[blockIdx.x] : blockIdx.x;
@@ -286,14 +290,14 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     const uint32_t rounds = (count + THREADS - 1) / THREADS;
     const uint32_t wbase = uint32_t(wave) * rounds * 64;  // wave w owns items [wbase, wbase + 64 rounds)
     K key[RMAX];
-    V val[RMAX];
+    uint32_t idx[RMAX];
     uint32_t dr[RMAX];
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (uint32_t(r) < rounds && i < count) {
             key[r] = keys[start + i];
-            val[r] = vals[start + i];
+            idx[r] = i;
         }
     }
     uint32_t* my = wcnt + wave * 256;
@@ -352,7 +356,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
             if (i < count) {
                 const uint32_t pos = my[dr[r] >> 16] + (dr[r] & 0xFFFFu);
                 s_key[pos] = key[r];
-                s_val[pos] = val[r];
+                s_idx[pos] = uint16_t(idx[r]);
             }
         }
         __syncthreads();
@@ -362,10 +366,17 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
             const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
             if (i < count) {
                 key[r] = s_key[i];
-                val[r] = s_val[i];
+                idx[r] = s_idx[i];
             }
         }
     }
+    V val[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; ++r) {
+        const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+        if (uint32_t(r) < rounds && i < count) val[r] = vals[start + idx[r]];
+    }
+    __syncthreads();  // every value is read before any is overwritten
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
         if (uint32_t(r) >= rounds) break;
