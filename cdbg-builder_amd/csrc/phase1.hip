@@ -384,8 +384,15 @@ void run_phase1(Job& J) {
     while ((uint64_t(1) << bbits) < J.B) ++bbits;
     SortPlan plan;
     plan.build({0, n});
+    bool in_temp = false;
     segmented_radix_sort<uint32_t, SuperRec>(key, rec, key_t, rec_t, plan, 0, bbits, bbits > 10 ? (bbits + 1) / 2 : bbits,
-                                             J.rs, st, "buckets");
+                                             J.rs, st, "buckets", &in_temp);
+    if (in_temp) {  // the sorted records are in the second buffers: use them as they are
+        P.sk_rec.swap(P.sk_rec_t);
+        P.key.swap(P.key_t);
+        rec = (SuperRec*)P.sk_rec.p;
+        key = (uint32_t*)P.key.p;
+    }
     uint64_t* b_start = P.b_start.ensure<uint64_t>(n);
     uint32_t* b_len = P.b_len.ensure<uint32_t>(n);
     uint8_t* b_flags = P.b_flags.ensure<uint8_t>(n);

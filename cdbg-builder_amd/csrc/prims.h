@@ -528,9 +528,12 @@ struct RadixScratch {
 };
 
 template <class K, class V>
+// The result is in keys/vals; with in_temp non-null an odd number of passes
+// leaves it in kt/vt instead (*in_temp = true) and saves the copy back.
 inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan& plan, int begin_bit,
                                  int end_bit, int digit_bits, RadixScratch& S, hipStream_t st,
-                                 const char* tag = "") {
+                                 const char* tag = "", bool* in_temp = nullptr) {
+    if (in_temp) *in_temp = false;
     if (plan.n == 0 || begin_bit >= end_bit) return;
     if (plan.n >= (uint64_t(1) << 32)) throw Error("segmented_radix_sort: more than 2^32 elements");
     if (digit_bits < 1 || digit_bits > 11) throw Error("segmented_radix_sort: digit width must be 1..11 bits");
@@ -561,7 +564,9 @@ inline void segmented_radix_sort(K* keys, V* vals, K* kt, V* vt, const SortPlan&
         std::swap(src_v, dst_v);
         ++passes;
     }
-    if (passes & 1) {
+    if ((passes & 1) && in_temp) {
+        *in_temp = true;
+    } else if (passes & 1) {
         CDBG_HIP(hipMemcpyAsync(keys, src_k, plan.n * sizeof(K), hipMemcpyDeviceToDevice, st));
         CDBG_HIP(hipMemcpyAsync(vals, src_v, plan.n * sizeof(V), hipMemcpyDeviceToDevice, st));
     }
