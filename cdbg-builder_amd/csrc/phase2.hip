@@ -43,6 +43,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 
 #include "job.h"
@@ -254,12 +255,17 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
 // THREADS threads and capacity CAP takes the listed cells (or all cells)
 // with more than `lo` items; cells above CAP go to the next class's list.
 // ---------------------------------------------------------------------------
-template <class K, class V, uint32_t CAP, int THREADS>
+// PACK (64-bit keys whose unsorted bits and a 13-bit position fit one word):
+// each item is the word (key bits below end_bit) << 13 | position, so a pass
+// moves 8 bytes per item; the bits above end_bit are the cell's own.
+constexpr int PACK_IB = 13;
+template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 inline size_t cell_sort_lds() {
-    return size_t(CAP) * (sizeof(K) + sizeof(uint16_t)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
+    return size_t(CAP) * (PACK ? sizeof(uint64_t) : sizeof(K) + sizeof(uint16_t)) +
+           ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
 }
 
-template <class K, class V, uint32_t CAP, int THREADS>
+template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* __restrict__ vals,
                                                        const uint32_t* __restrict__ cell_start,
                                                        const uint32_t* __restrict__ list, uint32_t n_items,
@@ -270,11 +276,13 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
     extern __shared__ uint64_t sm64[];
     static_assert(CAP <= 65536 && CAP % 2 == 0, "item indices are 16-bit");
+    static_assert(!PACK || (std::is_same<K, uint64_t>::value && CAP <= (1u << PACK_IB)), "packed items");
     // keys and their positions in the cell move through LDS; each value is
     // fetched once at the end from its original position
-    K* s_key = (K*)sm64;
+    using E = typename std::conditional<PACK, uint64_t, K>::type;  // what moves per item
+    E* s_key = (E*)sm64;
     uint16_t* s_idx = (uint16_t*)(s_key + CAP);
-    uint32_t* wcnt = (uint32_t*)(s_idx + CAP);  // [WAVES][256]
+    uint32_t* wcnt = (uint32_t*)(PACK ? (void*)(s_key + CAP) : (void*)(s_idx + CAP));  // [WAVES][256]
     uint32_t* s_tmp = wcnt + WAVES * 256;
     if (blockIdx.x >= n_items) return;
     const uint32_t cell = list ? list[blockIdx.x] : blockIdx.x;
@@ -289,17 +297,26 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const uint32_t rounds = (count + THREADS - 1) / THREADS;
     const uint32_t wbase = uint32_t(wave) * rounds * 64;  // wave w owns items [wbase, wbase + 64 rounds)
-    K key[RMAX];
+    E key[RMAX];
     uint32_t idx[RMAX];
     uint32_t dr[RMAX];
+    const uint64_t low = PACK ? (uint64_t(1) << end_bit) - 1 : 0;  // end_bit < 64 when packed
+    uint64_t hi = 0;  // packed: the cell's bits above end_bit
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (uint32_t(r) < rounds && i < count) {
-            key[r] = keys[start + i];
-            idx[r] = i;
+            if constexpr (PACK) {
+                const uint64_t kk = keys[start + i];
+                hi = kk & ~low;
+                key[r] = ((kk & low) << PACK_IB) | i;
+            } else {
+                key[r] = keys[start + i];
+                idx[r] = i;
+            }
         }
     }
+    const int sh0 = PACK ? PACK_IB : 0;  // digits of the packed word start above the position
     uint32_t* my = wcnt + wave * 256;
     for (int sh = 0; sh < end_bit; sh += 8) {
         const int bits = (end_bit - sh) < 8 ? (end_bit - sh) : 8;
@@ -311,7 +328,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
             if (uint32_t(r) >= rounds) break;
             const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
             const bool active = i < count;
-            const uint32_t dig = active ? digit_of(key[r], sh, mask) : 0u;
+            const uint32_t dig = active ? digit_of(key[r], sh + sh0, mask) : 0u;
             uint64_t peers = __ballot(active);
             for (int b = 0; b < bits; ++b) {
                 const bool bit = (dig >> b) & 1u;
@@ -356,7 +373,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
             if (i < count) {
                 const uint32_t pos = my[dr[r] >> 16] + (dr[r] & 0xFFFFu);
                 s_key[pos] = key[r];
-                s_idx[pos] = uint16_t(idx[r]);
+                if constexpr (!PACK) s_idx[pos] = uint16_t(idx[r]);
             }
         }
         __syncthreads();
@@ -366,7 +383,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
             const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
             if (i < count) {
                 key[r] = s_key[i];
-                idx[r] = s_idx[i];
+                if constexpr (!PACK) idx[r] = s_idx[i];
             }
         }
     }
@@ -374,7 +391,10 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
 #pragma unroll
     for (int r = 0; r < RMAX; ++r) {
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
-        if (uint32_t(r) < rounds && i < count) val[r] = vals[start + idx[r]];
+        if (uint32_t(r) < rounds && i < count) {
+            if constexpr (PACK) idx[r] = uint32_t(key[r]) & ((1u << PACK_IB) - 1);
+            val[r] = vals[start + idx[r]];
+        }
     }
     __syncthreads();  // every value is read before any is overwritten
 #pragma unroll
@@ -382,7 +402,10 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
         if (uint32_t(r) >= rounds) break;
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (i < count) {
-            keys[start + i] = key[r];
+            if constexpr (PACK)
+                keys[start + i] = hi | (key[r] >> PACK_IB);
+            else
+                keys[start + i] = key[r];
             vals[start + i] = val[r];
         }
     }
@@ -390,17 +413,23 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     // before it (the last pass left the sorted keys in LDS; a cell is one
     // bucket, so a bucket's first tuple is a head)
     const bool sorted_in_lds = end_bit > 0;
-    for (uint32_t i = threadIdx.x; i < count; i += THREADS)
-        heads[start + i] = (i == 0 || (sorted_in_lds && !KOps<K>::eq(s_key[i], s_key[i - 1]))) ? 1u : 0u;
+    for (uint32_t i = threadIdx.x; i < count; i += THREADS) {
+        bool differs;
+        if constexpr (PACK)
+            differs = (s_key[i] >> PACK_IB) != (s_key[i - (i > 0)] >> PACK_IB);
+        else
+            differs = !KOps<K>::eq(s_key[i], s_key[i - (i > 0)]);
+        heads[start + i] = (i == 0 || (sorted_in_lds && differs)) ? 1u : 0u;
+    }
 }
 
 // Lets a k_cell_sort class use its LDS (once per process).
-template <class K, class V, uint32_t CAP, int THREADS>
+template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 inline void cell_sort_prepare() {
     static const bool done = [] {
-        CDBG_HIP(hipFuncSetAttribute((const void*)&k_cell_sort<K, V, CAP, THREADS>,
+        CDBG_HIP(hipFuncSetAttribute((const void*)&k_cell_sort<K, V, CAP, THREADS, PACK>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     int(cell_sort_lds<K, V, CAP, THREADS>())));
+                                     int(cell_sort_lds<K, V, CAP, THREADS, PACK>())));
         return true;
     }();
     (void)done;
@@ -415,6 +444,25 @@ template <>
 struct CellClasses<Key128> {
     static constexpr uint32_t TINY = 512, SMALL = 1024, LARGE = 4096;
 };
+
+// One size class of the cell sort over `grid` cells; packed items when the
+// key is 64-bit and its unsorted bits leave room for the position.
+template <class K, uint32_t CAP, int THREADS>
+static void launch_cell_sort(uint32_t grid, hipStream_t st, K* keys, uint64_t* vals, const uint32_t* cell_start,
+                             const uint32_t* list, uint32_t n_items, uint32_t lo, int end_bit, uint32_t* next,
+                             unsigned long long* n_next, uint32_t* heads) {
+    if constexpr (std::is_same<K, uint64_t>::value) {
+        if (end_bit + PACK_IB <= 64) {
+            cell_sort_prepare<K, uint64_t, CAP, THREADS, true>();
+            k_cell_sort<K, uint64_t, CAP, THREADS, true><<<grid, THREADS, cell_sort_lds<K, uint64_t, CAP, THREADS, true>(), st>>>(
+                keys, vals, cell_start, list, n_items, lo, end_bit, next, n_next, heads);
+            return;
+        }
+    }
+    cell_sort_prepare<K, uint64_t, CAP, THREADS>();
+    k_cell_sort<K, uint64_t, CAP, THREADS><<<grid, THREADS, cell_sort_lds<K, uint64_t, CAP, THREADS>(), st>>>(
+        keys, vals, cell_start, list, n_items, lo, end_bit, next, n_next, heads);
+}
 
 // flag[c] = cell c holds more than cap keys
 __global__ void k_cell_over(uint32_t n_cells, const uint32_t* __restrict__ cell_start, uint32_t cap,
@@ -1943,10 +1991,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         uint32_t* over = P.cell_over.ensure<uint32_t>(n_cells);
         auto* novf = P.cell_novf.ensure<unsigned long long>(3);
         { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 3, 0ull); }
-        cell_sort_prepare<K, uint64_t, C0, 128>();
-        cell_sort_prepare<K, uint64_t, C1, 256>();
-        cell_sort_prepare<K, uint64_t, C2, 512>();
-        { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C0, 128><<<n_cells, 128, cell_sort_lds<K, uint64_t, C0, 128>(), st>>>(keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf, heads); }
+        { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C0, 128>(n_cells, st, keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf, heads); }
         // cells above the tiny class, listed by a scan (many: no shared counter)
         uint32_t* oflag = P.cell_oflag.ensure<uint32_t>(n_cells);
         uint32_t* oscan = P.cell_oscan.ensure<uint32_t>(uint64_t(n_cells) + 1);
@@ -1954,10 +1999,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         exclusive_scan<uint32_t, uint32_t>(oflag, oscan, n_cells, oscan + n_cells, J.rs.scan_tmp, st);
         const uint64_t n_mid = d2h<uint32_t>(oscan + n_cells, st);
         if (n_mid) { KSCOPE("k_compact", st); k_compact<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, oflag, oscan, mid); }
-        if (n_mid) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C1, 256><<<uint32_t(n_mid), 256, cell_sort_lds<K, uint64_t, C1, 256>(), st>>>(keys, vals, cell_start, mid, uint32_t(n_mid), C0, kb - D, mid2, novf + 1, heads); }
+        if (n_mid) { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C1, 256>(uint32_t(n_mid), st, keys, vals, cell_start, mid, uint32_t(n_mid), C0, kb - D, mid2, novf + 1, heads); }
         const uint64_t n_mid2 = n_mid ? d2h<unsigned long long>(novf + 1, st) : 0;
         if (n_mid2 > n_mid) throw Error("phase 2: cell sort listed more cells than exist");
-        if (n_mid2) { KSCOPE("k_cell_sort", st); k_cell_sort<K, uint64_t, C2, 512><<<uint32_t(n_mid2), 512, cell_sort_lds<K, uint64_t, C2, 512>(), st>>>(keys, vals, cell_start, mid2, uint32_t(n_mid2), C1, kb - D, over, novf + 2, heads); }
+        if (n_mid2) { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C2, 512>(uint32_t(n_mid2), st, keys, vals, cell_start, mid2, uint32_t(n_mid2), C1, kb - D, over, novf + 2, heads); }
         const uint64_t n_over = n_mid2 ? d2h<unsigned long long>(novf + 2, st) : 0;
         if (n_over > n_mid2) throw Error("phase 2: cell sort listed more cells than exist");
         if (n_over) {
