@@ -44,6 +44,7 @@
 
 #include <algorithm>
 #include <type_traits>
+#include <cstdlib>
 #include <cstring>
 
 #include "job.h"
@@ -1971,7 +1972,12 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     // rest of each cell in LDS
     const int kb = 2 * int(k);
     int D = 0;
-    while (D < 20 && double(nt) / nb / double(1ull << D) > 1400.0) ++D;
+    static const double cell_target = [] {  // mean keys per cell (CDBG_CELL_TARGET overrides)
+        const char* e = std::getenv("CDBG_CELL_TARGET");
+        const double v = e ? std::atof(e) : 0.0;
+        return v >= 16.0 ? v : 1400.0;
+    }();
+    while (D < 20 && double(nt) / nb / double(1ull << D) > cell_target) ++D;
     D = std::min(D, kb);
     SortPlan plan;
     plan.build(lb);
