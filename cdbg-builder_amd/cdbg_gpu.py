@@ -46,7 +46,8 @@ class SynthParams(C.Structure):
 
 
 class ServerParams(C.Structure):
-    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("keep_results", C.c_uint32)]
+    _fields_ = [("hbm_budget", C.c_uint64), ("slots", C.c_uint32), ("keep_results", C.c_uint32),
+                ("borrow_inputs", C.c_uint32)]
 
 
 class JobStatus(C.Structure):
@@ -72,6 +73,8 @@ _SIGS = {
     "cdbg_gpu_job_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
     "cdbg_gpu_job_kernel_times": (C.c_int64, [C.c_void_p, C.c_char_p, C.c_uint64]),
     "cdbg_gpu_device_bytes": (C.c_uint64, []),
+    "cdbg_gpu_alloc_stats": (None, [U64P]),
+    "cdbg_gpu_inject_oom": (None, [C.c_int64]),
     "cdbg_gpu_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "cdbg_gpu_job_set_chunk_tuples": (C.c_int, [C.c_void_p, C.c_uint64]),
     "cdbg_gpu_job_num_chunks": (C.c_int64, [C.c_void_p]),
@@ -86,7 +89,7 @@ _SIGS = {
                                                  C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
     "cdbg_gpu_job_write_phase2_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p,
                                                   C.POINTER(ColorState)]),
-    "cdbg_gpu_hbm_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint64]),
+    "cdbg_gpu_hbm_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64]),
     "cdbg_gpu_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerParams), C.POINTER(C.c_void_p)]),
     "cdbg_gpu_server_submit": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, U64P]),
@@ -222,8 +225,8 @@ class Context:
                                                     threads, C.byref(j.h)))
         return j
 
-    def server(self, slots=4, hbm_budget=0, keep_results=True):
-        return Server(self, slots, hbm_budget, keep_results)
+    def server(self, slots=4, hbm_budget=0, keep_results=True, borrow_inputs=True):
+        return Server(self, slots, hbm_budget, keep_results, borrow_inputs)
 
     def bucket_write(self, files, k, m, num_buckets, tmp_dir, threads=8):
         unc = np.zeros(num_buckets, np.uint64)
@@ -334,22 +337,36 @@ class Job:
         return words, offs
 
 
-def hbm_estimate(bases, k=31, chunk_occurrences=0):
-    return lib().cdbg_gpu_hbm_estimate(bases, k, chunk_occurrences)
+def alloc_stats():
+    """Device block cache counters (cdbg_gpu_alloc_stats)."""
+    v = np.zeros(8, np.uint64)
+    lib().cdbg_gpu_alloc_stats(_ptr(v, U64P))
+    keys = ("cached_bytes", "hits", "mallocs", "malloc_bytes", "malloc_ns", "trims", "trim_bytes", "trim_ns")
+    return dict(zip(keys, (int(x) for x in v)))
+
+
+def hbm_estimate(bases, k=31, m=21, chunk_occurrences=0):
+    return lib().cdbg_gpu_hbm_estimate(bases, k, m, chunk_occurrences)
 
 
 class Server:
-    """Served mode (include/cdbg_gpu.h): submit jobs, poll, fetch results."""
+    """Served mode (include/cdbg_gpu.h): submit jobs, poll, fetch results.
 
-    def __init__(self, ctx, slots=4, hbm_budget=0, keep_results=True):
-        p = ServerParams(hbm_budget, slots, 1 if keep_results else 0)
+    With borrow_inputs (the default) the server reads each request's sequence
+    array in place; this object keeps a reference to it until the job is done
+    (or released, or the server closed), so the caller need not."""
+
+    def __init__(self, ctx, slots=4, hbm_budget=0, keep_results=True, borrow_inputs=True):
+        p = ServerParams(hbm_budget, slots, 1 if keep_results else 0, 1 if borrow_inputs else 0)
         self.h = C.c_void_p()
+        self._held = {}  # job id -> sequence array the server reads in place
         _check(lib().cdbg_gpu_server_create(ctx.h, C.byref(p), C.byref(self.h)))
 
     def close(self):
         if self.h:
-            lib().cdbg_gpu_server_destroy(self.h)
+            lib().cdbg_gpu_server_destroy(self.h)  # joins the workers: no job reads its input after this
             self.h = C.c_void_p()
+        self._held.clear()
 
     def __enter__(self):
         return self
@@ -363,11 +380,15 @@ class Server:
         jid = C.c_uint64()
         _check(lib().cdbg_gpu_server_submit(self.h, _ptr(data, U8P), _ptr(off, U64P), len(off) - 1, k, m,
                                             num_buckets, C.byref(jid)))
+        self._held[jid.value] = data
         return jid.value
 
     def poll(self, jid, wait=False):
         st = JobStatus()
-        _check(lib().cdbg_gpu_server_poll(self.h, jid, 1 if wait else 0, C.byref(st)))
+        rc = lib().cdbg_gpu_server_poll(self.h, jid, 1 if wait else 0, C.byref(st))
+        if st.state in (2, -1):  # the worker is done with the input (set before the state)
+            self._held.pop(jid, None)
+        _check(rc)
         return {"state": st.state, "hbm_estimate": st.hbm_estimate, "chunk": st.chunk_occurrences,
                 "submit_s": st.submit_s, "start_s": st.start_s, "done_s": st.done_s,
                 "stats": st.stats.as_dict()}
@@ -383,3 +404,4 @@ class Server:
 
     def release(self, jid):
         _check(lib().cdbg_gpu_server_release(self.h, jid))
+        self._held.pop(jid, None)

@@ -327,6 +327,15 @@ CDBG_API int cdbg_gpu_bucket_process(cdbg_gpu_ctx* ctx, const char* tmp_dir, uin
 
 CDBG_API uint64_t cdbg_gpu_device_bytes(void) { return g_device_bytes.load(); }
 
+CDBG_API void cdbg_gpu_inject_oom(int64_t n) { g_block_cache.inject_oom = n < 0 ? -1 : n; }
+
+CDBG_API void cdbg_gpu_alloc_stats(uint64_t out[8]) {
+    const BlockCache& c = g_block_cache;
+    const uint64_t v[8] = {c.cached.load(), c.n_hit.load(),      c.n_malloc.load(), c.malloc_bytes.load(),
+                           c.malloc_ns.load(), c.n_trim.load(), c.trim_bytes.load(), c.trim_ns.load()};
+    for (int i = 0; i < 8; ++i) out[i] = v[i];
+}
+
 CDBG_API int cdbg_gpu_copy_probe(cdbg_gpu_ctx* ctx, uint64_t bytes, int iters, double* gbps) {
     CDBG_TRY({
         if (!ctx || !gbps || iters < 1) throw Error("bad argument");

@@ -20,7 +20,9 @@ struct Error : std::runtime_error {
 #define CDBG_HIP(call)                                                                      \
     do {                                                                                    \
         hipError_t e_ = (call);                                                             \
-        if (e_ != hipSuccess)                                                               \
+        if (e_ != hipSuccess) {                                                             \
+            (void)hipGetLastError(); /* reset: a later launch check must not see it */     \
             throw ::cdbg_amd::Error(std::string(#call) + ": " + hipGetErrorString(e_) + " (" + \
                                     __FILE__ + ":" + std::to_string(__LINE__) + ")");      \
+        }                                                                                   \
     } while (0)

@@ -43,6 +43,7 @@ struct KOps<uint64_t> {
         r = (r >> 2) | (K(v ^ 3u) << (2 * (k - 1)));
     }
     __device__ static bool le(const K& a, const K& b) { return a <= b; }
+    __device__ static K select(bool c, const K& a, const K& b) { return c ? a : b; }
     __device__ static bool eq(const K& a, const K& b) { return a == b; }
     __device__ static uint32_t base(const K& x, uint32_t i, uint32_t k) {  // i-th base from the left
         return uint32_t(x >> (2 * (k - 1 - i))) & 3u;
@@ -96,6 +97,9 @@ struct KOps<Key128> {
             r.lo |= c << sh;
     }
     __device__ static bool le(const K& a, const K& b) { return a.hi < b.hi || (a.hi == b.hi && a.lo <= b.lo); }
+    __device__ static K select(bool c, const K& a, const K& b) {  // word by word: no select of addresses
+        return K{c ? a.lo : b.lo, c ? a.hi : b.hi};
+    }
     __device__ static bool eq(const K& a, const K& b) { return a.lo == b.lo && a.hi == b.hi; }
     __device__ static uint32_t base(const K& x, uint32_t i, uint32_t k) {
         const uint32_t s = 2 * (k - 1 - i);

@@ -241,7 +241,7 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
         if (!(fl[j] & 2u) && ix[j] == nk - 1) contrib |= fwd ? BOUND_RIGHT : BOUND_LEFT;
         const bool owned = !((ix[j] == 0 && !(fl[j] & 4u)) || (ix[j] == nk - 1 && !(fl[j] & 8u)));
         const uint32_t rank = a.rank0 ? uint32_t(rk[j]) + ix[j] : uint32_t(t - rk[j]);
-        keys[t - a.t0] = fwd ? f : r;
+        keys[t - a.t0] = O::select(fwd, f, r);
         vals[t - a.t0] = (uint64_t(rank) << 32) | (uint64_t(col[j]) << 3) | (uint64_t(owned) << 2) | contrib;
     }
 }
@@ -914,17 +914,20 @@ __device__ __forceinline__ uint64_t delta_len(uint64_t x) {
     return 2ull * msb64d(b + 1) + 1 + b;
 }
 
-__device__ __forceinline__ void set_list(uint64_t desc, const SetTables& T, const uint32_t*& c, uint32_t& n,
-                                         uint32_t& single) {
+// The colours of a set: a list of a hash slot, or one colour held by value
+// (a pointer to a local would put it in scratch memory).
+struct SetColors {
+    const uint32_t* list;  // null for a singleton
+    uint32_t n, single;
+    __device__ __forceinline__ uint32_t operator[](uint32_t j) const { return list ? list[j] : single; }
+};
+
+__device__ __forceinline__ SetColors set_list(uint64_t desc, const SetTables& T) {
     if (desc >> 63) {
         const uint64_t s = desc & 0xFFFFFFFFFFFFull;
-        c = slot_list(T, s);
-        n = T.mt_llen[s];
-    } else {
-        single = uint32_t(desc);
-        c = &single;
-        n = 1;
+        return SetColors{slot_list(T, s), T.mt_llen[s], 0u};
     }
+    return SetColors{nullptr, 1u, uint32_t(desc)};
 }
 
 __global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t* __restrict__ set_desc, SetTables T,
@@ -938,9 +941,8 @@ __global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t*
             T.mt_cid[d & 0xFFFFFFFFFFFFull] = uint32_t(cid_base + i);
         else
             T.single_cid[uint32_t(d)] = uint32_t(cid_base + i);
-        const uint32_t* c;
-        uint32_t n, single;
-        set_list(d, T, c, n, single);
+        const SetColors c = set_list(d, T);
+        const uint32_t n = c.n;
         const uint32_t sparse = num_colors / 4, dense = uint32_t(uint64_t(num_colors) * 3 / 4);
         uint64_t L = delta_len(n);
         if (n < sparse) {
@@ -998,9 +1000,8 @@ __global__ void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_bas
                              unsigned long long* __restrict__ words, uint64_t* __restrict__ set_off) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_new) return;
-    const uint32_t* c;
-    uint32_t n, single;
-    set_list(set_desc[i], T, c, n, single);
+    const SetColors c = set_list(set_desc[i], T);
+    const uint32_t n = c.n;
     const uint32_t sparse = num_colors / 4, dense = uint32_t(uint64_t(num_colors) * 3 / 4);
     const uint64_t at = bit_base + boff[i];
     set_off[cid_base + i] = at;

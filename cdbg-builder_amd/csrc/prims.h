@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <map>
+#include <chrono>
 #include <mutex>
 #include <vector>
 
@@ -30,9 +31,11 @@ namespace cdbg_amd {
 // pages, so buffers released on a job stream are kept: a block goes back to
 // the cache with an event recorded on its stream, and a later request takes
 // the smallest cached block of at least its size (and at most twice it) that
-// is on the same stream or whose event has completed.  When hipMalloc runs
-// out of memory the cache is emptied (device-wide wait) and it is tried
-// again.  Process-wide and thread-safe.
+// is on the same stream or whose event has completed.  When free HBM runs
+// short, a larger cached block is taken, then cached blocks are handed back
+// to the driver (device-wide wait) before hipMalloc; when hipMalloc still
+// fails the cache is emptied and it is tried again.  Process-wide and
+// thread-safe.
 // ---------------------------------------------------------------------------
 struct BlockCache {
     struct Block {
@@ -43,23 +46,51 @@ struct BlockCache {
     std::mutex mu;
     std::multimap<size_t, Block> blocks;
     std::atomic<uint64_t> cached{0};
+    // counters: requests served from the cache, hipMalloc calls and their
+    // bytes and time, trims and the bytes they handed back
+    std::atomic<uint64_t> n_hit{0}, n_malloc{0}, malloc_bytes{0}, malloc_ns{0}, n_trim{0}, trim_bytes{0}, trim_ns{0};
+    static constexpr size_t RESERVE = size_t(2) << 30;
 
-    void* get(size_t bytes, hipStream_t st, size_t& got) {
-        {
-            std::lock_guard<std::mutex> lk(mu);
-            for (auto it = blocks.lower_bound(bytes); it != blocks.end() && it->first <= 2 * bytes; ++it) {
-                const Block& b = it->second;
-                if (b.st == st || hipEventQuery(b.ev) == hipSuccess) {
-                    void* p = b.p;
-                    got = it->first;
-                    (void)hipEventDestroy(b.ev);
-                    cached -= got;
-                    blocks.erase(it);
-                    return p;
-                }
+    // the smallest usable cached block of bytes .. max_bytes, or null
+    void* take(size_t bytes, size_t max_bytes, hipStream_t st, size_t& got) {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto it = blocks.lower_bound(bytes); it != blocks.end() && it->first <= max_bytes; ++it) {
+            const Block& b = it->second;
+            if (b.st == st || hipEventQuery(b.ev) == hipSuccess) {
+                void* p = b.p;
+                got = it->first;
+                (void)hipEventDestroy(b.ev);
+                cached -= got;
+                blocks.erase(it);
+                ++n_hit;
+                return p;
             }
         }
+        return nullptr;
+    }
+
+    // fault injection for tests: the request after this many more fails
+    // as out of memory (-1: none)
+    std::atomic<int64_t> inject_oom{-1};
+
+    void* get(size_t bytes, hipStream_t st, size_t& got) {
+        if (inject_oom.load() >= 0 && inject_oom.fetch_sub(1) == 0)
+            throw Error("device allocation: out of memory (injected)");
+        if (void* p = take(bytes, 2 * bytes, st, got)) return p;
+        // Leave RESERVE bytes to the runtime: it allocates at dispatch time
+        // (a queue's scratch for kernels with a private segment, kernel
+        // arguments, signals), and a dispatch whose allocation fails aborts
+        // its queue.  Short of that, a larger cached block is taken before
+        // any is handed back: HBM freed to the driver is cleared when it is
+        // allocated again, at tens of GB/s (~400 ms for 18 GB measured on
+        // MI355X), where fresh or cached memory costs next to nothing.
+        size_t free_b = 0, total_b = 0;
+        if (cached.load() && hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b < bytes + RESERVE) {
+            if (void* p = take(bytes, ~size_t(0), st, got)) return p;
+            trim(bytes + RESERVE - free_b);
+        }
         void* p = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
         if (hipMalloc(&p, bytes) != hipSuccess) {
             (void)hipGetLastError();
             trim(2 * bytes);  // the largest blocks first, about what is asked and as much again
@@ -69,6 +100,9 @@ struct BlockCache {
                 CDBG_HIP(hipMalloc(&p, bytes));
             }
         }
+        ++n_malloc;
+        malloc_bytes += bytes;
+        malloc_ns += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count());
         got = bytes;
         return p;
     }
@@ -89,6 +123,7 @@ struct BlockCache {
     void trim(uint64_t want = ~0ull) {
         std::lock_guard<std::mutex> lk(mu);
         if (blocks.empty()) return;
+        const auto t0 = std::chrono::steady_clock::now();
         (void)hipDeviceSynchronize();
         uint64_t freed = 0;
         while (!blocks.empty() && freed < want) {
@@ -99,6 +134,9 @@ struct BlockCache {
             cached -= it->first;
             blocks.erase(it);
         }
+        ++n_trim;
+        trim_bytes += freed;
+        trim_ns += uint64_t(std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now() - t0).count());
     }
 };
 inline BlockCache g_block_cache;
@@ -134,7 +172,6 @@ struct DevBuf {
         size_t bytes = n * sizeof(T);
         if (bytes < 256) bytes = 256;
         if (bytes > cap) {
-            if (t_grow_headroom && cap) bytes = std::max(bytes, cap + cap / 2);
             release();
             if (t_alloc_stream) {
                 p = g_block_cache.get(bytes, t_alloc_stream, bytes);
@@ -396,9 +433,12 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const K* __restric
     for (uint32_t b = threadIdx.x; b < RS_WAVES * R; b += RS_THREADS) wcnt[b] = 0;
     __syncthreads();
 
-    // rank: digit and wave-local rank of every item, kept in registers
+    // rank: digit and wave-local rank of every item, kept in registers.
+    // Values wider than 8 bytes are read when they are placed (held through
+    // the ranking they would not fit the registers and spill to scratch).
+    constexpr bool LATE = sizeof(V) > 8;
     K key[RS_ITEMS];
-    V val[RS_ITEMS];
+    V val[LATE ? 1 : RS_ITEMS];
     uint32_t dr[RS_ITEMS];  // digit << 16 | wave-local rank
     uint32_t* my = wcnt + wave * R;
 #pragma unroll
@@ -408,7 +448,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const K* __restric
         uint32_t dig = 0;
         if (active) {
             key[r] = keys[d.start + i];
-            val[r] = vals[d.start + i];
+            if constexpr (!LATE) val[r] = vals[d.start + i];
             dig = digit_of(key[r], shift, mask);
         }
         uint64_t peers = __ballot(active);
@@ -460,7 +500,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(const K* __restric
         if (i < d.count) {
             const uint32_t pos = my[dr[r] >> 16] + (dr[r] & 0xFFFFu);
             s_key[pos] = key[r];
-            s_val[pos] = val[r];
+            if constexpr (LATE)
+                s_val[pos] = vals[d.start + i];
+            else
+                s_val[pos] = val[r];
         }
     }
     __syncthreads();

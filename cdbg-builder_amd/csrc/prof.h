@@ -74,10 +74,6 @@ inline thread_local int64_t t_device_bytes = 0;
 // for the whole device).  Null: plain hipMalloc / hipFree.
 inline thread_local hipStream_t t_alloc_stream = nullptr;
 
-// A growing buffer takes 1.5 times its old size at least (server slots,
-// whose buffers follow the largest job seen so far).
-inline thread_local bool t_grow_headroom = false;
-
 // Sets t_alloc_stream for a scope.
 struct AllocStream {
     hipStream_t prev;

@@ -5,7 +5,9 @@
 // uncompressed bytes, live-RSS ceiling: bucket_walker.hpp:623-671).  Here the
 // unit of admission is a whole job and the budget is HBM:
 //
-//   * submit() copies the request to the host queue and returns an id;
+//   * submit() queues the request and returns an id; the sequence bytes are
+//     copied, or only referenced when the server borrows its callers' inputs
+//     (a host copy of a large request costs about as much as building it);
 //   * `slots` worker threads each own a Job (device buffers, one HIP stream)
 //     that stays resident between requests, so a slot's next job reuses its
 //     buffers; jobs on different slots overlap on the device;
@@ -49,27 +51,34 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// HBM model of one job, from the sizes of its device buffers: phase 1 holds
-// the input (1 B/base), 2-bit bases and ACGT bitmap (3/8 B/base), the tile
-// staging of super-k-mer starts and ends (6 B/base) and the records with
-// their sort buffers (~57 B per super-k-mer, about 0.2 per base); phase 2
-// holds ~160 B per k-mer occurrence of its chunk with 64-bit k-mers (~230 B
-// with 128-bit ones: wider tuples, sort buffers and node keys) plus the
-// output streams (~6 B per occurrence).  Occurrences are at most the bases.
-constexpr double P1_PER_BASE = 1.0 + 0.375 + 6.0 + 57.0 * 0.2;
+// HBM model of one job, from the sizes of its device buffers.  Random
+// sequence has a minimizer every (k - m + 2) / 2 positions, so about
+// 2 / (k - m + 2) super-k-mers per base, and a super-k-mer's records carry
+// one k-mer more than it starts: occurrences ~ bases + super-k-mers
+// (measured 1.19 per base at k=31 m=21, 1.05 at k=63 m=27; 5% margin).
+// Phase 1 holds the input (1 B/base), 2-bit bases and ACGT bitmap (3/8
+// B/base), the tile staging of super-k-mer starts and ends (6 B/base) and
+// the records with their sort buffers (~57 B per super-k-mer); phase 2 holds
+// ~160 B per k-mer occurrence of its chunk with 64-bit k-mers (~230 B with
+// 128-bit ones: wider tuples, sort buffers and node keys) plus the output
+// streams (~6 B per occurrence).
 constexpr uint64_t FIXED_BYTES = 64ull << 20;
+double supers_per_base(uint32_t k, uint32_t m) { return 2.0 / double(k - m + 2); }
+double occ_per_base(uint32_t k, uint32_t m) { return 1.05 * (1.0 + supers_per_base(k, m)); }
+double p1_per_base(uint32_t k, uint32_t m) { return 1.0 + 0.375 + 6.0 + 57.0 * supers_per_base(k, m); }
 double p2_per_occ(uint32_t k) { return k <= 31 ? 166.0 : 236.0; }
 
 }  // namespace
 
-uint64_t hbm_estimate(uint64_t bases, uint32_t k, uint64_t chunk_occ) {
-    const uint64_t occ = chunk_occ ? std::min(bases, chunk_occ) : bases;
-    return FIXED_BYTES + uint64_t(P1_PER_BASE * double(bases)) + uint64_t(p2_per_occ(k) * double(occ));
+uint64_t hbm_estimate(uint64_t bases, uint32_t k, uint32_t m, uint64_t chunk_occ) {
+    const uint64_t all = uint64_t(occ_per_base(k, m) * double(bases));
+    const uint64_t occ = chunk_occ ? std::min(all, chunk_occ) : all;
+    return FIXED_BYTES + uint64_t(p1_per_base(k, m) * double(bases)) + uint64_t(p2_per_occ(k) * double(occ));
 }
 
 // phase-2 chunk (k-mer occurrences) that keeps a job's estimate within cap
-uint64_t chunk_for_cap(uint64_t bases, uint32_t k, uint64_t cap) {
-    const double p1 = double(FIXED_BYTES) + P1_PER_BASE * double(bases);
+uint64_t chunk_for_cap(uint64_t bases, uint32_t k, uint32_t m, uint64_t cap) {
+    const double p1 = double(FIXED_BYTES) + p1_per_base(k, m) * double(bases);
     const double left = double(cap) - p1;
     const uint64_t occ = left > 0 ? uint64_t(left / p2_per_occ(k)) : 0;
     return std::max<uint64_t>(occ, 1u << 20);
@@ -77,7 +86,8 @@ uint64_t chunk_for_cap(uint64_t bases, uint32_t k, uint64_t cap) {
 
 struct SJob {
     uint64_t id = 0;
-    std::vector<uint8_t> bytes;
+    std::vector<uint8_t> bytes;   // owned copy of the sequences (empty when borrowed)
+    const uint8_t* in = nullptr;  // the sequences: bytes.data() or the caller's buffer
     std::vector<uint64_t> off;
     uint32_t nc = 0, k = 0, m = 0, B = 0;
     uint64_t est = 0, chunk = 0;
@@ -90,10 +100,11 @@ struct SJob {
 };
 
 struct Slot {
-    std::unique_ptr<Job> job;
-    int64_t held = 0;      // device bytes the slot's thread holds
-    uint64_t reserve = 0;  // while running: max(held, estimate)
+    std::unique_ptr<Job> job;  // touched by the slot's thread only
+    int64_t held = 0;          // device bytes the slot's thread holds
+    uint64_t reserve = 0;      // while running: max(held, estimate)
     bool busy = false;
+    bool resident = false;     // the slot keeps a job's buffers (mirror of `job`, under the lock)
 };
 
 struct Server {
@@ -101,6 +112,7 @@ struct Server {
     uint64_t budget = 0;
     uint32_t n_slots = 1;
     bool keep_results = true;
+    bool borrow = false;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<SJob*> queue;
@@ -111,6 +123,7 @@ struct Server {
     uint64_t result_bytes = 0;     // device bytes of finished jobs' results not yet released
     uint64_t next_id = 1;
     bool stop = false;
+    int excl = -1;  // slot that reruns a job alone after running out of memory
 
     uint64_t share() const { return budget / n_slots; }
 
@@ -126,17 +139,51 @@ struct Server {
         return total + mine <= budget || total == 0;
     }
 
+    // Waits until this slot may run alone: no other slot runs a job or
+    // keeps buffers, and no new job starts until it is done.
+    void run_alone(uint32_t me) {
+        (void)hipGetLastError();  // the failed call's error is not this attempt's
+        slots[me].job.reset();    // this slot's buffers back to the cache
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            slots[me].held = t_device_bytes;
+            slots[me].busy = false;  // a slot waiting here holds nothing (two may wait at once)
+            slots[me].resident = false;
+            cv_work.notify_all();
+            cv_work.wait(lk, [&] { return stop || excl < 0; });
+            if (stop) throw Error("server stopped");
+            excl = int(me);
+            cv_work.notify_all();  // idle slots hand their buffers back
+            cv_work.wait(lk, [&] {
+                if (stop) return true;
+                for (uint32_t s = 0; s < n_slots; ++s)
+                    if (s != me && (slots[s].busy || slots[s].resident)) return false;
+                return true;
+            });
+            if (stop) {
+                excl = -1;
+                throw Error("server stopped");
+            }
+            slots[me].busy = true;
+            slots[me].resident = true;
+        }
+        g_block_cache.trim();
+    }
+
     void worker(uint32_t me) {
         (void)hipSetDevice(ctx->device);
-        t_grow_headroom = true;  // the slot's buffers follow the largest job seen
         for (;;) {
             SJob* j = nullptr;
             bool release = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+                cv_work.wait(lk, [&] {
+                    return stop || (excl < 0 && !queue.empty()) || (excl >= 0 && slots[me].resident);
+                });
                 if (stop) break;
-                if (!admissible(me, *queue.front())) {
+                if (excl >= 0) {
+                    release = true;  // another slot runs alone: hand its buffers back
+                } else if (!admissible(me, *queue.front())) {
                     // this idle slot's resident buffers may be what keeps the head out
                     if (slots[me].job) {
                         release = true;
@@ -150,6 +197,7 @@ struct Server {
                     j->state = 1;
                     j->t_start = now_s();
                     slots[me].busy = true;
+                    slots[me].resident = true;
                     slots[me].reserve = std::max<uint64_t>(uint64_t(std::max<int64_t>(slots[me].held, 0)), j->est);
                 }
             }
@@ -158,6 +206,7 @@ struct Server {
                 {
                     std::lock_guard<std::mutex> lk(mu);
                     slots[me].held = t_device_bytes;
+                    slots[me].resident = false;
                 }
                 cv_work.notify_all();
                 continue;
@@ -168,13 +217,15 @@ struct Server {
                 err.clear();
                 try {
                     // cached blocks are reused by this job or handed back
-                    // when an allocation runs short; a kernel launch that
-                    // runs short fails the attempt, and the second one
-                    // starts from an empty cache
-                    if (attempt > 0) g_block_cache.trim();
+                    // when an allocation runs short.  A job that still runs
+                    // out of memory (the estimate is a model; cached blocks
+                    // may be up to twice what they serve) runs again alone:
+                    // the other slots finish their jobs and hand their
+                    // buffers back, and the cache is emptied.
+                    if (attempt > 0) run_alone(me);
                     if (!slots[me].job) slots[me].job = std::make_unique<Job>();
                     Job& J = *slots[me].job;
-                    job_set_input(J, ctx, j->bytes.data(), j->off.data(), j->nc, j->k, j->m, j->B);
+                    job_set_input(J, ctx, j->in, j->off.data(), j->nc, j->k, j->m, j->B);
                     J.chunk_tuples = j->chunk;
                     run_phase1(J);
                     run_phase2(J);
@@ -201,12 +252,15 @@ struct Server {
                 std::lock_guard<std::mutex> lk(mu);
                 slots[me].held = t_device_bytes;
                 slots[me].busy = false;
+                slots[me].resident = slots[me].job != nullptr;
                 slots[me].reserve = 0;
+                if (excl == int(me)) excl = -1;
                 j->t_done = now_s();
                 j->state = err.empty() ? 2 : -1;
                 j->error = err;
                 result_bytes += j->res_bytes;
                 j->bytes = std::vector<uint8_t>();  // the input is no longer needed
+                j->in = nullptr;
                 j->off = std::vector<uint64_t>();
             }
             cv_done.notify_all();
@@ -236,8 +290,9 @@ struct cdbg_gpu_server {
 
 extern "C" {
 
-CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint64_t chunk_occurrences) {
-    return hbm_estimate(bases, k, chunk_occurrences);
+CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint32_t m, uint64_t chunk_occurrences) {
+    if (k < 2 || m < 2 || m >= k) return 0;
+    return hbm_estimate(bases, k, m, chunk_occurrences);
 }
 
 CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params, cdbg_gpu_server** out) {
@@ -258,6 +313,7 @@ CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_par
         S.budget = budget;
         S.n_slots = params->slots;
         S.keep_results = params->keep_results != 0;
+        S.borrow = params->borrow_inputs != 0;
         S.slots.resize(S.n_slots);
         S.res_st = acquire_stream(ctx->device);
         for (uint32_t s = 0; s < S.n_slots; ++s) S.threads.emplace_back([&S, s] { S.worker(s); });
@@ -274,18 +330,25 @@ CDBG_API int cdbg_gpu_server_submit(cdbg_gpu_server* srv, const uint8_t* bytes, 
         if (m < 2 || m >= k) throw Error("m must satisfy 2 <= m < k");
         if (num_buckets == 0) throw Error("num_buckets must be >= 1");
         if (num_colors == 0) throw Error("no colours");
+        for (uint32_t c = 0; c < num_colors; ++c)
+            if (color_off[c + 1] < color_off[c]) throw Error("colour offsets must be non-decreasing");
         Server& S = srv->S;
         auto j = std::make_unique<SJob>();
         const uint64_t n = color_off[num_colors] - color_off[0];
-        j->bytes.assign(bytes + color_off[0], bytes + color_off[0] + n);
+        if (S.borrow) {
+            j->in = bytes + color_off[0];
+        } else {
+            j->bytes.assign(bytes + color_off[0], bytes + color_off[0] + n);
+            j->in = j->bytes.data();
+        }
         j->off.resize(num_colors + 1);
         for (uint32_t c = 0; c <= num_colors; ++c) j->off[c] = color_off[c] - color_off[0];
         j->nc = num_colors;
         j->k = k;
         j->m = m;
         j->B = num_buckets;
-        j->chunk = chunk_for_cap(n, k, S.share());
-        j->est = hbm_estimate(n, k, j->chunk);
+        j->chunk = chunk_for_cap(n, k, m, S.share());
+        j->est = hbm_estimate(n, k, m, j->chunk);
         j->t_submit = now_s();
         {
             std::lock_guard<std::mutex> lk(S.mu);

@@ -106,6 +106,17 @@ CDBG_API int64_t cdbg_gpu_job_kernel_times(cdbg_gpu_job* job, char* buf, uint64_
  * util.hpp:113-215). */
 CDBG_API uint64_t cdbg_gpu_device_bytes(void);
 
+/* Counters of the process-wide device block cache (released job buffers are
+ * kept for reuse instead of freed): out[0] bytes cached now, [1] requests
+ * served from the cache, [2] hipMalloc calls, [3] their bytes, [4] their
+ * nanoseconds, [5] trims (cached blocks handed back to the driver), [6]
+ * bytes trimmed, [7] nanoseconds in trims.  No counterpart in the reference
+ * (host RAM there is plain malloc). */
+CDBG_API void cdbg_gpu_alloc_stats(uint64_t out[8]);
+/* Testing aid: the device allocation after `n` more from the block cache
+ * fails as out of memory (n < 0: none).  No counterpart in the reference. */
+CDBG_API void cdbg_gpu_inject_oom(int64_t n);
+
 /* Device copy of `bytes` bytes repeated `iters` times; *gbps = read+write
  * GB/s.  Measurement aid (achievable HBM bandwidth, counter calibration); no
  * reference counterpart. */
@@ -179,6 +190,8 @@ typedef struct {
     uint32_t slots;        /* jobs on the device at once, 1..16 */
     uint32_t keep_results; /* 1: keep each finished job's artifacts in device memory (counted against
                               hbm_budget) until cdbg_gpu_server_release */
+    uint32_t borrow_inputs; /* 1: submit keeps the caller's sequence bytes instead of copying them; they
+                               must stay valid and unchanged until poll reports the job done or failed */
 } cdbg_gpu_server_params;
 typedef struct {
     int32_t state;              /* 0 queued, 1 running, 2 done, -1 failed */
@@ -188,12 +201,13 @@ typedef struct {
     cdbg_gpu_stats stats;
 } cdbg_gpu_job_status;
 /* HBM model: device bytes of a job over `bases` input bytes with k-mer
- * length k whose phase 2 runs in chunks of `chunk_occurrences` k-mer
- * occurrences (0: one chunk). */
-CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint64_t chunk_occurrences);
+ * length k and minimizer length m whose phase 2 runs in chunks of
+ * `chunk_occurrences` k-mer occurrences (0: one chunk); 0 for invalid k, m. */
+CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint32_t m, uint64_t chunk_occurrences);
 CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params,
                                     cdbg_gpu_server** out);
-/* Copies the request (layout of cdbg_gpu_job_create) and queues it. */
+/* Queues a request (layout of cdbg_gpu_job_create).  The colour offsets are
+ * copied; the sequence bytes are copied too unless the server borrows inputs. */
 CDBG_API int cdbg_gpu_server_submit(cdbg_gpu_server* srv, const uint8_t* bytes, const uint64_t* color_off,
                                     uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets,
                                     uint64_t* job_id);

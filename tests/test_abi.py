@@ -44,3 +44,38 @@ def test_library_carries_gfx950_code_only():
     data = open(G.LIB_PATH, "rb").read()
     targets = set(_re.findall(rb"amdhsa--(gfx[0-9a-z]+)", data))
     assert targets == {b"gfx950"}, targets
+
+
+def test_no_kernel_needs_scratch_memory():
+    """No kernel of the library has a private segment.  The runtime
+    allocates a queue's scratch at dispatch time, the first time a kernel
+    needs it there; with device memory taken by job buffers that allocation
+    fails and aborts the queue, so every kernel keeps its locals in
+    registers and LDS."""
+    import subprocess
+    import tempfile
+    llvm = "/opt/rocm/llvm/bin"
+    tools = [os.path.join(llvm, t) for t in ("llvm-objcopy", "clang-offload-bundler", "llvm-readelf")]
+    if not all(os.path.exists(t) for t in tools):
+        import pytest
+        pytest.skip("ROCm LLVM tools not present")
+    objcopy, bundler, readelf = tools
+    with tempfile.TemporaryDirectory() as td:
+        fat = os.path.join(td, "fatbin")
+        subprocess.run([objcopy, "--dump-section=.hip_fatbin=" + fat, G.LIB_PATH], check=True)
+        blob = open(fat, "rb").read()
+        magic = b"__CLANG_OFFLOAD_BUNDLE__"
+        starts = [m.start() for m in re.finditer(re.escape(magic), blob)]
+        assert starts, "no offload bundle in the library"
+        sizes = []
+        for i, s in enumerate(starts):
+            part = os.path.join(td, f"b{i}")
+            with open(part, "wb") as f:
+                f.write(blob[s:starts[i + 1] if i + 1 < len(starts) else len(blob)])
+            elf = part + ".elf"
+            subprocess.run([bundler, "--unbundle", "--type=o", "--input=" + part,
+                            "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", "--output=" + elf], check=True)
+            notes = subprocess.run([readelf, "--notes", elf], check=True, capture_output=True, text=True).stdout
+            sizes += [int(x) for x in re.findall(r"\.private_segment_fixed_size:\s*(\d+)", notes)]
+    assert len(sizes) > 50  # every kernel was seen
+    assert max(sizes) == 0, sorted(sizes)[-5:]

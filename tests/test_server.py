@@ -1,8 +1,8 @@
 """Served mode (cdbg_gpu_server_*): jobs of different sizes and k submitted
 together run on several slots at once, and each job's artifacts are the same
 bytes as those of the same job run alone on the plain path
-(cdbg_gpu_job_run); a budget that admits one job at a time gives the same
-results.  The HBM model is checked on the CPU: monotone in the input and
+(cdbg_gpu_job_run), whether the server reads the inputs in place or copies
+them; a budget that admits one job at a time gives the same results.  The HBM model is checked on the CPU: monotone in the input and
 bounded by chunking."""
 import numpy as np
 import pytest
@@ -13,11 +13,14 @@ from conftest import cdbg_gpu as G
 def test_hbm_estimate_is_monotone_and_chunk_bounded():
     e = [G.hbm_estimate(n) for n in (10**6, 10**7, 10**8)]
     assert e[0] < e[1] < e[2]
-    assert G.hbm_estimate(10**8, 31, 10**6) < G.hbm_estimate(10**8)
+    assert G.hbm_estimate(10**8, 31, 21, 10**6) < G.hbm_estimate(10**8)
     # phase 1 is not chunked: the estimate keeps growing with the input
-    assert G.hbm_estimate(10**9, 31, 10**6) > G.hbm_estimate(10**8, 31, 10**6)
+    assert G.hbm_estimate(10**9, 31, 21, 10**6) > G.hbm_estimate(10**8, 31, 21, 10**6)
     # 128-bit k-mers need more than 64-bit ones
-    assert G.hbm_estimate(10**8, 63) > G.hbm_estimate(10**8, 31)
+    assert G.hbm_estimate(10**8, 63, 27) > G.hbm_estimate(10**8, 31, 21)
+    # minimizers closer to k: shorter windows, more super-k-mers and occurrences
+    assert G.hbm_estimate(10**8, 31, 27) > G.hbm_estimate(10**8, 31, 21) > G.hbm_estimate(10**8, 31, 11)
+    assert G.hbm_estimate(10**8, 31, 31) == 0
 
 
 JOBS = [  # (seed, colours, genome, k, m, buckets)
@@ -47,14 +50,15 @@ def _alone(ctx, data, off, k, m, B):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("slots,budget", [(3, 0), (2, 1)])
-def test_served_results_equal_alone(gpu_ctx, slots, budget):
+@pytest.mark.parametrize("slots,budget,borrow", [(3, 0, True), (2, 1, False)])
+def test_served_results_equal_alone(gpu_ctx, slots, budget, borrow):
     inputs = _inputs()
     want = [_alone(gpu_ctx, *x) for x in inputs]
     if budget == 1:  # room for the largest job only: admission runs them one at a time
-        budget = max(G.hbm_estimate(len(x[0]), x[2]) for x in inputs) + (1 << 20)
-    with gpu_ctx.server(slots=slots, hbm_budget=budget) as srv:
+        budget = max(G.hbm_estimate(len(x[0]), x[2], x[3]) for x in inputs) + (1 << 20)
+    with gpu_ctx.server(slots=slots, hbm_budget=budget, borrow_inputs=borrow) as srv:
         ids = [srv.submit(*x) for x in inputs]
+        assert len(srv._held) == len(ids)
         for jid, (st, frags, words, offs) in zip(ids, want):
             s = srv.poll(jid, wait=True)
             assert s["state"] == 2
@@ -66,6 +70,33 @@ def test_served_results_equal_alone(gpu_ctx, slots, budget):
             assert np.array_equal(w2, words)
             assert np.array_equal(o2, offs)
             srv.release(jid)
+        assert not srv._held  # no input array outlives its job
+
+
+@pytest.mark.gpu
+def test_served_job_out_of_memory_runs_again_alone(gpu_ctx):
+    """A job whose device allocation fails (injected from the host) runs
+    again once the other slots have handed their buffers back, and its
+    results are those of the job run alone; jobs after it are unaffected."""
+    inputs = _inputs()[:4]
+    want = [_alone(gpu_ctx, *x) for x in inputs]
+    with gpu_ctx.server(slots=2) as srv:
+        G.lib().cdbg_gpu_inject_oom(40)  # inside the first jobs' phase 1 or 2
+        try:
+            ids = [srv.submit(*x) for x in inputs]
+            for jid, (st, frags, words, offs) in zip(ids, want):
+                s = srv.poll(jid, wait=True)
+                assert s["state"] == 2
+                assert s["stats"]["num_kmers"] == st["num_kmers"]
+                f2, w2, o2 = srv.result(jid)
+                assert f2.tobytes() == frags.tobytes()
+                assert np.array_equal(w2, words)
+                assert np.array_equal(o2, offs)
+                srv.release(jid)
+        finally:
+            G.lib().cdbg_gpu_inject_oom(-1)
+    a = G.alloc_stats()
+    assert a["trims"] >= 1 or a["cached_bytes"] == 0
 
 
 @pytest.mark.gpu
@@ -74,6 +105,8 @@ def test_served_rejects_bad_requests(gpu_ctx):
     with gpu_ctx.server(slots=2) as srv:
         with pytest.raises(RuntimeError, match="m must satisfy"):
             srv.submit(data, off, 31, 31, 16)
+        with pytest.raises(RuntimeError, match="non-decreasing"):
+            srv.submit(data, off[::-1].copy(), 31, 15, 16)
         with pytest.raises(RuntimeError, match="unknown job id"):
             srv.poll(12345)
         jid = srv.submit(data, off, 31, 15, 16)
