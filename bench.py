@@ -135,27 +135,40 @@ def served_leg(G, ctx, n_jobs, slots, rate, threads):
         lat_plain.append(fin - a)
     print(f"bench: plain {sum(service):.2f} s for {n_jobs} jobs", file=sys.stderr, flush=True)
 
+    def alloc_delta(a, reruns):
+        b = G.alloc_stats()
+        return {"reruns_alone": reruns, "trims": b["trims"] - a["trims"], "mallocs": b["mallocs"] - a["mallocs"],
+                "malloc_gb": round((b["malloc_bytes"] - a["malloc_bytes"]) / 1e9, 2),
+                "malloc_ms": round((b["malloc_ns"] - a["malloc_ns"]) / 1e6, 1),
+                "trim_ms": round((b["trim_ns"] - a["trim_ns"]) / 1e6, 1)}
+
     def closed_batch(srv, n_slots):
         warm = [srv.submit(*x) for x in inputs[:n_slots]]  # every slot loads its buffers once
         for jid in warm:
             srv.poll(jid, wait=True)
             srv.release(jid)
+        a = G.alloc_stats()
         t0 = time.perf_counter()
         ids = [srv.submit(*x) for x in inputs]
-        got = []
+        got, reruns = [], 0
         for jid in ids:
-            got.append(srv.poll(jid, wait=True)["stats"]["num_kmers"])
+            st = srv.poll(jid, wait=True)
+            got.append(st["stats"]["num_kmers"])
+            reruns += st["attempts"] > 1
             srv.release(jid)
         wall = time.perf_counter() - t0
         if got != kmers:
             raise RuntimeError("served results differ from the plain path")
-        print(f"bench: served, {n_slots} slot(s), all queued: {wall:.2f} s", file=sys.stderr, flush=True)
-        return sum(got) / wall
+        alloc = alloc_delta(a, reruns)
+        print(f"bench: served, {n_slots} slot(s), all queued: {wall:.2f} s, allocator {alloc}",
+              file=sys.stderr, flush=True)
+        return sum(got) / wall, alloc
 
     with ctx.server(slots=1) as srv:
-        resident_tput = closed_batch(srv, 1)
+        resident_tput, _ = closed_batch(srv, 1)
     with ctx.server(slots=slots) as srv:
-        served_tput = closed_batch(srv, slots)
+        served_tput, alloc_closed = closed_batch(srv, slots)
+        a0 = G.alloc_stats()
         t0 = time.perf_counter()
         ids = []
         for a, x in zip(arrivals, inputs):
@@ -163,17 +176,21 @@ def served_leg(G, ctx, n_jobs, slots, rate, threads):
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(*x))
-        lat_served = []
+        lat_served, reruns = [], 0
         for jid in ids:
             st = srv.poll(jid, wait=True)
             lat_served.append(st["done_s"] - st["submit_s"])
+            reruns += st["attempts"] > 1
             srv.release(jid)
+        alloc_open = alloc_delta(a0, reruns)
+        print(f"bench: served, Poisson arrivals: allocator {alloc_open}", file=sys.stderr, flush=True)
     pct = lambda v, q: round(1e3 * float(np.percentile(v, q)), 1)
     return {"jobs": n_jobs, "slots": slots, "bases": bases, "distinct_kmers": sum(kmers),
             "plain": {"value": round(plain_tput, 1), "p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99)},
             "served_1_slot": {"value": round(resident_tput, 1)},
             "served": {"value": round(served_tput, 1), "p50_ms": pct(lat_served, 50),
-                       "p99_ms": pct(lat_served, 99)},
+                       "p99_ms": pct(lat_served, 99), "allocator_all_queued": alloc_closed,
+                       "allocator_poisson": alloc_open},
             "served_over_plain": round(served_tput / plain_tput, 3),
             "arrival_rate": rate,
             "workload": f"{n_jobs} build requests: colours log-uniform [2,200], genomes U[0.5,5] Mbp, "

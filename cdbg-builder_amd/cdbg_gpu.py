@@ -51,7 +51,8 @@ class ServerParams(C.Structure):
 
 
 class JobStatus(C.Structure):
-    _fields_ = [("state", C.c_int32), ("hbm_estimate", C.c_uint64), ("chunk_occurrences", C.c_uint64),
+    _fields_ = [("state", C.c_int32), ("attempts", C.c_uint32), ("hbm_estimate", C.c_uint64),
+                ("chunk_occurrences", C.c_uint64),
                 ("submit_s", C.c_double), ("start_s", C.c_double), ("done_s", C.c_double),
                 ("stats", Stats)]
 
@@ -389,7 +390,8 @@ class Server:
         if st.state in (2, -1):  # the worker is done with the input (set before the state)
             self._held.pop(jid, None)
         _check(rc)
-        return {"state": st.state, "hbm_estimate": st.hbm_estimate, "chunk": st.chunk_occurrences,
+        return {"state": st.state, "attempts": st.attempts, "hbm_estimate": st.hbm_estimate,
+                "chunk": st.chunk_occurrences,
                 "submit_s": st.submit_s, "start_s": st.start_s, "done_s": st.done_s,
                 "stats": st.stats.as_dict()}
 

@@ -32,8 +32,10 @@ namespace cdbg_amd {
 // the cache with an event recorded on its stream, and a later request takes
 // the smallest cached block of at least its size (and at most twice it) that
 // is on the same stream or whose event has completed.  When free HBM runs
-// short, a larger cached block is taken, then cached blocks are handed back
-// to the driver (device-wide wait) before hipMalloc; when hipMalloc still
+// short, a block still in use on another stream is taken (the new stream
+// waits for the old one's event), or one up to four times the size, before
+// cached blocks are handed back to the driver (device-wide wait) and
+// hipMalloc is called; when hipMalloc still
 // fails the cache is emptied and it is tried again.  Process-wide and
 // thread-safe.
 // ---------------------------------------------------------------------------
@@ -51,15 +53,34 @@ struct BlockCache {
     std::atomic<uint64_t> n_hit{0}, n_malloc{0}, malloc_bytes{0}, malloc_ns{0}, n_trim{0}, trim_bytes{0}, trim_ns{0};
     static constexpr size_t RESERVE = size_t(2) << 30;
 
-    // the smallest usable cached block of bytes .. max_bytes, or null
-    void* take(size_t bytes, size_t max_bytes, hipStream_t st, size_t& got) {
+    std::vector<hipEvent_t> retired;  // events a stream was made to wait on; destroyed once complete
+
+    // The smallest cached block of bytes .. max_bytes that is usable on
+    // stream st now (released on st, or its event has completed), or with
+    // `wait`, also one still in use on another stream: st then waits for
+    // that stream's work up to the release (stream-ordered reuse).  Null if
+    // none.
+    void* take(size_t bytes, size_t max_bytes, hipStream_t st, size_t& got, bool wait = false) {
         std::lock_guard<std::mutex> lk(mu);
+        for (size_t i = 0; i < retired.size();) {
+            if (hipEventQuery(retired[i]) == hipSuccess) {
+                (void)hipEventDestroy(retired[i]);
+                retired[i] = retired.back();
+                retired.pop_back();
+            } else {
+                ++i;
+            }
+        }
         for (auto it = blocks.lower_bound(bytes); it != blocks.end() && it->first <= max_bytes; ++it) {
             const Block& b = it->second;
-            if (b.st == st || hipEventQuery(b.ev) == hipSuccess) {
+            const bool ready = b.st == st || hipEventQuery(b.ev) == hipSuccess;
+            if (ready || (wait && hipStreamWaitEvent(st, b.ev, 0) == hipSuccess)) {
                 void* p = b.p;
                 got = it->first;
-                (void)hipEventDestroy(b.ev);
+                if (ready)
+                    (void)hipEventDestroy(b.ev);
+                else
+                    retired.push_back(b.ev);
                 cached -= got;
                 blocks.erase(it);
                 ++n_hit;
@@ -80,13 +101,15 @@ struct BlockCache {
         // Leave RESERVE bytes to the runtime: it allocates at dispatch time
         // (a queue's scratch for kernels with a private segment, kernel
         // arguments, signals), and a dispatch whose allocation fails aborts
-        // its queue.  Short of that, a larger cached block is taken before
-        // any is handed back: HBM freed to the driver is cleared when it is
+        // its queue.  Short of that, a cached block still in use on another
+        // stream (waited for), then one up to four times the size, is taken
+        // before any is handed back: HBM freed to the driver is cleared when it is
         // allocated again, at tens of GB/s (~400 ms for 18 GB measured on
         // MI355X), where fresh or cached memory costs next to nothing.
         size_t free_b = 0, total_b = 0;
         if (cached.load() && hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b < bytes + RESERVE) {
-            if (void* p = take(bytes, ~size_t(0), st, got)) return p;
+            if (void* p = take(bytes, 2 * bytes, st, got, true)) return p;
+            if (void* p = take(bytes, 4 * bytes, st, got, true)) return p;
             trim(bytes + RESERVE - free_b);
         }
         void* p = nullptr;
@@ -125,6 +148,8 @@ struct BlockCache {
         if (blocks.empty()) return;
         const auto t0 = std::chrono::steady_clock::now();
         (void)hipDeviceSynchronize();
+        for (hipEvent_t e : retired) (void)hipEventDestroy(e);
+        retired.clear();
         uint64_t freed = 0;
         while (!blocks.empty() && freed < want) {
             auto it = std::prev(blocks.end());
@@ -161,7 +186,6 @@ struct DevBuf {
             else
                 (void)hipFree(p);
             g_device_bytes -= cap;
-            t_device_bytes -= int64_t(cap);
         }
         p = nullptr;
         cap = 0;
@@ -181,7 +205,6 @@ struct DevBuf {
             st = t_alloc_stream;
             cap = bytes;
             g_device_bytes += cap;
-            t_device_bytes += int64_t(cap);
         }
         return (T*)p;
     }

@@ -66,9 +66,6 @@ struct KScope {
 
 // Device bytes currently held by DevBuf allocations of this process.
 inline std::atomic<uint64_t> g_device_bytes{0};
-// Device bytes allocated (net) by the calling thread: a server slot's jobs
-// are only touched by that slot's thread, so this is the slot's footprint.
-inline thread_local int64_t t_device_bytes = 0;
 // Stream that the calling thread's device buffers are used on: released
 // buffers go to the block cache tagged with it (no hipFree, which would wait
 // for the whole device).  Null: plain hipMalloc / hipFree.

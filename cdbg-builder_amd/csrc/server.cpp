@@ -8,22 +8,21 @@
 //   * submit() queues the request and returns an id; the sequence bytes are
 //     copied, or only referenced when the server borrows its callers' inputs
 //     (a host copy of a large request costs about as much as building it);
-//   * `slots` worker threads each own a Job (device buffers, one HIP stream)
-//     that stays resident between requests, so a slot's next job reuses its
-//     buffers; jobs on different slots overlap on the device;
-//   * the queue head is admitted when a slot is free and the HBM the slots
-//     would hold with it (each running slot: the larger of what it holds and
-//     its job's estimate; each idle slot: what it holds) fits the budget.
-//     An idle slot whose buffers keep the head out releases them;
+//   * `slots` worker threads each run one job at a time on a HIP stream of
+//     their own, so jobs on different slots overlap on the device; a job's
+//     buffers come from the process-wide block cache and go back to it when
+//     the job ends (the cache keeps them on the device for the next job);
+//   * the queue head is admitted (FIFO) when a slot is free and its HBM
+//     estimate, the running jobs' estimates and the kept results fit the
+//     budget;
 //   * phase 2 of a job is chunked so that its estimate fits one slot's share
 //     (results do not depend on chunking);
 //   * a finished job's phase-2 artifacts are copied into device buffers of
 //     the job (they count against the budget until release()); poll()
 //     reports the state, the stats and the submit / start / done times
 //     (steady clock, seconds), result() copies the artifacts to the host.
-//   * released device buffers go to the block cache (no device-wide wait for
-//     a free); a job that runs out of device memory is run again once after
-//     the cache hands its blocks back to the driver.
+//   * a job that runs out of device memory anyway runs again once, alone
+//     (the other slots finish their jobs first) and from an empty cache.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,6 +91,7 @@ struct SJob {
     uint32_t nc = 0, k = 0, m = 0, B = 0;
     uint64_t est = 0, chunk = 0;
     int state = 0;  // 0 queued, 1 running, 2 done, -1 failed
+    uint32_t attempts = 0;
     std::string error;
     double t_submit = 0, t_start = 0, t_done = 0;
     cdbg_gpu_stats stats{};
@@ -100,11 +100,9 @@ struct SJob {
 };
 
 struct Slot {
-    std::unique_ptr<Job> job;  // touched by the slot's thread only
-    int64_t held = 0;          // device bytes the slot's thread holds
-    uint64_t reserve = 0;      // while running: max(held, estimate)
+    std::unique_ptr<Job> job;  // while running: the job's device buffers and stream (slot thread only)
+    uint64_t reserve = 0;      // while running: the job's estimate
     bool busy = false;
-    bool resident = false;     // the slot keeps a job's buffers (mirror of `job`, under the lock)
 };
 
 struct Server {
@@ -127,37 +125,31 @@ struct Server {
 
     uint64_t share() const { return budget / n_slots; }
 
-    // the queue head fits next to what the other slots hold or may grow to
+    // the queue head fits next to the running jobs' estimates and the kept results
     bool admissible(uint32_t me, const SJob& j) const {
         uint64_t total = result_bytes;
-        for (uint32_t s = 0; s < n_slots; ++s) {
-            if (s == me) continue;
-            total += slots[s].busy ? slots[s].reserve : uint64_t(std::max<int64_t>(slots[s].held, 0));
-        }
-        const uint64_t mine = std::max<uint64_t>(uint64_t(std::max<int64_t>(slots[me].held, 0)), j.est);
+        for (uint32_t s = 0; s < n_slots; ++s)
+            if (s != me && slots[s].busy) total += slots[s].reserve;
         // a job larger than the whole budget still runs once it is alone
-        return total + mine <= budget || total == 0;
+        return total + j.est <= budget || total == 0;
     }
 
-    // Waits until this slot may run alone: no other slot runs a job or
-    // keeps buffers, and no new job starts until it is done.
+    // Waits until this slot may run alone: no other slot runs a job, and no
+    // new job starts until it is done; then empties the block cache.
     void run_alone(uint32_t me) {
         (void)hipGetLastError();  // the failed call's error is not this attempt's
         slots[me].job.reset();    // this slot's buffers back to the cache
         {
             std::unique_lock<std::mutex> lk(mu);
-            slots[me].held = t_device_bytes;
             slots[me].busy = false;  // a slot waiting here holds nothing (two may wait at once)
-            slots[me].resident = false;
             cv_work.notify_all();
             cv_work.wait(lk, [&] { return stop || excl < 0; });
             if (stop) throw Error("server stopped");
             excl = int(me);
-            cv_work.notify_all();  // idle slots hand their buffers back
             cv_work.wait(lk, [&] {
                 if (stop) return true;
                 for (uint32_t s = 0; s < n_slots; ++s)
-                    if (s != me && (slots[s].busy || slots[s].resident)) return false;
+                    if (s != me && slots[s].busy) return false;
                 return true;
             });
             if (stop) {
@@ -165,7 +157,6 @@ struct Server {
                 throw Error("server stopped");
             }
             slots[me].busy = true;
-            slots[me].resident = true;
         }
         g_block_cache.trim();
     }
@@ -174,56 +165,31 @@ struct Server {
         (void)hipSetDevice(ctx->device);
         for (;;) {
             SJob* j = nullptr;
-            bool release = false;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] {
-                    return stop || (excl < 0 && !queue.empty()) || (excl >= 0 && slots[me].resident);
-                });
+                cv_work.wait(lk, [&] { return stop || (excl < 0 && !queue.empty() && admissible(me, *queue.front())); });
                 if (stop) break;
-                if (excl >= 0) {
-                    release = true;  // another slot runs alone: hand its buffers back
-                } else if (!admissible(me, *queue.front())) {
-                    // this idle slot's resident buffers may be what keeps the head out
-                    if (slots[me].job) {
-                        release = true;
-                    } else {
-                        cv_work.wait(lk);  // until a job ends or arrives
-                        continue;
-                    }
-                } else {
-                    j = queue.front();
-                    queue.pop_front();
-                    j->state = 1;
-                    j->t_start = now_s();
-                    slots[me].busy = true;
-                    slots[me].resident = true;
-                    slots[me].reserve = std::max<uint64_t>(uint64_t(std::max<int64_t>(slots[me].held, 0)), j->est);
-                }
-            }
-            if (release) {
-                slots[me].job.reset();
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    slots[me].held = t_device_bytes;
-                    slots[me].resident = false;
-                }
-                cv_work.notify_all();
-                continue;
+                j = queue.front();
+                queue.pop_front();
+                j->state = 1;
+                j->t_start = now_s();
+                slots[me].busy = true;
+                slots[me].reserve = j->est;
             }
             cv_work.notify_all();  // the next head may fit another slot
             std::string err;
             for (int attempt = 0; attempt < 2; ++attempt) {
                 err.clear();
+                j->attempts = uint32_t(attempt + 1);
                 try {
-                    // cached blocks are reused by this job or handed back
-                    // when an allocation runs short.  A job that still runs
-                    // out of memory (the estimate is a model; cached blocks
-                    // may be up to twice what they serve) runs again alone:
-                    // the other slots finish their jobs and hand their
-                    // buffers back, and the cache is emptied.
+                    // The job's buffers come from the block cache and go back
+                    // to it when the job ends, so the next job of any slot
+                    // reuses them.  A job that still runs out of memory (the
+                    // estimate is a model; cached blocks may be up to twice
+                    // what they serve) runs again alone: the other slots
+                    // finish their jobs and the cache is emptied.
                     if (attempt > 0) run_alone(me);
-                    if (!slots[me].job) slots[me].job = std::make_unique<Job>();
+                    slots[me].job = std::make_unique<Job>();
                     Job& J = *slots[me].job;
                     job_set_input(J, ctx, j->in, j->off.data(), j->nc, j->k, j->m, j->B);
                     J.chunk_tuples = j->chunk;
@@ -238,7 +204,6 @@ struct Server {
                         uint64_t* o = j->offsets.ensure<uint64_t>(J.stats.num_color_sets);
                         copy_phase2_results_device(J, f, w, o, res_st);
                         j->res_bytes = j->frags.cap + j->words.cap + j->offsets.cap;
-                        t_device_bytes -= int64_t(j->res_bytes);  // they belong to the server, not the slot
                     }
                 } catch (std::exception& e) {
                     err = e.what();
@@ -247,12 +212,10 @@ struct Server {
                 }
                 if (err.find("out of memory") == std::string::npos) break;
             }
-            if (!err.empty()) slots[me].job.reset();  // a failed job leaves no state behind
+            slots[me].job.reset();
             {
                 std::lock_guard<std::mutex> lk(mu);
-                slots[me].held = t_device_bytes;
                 slots[me].busy = false;
-                slots[me].resident = slots[me].job != nullptr;
                 slots[me].reserve = 0;
                 if (excl == int(me)) excl = -1;
                 j->t_done = now_s();
@@ -371,6 +334,7 @@ CDBG_API int cdbg_gpu_server_poll(cdbg_gpu_server* srv, uint64_t job_id, int wai
         SJob& j = *it->second;
         if (wait) S.cv_done.wait(lk, [&] { return j.state == 2 || j.state == -1; });
         out->state = j.state;
+        out->attempts = j.attempts;
         out->hbm_estimate = j.est;
         out->chunk_occurrences = j.chunk;
         out->submit_s = j.t_submit;
@@ -410,7 +374,6 @@ CDBG_API int cdbg_gpu_server_release(cdbg_gpu_server* srv, uint64_t job_id) {
         if (it == S.jobs.end()) throw Error("unknown job id " + std::to_string(job_id));
         if (it->second->state != 2 && it->second->state != -1) throw Error("job is still queued or running");
         S.result_bytes -= it->second->res_bytes;
-        t_device_bytes += int64_t(it->second->res_bytes);  // freed on this thread's count
         S.jobs.erase(it);
         S.cv_work.notify_all();  // the freed results may let the queue head in
     })
@@ -425,7 +388,6 @@ CDBG_API void cdbg_gpu_server_destroy(cdbg_gpu_server* srv) {
     }
     S.cv_work.notify_all();
     for (auto& t : S.threads) t.join();
-    for (auto& kv : S.jobs) t_device_bytes += int64_t(kv.second->res_bytes);
     S.jobs.clear();  // results free in order on res_st, which goes last
     hipStream_t rs = S.res_st;
     const int dev = S.ctx->device;

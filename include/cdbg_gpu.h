@@ -178,11 +178,13 @@ CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag
 /* Served mode: a request loop over build jobs (sequences + k).  The reference
  * runs one genome list per process; its phase-2 scheduler admits buckets
  * against a RAM budget (bucket_walker.hpp:623-671, cost 24 x uncompressed
- * bucket bytes).  Here `slots` worker threads each keep a job's device
- * buffers and HIP stream resident between requests; the queue head is
- * admitted (FIFO) when a slot is free and the HBM the slots would hold with
- * it fits hbm_budget, by the estimate of cdbg_gpu_hbm_estimate; jobs on
- * different slots overlap on the device.  A job's results are the same as
+ * bucket bytes).  Here `slots` worker threads each run one job at a time
+ * on a HIP stream of their own, so jobs on different slots overlap on the
+ * device; the queue head is admitted (FIFO) when a slot is free and its
+ * estimate (cdbg_gpu_hbm_estimate), the running jobs' estimates and the kept
+ * results fit hbm_budget.  Job buffers come from the process-wide block cache
+ * (cdbg_gpu_alloc_stats) and go back to it; a job that runs out of device
+ * memory anyway runs again once, alone.  A job's results are the same as
  * those of cdbg_gpu_job_run on the same input.  Thread-safe. */
 typedef struct cdbg_gpu_server cdbg_gpu_server;
 typedef struct {
@@ -195,6 +197,7 @@ typedef struct {
 } cdbg_gpu_server_params;
 typedef struct {
     int32_t state;              /* 0 queued, 1 running, 2 done, -1 failed */
+    uint32_t attempts;          /* runs of the job: 2 when it ran out of memory and ran again alone */
     uint64_t hbm_estimate;      /* bytes admitted for the job */
     uint64_t chunk_occurrences; /* phase-2 chunk that keeps the estimate within one slot's share */
     double submit_s, start_s, done_s; /* steady-clock seconds: submitted, admitted, artifacts complete */

@@ -63,6 +63,7 @@ def test_served_results_equal_alone(gpu_ctx, slots, budget, borrow):
             s = srv.poll(jid, wait=True)
             assert s["state"] == 2
             assert s["submit_s"] <= s["start_s"] <= s["done_s"]
+            assert s["attempts"] == 1
             for key in ("num_kmers", "num_color_sets", "num_frags", "frag_bytes", "cs_bits", "num_nodes"):
                 assert s["stats"][key] == st[key], key
             f2, w2, o2 = srv.result(jid)
@@ -84,15 +85,18 @@ def test_served_job_out_of_memory_runs_again_alone(gpu_ctx):
         G.lib().cdbg_gpu_inject_oom(40)  # inside the first jobs' phase 1 or 2
         try:
             ids = [srv.submit(*x) for x in inputs]
+            attempts = []
             for jid, (st, frags, words, offs) in zip(ids, want):
                 s = srv.poll(jid, wait=True)
                 assert s["state"] == 2
+                attempts.append(s["attempts"])
                 assert s["stats"]["num_kmers"] == st["num_kmers"]
                 f2, w2, o2 = srv.result(jid)
                 assert f2.tobytes() == frags.tobytes()
                 assert np.array_equal(w2, words)
                 assert np.array_equal(o2, offs)
                 srv.release(jid)
+            assert sorted(attempts) == [1, 1, 1, 2]  # exactly one job met the injected failure
         finally:
             G.lib().cdbg_gpu_inject_oom(-1)
     a = G.alloc_stats()
