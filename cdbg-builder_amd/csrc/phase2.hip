@@ -1754,6 +1754,23 @@ __global__ void k_frag_compact(uint64_t n, const uint32_t* __restrict__ fflag, c
     forder[d] = uint32_t(i);
 }
 
+// The same with 32-bit sort keys (local bucket << rb) | seed rank, for a chunk
+// whose buckets and ranks fit them (a rank is below its bucket's tuple count,
+// which is below 2^rb); a rank that does not fit is counted in *overflow.
+__global__ void k_frag_compact32(uint64_t n, const uint32_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
+                                 const uint64_t* __restrict__ frag_key, uint32_t b0, int rb,
+                                 uint32_t* __restrict__ fkey, uint32_t* __restrict__ forder,
+                                 unsigned long long* __restrict__ overflow) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n || !fflag[i]) return;
+    const uint32_t d = fscan[i];
+    const uint64_t key = frag_key[i];
+    const uint32_t r = uint32_t(key);
+    if (rb < 32 && (r >> rb)) atomicAdd(overflow, 1ull);
+    fkey[d] = uint32_t((uint64_t(uint32_t(key >> 32) - b0) << rb) | r);
+    forder[d] = uint32_t(i);
+}
+
 // ---------------------------------------------------------------------------
 // fragment records
 // ---------------------------------------------------------------------------
@@ -2217,16 +2234,32 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     exclusive_scan<uint32_t, uint32_t>(fflag, fscan, nn, fscan + nn, J.rs.scan_tmp, st);
     const uint64_t nf = d2h<uint32_t>(fscan + nn, st);
 
-    // --- fragments in (bucket, seed rank) order, appended to the job stream
+    // --- fragments in (bucket, seed rank) order, appended to the job stream.
+    // Keys are 32-bit when the chunk's local bucket and rank bits fit (4
+    // passes of 4+4 bytes instead of 6 of 8+4).
     uint64_t* fkey = P.fkey.ensure<uint64_t>(nf);
     uint64_t* fkey_t = P.fkey_t.ensure<uint64_t>(nf);
     uint32_t* forder = P.forder.ensure<uint32_t>(nf);
     uint32_t* forder_t = P.forder_t.ensure<uint32_t>(nf);
-    { KSCOPE("k_frag_compact", st); k_frag_compact<<<blocks_for(nn), TPB, 0, st>>>(nn, fflag, fscan, wa.frag_key, fkey, forder); }
+    uint64_t max_btup = 0;
+    for (uint32_t b = b0; b < b1; ++b) max_btup = std::max<uint64_t>(max_btup, h_btup[b + 1] - h_btup[b]);
+    const int rbits = bits_for(max_btup), lbits = bits_for(b1 - b0);
     SortPlan fplan;
     fplan.build({0, nf});
-    segmented_radix_sort<uint64_t, uint32_t>(fkey, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
-                                             "frags");
+    const char* force64 = getenv("CDBG_FRAG_KEY64");  // testing: the 64-bit key path
+    if (lbits + rbits <= 32 && !(force64 && *force64 == '1')) {
+        uint32_t* fk32 = (uint32_t*)fkey;
+        uint32_t* fk32_t = (uint32_t*)fkey_t;
+        { KSCOPE("k_frag_compact", st); k_frag_compact32<<<blocks_for(nn), TPB, 0, st>>>(nn, fflag, fscan, wa.frag_key, b0, rbits, fk32, forder, cnt + 12); }
+        bool in_temp = false;
+        segmented_radix_sort<uint32_t, uint32_t>(fk32, forder, fk32_t, forder_t, fplan, 0, lbits + rbits, 8, P.rs2, st,
+                                                 "frags", &in_temp);
+        if (in_temp) forder = forder_t;
+    } else {
+        { KSCOPE("k_frag_compact", st); k_frag_compact<<<blocks_for(nn), TPB, 0, st>>>(nn, fflag, fscan, wa.frag_key, fkey, forder); }
+        segmented_radix_sort<uint64_t, uint32_t>(fkey, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
+                                                 "frags");
+    }
     uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
     uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
     if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, pcid, k, fsize); }
@@ -2237,6 +2270,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
+    if (h_cnt[12]) throw Error("phase 2: fragment seed rank outside its bucket's tuples");
     P.frag_total += fbytes;
     P.n_frags += nf;
     P.n_nodes += nn;

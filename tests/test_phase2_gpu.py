@@ -145,3 +145,12 @@ def test_phase2_heavy_kmers(gpu_ctx, k, m, B):
     data = np.frombuffer(b"".join(seqs), np.uint8).copy()
     off = np.concatenate([[0], np.cumsum([len(x) for x in seqs])]).astype(np.uint64)
     _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
+
+
+@pytest.mark.parametrize("k,m,B", [(31, 21, 1024), (63, 27, 16)])
+def test_phase2_fragment_sort_64bit_keys(gpu_ctx, k, m, B, monkeypatch):
+    """Fragments are ordered with 32-bit (local bucket, seed rank) keys when
+    they fit; the 64-bit key path (large buckets) orders them the same."""
+    monkeypatch.setenv("CDBG_FRAG_KEY64", "1")
+    data, off = G.synth(seed=5, num_colors=5, genome_len=20000, snp_rate=0.01, indel_rate=0.001)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
