@@ -79,6 +79,83 @@ def step_bytes(name, s, k, launches, chunks):
     return table.get(name)
 
 
+def roofline(kt, s, k, steps, chunks, ms_step):
+    """Roofline of the dominant kernel that has an algorithmic-byte model
+    (bytes per launch / average launch time, both from `steps` timed steps)
+    and the aggregate over every modelled phase-2 kernel; plus the six
+    costliest kernels in ms per step."""
+    roof = None
+    agg_b = agg_ms = 0.0
+    for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
+        b = step_bytes(name, s, k, cnt / steps, chunks)
+        if b is None or cnt == 0:
+            continue
+        if name not in ("k_superkmers",):
+            agg_b += b * steps
+            agg_ms += ms
+        if roof is not None:
+            continue
+        per_launch = b * steps / cnt
+        avg_ms = ms / cnt
+        ach = per_launch / (avg_ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 3),
+                "launches_per_step": cnt // steps,
+                "share_of_step": round(ms / steps / ms_step, 3)}
+    if roof is not None and agg_ms > 0:
+        ach = agg_b / (agg_ms / 1e3) / 1e9
+        roof["phase2_modelled"] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                                   "share_of_step": round(agg_ms / steps / ms_step, 3)}
+    top = sorted(kt.items(), key=lambda x: -x[1][1])[:6]
+    return roof, {n: round(ms / steps, 2) for n, (c, ms) in top}
+
+
+# Skewed-bucket (batched) workload: 200 colours of 2 Mbp genomes (1% SNPs,
+# 0.1% indels of 1-10 bp), k=63 (128-bit k-mers), m=27, 4096 buckets whose
+# shares of the minimizer space follow Zipf(1.1).
+BATCHED = dict(colors=200, genome=2_000_000, k=63, m=27, buckets=4096, zipf=1.1)
+
+
+def batched_leg(G, ctx, steps, warmup, threads):
+    """Phases 1+2 over thousands of buckets of skewed sizes (one launch per
+    kernel covers every bucket of a chunk): distinct k-mers/s, bucket skew,
+    roofline of its dominant kernel."""
+    import numpy as np
+    c = BATCHED
+    t_gen = time.perf_counter()
+    data, off = G.synth(seed=3, num_colors=c["colors"], genome_len=c["genome"], snp_rate=0.01,
+                        indel_rate=0.001, indel_max=10, threads=threads)
+    print(f"bench: batched workload {c['colors']} colours, {len(data) / 1e9:.2f} GB in "
+          f"{time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    with ctx.job(data, off, c["k"], c["m"], c["buckets"]) as job:
+        job.set_bucket_map(G.zipf_bucket_cuts(c["buckets"], c["zipf"]))
+        for _ in range(warmup):
+            job.run()
+        job.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            job.run()
+        elapsed = time.perf_counter() - t0
+        s = job.stats()
+        kt = job.kernel_times()
+        chunks = job.num_chunks()
+        sizes = np.diff(job.bucket_offsets().astype(np.int64))
+    ms_step = 1e3 * elapsed / steps
+    roof, kernels = roofline(kt, s, c["k"], steps, chunks, ms_step)
+    value = s["num_kmers"] * steps / elapsed
+    print(f"bench: batched {steps} steps in {elapsed:.2f} s, {value / 1e9:.3f} G k-mers/s", file=sys.stderr,
+          flush=True)
+    return {"metric": "distinct k-mers/s (phases 1+2)", "value": round(value, 1), "ms_per_step": round(ms_step, 3),
+            "steps": steps, "ms_phase1": round(s["ms_phase1"], 2), "ms_phase2": round(s["ms_phase2"], 2),
+            "chunks": chunks, "num_kmers": s["num_kmers"], "num_occ": s["num_occ"],
+            "bucket_supers": {"max": int(sizes.max()), "median": int(np.median(sizes)), "min": int(sizes.min())},
+            "roofline": roof, "top_kernels_ms_per_step": kernels,
+            "workload": f"{c['colors']} colours x {c['genome'] / 1e6:g} Mbp, 1% SNP, 0.1% indel; "
+                        f"k={c['k']} (128-bit k-mers) m={c['m']}, {c['buckets']} buckets with "
+                        f"Zipf({c['zipf']}) shares of the minimizer space"}
+
+
 def serve_workload(n, rate, seed):
     """Build requests of the serving configuration: colours log-uniform in
     [2, 200], genomes U[0.5, 5] Mbp at U[0.5, 3]% divergence (SNPs; indels at a
@@ -227,6 +304,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel totals to stderr")
+    ap.add_argument("--no-batched", action="store_true", help="skip the skewed-bucket (batched) leg")
     ap.add_argument("--copy-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration)")
     args = ap.parse_args()
@@ -276,36 +354,13 @@ def main():
                      if b and cnt else "")
             print(f"  {name:32s} {cnt:6d} {ms:10.2f} ms{model}", file=sys.stderr)
 
-    # roofline of the dominant kernel that has an algorithmic-byte count, and
-    # the aggregate over every modelled phase-2 kernel
-    roof = None
-    agg_b = agg_ms = 0.0
-    for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
-        b = step_bytes(name, s, args.k, cnt / args.steps, chunks)
-        if b is None or cnt == 0:
-            continue
-        if name not in ("k_superkmers",):
-            agg_b += b * args.steps
-            agg_ms += ms
-        if roof is not None:
-            continue
-        per_launch = b * args.steps / cnt
-        avg_ms = ms / cnt
-        ach = per_launch / (avg_ms / 1e3) / 1e9
-        roof = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                "bytes_per_launch": int(per_launch), "avg_launch_ms": round(avg_ms, 3),
-                "launches_per_step": cnt // args.steps,
-                "copy_probe_gbps": round(copy_gbps, 1),
-                "share_of_step": round(ms / args.steps / ms_step, 3)}
-    if roof is not None and agg_ms > 0:
-        ach = agg_b / (agg_ms / 1e3) / 1e9
-        roof["phase2_modelled"] = {"achieved": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
-                                   "share_of_step": round(agg_ms / args.steps / ms_step, 3)}
-    top = sorted(kt.items(), key=lambda x: -x[1][1])[:6]
-    kernels = {n: round(ms / args.steps, 2) for n, (c, ms) in top}
+    roof, kernels = roofline(kt, s, args.k, args.steps, chunks, ms_step)
+    roof["copy_probe_gbps"] = round(copy_gbps, 1)
 
     job.close()
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(G, ctx, args.steps, args.warmup, args.cpu_cores)
     served = None
     if args.serve_jobs:
         served = served_leg(G, ctx, args.serve_jobs, args.serve_slots, args.serve_rate, args.cpu_cores)
@@ -334,6 +389,7 @@ def main():
         "roofline": roof,
         "top_kernels_ms_per_step": kernels,
         "chunks": chunks,
+        "batched": batched,
         "served": served,
         "cpu_baseline": cpu,
     }

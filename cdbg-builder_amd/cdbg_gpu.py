@@ -79,6 +79,7 @@ _SIGS = {
     "cdbg_gpu_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "cdbg_gpu_job_set_chunk_tuples": (C.c_int, [C.c_void_p, C.c_uint64]),
     "cdbg_gpu_job_num_chunks": (C.c_int64, [C.c_void_p]),
+    "cdbg_gpu_job_set_bucket_map": (C.c_int, [C.c_void_p, U64P, C.c_uint32]),
     "cdbg_gpu_job_frags": (C.c_int, [C.c_void_p, U8P]),
     "cdbg_gpu_job_color_sets": (C.c_int, [C.c_void_p, U64P, U64P]),
     "cdbg_gpu_bucket_write": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
@@ -305,6 +306,15 @@ class Job:
     def num_chunks(self):
         return lib().cdbg_gpu_job_num_chunks(self.h)
 
+    def set_bucket_map(self, cuts):
+        """Phase-1 bucket map (cdbg_gpu_job_set_bucket_map); None restores
+        the modulo map."""
+        if cuts is None:
+            _check(lib().cdbg_gpu_job_set_bucket_map(self.h, None, 0))
+            return
+        c = np.ascontiguousarray(cuts, dtype=np.uint64)
+        _check(lib().cdbg_gpu_job_set_bucket_map(self.h, _ptr(c, U64P), len(c)))
+
     def set_profiling(self, on=True):
         _check(lib().cdbg_gpu_job_set_profiling(self.h, 1 if on else 0))
 
@@ -336,6 +346,19 @@ class Job:
         offs = np.zeros(s["num_color_sets"], np.uint64)
         _check(lib().cdbg_gpu_job_color_sets(self.h, _ptr(words, U64P), _ptr(offs, U64P)))
         return words, offs
+
+
+def zipf_bucket_cuts(num_buckets, s):
+    """Cuts of a bucket map (cdbg_gpu_job_set_bucket_map) under which bucket b
+    receives a share of the minimizers proportional to 1 / (b + 1)^s (Zipf):
+    num_buckets - 1 ascending values below 2^32."""
+    w = [1.0 / (b + 1) ** s for b in range(num_buckets)]
+    tot = sum(w)
+    cuts, acc = [], 0.0
+    for b in range(num_buckets - 1):
+        acc += w[b]
+        cuts.append(min(int(acc / tot * 2.0 ** 32), (1 << 32) - 1))
+    return np.array(cuts, dtype=np.uint64)
 
 
 def alloc_stats():

@@ -150,6 +150,24 @@ CDBG_API int cdbg_gpu_job_create(cdbg_gpu_ctx* ctx, const uint8_t* bytes, const 
 
 CDBG_API void cdbg_gpu_job_destroy(cdbg_gpu_job* job) { delete job; }
 
+CDBG_API int cdbg_gpu_job_set_bucket_map(cdbg_gpu_job* job, const uint64_t* cuts, uint32_t n_cuts) {
+    CDBG_TRY({
+        if (!job) throw Error("null job");
+        Job& J = job->J;
+        if (!cuts) {
+            J.h_cuts.clear();
+        } else {
+            if (n_cuts + 1 != J.B) throw Error("bucket map: n_cuts must be num_buckets - 1");
+            for (uint32_t i = 0; i < n_cuts; ++i) {
+                if (cuts[i] >> 32) throw Error("bucket map: cuts must be below 2^32");
+                if (i && cuts[i] < cuts[i - 1]) throw Error("bucket map: cuts must be ascending");
+            }
+            J.h_cuts.assign(cuts, cuts + n_cuts);
+        }
+        J.phase1_done = J.phase2_done = false;
+    })
+}
+
 CDBG_API int cdbg_gpu_job_phase1(cdbg_gpu_job* job) {
     CDBG_TRY({
         if (!job) throw Error("null job");

@@ -22,6 +22,7 @@ struct Phase1Bufs {
     DevBuf b_start, b_len, b_flags, b_color;                // supers, bucket-major
     DevBuf bucket_off;                 // B+1
     DevBuf rank0;                      // per record: stream rank of its first k-mer (bucket files only)
+    DevBuf cuts;                       // bucket map (B - 1 cuts), when one is set
 };
 
 struct Phase2Bufs;  // defined in phase2.hip
@@ -58,6 +59,7 @@ struct Job {
     bool from_files = false;  // phase-1 records were read from bucket files
     uint64_t n_supers = 0;
     std::vector<uint64_t> h_bucket_off;  // host copy (B+1)
+    std::vector<uint64_t> h_cuts;        // bucket map: B - 1 ascending cuts of (minimizer >> 1), or empty
     cdbg_gpu_stats stats{};
     Profiler prof;  // per-kernel event timing, off unless enabled
     uint64_t chunk_tuples = 0;  // phase-2 chunk budget in k-mer occurrences (0 = from free HBM)

@@ -93,12 +93,31 @@ struct SkArgs {
     uint64_t n_bases;
     uint64_t n_pk_words, n_vd_words;
     uint32_t k, m, W, B;
+    const uint64_t* cuts;  // null: bucket (h >> 1) % B; else B - 1 ascending cuts of the low 32 bits of h >> 1
     HashTabs T;
     uint32_t* cnt_s;  // per-tile start counts
     uint32_t* cnt_e;  // per-tile end counts
     uint32_t* st_s;   // per tile SK_TILE slots: bucket << 16 | flags << 12 | offset in tile
     uint16_t* st_e;   // per tile SK_TILE slots: flags << 12 | offset in tile
 };
+
+// Bucket of a minimizer: (h >> 1) % B (bucket_ingester.hpp:67), or under a
+// bucket map the number of cuts <= the low 32 bits of h >> 1 (uniform, where
+// the high bits of a window minimum are not; every minimizer still has one
+// bucket, so all super-k-mers of a minimizer stay together).
+__device__ __forceinline__ uint32_t bucket_of(uint64_t h, const SkArgs& a) {
+    if (!a.cuts) return uint32_t((h >> 1) % a.B);
+    const uint64_t x = (h >> 1) & 0xFFFFFFFFull;
+    uint32_t lo = 0, hi = a.B - 1;  // first cut > x
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.cuts[mid] <= x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
 
 __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     // one pad slot per 16 hashes: threads own 16 consecutive hashes, so an
@@ -208,7 +227,7 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
             const bool begin = !valid_k1(j - 1);
             const bool own_first = begin || mn[i + 1] < mn[i];
             const uint32_t fl = (begin ? 1u : 0u) | (own_first ? 4u : 0u);
-            st_s[os++] = (uint32_t((mn[i + 1] >> 1) % a.B) << 16) | (fl << 12) | lo;
+            st_s[os++] = (bucket_of(mn[i + 1], a) << 16) | (fl << 12) | lo;
         }
         if (ends & (1u << i)) {
             const bool end = !valid_k1(j + 1);
@@ -338,6 +357,13 @@ void run_phase1(Job& J) {
     a.m = J.m;
     a.W = J.k - J.m;
     a.B = J.B;
+    if (!J.h_cuts.empty() && J.h_cuts.size() != J.B - 1) throw Error("phase 1: the bucket map has the wrong size");
+    a.cuts = nullptr;
+    if (!J.h_cuts.empty()) {
+        uint64_t* d = P.cuts.ensure<uint64_t>(J.h_cuts.size());
+        CDBG_HIP(hipMemcpyAsync(d, J.h_cuts.data(), J.h_cuts.size() * 8, hipMemcpyHostToDevice, st));
+        a.cuts = d;
+    }
     const uint64_t S[4] = {NT_SEED0, NT_SEED1, NT_SEED2, NT_SEED3};
     auto hrot = [](uint64_t x, uint32_t r) { r &= 63; return r ? (x << r) | (x >> (64 - r)) : x; };
     for (int b = 0; b < 4; ++b) {

@@ -472,25 +472,36 @@ __global__ void k_cell_over(uint32_t n_cells, const uint32_t* __restrict__ cell_
     if (c < n_cells) flag[c] = cell_start[c + 1] - cell_start[c] > cap ? 1u : 0u;
 }
 
-// Cells are dense: cell c = (local bucket c >> D, top-D-bit value c & (2^D - 1))
-// and cell_start[c] is the first tuple of the bucket whose top D bits are >= the
-// value (binary search over the bucket's sorted keys); empty cells have equal
-// starts.  cell_start[n_cells] = n_tuples.
+// Cells are dense: local bucket lb owns cells cell_base[lb] .. cell_base[lb+1]
+// - 1, one per value of its top D_lb key bits (2^D_lb of them), and
+// cell_start[c] is the first tuple of the bucket whose top D_lb bits are >=
+// the cell's value (binary search over the bucket's sorted keys); empty cells
+// have equal starts.  cell_start[n_cells] = n_tuples.
 template <class K>
-__global__ void k_cell_starts(uint32_t n_cells, uint32_t D, const uint64_t* __restrict__ lbtup, uint64_t n_tuples,
-                              const K* __restrict__ keys, int sh, uint32_t* __restrict__ cell_start) {
+__global__ void k_cell_starts(uint32_t n_cells, uint32_t nb, const uint32_t* __restrict__ cell_base,
+                              const uint8_t* __restrict__ dbits, const uint64_t* __restrict__ lbtup,
+                              uint64_t n_tuples, const K* __restrict__ keys, int kb,
+                              uint32_t* __restrict__ cell_start) {
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (c > n_cells) return;
     if (c == n_cells) {
         cell_start[c] = uint32_t(n_tuples);
         return;
     }
-    const uint32_t lb = uint32_t(c >> D), v = uint32_t(c) & ((1u << D) - 1u);
-    uint64_t lo = lbtup[lb], hi = lbtup[lb + 1];
+    uint32_t l = 0, h = nb;  // cell_base[l] <= c < cell_base[h]
+    while (h - l > 1) {
+        const uint32_t mid = (l + h) >> 1;
+        if (cell_base[mid] <= c)
+            l = mid;
+        else
+            h = mid;
+    }
+    const uint32_t v = uint32_t(c) - cell_base[l], D = dbits[l];
+    uint64_t lo = lbtup[l], hi = lbtup[l + 1];
     if (v == 0) hi = lo;
     while (lo < hi) {
         const uint64_t mid = (lo + hi) >> 1;
-        if (digit_of(keys[mid], sh, 0xFFFFFFFFu) < v)
+        if (digit_of(keys[mid], kb - int(D), 0xFFFFFFFFu) < v)
             lo = mid + 1;
         else
             hi = mid;
@@ -1886,7 +1897,7 @@ struct Phase2Bufs {
     uint64_t mt_cap = 0, mt_count = 0;
     // per chunk (reused)
     DevBuf keys, vals, keys_t, vals_t, block_super;
-    DevBuf cell_start, cell_mid, cell_mid2, cell_over, cell_novf, cell_oflag, cell_oscan, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
+    DevBuf cell_start, cell_base, cell_dbits, cell_mid, cell_mid2, cell_over, cell_novf, cell_oflag, cell_oscan, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
     DevBuf heads, hscan, node_start, node_off;
     DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_scan, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
@@ -1983,29 +1994,59 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     { KSCOPE("k_block_super", st); k_block_super<<<blocks_for(s1 - s0), TPB, 0, st>>>(s0, s1, t0, n_eblocks, P.tup_off.as<uint64_t>(), block_super); }
     { KSCOPE("k_expand", st); k_expand<K><<<uint32_t(n_eblocks), TPB, 0, st>>>(ea, block_super, keys, vals); }
 
-    // --- sort tuples by k-mer inside every bucket (stable): the top D bits by
-    // two global passes, which cuts every bucket into cells of 700 - 1400
-    // keys (measured best on MI355X: larger cells amortise the per-cell
-    // passes, an 11-bit single global pass scatters too slowly), then the
-    // rest of each cell in LDS
+    // --- sort tuples by k-mer inside every bucket (stable): the top D_max
+    // bits by one or two global passes, which cut every bucket b into 2^D_b
+    // cells of 500 - 1000 keys (measured best on MI355X against 350-700 and
+    // 700-1400: larger cells amortise the per-cell passes, smaller ones fill
+    // the CUs; an 11-bit single global pass scatters too slowly; D_b follows
+    // the bucket's size, so skewed buckets get cells of the same size), then
+    // the rest of each cell in LDS.  Cells sort the
+    // bits below the shallowest bucket's prefix; the deeper buckets' extra
+    // bits are constant in a cell or already in order, so sorting them again
+    // changes nothing.  D_b >= D_max - 8 bounds the extra passes to one and
+    // the cell count of small buckets; with 64-bit keys D_b >= 2k - 51 when
+    // the largest bucket needs it, so the packed cell sort applies to all.
     const int kb = 2 * int(k);
-    int D = 0;
     static const double cell_target = [] {  // mean keys per cell (CDBG_CELL_TARGET overrides)
         const char* e = std::getenv("CDBG_CELL_TARGET");
         const double v = e ? std::atof(e) : 0.0;
-        return v >= 16.0 ? v : 1400.0;
+        return v >= 16.0 ? v : 1000.0;
     }();
-    while (D < 20 && double(nt) / nb / double(1ull << D) > cell_target) ++D;
-    D = std::min(D, kb);
+    std::vector<uint8_t> h_dbits(nb);
+    int D_max = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        int d = 0;
+        while (d < 20 && double(lb[b + 1] - lb[b]) / double(1ull << d) > cell_target) ++d;
+        h_dbits[b] = uint8_t(std::min(d, kb));
+        D_max = std::max(D_max, int(h_dbits[b]));
+    }
+    int D_floor = std::max(D_max - 8, 0);
+    if (std::is_same<K, uint64_t>::value && D_max >= kb - (64 - PACK_IB)) D_floor = std::max(D_floor, kb - (64 - PACK_IB));
+    int D_min = kb;
+    std::vector<uint32_t> h_cbase(nb + 1, 0);
+    for (uint32_t b = 0; b < nb; ++b) {
+        h_dbits[b] = uint8_t(std::max<int>(h_dbits[b], D_floor));
+        D_min = std::min(D_min, int(h_dbits[b]));
+        const uint64_t nc = uint64_t(h_cbase[b]) + (1ull << h_dbits[b]);
+        if (nc >= (1ull << 32) - 1) throw Error("phase 2: too many sort cells in one chunk");
+        h_cbase[b + 1] = uint32_t(nc);
+    }
     SortPlan plan;
     plan.build(lb);
-    if (D > 0) segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D, kb, D <= 8 ? D : (D + 1) / 2, P.rs2, st, "tuples");
+    if (D_max > 0)
+        segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D_max, kb,
+                                          D_max <= 8 ? D_max : (D_max + 1) / 2, P.rs2, st, "tuples");
+    const int D = D_min;  // the cells sort bits [0, kb - D_min)
     uint32_t* heads = P.heads.ensure<uint32_t>(nt);
     {
-        if (uint64_t(nb) << D >= (1ull << 32)) throw Error("phase 2: too many sort cells in one chunk");
-        const uint32_t n_cells = nb << D;
+        const uint32_t n_cells = h_cbase[nb];
+        uint32_t* cbase = P.cell_base.ensure<uint32_t>(uint64_t(nb) + 1);
+        uint8_t* dbits = P.cell_dbits.ensure<uint8_t>(nb);
+        CDBG_HIP(hipMemcpyAsync(cbase, h_cbase.data(), (uint64_t(nb) + 1) * 4, hipMemcpyHostToDevice, st));
+        CDBG_HIP(hipMemcpyAsync(dbits, h_dbits.data(), nb, hipMemcpyHostToDevice, st));
         uint32_t* cell_start = P.cell_start.ensure<uint32_t>(uint64_t(n_cells) + 1);
-        { KSCOPE("k_cell_starts", st); k_cell_starts<K><<<blocks_for(uint64_t(n_cells) + 1), TPB, 0, st>>>(n_cells, uint32_t(D), lbtup, nt, keys, kb - D, cell_start); }
+        { KSCOPE("k_cell_starts", st); k_cell_starts<K><<<blocks_for(uint64_t(n_cells) + 1), TPB, 0, st>>>(n_cells, nb, cbase, dbits, lbtup, nt, keys, kb, cell_start); }
+        CDBG_HIP(hipStreamSynchronize(st));  // h_cbase / h_dbits are pageable
         // the tiny class over every cell, then each larger class over the
         // cells the previous one passed on; what is left goes to the global
         // passes

@@ -130,6 +130,16 @@ CDBG_API int cdbg_gpu_copy_probe(cdbg_gpu_ctx* ctx, uint64_t bytes, int iters, d
 CDBG_API int cdbg_gpu_job_set_chunk_tuples(cdbg_gpu_job* job, uint64_t max_tuples);
 CDBG_API int64_t cdbg_gpu_job_num_chunks(cdbg_gpu_job* job);
 
+/* Bucket map for phase 1: a super-k-mer whose minimizer hash is h goes to
+ * the bucket b with cuts[b-1] <= x < cuts[b], x = the low 32 bits of h >> 1
+ * (n_cuts = num_buckets - 1 ascending values below 2^32; cuts = NULL restores
+ * (h >> 1) % num_buckets).  Every
+ * minimizer keeps one bucket, so the graph is the same; bucket sizes follow
+ * the map (skewed-bucket workloads).  No counterpart in the reference, which
+ * always uses the modulo (bucket_ingester.hpp:67).  Takes effect at the next
+ * phase 1. */
+CDBG_API int cdbg_gpu_job_set_bucket_map(cdbg_gpu_job* job, const uint64_t* cuts, uint32_t n_cuts);
+
 /* Phase-2 artifacts: the frag_unitigs.bin byte stream (frag_bytes bytes),
  * the colour-set bit stream (ceil(cs_bits/64) words) and the bit offset of
  * every class (num_color_sets words). */

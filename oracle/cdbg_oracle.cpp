@@ -178,6 +178,17 @@ struct Super {
 // contains (minimizer.hpp:94-157, bucket_ingester.hpp:52); a super-k-mer is a
 // maximal stretch of consecutive (k-1)-mers with equal minimizer VALUE
 // (:106-109) and spans k-mers [a-1 .. b], clamped to the run (:110,121).
+// Bucket map of the build being run: null for (h >> 1) % B
+// (bucket_ingester.hpp:67), else B - 1 ascending cuts of the low 32 bits of
+// h >> 1 (a bucket map of this repository's library, for skewed-bucket
+// workloads; the reference has none).
+static const u64* g_cuts = nullptr;
+static u32 bucket_of(u64 h, u32 B) {
+    if (!g_cuts) return u32((h >> 1) % B);
+    const u64 x = (h >> 1) & 0xFFFFFFFFull;
+    return u32(std::upper_bound(g_cuts, g_cuts + (B - 1), x) - g_cuts);
+}
+
 static void supers_of_run(const u8* b, u32 L, u32 k, u32 m, u32 B, u32 color, u64 abs0,
                           std::vector<Super>& out) {
     const u32 K = L - k + 1;  // k-mers 0..K-1, (k-1)-mers 0..K
@@ -199,7 +210,7 @@ static void supers_of_run(const u8* b, u32 L, u32 k, u32 m, u32 B, u32 color, u6
         s.len = (last_kmer - first_kmer) + k;
         s.flags = u8((begin ? SK_BEGIN : 0) | (end ? SK_END : 0) | (own_first ? SK_OWNS_FIRST : 0) |
                      (own_last ? SK_OWNS_LAST : 0));
-        s.bucket = u32((mn[a] >> 1) % B);  // bucket_ingester.hpp:67
+        s.bucket = bucket_of(mn[a], B);  // bucket_ingester.hpp:67
         s.color = color;
         out.push_back(s);
     };
@@ -968,6 +979,19 @@ void* orc_build(const uint8_t* seq, const uint64_t* color_off, uint32_t num_colo
         g_err = e.what();
         return nullptr;
     }
+}
+
+// The same under a bucket map (n_cuts = B - 1 ascending cuts).
+void* orc_build_mapped(const uint8_t* seq, const uint64_t* color_off, uint32_t num_colors, uint32_t k,
+                       uint32_t m, uint32_t B, uint32_t threads, const uint64_t* cuts, uint32_t n_cuts) {
+    if (!cuts || n_cuts + 1 != B) {
+        g_err = "bucket map: n_cuts must be B - 1";
+        return nullptr;
+    }
+    g_cuts = cuts;
+    void* r = orc_build(seq, color_off, num_colors, k, m, B, threads);
+    g_cuts = nullptr;
+    return r;
 }
 
 // Phase 2 over bucket files bucket_<b>.bin in `dir` (the drop-in input).

@@ -40,6 +40,8 @@ def orc():
                                         C.c_uint32, C.c_uint32, C.c_uint64, U64P, U32P, U8P, U32P]),
             "orc_build": (C.c_void_p, [U8P, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
                                        C.c_uint32, C.c_uint32]),
+            "orc_build_mapped": (C.c_void_p, [U8P, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                              C.c_uint32, C.c_uint32, U64P, C.c_uint32]),
             "orc_process_bucket_files": (C.c_void_p, [C.c_char_p, C.c_uint32, C.c_uint32,
                                                       C.c_uint32, C.c_uint32]),
             "orc_free": (None, [C.c_void_p]),
@@ -94,10 +96,15 @@ STAT = {"num_kmers": 0, "num_nodes": 1, "num_occ": 2, "num_frags": 3, "num_sets"
 class Build:
     """Oracle phases 1+2 over the per-occurrence bucket stream."""
 
-    def __init__(self, data, color_off, k, m, B, threads=1):
+    def __init__(self, data, color_off, k, m, B, threads=1, cuts=None):
         data = np.ascontiguousarray(data, np.uint8)
         off = np.ascontiguousarray(color_off, np.uint64)
-        self.h = orc().orc_build(_p(data, U8P), _p(off, U64P), len(off) - 1, k, m, B, threads)
+        if cuts is None:
+            self.h = orc().orc_build(_p(data, U8P), _p(off, U64P), len(off) - 1, k, m, B, threads)
+        else:  # bucket map: B - 1 ascending cuts of (minimizer >> 1)
+            c = np.ascontiguousarray(cuts, np.uint64)
+            self.h = orc().orc_build_mapped(_p(data, U8P), _p(off, U64P), len(off) - 1, k, m, B, threads,
+                                            _p(c, U64P), len(c))
         if not self.h:
             raise RuntimeError(orc().orc_last_error().decode())
 

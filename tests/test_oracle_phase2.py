@@ -47,8 +47,8 @@ def _final_unitigs(ob, k, num_colors):
     return [(seq, masks[cid]) for seq, cid, _ in pieces], masks
 
 
-def _check_against_truth(data, off, k, m, B):
-    ob = O.Build(data, off, k, m, B)
+def _check_against_truth(data, off, k, m, B, cuts=None):
+    ob = O.Build(data, off, k, m, B, cuts=cuts)
     nc = len(off) - 1
     S = C.color_map(_records(data, off), k)
     assert ob.stat("num_kmers") == len(S)
@@ -123,6 +123,26 @@ def test_unitigs_equal_ground_truth_synthetic_pangenome():
     data, off = G.synth(seed=9, num_colors=5, genome_len=4000, snp_rate=0.01, indel_rate=0.002,
                         pav_frac=0.05, pav_min=100, pav_max=400, n_inversions=1, inv_min=300, inv_max=800)
     _check_against_truth(data, off, 21, 9, 16)
+
+
+@pytest.mark.parametrize("k,m,B,s", [(21, 9, 64, 1.1), (31, 19, 256, 1.5)])
+def test_unitigs_equal_ground_truth_under_skewed_bucket_map(k, m, B, s):
+    """A Zipf bucket map (every minimizer keeps one bucket) gives skewed
+    bucket sizes and the same graph."""
+    data, off = G.synth(seed=4, num_colors=6, genome_len=6000, snp_rate=0.01, indel_rate=0.001)
+    cuts = G.zipf_bucket_cuts(B, s)
+    sizes = np.bincount(O.Build(data, off, k, m, B, cuts=cuts).supers()["bucket"], minlength=B)
+    plain = np.bincount(O.Build(data, off, k, m, B).supers()["bucket"], minlength=B)
+    assert sizes.sum() == plain.sum()
+    assert sizes[0] > 5 * plain.max()  # bucket 0 takes the largest share of minimizers
+    assert _check_against_truth(data, off, k, m, B, cuts=cuts) > 10
+
+
+def test_zipf_bucket_cuts_shape():
+    c = G.zipf_bucket_cuts(4096, 1.1).astype(object)
+    assert len(c) == 4095 and all(a <= b for a, b in zip(c, c[1:])) and c[-1] < 2 ** 32
+    share = [c[0]] + [b - a for a, b in zip(c, c[1:])] + [2 ** 32 - c[-1]]
+    assert 9000 < share[0] / share[-1] < 9800  # 4096^1.1
 
 
 SAL = sorted(glob.glob("/root/reference/test_data/salmonella_10/*.fasta.gz"))

@@ -16,9 +16,11 @@ pytestmark = pytest.mark.gpu
 ACGT = np.frombuffer(b"ACGT", np.uint8)
 
 
-def _run_both(ctx, data, off, k, m, B, chunk=0):
-    ob = O.Build(data, off, k, m, B, threads=4)
+def _run_both(ctx, data, off, k, m, B, chunk=0, cuts=None):
+    ob = O.Build(data, off, k, m, B, threads=4, cuts=cuts)
     with ctx.job(data, off, k, m, B) as job:
+        if cuts is not None:
+            job.set_bucket_map(cuts)
         if chunk:
             job.set_chunk_tuples(chunk)
         job.run()
@@ -154,3 +156,12 @@ def test_phase2_fragment_sort_64bit_keys(gpu_ctx, k, m, B, monkeypatch):
     monkeypatch.setenv("CDBG_FRAG_KEY64", "1")
     data, off = G.synth(seed=5, num_colors=5, genome_len=20000, snp_rate=0.01, indel_rate=0.001)
     _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
+
+
+@pytest.mark.parametrize("k,m,B,s,chunk", [(31, 19, 512, 1.1, 0), (63, 27, 256, 1.1, 0), (31, 21, 1024, 1.3, 40000)])
+def test_phase2_skewed_bucket_map(gpu_ctx, k, m, B, s, chunk):
+    """Zipf bucket sizes (a bucket map): one bucket holds a large share of
+    the occurrences, most hold a few; results equal the oracle under the same
+    map, also in several chunks."""
+    data, off = G.synth(seed=12, num_colors=6, genome_len=30000, snp_rate=0.01, indel_rate=0.001)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B, chunk=chunk, cuts=G.zipf_bucket_cuts(B, s)))
