@@ -198,23 +198,51 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
     const uint64_t TE = min(T0 + EXP_TILE, a.t0 + a.n);  // tiles end
     const uint64_t sa = a.s0 + block_super[blockIdx.x];
     const uint32_t ns = block_super[blockIdx.x + 1] - block_super[blockIdx.x] + 1;  // <= EXP_TILE + 1
+    // the record of every tuple of the tile: each record marks its first
+    // tuple, and an inclusive max-scan carries the mark to the tuples after it
+    __shared__ uint32_t s_rec[EXP_TILE];
+    __shared__ uint32_t s_wmax[TPB / 64];
     for (uint32_t i = threadIdx.x; i <= ns; i += TPB) s_off[i] = a.tup_off[sa + i];
+#pragma unroll
+    for (uint32_t j = 0; j < EXP_PER; ++j) s_rec[threadIdx.x * EXP_PER + j] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ns; i += TPB) {
+        const uint64_t f = s_off[i] > T0 ? s_off[i] - T0 : 0;  // only the first record starts before the tile
+        if (f < EXP_TILE) s_rec[f] = i;
+    }
+    __syncthreads();
+    {
+        uint32_t v[EXP_PER];
+        uint32_t m = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < EXP_PER; ++j) {
+            m = max(m, s_rec[threadIdx.x * EXP_PER + j]);
+            v[j] = m;
+        }
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        uint32_t x = m;  // inclusive max over the lanes before, then the waves before
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x = max(x, y);
+        }
+        if (lane == 63) s_wmax[wave] = x;
+        uint32_t before = __shfl_up(x, 1, 64);
+        if (lane == 0) before = 0;
+        __syncthreads();
+        for (int w = 0; w < wave; ++w) before = max(before, s_wmax[w]);
+#pragma unroll
+        for (uint32_t j = 0; j < EXP_PER; ++j) s_rec[threadIdx.x * EXP_PER + j] = max(v[j], before);
+    }
     __syncthreads();
     uint64_t sup[EXP_PER];
     uint32_t ix[EXP_PER];
 #pragma unroll
     for (uint32_t j = 0; j < EXP_PER; ++j) {
         const uint64_t t = T0 + threadIdx.x + j * TPB;
-        uint32_t lo = 0, hi = ns;  // s_off[lo] <= t < s_off[hi]
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) >> 1;
-            if (s_off[mid] <= t)
-                lo = mid;
-            else
-                hi = mid;
-        }
-        sup[j] = sa + lo;
-        ix[j] = uint32_t(t - s_off[lo]);
+        const uint32_t r = s_rec[threadIdx.x + j * TPB];  // s_off[r] <= t < s_off[r + 1]
+        sup[j] = sa + r;
+        ix[j] = uint32_t(t - s_off[r]);
     }
     uint32_t len[EXP_PER], fl[EXP_PER], col[EXP_PER];
     uint64_t st[EXP_PER], rk[EXP_PER];
