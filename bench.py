@@ -60,9 +60,9 @@ def step_bytes(name, s, k, launches, chunks):
         "k_expand": ns * 29 + nb // 4 + n * (wk + 8),
         # the 2-bit bases and the ACGT bitmap in; 4-byte start and 2-byte end entries out
         "k_superkmers": nb // 4 + nb // 8 + ns * 6,
-        # values, head flags in; node key, meta, rank, colour count, set hash, long-run flag,
-        # and the owned colours out
-        "k_nodes": n * (8 + 4) + nn * (4 + 4 + wk) + nn * (wk + 1 + 4 + 4 + 8 + 4) + n * 4,
+        # values, one-byte head flags in; node key, meta, rank, colour count, set hash,
+        # long-run flag, and the owned colours out
+        "k_nodes": n * (8 + 1) + nn * (4 + 4 + wk) + nn * (wk + 1 + 4 + 4 + 8 + 4) + n * 4,
         # node key, bucket and meta in; 8 neighbour lookups (two directory words and a key
         # each); mask, meta/mask and the two neighbour words out
         "k_masks": nn * (wk + 4 + 1) + 8 * nn * (8 + wk) + nn * (1 + 2 + 8),

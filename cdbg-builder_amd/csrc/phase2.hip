@@ -300,7 +300,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
                                                        const uint32_t* __restrict__ list, uint32_t n_items,
                                                        uint32_t lo, int end_bit, uint32_t* __restrict__ next,
                                                        unsigned long long* __restrict__ n_next,
-                                                       uint32_t* __restrict__ heads) {
+                                                       uint8_t* __restrict__ heads) {
     constexpr int WAVES = THREADS / 64;
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
     extern __shared__ uint64_t sm64[];
@@ -316,7 +316,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     if (blockIdx.x >= n_items) return;
     const uint32_t cell = list ? list[blockIdx.x] : blockIdx.x;
     const uint32_t start = cell_start[cell], count = cell_start[cell + 1] - start;
-    if (count == 1 && lo == 0 && threadIdx.x == 0) heads[start] = 1u;  // a one-key cell is one node
+    if (count == 1 && lo == 0 && threadIdx.x == 0) heads[start] = 1;  // a one-key cell is one node
     if (count <= 1 || count <= lo) return;
     if (count > CAP) {
         if (next && threadIdx.x == 0) next[atomicAdd(n_next, 1ull)] = cell;
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
             differs = (s_key[i] >> PACK_IB) != (s_key[i - (i > 0)] >> PACK_IB);
         else
             differs = !KOps<K>::eq(s_key[i], s_key[i - (i > 0)]);
-        heads[start + i] = (i == 0 || (sorted_in_lds && differs)) ? 1u : 0u;
+        heads[start + i] = (i == 0 || (sorted_in_lds && differs)) ? 1 : 0;
     }
 }
 
@@ -479,7 +479,7 @@ struct CellClasses<Key128> {
 template <class K, uint32_t CAP, int THREADS>
 static void launch_cell_sort(uint32_t grid, hipStream_t st, K* keys, uint64_t* vals, const uint32_t* cell_start,
                              const uint32_t* list, uint32_t n_items, uint32_t lo, int end_bit, uint32_t* next,
-                             unsigned long long* n_next, uint32_t* heads) {
+                             unsigned long long* n_next, uint8_t* heads) {
     if constexpr (std::is_same<K, uint64_t>::value) {
         if (end_bit + PACK_IB <= 64) {
             cell_sort_prepare<K, uint64_t, CAP, THREADS, true>();
@@ -541,7 +541,7 @@ __global__ void k_cell_starts(uint32_t n_cells, uint32_t nb, const uint32_t* __r
 // of cell c maps to start[c] + j - pre[c].
 template <class K>
 __global__ void k_cells_heads(const uint32_t* __restrict__ start, const uint64_t* __restrict__ pre, uint32_t n_list,
-                              const K* __restrict__ ck, uint32_t* __restrict__ heads) {
+                              const K* __restrict__ ck, uint8_t* __restrict__ heads) {
     const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (j >= pre[n_list]) return;
     uint32_t lo = 0, hi = n_list;
@@ -552,7 +552,7 @@ __global__ void k_cells_heads(const uint32_t* __restrict__ start, const uint64_t
         else
             hi = mid;
     }
-    heads[start[lo] + (j - pre[lo])] = (j == pre[lo] || !KOps<K>::eq(ck[j], ck[j - 1])) ? 1u : 0u;
+    heads[start[lo] + (j - pre[lo])] = (j == pre[lo] || !KOps<K>::eq(ck[j], ck[j - 1])) ? 1 : 0;
 }
 
 // Moves the elements of listed cells between the tuple arrays and a compact
@@ -585,7 +585,7 @@ __global__ void k_cells_move(bool to_compact, const uint32_t* __restrict__ start
 // ---------------------------------------------------------------------------
 // nodes
 // ---------------------------------------------------------------------------
-__global__ void k_node_start(uint64_t n, const uint32_t* __restrict__ heads, const uint32_t* __restrict__ hscan,
+__global__ void k_node_start(uint64_t n, const uint8_t* __restrict__ heads, const uint32_t* __restrict__ hscan,
                              uint32_t n_nodes, uint32_t* __restrict__ node_start) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n && heads[i]) node_start[hscan[i]] = uint32_t(i);
@@ -639,7 +639,7 @@ constexpr uint32_t NT_TILE = 1024;
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
 template <class K>
-__global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint32_t* __restrict__ heads,
+__global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __restrict__ heads,
                                                const uint32_t* __restrict__ hscan,
                                                const uint32_t* __restrict__ node_start, const K* __restrict__ keys,
                                                const uint64_t* __restrict__ vals, K* __restrict__ node_key,
@@ -2065,7 +2065,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D_max, kb,
                                           D_max <= 8 ? D_max : (D_max + 1) / 2, P.rs2, st, "tuples");
     const int D = D_min;  // the cells sort bits [0, kb - D_min)
-    uint32_t* heads = P.heads.ensure<uint32_t>(nt);
+    uint8_t* heads = P.heads.ensure<uint8_t>(nt);  // 1 at the first tuple of every node
     {
         const uint32_t n_cells = h_cbase[nb];
         uint32_t* cbase = P.cell_base.ensure<uint32_t>(uint64_t(nb) + 1);
@@ -2134,7 +2134,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint32_t* hscan = P.hscan.ensure<uint32_t>(nt + 1);
     uint64_t* dtot = P.total.ensure<uint64_t>(8);
     uint32_t* dtot32 = (uint32_t*)(dtot + 1);
-    exclusive_scan<uint32_t, uint32_t>(heads, hscan, nt, dtot32, J.rs.scan_tmp, st);
+    exclusive_scan<uint8_t, uint32_t>(heads, hscan, nt, dtot32, J.rs.scan_tmp, st);
     const uint32_t nn = d2h<uint32_t>(dtot32, st);
     if (nn >= (1u << 31)) throw Error("phase 2: more than 2^31 nodes in one chunk");
     uint32_t* node_start = P.node_start.ensure<uint32_t>(uint64_t(nn) + 1);
