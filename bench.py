@@ -70,7 +70,7 @@ def step_bytes(name, s, k, launches, chunks):
         "k_dir_build": nn * (wk + 4) + nn * 4,
         # per path position: node record + meta/mask + path entry in; key, cid, rank,
         # meta/mask, orientation, fragment flag out
-        "k_walk_record": nn * ((wk + 8) + 2 + 4) + nn * (wk + 4 + 4 + 2 + 1 + 4),
+        "k_walk_record": nn * ((wk + 8) + 2 + 4) + nn * (wk + 4 + 4 + 2 + 1 + 1),
         # fragment order + descriptors, per position key/cid/orientation, records out
         "k_frag_write": nf * (4 + 4 + 1 + 1) + nn * (wk + 4 + 1) + F,
         # primary nodes: meta, bucket, rank, colour count, offset, hash, first colour in; slot out

@@ -645,7 +645,7 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
                                                const uint64_t* __restrict__ vals, K* __restrict__ node_key,
                                                uint8_t* __restrict__ node_meta, uint32_t* __restrict__ node_rank,
                                                uint32_t* __restrict__ node_ncol, uint64_t* __restrict__ node_hash,
-                                               uint32_t* __restrict__ colors, uint32_t* __restrict__ big_flag) {
+                                               uint32_t* __restrict__ colors, uint8_t* __restrict__ big_flag) {
     __shared__ uint64_t s_val[NT_TILE + BIG_RUN];
     __shared__ uint32_t s_list[NT_TILE];
     __shared__ uint32_t s_wn[TPB / 64 + 1];
@@ -791,7 +791,8 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
 }
 
 // Indices of flagged items: list[scan[i]] = i.
-__global__ void k_compact(uint64_t n, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
+template <class F>
+__global__ void k_compact(uint64_t n, const F* __restrict__ flag, const uint32_t* __restrict__ scan,
                           uint32_t* __restrict__ list) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n && flag[i]) list[scan[i]] = uint32_t(i);
@@ -1351,7 +1352,7 @@ struct WalkArgs {
     uint8_t* frag_open;
     uint64_t* frag_key;
     uint8_t* frag_dir;             // 1: the fragment is its position range read backwards
-    uint32_t* fflag;               // 1 at the first position of a fragment
+    uint8_t* fflag;                // 1 at the first position of a fragment
     // simple paths, recorded end to end
     const uint32_t* comp_x;        // owning end (the end list: a path is stored at its owner's index)
     uint8_t* comp_ex;              // port the path leaves it by
@@ -1522,11 +1523,11 @@ __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, 
 }
 
 // Path ends (a node with at most one link) in node order.
-__global__ void k_end_flags(uint32_t n_nodes, const uint32_t* __restrict__ link, uint32_t* __restrict__ flag) {
+__global__ void k_end_flags(uint32_t n_nodes, const uint32_t* __restrict__ link, uint8_t* __restrict__ flag) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n < n_nodes) flag[n] = (link[2 * n + PORT_L] == NONE32 || link[2 * n + PORT_R] == NONE32) ? 1u : 0u;
+    if (n < n_nodes) flag[n] = (link[2 * n + PORT_L] == NONE32 || link[2 * n + PORT_R] == NONE32) ? 1 : 0;
 }
-__global__ void k_end_list(uint32_t n_nodes, const uint32_t* __restrict__ flag, const uint32_t* __restrict__ scan,
+__global__ void k_end_list(uint32_t n_nodes, const uint8_t* __restrict__ flag, const uint32_t* __restrict__ scan,
                            uint32_t* __restrict__ ends) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n < n_nodes && flag[n]) ends[scan[n]] = uint32_t(n);
@@ -1783,7 +1784,7 @@ __global__ void k_walk_cycles(WalkArgs a) {
 
 // Dense fragment list from the first-position flags: fkey[d] / forder[d] =
 // the key and first position of the d-th fragment in position order.
-__global__ void k_frag_compact(uint64_t n, const uint32_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
+__global__ void k_frag_compact(uint64_t n, const uint8_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
                                const uint64_t* __restrict__ frag_key, uint64_t* __restrict__ fkey,
                                uint32_t* __restrict__ forder) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1796,7 +1797,7 @@ __global__ void k_frag_compact(uint64_t n, const uint32_t* __restrict__ fflag, c
 // The same with 32-bit sort keys (local bucket << rb) | seed rank, for a chunk
 // whose buckets and ranks fit them (a rank is below its bucket's tuple count,
 // which is below 2^rb); a rank that does not fit is counted in *overflow.
-__global__ void k_frag_compact32(uint64_t n, const uint32_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
+__global__ void k_frag_compact32(uint64_t n, const uint8_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
                                  const uint64_t* __restrict__ frag_key, uint32_t b0, int rb,
                                  uint32_t* __restrict__ fkey, uint32_t* __restrict__ forder,
                                  unsigned long long* __restrict__ overflow) {
@@ -2153,10 +2154,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint32_t* coff = node_start;
     unsigned long long* cnt = P.counters.ensure<unsigned long long>(N_COUNTERS);
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, N_COUNTERS, 0ull); }
-    uint32_t* big_flag = P.big_flag.ensure<uint32_t>(nn);
+    uint8_t* big_flag = P.big_flag.ensure<uint8_t>(nn);
     uint32_t* big_scan = P.big_scan.ensure<uint32_t>(uint64_t(nn) + 1);
     { KSCOPE("k_nodes", st); k_nodes<K><<<blocks_for(nt, NT_TILE), TPB, 0, st>>>(nt, heads, hscan, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
-    exclusive_scan<uint32_t, uint32_t>(big_flag, big_scan, nn, big_scan + nn, J.rs.scan_tmp, st);
+    exclusive_scan<uint8_t, uint32_t>(big_flag, big_scan, nn, big_scan + nn, J.rs.scan_tmp, st);
     const uint64_t nbig = d2h<uint32_t>(big_scan + nn, st);
     unsigned long long h_cnt[N_COUNTERS];
     if (nbig) { KSCOPE("k_compact", st); k_compact<<<blocks_for(nn), TPB, 0, st>>>(nn, big_flag, big_scan, big); }
@@ -2251,7 +2252,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.frag_open = P.frag_open.ensure<uint8_t>(nn);
     wa.frag_key = P.frag_key.ensure<uint64_t>(nn);
     wa.frag_dir = P.frag_dir.ensure<uint8_t>(nn);
-    uint32_t* fflag = P.fflag.ensure<uint32_t>(nn);
+    uint8_t* fflag = P.fflag.ensure<uint8_t>(nn);
     wa.fflag = fflag;
     wa.other_x = P.other_x.ensure<uint32_t>(nn);
     wa.other_ex = P.other_ex.ensure<uint8_t>(nn);
@@ -2265,10 +2266,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.pcid = pcid;
     wa.node_rec = node_rec;
     wa.node_mm = node_mm;
-    uint32_t* eflag = P.eflag.ensure<uint32_t>(nn);
+    uint8_t* eflag = P.eflag.ensure<uint8_t>(nn);
     uint32_t* escan = P.escan.ensure<uint32_t>(uint64_t(nn) + 1);
     { KSCOPE("k_end_flags", st); k_end_flags<<<blocks_for(nn), TPB, 0, st>>>(nn, link, eflag); }
-    exclusive_scan<uint32_t, uint32_t>(eflag, escan, nn, escan + nn, J.rs.scan_tmp, st);
+    exclusive_scan<uint8_t, uint32_t>(eflag, escan, nn, escan + nn, J.rs.scan_tmp, st);
     const uint32_t n_ends = d2h<uint32_t>(escan + nn, st);
     uint32_t* ends = P.ends.ensure<uint32_t>(n_ends);
     { KSCOPE("k_end_list", st); k_end_list<<<blocks_for(nn), TPB, 0, st>>>(nn, eflag, escan, ends); }
@@ -2300,7 +2301,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         throw Error("phase 2: walk covered " + std::to_string(h_cnt[3]) + " of " + std::to_string(nn) +
                     " nodes (" + std::to_string(h_cnt[5]) + " errors)");
     uint32_t* fscan = P.fscan.ensure<uint32_t>(uint64_t(nn) + 1);
-    exclusive_scan<uint32_t, uint32_t>(fflag, fscan, nn, fscan + nn, J.rs.scan_tmp, st);
+    exclusive_scan<uint8_t, uint32_t>(fflag, fscan, nn, fscan + nn, J.rs.scan_tmp, st);
     const uint64_t nf = d2h<uint32_t>(fscan + nn, st);
 
     // --- fragments in (bucket, seed rank) order, appended to the job stream.
