@@ -585,11 +585,25 @@ __global__ void k_cells_move(bool to_compact, const uint32_t* __restrict__ start
 // ---------------------------------------------------------------------------
 // nodes
 // ---------------------------------------------------------------------------
-__global__ void k_node_start(uint64_t n, const uint8_t* __restrict__ heads, const uint32_t* __restrict__ hscan,
-                             uint32_t n_nodes, uint32_t* __restrict__ node_start) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n && heads[i]) node_start[hscan[i]] = uint32_t(i);
-    if (i == 0) node_start[n_nodes] = uint32_t(n);
+// Heads (node starts) of every NT_TILE-tuple tile of k_nodes.
+constexpr uint32_t NT_TILE = 1024;
+__global__ __launch_bounds__(TPB) void k_tile_heads(uint64_t nt, const uint8_t* __restrict__ heads,
+                                                    uint32_t* __restrict__ tile_cnt) {
+    __shared__ uint64_t s_tmp[TPB / 64 + 1];
+    const uint64_t T0 = uint64_t(blockIdx.x) * NT_TILE;
+    uint64_t c = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < NT_TILE / TPB; ++r) {
+        const uint64_t t = T0 + r * TPB + threadIdx.x;
+        c += t < nt && heads[t];
+    }
+    uint64_t tot;
+    (void)block_exclusive_scan<uint64_t>(c, s_tmp, tot);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = uint32_t(tot);
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 // node -> local bucket, by binary search over node_off staged in LDS
@@ -606,12 +620,22 @@ __global__ void k_node_bucket(uint32_t n_nodes, const uint64_t* __restrict__ nod
     node_bucket[n] = staged ? bucket_of(s_off, nb, n) : bucket_of(node_off, nb, n);
 }
 
-__global__ void k_node_off(uint32_t nb, const uint64_t* __restrict__ lbtup, uint64_t n_tuples,
-                           const uint32_t* __restrict__ hscan, uint32_t n_nodes, uint64_t* __restrict__ node_off) {
+// First node of every bucket: the nodes that start before its first tuple
+// (a bucket's first tuple is a head), by binary search over the node starts.
+__global__ void k_node_off(uint32_t nb, const uint64_t* __restrict__ lbtup, const uint32_t* __restrict__ node_start,
+                           uint32_t n_nodes, uint64_t* __restrict__ node_off) {
     const uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b > nb) return;
     const uint64_t t = lbtup[b];
-    node_off[b] = t < n_tuples ? hscan[t] : n_nodes;
+    uint32_t lo = 0, hi = n_nodes;  // first node starting at or after t
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (node_start[mid] < t)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    node_off[b] = lo;
 }
 
 // Nodes with more than BIG_RUN occurrences are reduced by a whole wave.
@@ -634,14 +658,15 @@ __device__ __forceinline__ uint64_t set_hash_fin(uint64_t sum, uint32_t n) {
 // (bucket_walker.hpp:228-249 merge of map entries).  A block takes the nodes
 // whose first occurrence lies in its tile of NT_TILE tuples and stages the
 // tile (plus the BIG_RUN tuples after it) in LDS; runs longer than BIG_RUN
-// are queued for k_nodes_big.
-constexpr uint32_t NT_TILE = 1024;
+// are queued for k_nodes_big.  A node's index is its tile's first node
+// (scanned tile head counts) plus its rank among the tile's heads; the node
+// writes its own start.
 __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
 template <class K>
 __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __restrict__ heads,
-                                               const uint32_t* __restrict__ hscan,
-                                               const uint32_t* __restrict__ node_start, const K* __restrict__ keys,
+                                               const uint32_t* __restrict__ tile_base,
+                                               uint32_t* __restrict__ node_start, const K* __restrict__ keys,
                                                const uint64_t* __restrict__ vals, K* __restrict__ node_key,
                                                uint8_t* __restrict__ node_meta, uint32_t* __restrict__ node_rank,
                                                uint32_t* __restrict__ node_ncol, uint64_t* __restrict__ node_hash,
@@ -649,9 +674,16 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
     __shared__ uint64_t s_val[NT_TILE + BIG_RUN];
     __shared__ uint32_t s_list[NT_TILE];
     __shared__ uint32_t s_wn[TPB / 64 + 1];
+    __shared__ uint32_t s_next;  // first head in [T1, TE); nt when the tuples end first; NONE32: none
     const uint64_t T0 = uint64_t(blockIdx.x) * NT_TILE;
     const uint64_t T1 = min(T0 + NT_TILE, nt), TE = min(T1 + BIG_RUN, nt);
     for (uint32_t i = threadIdx.x; i < TE - T0; i += TPB) s_val[i] = vals[T0 + i];
+    static_assert(BIG_RUN <= 64, "one wave looks past the tile");
+    if (threadIdx.x < 64) {
+        const uint64_t t = T1 + threadIdx.x;
+        const uint64_t m = __ballot(threadIdx.x < BIG_RUN && t < TE && heads[t]);
+        if (threadIdx.x == 0) s_next = m ? uint32_t(T1 + __ffsll((unsigned long long)m) - 1) : (TE == nt ? uint32_t(nt) : NONE32);
+    }
     // heads of the tile, in order: wave w lists the heads of its 4 x 64 tuples
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t lt = lanes_below(lane);
@@ -685,13 +717,15 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
     }
     __syncthreads();
     const uint32_t n_heads = s_wn[TPB / 64];
+    const uint32_t base = tile_base[blockIdx.x];
     for (uint32_t j = threadIdx.x; j < n_heads; j += TPB) {
         const uint32_t hd = s_list[j];
         const uint32_t t0 = uint32_t(T0 + hd);
-        const uint32_t n = hscan[t0];
-        const uint32_t t1 = node_start[n + 1];
+        const uint32_t n = base + j;
+        const uint32_t t1 = j + 1 < n_heads ? uint32_t(T0 + s_list[j + 1]) : s_next;
+        node_start[n] = t0;
         node_key[n] = keys[t0];
-        const bool is_big = t1 - t0 > BIG_RUN;
+        const bool is_big = t1 == NONE32 || t1 - t0 > BIG_RUN;
         big_flag[n] = is_big;
         if (is_big) continue;
         uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
@@ -2132,18 +2166,19 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     }
 
     // --- nodes
-    uint32_t* hscan = P.hscan.ensure<uint32_t>(nt + 1);
+    // node numbers: heads counted per k_nodes tile and scanned over the tiles
+    const uint64_t n_ntiles = (nt + NT_TILE - 1) / NT_TILE;
+    uint32_t* tile_cnt = P.hscan.ensure<uint32_t>(2 * (n_ntiles + 1));
+    uint32_t* tile_base = tile_cnt + n_ntiles + 1;
     uint64_t* dtot = P.total.ensure<uint64_t>(8);
     uint32_t* dtot32 = (uint32_t*)(dtot + 1);
-    exclusive_scan<uint8_t, uint32_t>(heads, hscan, nt, dtot32, J.rs.scan_tmp, st);
+    { KSCOPE("k_tile_heads", st); k_tile_heads<<<uint32_t(n_ntiles), TPB, 0, st>>>(nt, heads, tile_cnt); }
+    exclusive_scan<uint32_t, uint32_t>(tile_cnt, tile_base, n_ntiles, dtot32, J.rs.scan_tmp, st);
     const uint32_t nn = d2h<uint32_t>(dtot32, st);
     if (nn >= (1u << 31)) throw Error("phase 2: more than 2^31 nodes in one chunk");
     uint32_t* node_start = P.node_start.ensure<uint32_t>(uint64_t(nn) + 1);
     uint64_t* node_off = P.node_off.ensure<uint64_t>(nb + 1);
-    { KSCOPE("k_node_start", st); k_node_start<<<blocks_for(nt), TPB, 0, st>>>(nt, heads, hscan, nn, node_start); }
-    { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, nt, hscan, nn, node_off); }
     uint32_t* node_bucket = P.node_bucket.ensure<uint32_t>(nn);
-    { KSCOPE("k_node_bucket", st); k_node_bucket<<<blocks_for(nn), TPB, 0, st>>>(nn, node_off, nb, node_bucket); }
     K* node_key = P.node_key.ensure<K>(nn);
     uint8_t* meta = P.node_meta.ensure<uint8_t>(nn);
     uint32_t* rank = P.node_rank.ensure<uint32_t>(nn);
@@ -2156,7 +2191,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, N_COUNTERS, 0ull); }
     uint8_t* big_flag = P.big_flag.ensure<uint8_t>(nn);
     uint32_t* big_scan = P.big_scan.ensure<uint32_t>(uint64_t(nn) + 1);
-    { KSCOPE("k_nodes", st); k_nodes<K><<<blocks_for(nt, NT_TILE), TPB, 0, st>>>(nt, heads, hscan, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
+    { KSCOPE("k_nodes", st); k_nodes<K><<<uint32_t(n_ntiles), TPB, 0, st>>>(nt, heads, tile_base, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
+    { KSCOPE("k_set_u32", st); k_set_u32<<<1, 64, 0, st>>>(node_start + nn, uint32_t(nt)); }
+    { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, node_start, nn, node_off); }
+    { KSCOPE("k_node_bucket", st); k_node_bucket<<<blocks_for(nn), TPB, 0, st>>>(nn, node_off, nb, node_bucket); }
     exclusive_scan<uint8_t, uint32_t>(big_flag, big_scan, nn, big_scan + nn, J.rs.scan_tmp, st);
     const uint64_t nbig = d2h<uint32_t>(big_scan + nn, st);
     unsigned long long h_cnt[N_COUNTERS];
