@@ -1561,11 +1561,6 @@ __global__ void k_end_flags(uint32_t n_nodes, const uint32_t* __restrict__ link,
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n < n_nodes) flag[n] = (link[2 * n + PORT_L] == NONE32 || link[2 * n + PORT_R] == NONE32) ? 1 : 0;
 }
-__global__ void k_end_list(uint32_t n_nodes, const uint8_t* __restrict__ flag, const uint32_t* __restrict__ scan,
-                           uint32_t* __restrict__ ends) {
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n < n_nodes && flag[n]) ends[scan[n]] = uint32_t(n);
-}
 
 // Path ends walk to the other end; the end with the smaller index owns the
 // path.  Paths that never pass a node twice are queued for recording (with
@@ -1816,33 +1811,26 @@ __global__ void k_walk_cycles(WalkArgs a) {
     simulate_component<K>(a, uint32_t(x), PORT_R, true, simple ? len : 0, seed);
 }
 
-// Dense fragment list from the first-position flags: fkey[d] / forder[d] =
-// the key and first position of the d-th fragment in position order.
-__global__ void k_frag_compact(uint64_t n, const uint8_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
-                               const uint64_t* __restrict__ frag_key, uint64_t* __restrict__ fkey,
-                               uint32_t* __restrict__ forder) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n || !fflag[i]) return;
-    const uint32_t d = fscan[i];
-    fkey[d] = frag_key[i];
-    forder[d] = uint32_t(i);
+// Sort keys of the fragments: forder[d] = the d-th fragment's first position
+// (ascending), fkey[d] = its (bucket, seed rank) key.
+__global__ void k_frag_keys(uint64_t nf, const uint32_t* __restrict__ forder, const uint64_t* __restrict__ frag_key,
+                            uint64_t* __restrict__ fkey) {
+    const uint64_t d = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (d < nf) fkey[d] = frag_key[forder[d]];
 }
 
 // The same with 32-bit sort keys (local bucket << rb) | seed rank, for a chunk
 // whose buckets and ranks fit them (a rank is below its bucket's tuple count,
 // which is below 2^rb); a rank that does not fit is counted in *overflow.
-__global__ void k_frag_compact32(uint64_t n, const uint8_t* __restrict__ fflag, const uint32_t* __restrict__ fscan,
-                                 const uint64_t* __restrict__ frag_key, uint32_t b0, int rb,
-                                 uint32_t* __restrict__ fkey, uint32_t* __restrict__ forder,
-                                 unsigned long long* __restrict__ overflow) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n || !fflag[i]) return;
-    const uint32_t d = fscan[i];
-    const uint64_t key = frag_key[i];
+__global__ void k_frag_keys32(uint64_t nf, const uint32_t* __restrict__ forder,
+                              const uint64_t* __restrict__ frag_key, uint32_t b0, int rb,
+                              uint32_t* __restrict__ fkey, unsigned long long* __restrict__ overflow) {
+    const uint64_t d = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (d >= nf) return;
+    const uint64_t key = frag_key[forder[d]];
     const uint32_t r = uint32_t(key);
     if (rb < 32 && (r >> rb)) atomicAdd(overflow, 1ull);
     fkey[d] = uint32_t((uint64_t(uint32_t(key >> 32) - b0) << rb) | r);
-    forder[d] = uint32_t(i);
 }
 
 // ---------------------------------------------------------------------------
@@ -2190,15 +2178,14 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     unsigned long long* cnt = P.counters.ensure<unsigned long long>(N_COUNTERS);
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, N_COUNTERS, 0ull); }
     uint8_t* big_flag = P.big_flag.ensure<uint8_t>(nn);
-    uint32_t* big_scan = P.big_scan.ensure<uint32_t>(uint64_t(nn) + 1);
     { KSCOPE("k_nodes", st); k_nodes<K><<<uint32_t(n_ntiles), TPB, 0, st>>>(nt, heads, tile_base, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
     { KSCOPE("k_set_u32", st); k_set_u32<<<1, 64, 0, st>>>(node_start + nn, uint32_t(nt)); }
     { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, node_start, nn, node_off); }
     { KSCOPE("k_node_bucket", st); k_node_bucket<<<blocks_for(nn), TPB, 0, st>>>(nn, node_off, nb, node_bucket); }
-    exclusive_scan<uint8_t, uint32_t>(big_flag, big_scan, nn, big_scan + nn, J.rs.scan_tmp, st);
-    const uint64_t nbig = d2h<uint32_t>(big_scan + nn, st);
+    uint32_t* d_nbig = (uint32_t*)(dtot + 2);
+    compact_flags<uint8_t>(big_flag, nn, big, d_nbig, P.big_scan, J.rs.scan_tmp, st);
+    const uint64_t nbig = d2h<uint32_t>(d_nbig, st);
     unsigned long long h_cnt[N_COUNTERS];
-    if (nbig) { KSCOPE("k_compact", st); k_compact<<<blocks_for(nn), TPB, 0, st>>>(nn, big_flag, big_scan, big); }
     if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors); }
     { KSCOPE("k_count_nodes", st); k_count_nodes<<<1024, TPB, 0, st>>>(nn, meta, ncol, cnt); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
@@ -2305,12 +2292,11 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.node_rec = node_rec;
     wa.node_mm = node_mm;
     uint8_t* eflag = P.eflag.ensure<uint8_t>(nn);
-    uint32_t* escan = P.escan.ensure<uint32_t>(uint64_t(nn) + 1);
+    uint32_t* ends = P.ends.ensure<uint32_t>(nn);
+    uint32_t* d_count = (uint32_t*)(P.total.ensure<uint64_t>(8) + 2);
     { KSCOPE("k_end_flags", st); k_end_flags<<<blocks_for(nn), TPB, 0, st>>>(nn, link, eflag); }
-    exclusive_scan<uint8_t, uint32_t>(eflag, escan, nn, escan + nn, J.rs.scan_tmp, st);
-    const uint32_t n_ends = d2h<uint32_t>(escan + nn, st);
-    uint32_t* ends = P.ends.ensure<uint32_t>(n_ends);
-    { KSCOPE("k_end_list", st); k_end_list<<<blocks_for(nn), TPB, 0, st>>>(nn, eflag, escan, ends); }
+    compact_flags<uint8_t>(eflag, nn, ends, d_count, P.escan, J.rs.scan_tmp, st);
+    const uint32_t n_ends = d2h<uint32_t>(d_count, st);
     wa.comp_x = ends;
     wa.comp_ex = P.comp_ex.ensure<uint8_t>(n_ends);
     wa.comp_len = P.comp_len.ensure<uint32_t>(n_ends);
@@ -2338,16 +2324,15 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (h_cnt[5] || h_cnt[3] != nn)
         throw Error("phase 2: walk covered " + std::to_string(h_cnt[3]) + " of " + std::to_string(nn) +
                     " nodes (" + std::to_string(h_cnt[5]) + " errors)");
-    uint32_t* fscan = P.fscan.ensure<uint32_t>(uint64_t(nn) + 1);
-    exclusive_scan<uint8_t, uint32_t>(fflag, fscan, nn, fscan + nn, J.rs.scan_tmp, st);
-    const uint64_t nf = d2h<uint32_t>(fscan + nn, st);
+    uint32_t* forder = P.forder.ensure<uint32_t>(nn);  // first positions of the fragments, ascending
+    compact_flags<uint8_t>(fflag, nn, forder, d_count, P.fscan, J.rs.scan_tmp, st);
+    const uint64_t nf = d2h<uint32_t>(d_count, st);
 
     // --- fragments in (bucket, seed rank) order, appended to the job stream.
     // Keys are 32-bit when the chunk's local bucket and rank bits fit (4
     // passes of 4+4 bytes instead of 6 of 8+4).
     uint64_t* fkey = P.fkey.ensure<uint64_t>(nf);
     uint64_t* fkey_t = P.fkey_t.ensure<uint64_t>(nf);
-    uint32_t* forder = P.forder.ensure<uint32_t>(nf);
     uint32_t* forder_t = P.forder_t.ensure<uint32_t>(nf);
     uint64_t max_btup = 0;
     for (uint32_t b = b0; b < b1; ++b) max_btup = std::max<uint64_t>(max_btup, h_btup[b + 1] - h_btup[b]);
@@ -2358,13 +2343,13 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (lbits + rbits <= 32 && !(force64 && *force64 == '1')) {
         uint32_t* fk32 = (uint32_t*)fkey;
         uint32_t* fk32_t = (uint32_t*)fkey_t;
-        { KSCOPE("k_frag_compact", st); k_frag_compact32<<<blocks_for(nn), TPB, 0, st>>>(nn, fflag, fscan, wa.frag_key, b0, rbits, fk32, forder, cnt + 12); }
+        if (nf) { KSCOPE("k_frag_keys", st); k_frag_keys32<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_key, b0, rbits, fk32, cnt + 12); }
         bool in_temp = false;
         segmented_radix_sort<uint32_t, uint32_t>(fk32, forder, fk32_t, forder_t, fplan, 0, lbits + rbits, 8, P.rs2, st,
                                                  "frags", &in_temp);
         if (in_temp) forder = forder_t;
     } else {
-        { KSCOPE("k_frag_compact", st); k_frag_compact<<<blocks_for(nn), TPB, 0, st>>>(nn, fflag, fscan, wa.frag_key, fkey, forder); }
+        if (nf) { KSCOPE("k_frag_keys", st); k_frag_keys<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_key, fkey); }
         segmented_radix_sort<uint64_t, uint32_t>(fkey, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
                                                  "frags");
     }

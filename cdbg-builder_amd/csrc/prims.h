@@ -379,6 +379,88 @@ inline void exclusive_scan(const Tin* in, Tout* out, uint64_t n, Tout* d_total, 
     CDBG_HIP(hipGetLastError());
 }
 // ---------------------------------------------------------------------------
+// Stream compaction: list[j] = index of the j-th nonzero flag, in order.
+// Flags are counted per 4096-flag tile, the tile counts scanned, and each
+// tile writes its indices at its base: the flags are read twice and nothing
+// of size n is written (a scan of the flags would write an n-entry array).
+// ---------------------------------------------------------------------------
+constexpr int CF_ITEMS = SCAN_ITEMS;
+constexpr int CF_TILE = SCAN_THREADS * CF_ITEMS;
+
+template <class F>
+__global__ __launch_bounds__(SCAN_THREADS) void k_flag_count(const F* __restrict__ flags, uint64_t n,
+                                                             uint32_t* __restrict__ cnt) {
+    __shared__ uint32_t s_tmp[SCAN_THREADS / 64 + 1];
+    const uint64_t T0 = uint64_t(blockIdx.x) * CF_TILE;
+    uint32_t c = 0;
+#pragma unroll
+    for (int r = 0; r < CF_ITEMS; ++r) {
+        const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
+        c += i < n && flags[i] != 0;
+    }
+    uint32_t tot;
+    (void)block_exclusive_scan<uint32_t>(c, s_tmp, tot);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+}
+
+// Round r of the tile is flags T0 + 256 r .. T0 + 256 r + 255: a flag's rank
+// is the count of the rounds before it, of the waves before it in its round
+// and of the lanes before it in its wave.
+template <class F>
+__global__ __launch_bounds__(SCAN_THREADS) void k_flag_compact(const F* __restrict__ flags, uint64_t n,
+                                                               const uint32_t* __restrict__ base,
+                                                               uint32_t* __restrict__ list) {
+    constexpr int WAVES = SCAN_THREADS / 64;
+    __shared__ uint32_t s_cnt[CF_ITEMS * WAVES];
+    const uint64_t T0 = uint64_t(blockIdx.x) * CF_TILE;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint64_t m[CF_ITEMS];
+#pragma unroll
+    for (int r = 0; r < CF_ITEMS; ++r) {
+        const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
+        m[r] = __ballot(i < n && flags[i] != 0);
+        if (lane == 0) s_cnt[r * WAVES + wave] = uint32_t(__popcll(m[r]));
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {  // exclusive scan of the CF_ITEMS * WAVES counts (<= 64) by one wave
+        const uint32_t v = threadIdx.x < CF_ITEMS * WAVES ? s_cnt[threadIdx.x] : 0u;
+        uint32_t x = v;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (threadIdx.x < CF_ITEMS * WAVES) s_cnt[threadIdx.x] = x - v;
+    }
+    __syncthreads();
+    const uint32_t b = base[blockIdx.x];
+#pragma unroll
+    for (int r = 0; r < CF_ITEMS; ++r) {
+        if ((m[r] >> lane) & 1ull) {
+            const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
+            list[b + s_cnt[r * WAVES + wave] + uint32_t(__popcll(m[r] & lt))] = uint32_t(i);
+        }
+    }
+}
+
+// Compacts the nonzero flags of [0, n) into list; *d_count (device) = their
+// number.  `cnt` is scratch of 2 (n / 4096 + 2) words.
+template <class F>
+inline void compact_flags(const F* flags, uint64_t n, uint32_t* list, uint32_t* d_count, DevBuf& cnt, DevBuf& tmp,
+                          hipStream_t st) {
+    static_assert(CF_ITEMS * (SCAN_THREADS / 64) <= 64, "one wave scans a tile's round counts");
+    uint64_t nb = (n + CF_TILE - 1) / CF_TILE;
+    if (nb == 0) nb = 1;
+    uint32_t* c = cnt.ensure<uint32_t>(2 * (nb + 1));
+    uint32_t* b = c + nb + 1;
+    { KSCOPE("k_flag_count", st); k_flag_count<F><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(flags, n, c); }
+    exclusive_scan<uint32_t, uint32_t>(c, b, nb, d_count, tmp, st);
+    { KSCOPE("k_flag_compact", st); k_flag_compact<F><<<dim3(uint32_t(nb)), SCAN_THREADS, 0, st>>>(flags, n, b, list); }
+    CDBG_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 // Segmented stable LSD radix sort.
 // ---------------------------------------------------------------------------
 constexpr int RS_THREADS = 512;
