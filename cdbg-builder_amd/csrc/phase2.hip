@@ -589,17 +589,17 @@ __global__ void k_cells_move(bool to_compact, const uint32_t* __restrict__ start
 constexpr uint32_t NT_TILE = 1024;
 __global__ __launch_bounds__(TPB) void k_tile_heads(uint64_t nt, const uint8_t* __restrict__ heads,
                                                     uint32_t* __restrict__ tile_cnt) {
-    __shared__ uint64_t s_tmp[TPB / 64 + 1];
-    const uint64_t T0 = uint64_t(blockIdx.x) * NT_TILE;
-    uint64_t c = 0;
-#pragma unroll
-    for (uint32_t r = 0; r < NT_TILE / TPB; ++r) {
-        const uint64_t t = T0 + r * TPB + threadIdx.x;
-        c += t < nt && heads[t];
-    }
-    uint64_t tot;
-    (void)block_exclusive_scan<uint64_t>(c, s_tmp, tot);
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = uint32_t(tot);
+    static_assert(NT_TILE == 4 * TPB, "one 4-byte load per thread");
+    __shared__ uint32_t s_tmp[TPB / 64 + 1];
+    const uint64_t t0 = uint64_t(blockIdx.x) * NT_TILE + 4ull * threadIdx.x;
+    uint32_t c = 0;
+    if (t0 + 4 <= nt)
+        c = nonzero_bytes(*reinterpret_cast<const uint32_t*>(heads + t0));
+    else
+        for (uint64_t t = t0; t < nt; ++t) c += heads[t] != 0;
+    uint32_t tot;
+    (void)block_exclusive_scan<uint32_t>(c, s_tmp, tot);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) {

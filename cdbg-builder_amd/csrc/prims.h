@@ -387,65 +387,108 @@ inline void exclusive_scan(const Tin* in, Tout* out, uint64_t n, Tout* d_total, 
 constexpr int CF_ITEMS = SCAN_ITEMS;
 constexpr int CF_TILE = SCAN_THREADS * CF_ITEMS;
 
+// nonzero bytes of a word
+__device__ __forceinline__ uint32_t nonzero_bytes(uint32_t w) {
+    return uint32_t(__popc((((w & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | w) & 0x80808080u));
+}
+
+// Byte flags are read 16 per lane (one 16-byte load), element i of the tile
+// at lane i / 16; wider flags one per lane per round.
 template <class F>
 __global__ __launch_bounds__(SCAN_THREADS) void k_flag_count(const F* __restrict__ flags, uint64_t n,
                                                              uint32_t* __restrict__ cnt) {
     __shared__ uint32_t s_tmp[SCAN_THREADS / 64 + 1];
     const uint64_t T0 = uint64_t(blockIdx.x) * CF_TILE;
     uint32_t c = 0;
+    if constexpr (sizeof(F) == 1) {
+        const uint64_t i0 = T0 + uint64_t(threadIdx.x) * CF_ITEMS;
+        if (i0 + CF_ITEMS <= n) {
+            const uint4 w = *reinterpret_cast<const uint4*>(flags + i0);
+            c = nonzero_bytes(w.x) + nonzero_bytes(w.y) + nonzero_bytes(w.z) + nonzero_bytes(w.w);
+        } else {
+            for (uint64_t i = i0; i < n; ++i) c += flags[i] != 0;
+        }
+    } else {
 #pragma unroll
-    for (int r = 0; r < CF_ITEMS; ++r) {
-        const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
-        c += i < n && flags[i] != 0;
+        for (int r = 0; r < CF_ITEMS; ++r) {
+            const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
+            c += i < n && flags[i] != 0;
+        }
     }
     uint32_t tot;
     (void)block_exclusive_scan<uint32_t>(c, s_tmp, tot);
     if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
-// Round r of the tile is flags T0 + 256 r .. T0 + 256 r + 255: a flag's rank
-// is the count of the rounds before it, of the waves before it in its round
-// and of the lanes before it in its wave.
+// A flag's rank in its tile is the count of flags before it: byte flags by a
+// block scan of the lanes' 16-flag counts and the flags before it in its
+// lane; wider flags by rounds (round r = flags T0 + 256 r .. T0 + 256 r +
+// 255): the rounds before it, the waves before it in its round and the lanes
+// before it in its wave.
 template <class F>
 __global__ __launch_bounds__(SCAN_THREADS) void k_flag_compact(const F* __restrict__ flags, uint64_t n,
                                                                const uint32_t* __restrict__ base,
                                                                uint32_t* __restrict__ list) {
     constexpr int WAVES = SCAN_THREADS / 64;
-    __shared__ uint32_t s_cnt[CF_ITEMS * WAVES];
     const uint64_t T0 = uint64_t(blockIdx.x) * CF_TILE;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    uint64_t m[CF_ITEMS];
-#pragma unroll
-    for (int r = 0; r < CF_ITEMS; ++r) {
-        const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
-        m[r] = __ballot(i < n && flags[i] != 0);
-        if (lane == 0) s_cnt[r * WAVES + wave] = uint32_t(__popcll(m[r]));
-    }
-    __syncthreads();
-    if (threadIdx.x < 64) {  // exclusive scan of the CF_ITEMS * WAVES counts (<= 64) by one wave
-        const uint32_t v = threadIdx.x < CF_ITEMS * WAVES ? s_cnt[threadIdx.x] : 0u;
-        uint32_t x = v;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
-        if (threadIdx.x < CF_ITEMS * WAVES) s_cnt[threadIdx.x] = x - v;
-    }
-    __syncthreads();
     const uint32_t b = base[blockIdx.x];
+    if constexpr (sizeof(F) == 1) {
+        __shared__ uint32_t s_tmp[WAVES + 1];
+        const uint64_t i0 = T0 + uint64_t(threadIdx.x) * CF_ITEMS;
+        uint32_t bits = 0;  // bit j: flag i0 + j is set
+        if (i0 + CF_ITEMS <= n) {
+            const uint4 w = *reinterpret_cast<const uint4*>(flags + i0);
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-    for (int r = 0; r < CF_ITEMS; ++r) {
-        if ((m[r] >> lane) & 1ull) {
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) bits |= uint32_t(((ws[q] >> (8 * j)) & 0xFFu) != 0) << (4 * q + j);
+        } else {
+            for (uint64_t i = i0; i < n; ++i) bits |= uint32_t(flags[i] != 0) << (i - i0);
+        }
+        uint32_t tot;
+        uint32_t at = b + block_exclusive_scan<uint32_t>(uint32_t(__popc(bits)), s_tmp, tot);
+        while (bits) {
+            const int j = __ffs(bits) - 1;
+            list[at++] = uint32_t(i0 + j);
+            bits &= bits - 1;
+        }
+    } else {
+        __shared__ uint32_t s_cnt[CF_ITEMS * WAVES];
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        uint64_t m[CF_ITEMS];
+#pragma unroll
+        for (int r = 0; r < CF_ITEMS; ++r) {
             const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
-            list[b + s_cnt[r * WAVES + wave] + uint32_t(__popcll(m[r] & lt))] = uint32_t(i);
+            m[r] = __ballot(i < n && flags[i] != 0);
+            if (lane == 0) s_cnt[r * WAVES + wave] = uint32_t(__popcll(m[r]));
+        }
+        __syncthreads();
+        if (threadIdx.x < 64) {  // exclusive scan of the CF_ITEMS * WAVES counts (<= 64) by one wave
+            const uint32_t v = threadIdx.x < CF_ITEMS * WAVES ? s_cnt[threadIdx.x] : 0u;
+            uint32_t x = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            if (threadIdx.x < CF_ITEMS * WAVES) s_cnt[threadIdx.x] = x - v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < CF_ITEMS; ++r) {
+            if ((m[r] >> lane) & 1ull) {
+                const uint64_t i = T0 + uint64_t(r) * SCAN_THREADS + threadIdx.x;
+                list[b + s_cnt[r * WAVES + wave] + uint32_t(__popcll(m[r] & lt))] = uint32_t(i);
+            }
         }
     }
 }
 
 // Compacts the nonzero flags of [0, n) into list; *d_count (device) = their
-// number.  `cnt` is scratch of 2 (n / 4096 + 2) words.
+// number.  `cnt` is scratch of 2 (n / 4096 + 2) words.  Byte flags must
+// start 16-byte aligned (device buffers are).
 template <class F>
 inline void compact_flags(const F* flags, uint64_t n, uint32_t* list, uint32_t* d_count, DevBuf& cnt, DevBuf& tmp,
                           hipStream_t st) {
