@@ -960,7 +960,7 @@ __global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restri
 }
 
 // sets with a key but no id yet: singletons [0, nc) then hash slots [0, cap)
-__global__ void k_newset_flags(uint32_t nc, SetTables T, uint32_t* __restrict__ fl) {
+__global__ void k_newset_flags(uint32_t nc, SetTables T, uint8_t* __restrict__ fl) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < nc)
         fl[i] = T.single_min[i] != ~0ull && T.single_cid[i] == NONE32;
@@ -968,17 +968,18 @@ __global__ void k_newset_flags(uint32_t nc, SetTables T, uint32_t* __restrict__ 
         fl[i] = T.mt_hk[2 * (i - nc)] != 0ull && T.mt_cid[i - nc] == NONE32;
 }
 
-__global__ void k_newset_emit(uint32_t nc, SetTables T, const uint32_t* __restrict__ fl, const uint64_t* __restrict__ pos,
+// The d-th new set (list[d] = its flag index): its key and descriptor.
+__global__ void k_newset_emit(uint64_t n_new, uint32_t nc, SetTables T, const uint32_t* __restrict__ list,
                               uint64_t* __restrict__ set_key, uint64_t* __restrict__ set_desc) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nc + T.cap || !fl[i]) return;
-    const uint64_t p = pos[i];
+    const uint64_t d = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (d >= n_new) return;
+    const uint64_t i = list[d];
     if (i < nc) {
-        set_key[p] = T.single_min[i];
-        set_desc[p] = i;
+        set_key[d] = T.single_min[i];
+        set_desc[d] = i;
     } else {
-        set_key[p] = T.mt_hk[2 * (i - nc) + 1];
-        set_desc[p] = (uint64_t(1) << 63) | (i - nc);
+        set_key[d] = T.mt_hk[2 * (i - nc) + 1];
+        set_desc[d] = (uint64_t(1) << 63) | (i - nc);
     }
 }
 
@@ -2204,16 +2205,18 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (nbig) { KSCOPE("k_set_verify_big", st); k_set_verify_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
     const uint32_t NC = J.num_colors;
     const uint64_t nflag = NC + P.mt_cap;
-    uint32_t* set_fl = P.set_fl.ensure<uint32_t>(nflag);
-    uint64_t* set_pos = P.set_pos.ensure<uint64_t>(nflag);
+    if (nflag >= (1ull << 32)) throw Error("phase 2: colour-set table too large");
+    uint8_t* set_fl = P.set_fl.ensure<uint8_t>(nflag);
+    uint32_t* set_list = P.set_pos.ensure<uint32_t>(nflag);
+    uint32_t* d_new = (uint32_t*)(dtot + 3);
     { KSCOPE("k_newset_flags", st); k_newset_flags<<<blocks_for(nflag), TPB, 0, st>>>(NC, T, set_fl); }
-    exclusive_scan<uint32_t, uint64_t>(set_fl, set_pos, nflag, dtot + 2, J.rs.scan_tmp, st);
-    const uint64_t n_new = d2h<uint64_t>(dtot + 2, st);
+    compact_flags<uint8_t>(set_fl, nflag, set_list, d_new, P.set_bits, J.rs.scan_tmp, st);
+    const uint64_t n_new = d2h<uint32_t>(d_new, st);
     uint64_t* set_key = P.set_key.ensure<uint64_t>(n_new);
     uint64_t* set_desc = P.set_desc.ensure<uint64_t>(n_new);
     uint64_t* set_key_t = P.set_key_t.ensure<uint64_t>(n_new);
     uint64_t* set_desc_t = P.set_desc_t.ensure<uint64_t>(n_new);
-    { KSCOPE("k_newset_emit", st); k_newset_emit<<<blocks_for(nflag), TPB, 0, st>>>(NC, T, set_fl, set_pos, set_key, set_desc); }
+    if (n_new) { KSCOPE("k_newset_emit", st); k_newset_emit<<<blocks_for(n_new), TPB, 0, st>>>(n_new, NC, T, set_list, set_key, set_desc); }
     SortPlan splan;
     splan.build({0, n_new});
     segmented_radix_sort<uint64_t, uint64_t>(set_key, set_desc, set_key_t, set_desc_t, splan, 0, 32 + bbits, 8, P.rs2,
