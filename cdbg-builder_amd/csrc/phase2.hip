@@ -606,20 +606,6 @@ __global__ void k_set_u32(uint32_t* p, uint32_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
-// node -> local bucket, by binary search over node_off staged in LDS
-constexpr uint32_t NB_LDS = 4096;
-__global__ void k_node_bucket(uint32_t n_nodes, const uint64_t* __restrict__ node_off, uint32_t nb,
-                              uint32_t* __restrict__ node_bucket) {
-    __shared__ uint64_t s_off[NB_LDS + 1];
-    const bool staged = nb <= NB_LDS;
-    if (staged)
-        for (uint32_t i = threadIdx.x; i <= nb; i += blockDim.x) s_off[i] = node_off[i];
-    __syncthreads();
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
-    node_bucket[n] = staged ? bucket_of(s_off, nb, n) : bucket_of(node_off, nb, n);
-}
-
 // First node of every bucket: the nodes that start before its first tuple
 // (a bucket's first tuple is a head), by binary search over the node starts.
 __global__ void k_node_off(uint32_t nb, const uint64_t* __restrict__ lbtup, const uint32_t* __restrict__ node_start,
@@ -666,6 +652,8 @@ __device__ __forceinline__ uint64_t lanes_below(int lane) { return lane == 0 ? 0
 template <class K>
 __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __restrict__ heads,
                                                const uint32_t* __restrict__ tile_base,
+                                               const uint64_t* __restrict__ lbtup, uint32_t nb,
+                                               uint32_t* __restrict__ node_bucket,
                                                uint32_t* __restrict__ node_start, const K* __restrict__ keys,
                                                const uint64_t* __restrict__ vals, K* __restrict__ node_key,
                                                uint8_t* __restrict__ node_meta, uint32_t* __restrict__ node_rank,
@@ -675,8 +663,11 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
     __shared__ uint32_t s_list[NT_TILE];
     __shared__ uint32_t s_wn[TPB / 64 + 1];
     __shared__ uint32_t s_next;  // first head in [T1, TE); nt when the tuples end first; NONE32: none
+    __shared__ uint32_t s_blo, s_bhi;  // local buckets of the tile's first and last tuples
     const uint64_t T0 = uint64_t(blockIdx.x) * NT_TILE;
     const uint64_t T1 = min(T0 + NT_TILE, nt), TE = min(T1 + BIG_RUN, nt);
+    if (threadIdx.x == 64) s_blo = bucket_of(lbtup, nb, T0);
+    if (threadIdx.x == 128) s_bhi = bucket_of(lbtup, nb, T1 - 1);
     for (uint32_t i = threadIdx.x; i < TE - T0; i += TPB) s_val[i] = vals[T0 + i];
     static_assert(BIG_RUN <= 64, "one wave looks past the tile");
     if (threadIdx.x < 64) {
@@ -725,6 +716,18 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
         const uint32_t t1 = j + 1 < n_heads ? uint32_t(T0 + s_list[j + 1]) : s_next;
         node_start[n] = t0;
         node_key[n] = keys[t0];
+        uint32_t bk = s_blo;  // a tile seldom spans two buckets
+        if (s_bhi != s_blo) {
+            uint32_t hi = s_bhi + 1;
+            while (hi - bk > 1) {
+                const uint32_t mid = (bk + hi) >> 1;
+                if (lbtup[mid] <= t0)
+                    bk = mid;
+                else
+                    hi = mid;
+            }
+        }
+        node_bucket[n] = bk;
         const bool is_big = t1 == NONE32 || t1 - t0 > BIG_RUN;
         big_flag[n] = is_big;
         if (is_big) continue;
@@ -2179,10 +2182,9 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     unsigned long long* cnt = P.counters.ensure<unsigned long long>(N_COUNTERS);
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, N_COUNTERS, 0ull); }
     uint8_t* big_flag = P.big_flag.ensure<uint8_t>(nn);
-    { KSCOPE("k_nodes", st); k_nodes<K><<<uint32_t(n_ntiles), TPB, 0, st>>>(nt, heads, tile_base, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
+    { KSCOPE("k_nodes", st); k_nodes<K><<<uint32_t(n_ntiles), TPB, 0, st>>>(nt, heads, tile_base, lbtup, nb, node_bucket, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
     { KSCOPE("k_set_u32", st); k_set_u32<<<1, 64, 0, st>>>(node_start + nn, uint32_t(nt)); }
     { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, node_start, nn, node_off); }
-    { KSCOPE("k_node_bucket", st); k_node_bucket<<<blocks_for(nn), TPB, 0, st>>>(nn, node_off, nb, node_bucket); }
     uint32_t* d_nbig = (uint32_t*)(dtot + 2);
     compact_flags<uint8_t>(big_flag, nn, big, d_nbig, P.big_scan, J.rs.scan_tmp, st);
     const uint64_t nbig = d2h<uint32_t>(d_nbig, st);
