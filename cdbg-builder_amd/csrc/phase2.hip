@@ -2190,7 +2190,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint64_t nbig = d2h<uint32_t>(d_nbig, st);
     unsigned long long h_cnt[N_COUNTERS];
     if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors); }
-    { KSCOPE("k_count_nodes", st); k_count_nodes<<<1024, TPB, 0, st>>>(nn, meta, ncol, cnt); }
+    { KSCOPE("k_count_nodes", st); k_count_nodes<<<4096, TPB, 0, st>>>(nn, meta, ncol, cnt); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     const uint64_t n_primary = h_cnt[0], n_multi = h_cnt[1];
