@@ -1840,19 +1840,6 @@ __global__ void k_frag_keys32(uint64_t nf, const uint32_t* __restrict__ forder,
 // ---------------------------------------------------------------------------
 // fragment records
 // ---------------------------------------------------------------------------
-// Record bytes of each fragment in output order (unitig_spill.hpp:491-522).
-__global__ void k_frag_size(uint64_t n_frags, const uint32_t* __restrict__ order, const uint32_t* __restrict__ frag_n,
-                            const uint32_t* __restrict__ pcid, uint32_t k, uint64_t* __restrict__ size) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_frags) return;
-    const uint32_t f = order[i];  // the fragment's first position
-    const uint64_t o = f;
-    const uint32_t n = frag_n[f];
-    uint32_t runs = 1;  // run count is the same read either way
-    for (uint32_t j = 1; j < n; ++j) runs += pcid[o + j] != pcid[o + j - 1];
-    size[i] = 1 + 4 + 12ull * runs + 4 + (k + n - 1);
-}
-
 __device__ __forceinline__ void put_bytes(uint8_t* p, uint64_t v, int n) {
     for (int i = 0; i < n; ++i) p[i] = uint8_t(v >> (8 * i));
 }
@@ -1900,42 +1887,84 @@ __device__ void write_record(uint8_t* p, uint8_t open, uint64_t o, uint32_t n, u
     }
 }
 
-// A block writes the records of TPB consecutive fragments (in output order),
-// staged in LDS when they fit and then copied out with aligned 32-bit stores.
+// Record bytes (unitig_spill.hpp:491-522) are staged in LDS up to FW_LDS per block.
 constexpr uint32_t FW_LDS = 32768;
+
+// Fragments in position order (d = the d-th fragment start, ascending): the
+// record size of each (read from consecutive positions) and d itself, the
+// value the output sort carries.
+__global__ void k_frag_psize(uint64_t nf, const uint32_t* __restrict__ fpos, const uint32_t* __restrict__ frag_n,
+                             const uint32_t* __restrict__ pcid, uint32_t k, uint32_t* __restrict__ psize,
+                             uint32_t* __restrict__ dval) {
+    const uint64_t d = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (d >= nf) return;
+    const uint64_t o = fpos[d];
+    const uint32_t n = frag_n[o];
+    uint32_t runs = 1;  // run count is the same read either way
+    for (uint32_t j = 1; j < n; ++j) runs += pcid[o + j] != pcid[o + j - 1];
+    psize[d] = 1 + 4 + 12 * runs + 4 + (k + n - 1);
+    dval[d] = uint32_t(d);
+}
+
+// osize[i] = size of the i-th fragment in output order
+__global__ void k_frag_osize(uint64_t nf, const uint32_t* __restrict__ dval, const uint32_t* __restrict__ psize,
+                             uint64_t* __restrict__ osize) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nf) osize[i] = psize[dval[i]];
+}
+
+// pboff[d] = output offset of the d-th fragment in position order
+__global__ void k_frag_pboff(uint64_t nf, const uint32_t* __restrict__ dval, const uint64_t* __restrict__ boff,
+                             uint64_t* __restrict__ pboff) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < nf) pboff[dval[i]] = boff[i];
+}
+
+// A block builds the records of TPB consecutive fragments in position order
+// (their nodes are neighbours on the path arrays, so the reads coalesce) in
+// LDS, then each wave copies whole records, one at a time with its 64 lanes,
+// to their output offsets.  Blocks whose records exceed the LDS stage write
+// them directly.
 template <class K>
-__global__ __launch_bounds__(TPB) void k_frag_write(uint64_t n_frags, const uint32_t* __restrict__ order,
-                                                    const uint32_t* __restrict__ frag_n,
-                                                    const uint8_t* __restrict__ frag_open,
-                                                    const uint8_t* __restrict__ frag_dir,
-                                                    const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
-                                                    const uint32_t* __restrict__ pcid, uint32_t k,
-                                                    const uint64_t* __restrict__ boff, uint8_t* __restrict__ out) {
+__global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint32_t* __restrict__ fpos,
+                                                        const uint32_t* __restrict__ frag_n,
+                                                        const uint8_t* __restrict__ frag_open,
+                                                        const uint8_t* __restrict__ frag_dir,
+                                                        const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
+                                                        const uint32_t* __restrict__ pcid, uint32_t k,
+                                                        const uint32_t* __restrict__ psize,
+                                                        const uint64_t* __restrict__ pboff, uint8_t* __restrict__ out) {
     __shared__ uint8_t s_out[FW_LDS];
-    const uint64_t i0 = uint64_t(blockIdx.x) * TPB;
-    const uint64_t i1 = min(i0 + TPB, n_frags);
-    const uint64_t b0 = boff[i0], b1 = boff[i1];
-    const bool staged = b1 - b0 <= FW_LDS;
-    const uint64_t i = i0 + threadIdx.x;
-    if (i < i1) {
-        const uint32_t f = order[i];
-        uint8_t* p = staged ? s_out + (boff[i] - b0) : out + boff[i];
+    __shared__ uint32_t s_loff[TPB], s_size[TPB];
+    __shared__ uint64_t s_dst[TPB];
+    __shared__ uint32_t s_tmp[TPB / 64 + 1];
+    const uint64_t d = uint64_t(blockIdx.x) * TPB + threadIdx.x;
+    const bool live = d < nf;
+    const uint32_t sz = live ? psize[d] : 0u;
+    uint32_t total;
+    const uint32_t loff = block_exclusive_scan<uint32_t>(sz, s_tmp, total);
+    const bool staged = total <= FW_LDS;
+    if (live) {
+        const uint32_t f = fpos[d];
+        const uint64_t dst = pboff[d];
+        uint8_t* p = staged ? s_out + loff : out + dst;
         write_record<K>(p, frag_open[f], f, frag_n[f], frag_dir[f], frc, pkey, pcid, k);
+        s_loff[threadIdx.x] = loff;
+        s_size[threadIdx.x] = sz;
+        s_dst[threadIdx.x] = dst;
+    } else {
+        s_size[threadIdx.x] = 0;
     }
     if (!staged) return;
     __syncthreads();
-    const uint32_t len = uint32_t(b1 - b0);
-    const uint32_t mis = uint32_t((4 - (reinterpret_cast<uintptr_t>(out + b0) & 3)) & 3);
-    const uint32_t head = mis < len ? mis : len;
-    if (threadIdx.x < head) out[b0 + threadIdx.x] = s_out[threadIdx.x];
-    const uint32_t words = (len - head) / 4;
-    uint32_t* ow = (uint32_t*)(out + b0 + head);
-    for (uint32_t w = threadIdx.x; w < words; w += TPB) {
-        const uint8_t* q = s_out + head + 4 * w;
-        ow[w] = uint32_t(q[0]) | (uint32_t(q[1]) << 8) | (uint32_t(q[2]) << 16) | (uint32_t(q[3]) << 24);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = wave; r < TPB; r += TPB / 64) {
+        const uint32_t n = s_size[r];
+        if (n == 0) continue;
+        const uint8_t* src = s_out + s_loff[r];
+        uint8_t* dst = out + s_dst[r];
+        for (uint32_t j = uint32_t(lane); j < n; j += 64) dst[j] = src[j];
     }
-    const uint32_t tail = head + 4 * words;
-    if (threadIdx.x < len - tail) out[b0 + tail + threadIdx.x] = s_out[tail + threadIdx.x];
 }
 
 }  // namespace
@@ -1960,7 +1989,7 @@ struct Phase2Bufs {
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fscan, fkey;
     DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
     DevBuf node_rec, node_mm, eflag, escan, ends;
-    DevBuf fkey_t, forder, forder_t, fsize, fboff;
+    DevBuf fkey_t, forder, forder_t, fdval, fpsize, fpboff, fsize, fboff;
     RadixScratch rs2;
     // results
     uint64_t n_tuples = 0, n_nodes = 0, n_sets = 0, n_frags = 0, frag_total = 0, cs_bits = 0;
@@ -2338,7 +2367,10 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     // passes of 4+4 bytes instead of 6 of 8+4).
     uint64_t* fkey = P.fkey.ensure<uint64_t>(nf);
     uint64_t* fkey_t = P.fkey_t.ensure<uint64_t>(nf);
-    uint32_t* forder_t = P.forder_t.ensure<uint32_t>(nf);
+    uint32_t* dval = P.fdval.ensure<uint32_t>(nf);  // sorted: output rank -> position-order index
+    uint32_t* dval_t = P.forder_t.ensure<uint32_t>(nf);
+    uint32_t* psize = P.fpsize.ensure<uint32_t>(nf);
+    if (nf) { KSCOPE("k_frag_psize", st); k_frag_psize<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, pcid, k, psize, dval); }
     uint64_t max_btup = 0;
     for (uint32_t b = b0; b < b1; ++b) max_btup = std::max<uint64_t>(max_btup, h_btup[b + 1] - h_btup[b]);
     const int rbits = bits_for(max_btup), lbits = bits_for(b1 - b0);
@@ -2350,21 +2382,25 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         uint32_t* fk32_t = (uint32_t*)fkey_t;
         if (nf) { KSCOPE("k_frag_keys", st); k_frag_keys32<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_key, b0, rbits, fk32, cnt + 12); }
         bool in_temp = false;
-        segmented_radix_sort<uint32_t, uint32_t>(fk32, forder, fk32_t, forder_t, fplan, 0, lbits + rbits, 8, P.rs2, st,
+        segmented_radix_sort<uint32_t, uint32_t>(fk32, dval, fk32_t, dval_t, fplan, 0, lbits + rbits, 8, P.rs2, st,
                                                  "frags", &in_temp);
-        if (in_temp) forder = forder_t;
+        if (in_temp) dval = dval_t;
     } else {
         if (nf) { KSCOPE("k_frag_keys", st); k_frag_keys<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_key, fkey); }
-        segmented_radix_sort<uint64_t, uint32_t>(fkey, forder, fkey_t, forder_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
+        segmented_radix_sort<uint64_t, uint32_t>(fkey, dval, fkey_t, dval_t, fplan, 0, 32 + bbits, 8, P.rs2, st,
                                                  "frags");
     }
+    // record sizes in output order, their offsets, and the offsets in
+    // position order; the records are then built in position order
     uint64_t* fsize = P.fsize.ensure<uint64_t>(nf);
     uint64_t* fboff = P.fboff.ensure<uint64_t>(nf + 1);
-    if (nf) { KSCOPE("k_frag_size", st); k_frag_size<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, pcid, k, fsize); }
+    uint64_t* pboff = P.fpboff.ensure<uint64_t>(nf);
+    if (nf) { KSCOPE("k_frag_osize", st); k_frag_osize<<<blocks_for(nf), TPB, 0, st>>>(nf, dval, psize, fsize); }
     exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
+    if (nf) { KSCOPE("k_frag_pboff", st); k_frag_pboff<<<blocks_for(nf), TPB, 0, st>>>(nf, dval, fboff, pboff); }
     const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
     uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st);
-    if (nf) { KSCOPE("k_frag_write", st); k_frag_write<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, fboff, fout + P.frag_total); }
+    if (nf) { KSCOPE("k_frag_write", st); k_frag_write_pos<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout + P.frag_total); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
