@@ -311,8 +311,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU sharding is design-only "
-              "(see DESIGN.md) and no N>1 figure is produced.", file=sys.stderr)
+        print("bench.py: runs on one GPU only (buckets would shard across GPUs by size, with one "
+              "all-to-all for colour-set interning; not built), so no figure for N > 1 is produced.",
+              file=sys.stderr)
         sys.exit(2)
 
     G = load_pkg()
