@@ -26,9 +26,10 @@
 // Kernels per chunk (tuple = one k-mer occurrence):
 //   k_expand<K>               thread per tuple: canonical k-mer straight from
 //                             the 2-bit window, rank|colour|owned|contrib
-//   segmented radix sort      tuples by k-mer inside each bucket
-//   heads (from the cell sort) / scan / k_node_* nodes: flags, primary,
-//                             min rank, owned colours
+//   tuple sort                global radix passes on each bucket's top bits,
+//                             then LDS cell sorts, which flag the node heads
+//   k_tile_heads / k_nodes    node per run of equal keys, numbered per tile:
+//                             flags, primary, min rank, owned colours
 //   k_set_*                   colour-set interning: singleton table keyed by
 //                             colour, content-hash table for larger sets with
 //                             an exact list compare against the stored list;
@@ -37,9 +38,13 @@
 //   k_dir_build<K>            per-bucket directory over the sorted node keys
 //   k_masks<K> / k_links      8-neighbour ext masks (local_ext_mask :260-271)
 //                             and clean links (walk_step :317-336)
-//   k_walk_paths/cycles       one thread per component emulates the
-//                             rank-ordered seed loop
+//   k_walk_ends/record/frags  path ends walk their paths, paths are recorded
+//   (+ k_walk_other/cycles)   end to end, the rank-ordered seed loop runs over
+//                             the recorded positions (general loop for hairpin
+//                             paths and cycles)
 //   fragment sort + k_frag_*  frag_unitigs.bin records (unitig_spill.hpp:491-522)
+// Lists of flagged items (path ends, fragments, long runs, new sets) come
+// from stream compaction (compact_flags).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1984,13 +1989,13 @@ struct Phase2Bufs {
     // per chunk (reused)
     DevBuf keys, vals, keys_t, vals_t, block_super;
     DevBuf cell_start, cell_base, cell_dbits, cell_mid, cell_mid2, cell_over, cell_novf, cell_oflag, cell_oscan, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
-    DevBuf heads, hscan, node_start, node_off;
-    DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_scan, colors, counters;
+    DevBuf heads, tile_heads, node_start, node_off;
+    DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_cnt, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
     DevBuf node_bucket, dir, filt, node_mask, nb, link, visited;
-    DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fscan, fkey;
+    DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fcnt, fkey;
     DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
-    DevBuf node_rec, node_mm, eflag, escan, ends;
+    DevBuf node_rec, node_mm, eflag, ecnt, ends;
     DevBuf fkey_t, forder, forder_t, fdval, fpsize, fpboff, fsize, fboff;
     RadixScratch rs2;
     // results
@@ -2191,7 +2196,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     // --- nodes
     // node numbers: heads counted per k_nodes tile and scanned over the tiles
     const uint64_t n_ntiles = (nt + NT_TILE - 1) / NT_TILE;
-    uint32_t* tile_cnt = P.hscan.ensure<uint32_t>(2 * (n_ntiles + 1));
+    uint32_t* tile_cnt = P.tile_heads.ensure<uint32_t>(2 * (n_ntiles + 1));
     uint32_t* tile_base = tile_cnt + n_ntiles + 1;
     uint64_t* dtot = P.total.ensure<uint64_t>(8);
     uint32_t* dtot32 = (uint32_t*)(dtot + 1);
@@ -2217,7 +2222,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     { KSCOPE("k_set_u32", st); k_set_u32<<<1, 64, 0, st>>>(node_start + nn, uint32_t(nt)); }
     { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, node_start, nn, node_off); }
     uint32_t* d_nbig = (uint32_t*)(dtot + 2);
-    compact_flags<uint8_t>(big_flag, nn, big, d_nbig, P.big_scan, J.rs.scan_tmp, st);
+    compact_flags<uint8_t>(big_flag, nn, big, d_nbig, P.big_cnt, J.rs.scan_tmp, st);
     const uint64_t nbig = d2h<uint32_t>(d_nbig, st);
     unsigned long long h_cnt[N_COUNTERS];
     if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors); }
@@ -2331,7 +2336,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint32_t* ends = P.ends.ensure<uint32_t>(nn);
     uint32_t* d_count = (uint32_t*)(P.total.ensure<uint64_t>(8) + 2);
     { KSCOPE("k_end_flags", st); k_end_flags<<<blocks_for(nn), TPB, 0, st>>>(nn, link, eflag); }
-    compact_flags<uint8_t>(eflag, nn, ends, d_count, P.escan, J.rs.scan_tmp, st);
+    compact_flags<uint8_t>(eflag, nn, ends, d_count, P.ecnt, J.rs.scan_tmp, st);
     const uint32_t n_ends = d2h<uint32_t>(d_count, st);
     wa.comp_x = ends;
     wa.comp_ex = P.comp_ex.ensure<uint8_t>(n_ends);
@@ -2361,7 +2366,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         throw Error("phase 2: walk covered " + std::to_string(h_cnt[3]) + " of " + std::to_string(nn) +
                     " nodes (" + std::to_string(h_cnt[5]) + " errors)");
     uint32_t* forder = P.forder.ensure<uint32_t>(nn);  // first positions of the fragments, ascending
-    compact_flags<uint8_t>(fflag, nn, forder, d_count, P.fscan, J.rs.scan_tmp, st);
+    compact_flags<uint8_t>(fflag, nn, forder, d_count, P.fcnt, J.rs.scan_tmp, st);
     const uint64_t nf = d2h<uint32_t>(d_count, st);
 
     // --- fragments in (bucket, seed rank) order, appended to the job stream.
