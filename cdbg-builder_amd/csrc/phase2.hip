@@ -1121,11 +1121,12 @@ __global__ void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_bas
 }
 
 // Copies the lists of the chunk's new multi-colour sets into the job-wide
-// store (one wave per set) and points their slots at the copies.
+// store (STORE_LANES lanes per set) and points their slots at the copies.
+constexpr int STORE_LANES = 16;  // lanes per set (most lists are short)
 __global__ __launch_bounds__(TPB) void k_store_new(uint64_t n_new, const uint64_t* __restrict__ set_desc,
                                                    const uint64_t* __restrict__ keep_off, uint64_t base, SetTables T) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & (STORE_LANES - 1);
+    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / STORE_LANES;
     if (i >= n_new) return;
     const uint64_t d = set_desc[i];
     if (!(d >> 63)) return;
@@ -1133,7 +1134,7 @@ __global__ __launch_bounds__(TPB) void k_store_new(uint64_t n_new, const uint64_
     const uint32_t* src = slot_list(T, s);
     const uint32_t n = T.mt_llen[s];
     uint32_t* dst = T.store + base + keep_off[i];
-    for (uint32_t j = uint32_t(lane); j < n; j += 64) dst[j] = src[j];
+    for (uint32_t j = uint32_t(lane); j < n; j += STORE_LANES) dst[j] = src[j];
     if (lane == 0) T.mt_loff[s] = base + keep_off[i];
 }
 
@@ -1970,7 +1971,7 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
         if (n == 0) continue;
         const uint8_t* src = s_out + s_loff[r];
         uint8_t* dst = out + s_dst[r];
-        for (uint32_t j = uint32_t(lane); j < n; j += 64) dst[j] = src[j];
+        for (uint32_t j = uint32_t(lane); j < n; j += STORE_LANES) dst[j] = src[j];
     }
 }
 
@@ -2285,7 +2286,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
     NodeRec<K>* node_rec = P.node_rec.ensure<NodeRec<K>>(nn);
     { KSCOPE("k_node_cid", st); k_node_cid<K><<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, node_key, rank, node_rec); }
-    if (n_new) { KSCOPE("k_store_new", st); k_store_new<<<blocks_for(n_new, TPB / 64), TPB, 0, st>>>(n_new, set_desc, keep_off, P.store_used, T); }
+    if (n_new) { KSCOPE("k_store_new", st); k_store_new<<<blocks_for(n_new, TPB / STORE_LANES), TPB, 0, st>>>(n_new, set_desc, keep_off, P.store_used, T); }
     P.store_used += kept;
     P.n_sets += n_new;
     P.cs_bits += new_bits;
