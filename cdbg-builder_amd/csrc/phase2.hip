@@ -1369,15 +1369,19 @@ __global__ __attribute__((amdgpu_waves_per_eu(sizeof(K) == 8 ? 8 : 6))) void k_m
 
 __device__ __forceinline__ uint32_t nib(uint32_t m, uint32_t port) { return port == PORT_R ? (m & 0xfu) : (m >> 4); }
 
+// Also flags the path ends: nodes with at most one link.
 __global__ void k_links(uint32_t n_nodes, const uint8_t* __restrict__ mask, const uint32_t* __restrict__ nb,
-                        uint32_t* __restrict__ link) {
+                        uint32_t* __restrict__ link, uint8_t* __restrict__ end_flag) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
+    bool end = false;
     for (uint32_t p = 0; p < 2; ++p) {
         uint32_t l = nb[2 * n + p];
         if (l != NONE32 && __popc(nib(mask[l >> 1], l & 1u)) != 1) l = NONE32;
         link[2 * n + p] = l;
+        end |= l == NONE32;
     }
+    end_flag[n] = end ? 1 : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1566,12 +1570,6 @@ __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, 
         seed = best;
     }
     atomicAdd(&a.counters[5], 1ull);
-}
-
-// Path ends (a node with at most one link) in node order.
-__global__ void k_end_flags(uint32_t n_nodes, const uint32_t* __restrict__ link, uint8_t* __restrict__ flag) {
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n < n_nodes) flag[n] = (link[2 * n + PORT_L] == NONE32 || link[2 * n + PORT_R] == NONE32) ? 1 : 0;
 }
 
 // Path ends walk to the other end; the end with the smaller index owns the
@@ -2301,7 +2299,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
     uint16_t* node_mm = P.node_mm.ensure<uint16_t>(nn);
     { KSCOPE("k_masks", st); k_masks<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, node_off, dir, filt, k, meta, node_mask, node_mm, nbr); }
-    { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link); }
+    uint8_t* eflag = P.eflag.ensure<uint8_t>(nn);
+    { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link, eflag); }
 
     // --- walk
     uint8_t* visited = P.visited.ensure<uint8_t>(nn);
@@ -2333,10 +2332,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.pcid = pcid;
     wa.node_rec = node_rec;
     wa.node_mm = node_mm;
-    uint8_t* eflag = P.eflag.ensure<uint8_t>(nn);
     uint32_t* ends = P.ends.ensure<uint32_t>(nn);
     uint32_t* d_count = (uint32_t*)(P.total.ensure<uint64_t>(8) + 2);
-    { KSCOPE("k_end_flags", st); k_end_flags<<<blocks_for(nn), TPB, 0, st>>>(nn, link, eflag); }
     compact_flags<uint8_t>(eflag, nn, ends, d_count, P.ecnt, J.rs.scan_tmp, st);
     const uint32_t n_ends = d2h<uint32_t>(d_count, st);
     wa.comp_x = ends;
