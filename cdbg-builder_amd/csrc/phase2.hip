@@ -881,18 +881,26 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
                              const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
                              const uint64_t* __restrict__ node_hash, const uint32_t* __restrict__ rank,
                              const uint32_t* __restrict__ node_bucket, uint32_t b0, SetTables T,
-                             uint32_t* __restrict__ node_slot) {
+                             uint32_t* __restrict__ node_slot, uint8_t* __restrict__ created) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes || !(meta[n] & 4u)) return;
+    if (n >= n_nodes) return;
+    if (!(meta[n] & 4u)) {
+        created[n] = 0;
+        return;
+    }
     const uint32_t b = b0 + node_bucket[n];
     const unsigned long long key = (uint64_t(b) << 32) | rank[n];
     const uint32_t nc = ncol[n];
     const uint32_t* c = colors + coff[n];
     if (nc == 1) {
-        // keys only decrease: a plain read that is already <= key skips the atomic
-        if (T.single_min[c[0]] > key) atomicMin(&T.single_min[c[0]], key);
+        // keys only decrease: a plain read that is already <= key skips the
+        // atomic; the one thread that finds the slot empty creates the set
+        bool made = false;
+        if (T.single_min[c[0]] > key) made = atomicMin(&T.single_min[c[0]], key) == ~0ull;
+        created[n] = made ? 1 : 0;
         return;
     }
+    bool made = false;
     const unsigned long long h = node_hash[n];
     uint64_t s = h & (T.cap - 1);
     for (uint64_t probe = 0; probe < T.cap; ++probe) {
@@ -903,15 +911,18 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
                 T.mt_loff[s] = IN_CHUNK | coff[n];
                 T.mt_llen[s] = nc;
                 cur = h;
+                made = true;
             }
         }
         if (cur == h) {
             if (T.mt_hk[2 * s + 1] > key) atomicMin(&T.mt_hk[2 * s + 1], key);
             node_slot[n] = uint32_t(s);
+            created[n] = made ? 1 : 0;
             return;
         }
         s = (s + 1) & (T.cap - 1);
     }
+    created[n] = 0;  // table full (cannot happen at the load kept below 1/2)
 }
 
 // Exact check of every multi-colour node's list against the list stored with
@@ -967,27 +978,23 @@ __global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restri
     }
 }
 
-// sets with a key but no id yet: singletons [0, nc) then hash slots [0, cap)
-__global__ void k_newset_flags(uint32_t nc, SetTables T, uint8_t* __restrict__ fl) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < nc)
-        fl[i] = T.single_min[i] != ~0ull && T.single_cid[i] == NONE32;
-    else if (i < nc + T.cap)
-        fl[i] = T.mt_hk[2 * (i - nc)] != 0ull && T.mt_cid[i - nc] == NONE32;
-}
-
-// The d-th new set (list[d] = its flag index): its key and descriptor.
-__global__ void k_newset_emit(uint64_t n_new, uint32_t nc, SetTables T, const uint32_t* __restrict__ list,
-                              uint64_t* __restrict__ set_key, uint64_t* __restrict__ set_desc) {
+// The sets created in this chunk, one per listed creating node: the
+// singleton {c} or the node's hash slot, with the set's min key.
+__global__ void k_newset_emit(uint64_t n_new, const uint32_t* __restrict__ list, const uint32_t* __restrict__ ncol,
+                              const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                              const uint32_t* __restrict__ node_slot, SetTables T, uint64_t* __restrict__ set_key,
+                              uint64_t* __restrict__ set_desc) {
     const uint64_t d = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (d >= n_new) return;
-    const uint64_t i = list[d];
-    if (i < nc) {
-        set_key[d] = T.single_min[i];
-        set_desc[d] = i;
+    const uint32_t n = list[d];
+    if (ncol[n] == 1) {
+        const uint32_t c = colors[coff[n]];
+        set_key[d] = T.single_min[c];
+        set_desc[d] = c;
     } else {
-        set_key[d] = T.mt_hk[2 * (i - nc) + 1];
-        set_desc[d] = (uint64_t(1) << 63) | (i - nc);
+        const uint32_t s = node_slot[n];
+        set_key[d] = T.mt_hk[2 * uint64_t(s) + 1];
+        set_desc[d] = (uint64_t(1) << 63) | s;
     }
 }
 
@@ -2237,23 +2244,20 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     SetTables T = set_tables(P);
     T.cur_colors = colors;
     uint32_t* node_slot = P.node_slot.ensure<uint32_t>(nn);
-    { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, nhash, rank, node_bucket, b0, T, node_slot); }
+    uint8_t* created = P.set_fl.ensure<uint8_t>(nn);  // 1: the node created its set in this chunk
+    { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, nhash, rank, node_bucket, b0, T, node_slot, created); }
     { KSCOPE("k_set_verify", st); k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
     if (nbig) { KSCOPE("k_set_verify_big", st); k_set_verify_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
     const uint32_t NC = J.num_colors;
-    const uint64_t nflag = NC + P.mt_cap;
-    if (nflag >= (1ull << 32)) throw Error("phase 2: colour-set table too large");
-    uint8_t* set_fl = P.set_fl.ensure<uint8_t>(nflag);
-    uint32_t* set_list = P.set_pos.ensure<uint32_t>(nflag);
+    uint32_t* set_list = P.set_pos.ensure<uint32_t>(nn);
     uint32_t* d_new = (uint32_t*)(dtot + 3);
-    { KSCOPE("k_newset_flags", st); k_newset_flags<<<blocks_for(nflag), TPB, 0, st>>>(NC, T, set_fl); }
-    compact_flags<uint8_t>(set_fl, nflag, set_list, d_new, P.set_bits, J.rs.scan_tmp, st);
+    compact_flags<uint8_t>(created, nn, set_list, d_new, P.set_bits, J.rs.scan_tmp, st);
     const uint64_t n_new = d2h<uint32_t>(d_new, st);
     uint64_t* set_key = P.set_key.ensure<uint64_t>(n_new);
     uint64_t* set_desc = P.set_desc.ensure<uint64_t>(n_new);
     uint64_t* set_key_t = P.set_key_t.ensure<uint64_t>(n_new);
     uint64_t* set_desc_t = P.set_desc_t.ensure<uint64_t>(n_new);
-    if (n_new) { KSCOPE("k_newset_emit", st); k_newset_emit<<<blocks_for(n_new), TPB, 0, st>>>(n_new, NC, T, set_list, set_key, set_desc); }
+    if (n_new) { KSCOPE("k_newset_emit", st); k_newset_emit<<<blocks_for(n_new), TPB, 0, st>>>(n_new, set_list, ncol, coff, colors, node_slot, T, set_key, set_desc); }
     SortPlan splan;
     splan.build({0, n_new});
     segmented_radix_sort<uint64_t, uint64_t>(set_key, set_desc, set_key_t, set_desc_t, splan, 0, 32 + bbits, 8, P.rs2,
