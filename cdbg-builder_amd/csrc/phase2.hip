@@ -1712,15 +1712,43 @@ __global__ __launch_bounds__(TPB) void k_walk_record(WalkArgs a, uint64_t n_comp
 // backward then forward along the path (positions -1 / +1 when the seed lies
 // forward on the path, +1 / -1 otherwise), stopping before a placed position
 // or after a contig-break node.
-__global__ void k_walk_frags(WalkArgs a, uint64_t n_comp) {
+// The paths of a wave's components occupy one contiguous range of positions;
+// when it holds at most FR_CAP positions their orientation / info / rank are
+// staged in LDS (coalesced loads) and the seed loops run there, else on the
+// global arrays.  Placement marks (bit 1 of the orientation byte) live where
+// the loop runs; only the fragment descriptors are written out.
+constexpr uint32_t FR_CAP = 512;
+__global__ __launch_bounds__(TPB) void k_walk_frags(WalkArgs a, uint64_t n_comp) {
+    __shared__ uint8_t s_rc[TPB / 64][FR_CAP];
+    __shared__ uint16_t s_info[TPB / 64][FR_CAP];
+    __shared__ uint32_t s_rk[TPB / 64][FR_CAP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (c >= n_comp) return;
-    const uint32_t L = a.comp_len[c];
+    const bool valid = c < n_comp;
+    const uint32_t L = valid ? a.comp_len[c] : 0u;
+    const uint64_t off = valid ? a.comp_off[c] : 0ull;
+    const uint64_t base = __shfl(off, 0, 64);  // offsets grow with the lane
+    uint64_t last = valid ? off + L : 0ull;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t y = __shfl_xor(last, o, 64);
+        last = y > last ? y : last;
+    }
+    const bool staged = last - base <= FR_CAP;  // the same for the whole wave
+    if (staged) {
+        const uint32_t total = uint32_t(last - base);
+        for (uint32_t j = uint32_t(lane); j < total; j += 64) {
+            s_rc[wave][j] = a.prc[base + j];
+            s_info[wave][j] = a.pinfo[base + j];
+            s_rk[wave][j] = a.prank[base + j];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
     if (L == 0) return;
-    const uint64_t off = a.comp_off[c];
-    uint8_t* rc = a.prc + off;
-    const uint16_t* info = a.pinfo + off;
-    const uint32_t* rk = a.prank + off;
+    uint8_t* rc = staged ? &s_rc[wave][off - base] : a.prc + off;
+    const uint16_t* info = staged ? &s_info[wave][off - base] : a.pinfo + off;
+    const uint32_t* rk = staged ? &s_rk[wave][off - base] : a.prank + off;
     const uint32_t b = a.b0 + a.node_bucket[a.comp_x[c]];
     uint32_t placed = 0;
     while (placed < L) {
