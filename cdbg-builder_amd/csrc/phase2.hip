@@ -88,11 +88,6 @@ __device__ __forceinline__ uint32_t bucket_of(const uint64_t* off, uint32_t B, u
     return lo;
 }
 
-__global__ void k_iota(uint32_t* p, uint64_t n) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = uint32_t(i);
-}
-
 template <class T>
 __global__ void k_fill(T* p, uint64_t n, T v) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
