@@ -2116,7 +2116,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
 
     // --- sort tuples by k-mer inside every bucket (stable): the top D_max
     // bits by one or two global passes, which cut every bucket b into 2^D_b
-    // cells of 500 - 1000 keys (measured best on MI355X against 350-700 and
+    // cells of 500 - 1000 keys with 64-bit keys, 350 - 700 with 128-bit
+    // ones (measured best on MI355X against 350-700 and
     // 700-1400: larger cells amortise the per-cell passes, smaller ones fill
     // the CUs; an 11-bit single global pass scatters too slowly; D_b follows
     // the bucket's size, so skewed buckets get cells of the same size), then
@@ -2127,11 +2128,14 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     // the cell count of small buckets; with 64-bit keys D_b >= 2k - 51 when
     // the largest bucket needs it, so the packed cell sort applies to all.
     const int kb = 2 * int(k);
-    static const double cell_target = [] {  // mean keys per cell (CDBG_CELL_TARGET overrides)
+    // mean keys per cell (CDBG_CELL_TARGET overrides): 1000 for 64-bit keys,
+    // 700 for 128-bit ones (whose 1024-key LDS class the larger cells overflow)
+    static const double cell_target_env = [] {
         const char* e = std::getenv("CDBG_CELL_TARGET");
         const double v = e ? std::atof(e) : 0.0;
-        return v >= 16.0 ? v : 1000.0;
+        return v >= 16.0 ? v : 0.0;
     }();
+    const double cell_target = cell_target_env > 0.0 ? cell_target_env : (sizeof(K) == 8 ? 1000.0 : 700.0);
     std::vector<uint8_t> h_dbits(nb);
     int D_max = 0;
     for (uint32_t b = 0; b < nb; ++b) {
