@@ -36,8 +36,10 @@
 //                             new sets sorted by key, ids assigned, hybrid
 //                             encoded (hybrid_color_sets.hpp:41-96)
 //   k_dir_build<K>            per-bucket directory over the sorted node keys
-//   k_masks<K> / k_links      8-neighbour ext masks (local_ext_mask :260-271)
-//                             and clean links (walk_step :317-336)
+//   k_masks<K> / k_links<K>   8-neighbour ext masks (local_ext_mask :260-271)
+//                             and clean links (walk_step :317-336), kept with
+//                             the k-mer, rank and colour-set id in one record
+//                             per node (one gather per walked position)
 //   k_walk_ends/record/frags  path ends walk their paths, paths are recorded
 //   (+ k_walk_other/cycles)   end to end, the rank-ordered seed loop runs over
 //                             the recorded positions (general loop for hairpin
@@ -1140,26 +1142,16 @@ __global__ __launch_bounds__(TPB) void k_store_new(uint64_t n_new, const uint64_
     if (lane == 0) T.mt_loff[s] = base + keep_off[i];
 }
 
-// What the walk gathers per node besides its meta and mask.
+// Everything the walk reads of a node, in one record: a path position
+// gathers one cache line (32 B for 64-bit keys, 64 B for 128-bit keys).
 template <class K>
-struct NodeRec {
+struct alignas(sizeof(K) == 8 ? 32 : 64) NodeRec {
     K key;
     uint32_t rank;
     uint32_t cid;
+    uint32_t link[2];  // per port: (neighbour << 1) | entry port, NONE32 when not a unitig link
+    uint16_t mm;       // meta | mask << 8
 };
-
-// Colour-set id of every node, stored with its k-mer and seed rank.
-template <class K>
-__global__ void k_node_cid(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
-                           const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
-                           const uint32_t* __restrict__ node_slot, SetTables T, const K* __restrict__ node_key,
-                           const uint32_t* __restrict__ rank, NodeRec<K>* __restrict__ rec) {
-    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes) return;
-    uint32_t cid = NONE32;
-    if (meta[n] & 4u) cid = ncol[n] == 1 ? T.single_cid[colors[coff[n]]] : T.mt_cid[node_slot[n]];
-    rec[n] = NodeRec<K>{node_key[n], rank[n], cid};
-}
 
 // ---------------------------------------------------------------------------
 // per-bucket directory over the sorted node keys, masks, links
@@ -1340,9 +1332,8 @@ __device__ __forceinline__ uint32_t side_lookup(const K& x, uint32_t k, const K*
 template <class K>
 __global__ __attribute__((amdgpu_waves_per_eu(sizeof(K) == 8 ? 8 : 6))) void k_masks(uint32_t n_nodes, const K* __restrict__ node_key, const uint32_t* __restrict__ node_bucket,
                         const uint64_t* __restrict__ node_off, const uint32_t* __restrict__ dir,
-                        const uint32_t* __restrict__ filt, uint32_t k,
-                        const uint8_t* __restrict__ meta, uint8_t* __restrict__ mask_out,
-                        uint16_t* __restrict__ node_mm, uint32_t* __restrict__ nb) {
+                        const uint32_t* __restrict__ filt, uint32_t k, uint8_t* __restrict__ mask_out,
+                        uint32_t* __restrict__ nb) {
     const uint32_t nblk = (n_nodes + TPB - 1) / TPB, blk = tile_of(blockIdx.x, nblk);
     const uint64_t n = uint64_t(blk) * TPB + threadIdx.x;
     if (blk >= nblk || n >= n_nodes) return;
@@ -1364,26 +1355,45 @@ __global__ __attribute__((amdgpu_waves_per_eu(sizeof(K) == 8 ? 8 : 6))) void k_m
     }
     const uint32_t mask = mR | (mL << 4);
     mask_out[n] = uint8_t(mask);
-    node_mm[n] = uint16_t(meta[n] | (mask << 8));
     nb[2 * n + PORT_R] = __popc(mR) == 1 ? (sR << 1) | pR : NONE32;
     nb[2 * n + PORT_L] = __popc(mL) == 1 ? (sL << 1) | pL : NONE32;
 }
 
 __device__ __forceinline__ uint32_t nib(uint32_t m, uint32_t port) { return port == PORT_R ? (m & 0xfu) : (m >> 4); }
 
-// Also flags the path ends: nodes with at most one link.
+// Keeps a neighbour as a link when the neighbour's entry side has exactly one
+// extension too, and assembles the node's walk record (k-mer, seed rank,
+// colour-set id, links, meta and mask); the links are also stored alone, 8 B
+// per node, for the walks that read nothing else (a bucket's links stay in
+// cache).  Also flags the path ends: nodes with at most one link.
+template <class K>
 __global__ void k_links(uint32_t n_nodes, const uint8_t* __restrict__ mask, const uint32_t* __restrict__ nb,
+                        const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
+                        const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
+                        const uint32_t* __restrict__ node_slot, SetTables T, const K* __restrict__ node_key,
+                        const uint32_t* __restrict__ rank, NodeRec<K>* __restrict__ rec,
                         uint32_t* __restrict__ link, uint8_t* __restrict__ end_flag) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
+    NodeRec<K> r;
     bool end = false;
+#pragma unroll
     for (uint32_t p = 0; p < 2; ++p) {
         uint32_t l = nb[2 * n + p];
         if (l != NONE32 && __popc(nib(mask[l >> 1], l & 1u)) != 1) l = NONE32;
+        r.link[p] = l;
         link[2 * n + p] = l;
         end |= l == NONE32;
     }
     end_flag[n] = end ? 1 : 0;
+    const uint8_t mt = meta[n];
+    uint32_t cid = NONE32;
+    if (mt & 4u) cid = ncol[n] == 1 ? T.single_cid[colors[coff[n]]] : T.mt_cid[node_slot[n]];
+    r.key = node_key[n];
+    r.rank = rank[n];
+    r.cid = cid;
+    r.mm = uint16_t(mt | (uint32_t(mask[n]) << 8));
+    rec[n] = r;
 }
 
 // ---------------------------------------------------------------------------
@@ -1391,7 +1401,7 @@ __global__ void k_links(uint32_t n_nodes, const uint8_t* __restrict__ mask, cons
 // ---------------------------------------------------------------------------
 struct WalkArgs {
     uint32_t n_nodes;
-    const uint32_t* link;
+    const uint32_t* link;          // [2 n + port]: (neighbour << 1) | entry port, or NONE32
     const uint8_t* meta;
     const uint8_t* mask;
     const uint32_t* rank;
@@ -1420,7 +1430,6 @@ struct WalkArgs {
     void* pkey;                    // the node's k-mer
     uint32_t* pcid;                // the node's colour-set id
     const void* node_rec;          // NodeRec<K> per node
-    const uint16_t* node_mm;       // meta | mask << 8 per node
     bool mark_visited;             // nodes outside simple paths exist: keep `visited` exact
 };
 
@@ -1433,7 +1442,7 @@ __device__ __forceinline__ void put_position(const WalkArgs& a, uint64_t pos, ui
     ((K*)a.pkey)[pos] = r.key;
     a.pcid[pos] = r.cid;
     a.prank[pos] = r.rank;
-    a.pinfo[pos] = a.node_mm[node];
+    a.pinfo[pos] = r.mm;
     a.prc[pos] = o;
     a.fflag[pos] = 0;
 }
@@ -1654,32 +1663,40 @@ __global__ __launch_bounds__(TPB) void k_walk_record(WalkArgs a, uint64_t n_comp
     }
     const bool staged = last - base <= REC_CAP;  // the same for the whole wave
     const NodeRec<K>* rec = (const NodeRec<K>*)a.node_rec;
-    auto put = [&](uint32_t i, uint32_t node, uint8_t o) {
+    // one record gather per position: the next node of a long path comes
+    // from the record of the current one
+    auto put = [&](uint32_t i, const NodeRec<K>& r, uint32_t node, uint8_t o) {
         if (staged) {
             const uint32_t j = uint32_t(off - base) + i;
-            const NodeRec<K> r = rec[node];
             s_key[wave][j] = r.key;
             s_cid[wave][j] = r.cid;
             s_rank[wave][j] = r.rank;
-            s_info[wave][j] = a.node_mm[node];
+            s_info[wave][j] = r.mm;
             s_rc[wave][j] = o;
         } else {
-            put_position<K>(a, off + i, node, o);
+            const uint64_t pos = off + i;
+            ((K*)a.pkey)[pos] = r.key;
+            a.pcid[pos] = r.cid;
+            a.prank[pos] = r.rank;
+            a.pinfo[pos] = r.mm;
+            a.prc[pos] = o;
+            a.fflag[pos] = 0;
         }
         if (a.mark_visited) a.visited[node] = 1;
     };
     if (L > 0 && L <= PATH_BUF) {
         for (uint32_t i = 0; i < L; ++i) {
-            const uint32_t v = a.comp_nodes[c * PATH_BUF + i];
-            put(i, v & 0x7FFFFFFFu, uint8_t(v >> 31));
+            const uint32_t v = a.comp_nodes[c * PATH_BUF + i], node = v & 0x7FFFFFFFu;
+            put(i, rec[node], node, uint8_t(v >> 31));
         }
     } else if (L > 0) {
         uint32_t cur = a.comp_x[c], ex = a.comp_ex[c];
         uint8_t o = ex == PORT_L ? 1 : 0;
         for (uint32_t i = 0;; ++i) {
-            put(i, cur, o);
+            const NodeRec<K> r = rec[cur];
+            put(i, r, cur, o);
             if (i + 1 == L) break;
-            const uint32_t l = a.link[2 * cur + ex];
+            const uint32_t l = ex ? r.link[1] : r.link[0];  // a constant index keeps r in registers
             const uint32_t q = l & 1u;
             cur = l >> 1;
             ex = 1u - q;
@@ -1999,7 +2016,7 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
         if (n == 0) continue;
         const uint8_t* src = s_out + s_loff[r];
         uint8_t* dst = out + s_dst[r];
-        for (uint32_t j = uint32_t(lane); j < n; j += STORE_LANES) dst[j] = src[j];
+        for (uint32_t j = uint32_t(lane); j < n; j += 64) dst[j] = src[j];
     }
 }
 
@@ -2024,7 +2041,7 @@ struct Phase2Bufs {
     DevBuf node_bucket, dir, filt, node_mask, nb, link, visited;
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fcnt, fkey;
     DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
-    DevBuf node_rec, node_mm, eflag, ecnt, ends;
+    DevBuf node_rec, eflag, ecnt, ends;
     DevBuf fkey_t, forder, forder_t, fdval, fpsize, fpboff, fsize, fboff;
     RadixScratch rs2;
     // results
@@ -2313,8 +2330,6 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     }
     uint64_t* set_off = P.set_off.ensure_keep<uint64_t>(cid_base + n_new, cid_base * 8, st);
     if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
-    NodeRec<K>* node_rec = P.node_rec.ensure<NodeRec<K>>(nn);
-    { KSCOPE("k_node_cid", st); k_node_cid<K><<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, node_key, rank, node_rec); }
     if (n_new) { KSCOPE("k_store_new", st); k_store_new<<<blocks_for(n_new, TPB / STORE_LANES), TPB, 0, st>>>(n_new, set_desc, keep_off, P.store_used, T); }
     P.store_used += kept;
     P.n_sets += n_new;
@@ -2327,11 +2342,11 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     { KSCOPE("k_dir_build", st); k_dir_build<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_key, node_bucket, node_off, k, dir, filt); }
     uint8_t* node_mask = P.node_mask.ensure<uint8_t>(nn);
     uint32_t* nbr = P.nb.ensure<uint32_t>(2ull * nn);
-    uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
-    uint16_t* node_mm = P.node_mm.ensure<uint16_t>(nn);
-    { KSCOPE("k_masks", st); k_masks<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, node_off, dir, filt, k, meta, node_mask, node_mm, nbr); }
+    { KSCOPE("k_masks", st); k_masks<K><<<xcd_grid(blocks_for(nn)), TPB, 0, st>>>(nn, node_key, node_bucket, node_off, dir, filt, k, node_mask, nbr); }
     uint8_t* eflag = P.eflag.ensure<uint8_t>(nn);
-    { KSCOPE("k_links", st); k_links<<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, link, eflag); }
+    NodeRec<K>* node_rec = P.node_rec.ensure<NodeRec<K>>(nn);
+    uint32_t* link = P.link.ensure<uint32_t>(2ull * nn);
+    { KSCOPE("k_links", st); k_links<K><<<blocks_for(nn), TPB, 0, st>>>(nn, node_mask, nbr, meta, ncol, coff, colors, node_slot, T, node_key, rank, node_rec, link, eflag); }
 
     // --- walk
     uint8_t* visited = P.visited.ensure<uint8_t>(nn);
@@ -2362,7 +2377,6 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.pkey = pkey;
     wa.pcid = pcid;
     wa.node_rec = node_rec;
-    wa.node_mm = node_mm;
     uint32_t* ends = P.ends.ensure<uint32_t>(nn);
     uint32_t* d_count = (uint32_t*)(P.total.ensure<uint64_t>(8) + 2);
     compact_flags<uint8_t>(eflag, nn, ends, d_count, P.ecnt, J.rs.scan_tmp, st);
