@@ -1056,70 +1056,138 @@ __global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t*
     block_add(&counters[2], ints);
 }
 
-struct BitWriter {
-    unsigned long long* words;
-    uint64_t pos;
-    __device__ void put(uint64_t v, uint32_t len) {  // low `len` bits of v, LSB first
-        if (len == 0) return;
-        if (len < 64) v &= (uint64_t(1) << len) - 1;
-        const uint64_t w = pos >> 6;
-        const uint32_t o = uint32_t(pos & 63);
-        atomicOr(&words[w], (unsigned long long)(v << o));
-        if (o && o + len > 64) atomicOr(&words[w + 1], (unsigned long long)(v >> (64 - o)));
-        pos += len;
-    }
-    __device__ void gamma(uint64_t x) {
-        const uint64_t xx = x + 1;
-        const uint32_t b = msb64d(xx);
-        put(0, b);
-        put(((xx & ((uint64_t(1) << b) - 1)) << 1) | 1u, b + 1);
-    }
-    __device__ void delta(uint64_t x) {
-        const uint64_t xx = x + 1;
-        const uint32_t b = msb64d(xx);
-        gamma(b);
-        put(xx & ((uint64_t(1) << b) - 1), b);
-    }
-};
+// ORs the low `len` (<= 64) bits of v in at bit `pos` (words zeroed beforehand).
+__device__ __forceinline__ void or_bits(unsigned long long* words, uint64_t pos, uint64_t v, uint32_t len) {
+    if (len == 0) return;
+    const uint64_t w = pos >> 6;
+    const uint32_t o = uint32_t(pos & 63);
+    atomicOr(&words[w], (unsigned long long)(v << o));
+    if (o && o + len > 64) atomicOr(&words[w + 1], (unsigned long long)(v >> (64 - o)));
+}
 
-__global__ void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_base, const uint64_t* __restrict__ set_desc,
-                             const uint64_t* __restrict__ boff, SetTables T, uint32_t num_colors,
-                             unsigned long long* __restrict__ words, uint64_t* __restrict__ set_off) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_new) return;
+// Elias delta code of x (x + 1 <= 2^32) as one bit string, LSB first: gamma(b)
+// = b' zero bits, then b' + 1 bits holding (b + 1) with its leading 1 moved to
+// the front, then the b low bits of x + 1 (b = floor(log2(x + 1)),
+// b' = floor(log2(b + 1))); returns its length (<= 43).
+__device__ __forceinline__ uint32_t delta_code(uint64_t x, uint64_t& code) {
+    const uint64_t xx = x + 1;
+    const uint32_t b = msb64d(xx);
+    const uint64_t g = uint64_t(b) + 1;
+    const uint32_t bb = msb64d(g);
+    const uint64_t gbits = ((g & ((uint64_t(1) << bb) - 1)) << 1) | 1u;
+    code = (gbits << bb) | ((xx & ((uint64_t(1) << b) - 1)) << (2 * bb + 1));
+    return 2 * bb + 1 + b;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
+    }
+    return v;
+}
+
+// Colours of c in [lo, hi) (hi - lo <= 64) as bits from lo.
+__device__ __forceinline__ uint64_t present_bits(const SetColors& c, uint32_t lo, uint32_t hi) {
+    uint32_t a = 0, z = c.n;  // first colour >= lo
+    while (a < z) {
+        const uint32_t mid = (a + z) >> 1;
+        if (c[mid] < lo)
+            a = mid + 1;
+        else
+            z = mid;
+    }
+    uint64_t bits = 0;
+    for (uint32_t j = a; j < c.n; ++j) {
+        const uint32_t v = c[j];
+        if (v >= hi) break;
+        bits |= uint64_t(1) << (v - lo);
+    }
+    return bits;
+}
+
+// Hybrid colour-set codes (hybrid_color_sets.hpp:41-96), one wave per set:
+// delta(n), then below n_colors / 4 colours the delta gaps of the list, below
+// 3/4 the n_colors-bit bitmap, else the delta gaps of the absent colours.  A
+// lane encodes one list element, or one 64-colour range of the bitmap / of
+// the absent colours; wave scans place the codes.
+__global__ __launch_bounds__(TPB) void k_set_encode(uint64_t n_new, uint64_t cid_base, uint64_t bit_base,
+                                                    const uint64_t* __restrict__ set_desc,
+                                                    const uint64_t* __restrict__ boff, SetTables T, uint32_t num_colors,
+                                                    unsigned long long* __restrict__ words,
+                                                    uint64_t* __restrict__ set_off) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    if (i >= n_new) return;  // the same for the whole wave
     const SetColors c = set_list(set_desc[i], T);
     const uint32_t n = c.n;
     const uint32_t sparse = num_colors / 4, dense = uint32_t(uint64_t(num_colors) * 3 / 4);
     const uint64_t at = bit_base + boff[i];
-    set_off[cid_base + i] = at;
-    BitWriter w{words, at};
-    w.delta(n);
+    uint64_t code;
+    const uint32_t hl = delta_code(n, code);
+    if (lane == 0) {
+        set_off[cid_base + i] = at;
+        or_bits(words, at, code, hl);
+    }
+    uint64_t pos = at + hl;
     if (n < sparse) {
-        w.delta(c[0]);
-        for (uint32_t j = 1; j < n; ++j) w.delta(uint64_t(c[j] - (c[j - 1] + 1)));
-    } else if (n < dense) {
-        uint64_t acc = 0;
-        uint32_t filled = 0, j = 0;
-        for (uint32_t v = 0; v < num_colors; ++v) {
-            const bool bit = j < n && c[j] == v;
-            if (bit) ++j;
-            acc |= uint64_t(bit) << filled;
-            if (++filled == 64) {
-                w.put(acc, 64);
-                acc = 0;
-                filled = 0;
+        for (uint32_t j0 = 0; j0 < n; j0 += 64) {
+            const uint32_t j = j0 + uint32_t(lane);
+            uint32_t len = 0;
+            if (j < n) {
+                const uint32_t v = c[j];
+                len = delta_code(j == 0 ? uint64_t(v) : uint64_t(v - (c[j - 1] + 1)), code);
             }
+            const uint32_t incl = wave_incl_sum(len);
+            if (j < n) or_bits(words, pos + incl - len, code, len);
+            pos += __shfl(incl, 63, 64);
         }
-        w.put(acc, filled);
+    } else if (n < dense) {
+        for (uint64_t lo = uint64_t(lane) * 64; lo < num_colors; lo += 64 * 64) {
+            const uint32_t hi = uint32_t(lo + 64 < num_colors ? lo + 64 : num_colors);
+            or_bits(words, pos + lo, present_bits(c, uint32_t(lo), hi), hi - uint32_t(lo));
+        }
     } else {
-        uint32_t j = 0, prev = 0xFFFFFFFFu;
-        for (uint32_t v = 0; v < num_colors; ++v) {
-            if (j < n && c[j] == v) {
-                ++j;
-                continue;
+        int64_t prev = -1;  // last absent colour before this round of ranges
+        for (uint64_t lo0 = 0; lo0 < num_colors; lo0 += 64 * 64) {
+            const uint64_t lo = lo0 + uint64_t(lane) * 64;
+            uint64_t absent = 0;
+            if (lo < num_colors) {
+                const uint32_t hi = uint32_t(lo + 64 < num_colors ? lo + 64 : num_colors);
+                const uint32_t w = hi - uint32_t(lo);
+                absent = ~present_bits(c, uint32_t(lo), hi) & (w == 64 ? ~0ull : (uint64_t(1) << w) - 1);
             }
-            w.delta(uint64_t(v - (prev + 1)));
-            prev = v;
+            // the last absent colour of the lower ranges (+1; 0: none)
+            uint64_t mx = absent ? lo + msb64d(absent) + 1 : 0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint64_t y = __shfl_up(mx, o, 64);
+                if (lane >= o && y > mx) mx = y;
+            }
+            const uint64_t below = __shfl_up(mx, 1, 64);
+            const int64_t p0 = lane > 0 && below ? int64_t(below) - 1 : prev;
+            uint32_t len = 0;
+            int64_t p = p0;
+            for (uint64_t m = absent; m; m &= m - 1) {
+                const int64_t v = int64_t(lo) + __ffsll((unsigned long long)m) - 1;
+                len += uint32_t(delta_len(uint64_t(v - (p + 1))));
+                p = v;
+            }
+            const uint32_t incl = wave_incl_sum(len);
+            uint64_t q = pos + incl - len;
+            p = p0;
+            for (uint64_t m = absent; m; m &= m - 1) {
+                const int64_t v = int64_t(lo) + __ffsll((unsigned long long)m) - 1;
+                const uint32_t l = delta_code(uint64_t(v - (p + 1)), code);
+                or_bits(words, q, code, l);
+                q += l;
+                p = v;
+            }
+            pos += __shfl(incl, 63, 64);
+            const uint64_t top = __shfl(mx, 63, 64);
+            if (top) prev = int64_t(top) - 1;
         }
     }
 }
@@ -1940,8 +2008,58 @@ __device__ void write_record(uint8_t* p, uint8_t open, uint64_t o, uint32_t n, u
     }
 }
 
-// Record bytes (unitig_spill.hpp:491-522) are staged in LDS up to FW_LDS per block.
-constexpr uint32_t FW_LDS = 32768;
+// The same record from positions staged in LDS: per position its colour-set
+// id and pb = orientation | first base << 1 | last base << 3 of its k-mer;
+// only the fragment's first k-mer in output order (`first`, at position
+// o + (rev ? n - 1 : 0)) is read whole.  The last base of the k-mer read
+// reverse-complemented is the complement (3 - b) of its first base.
+template <class K>
+__device__ __forceinline__ void write_record_lds(uint8_t* p, uint8_t open, uint32_t o, uint32_t n, uint32_t rev,
+                                                 const uint32_t* s_cid, const uint8_t* s_pb, const K& first,
+                                                 uint32_t k) {
+    using O = KOps<K>;
+    p[0] = open;
+    uint8_t* pr = p + 5;
+    uint32_t runs = 0, last = 0, cnt = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t c = s_cid[o + (rev ? n - 1 - j : j)];
+        if (j > 0 && c != last) {
+            put_bytes(pr, last == NONE32 ? ~0ull : uint64_t(last), 8);
+            put_bytes(pr + 8, cnt, 4);
+            pr += 12;
+            ++runs;
+            cnt = 0;
+        }
+        last = c;
+        ++cnt;
+    }
+    put_bytes(pr, last == NONE32 ? ~0ull : uint64_t(last), 8);
+    put_bytes(pr + 8, cnt, 4);
+    pr += 12;
+    ++runs;
+    put_bytes(p + 1, runs, 4);
+    const uint32_t seq_len = k + n - 1;
+    put_bytes(pr, seq_len, 4);
+    uint8_t* ps = pr + 4;
+    const char ACGT[4] = {'A', 'C', 'G', 'T'};
+    const uint32_t o0 = o + (rev ? n - 1 : 0);
+    const K km = ((s_pb[o0] & 1u) ^ rev) ? O::rc(first, k) : first;
+    for (uint32_t t = 0; t < k; ++t) ps[t] = ACGT[O::base(km, t, k)];
+    for (uint32_t j = 1; j < n; ++j) {
+        const uint32_t pb = s_pb[o + (rev ? n - 1 - j : j)];
+        const uint32_t lb = ((pb & 1u) ^ rev) ? 3u - ((pb >> 1) & 3u) : (pb >> 3) & 3u;
+        ps[k + j - 1] = ACGT[lb];
+    }
+}
+
+// Record bytes (unitig_spill.hpp:491-522) are staged in LDS up to fw_lds<K>
+// per block, the positions the block's records read up to fw_pos<K> (k-mers
+// of 128-bit keys make longer records: their stage keeps the record room, and
+// four blocks per CU).
+template <class K>
+constexpr uint32_t fw_lds() { return sizeof(K) == 8 ? 24576 : 32768; }
+template <class K>
+constexpr uint32_t fw_pos() { return sizeof(K) == 8 ? 2048 : 512; }
 
 // Fragments in position order (d = the d-th fragment start, ascending): the
 // record size of each (read from consecutive positions) and d itself, the
@@ -1974,10 +2092,11 @@ __global__ void k_frag_pboff(uint64_t nf, const uint32_t* __restrict__ dval, con
 }
 
 // A block builds the records of TPB consecutive fragments in position order
-// (their nodes are neighbours on the path arrays, so the reads coalesce) in
-// LDS, then each wave copies whole records, one at a time with its 64 lanes,
-// to their output offsets.  Blocks whose records exceed the LDS stage write
-// them directly.
+// in LDS, then each wave copies whole records, one at a time with its 64
+// lanes, to their output offsets.  The fragments tile one contiguous range of
+// positions; when it holds at most FW_POS positions, their colour-set ids and
+// orientation / end bases are first loaded into LDS with coalesced loads.
+// Blocks whose records exceed the LDS stage write them directly.
 template <class K>
 __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint32_t* __restrict__ fpos,
                                                         const uint32_t* __restrict__ frag_n,
@@ -1987,21 +2106,40 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
                                                         const uint32_t* __restrict__ pcid, uint32_t k,
                                                         const uint32_t* __restrict__ psize,
                                                         const uint64_t* __restrict__ pboff, uint8_t* __restrict__ out) {
+    using O = KOps<K>;
+    constexpr uint32_t FW_LDS = fw_lds<K>(), FW_POS = fw_pos<K>();
     __shared__ uint8_t s_out[FW_LDS];
+    __shared__ uint32_t s_cid[FW_POS];
+    __shared__ uint8_t s_pb[FW_POS];
     __shared__ uint32_t s_loff[TPB], s_size[TPB];
     __shared__ uint64_t s_dst[TPB];
     __shared__ uint32_t s_tmp[TPB / 64 + 1];
-    const uint64_t d = uint64_t(blockIdx.x) * TPB + threadIdx.x;
+    const uint64_t d0 = uint64_t(blockIdx.x) * TPB, d = d0 + threadIdx.x;
     const bool live = d < nf;
+    // the block's positions: from its first fragment's start to its last one's end
+    const uint64_t dl = (nf < d0 + TPB ? nf : d0 + TPB) - 1;
+    const uint32_t p0 = fpos[d0], pl = fpos[dl];
+    const uint32_t np = pl + frag_n[pl] - p0;
+    const bool pstaged = np <= FW_POS;
+    if (pstaged)
+        for (uint32_t i = threadIdx.x; i < np; i += TPB) {
+            const K key = pkey[p0 + i];
+            s_cid[i] = pcid[p0 + i];
+            s_pb[i] = uint8_t((frc[p0 + i] & 1u) | (O::base(key, 0, k) << 1) | (O::base(key, k - 1, k) << 3));
+        }
     const uint32_t sz = live ? psize[d] : 0u;
     uint32_t total;
-    const uint32_t loff = block_exclusive_scan<uint32_t>(sz, s_tmp, total);
+    const uint32_t loff = block_exclusive_scan<uint32_t>(sz, s_tmp, total);  // its barriers publish the stage
     const bool staged = total <= FW_LDS;
     if (live) {
         const uint32_t f = fpos[d];
         const uint64_t dst = pboff[d];
         uint8_t* p = staged ? s_out + loff : out + dst;
-        write_record<K>(p, frag_open[f], f, frag_n[f], frag_dir[f], frc, pkey, pcid, k);
+        const uint32_t n = frag_n[f], rev = frag_dir[f];
+        if (pstaged)
+            write_record_lds<K>(p, frag_open[f], f - p0, n, rev, s_cid, s_pb, pkey[f + (rev ? n - 1 : 0)], k);
+        else
+            write_record<K>(p, frag_open[f], f, n, rev, frc, pkey, pcid, k);
         s_loff[threadIdx.x] = loff;
         s_size[threadIdx.x] = sz;
         s_dst[threadIdx.x] = dst;
@@ -2329,7 +2467,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         P.words_zeroed = words_needed;
     }
     uint64_t* set_off = P.set_off.ensure_keep<uint64_t>(cid_base + n_new, cid_base * 8, st);
-    if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
+    if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new, TPB / 64), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
     if (n_new) { KSCOPE("k_store_new", st); k_store_new<<<blocks_for(n_new, TPB / STORE_LANES), TPB, 0, st>>>(n_new, set_desc, keep_off, P.store_used, T); }
     P.store_used += kept;
     P.n_sets += n_new;
