@@ -1089,23 +1089,53 @@ __device__ __forceinline__ uint32_t wave_incl_sum(uint32_t v) {
     return v;
 }
 
-// Colours of c in [lo, hi) (hi - lo <= 64) as bits from lo.
-__device__ __forceinline__ uint64_t present_bits(const SetColors& c, uint32_t lo, uint32_t hi) {
-    uint32_t a = 0, z = c.n;  // first colour >= lo
-    while (a < z) {
-        const uint32_t mid = (a + z) >> 1;
-        if (c[mid] < lo)
-            a = mid + 1;
-        else
-            z = mid;
+// Colours of c in the round [lo0, lo0 + 4096) as 64 words in the wave's LDS
+// row; returns lane q's word (colours lo0 + 64 q + bit).  `j`, the same in
+// every lane, is the first list element >= lo0 and moves past the round: the
+// lanes take 64 list elements at a time (coalesced) and OR their bits in.
+constexpr uint32_t ROUND_COLORS = 64 * 64;
+__device__ __forceinline__ uint64_t round_bits(const SetColors& c, uint64_t lo0, uint32_t& j,
+                                               unsigned long long* row) {
+    const int lane = threadIdx.x & 63;
+    row[lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    for (;;) {
+        const uint32_t jj = j + uint32_t(lane);
+        const uint64_t v = jj < c.n ? uint64_t(c[jj]) : ~0ull;
+        const bool in = v < lo0 + ROUND_COLORS;  // the list is sorted: a prefix of the lanes
+        if (in) atomicOr(&row[(v - lo0) >> 6], 1ull << (v & 63));
+        const uint32_t cnt = uint32_t(__popcll(__ballot(in)));
+        j += cnt;
+        if (cnt < 64) break;
     }
-    uint64_t bits = 0;
-    for (uint32_t j = a; j < c.n; ++j) {
-        const uint32_t v = c[j];
-        if (v >= hi) break;
-        bits |= uint64_t(1) << (v - lo);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    return row[lane];
+}
+
+// Absent colours of lane's range [lo, lo + 64) given the present bits.
+__device__ __forceinline__ uint64_t absent_bits(uint64_t present, uint64_t lo, uint32_t num_colors) {
+    if (lo >= num_colors) return 0;
+    const uint64_t w = num_colors - lo;
+    return ~present & (w >= 64 ? ~0ull : (uint64_t(1) << w) - 1);
+}
+
+// Absent colour before lane's range: the last absent colour of the lower
+// lanes' ranges, else `prev` (the last one of earlier rounds, -1: none).
+// `top` receives the last absent colour up to this round's end (or prev).
+__device__ __forceinline__ int64_t absent_before(uint64_t absent, uint64_t lo, int64_t prev, int64_t& top) {
+    const int lane = threadIdx.x & 63;
+    uint64_t mx = absent ? lo + msb64d(absent) + 1 : 0;  // last absent + 1 (0: none)
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(mx, o, 64);
+        if (lane >= o && y > mx) mx = y;
     }
-    return bits;
+    const uint64_t below = __shfl_up(mx, 1, 64);
+    const uint64_t all = __shfl(mx, 63, 64);
+    top = all ? int64_t(all) - 1 : prev;
+    return lane > 0 && below ? int64_t(below) - 1 : prev;
 }
 
 // Hybrid colour-set codes (hybrid_color_sets.hpp:41-96), one wave per set:
@@ -1118,7 +1148,8 @@ __global__ __launch_bounds__(TPB) void k_set_encode(uint64_t n_new, uint64_t cid
                                                     const uint64_t* __restrict__ boff, SetTables T, uint32_t num_colors,
                                                     unsigned long long* __restrict__ words,
                                                     uint64_t* __restrict__ set_off) {
-    const int lane = threadIdx.x & 63;
+    __shared__ unsigned long long s_row[TPB / 64][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     if (i >= n_new) return;  // the same for the whole wave
     const SetColors c = set_list(set_desc[i], T);
@@ -1145,29 +1176,23 @@ __global__ __launch_bounds__(TPB) void k_set_encode(uint64_t n_new, uint64_t cid
             pos += __shfl(incl, 63, 64);
         }
     } else if (n < dense) {
-        for (uint64_t lo = uint64_t(lane) * 64; lo < num_colors; lo += 64 * 64) {
-            const uint32_t hi = uint32_t(lo + 64 < num_colors ? lo + 64 : num_colors);
-            or_bits(words, pos + lo, present_bits(c, uint32_t(lo), hi), hi - uint32_t(lo));
+        uint32_t j = 0;
+        for (uint64_t lo0 = 0; lo0 < num_colors; lo0 += ROUND_COLORS) {
+            const uint64_t present = round_bits(c, lo0, j, s_row[wave]);
+            const uint64_t lo = lo0 + uint64_t(lane) * 64;
+            if (lo < num_colors) {
+                const uint32_t w = uint32_t(num_colors - lo < 64 ? num_colors - lo : 64);
+                or_bits(words, pos + lo, present, w);
+            }
         }
     } else {
         int64_t prev = -1;  // last absent colour before this round of ranges
-        for (uint64_t lo0 = 0; lo0 < num_colors; lo0 += 64 * 64) {
+        uint32_t j = 0;
+        for (uint64_t lo0 = 0; lo0 < num_colors; lo0 += ROUND_COLORS) {
             const uint64_t lo = lo0 + uint64_t(lane) * 64;
-            uint64_t absent = 0;
-            if (lo < num_colors) {
-                const uint32_t hi = uint32_t(lo + 64 < num_colors ? lo + 64 : num_colors);
-                const uint32_t w = hi - uint32_t(lo);
-                absent = ~present_bits(c, uint32_t(lo), hi) & (w == 64 ? ~0ull : (uint64_t(1) << w) - 1);
-            }
-            // the last absent colour of the lower ranges (+1; 0: none)
-            uint64_t mx = absent ? lo + msb64d(absent) + 1 : 0;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint64_t y = __shfl_up(mx, o, 64);
-                if (lane >= o && y > mx) mx = y;
-            }
-            const uint64_t below = __shfl_up(mx, 1, 64);
-            const int64_t p0 = lane > 0 && below ? int64_t(below) - 1 : prev;
+            const uint64_t absent = absent_bits(round_bits(c, lo0, j, s_row[wave]), lo, num_colors);
+            int64_t top;
+            const int64_t p0 = absent_before(absent, lo, prev, top);
             uint32_t len = 0;
             int64_t p = p0;
             for (uint64_t m = absent; m; m &= m - 1) {
@@ -1186,8 +1211,7 @@ __global__ __launch_bounds__(TPB) void k_set_encode(uint64_t n_new, uint64_t cid
                 p = v;
             }
             pos += __shfl(incl, 63, 64);
-            const uint64_t top = __shfl(mx, 63, 64);
-            if (top) prev = int64_t(top) - 1;
+            prev = top;
         }
     }
 }
