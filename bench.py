@@ -63,16 +63,22 @@ def step_bytes(name, s, k, launches, chunks):
         # values, one-byte head flags in; node key, meta, rank, colour count, set hash,
         # long-run flag, and the owned colours out
         "k_nodes": n * (8 + 1) + nn * (4 + 4 + wk) + nn * (wk + 1 + 4 + 4 + 8 + 4) + n * 4,
-        # node key, bucket and meta in; 8 neighbour lookups (two directory words and a key
-        # each); mask, meta/mask and the two neighbour words out
-        "k_masks": nn * (wk + 4 + 1) + 8 * nn * (8 + wk) + nn * (1 + 2 + 8),
+        # node key and bucket in; 8 neighbour lookups (two directory words and a key
+        # each); mask and the two neighbour words out
+        "k_masks": nn * (wk + 4) + 8 * nn * (8 + wk) + nn * (1 + 8),
         # node key and bucket in, one directory word out per node
         "k_dir_build": nn * (wk + 4) + nn * 4,
-        # per path position: node record + meta/mask + path entry in; key, cid, rank,
+        # per path position: the used fields of the node's walk record (key, rank,
+        # colour-set id, meta/mask, one link) + path entry in; key, cid, rank,
         # meta/mask, orientation, fragment flag out
-        "k_walk_record": nn * ((wk + 8) + 2 + 4) + nn * (wk + 4 + 4 + 2 + 1 + 1),
-        # fragment order + descriptors, per position key/cid/orientation, records out
-        "k_frag_write": nf * (4 + 4 + 1 + 1) + nn * (wk + 4 + 1) + F,
+        "k_walk_record": nn * (wk + 4 + 4 + 2 + 4 + 4) + nn * (wk + 4 + 4 + 2 + 1 + 1),
+        # neighbour words, the neighbours' masks, meta, mask, colour count/offset/slot,
+        # the set id (or the colour and its singleton id), key and rank in; the walk
+        # record (32 B, 64 B for 128-bit keys), links and end flag out
+        "k_links": nn * (8 + 2 + 1 + 1 + 4 + 4 + 4 + 8 + wk + 4) + nn * ((32 if wk == 8 else 64) + 8 + 1),
+        # fragment start, record size, output offset, length, open/direction flags, per
+        # position key/cid/orientation in; records out
+        "k_frag_write": nf * (4 + 4 + 8 + 4 + 1 + 1) + nn * (wk + 4 + 1) + F,
         # primary nodes: meta, bucket, rank, colour count, offset, hash, first colour in; slot out
         "k_set_insert": nn * 1 + nk * (4 + 4 + 4 + 4 + 8 + 4 + 4),
     }
