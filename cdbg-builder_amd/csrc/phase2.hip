@@ -2129,7 +2129,8 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
                                                         const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
                                                         const uint32_t* __restrict__ pcid, uint32_t k,
                                                         const uint32_t* __restrict__ psize,
-                                                        const uint64_t* __restrict__ pboff, uint8_t* __restrict__ out) {
+                                                        const uint64_t* __restrict__ pboff, uint8_t* __restrict__ out,
+                                                        uint32_t pos_cap) {
     using O = KOps<K>;
     constexpr uint32_t FW_LDS = fw_lds<K>(), FW_POS = fw_pos<K>();
     __shared__ uint8_t s_out[FW_LDS];
@@ -2144,7 +2145,7 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
     const uint64_t dl = (nf < d0 + TPB ? nf : d0 + TPB) - 1;
     const uint32_t p0 = fpos[d0], pl = fpos[dl];
     const uint32_t np = pl + frag_n[pl] - p0;
-    const bool pstaged = np <= FW_POS;
+    const bool pstaged = np <= FW_POS && np <= pos_cap;
     if (pstaged)
         for (uint32_t i = threadIdx.x; i < np; i += TPB) {
             const K key = pkey[p0 + i];
@@ -2589,6 +2590,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     SortPlan fplan;
     fplan.build({0, nf});
     const char* force64 = getenv("CDBG_FRAG_KEY64");  // testing: the 64-bit key path
+    const char* nostage = getenv("CDBG_FRAG_NOSTAGE");  // testing: records read positions from HBM
+    const uint32_t pos_cap = nostage && *nostage == '1' ? 0u : ~0u;
     if (lbits + rbits <= 32 && !(force64 && *force64 == '1')) {
         uint32_t* fk32 = (uint32_t*)fkey;
         uint32_t* fk32_t = (uint32_t*)fkey_t;
@@ -2612,7 +2615,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (nf) { KSCOPE("k_frag_pboff", st); k_frag_pboff<<<blocks_for(nf), TPB, 0, st>>>(nf, dval, fboff, pboff); }
     const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
     uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st);
-    if (nf) { KSCOPE("k_frag_write", st); k_frag_write_pos<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout + P.frag_total); }
+    if (nf) { KSCOPE("k_frag_write", st); k_frag_write_pos<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout + P.frag_total, pos_cap); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));

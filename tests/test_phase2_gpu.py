@@ -158,6 +158,19 @@ def test_phase2_fragment_sort_64bit_keys(gpu_ctx, k, m, B, monkeypatch):
     _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
 
 
+@pytest.mark.parametrize("stage", ["0", "1"])
+@pytest.mark.parametrize("k,m,B", [(31, 21, 256), (63, 27, 64)])
+def test_phase2_fragment_writer_paths(gpu_ctx, k, m, B, stage, monkeypatch):
+    """Fragment records are built from positions staged in LDS when a block's
+    positions fit the stage (many short fragments: 60 colours at 3% SNPs),
+    else from HBM; forced off, every block reads HBM.  Both give the
+    oracle's bytes."""
+    if stage == "0":
+        monkeypatch.setenv("CDBG_FRAG_NOSTAGE", "1")
+    data, off = G.synth(seed=21, num_colors=60, genome_len=4000, snp_rate=0.03, indel_rate=0.002)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
+
+
 @pytest.mark.parametrize("k,m,B,s,chunk", [(31, 19, 512, 1.1, 0), (63, 27, 256, 1.1, 0), (31, 21, 1024, 1.3, 40000)])
 def test_phase2_skewed_bucket_map(gpu_ctx, k, m, B, s, chunk):
     """Zipf bucket sizes (a bucket map): one bucket holds a large share of
