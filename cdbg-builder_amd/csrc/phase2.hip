@@ -2273,6 +2273,14 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (nt == 0) return;
     if (nt >= (1ull << 32)) throw Error("phase 2: a chunk holds more than 2^32 k-mer occurrences");
     const int bbits = bits_for(J.B);
+    // size of a job-wide stream at the job's end, extrapolated from its size
+    // after this chunk by the share of occurrences processed (+5%); 0 after
+    // the last chunk
+    const uint64_t t_all = h_btup.back(), t_done = h_btup[b1];
+    auto grown = [&](uint64_t n_now) -> size_t {
+        if (t_done >= t_all || t_done == 0) return 0;
+        return size_t(double(n_now) * double(t_all) / double(t_done) * 1.05);
+    };
 
     // local bucket tuple offsets
     std::vector<uint64_t> lb(nb + 1);
@@ -2477,21 +2485,21 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (n_new) { KSCOPE("k_set_cid_len", st); k_set_cid_len<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, set_desc, T, NC, set_bits, keep, cnt); }
     exclusive_scan<uint64_t, uint64_t>(keep, keep_off, n_new, keep_off + n_new, J.rs.scan_tmp, st);
     const uint64_t kept = d2h<uint64_t>(keep_off + n_new, st);
-    P.store.ensure_keep<uint32_t>(P.store_used + kept + 1, P.store_used * 4, st);
+    P.store.ensure_keep<uint32_t>(P.store_used + kept + 1, P.store_used * 4, st, grown(P.store_used + kept + 1));
     T = set_tables(P);
     T.cur_colors = colors;
     exclusive_scan<uint64_t, uint64_t>(set_bits, set_boff, n_new, set_boff + n_new, J.rs.scan_tmp, st);
     const uint64_t new_bits = d2h<uint64_t>(set_boff + n_new, st);
     const uint64_t bit_base = P.cs_bits;
     const uint64_t words_needed = (bit_base + new_bits + 63) / 64 + 1;
-    auto* words = P.cs_words.ensure_keep<unsigned long long>(words_needed, P.words_zeroed * 8, st);
+    auto* words = P.cs_words.ensure_keep<unsigned long long>(words_needed, P.words_zeroed * 8, st, grown(words_needed));
     if (words_needed > P.words_zeroed) {
         KSCOPE("k_fill", st);
         k_fill<unsigned long long><<<blocks_for(words_needed - P.words_zeroed), TPB, 0, st>>>(
             words + P.words_zeroed, words_needed - P.words_zeroed, 0ull);
         P.words_zeroed = words_needed;
     }
-    uint64_t* set_off = P.set_off.ensure_keep<uint64_t>(cid_base + n_new, cid_base * 8, st);
+    uint64_t* set_off = P.set_off.ensure_keep<uint64_t>(cid_base + n_new, cid_base * 8, st, grown(cid_base + n_new));
     if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new, TPB / 64), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
     if (n_new) { KSCOPE("k_store_new", st); k_store_new<<<blocks_for(n_new, TPB / STORE_LANES), TPB, 0, st>>>(n_new, set_desc, keep_off, P.store_used, T); }
     P.store_used += kept;
@@ -2614,7 +2622,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
     if (nf) { KSCOPE("k_frag_pboff", st); k_frag_pboff<<<blocks_for(nf), TPB, 0, st>>>(nf, dval, fboff, pboff); }
     const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
-    uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st);
+    uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st, grown(P.frag_total + fbytes));
     if (nf) { KSCOPE("k_frag_write", st); k_frag_write_pos<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout + P.frag_total, pos_cap); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));

@@ -220,11 +220,15 @@ struct DevBuf {
     // Grows to hold n elements keeping the first `keep_bytes` bytes
     // (stream-ordered copy; waits for it before freeing the old block).
     template <class T>
-    T* ensure_keep(size_t n, size_t keep_bytes, hipStream_t st) {
+    // Grows to at least n elements keeping the first keep_bytes; a growth
+    // allocates at least 1.5x the old capacity and at least `hint` elements
+    // (the caller's estimate of the final size, so a stream that grows chunk
+    // by chunk is copied about once).
+    T* ensure_keep(size_t n, size_t keep_bytes, hipStream_t st, size_t hint = 0) {
         size_t bytes = n * sizeof(T);
         if (bytes <= cap) return (T*)p;
         DevBuf nb;
-        nb.ensure<uint8_t>(std::max(bytes, cap + cap / 2));
+        nb.ensure<uint8_t>(std::max(std::max(bytes, cap + cap / 2), hint * sizeof(T)));
         if (keep_bytes && p) CDBG_HIP(hipMemcpyAsync(nb.p, p, std::min(keep_bytes, cap), hipMemcpyDeviceToDevice, st));
         CDBG_HIP(hipStreamSynchronize(st));
         swap(nb);
