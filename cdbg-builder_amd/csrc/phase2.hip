@@ -1729,12 +1729,17 @@ __global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a, const uint32_t* _
 // Writes path c end to end from position comp_off[c]: per position the
 // node's data and its orientation along the path (1 when it is passed
 // entering by R).  The paths of a wave's ends occupy one contiguous range of
-// positions; when it holds at most REC_CAP positions it is assembled in LDS
-// and written out with coalesced stores.
-constexpr uint32_t REC_CAP = 512;
+// positions; when it holds at most rec_cap<K>() positions it is assembled in
+// LDS and written out with coalesced stores.  The stage size was measured:
+// for 64-bit keys 320 positions (24 KB per block, six blocks per CU: the
+// gathers want waves in flight) beat 384, 512 and 256 (too many waves
+// overflow the stage); 128-bit keys did best with 512.
+template <class K>
+constexpr uint32_t rec_cap() { return sizeof(K) == 8 ? 320 : 512; }
 
 template <class K>
 __global__ __launch_bounds__(TPB) void k_walk_record(WalkArgs a, uint64_t n_comp) {
+    constexpr uint32_t REC_CAP = rec_cap<K>();
     __shared__ K s_key[TPB / 64][REC_CAP];
     __shared__ uint32_t s_cid[TPB / 64][REC_CAP];
     __shared__ uint32_t s_rank[TPB / 64][REC_CAP];
