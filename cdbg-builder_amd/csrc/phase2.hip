@@ -2082,13 +2082,16 @@ __device__ __forceinline__ void write_record_lds(uint8_t* p, uint8_t open, uint3
 }
 
 // Record bytes (unitig_spill.hpp:491-522) are staged in LDS up to fw_lds<K>
-// per block, the positions the block's records read up to fw_pos<K> (k-mers
-// of 128-bit keys make longer records: their stage keeps the record room, and
-// four blocks per CU).
+// per block, the positions the block's records read up to fw_pos<K>.  For
+// 64-bit keys 16 KB + 1024 positions (~25 KB per block, six blocks per CU)
+// measured fastest: 24 KB + 2048 positions was 20% slower (four blocks per
+// CU), a 12 KB record stage twice as slow (most blocks no longer fit and write
+// their records byte by byte).  Records of 128-bit keys' k-mers are longer:
+// their stage keeps 32 KB.
 template <class K>
-constexpr uint32_t fw_lds() { return sizeof(K) == 8 ? 24576 : 32768; }
+constexpr uint32_t fw_lds() { return sizeof(K) == 8 ? 16384 : 32768; }
 template <class K>
-constexpr uint32_t fw_pos() { return sizeof(K) == 8 ? 2048 : 512; }
+constexpr uint32_t fw_pos() { return sizeof(K) == 8 ? 1024 : 512; }
 
 // Fragments in position order (d = the d-th fragment start, ascending): the
 // record size of each (read from consecutive positions) and d itself, the
