@@ -92,6 +92,7 @@ _SIGS = {
     "cdbg_gpu_job_write_phase2_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p,
                                                   C.POINTER(ColorState)]),
     "cdbg_gpu_hbm_estimate": (C.c_uint64, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint64]),
+    "cdbg_gpu_chunk_budget": (C.c_uint64, [C.c_uint64, C.c_uint32]),
     "cdbg_gpu_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerParams), C.POINTER(C.c_void_p)]),
     "cdbg_gpu_server_submit": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
                                          C.c_uint32, U64P]),
@@ -371,6 +372,12 @@ def alloc_stats():
 
 def hbm_estimate(bases, k=31, m=21, chunk_occurrences=0):
     return lib().cdbg_gpu_hbm_estimate(bases, k, m, chunk_occurrences)
+
+
+def chunk_budget(usable_bytes, k):
+    """Phase-2 chunk (k-mer occurrences) the automatic budget plans for
+    `usable_bytes` of free device memory at k-mer length k."""
+    return lib().cdbg_gpu_chunk_budget(usable_bytes, k)
 
 
 class Server:

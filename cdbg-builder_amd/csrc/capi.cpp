@@ -174,10 +174,9 @@ CDBG_API int cdbg_gpu_job_phase1(cdbg_gpu_job* job) {
         if (job->J.from_files) throw Error("this job was loaded from bucket files: it has no sequences for phase 1");
         CDBG_HIP(hipSetDevice(job->J.ctx->device));
         job->J.phase2_done = false;
-        g_prof = &job->J.prof;
+        ProfScope prof(&job->J.prof);
         run_phase1(job->J);
         job->J.prof.collect();
-        g_prof = nullptr;
     })
 }
 
@@ -186,10 +185,9 @@ CDBG_API int cdbg_gpu_job_phase2(cdbg_gpu_job* job) {
         if (!job) throw Error("null job");
         if (!job->J.phase1_done) throw Error("phase 2 requires phase 1");
         CDBG_HIP(hipSetDevice(job->J.ctx->device));
-        g_prof = &job->J.prof;
+        ProfScope prof(&job->J.prof);
         run_phase2(job->J);
         job->J.prof.collect();
-        g_prof = nullptr;
     })
 }
 

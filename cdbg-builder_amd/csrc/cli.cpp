@@ -86,6 +86,9 @@ uint32_t best_m(uint32_t k) {
     return (k + 2) / 4;
 }
 
+// most buckets phase 1 supports (16-bit bucket ids)
+constexpr uint64_t MAX_BUCKETS = 65536;
+
 // validate_and_resolve_config / auto_bucket_count_ (builder.hpp:550-649)
 uint32_t resolve(Config& c) {
     if (c.k < 2 || c.k > 63) throw std::runtime_error("k must satisfy 2 <= k <= 63");
@@ -114,7 +117,12 @@ uint32_t resolve(Config& c) {
     const double b = M / per_bucket;
     uint64_t n = b < 1.0 ? 1 : uint64_t(b);
     if (n < 64) n = 64;
-    if (n > UINT32_MAX) n = UINT32_MAX;
+    if (n > MAX_BUCKETS) {
+        // bucket ids are 16-bit on the device; the reference clamps its own
+        // count to the file-descriptor limit the same way (builder.hpp:654-675)
+        std::cerr << "warning: -g " << c.g << " sizes " << n << " buckets; clamped to " << MAX_BUCKETS << "\n";
+        n = MAX_BUCKETS;
+    }
     return uint32_t(n);
 }
 

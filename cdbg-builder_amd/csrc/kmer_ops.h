@@ -48,7 +48,6 @@ struct KOps<uint64_t> {
     __device__ static uint32_t base(const K& x, uint32_t i, uint32_t k) {  // i-th base from the left
         return uint32_t(x >> (2 * (k - 1 - i))) & 3u;
     }
-    __device__ static uint64_t hash(const K& x) { return mix64(x); }
 };
 
 template <>
@@ -104,9 +103,6 @@ struct KOps<Key128> {
     __device__ static uint32_t base(const K& x, uint32_t i, uint32_t k) {
         const uint32_t s = 2 * (k - 1 - i);
         return s >= 64 ? uint32_t(x.hi >> (s - 64)) & 3u : uint32_t(x.lo >> s) & 3u;
-    }
-    __device__ static uint64_t hash(const K& x) {
-        return mix64(x.lo) ^ (mix64(x.hi + 0x9e3779b97f4a7c15ULL) << 1);
     }
 };
 

@@ -54,6 +54,7 @@
 #include <cstdlib>
 #include <cstring>
 
+#include "hbm_model.h"
 #include "job.h"
 #include "kmer_ops.h"
 
@@ -2693,14 +2694,12 @@ static void phase2_impl(Job& J) {
         size_t free_b = 0, total_b = 0;
         CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
         free_b += size_t(g_block_cache.cached.load());  // cached blocks are free for this job
-        // ~160 B per occurrence covers tuples, sort buffers, nodes, table,
-        // walk and fragment arrays; buffers this phase already holds from an
-        // earlier run count as available; 40% is kept for the growing output
-        // streams.
+        // the job's per-occurrence model (tuples, sort buffers, nodes,
+        // table, walk and fragment arrays; wider for 128-bit keys); buffers
+        // this phase already holds from an earlier run count as available
         const uint64_t held = g_device_bytes.load();
         const uint64_t usable = uint64_t(free_b) + std::min<uint64_t>(held, uint64_t(free_b)) / 4;
-        budget = std::max<uint64_t>(1u << 20, uint64_t(0.6 * double(usable)) / 160);
-        budget = std::min<uint64_t>(budget, (1ull << 31) - 1);
+        budget = chunk_budget(usable, k);
         J.auto_chunk = budget;  // later runs of this job plan the same chunks
     }
     uint32_t nchunks = 0;

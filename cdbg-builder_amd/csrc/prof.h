@@ -48,10 +48,34 @@ struct Profiler {
         }
         pending.clear();
     }
+    // drops the events of launches that did not complete (a failed run)
+    void discard() {
+        for (auto& r : pending) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        pending.clear();
+    }
 };
 
 // The profiler of the job running on this thread (null when none).
 inline thread_local Profiler* g_prof = nullptr;
+
+// Makes `p` the profiler of this thread for a scope.  On every exit the
+// previous profiler is restored and events still pending are dropped, so a
+// run that throws leaves neither a pointer to its job's profiler nor live
+// events behind (a successful run collects its events before the scope ends).
+struct ProfScope {
+    Profiler* prev;
+    Profiler* p;
+    explicit ProfScope(Profiler* q) : prev(g_prof), p(q) { g_prof = q; }
+    ~ProfScope() {
+        if (p) p->discard();
+        g_prof = prev;
+    }
+    ProfScope(const ProfScope&) = delete;
+    ProfScope& operator=(const ProfScope&) = delete;
+};
 
 struct KScope {
     hipStream_t st;

@@ -38,6 +38,7 @@
 
 #include "cdbg_gpu.h"
 #include "common.h"
+#include "hbm_model.h"
 #include "job.h"
 
 namespace cdbg_amd {
@@ -50,34 +51,17 @@ double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-// HBM model of one job, from the sizes of its device buffers.  Random
-// sequence has a minimizer every (k - m + 2) / 2 positions, so about
-// 2 / (k - m + 2) super-k-mers per base, and a super-k-mer's records carry
-// one k-mer more than it starts: occurrences ~ bases + super-k-mers
-// (measured 1.19 per base at k=31 m=21, 1.05 at k=63 m=27; 5% margin).
-// Phase 1 holds the input (1 B/base), 2-bit bases and ACGT bitmap (3/8
-// B/base), the tile staging of super-k-mer starts and ends (6 B/base) and
-// the records with their sort buffers (~57 B per super-k-mer); phase 2 holds
-// ~160 B per k-mer occurrence of its chunk with 64-bit k-mers (~230 B with
-// 128-bit ones: wider tuples, sort buffers and node keys) plus the output
-// streams (~6 B per occurrence).
-constexpr uint64_t FIXED_BYTES = 64ull << 20;
-double supers_per_base(uint32_t k, uint32_t m) { return 2.0 / double(k - m + 2); }
-double occ_per_base(uint32_t k, uint32_t m) { return 1.05 * (1.0 + supers_per_base(k, m)); }
-double p1_per_base(uint32_t k, uint32_t m) { return 1.0 + 0.375 + 6.0 + 57.0 * supers_per_base(k, m); }
-double p2_per_occ(uint32_t k) { return k <= 31 ? 166.0 : 236.0; }
-
 }  // namespace
 
 uint64_t hbm_estimate(uint64_t bases, uint32_t k, uint32_t m, uint64_t chunk_occ) {
     const uint64_t all = uint64_t(occ_per_base(k, m) * double(bases));
     const uint64_t occ = chunk_occ ? std::min(all, chunk_occ) : all;
-    return FIXED_BYTES + uint64_t(p1_per_base(k, m) * double(bases)) + uint64_t(p2_per_occ(k) * double(occ));
+    return HBM_FIXED_BYTES + uint64_t(p1_per_base(k, m) * double(bases)) + uint64_t(p2_per_occ(k) * double(occ));
 }
 
 // phase-2 chunk (k-mer occurrences) that keeps a job's estimate within cap
 uint64_t chunk_for_cap(uint64_t bases, uint32_t k, uint32_t m, uint64_t cap) {
-    const double p1 = double(FIXED_BYTES) + p1_per_base(k, m) * double(bases);
+    const double p1 = double(HBM_FIXED_BYTES) + p1_per_base(k, m) * double(bases);
     const double left = double(cap) - p1;
     const uint64_t occ = left > 0 ? uint64_t(left / p2_per_occ(k)) : 0;
     return std::max<uint64_t>(occ, 1u << 20);
@@ -256,6 +240,11 @@ extern "C" {
 CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint32_t m, uint64_t chunk_occurrences) {
     if (k < 2 || m < 2 || m >= k) return 0;
     return hbm_estimate(bases, k, m, chunk_occurrences);
+}
+
+CDBG_API uint64_t cdbg_gpu_chunk_budget(uint64_t usable_bytes, uint32_t k) {
+    if (k < 2 || k > 63) return 0;
+    return chunk_budget(usable_bytes, k);
 }
 
 CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params, cdbg_gpu_server** out) {

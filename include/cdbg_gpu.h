@@ -217,6 +217,12 @@ typedef struct {
  * length k and minimizer length m whose phase 2 runs in chunks of
  * `chunk_occurrences` k-mer occurrences (0: one chunk); 0 for invalid k, m. */
 CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint32_t m, uint64_t chunk_occurrences);
+/* Phase-2 chunk (k-mer occurrences) that the automatic budget plans when
+ * `usable_bytes` of device memory are free for phase 2 at k-mer length k
+ * (the per-occurrence model of cdbg_gpu_hbm_estimate: 166 B with 64-bit
+ * k-mers, 236 B with 128-bit ones); 0 for invalid k.  The reference sizes its
+ * admission by 24 x uncompressed bucket bytes (bucket_walker.hpp:631,647). */
+CDBG_API uint64_t cdbg_gpu_chunk_budget(uint64_t usable_bytes, uint32_t k);
 CDBG_API int cdbg_gpu_server_create(cdbg_gpu_ctx* ctx, const cdbg_gpu_server_params* params,
                                     cdbg_gpu_server** out);
 /* Queues a request (layout of cdbg_gpu_job_create).  The colour offsets are

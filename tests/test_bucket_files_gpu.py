@@ -9,70 +9,27 @@
   same files.  Also on bucket files whose records carry several colours and
   arrive in another order (what the reference's compacting writer produces).
 """
-import ctypes
 import os
 import struct
 
 import numpy as np
 import pytest
 
+import bucket_format as F
 import oracle as O
 from conftest import cdbg_gpu as G
 from test_host_io import SYS
 
 pytestmark = pytest.mark.gpu
-ACGT = b"ACGT"
-
-
-def _varint(buf, pos):
-    v = sh = 0
-    while True:
-        b = buf[pos]
-        pos += 1
-        v |= (b & 0x7F) << sh
-        if not b & 0x80:
-            return v, pos
-        sh += 7
 
 
 def _frames(path):
     raw = open(path, "rb").read()
-    out, p = bytearray(), 0
-    while p + 4 <= len(raw):
-        u, = struct.unpack_from("<I", raw, p)
-        p += 4
-        if u == 0:
-            break
-        c, = struct.unpack_from("<I", raw, p)
-        p += 4
-        blk = raw[p:p + c]
-        p += c
-        if SYS is not None:
-            buf = ctypes.create_string_buffer(u)
-            assert SYS.LZ4_decompress_safe(blk, buf, c, u) == u
-            out += buf.raw
-        else:
-            out += G.lz4_decompress(blk, u)
-    return bytes(out)
+    return F.frames(raw, F.sys_lz4_decompress(SYS) if SYS is not None else G.lz4_decompress)
 
 
 def _records(path):
-    d, p, recs = _frames(path), 0, []
-    while p < len(d):
-        nc, p = _varint(d, p)
-        cols, prev = [], 0
-        for _ in range(nc):
-            x, p = _varint(d, p)
-            prev += x
-            cols.append(prev)
-        lf, p = _varint(d, p)
-        ln, fl = lf >> 4, lf & 15
-        nb = (ln + 3) // 4
-        bases = bytes(ACGT[(d[p + i // 4] >> (2 * (i % 4))) & 3] for i in range(ln))
-        assert all((d[p + nb - 1] >> (2 * j)) & 3 == 0 for j in range(ln - 4 * (nb - 1), 4)), "padding bits"
-        p += nb
-        recs.append((tuple(cols), fl, bases))
-    return recs
+    return F.records(_frames(path))
 
 
 def _fasta_job(tmp_path, colors=5, glen=20000, seed=6):
@@ -180,7 +137,7 @@ def _write_bucket(path, recs, frame=700):
         stream += _enc_varint((len(bases) << 4) | fl)
         packed = bytearray((len(bases) + 3) // 4)
         for i, ch in enumerate(bases):
-            packed[i // 4] |= ACGT.index(ch) << (2 * (i % 4))
+            packed[i // 4] |= F.ACGT.index(ch) << (2 * (i % 4))
         stream += packed
     with open(path, "wb") as f:
         for o in range(0, len(stream), frame):

@@ -63,6 +63,17 @@ def test_cli_rejects_nonempty_tmp_dir(tmp_path):
     assert r.returncode == 1 and "is not empty" in r.stderr
 
 
+@needs_cli
+def test_cli_clamps_auto_bucket_count(tmp_path):
+    # -g 100 at -t 1 sizes ~1.2e5 buckets by the reference's model
+    # (builder.hpp:608-621); phase 1 keeps 16-bit bucket ids, so the count is
+    # clamped to 65536 with a warning instead of failing after the input is read
+    lst, _ = _list(tmp_path, colors=1, glen=200)
+    r = _run("-i", lst, "-o", tmp_path / "out", "-k", 31, "-g", 100, "-t", 1, "-d", tmp_path / "s")
+    assert "clamped to 65536" in r.stderr
+    assert "num_buckets = 65536" in r.stdout
+
+
 @pytest.mark.gpu
 @needs_cli
 @pytest.mark.parametrize("k,m,b", [(31, 19, 6), (63, 27, 5)])
