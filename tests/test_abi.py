@@ -79,3 +79,15 @@ def test_no_kernel_needs_scratch_memory():
             sizes += [int(x) for x in re.findall(r"\.private_segment_fixed_size:\s*(\d+)", notes)]
     assert len(sizes) > 50  # every kernel was seen
     assert max(sizes) == 0, sorted(sizes)[-5:]
+
+
+def test_chunk_budget_follows_key_width():
+    """The automatic phase-2 chunk uses the same per-occurrence model as the
+    server's admission (166 B with 64-bit k-mers, 236 B with 128-bit ones), so
+    a k > 31 job plans proportionally smaller chunks."""
+    usable = 200 << 30
+    b31, b32, b63 = (G.chunk_budget(usable, k) for k in (31, 32, 63))
+    assert b32 == b63 < b31
+    assert abs(b31 / b63 - 236.0 / 166.0) < 1e-3
+    assert G.chunk_budget(1 << 20, 31) == 1 << 20  # floor
+    assert G.chunk_budget(1 << 50, 31) == (1 << 31) - 1  # ceiling
