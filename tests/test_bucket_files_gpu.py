@@ -167,3 +167,27 @@ def test_multicolour_records(gpu_ctx, tmp_path, k, m, B):
         rng.shuffle(recs)
         _write_bucket(str(d / f"bucket_{b}.bin"), recs)
     _compare_phase2(gpu_ctx, tmp_path, str(d), B, k, len(files))
+
+
+def test_failed_profiled_run_then_file_write(gpu_ctx, tmp_path):
+    """A profiled run that fails (an injected device out-of-memory inside
+    phase 1 or 2) leaves no profiler behind on its thread: the job is
+    destroyed, and the next job's bucket-file write on the same thread (whose
+    launches are profiled only when its own job enables it) runs normally."""
+    data, off = G.synth(seed=2, num_colors=2, genome_len=5000)
+    for n in (3, 25):
+        with gpu_ctx.job(data, off, 31, 19, 16) as j:
+            j.set_profiling(True)
+            G.lib().cdbg_gpu_inject_oom(n)
+            try:
+                with pytest.raises(RuntimeError, match="injected"):
+                    j.run()
+            finally:
+                G.lib().cdbg_gpu_inject_oom(-1)
+    d = str(tmp_path / "b")
+    with gpu_ctx.job(data, off, 31, 19, 16) as j2:
+        j2.run()
+        j2.write_bucket_files(d)
+    ob = O.Build(data, off, 31, 19, 16)
+    sup = ob.supers(1)
+    assert sum(len(_records(os.path.join(d, f"bucket_{b}.bin"))) for b in range(16)) == len(sup["start"])

@@ -54,6 +54,9 @@ def _assert_same(ob, s, gf, gw, go):
     (31, 21, 1024, 6, 30000),
     (31, 12, 16, 3, 15000),
     (63, 27, 256, 5, 20000),     # 128-bit k-mers
+    (32, 19, 128, 5, 20000),     # 128-bit k-mers whose high word is empty
+    (33, 21, 96, 5, 20000),      # one base in the high word
+    (47, 23, 200, 5, 20000),
     (15, 9, 37, 4, 8000),
     (21, 11, 1, 2, 5000),        # one bucket: no cross-bucket boundaries
 ])
@@ -118,7 +121,7 @@ def test_phase2_distinct_kmers_bruteforce(gpu_ctx):
 
 
 @pytest.mark.parametrize("k,m,B,chunk", [(31, 19, 64, 20000), (31, 19, 64, 1), (63, 27, 97, 30000),
-                                         (31, 21, 1024, 50000)])
+                                         (31, 21, 1024, 50000), (32, 19, 80, 30000)])
 def test_phase2_chunked(gpu_ctx, k, m, B, chunk):
     """Buckets processed in several chunks: colour-set ids, the colour-set
     stream and the fragment stream continue across chunks."""
@@ -171,7 +174,8 @@ def test_phase2_fragment_writer_paths(gpu_ctx, k, m, B, stage, monkeypatch):
     _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
 
 
-@pytest.mark.parametrize("k,m,B,s,chunk", [(31, 19, 512, 1.1, 0), (63, 27, 256, 1.1, 0), (31, 21, 1024, 1.3, 40000)])
+@pytest.mark.parametrize("k,m,B,s,chunk", [(31, 19, 512, 1.1, 0), (63, 27, 256, 1.1, 0), (31, 21, 1024, 1.3, 40000),
+                                           (32, 19, 300, 1.1, 0)])
 def test_phase2_skewed_bucket_map(gpu_ctx, k, m, B, s, chunk):
     """Zipf bucket sizes (a bucket map): one bucket holds a large share of
     the occurrences, most hold a few; results equal the oracle under the same
