@@ -80,6 +80,9 @@ _SIGS = {
     "cdbg_gpu_job_set_chunk_tuples": (C.c_int, [C.c_void_p, C.c_uint64]),
     "cdbg_gpu_job_num_chunks": (C.c_int64, [C.c_void_p]),
     "cdbg_gpu_job_set_bucket_map": (C.c_int, [C.c_void_p, U64P, C.c_uint32]),
+    "cdbg_gpu_phase1_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_void_p, U64P, C.c_void_p]),
     "cdbg_gpu_job_frags": (C.c_int, [C.c_void_p, U8P]),
     "cdbg_gpu_job_color_sets": (C.c_int, [C.c_void_p, U64P, U64P]),
     "cdbg_gpu_bucket_write": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,

@@ -73,6 +73,7 @@ struct Job {
 void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off, uint32_t num_colors,
                    uint32_t k, uint32_t m, uint32_t num_buckets);
 void run_phase1(Job& J);
+void phase1_core(Job& J);  // run_phase1 without timing or allocation scope
 void run_phase2(Job& J);
 void free_phase2(Job& J);
 // Host copies of the phase-2 results (any pointer may be null).

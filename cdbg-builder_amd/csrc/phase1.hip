@@ -333,14 +333,13 @@ inline uint64_t d2h_u64(const uint64_t* d, hipStream_t st) {
 
 }  // namespace
 
-void run_phase1(Job& J) {
-    AllocStream alloc_on_job_stream(J.st);
+// Phase 1 over J's staged input on J's stream: fills J.p1's bucket-major
+// records and bucket offsets (device and host) and J.n_supers.  Every device
+// buffer comes from DevBuf::ensure, so the caller's allocation scope (block
+// cache or arena) decides where they live.
+void phase1_core(Job& J) {
     hipStream_t st = J.st;
     Phase1Bufs& P = J.p1;
-    hipEvent_t e0, e1;
-    CDBG_HIP(hipEventCreate(&e0));
-    CDBG_HIP(hipEventCreate(&e1));
-    CDBG_HIP(hipEventRecord(e0, st));
 
     const uint64_t ngroups = J.n_pad / 32;
     uint32_t* packed = P.packed.ensure<uint32_t>(ngroups * 2);
@@ -430,6 +429,17 @@ void run_phase1(Job& J) {
     CDBG_HIP(hipGetLastError());
     J.h_bucket_off.resize(J.B + 1);
     CDBG_HIP(hipMemcpyAsync(J.h_bucket_off.data(), boff, (J.B + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+}
+
+void run_phase1(Job& J) {
+    AllocStream alloc_on_job_stream(J.st);
+    hipStream_t st = J.st;
+    hipEvent_t e0, e1;
+    CDBG_HIP(hipEventCreate(&e0));
+    CDBG_HIP(hipEventCreate(&e1));
+    CDBG_HIP(hipEventRecord(e0, st));
+    phase1_core(J);
     CDBG_HIP(hipEventRecord(e1, st));
     CDBG_HIP(hipStreamSynchronize(st));
     float ms = 0;
@@ -438,7 +448,7 @@ void run_phase1(Job& J) {
     (void)hipEventDestroy(e1);
     J.stats.ms_phase1 = ms;
     J.stats.num_bases = J.n_bytes;
-    J.stats.num_supers = n;
+    J.stats.num_supers = J.n_supers;
     J.phase1_done = true;
 }
 

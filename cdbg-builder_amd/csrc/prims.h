@@ -19,6 +19,7 @@
 #include <map>
 #include <chrono>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "common.h"
@@ -167,20 +168,51 @@ struct BlockCache {
 inline BlockCache g_block_cache;
 
 // ---------------------------------------------------------------------------
+// Bump allocator over a device range the caller owns.  While a thread has one
+// set (ArenaScope), every DevBuf that grows takes its block from the range
+// and nothing is allocated or freed; running past the end throws before any
+// kernel sees the block.  Blocks are 256-byte aligned.
+// ---------------------------------------------------------------------------
+struct Arena {
+    uint8_t* base = nullptr;
+    size_t cap = 0, used = 0;
+    void* take(size_t bytes) {
+        const size_t at = (used + 255) & ~size_t(255);
+        if (at > cap || bytes > cap - at)
+            throw Error("scratch exhausted: " + std::to_string(at + bytes) + " bytes needed, " +
+                        std::to_string(cap) + " available");
+        used = at + bytes;
+        return base + at;
+    }
+};
+inline thread_local Arena* t_arena = nullptr;
+
+struct ArenaScope {
+    Arena* prev;
+    explicit ArenaScope(Arena* a) : prev(t_arena) { t_arena = a; }
+    ~ArenaScope() { t_arena = prev; }
+    ArenaScope(const ArenaScope&) = delete;
+    ArenaScope& operator=(const ArenaScope&) = delete;
+};
+
+// ---------------------------------------------------------------------------
 // Device buffer that only grows (contents are not preserved on growth).
 // With a t_alloc_stream set it comes from, and goes back to, the block
-// cache; otherwise hipMalloc / hipFree.
+// cache; with an arena set it comes from the arena; otherwise hipMalloc /
+// hipFree.  A buffer that borrows memory (an arena block, or a caller's
+// buffer adopted with borrow()) frees nothing.
 // ---------------------------------------------------------------------------
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
     hipStream_t st = nullptr;  // stream the buffer is used on (null: not cached)
+    bool borrowed = false;     // memory owned elsewhere
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) {
+        if (p && !borrowed) {
             if (st)
                 g_block_cache.put(p, cap, st);
             else
@@ -190,6 +222,14 @@ struct DevBuf {
         p = nullptr;
         cap = 0;
         st = nullptr;
+        borrowed = false;
+    }
+    // adopts `bytes` bytes at q that stay owned by the caller
+    void borrow(void* q, size_t bytes) {
+        release();
+        p = q;
+        cap = bytes;
+        borrowed = true;
     }
     template <class T>
     T* ensure(size_t n) {
@@ -197,6 +237,12 @@ struct DevBuf {
         if (bytes < 256) bytes = 256;
         if (bytes > cap) {
             release();
+            if (t_arena) {
+                p = t_arena->take(bytes);
+                cap = bytes;
+                borrowed = true;
+                return (T*)p;
+            }
             if (t_alloc_stream) {
                 p = g_block_cache.get(bytes, t_alloc_stream, bytes);
             } else {
@@ -216,6 +262,7 @@ struct DevBuf {
         std::swap(p, o.p);
         std::swap(cap, o.cap);
         std::swap(st, o.st);
+        std::swap(borrowed, o.borrowed);
     }
     // Grows to hold n elements keeping the first `keep_bytes` bytes
     // (stream-ordered copy; waits for it before freeing the old block).

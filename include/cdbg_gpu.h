@@ -140,6 +140,31 @@ CDBG_API int64_t cdbg_gpu_job_num_chunks(cdbg_gpu_job* job);
  * phase 1. */
 CDBG_API int cdbg_gpu_job_set_bucket_map(cdbg_gpu_job* job, const uint64_t* cuts, uint32_t n_cuts);
 
+/* Allocation-free phase 1.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same kernels as
+ * cdbg_gpu_job_phase1 run over d_bytes, a DEVICE array laid out as
+ * cdbg_gpu_job_create's `bytes` (colour c = d_bytes[color_off[c] ..
+ * color_off[c+1]), color_off on the host), and write the super-k-mer records
+ * bucket-major, as cdbg_gpu_job_supers(which = 1) returns them, into the
+ * caller's device arrays (max_supers entries each) and the num_buckets + 1
+ * bucket offsets into d_bucket_off; *n_supers (host) receives the count.
+ * Every intermediate buffer lives in a file-scope device array of
+ * CDBG_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: no
+ * hipMalloc or hipFree is called.  Inputs above CDBG_NOALLOC_MAX_BASES bytes,
+ * more than max_supers records, or intermediates beyond the array fail with a
+ * message and write nothing beyond max_supers.  No bucket map.  The work is
+ * ordered on `stream` (a hipStream_t; NULL: the null stream) and finished
+ * when the call returns.  Calls in one process share each GPU's array and are
+ * serialised by a lock; overlapping them in any other way would race. */
+#define CDBG_NOALLOC_SCRATCH_BYTES (64u << 20)
+#define CDBG_NOALLOC_MAX_BASES (4u << 20)
+CDBG_API int cdbg_gpu_phase1_noalloc(cdbg_gpu_ctx* ctx, const uint8_t* d_bytes, const uint64_t* color_off,
+                                     uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets,
+                                     uint64_t max_supers, uint64_t* d_start, uint32_t* d_len, uint8_t* d_flags,
+                                     uint32_t* d_bucket, uint32_t* d_color, uint64_t* d_bucket_off,
+                                     uint64_t* n_supers, void* stream);
+
 /* Phase-2 artifacts: the frag_unitigs.bin byte stream (frag_bytes bytes),
  * the colour-set bit stream (ceil(cs_bits/64) words) and the bit offset of
  * every class (num_color_sets words). */
