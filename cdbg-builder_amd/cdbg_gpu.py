@@ -38,6 +38,14 @@ class ColorState(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class StitchStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("num_fragments", "open_ends", "joins", "unmatched_ends", "cycles",
+                                          "num_unitigs", "unitig_kmers", "foreign_dropped")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class SynthParams(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("num_colors", C.c_uint32), ("genome_len", C.c_uint64),
                 ("snp_rate", C.c_double), ("indel_rate", C.c_double), ("indel_max", C.c_uint32),
@@ -103,6 +111,15 @@ _SIGS = {
     "cdbg_gpu_server_result": (C.c_int, [C.c_void_p, C.c_uint64, U8P, U64P, U64P]),
     "cdbg_gpu_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
     "cdbg_gpu_server_destroy": (None, [C.c_void_p]),
+    "cdbg_stitch": (C.c_void_p, [U8P, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "cdbg_stitch_file": (C.c_void_p, [C.c_char_p, C.c_uint32, C.c_uint32]),
+    "cdbg_unitigs_stats": (C.c_int, [C.c_void_p, C.POINTER(StitchStats)]),
+    "cdbg_unitigs_bases": (C.c_uint64, [C.c_void_p]),
+    "cdbg_unitigs_export": (C.c_int, [C.c_void_p, C.c_char_p, U64P, U64P]),
+    "cdbg_unitigs_free": (None, [C.c_void_p]),
+    "cdbg_emit_fasta": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "cdbg_finalize_color_sets": (C.c_int, [C.c_char_p, C.POINTER(ColorState)]),
+    "cdbg_write_metadata": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64]),
     "cdbg_load_fasta": (C.c_void_p, [C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32]),
     "cdbg_load_fasta_list": (C.c_void_p, [C.c_char_p, C.c_uint32]),
     "cdbg_seqs_total_bytes": (C.c_uint64, [C.c_void_p]),
@@ -183,6 +200,73 @@ def load_fasta(files, threads=8):
     finally:
         lib().cdbg_seqs_free(h)
     return data, off
+
+
+class Unitigs:
+    """Monochromatic unitigs of a stitch (cdbg_stitch / cdbg_stitch_file), in
+    emit order: ascending colour-set id."""
+
+    def __init__(self, h):
+        if not h:
+            raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+        self.h = C.c_void_p(h)
+
+    def close(self):
+        if self.h:
+            lib().cdbg_unitigs_free(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def stats(self):
+        st = StitchStats()
+        _check(lib().cdbg_unitigs_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def export(self):
+        """(bases bytes, offsets u64[n+1], cids u64[n])."""
+        n = self.stats()["num_unitigs"]
+        nb = lib().cdbg_unitigs_bases(self.h)
+        seq = C.create_string_buffer(max(nb, 1))
+        offs = np.zeros(n + 1, np.uint64)
+        cids = np.zeros(n, np.uint64)
+        _check(lib().cdbg_unitigs_export(self.h, seq, _ptr(offs, U64P), _ptr(cids, U64P)))
+        return seq.raw[:nb], offs, cids
+
+    def unitigs(self):
+        """List of (sequence bytes, cid)."""
+        seq, offs, cids = self.export()
+        return [(seq[int(offs[i]):int(offs[i + 1])], int(cids[i])) for i in range(len(cids))]
+
+    def emit_fasta(self, out_basename):
+        _check(lib().cdbg_emit_fasta(self.h, os.fsencode(out_basename)))
+
+
+def stitch(frags, k, threads=8):
+    """Phase 3 over a frag_unitigs.bin byte stream (numpy u8 array or bytes)."""
+    a = np.ascontiguousarray(np.frombuffer(bytes(frags), np.uint8) if isinstance(frags, (bytes, bytearray))
+                             else frags, dtype=np.uint8)
+    return Unitigs(lib().cdbg_stitch(_ptr(a, U8P), len(a), k, threads))
+
+
+def stitch_file(path, k, threads=8):
+    return Unitigs(lib().cdbg_stitch_file(os.fsencode(path), k, threads))
+
+
+def finalize_color_sets(cs_path, state):
+    """Finish <out>.color_sets from phase 2's files and colour state (a dict
+    from bucket_process / write_phase2_files)."""
+    st = ColorState(**{n: state[n] for n, _ in ColorState._fields_})
+    _check(lib().cdbg_finalize_color_sets(os.fsencode(cs_path), C.byref(st)))
+
+
+def write_metadata(out_basename, k, num_kmers, num_colors, num_unitigs, num_color_sets):
+    _check(lib().cdbg_write_metadata(os.fsencode(out_basename), k, num_kmers, num_colors, num_unitigs,
+                                     num_color_sets))
 
 
 def lz4_compress(src):

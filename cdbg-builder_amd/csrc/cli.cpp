@@ -1,11 +1,12 @@
-// cdbg-build: host driver of the GPU phases, with the reference's command
-// line (src/main.cpp:13-74) and configuration rules (builder.hpp:550-649).
-// It runs phase 1 on the device and writes the bucket files, then phase 2
-// over those files, leaving frag_unitigs.bin in the scratch directory and
-// <out>.color_sets / <out>.color_sets.tmp_offsets as the reference leaves them
-// after bucket-process.  Stitch and emit (phases 3-4) are the host stages
-// that consume these artifacts; this driver reports the phase-2 state they
-// need in <out>.phase2.txt.
+// cdbg-build: driver of the four phases, with the reference's command line
+// (src/main.cpp:13-74) and configuration rules (builder.hpp:550-649).  Phase 1
+// runs on the device and writes the bucket files, phase 2 runs on the device
+// over those files and leaves frag_unitigs.bin in the scratch directory and
+// <out>.color_sets / <out>.color_sets.tmp_offsets as the reference leaves
+// them after bucket-process; the host stitch and emit (phases 3-4) then write
+// <out>.fa, <out>.u2c, the finished <out>.color_sets and <out>.metadata.txt
+// (builder.hpp:72-79 order).  --phase2-only stops after phase 2 and reports
+// the state the reference's own phases 3-4 need in <out>.phase2.txt.
 //
 // Deviations: -b accepts [1, 16] (the reference's [10, 16] rejects the
 // 64-bucket configuration); k runs up to 63 in every build (64- or 128-bit
@@ -33,14 +34,14 @@ struct Config {
     uint32_t k = 0, t = 1, m = 0, b = 0;
     double g = 0, alpha = 2.0, beta = 7.0;
     uint64_t flush = 0, spill = 0;
-    bool verbose = false, keep_tmp = false, resident = false;
+    bool verbose = false, keep_tmp = false, resident = false, phase2_only = false;
 };
 
 void usage(const char* p) {
     std::fprintf(stderr,
                  "usage: %s -i <filenames_list> -o <out_basename> -k <k> [-t threads] [-m m] [-b buckets_log2]\n"
                  "          [-d tmp_dir] [-g max_ram_gb] [--alpha a] [--beta b] [--flush n] [--spill n]\n"
-                 "          [--verbose] [--keep-tmp] [--resident]\n",
+                 "          [--verbose] [--keep-tmp] [--resident] [--phase2-only]\n",
                  p);
 }
 
@@ -67,6 +68,7 @@ bool parse(int argc, char** argv, Config& c) {
         else if (a == "--verbose") c.verbose = true;
         else if (a == "--keep-tmp") c.keep_tmp = true;
         else if (a == "--resident") c.resident = true;
+        else if (a == "--phase2-only") c.phase2_only = true;
         else if (a == "-h" || a == "--help") return false;
         else throw std::runtime_error("unknown option " + a);
     }
@@ -208,14 +210,34 @@ int main(int argc, char** argv) {
         std::cout << "  bucket fragments: " << st.num_fragments << "\n"
                   << "  distinct color sets: " << st.num_color_sets << "\n"
                   << "  distinct k-mers: " << st.num_kmers << "\n";
-        std::ofstream meta(c.out + ".phase2.txt");
-        meta << "k=" << c.k << "\nnum_kmers=" << st.num_kmers << "\nnum_colors=" << nc
-             << "\nnum_color_sets=" << st.num_color_sets << "\nnum_fragments=" << st.num_fragments
-             << "\ncolor_set_bits=" << st.num_bits << "\ncolor_set_total_integers=" << st.total_integers
-             << "\ncolor_set_trailing_word=" << st.trailing_word << "\nfrag_unitigs=" << frag << "\n";
+        if (c.phase2_only) {
+            std::ofstream meta(c.out + ".phase2.txt");
+            meta << "k=" << c.k << "\nnum_kmers=" << st.num_kmers << "\nnum_colors=" << nc
+                 << "\nnum_color_sets=" << st.num_color_sets << "\nnum_fragments=" << st.num_fragments
+                 << "\ncolor_set_bits=" << st.num_bits << "\ncolor_set_total_integers=" << st.total_integers
+                 << "\ncolor_set_trailing_word=" << st.trailing_word << "\nfrag_unitigs=" << frag << "\n";
+            cdbg_gpu_ctx_destroy(ctx);
+            std::cout << "done. wrote " << cs << ", " << cs << ".tmp_offsets, " << frag << " and " << c.out
+                      << ".phase2.txt\n";
+            return 0;
+        }
         cdbg_gpu_ctx_destroy(ctx);
-        std::cout << "done. wrote " << cs << ", " << cs << ".tmp_offsets, " << frag << " and " << c.out
-                  << ".phase2.txt\n";
+        // phase 3: stitch; phase 4: emit
+        void* U = cdbg_stitch_file(frag.c_str(), c.k, c.t);
+        if (!U) throw std::runtime_error(cdbg_gpu_last_error());
+        cdbg_stitch_stats ss{};
+        int rc = cdbg_unitigs_stats(U, &ss);
+        if (rc == 0) rc = cdbg_emit_fasta(U, c.out.c_str());
+        cdbg_unitigs_free(U);
+        check(rc);
+        std::cout << "  stitch: " << ss.joins << " joins, " << ss.cycles << " cycles; num_unitigs = " << ss.num_unitigs
+                  << "\n";
+        check(cdbg_finalize_color_sets(cs.c_str(), &st));
+        check(cdbg_write_metadata(c.out.c_str(), c.k, st.num_kmers, nc, ss.num_unitigs, st.num_color_sets));
+        if (!c.keep_tmp) std::filesystem::remove(frag);
+        std::cout << "done. wrote " << c.out << ".fa, " << c.out << ".u2c, " << cs << " and " << c.out
+                  << ".metadata.txt\n";
+        return 0;
     } catch (std::exception& e) {
         std::cerr << "error: " << e.what() << '\n';
         return 1;

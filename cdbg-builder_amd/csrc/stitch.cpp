@@ -1,0 +1,501 @@
+// Phase 3 (stitch) on the host: joins the open ends of the phase-2 fragments
+// into maximal tigs and cuts every tig into monochromatic unitigs.
+//
+// Semantics (stitch_extmem.hpp:112-355): an open fragment side continues into
+// another bucket, and the two fragments that meet there both hold the full
+// boundary k-mer, so open sides are keyed on their canonical boundary k-mer;
+// two sides with the same key and compatible orientation join
+// (ext_pair_compatible :296-300), the shared k-mer is kept once, and its
+// colour comes from the side that owns it (ext_concat_runs :170-226).  A tig
+// is cut at colour-set changes; foreign placeholder runs at a tig's ends are
+// dropped, their k-mers being owned and emitted elsewhere
+// (ext_split_monochromatic :112-151).
+//
+// Method (this file's own): every open side's key is computed in parallel,
+// the keys are bucketed by hash and sorted per bucket, equal keys pair up,
+// and each chain of joined fragments is walked once from the end with the
+// smaller (fragment, side), found first by following the pairing alone; the
+// walks run in parallel over contiguous ranges of chains.  Chains that close
+// on themselves are what is left: they are walked in fragment order, kept as
+// one cycle (the shared k-mer once) and, when they carry several colour
+// sets, rotated to start at a colour change before they are cut.  The result
+// is the reference's unitig set (its round-based doubling join reaches the
+// same pairs); ids and order follow this file's rule: ascending colour-set
+// id, then chain order.
+#include "stitch.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "common.h"
+
+namespace cdbg_amd {
+
+int set_error(const std::string& msg);
+
+namespace {
+
+using u128 = unsigned __int128;
+constexpr uint64_t NONE = ~0ull;
+
+inline uint32_t rd32(const uint8_t* p) {
+    uint32_t v;
+    std::memcpy(&v, p, 4);
+    return v;
+}
+inline uint64_t rd64(const uint8_t* p) {
+    uint64_t v;
+    std::memcpy(&v, p, 8);
+    return v;
+}
+
+struct Run {
+    uint64_t cid;
+    uint32_t n;
+};
+
+// One frag_unitigs.bin record: [u8 open][u32 nruns]{[u64 cid][u32 n]}[u32 len][seq]
+// (unitig_spill.hpp:491-522, parser :646-665).
+struct FragView {
+    uint8_t open;
+    uint32_t nruns;
+    const uint8_t* runs;
+    uint32_t len;
+    const char* seq;
+    Run run(uint32_t i) const { return Run{rd64(runs + 12 * i), rd32(runs + 12 * i + 8)}; }
+};
+
+inline FragView view_at(const uint8_t* buf, uint64_t off) {
+    FragView f;
+    f.open = buf[off];
+    f.nruns = rd32(buf + off + 1);
+    f.runs = buf + off + 5;
+    f.len = rd32(buf + off + 5 + 12 * uint64_t(f.nruns));
+    f.seq = (const char*)(buf + off + 9 + 12 * uint64_t(f.nruns));
+    return f;
+}
+
+inline int code_of(char c) {
+    switch (c) {
+        case 'A': return 0;
+        case 'C': return 1;
+        case 'G': return 2;
+        case 'T': return 3;
+        default: return -1;
+    }
+}
+
+inline char comp(char c) {
+    switch (c) {
+        case 'A': return 'T';
+        case 'C': return 'G';
+        case 'G': return 'C';
+        default: return 'A';
+    }
+}
+
+// canonical k-mer of seq[0, k): min of the 2-bit forward and reverse-complement words
+inline u128 canonical_kmer(const char* s, uint32_t k, bool& fwd_is_canonical) {
+    u128 f = 0, r = 0;
+    for (uint32_t i = 0; i < k; ++i) {
+        const int c = code_of(s[i]);
+        if (c < 0) throw Error("stitch: non-ACGT base in a fragment sequence");
+        f = (f << 2) | u128(c);
+        r |= u128(3 - c) << (2 * i);
+    }
+    fwd_is_canonical = f <= r;
+    return fwd_is_canonical ? f : r;
+}
+
+inline uint64_t mix(u128 x) {
+    uint64_t z = uint64_t(x) ^ (uint64_t(x >> 64) * 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+struct End {
+    u128 key;
+    uint64_t id;  // frag << 2 | side << 1 | fwd_is_canonical   (side 0 left, 1 right)
+};
+
+template <class F>
+void parallel_for(uint64_t n, uint32_t threads, F&& fn) {  // fn(begin, end, thread)
+    threads = std::max<uint32_t>(1, std::min<uint64_t>(threads, std::max<uint64_t>(n / 4096, 1)));
+    if (threads == 1) {
+        fn(uint64_t(0), n, 0u);
+        return;
+    }
+    std::vector<std::thread> th;
+    std::string err;
+    std::atomic<bool> failed{false};
+    for (uint32_t t = 0; t < threads; ++t)
+        th.emplace_back([&, t] {
+            try {
+                fn(n * t / threads, n * (t + 1) / threads, t);
+            } catch (std::exception& e) {
+                if (!failed.exchange(true)) err = e.what();
+            }
+        });
+    for (auto& x : th) x.join();
+    if (failed) throw Error(err);
+}
+
+// A tig under construction: bases and the colour-set run of every k-mer.
+struct Tig {
+    std::string seq;
+    std::vector<Run> runs;
+    void push(uint64_t cid, uint32_t n) {
+        if (!n) return;
+        if (!runs.empty() && runs.back().cid == cid)
+            runs.back().n += n;
+        else
+            runs.push_back(Run{cid, n});
+    }
+};
+
+struct Stitcher {
+    const uint8_t* buf;
+    uint64_t n_bytes;
+    uint32_t k;
+    uint32_t threads;
+    std::vector<uint64_t> off;      // record start of every fragment
+    std::vector<uint64_t> partner;  // (frag << 1 | side) -> partner's (frag << 1 | side), NONE
+    std::vector<uint8_t> used;
+    cdbg_stitch_stats st{};
+
+    FragView frag(uint64_t i) const { return view_at(buf, off[i]); }
+
+    void index() {
+        uint64_t p = 0;
+        while (p < n_bytes) {
+            if (n_bytes - p < 9) throw Error("stitch: truncated fragment record");
+            const uint64_t nr = rd32(buf + p + 1);
+            if (n_bytes - p < 9 + 12 * nr) throw Error("stitch: truncated fragment record");
+            const uint64_t len = rd32(buf + p + 5 + 12 * nr);
+            if (n_bytes - p - 9 - 12 * nr < len) throw Error("stitch: truncated fragment record");
+            off.push_back(p);
+            p += 9 + 12 * nr + len;
+        }
+        st.num_fragments = off.size();
+        // every record's runs cover its k-mers
+        parallel_for(off.size(), threads, [&](uint64_t b, uint64_t e, uint32_t) {
+            for (uint64_t i = b; i < e; ++i) {
+                const FragView f = frag(i);
+                if (f.len < k) throw Error("stitch: a fragment is shorter than k");
+                uint64_t s = 0;
+                for (uint32_t r = 0; r < f.nruns; ++r) s += f.run(r).n;
+                if (s != uint64_t(f.len) - k + 1) throw Error("stitch: colour runs do not cover a fragment's k-mers");
+            }
+        });
+    }
+
+    void match() {
+        const uint64_t nf = off.size();
+        partner.assign(2 * nf, NONE);
+        // open sides, in fragment order
+        std::vector<std::vector<End>> part(threads);
+        parallel_for(nf, threads, [&](uint64_t b, uint64_t e, uint32_t t) {
+            auto& v = part[t];
+            for (uint64_t i = b; i < e; ++i) {
+                const FragView f = frag(i);
+                for (uint32_t side = 0; side < 2; ++side) {
+                    if (!(f.open & (side ? FRAG_OPEN_RIGHT : FRAG_OPEN_LEFT))) continue;
+                    bool fwd;
+                    const u128 key = canonical_kmer(f.seq + (side ? f.len - k : 0), k, fwd);
+                    v.push_back(End{key, (i << 2) | (uint64_t(side) << 1) | uint64_t(fwd)});
+                }
+            }
+        });
+        // hash buckets of ends, each sorted by key
+        constexpr uint32_t HB = 12, NB = 1u << HB;
+        std::vector<std::vector<uint64_t>> hcnt(part.size(), std::vector<uint64_t>(NB + 1, 0));
+        parallel_for(part.size(), threads, [&](uint64_t b, uint64_t e, uint32_t) {
+            for (uint64_t t = b; t < e; ++t)
+                for (const End& x : part[t]) ++hcnt[t][mix(x.key) >> (64 - HB)];
+        });
+        std::vector<uint64_t> boff(NB + 1, 0);
+        std::vector<std::vector<uint64_t>> at(part.size(), std::vector<uint64_t>(NB, 0));
+        uint64_t run = 0;
+        for (uint32_t h = 0; h < NB; ++h) {
+            boff[h] = run;
+            for (size_t t = 0; t < part.size(); ++t) {
+                at[t][h] = run;
+                run += hcnt[t][h];
+            }
+        }
+        boff[NB] = run;
+        st.open_ends = run;
+        std::vector<End> ends(run);
+        parallel_for(part.size(), threads, [&](uint64_t b, uint64_t e, uint32_t) {
+            for (uint64_t t = b; t < e; ++t) {
+                for (const End& x : part[t]) ends[at[t][mix(x.key) >> (64 - HB)]++] = x;
+                std::vector<End>().swap(part[t]);
+            }
+        });
+        std::atomic<uint64_t> joins{0}, unmatched{0};
+        std::atomic<uint32_t> next{0};
+        parallel_for(threads, threads, [&](uint64_t, uint64_t, uint32_t) {
+            for (;;) {
+                const uint32_t h = next.fetch_add(1);
+                if (h >= NB) break;
+                End* a = ends.data() + boff[h];
+                End* z = ends.data() + boff[h + 1];
+                std::sort(a, z, [](const End& x, const End& y) { return x.key < y.key || (x.key == y.key && x.id < y.id); });
+                for (End* p = a; p < z;) {
+                    End* q = p + 1;
+                    while (q < z && q->key == p->key) ++q;
+                    if (q - p > 2) throw Error("stitch: more than two open fragment ends share a boundary k-mer");
+                    bool paired = false;
+                    if (q - p == 2) {
+                        const uint64_t sa = (p[0].id >> 1) & 1, sb = (p[1].id >> 1) & 1;
+                        const bool fa = p[0].id & 1, fb = p[1].id & 1;
+                        const bool ok = sa != sb ? fa == fb : fa != fb;
+                        if (ok && (p[0].id >> 1) != (p[1].id >> 1)) {
+                            partner[p[0].id >> 1] = p[1].id >> 1;
+                            partner[p[1].id >> 1] = p[0].id >> 1;
+                            paired = true;
+                            joins.fetch_add(1);
+                        }
+                    }
+                    if (!paired) unmatched.fetch_add(uint64_t(q - p));
+                    p = q;
+                }
+            }
+        });
+        st.joins = joins;
+        st.unmatched_ends = unmatched;
+    }
+
+    // fragment i entered at `side` (0 left, 1 right): appended to t in the
+    // walk's orientation, reversed (reverse complement) when entered at its
+    // right side; `join` drops the shared first k-mer of the fragment
+    void append(Tig& t, uint64_t i, uint32_t side, bool join) const {
+        const FragView f = frag(i);
+        const bool rev = side == 1;
+        std::vector<Run> r(f.nruns);
+        for (uint32_t j = 0; j < f.nruns; ++j) r[j] = f.run(rev ? f.nruns - 1 - j : j);
+        uint32_t skip = 0;  // k-mers of r already in t
+        if (join) {
+            // the shared k-mer is t's last and r's first: keep one copy, with
+            // the owning side's colour (foreign when neither owns it here)
+            const uint64_t a = t.runs.back().cid, b = r.front().cid;
+            const uint64_t real = a != COLOR_FOREIGN ? a : b;
+            if (real != a) {
+                if (--t.runs.back().n == 0) t.runs.pop_back();
+                t.push(real, 1);
+            }
+            skip = 1;
+        }
+        for (const Run& x : r) {
+            const uint32_t take = x.n > skip ? x.n - skip : 0;
+            skip = x.n > skip ? 0 : skip - x.n;
+            t.push(x.cid, take);
+        }
+        const uint64_t from = join ? k : 0;
+        if (!rev) {
+            t.seq.append(f.seq + from, f.len - from);
+        } else {
+            for (uint64_t j = from; j < f.len; ++j) t.seq.push_back(comp(f.seq[f.len - 1 - j]));
+        }
+    }
+
+    // other end of the chain that enters fragment i at `side`
+    uint64_t far_end(uint64_t i, uint32_t side) const {
+        uint64_t cur = (i << 1) | (1 - side);
+        for (uint64_t steps = 0;; ++steps) {
+            const uint64_t p = partner[cur];
+            if (p == NONE) return cur;
+            if (steps > off.size()) throw Error("stitch: a chain does not end");
+            cur = p ^ 1;  // leave the partner fragment by its other side
+        }
+    }
+
+    // walks the chain entering fragment `start` at `side`; returns the tig and
+    // whether it closed on itself
+    bool walk(uint64_t start, uint32_t side, Tig& t, std::vector<uint8_t>& mark) const {
+        t.seq.clear();
+        t.runs.clear();
+        append(t, start, side, false);
+        mark[start] = 1;
+        uint64_t cur = (start << 1) | (1 - side);
+        for (;;) {
+            const uint64_t p = partner[cur];
+            if (p == NONE) return false;
+            const uint64_t nxt = p >> 1;
+            const uint32_t nside = uint32_t(p & 1);
+            if (mark[nxt]) {
+                if (nxt != start || nside != side) throw Error("stitch: a chain re-enters a fragment");
+                // closed: the last k-mer is the first one again; keep it once,
+                // with the owner's colour
+                const uint64_t last = t.runs.back().cid;
+                if (--t.runs.back().n == 0) t.runs.pop_back();
+                if (t.runs.front().cid == COLOR_FOREIGN && last != COLOR_FOREIGN) {
+                    if (--t.runs.front().n == 0) t.runs.erase(t.runs.begin());
+                    if (!t.runs.empty() && t.runs.front().cid == last)
+                        ++t.runs.front().n;
+                    else
+                        t.runs.insert(t.runs.begin(), Run{last, 1});
+                }
+                t.seq.pop_back();
+                return true;
+            }
+            append(t, nxt, nside, true);
+            mark[nxt] = 1;
+            cur = (nxt << 1) | (1 - nside);
+        }
+    }
+};
+
+// Cuts a linear tig into monochromatic unitigs (ext_split_monochromatic).
+void cut_linear(const Tig& t, uint32_t k, UnitigList& out, cdbg_stitch_stats& st) {
+    size_t lo = 0, hi = t.runs.size();
+    uint64_t base = 0;
+    if (lo < hi && t.runs[lo].cid == COLOR_FOREIGN) {
+        st.foreign_dropped += t.runs[lo].n;
+        base += t.runs[lo].n;
+        ++lo;
+    }
+    if (lo < hi && t.runs[hi - 1].cid == COLOR_FOREIGN) {
+        st.foreign_dropped += t.runs[hi - 1].n;
+        --hi;
+    }
+    for (size_t i = lo; i < hi; ++i) {
+        const Run& r = t.runs[i];
+        if (r.cid == COLOR_FOREIGN) throw Error("stitch: a foreign k-mer inside a tig (its owner never joined)");
+        UnitigList::Item it{r.cid, out.bases.size(), uint64_t(r.n) + k - 1};
+        out.bases.append(t.seq, base, it.len);
+        out.items.push_back(it);
+        st.unitig_kmers += r.n;
+        base += r.n;
+    }
+}
+
+// Cuts a cycle of n k-mers whose sequence holds n + k - 1 bases (the last
+// k - 1 repeat the first): one circular unitig when it has one colour set,
+// else rotated to start at a colour change and cut like a linear tig.
+void cut_cycle(Tig& t, uint32_t k, UnitigList& out, cdbg_stitch_stats& st) {
+    uint64_t n = 0;
+    for (const Run& r : t.runs) {
+        if (r.cid == COLOR_FOREIGN) throw Error("stitch: a foreign k-mer in a closed tig");
+        n += r.n;
+    }
+    if (t.runs.size() == 1) {
+        UnitigList::Item it{t.runs[0].cid, out.bases.size(), t.seq.size()};
+        out.bases.append(t.seq);
+        out.items.push_back(it);
+        st.unitig_kmers += n;
+        return;
+    }
+    if (t.runs.front().cid == t.runs.back().cid) {
+        // the seam lies inside a run: start the cycle at that run's first
+        // k-mer (a colour change), so every piece is a whole run
+        const uint64_t r = n - t.runs.back().n;
+        std::string seq(n + k - 1, 'A');
+        for (uint64_t i = 0; i < n + k - 1; ++i) seq[i] = t.seq[(r + i) % n];
+        t.seq.swap(seq);
+        t.runs.front().n += t.runs.back().n;
+        t.runs.pop_back();
+    }
+    cut_linear(t, k, out, st);
+}
+
+}  // namespace
+
+void Unitigs::build_order() {
+    if (!order.empty() || lists.empty()) return;
+    uint64_t total = 0;
+    for (auto& l : lists) total += l.items.size();
+    order.reserve(total);
+    // stable merge by cid: the smallest head, ties to the earlier list
+    std::vector<uint64_t> pos(lists.size(), 0);
+    using H = std::pair<uint64_t, uint32_t>;  // (cid, list)
+    std::vector<H> heap;
+    auto cmp = [](const H& a, const H& b) { return a.first > b.first || (a.first == b.first && a.second > b.second); };
+    for (uint32_t l = 0; l < lists.size(); ++l)
+        if (!lists[l].items.empty()) heap.push_back(H{lists[l].items[0].cid, l});
+    std::make_heap(heap.begin(), heap.end(), cmp);
+    while (!heap.empty()) {
+        std::pop_heap(heap.begin(), heap.end(), cmp);
+        const uint32_t l = heap.back().second;
+        heap.pop_back();
+        const uint64_t cid = lists[l].items[pos[l]].cid;
+        // take this list's whole run of equal cids
+        while (pos[l] < lists[l].items.size() && lists[l].items[pos[l]].cid == cid) order.push_back({l, pos[l]++});
+        if (pos[l] < lists[l].items.size()) {
+            heap.push_back(H{lists[l].items[pos[l]].cid, l});
+            std::push_heap(heap.begin(), heap.end(), cmp);
+        }
+    }
+}
+
+Unitigs* stitch_fragments(const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads) {
+    if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
+    if (threads == 0) threads = 1;
+    Stitcher S{buf, n, k, threads, {}, {}, {}, {}};
+    S.index();
+    S.match();
+    const uint64_t nf = S.off.size();
+    // chains: started from the end with the smaller (fragment, side)
+    std::vector<std::vector<uint64_t>> part(threads);
+    parallel_for(nf, threads, [&](uint64_t b, uint64_t e, uint32_t t) {
+        for (uint64_t i = b; i < e; ++i)
+            for (uint32_t side = 0; side < 2; ++side) {
+                if (S.partner[(i << 1) | side] != NONE) continue;
+                const uint64_t me = (i << 1) | side, other = S.far_end(i, side);
+                if (me <= other) part[t].push_back(me);
+                break;  // a fragment starts at most one chain: its left side first
+            }
+    });
+    std::vector<uint64_t> starts;
+    for (auto& p : part) {
+        starts.insert(starts.end(), p.begin(), p.end());
+        std::vector<uint64_t>().swap(p);
+    }
+    auto* U = new Unitigs();
+    U->k = k;
+    const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads, std::max<uint64_t>(starts.size() / 4096, 1)));
+    U->lists.resize(T + 1);  // the last list holds the cycles
+    S.used.assign(nf, 0);
+    std::vector<cdbg_stitch_stats> tst(T + 1, cdbg_stitch_stats{});
+    try {
+        parallel_for(starts.size(), T, [&](uint64_t b, uint64_t e, uint32_t t) {
+            Tig tig;
+            auto& out = U->lists[t];
+            for (uint64_t j = b; j < e; ++j) {
+                if (S.walk(starts[j] >> 1, uint32_t(starts[j] & 1), tig, S.used))
+                    throw Error("stitch: a chain from a free end closed on itself");
+                cut_linear(tig, k, out, tst[t]);
+            }
+        });
+        // what is left are cycles of joined fragments
+        Tig tig;
+        for (uint64_t i = 0; i < nf; ++i) {
+            if (S.used[i]) continue;
+            if (!S.walk(i, 0, tig, S.used)) throw Error("stitch: an unvisited fragment is not on a cycle");
+            ++tst[T].cycles;
+            cut_cycle(tig, k, U->lists[T], tst[T]);
+        }
+        for (auto& l : U->lists)
+            std::stable_sort(l.items.begin(), l.items.end(),
+                             [](const UnitigList::Item& a, const UnitigList::Item& b) { return a.cid < b.cid; });
+    } catch (...) {
+        delete U;
+        throw;
+    }
+    U->stats = S.st;
+    for (auto& s : tst) {
+        U->stats.cycles += s.cycles;
+        U->stats.unitig_kmers += s.unitig_kmers;
+        U->stats.foreign_dropped += s.foreign_dropped;
+    }
+    for (auto& l : U->lists) U->stats.num_unitigs += l.items.size();
+    return U;
+}
+
+}  // namespace cdbg_amd

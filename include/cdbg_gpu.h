@@ -268,6 +268,48 @@ CDBG_API int cdbg_gpu_server_release(cdbg_gpu_server* srv, uint64_t job_id);
 /* Stops the workers after their running jobs; queued jobs are dropped. */
 CDBG_API void cdbg_gpu_server_destroy(cdbg_gpu_server* srv);
 
+/* Phase 3, stitch, on the host (replaces stitch_unitigs_extmem_file_streaming,
+ * phase3_stitch/stitch_extmem.hpp:1012-1129, called from builder.hpp): reads
+ * frag_unitigs.bin records, joins open fragment ends that share their
+ * canonical boundary k-mer with compatible orientation, keeps the shared
+ * k-mer once with its owner's colour set, and cuts the joined tigs into
+ * monochromatic unitigs, dropping foreign placeholder runs at tig ends
+ * (stitch_extmem.hpp:112-355).  Returns a handle of the unitigs in emit order
+ * (ascending colour-set id), or NULL (message in cdbg_gpu_last_error). */
+typedef struct {
+    uint64_t num_fragments;   /* fragment records read */
+    uint64_t open_ends;       /* open fragment sides */
+    uint64_t joins;           /* pairs of open sides joined */
+    uint64_t unmatched_ends;  /* open sides without a partner (ends of the graph) */
+    uint64_t cycles;          /* tigs that close on themselves */
+    uint64_t num_unitigs;     /* monochromatic unitigs */
+    uint64_t unitig_kmers;    /* k-mers over all unitigs (equals num_kmers) */
+    uint64_t foreign_dropped; /* foreign placeholder k-mers dropped at tig ends */
+} cdbg_stitch_stats;
+CDBG_API void* cdbg_stitch(const uint8_t* frags, uint64_t n_bytes, uint32_t k, uint32_t threads);
+CDBG_API void* cdbg_stitch_file(const char* frag_path, uint32_t k, uint32_t threads);
+CDBG_API int cdbg_unitigs_stats(void* unitigs, cdbg_stitch_stats* out);
+/* Total bases of the unitigs, and the unitigs themselves in emit order:
+ * sequences concatenated into seq (cdbg_unitigs_bases bytes), offsets
+ * (num_unitigs + 1 entries) and colour-set ids (num_unitigs); any pointer may
+ * be NULL. */
+CDBG_API uint64_t cdbg_unitigs_bases(void* unitigs);
+CDBG_API int cdbg_unitigs_export(void* unitigs, char* seq, uint64_t* offsets, uint64_t* cids);
+CDBG_API void cdbg_unitigs_free(void* unitigs);
+
+/* Phase 4, emit, on the host (replaces emit_fasta and emit_colors,
+ * phase4_emit/emit.hpp:96-180, and builder::write_metadata_, builder.hpp:752-764):
+ * <out>.fa (">cid\n<seq>\n" in ascending cid) and <out>.u2c (bit i set iff
+ * unitig i ends a run of equal cids); the finished <out>.color_sets from the
+ * placeholder, the complete words and <out>.color_sets.tmp_offsets that
+ * cdbg_gpu_bucket_process wrote plus its colour state (trailing word,
+ * Elias-Fano of the class offsets, header; streaming_color_set_dict.hpp:221-294;
+ * the sidecar is removed); <out>.metadata.txt. */
+CDBG_API int cdbg_emit_fasta(void* unitigs, const char* out_basename);
+CDBG_API int cdbg_finalize_color_sets(const char* color_sets_path, const cdbg_gpu_color_state* state);
+CDBG_API int cdbg_write_metadata(const char* out_basename, uint32_t k, uint64_t num_kmers, uint32_t num_colors,
+                                 uint64_t num_unitigs, uint64_t num_color_sets);
+
 /* Host input: FASTA/FASTQ (plain or gzip) with kseq record semantics
  * (external/kseq.h:161-201).  Returns a handle holding the job layout
  * (bytes + colour offsets) or NULL. */

@@ -77,11 +77,14 @@ def test_cli_clamps_auto_bucket_count(tmp_path):
 @pytest.mark.gpu
 @needs_cli
 @pytest.mark.parametrize("k,m,b", [(31, 19, 6), (63, 27, 5)])
-def test_cli_end_to_end(tmp_path, k, m, b):
+def test_cli_phase2_only(tmp_path, k, m, b):
+    """--phase2-only leaves the artifacts the reference's own stitch and emit
+    read: frag_unitigs.bin, the placeholder + complete words of
+    .color_sets, its .tmp_offsets sidecar, and the dictionary state."""
     lst, paths = _list(tmp_path)
     d = tmp_path / "scratch"
     out = tmp_path / "out"
-    r = _run("-i", lst, "-o", out, "-k", k, "-m", m, "-b", b, "-d", d, "-t", 4, "--keep-tmp")
+    r = _run("-i", lst, "-o", out, "-k", k, "-m", m, "-b", b, "-d", d, "-t", 4, "--keep-tmp", "--phase2-only")
     assert r.returncode == 0, r.stderr
     B = 1 << b
     ob = O.Build.from_bucket_files(str(d), B, k, len(paths))
@@ -95,8 +98,63 @@ def test_cli_end_to_end(tmp_path, k, m, b):
     assert int(meta["num_color_sets"]) == ob.stat("num_sets")
 
     d2, out2 = tmp_path / "scratch2", tmp_path / "res"
-    r = _run("-i", lst, "-o", out2, "-k", k, "-m", m, "-b", b, "-d", d2, "--resident")
+    r = _run("-i", lst, "-o", out2, "-k", k, "-m", m, "-b", b, "-d", d2, "--resident", "--phase2-only")
     assert r.returncode == 0, r.stderr
     assert (d2 / "frag_unitigs.bin").read_bytes() == (d / "frag_unitigs.bin").read_bytes()
     assert (tmp_path / "res.color_sets").read_bytes() == (tmp_path / "out.color_sets").read_bytes()
     assert (tmp_path / "res.color_sets.tmp_offsets").read_bytes() == offs.tobytes()
+
+
+def check_final_outputs(out, d, ob, k, num_colors):
+    """The four deliverables of a full run against the oracle's phase 2 over
+    the bucket files the run kept: .fa = the host stitch of the oracle's
+    fragments, ascending cid; .u2c one bit per unitig, one set bit per colour
+    set; .color_sets header, every stream word and the Elias-Fano offsets;
+    metadata fields."""
+    import struct
+    import numpy as np
+    from test_stitch_emit import read_elias_fano
+    with G.stitch(ob.frags(), k) as U:
+        want = U.unitigs()
+    lines = open(f"{out}.fa", "rb").read().split(b"\n")
+    fa = [(lines[2 * i + 1], int(lines[2 * i][1:])) for i in range(len(lines) // 2)]
+    assert fa == want
+    raw = open(f"{out}.u2c", "rb").read()
+    nbits, nwords = struct.unpack_from("<QQ", raw)
+    assert nbits == len(want)
+    u2c = np.frombuffer(raw[16:], np.uint64)
+    assert sum(bin(int(w)).count("1") for w in u2c) == ob.stat("num_sets")
+    buf = open(f"{out}.color_sets", "rb").read()
+    words, offs = ob.color_sets()
+    nw = (ob.stat("cs_bits") + 63) // 64
+    assert struct.unpack_from("<IIIQQ", buf) == (num_colors, num_colors // 4, 3 * num_colors // 4,
+                                                 ob.stat("cs_bits"), nw)
+    assert buf[28:28 + 8 * nw] == words[:nw].tobytes()
+    vals, end = read_elias_fano(buf, 28 + 8 * nw)
+    assert end == len(buf) and vals == [int(x) for x in offs] + [ob.stat("cs_bits")]
+    assert not os.path.exists(f"{out}.color_sets.tmp_offsets")
+    meta = dict(x.split("=") for x in open(f"{out}.metadata.txt").read().split())
+    assert meta == {"k": str(k), "num_kmers": str(ob.stat("num_kmers")), "num_colors": str(num_colors),
+                    "num_unitigs": str(len(want)), "num_color_sets": str(ob.stat("num_sets"))}
+    return len(want)
+
+
+@pytest.mark.gpu
+@needs_cli
+@pytest.mark.parametrize("k,m,b", [(31, 19, 6), (63, 27, 5)])
+def test_cli_end_to_end(tmp_path, k, m, b):
+    lst, paths = _list(tmp_path)
+    d = tmp_path / "scratch"
+    out = tmp_path / "out"
+    r = _run("-i", lst, "-o", out, "-k", k, "-m", m, "-b", b, "-d", d, "-t", 4, "--keep-tmp")
+    assert r.returncode == 0, r.stderr
+    ob = O.Build.from_bucket_files(str(d), 1 << b, k, len(paths))
+    assert (d / "frag_unitigs.bin").read_bytes() == ob.frags().tobytes()
+    n = check_final_outputs(out, d, ob, k, len(paths))
+    assert f"num_unitigs = {n}" in r.stdout
+    # resident mode: the same deliverables
+    d2, out2 = tmp_path / "scratch2", tmp_path / "res"
+    r = _run("-i", lst, "-o", out2, "-k", k, "-m", m, "-b", b, "-d", d2, "--resident")
+    assert r.returncode == 0, r.stderr
+    for ext in (".fa", ".u2c", ".color_sets", ".metadata.txt"):
+        assert open(f"{out2}{ext}", "rb").read() == open(f"{out}{ext}", "rb").read(), ext

@@ -4,7 +4,8 @@ byte for byte against the oracle on the same inputs):
 * small plain build: 10 colours x 200 kbp (seed 1, 1% SNPs, 0.1% indels of
   1-10 bp), uncompressed 80-column FASTA files through cdbg-build -k 31 -m 19
   -b 6 --keep-tmp, against the oracle's phase 2 over the bucket files the run
-  kept and the oracle's phases 1+2 over the same sequences;
+  kept and the oracle's phases 1+2 over the same sequences; the final .fa,
+  .u2c, .color_sets and metadata against the oracle's fragments stitched;
 * pangenome parameters (k=31, m=21, 1024 buckets, 2% SNPs, 0.2% indels, 1%
   presence/absence deletions, 50 inversions) on the first 30 colours of the
   5 Mbp base genome;
@@ -24,8 +25,10 @@ import sys
 import numpy as np
 import pytest
 
+import ccdbg_check as C
 import oracle as O
 from conftest import PKG_DIR, ROOT, cdbg_gpu as G
+from test_cli import check_final_outputs
 
 sys.path.insert(0, ROOT)
 import bench  # noqa: E402  (the serving stream and the pangenome parameters)
@@ -74,10 +77,9 @@ def test_small_plain_build_through_cli(gpu_ctx, tmp_path):
     assert r.returncode == 0, r.stderr
     ob = O.Build.from_bucket_files(str(d), 64, 31, 10, threads=THREADS)
     assert (d / "frag_unitigs.bin").read_bytes() == ob.frags().tobytes()
-    words, offs = ob.color_sets()
-    full = ob.stat("cs_bits") // 64
-    assert (tmp_path / "out.color_sets").read_bytes()[:28 + 8 * full] == bytes(28) + words[:full].tobytes()
-    assert (tmp_path / "out.color_sets.tmp_offsets").read_bytes() == offs.tobytes()
+    n = check_final_outputs(out, d, ob, 31, 10)
+    # the unitig count of an independent test-side stitch over the same fragments
+    assert n == len(C.split_monochromatic(C.stitch(C.parse_frags(ob.frags()), 31), 31))
     # the same sequences in memory: phases 1+2 of the oracle and of the GPU
     om = O.Build(data, off, 31, 19, 64, threads=THREADS)
     assert om.stat("num_kmers") == ob.stat("num_kmers")
