@@ -49,12 +49,6 @@ __device__ __forceinline__ uint64_t rotl(uint64_t x, uint32_t r) {
     return r ? (x << r) | (x >> (64u - r)) : x;
 }
 
-__device__ __forceinline__ uint64_t sel4(uint32_t b, uint64_t a0, uint64_t a1, uint64_t a2, uint64_t a3) {
-    uint64_t lo = (b & 1u) ? a1 : a0;
-    uint64_t hi = (b & 1u) ? a3 : a2;
-    return (b & 2u) ? hi : lo;
-}
-
 struct HashTabs {
     uint64_t s[4];    // S[b]
     uint64_t sm[4];   // rol(S[b], m)          (leaving base, forward strand)
@@ -119,6 +113,49 @@ __device__ __forceinline__ uint32_t bucket_of(uint64_t h, const SkArgs& a) {
     return lo;
 }
 
+// Minima of the 18 windows of W hashes starting at s_h positions q0 .. q0+17
+// (the exact sliding-window minimum of windowed_min, minimizer.hpp:94-157;
+// only the value is used).  Windows are taken in groups of G (G <= W): the
+// windows starting at t0 .. t0+G-1 all contain the core [t0+G-1, t0+W-1],
+// so window t0+i is the minimum of the core, the suffix [t0+i, t0+G-2] of
+// the group's first G-1 hashes and the prefix [t0+W, t0+W+i-1] of the G-1
+// hashes after the core: W + G - 1 reads per group instead of G W.  G = 1 is
+// the direct minimum.
+template <int G>
+__device__ __forceinline__ void window_minima(const uint64_t* s_h, int q0, uint32_t W,
+                                              uint64_t (&mn)[SK_PER_THREAD + 2]) {
+    static_assert((SK_PER_THREAD + 2) % G == 0, "whole groups");
+#pragma unroll
+    for (int g = 0; g < (SK_PER_THREAD + 2) / G; ++g) {
+        const int t0 = q0 + g * G;
+        uint64_t core = ~0ull;
+        for (uint32_t i = G - 1; i < W; ++i) {
+            const uint64_t h = s_h[sk_hidx(t0 + int(i))];
+            core = h < core ? h : core;
+        }
+        uint64_t suf[G], pre[G];
+        suf[G - 1] = ~0ull;
+        pre[0] = ~0ull;
+#pragma unroll
+        for (int i = G - 2; i >= 0; --i) {
+            const uint64_t h = s_h[sk_hidx(t0 + i)];
+            suf[i] = h < suf[i + 1] ? h : suf[i + 1];
+        }
+#pragma unroll
+        for (int i = 1; i < G; ++i) {
+            const uint64_t h = s_h[sk_hidx(t0 + int(W) + i - 1)];
+            pre[i] = h < pre[i - 1] ? h : pre[i - 1];
+        }
+#pragma unroll
+        for (int i = 0; i < G; ++i) {
+            uint64_t v = core;
+            v = suf[i] < v ? suf[i] : v;
+            v = pre[i] < v ? pre[i] : v;
+            mn[g * G + i] = v;
+        }
+    }
+}
+
 __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     // one pad slot per 16 hashes: threads own 16 consecutive hashes, so an
     // unpadded stride of 128 bytes would put a wave's accesses in 2 banks
@@ -126,8 +163,13 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     __shared__ uint32_t s_pk[SK_NPK];
     __shared__ uint32_t s_vd[SK_NVD];
     __shared__ uint64_t s_tmp[SK_THREADS / 64 + 1];
+    __shared__ uint64_t s_tab[16];  // the four per-base tables S, sm, rc1, rcm: a lookup is one LDS read
 
     const int64_t p0 = int64_t(blockIdx.x) * SK_TILE;
+    if (threadIdx.x < 16) {
+        const uint32_t t = threadIdx.x >> 2, b = threadIdx.x & 3u;
+        s_tab[threadIdx.x] = t == 0 ? a.T.s[b] : t == 1 ? a.T.sm[b] : t == 2 ? a.T.rc1[b] : a.T.rcm[b];
+    }
     const int64_t pk_w0 = p0 / 16 - 1;  // first staged packed word
     const int64_t vd_w0 = p0 / 32 - 1;  // first staged validity word
     for (int i = threadIdx.x; i < SK_NPK; i += SK_THREADS) {
@@ -154,34 +196,32 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
         uint64_t f = 0, r = 0;
         for (uint32_t i = 0; i < m; ++i) {
             const uint32_t b = base_at(pos + i);
-            f = rotl(f, 1) ^ sel4(b, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]);
-            r ^= rotl(sel4(b ^ 3u, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]), i);
+            f = rotl(f, 1) ^ s_tab[b];
+            r ^= rotl(s_tab[b ^ 3u], i);
         }
         s_h[sk_hidx(q0)] = f <= r ? f : r;
         for (int i = 1; i < cnt; ++i) {
             const uint32_t bo = base_at(pos + i - 1);
             const uint32_t bi = base_at(pos + i + m - 1);
-            f = rotl(f, 1) ^ sel4(bo, a.T.sm[0], a.T.sm[1], a.T.sm[2], a.T.sm[3]) ^
-                sel4(bi, a.T.s[0], a.T.s[1], a.T.s[2], a.T.s[3]);
-            r = rotl(r, 63) ^ sel4(bo, a.T.rc1[0], a.T.rc1[1], a.T.rc1[2], a.T.rc1[3]) ^
-                sel4(bi, a.T.rcm[0], a.T.rcm[1], a.T.rcm[2], a.T.rcm[3]);
+            f = rotl(f, 1) ^ s_tab[4 + bo] ^ s_tab[bi];
+            r = rotl(r, 63) ^ s_tab[8 + bo] ^ s_tab[12 + bi];
             s_h[sk_hidx(q0 + i)] = f <= r ? f : r;
         }
     }
     __syncthreads();
     // 2. window minima of this thread's (k-1)-mers and one neighbour on each
-    //    side: mn[t] = minimizer of (k-1)-mer j0 - 1 + t, t = 0 .. 17
+    //    side: mn[t] = minimizer of (k-1)-mer j0 - 1 + t, t = 0 .. 17, in
+    //    groups of G consecutive windows that share their middle hashes
     const int q0 = int(threadIdx.x) * SK_PER_THREAD;  // s_h index of (k-1)-mer j0 - 1
     uint64_t mn[SK_PER_THREAD + 2];
-#pragma unroll
-    for (int t = 0; t < SK_PER_THREAD + 2; ++t) {
-        uint64_t best = ~0ull;
-        for (uint32_t i = 0; i < W; ++i) {
-            const uint64_t h = s_h[sk_hidx(q0 + t + i)];
-            best = h < best ? h : best;
-        }
-        mn[t] = best;
-    }
+    if (W >= 18)
+        window_minima<18>(s_h, q0, W, mn);
+    else if (W >= 9)
+        window_minima<9>(s_h, q0, W, mn);
+    else if (W >= 6)
+        window_minima<6>(s_h, q0, W, mn);
+    else
+        window_minima<1>(s_h, q0, W, mn);
 
     // 3. (k-1)-mer validity and super-k-mer boundaries
     const uint64_t kmask = (uint64_t(1) << (a.k - 1)) - 1;

@@ -72,3 +72,16 @@ def test_phase1_no_valid_runs(gpu_ctx):
         job.phase1()
         assert job.stats()["num_supers"] == 0
         assert np.all(job.bucket_offsets() == 0)
+
+
+@pytest.mark.parametrize("k,m", [(20, 19), (26, 21), (27, 21), (29, 21), (30, 21), (38, 21), (39, 21),
+                                 (63, 27), (63, 2)])
+def test_phase1_window_widths(gpu_ctx, k, m):
+    """Window widths W = k - m across every grouping of the window minimum
+    (W = 1, 5: direct; 6, 8: groups of 6; 9, 17: groups of 9; 18, 36, 61:
+    groups of 18) against the oracle's direct minimum."""
+    data, off = G.synth(seed=7, num_colors=4, genome_len=30000, snp_rate=0.01, indel_rate=0.001)
+    ob = O.Build(data, off, k, m, 64, threads=4)
+    with gpu_ctx.job(data, off, k, m, 64) as job:
+        job.phase1()
+        _compare(job, ob, 64)
