@@ -42,64 +42,62 @@ WORKLOAD = dict(seed=2, snp_rate=0.02, indel_rate=0.002, indel_max=10, pav_frac=
                 pav_min=1000, pav_max=5000, n_inversions=50, inv_min=10000, inv_max=100000)
 
 
+def record_bytes(s, k):
+    """Uncompressed bucket-record bytes R of the job: per super-k-mer
+    varint(1 colour) + varint(colour) + varint(len << 4 | flags) + ceil(len/4)
+    packed bases (super_kmer.hpp:105-117); colours and lengths below 2^14
+    take 2-byte varints, the length sum is occurrences + (k - 1) per record."""
+    ns, n = s["num_supers"], s["num_tuples"]
+    return ns * 5 + (n + ns * (k - 1)) // 4 + ns // 2
+
+
 def step_bytes(name, s, k, launches, chunks):
-    """Algorithmic (compulsory) HBM bytes moved by all launches of `name` in
-    one step: every buffer touched once, from the job's sizes.  `launches` is
-    the kernel's launch count per step, `chunks` the phase-2 chunks."""
+    """Algorithmic (compulsory) HBM bytes of all launches of `name` in one
+    step, by the per-kernel formulas of the hot-path scope table: tuples of
+    W_k + T bytes (8 + 4 with 64-bit k-mers, 16 + 4 with 128-bit ones), every
+    buffer touched once.  Kernels without a formula return None (their time
+    still counts in the phase-2 aggregate).  `launches` is the kernel's launch
+    count per step, `chunks` the phase-2 chunks."""
     wk = 8 if k <= 31 else 16
-    n, nn, ns, nb = s["num_tuples"], s["num_nodes"], s["num_supers"], s["num_bases"]
-    nf, F, nk = s["num_frags"], s["frag_bytes"], s["num_kmers"]
-    passes = launches / max(chunks, 1)  # radix passes per chunk
+    t = wk + 4
+    n, nn, ns, G = s["num_tuples"], s["num_nodes"], s["num_supers"], s["num_bases"]
+    F, nd, sets, ti = s["frag_bytes"], s["num_kmers"], s["num_color_sets"], s["total_integers"]
+    R = record_bytes(s, k)
+    passes = launches / max(chunks, 1)  # global radix passes per chunk
     table = {
-        # the tuples of every chunk: read key+value, write key+value, per global pass
-        "k_radix_scatter:tuples": passes * 2 * n * (wk + 8),
-        "k_radix_hist:tuples": passes * n * wk,
-        # every tuple is sorted in LDS by exactly one cell class: read and written once
-        "k_cell_sort": n * (wk + 8) * 2,
-        # super records + 2-bit bases in, one (k-mer, value) tuple out per k-mer
-        "k_expand": ns * 29 + nb // 4 + n * (wk + 8),
-        # the 2-bit bases and the ACGT bitmap in; 4-byte start and 2-byte end entries out
-        "k_superkmers": nb // 4 + nb // 8 + ns * 6,
-        # values, one-byte head flags in; node key, meta, rank, colour count, set hash,
-        # long-run flag, and the owned colours out
-        "k_nodes": n * (8 + 1) + nn * (4 + 4 + wk) + nn * (wk + 1 + 4 + 4 + 8 + 4) + n * 4,
-        # node key and bucket in; 8 neighbour lookups (two directory words and a key
-        # each); mask and the two neighbour words out
-        "k_masks": nn * (wk + 4) + 8 * nn * (8 + wk) + nn * (1 + 8),
-        # node key and bucket in, one directory word out per node
-        "k_dir_build": nn * (wk + 4) + nn * 4,
-        # per path position: the used fields of the node's walk record (key, rank,
-        # colour-set id, meta/mask, one link) + path entry in; key, cid, rank,
-        # meta/mask, orientation, fragment flag out
-        "k_walk_record": nn * (wk + 4 + 4 + 2 + 4 + 4) + nn * (wk + 4 + 4 + 2 + 1 + 1),
-        # neighbour words, the neighbours' masks, meta, mask, colour count/offset/slot,
-        # the set id (or the colour and its singleton id), key and rank in; the walk
-        # record (32 B, 64 B for 128-bit keys), links and end flag out
-        "k_links": nn * (8 + 2 + 1 + 1 + 4 + 4 + 4 + 8 + wk + 4) + nn * ((32 if wk == 8 else 64) + 8 + 1),
-        # fragment start, record size, output offset, length, open/direction flags, per
-        # position key/cid/orientation in; records out
-        "k_frag_write": nf * (4 + 4 + 8 + 4 + 1 + 1) + nn * (wk + 4 + 1) + F,
-        # primary nodes: meta, bucket, rank, colour count, offset, hash, first colour in; slot out
-        "k_set_insert": nn * 1 + nk * (4 + 4 + 4 + 4 + 8 + 4 + 4),
+        "k_pack2bit": G + G // 4,                         # A + G/4
+        "k_superkmers": G // 4 + 16 * ns,                 # G/4 + 16 N_sk
+        "k_radix_scatter:buckets": 16 * ns + 2 * R,       # bucket scatter: 16 N_sk + 2R
+        "k_expand": R + n * t,                            # R + N_occ (W_k + T)
+        "k_radix_scatter:tuples": passes * 2 * n * t,     # each radix pass: 2 N_occ (W_k + T)
+        "k_cell_sort": 2 * n * t,                         # the last pass, in LDS: 2 N_occ (W_k + T)
+        "k_nodes": n * t + nn * (wk + 5),                 # segment reduce: N_occ (W_k + T) + N_b (W_k + 5)
+        "k_set_insert": 4 * ti + 4 * nd + 32 * sets,      # colour-set hash/intern
+        "k_dir_build": nn * (wk + 4),                     # table build: N_b (W_k + 4)
+        "k_masks": 8 * (wk + 4) * nn + nn,                # degree: 8 (W_k + 4) N_b + N_b
+        "k_frag_write": nn * (wk + 4) + F,                # fragment emit: N_b (W_k + 4) + F
     }
     return table.get(name)
 
 
+PHASE1_KERNELS = ("k_pack2bit", "k_superkmers", "k_zip_supers", "k_split_supers", "k_bucket_offsets",
+                  "k_radix_hist:buckets", "k_radix_scatter:buckets", "k_count_kmers")
+
+
 def roofline(kt, s, k, steps, chunks, ms_step):
-    """Roofline of the dominant kernel that has an algorithmic-byte model
-    (bytes per launch / average launch time, both from `steps` timed steps)
-    and the aggregate over every modelled phase-2 kernel; plus the six
-    costliest kernels in ms per step."""
+    """Roofline of the dominant kernel that has an algorithmic-byte formula
+    (bytes per launch / average launch time, both from `steps` timed steps),
+    the phase-2 aggregate (modelled bytes of every phase-2 kernel over the
+    time of all phase-2 kernels, modelled or not), and the six costliest
+    kernels in ms per step."""
     roof = None
     agg_b = agg_ms = 0.0
     for name, (cnt, ms) in sorted(kt.items(), key=lambda x: -x[1][1]):
-        b = step_bytes(name, s, k, cnt / steps, chunks)
-        if b is None or cnt == 0:
-            continue
-        if name not in ("k_superkmers",):
-            agg_b += b * steps
+        b = step_bytes(name, s, k, cnt / steps, chunks) if cnt else None
+        if name not in PHASE1_KERNELS:
             agg_ms += ms
-        if roof is not None:
+            agg_b += (b or 0) * steps
+        if b is None or roof is not None:
             continue
         per_launch = b * steps / cnt
         avg_ms = ms / cnt
@@ -115,6 +113,70 @@ def roofline(kt, s, k, steps, chunks, ms_step):
                                    "share_of_step": round(agg_ms / steps / ms_step, 3)}
     top = sorted(kt.items(), key=lambda x: -x[1][1])[:6]
     return roof, {n: round(ms / steps, 2) for n, (c, ms) in top}
+
+
+def kernel_base(name):
+    n = name.replace("(anonymous namespace)::", "").split("(")[0]
+    return n.split("<")[0].split("::")[-1]
+
+
+def pmc_child(args):
+    """One phases-1+2 run of the workload and a 1 GiB copy probe, for a
+    profiler run of the parent (no output besides the counters)."""
+    G = load_pkg()
+    data, off = G.synth(num_colors=args.colors, genome_len=args.genome, threads=args.cpu_cores, **WORKLOAD)
+    ctx = G.Context(0)
+    with ctx.job(data, off, args.k, args.m, args.buckets) as job:
+        job.run()
+    ctx.copy_probe(1 << 30, 1)
+    ctx.close()
+
+
+def measure_traffic(args, kernel, launches):
+    """HBM bytes per launch of `kernel` from two rocprofv3 counter runs of
+    one step (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch), each scaled by
+    what the counter reads for the 1 GiB copy probe of the same run
+    (k_copy_probe reads and writes 1 GiB).  None when the profiler is absent
+    or a run fails."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof or not kernel:
+        return None
+    base = kernel.split(":")[0]
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        with tempfile.TemporaryDirectory(dir="/tmp") as d:
+            cmd = [prof, "--pmc", counter, "-d", d, "-o", "run", "--output-format", "csv", "--",
+                   sys.executable, os.path.abspath(__file__), "--pmc-child", "--colors", str(args.colors),
+                   "--genome", str(args.genome), "--k", str(args.k), "--m", str(args.m),
+                   "--buckets", str(args.buckets), "--cpu-cores", str(args.cpu_cores)]
+            try:
+                subprocess.run(cmd, check=True, timeout=600, stdout=subprocess.DEVNULL,
+                               stderr=subprocess.DEVNULL, env=dict(os.environ, TMPDIR="/tmp"))
+            except (subprocess.SubprocessError, OSError):
+                return None
+            rows = []
+            for root, _, files in os.walk(d):
+                for f in files:
+                    if f.endswith("counter_collection.csv"):
+                        rows += list(csv.DictReader(open(os.path.join(root, f))))
+            kv = [float(r["Counter_Value"]) for r in rows if kernel_base(r["Kernel_Name"]) == base]
+            probe = [float(r["Counter_Value"]) for r in rows if kernel_base(r["Kernel_Name"]) == "k_copy_probe"]
+            if not kv or not probe:
+                return None
+            scale = float(1 << 30) / (probe[-1] * 1024.0)  # true bytes per counted byte
+            # the step's bytes of the kernel (a tagged kernel shares its name with
+            # other sorts: the largest `launches` dispatches are its own) over the
+            # timed run's launches per step: the child may plan other chunks
+            # (its free-memory reading differs under the profiler), and the
+            # bytes of a step do not depend on the chunking
+            kv = sorted(kv)[-launches:] if base != kernel else kv
+            out[counter] = sum(kv) * 1024.0 * scale / launches
+    return {"fetch_bytes": int(out["FETCH_SIZE"]), "write_bytes": int(out["WRITE_SIZE"]),
+            "traffic": int(out["FETCH_SIZE"] + out["WRITE_SIZE"])}
 
 
 # Skewed-bucket (batched) workload: 200 colours of 2 Mbp genomes (1% SNPs,
@@ -287,10 +349,62 @@ def cpu_baseline(G, sample_colors, genome, k, m, B, cores):
     data, off = G.synth(num_colors=sample_colors, genome_len=genome, threads=cores, **WORKLOAD)
     ob = O.Build(data, off, k, m, B, threads=cores)
     t1, t2 = ob.times()
-    nk = ob.stat("num_kmers")
+    nk, nocc = ob.stat("num_kmers"), ob.stat("num_occ")
     return {"value": nk / (t1 + t2), "unit": "distinct k-mers/s", "cores": cores, "kind": "port",
+            "occurrences_per_s": round(nocc / (t1 + t2), 1), "bases_per_s": round(len(data) / (t1 + t2), 1),
             "sample": f"first {sample_colors} of the workload's colours ({len(data)/1e6:.1f} Mbp), "
-                      f"phases 1+2 of the oracle, {nk} distinct k-mers in {t1 + t2:.1f} s"}
+                      f"phases 1+2 of the oracle, {nk} distinct k-mers ({nocc} k-mer occurrences) "
+                      f"in {t1 + t2:.1f} s"}
+
+
+def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
+    """The drop-in file path, timed by wall clock: gzipped FASTA files (one
+    per colour, level 6) through cdbg_gpu_bucket_write (inflate + parse,
+    upload, phase 1, LZ4 bucket files) and cdbg_gpu_bucket_process (read +
+    decompress bucket files, upload, phase 2, frag_unitigs.bin and
+    .color_sets), then the host stitch and emit (.fa, .u2c, finished
+    .color_sets, metadata)."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="cdbg_bench_", dir="/tmp")
+    try:
+        t0 = time.perf_counter()
+        files = G.synth_fasta_files(os.path.join(d, "in"), colors, genome, gz=True, threads=threads,
+                                    **WORKLOAD)
+        t_gen = time.perf_counter() - t0
+        gz_bytes = sum(os.path.getsize(f) for f in files)
+        tmp, out = os.path.join(d, "tmp"), os.path.join(d, "out")
+        os.makedirs(tmp)
+        t = {}
+        t0 = time.perf_counter()
+        unc = ctx.bucket_write(files, k, m, B, tmp, threads=threads)
+        t["bucket_write_s"] = time.perf_counter() - t0
+        frag = os.path.join(tmp, "frag_unitigs.bin")
+        t0 = time.perf_counter()
+        st = ctx.bucket_process(tmp, B, k, colors, frag, out + ".color_sets", threads=threads)
+        t["bucket_process_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        with G.stitch_file(frag, k, threads=threads) as U:
+            nu = U.stats()["num_unitigs"]
+            t["stitch_s"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            U.emit_fasta(out)
+        G.finalize_color_sets(out + ".color_sets", st)
+        G.write_metadata(out, k, st["num_kmers"], colors, nu, st["num_color_sets"])
+        t["emit_s"] = time.perf_counter() - t0
+        p12 = t["bucket_write_s"] + t["bucket_process_s"]
+        res = {"metric": "distinct k-mers/s (phases 1+2, files)", "value": round(st["num_kmers"] / p12, 1),
+               "end_to_end_kmers_per_s": round(st["num_kmers"] / sum(t.values()), 1),
+               "num_kmers": st["num_kmers"], "num_unitigs": nu, "gz_input_bytes": gz_bytes,
+               "bucket_bytes_uncompressed": int(unc.sum()), "frag_bytes": os.path.getsize(frag),
+               "fa_bytes": os.path.getsize(out + ".fa"), "host_threads": threads,
+               **{kk: round(v, 3) for kk, v in t.items()},
+               "workload": f"first {colors} colours of the pangenome workload as gzipped FASTA (generated "
+                           f"in {t_gen:.1f} s, untimed); k={k} m={m} B={B}; scratch under /tmp"}
+        print(f"bench: file path {json.dumps(res)}", file=sys.stderr, flush=True)
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def main():
@@ -313,7 +427,14 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the skewed-bucket (batched) leg")
     ap.add_argument("--copy-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the two profiler runs that measure traffic")
+    ap.add_argument("--file-colors", type=int, default=100,
+                    help="colours of the gzipped-FASTA file-path leg (0: skip it)")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -365,6 +486,18 @@ def main():
     roof["copy_probe_gbps"] = round(copy_gbps, 1)
 
     job.close()
+    if not args.no_traffic:
+        tr = measure_traffic(args, roof["kernel"], roof["launches_per_step"])
+        if tr:
+            roof["traffic"] = tr["traffic"]
+            roof["traffic_fetch"] = tr["fetch_bytes"]
+            roof["traffic_write"] = tr["write_bytes"]
+            roof["traffic_over_model"] = round(tr["traffic"] / roof["bytes_per_launch"], 3)
+        print(f"bench: traffic of {roof['kernel']}: {tr}", file=sys.stderr, flush=True)
+    files = None
+    if args.file_colors:
+        files = file_path_leg(G, ctx, min(args.file_colors, args.colors), args.genome, args.k, args.m,
+                              args.buckets, args.cpu_cores)
     batched = None
     if not args.no_batched:
         batched = batched_leg(G, ctx, args.steps, args.warmup, args.cpu_cores)
@@ -398,6 +531,7 @@ def main():
         "chunks": chunks,
         "batched": batched,
         "served": served,
+        "file_path": files,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -181,6 +181,29 @@ def synth(seed, num_colors, genome_len, snp_rate=0.01, indel_rate=0.001, indel_m
     return data, off
 
 
+def synth_fasta_files(d, num_colors, genome_len, gz=True, threads=8, seed=1, **mut):
+    """Writes the synthetic pangenome as one FASTA file per colour (80-column
+    lines, gzip level 6 with gz) under directory d; returns the paths."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(d, exist_ok=True)
+    p = SynthParams(seed, num_colors, genome_len, mut.get("snp_rate", 0.01), mut.get("indel_rate", 0.001),
+                    mut.get("indel_max", 10), mut.get("pav_frac", 0.0), mut.get("pav_min", 1000),
+                    mut.get("pav_max", 5000), mut.get("n_inversions", 0), mut.get("inv_min", 10000),
+                    mut.get("inv_max", 100000))
+    h = lib().cdbg_synth_create(C.byref(p), threads)
+    if not h:
+        raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+    paths = [os.path.join(d, f"color_{c}.fa" + (".gz" if gz else "")) for c in range(num_colors)]
+    try:
+        def one(c):
+            _check(lib().cdbg_synth_write_fasta(h, c, os.fsencode(paths[c]), 1 if gz else 0))
+        with ThreadPoolExecutor(max(1, threads)) as ex:
+            list(ex.map(one, range(num_colors)))
+    finally:
+        lib().cdbg_synth_free(h)
+    return paths
+
+
 def _cstrs(items):
     arr = (C.c_char_p * len(items))(*[os.fsencode(x) for x in items])
     return arr

@@ -27,6 +27,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -80,15 +81,19 @@ inline FragView view_at(const uint8_t* buf, uint64_t off) {
     return f;
 }
 
-inline int code_of(char c) {
-    switch (c) {
-        case 'A': return 0;
-        case 'C': return 1;
-        case 'G': return 2;
-        case 'T': return 3;
-        default: return -1;
+// 2-bit code of an ACGT byte, -1 for any other byte
+struct CodeTable {
+    int8_t v[256];
+    CodeTable() {
+        for (int i = 0; i < 256; ++i) v[i] = -1;
+        v[int('A')] = 0;
+        v[int('C')] = 1;
+        v[int('G')] = 2;
+        v[int('T')] = 3;
     }
-}
+};
+const CodeTable g_code;
+inline int code_of(char c) { return g_code.v[uint8_t(c)]; }
 
 inline char comp(char c) {
     switch (c) {
@@ -102,12 +107,14 @@ inline char comp(char c) {
 // canonical k-mer of seq[0, k): min of the 2-bit forward and reverse-complement words
 inline u128 canonical_kmer(const char* s, uint32_t k, bool& fwd_is_canonical) {
     u128 f = 0, r = 0;
+    int bad = 0;
     for (uint32_t i = 0; i < k; ++i) {
-        const int c = code_of(s[i]);
-        if (c < 0) throw Error("stitch: non-ACGT base in a fragment sequence");
-        f = (f << 2) | u128(c);
-        r |= u128(3 - c) << (2 * i);
+        const int c = code_of(s[i]), d = code_of(s[k - 1 - i]);
+        bad |= c | d;
+        f = (f << 2) | u128(c & 3);
+        r = (r << 2) | u128(3 - (d & 3));
     }
+    if (bad < 0) throw Error("stitch: non-ACGT base in a fragment sequence");
     fwd_is_canonical = f <= r;
     return fwd_is_canonical ? f : r;
 }
@@ -124,9 +131,11 @@ struct End {
     uint64_t id;  // frag << 2 | side << 1 | fwd_is_canonical   (side 0 left, 1 right)
 };
 
+// fn(begin, end, thread) over `threads` contiguous ranges of [0, n); ranges
+// hold at least `grain` items
 template <class F>
-void parallel_for(uint64_t n, uint32_t threads, F&& fn) {  // fn(begin, end, thread)
-    threads = std::max<uint32_t>(1, std::min<uint64_t>(threads, std::max<uint64_t>(n / 4096, 1)));
+void parallel_for(uint64_t n, uint32_t threads, F&& fn, uint64_t grain = 4096) {
+    threads = std::max<uint32_t>(1, std::min<uint64_t>(threads, std::max<uint64_t>(n / grain, 1)));
     if (threads == 1) {
         fn(uint64_t(0), n, 0u);
         return;
@@ -197,11 +206,13 @@ struct Stitcher {
 
     void match() {
         const uint64_t nf = off.size();
-        partner.assign(2 * nf, NONE);
+        partner.resize(2 * nf);
         // open sides, in fragment order
         std::vector<std::vector<End>> part(threads);
         parallel_for(nf, threads, [&](uint64_t b, uint64_t e, uint32_t t) {
+            std::fill(partner.begin() + 2 * b, partner.begin() + 2 * e, NONE);
             auto& v = part[t];
+            v.reserve(2 * (e - b));
             for (uint64_t i = b; i < e; ++i) {
                 const FragView f = frag(i);
                 for (uint32_t side = 0; side < 2; ++side) {
@@ -218,7 +229,7 @@ struct Stitcher {
         parallel_for(part.size(), threads, [&](uint64_t b, uint64_t e, uint32_t) {
             for (uint64_t t = b; t < e; ++t)
                 for (const End& x : part[t]) ++hcnt[t][mix(x.key) >> (64 - HB)];
-        });
+        }, 1);
         std::vector<uint64_t> boff(NB + 1, 0);
         std::vector<std::vector<uint64_t>> at(part.size(), std::vector<uint64_t>(NB, 0));
         uint64_t run = 0;
@@ -231,21 +242,21 @@ struct Stitcher {
         }
         boff[NB] = run;
         st.open_ends = run;
-        std::vector<End> ends(run);
+        std::unique_ptr<End[]> ends(new End[run]);  // every entry is written below
         parallel_for(part.size(), threads, [&](uint64_t b, uint64_t e, uint32_t) {
             for (uint64_t t = b; t < e; ++t) {
                 for (const End& x : part[t]) ends[at[t][mix(x.key) >> (64 - HB)]++] = x;
                 std::vector<End>().swap(part[t]);
             }
-        });
+        }, 1);
         std::atomic<uint64_t> joins{0}, unmatched{0};
         std::atomic<uint32_t> next{0};
         parallel_for(threads, threads, [&](uint64_t, uint64_t, uint32_t) {
             for (;;) {
                 const uint32_t h = next.fetch_add(1);
                 if (h >= NB) break;
-                End* a = ends.data() + boff[h];
-                End* z = ends.data() + boff[h + 1];
+                End* a = ends.get() + boff[h];
+                End* z = ends.get() + boff[h + 1];
                 std::sort(a, z, [](const End& x, const End& y) { return x.key < y.key || (x.key == y.key && x.id < y.id); });
                 for (End* p = a; p < z;) {
                     End* q = p + 1;
@@ -267,7 +278,7 @@ struct Stitcher {
                     p = q;
                 }
             }
-        });
+        }, 1);
         st.joins = joins;
         st.unmatched_ends = unmatched;
     }
@@ -278,13 +289,12 @@ struct Stitcher {
     void append(Tig& t, uint64_t i, uint32_t side, bool join) const {
         const FragView f = frag(i);
         const bool rev = side == 1;
-        std::vector<Run> r(f.nruns);
-        for (uint32_t j = 0; j < f.nruns; ++j) r[j] = f.run(rev ? f.nruns - 1 - j : j);
-        uint32_t skip = 0;  // k-mers of r already in t
+        auto run_at = [&](uint32_t j) { return f.run(rev ? f.nruns - 1 - j : j); };  // in walk order
+        uint32_t skip = 0;  // k-mers of the fragment already in t
         if (join) {
-            // the shared k-mer is t's last and r's first: keep one copy, with
-            // the owning side's colour (foreign when neither owns it here)
-            const uint64_t a = t.runs.back().cid, b = r.front().cid;
+            // the shared k-mer is t's last and the fragment's first: keep one
+            // copy, with the owning side's colour (foreign when neither owns it)
+            const uint64_t a = t.runs.back().cid, b = run_at(0).cid;
             const uint64_t real = a != COLOR_FOREIGN ? a : b;
             if (real != a) {
                 if (--t.runs.back().n == 0) t.runs.pop_back();
@@ -292,7 +302,8 @@ struct Stitcher {
             }
             skip = 1;
         }
-        for (const Run& x : r) {
+        for (uint32_t j = 0; j < f.nruns; ++j) {
+            const Run x = run_at(j);
             const uint32_t take = x.n > skip ? x.n - skip : 0;
             skip = x.n > skip ? 0 : skip - x.n;
             t.push(x.cid, take);
