@@ -88,6 +88,8 @@ _SIGS = {
     "cdbg_gpu_job_set_chunk_tuples": (C.c_int, [C.c_void_p, C.c_uint64]),
     "cdbg_gpu_job_num_chunks": (C.c_int64, [C.c_void_p]),
     "cdbg_gpu_job_set_bucket_map": (C.c_int, [C.c_void_p, U64P, C.c_uint32]),
+    "cdbg_gpu_job_set_phase1_chunk_bases": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "cdbg_gpu_job_num_phase1_ranges": (C.c_int64, [C.c_void_p]),
     "cdbg_gpu_phase1_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, U64P, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_uint32, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                           C.c_void_p, C.c_void_p, C.c_void_p, U64P, C.c_void_p]),
@@ -413,6 +415,12 @@ class Job:
 
     def set_chunk_tuples(self, n):
         _check(lib().cdbg_gpu_job_set_chunk_tuples(self.h, n))
+
+    def set_phase1_chunk_bases(self, n):
+        _check(lib().cdbg_gpu_job_set_phase1_chunk_bases(self.h, n))
+
+    def num_phase1_ranges(self):
+        return lib().cdbg_gpu_job_num_phase1_ranges(self.h)
 
     def num_chunks(self):
         return lib().cdbg_gpu_job_num_chunks(self.h)

@@ -86,8 +86,10 @@ void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64
     uint8_t* d = J.bytes.ensure<uint8_t>(J.n_pad);
     CDBG_HIP(hipMemsetAsync(d + J.n_bytes, '\n', J.n_pad - J.n_bytes, J.st));
     CDBG_HIP(hipMemcpyAsync(d, bytes + color_off[0], J.n_bytes, hipMemcpyHostToDevice, J.st));
-    std::vector<uint64_t> off(num_colors + 1);
+    std::vector<uint64_t>& off = J.h_color_off;
+    off.resize(num_colors + 1);
     for (uint32_t c = 0; c <= num_colors; ++c) off[c] = color_off[c] - color_off[0];
+    J.p1_auto_bases = 0;
     uint64_t* doff = J.color_off.ensure<uint64_t>(num_colors + 1);
     CDBG_HIP(hipMemcpyAsync(doff, off.data(), off.size() * 8, hipMemcpyHostToDevice, J.st));
     CDBG_HIP(hipStreamSynchronize(J.st));  // the host arrays are pageable and the caller's
@@ -166,6 +168,18 @@ CDBG_API int cdbg_gpu_job_set_bucket_map(cdbg_gpu_job* job, const uint64_t* cuts
         }
         J.phase1_done = J.phase2_done = false;
     })
+}
+
+CDBG_API int cdbg_gpu_job_set_phase1_chunk_bases(cdbg_gpu_job* job, uint64_t max_bases) {
+    if (!job) return set_error("null job");
+    job->J.p1_chunk_bases = max_bases;
+    job->J.phase1_done = job->J.phase2_done = false;
+    return 0;
+}
+
+CDBG_API int64_t cdbg_gpu_job_num_phase1_ranges(cdbg_gpu_job* job) {
+    if (!job) return set_error("null job");
+    return job->J.p1_ranges;
 }
 
 CDBG_API int cdbg_gpu_job_phase1(cdbg_gpu_job* job) {

@@ -23,6 +23,7 @@ struct Phase1Bufs {
     DevBuf bucket_off;                 // B+1
     DevBuf rank0;                      // per record: stream rank of its first k-mer (bucket files only)
     DevBuf cuts;                       // bucket map (B - 1 cuts), when one is set
+    DevBuf rkey_t, sk_all, range_tab;  // colour ranges: sort buffer, every range's sorted records, merge table
 };
 
 struct Phase2Bufs;  // defined in phase2.hip
@@ -65,6 +66,10 @@ struct Job {
     uint64_t chunk_tuples = 0;  // phase-2 chunk budget in k-mer occurrences (0 = from free HBM)
     uint64_t auto_chunk = 0;    // budget derived from free HBM on the first run, then kept
     uint32_t n_chunks = 0;      // chunks used by the last phase-2 run
+    std::vector<uint64_t> h_color_off;  // colour offsets (relative to the first colour), host copy
+    uint64_t p1_chunk_bases = 0;  // phase-1 colour range limit in bases (0 = from free HBM, ~0 = one range)
+    uint64_t p1_auto_bases = 0;   // limit derived from free HBM on the first run, then kept (~0: one range)
+    uint32_t p1_ranges = 0;       // colour ranges used by the last phase-1 run
     ~Job();
 };
 

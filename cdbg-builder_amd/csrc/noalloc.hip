@@ -94,6 +94,7 @@ CDBG_API int cdbg_gpu_phase1_noalloc(cdbg_gpu_ctx* ctx, const uint8_t* d_bytes, 
         J.num_colors = num_colors;
         J.n_bytes = n_bytes;
         J.n_pad = ((n_bytes + 64) / 64) * 64 + 64;  // as cdbg_gpu_job_create pads
+        J.p1_chunk_bases = ~0ull;                   // one pass over the input
         hipStream_t st = J.st;
         uint8_t* d = J.bytes.ensure<uint8_t>(J.n_pad);
         if (n_bytes) CDBG_HIP(hipMemcpyAsync(d, d_bytes + color_off[0], n_bytes, hipMemcpyDeviceToDevice, st));

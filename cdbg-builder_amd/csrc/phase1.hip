@@ -24,6 +24,7 @@
 // record stream phase 2 consumes.
 #include <hip/hip_runtime.h>
 
+#include "hbm_model.h"
 #include "job.h"
 
 namespace cdbg_amd {
@@ -85,6 +86,8 @@ struct SkArgs {
     const uint32_t* packed;
     const uint32_t* valid;
     uint64_t n_bases;
+    uint64_t pos_lo, pos_hi;  // super-k-mers starting and ending in [pos_lo, pos_hi): this launch's share
+    uint64_t tile0;           // first tile of the launch (block b works on tile tile0 + b)
     uint64_t n_pk_words, n_vd_words;
     uint32_t k, m, W, B;
     const uint64_t* cuts;  // null: bucket (h >> 1) % B; else B - 1 ascending cuts of the low 32 bits of h >> 1
@@ -165,7 +168,7 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     __shared__ uint64_t s_tmp[SK_THREADS / 64 + 1];
     __shared__ uint64_t s_tab[16];  // the four per-base tables S, sm, rc1, rcm: a lookup is one LDS read
 
-    const int64_t p0 = int64_t(blockIdx.x) * SK_TILE;
+    const int64_t p0 = int64_t(a.tile0 + blockIdx.x) * SK_TILE;
     if (threadIdx.x < 16) {
         const uint32_t t = threadIdx.x >> 2, b = threadIdx.x & 3u;
         s_tab[threadIdx.x] = t == 0 ? a.T.s[b] : t == 1 ? a.T.sm[b] : t == 2 ? a.T.rc1[b] : a.T.rcm[b];
@@ -242,7 +245,7 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
     for (int i = 0; i < SK_PER_THREAD; ++i) {
         const int64_t j = j0 + i;
         const bool vnext = valid_k1(j + 1);
-        if (vcur && (vprev || vnext) && uint64_t(j) < a.n_bases) {
+        if (vcur && (vprev || vnext) && uint64_t(j) >= a.pos_lo && uint64_t(j) < a.pos_hi) {
             if (!vprev || mn[i + 1] != mn[i]) starts |= 1u << i;
             if (!vnext || mn[i + 1] != mn[i + 2]) ends |= 1u << i;
         }
@@ -288,7 +291,7 @@ struct SuperRec {
 // Pairs the i-th start with the i-th end: record, and its bucket as sort key.
 // One block per tile; its starts are global supers off_s[t] .. off_s[t] +
 // cnt_s[t] - 1, and the i-th end lies in the first tile whose ends reach i.
-__global__ __launch_bounds__(256) void k_zip_supers(uint64_t ntiles, const uint32_t* __restrict__ cnt_s,
+__global__ __launch_bounds__(256) void k_zip_supers(uint64_t tile0, uint64_t ntiles, const uint32_t* __restrict__ cnt_s,
                                                     const uint32_t* __restrict__ cnt_e,
                                                     const uint64_t* __restrict__ off_s,
                                                     const uint64_t* __restrict__ off_e,
@@ -302,14 +305,14 @@ __global__ __launch_bounds__(256) void k_zip_supers(uint64_t ntiles, const uint3
     for (uint32_t j = threadIdx.x; j < ns; j += blockDim.x) {
         const uint64_t i = off_s[t] + j;
         const uint32_t se = st_s[t * SK_TILE + j];
-        const uint64_t sj = t * SK_TILE + (se & 0xFFFu);
+        const uint64_t sj = (tile0 + t) * SK_TILE + (se & 0xFFFu);
         const uint32_t sfl = (se >> 12) & 15u;
         const uint64_t s = (sfl & 1u) ? sj : sj - 1;
         uint64_t te = t;
         while (te > 0 && off_e[te] > i) --te;
         while (te + 1 < ntiles && off_e[te] + cnt_e[te] <= i) ++te;
         const uint32_t ee = st_e[te * SK_TILE + (i - off_e[te])];
-        const uint64_t ej = te * SK_TILE + (ee & 0xFFFu);
+        const uint64_t ej = (tile0 + te) * SK_TILE + (ee & 0xFFFu);
         const uint32_t efl = (ee >> 12) & 15u;
         const uint64_t e = ((efl & 2u) ? ej - 1 : ej) + k;
         // colour = last c with color_off[c] <= s
@@ -357,6 +360,35 @@ __global__ void k_bucket_offsets(uint64_t n, const uint32_t* __restrict__ key, u
         for (int64_t b = cur + 1; b <= int64_t(B); ++b) off[b] = n;
 }
 
+// Chunked phase 1: the bucket-sorted records of one colour range (n_c of
+// them, bucket b at [off_c[b], off_c[b+1])) placed into the job's
+// bucket-major arrays, bucket b of this range starting at base_c[b] (after
+// the same bucket of every earlier range: the records stay in position order
+// inside a bucket).
+__global__ void k_merge_supers(uint64_t n_c, uint32_t B, const SuperRec* __restrict__ rec,
+                               const uint64_t* __restrict__ off_c, const uint64_t* __restrict__ base_c,
+                               uint64_t* __restrict__ b_start, uint32_t* __restrict__ b_len,
+                               uint8_t* __restrict__ b_flags, uint32_t* __restrict__ b_color,
+                               uint32_t* __restrict__ bucket) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_c) return;
+    uint32_t lo = 0, hi = B;  // last b with off_c[b] <= i
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off_c[mid] <= i)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint64_t d = base_c[lo] + (i - off_c[lo]);
+    const SuperRec r = rec[i];
+    b_start[d] = r.start;
+    b_len[d] = r.lenfl >> 4;
+    b_flags[d] = uint8_t(r.lenfl & 15u);
+    b_color[d] = r.color;
+    bucket[d] = lo;
+}
+
 inline uint32_t blocks_for(uint64_t n, uint32_t t) { return uint32_t((n + t - 1) / t); }
 
 __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
@@ -373,10 +405,95 @@ inline uint64_t d2h_u64(const uint64_t* d, hipStream_t st) {
 
 }  // namespace
 
+namespace {
+
+// Super-k-mers that start and end in [lo, hi) (a union of whole colours): the
+// tiles covering the range are processed, their records (position order) and
+// buckets written to rec_b / key_b; returns the count.
+uint64_t supers_of_range(Job& J, SkArgs a, uint64_t lo, uint64_t hi, DevBuf& rec_b, DevBuf& key_b,
+                         unsigned long long* too_long) {
+    hipStream_t st = J.st;
+    Phase1Bufs& P = J.p1;
+    const uint64_t t0 = lo / SK_TILE, t1 = (hi + SK_TILE - 1) / SK_TILE, ntiles = t1 - t0;
+    if (ntiles == 0) return 0;
+    if (ntiles >= (1ull << 31)) throw Error("input too large for one job");
+    a.pos_lo = lo;
+    a.pos_hi = hi;
+    a.tile0 = t0;
+    uint32_t* cnt = P.tile_cnt.ensure<uint32_t>(2 * ntiles);
+    uint64_t* off = P.tile_off.ensure<uint64_t>(2 * ntiles);
+    uint64_t* tot = P.totals.as<uint64_t>();
+    a.cnt_s = cnt;
+    a.cnt_e = cnt + ntiles;
+    a.st_s = P.st_s.ensure<uint32_t>(ntiles * SK_TILE);
+    a.st_e = P.st_e.ensure<uint16_t>(ntiles * SK_TILE);
+    { KSCOPE("k_superkmers", st); k_superkmers<<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
+    exclusive_scan<uint32_t, uint64_t>(cnt, off, ntiles, tot, J.rs.scan_tmp, st);
+    exclusive_scan<uint32_t, uint64_t>(cnt + ntiles, off + ntiles, ntiles, tot + 1, J.rs.scan_tmp, st);
+    uint64_t h_tot[2];
+    CDBG_HIP(hipMemcpyAsync(h_tot, tot, sizeof h_tot, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    if (h_tot[0] != h_tot[1]) throw Error("phase 1: super-k-mer starts and ends do not pair up");
+    const uint64_t n = h_tot[0];
+    if (n >= (1ull << 32)) throw Error("phase 1: more than 2^32 super-k-mers in one job");
+    SuperRec* rec = (SuperRec*)rec_b.ensure<SuperRec>(n);
+    uint32_t* key = key_b.ensure<uint32_t>(n);
+    if (n) {
+        KSCOPE("k_zip_supers", st);
+        k_zip_supers<<<uint32_t(ntiles), 256, 0, st>>>(t0, ntiles, cnt, cnt + ntiles, off, off + ntiles, a.st_s,
+                                                      a.st_e, J.k, J.color_off.as<uint64_t>(), J.num_colors, rec,
+                                                      key, too_long);
+    }
+    return n;
+}
+
+// Colour ranges of phase 1: one range unless the job's phase-1 working set
+// (staging 6 B/base, records and their sort buffers 40 B per super-k-mer,
+// the 2-bit bases and the bucket-major records) would not fit 70% of the
+// free device memory; then ranges of whole colours whose staging and sort
+// buffers fit what the job-wide arrays leave (the plan of the first run is
+// kept for the job).
+std::vector<std::pair<uint64_t, uint64_t>> phase1_ranges(Job& J) {
+    uint64_t cap = J.p1_chunk_bases;
+    if (cap == 0) {
+        if (J.p1_auto_bases == 0) {
+            size_t free_b = 0, total_b = 0;
+            CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
+            const double usable = double(free_b) + double(g_block_cache.cached.load());
+            const double spb = supers_per_base(J.k, J.m), nb = double(J.n_bytes);
+            const double per_base = 6.0 + 40.0 * spb, global = 0.375 + 37.0 * spb;
+            if (nb * (per_base + global) <= 0.7 * usable) {
+                J.p1_auto_bases = ~0ull;
+            } else {
+                const double left = 0.6 * usable - nb * global;
+                J.p1_auto_bases = uint64_t(std::max(double(64ull << 20), left / per_base));
+            }
+        }
+        cap = J.p1_auto_bases;
+    }
+    std::vector<std::pair<uint64_t, uint64_t>> r;
+    const auto& co = J.h_color_off;
+    uint64_t lo = 0;
+    if (cap < J.n_bytes && co.size() == size_t(J.num_colors) + 1) {
+        for (uint32_t c = 1; c < J.num_colors; ++c)
+            if (co[c + 1] - lo > cap && co[c] > lo) {
+                r.push_back({lo, co[c]});
+                lo = co[c];
+            }
+    }
+    r.push_back({lo, J.n_bytes});
+    return r;
+}
+
+}  // namespace
+
 // Phase 1 over J's staged input on J's stream: fills J.p1's bucket-major
 // records and bucket offsets (device and host) and J.n_supers.  Every device
 // buffer comes from DevBuf::ensure, so the caller's allocation scope (block
-// cache or arena) decides where they live.
+// cache or arena) decides where they live.  A job too large for one pass runs
+// in colour ranges: each range's records are sorted by bucket and kept, and
+// the ranges are then merged bucket by bucket in range order, which is the
+// position order of one pass.
 void phase1_core(Job& J) {
     hipStream_t st = J.st;
     Phase1Bufs& P = J.p1;
@@ -411,64 +528,113 @@ void phase1_core(Job& J) {
         a.T.rc1[b] = hrot(S[b ^ 3], 63);
         a.T.rcm[b] = hrot(S[b ^ 3], J.m - 1);
     }
-    const uint64_t ntiles = (J.n_bytes + SK_TILE - 1) / SK_TILE;
-    if (ntiles >= (1ull << 31)) throw Error("input too large for one job");
-    uint32_t* cnt = P.tile_cnt.ensure<uint32_t>(2 * ntiles);
-    uint64_t* off = P.tile_off.ensure<uint64_t>(2 * ntiles);
-    uint64_t* tot = P.totals.ensure<uint64_t>(2);
-    a.cnt_s = cnt;
-    a.cnt_e = cnt + ntiles;
-    a.st_s = P.st_s.ensure<uint32_t>(ntiles * SK_TILE);
-    a.st_e = P.st_e.ensure<uint16_t>(ntiles * SK_TILE);
     if (J.B > 65536) throw Error("phase 1: more than 2^16 buckets");
-    { KSCOPE("k_superkmers", st); k_superkmers<<<uint32_t(ntiles), SK_THREADS, 0, st>>>(a); }
-    exclusive_scan<uint32_t, uint64_t>(cnt, off, ntiles, tot, J.rs.scan_tmp, st);
-    exclusive_scan<uint32_t, uint64_t>(cnt + ntiles, off + ntiles, ntiles, tot + 1, J.rs.scan_tmp, st);
-    uint64_t h_tot[2];
-    CDBG_HIP(hipMemcpyAsync(h_tot, tot, sizeof h_tot, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipStreamSynchronize(st));
-    if (h_tot[0] != h_tot[1]) throw Error("phase 1: super-k-mer starts and ends do not pair up");
-    const uint64_t n = h_tot[0];
-    J.n_supers = n;
-    if (n >= (1ull << 32)) throw Error("phase 1: more than 2^32 super-k-mers in one job");
-
-    // Bucket scatter: stable sort of the records by bucket.
-    SuperRec* rec = (SuperRec*)P.sk_rec.ensure<SuperRec>(n);
-    SuperRec* rec_t = (SuperRec*)P.sk_rec_t.ensure<SuperRec>(n);
-    uint32_t* key = P.key.ensure<uint32_t>(n);
-    uint32_t* key_t = P.key_t.ensure<uint32_t>(n);
-    auto* too_long = P.totals.ensure<uint64_t>(3) + 2;
-    { KSCOPE("k_fill", st); k_fill_u64<<<1, 64, 0, st>>>(too_long, 1, 0ull); }
-    if (n) {
-        KSCOPE("k_zip_supers", st);
-        k_zip_supers<<<uint32_t(ntiles), 256, 0, st>>>(ntiles, cnt, cnt + ntiles, off, off + ntiles, a.st_s, a.st_e,
-                                                      J.k, J.color_off.as<uint64_t>(), J.num_colors, rec, key,
-                                                      (unsigned long long*)too_long);
-    }
+    auto* too_long = (unsigned long long*)(P.totals.ensure<uint64_t>(4) + 2);
+    { KSCOPE("k_fill", st); k_fill_u64<<<1, 64, 0, st>>>((uint64_t*)too_long, 1, 0ull); }
     int bbits = 0;
     while ((uint64_t(1) << bbits) < J.B) ++bbits;
-    SortPlan plan;
-    plan.build({0, n});
-    bool in_temp = false;
-    segmented_radix_sort<uint32_t, SuperRec>(key, rec, key_t, rec_t, plan, 0, bbits, bbits > 10 ? (bbits + 1) / 2 : bbits,
-                                             J.rs, st, "buckets", &in_temp);
-    if (in_temp) {  // the sorted records are in the second buffers: use them as they are
-        P.sk_rec.swap(P.sk_rec_t);
-        P.key.swap(P.key_t);
-        rec = (SuperRec*)P.sk_rec.p;
-        key = (uint32_t*)P.key.p;
+    const int dbits = bbits > 10 ? (bbits + 1) / 2 : bbits;
+    const auto ranges = phase1_ranges(J);
+    J.p1_ranges = uint32_t(ranges.size());
+    uint64_t n = 0;
+    if (ranges.size() == 1) {
+        // Bucket scatter: stable sort of the records by bucket.
+        n = supers_of_range(J, a, 0, J.n_bytes, P.sk_rec, P.key, too_long);
+        SuperRec* rec = (SuperRec*)P.sk_rec.ensure<SuperRec>(n);
+        SuperRec* rec_t = (SuperRec*)P.sk_rec_t.ensure<SuperRec>(n);
+        uint32_t* key = P.key.ensure<uint32_t>(n);
+        uint32_t* key_t = P.key_t.ensure<uint32_t>(n);
+        SortPlan plan;
+        plan.build({0, n});
+        bool in_temp = false;
+        segmented_radix_sort<uint32_t, SuperRec>(key, rec, key_t, rec_t, plan, 0, bbits, dbits, J.rs, st, "buckets",
+                                                 &in_temp);
+        if (in_temp) {  // the sorted records are in the second buffers: use them as they are
+            P.sk_rec.swap(P.sk_rec_t);
+            P.key.swap(P.key_t);
+            rec = (SuperRec*)P.sk_rec.p;
+            key = (uint32_t*)P.key.p;
+        }
+        uint64_t* b_start = P.b_start.ensure<uint64_t>(n);
+        uint32_t* b_len = P.b_len.ensure<uint32_t>(n);
+        uint8_t* b_flags = P.b_flags.ensure<uint8_t>(n);
+        uint32_t* b_color = P.b_color.ensure<uint32_t>(n);
+        uint64_t* boff = P.bucket_off.ensure<uint64_t>(J.B + 1);
+        if (n) { KSCOPE("k_split_supers", st); k_split_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, rec, b_start, b_len, b_flags, b_color); }
+        { KSCOPE("k_bucket_offsets", st); k_bucket_offsets<<<blocks_for(std::max<uint64_t>(n, uint64_t(J.B) + 1), 256), 256, 0, st>>>(n, key, J.B, boff); }
+        CDBG_HIP(hipGetLastError());
+        J.h_bucket_off.resize(J.B + 1);
+        CDBG_HIP(hipMemcpyAsync(J.h_bucket_off.data(), boff, (J.B + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    } else {
+        // every range sorted by bucket on its own and kept (sk_all), with its
+        // bucket offsets on the host
+        const uint64_t B = J.B;
+        std::vector<uint64_t> roff{0};
+        std::vector<std::vector<uint64_t>> h_off;
+        for (size_t r = 0; r < ranges.size(); ++r) {
+            const uint64_t nr = supers_of_range(J, a, ranges[r].first, ranges[r].second, P.sk_rec, P.key_t, too_long);
+            SuperRec* rec = (SuperRec*)P.sk_rec.ensure<SuperRec>(nr);
+            SuperRec* rec_t = (SuperRec*)P.sk_rec_t.ensure<SuperRec>(nr);
+            uint32_t* key = P.key_t.ensure<uint32_t>(nr);
+            uint32_t* key_t = P.rkey_t.ensure<uint32_t>(nr);
+            SortPlan plan;
+            plan.build({0, nr});
+            bool in_temp = false;
+            segmented_radix_sort<uint32_t, SuperRec>(key, rec, key_t, rec_t, plan, 0, bbits, dbits, J.rs, st,
+                                                     "buckets", &in_temp);
+            if (in_temp) {
+                rec = rec_t;
+                key = key_t;
+            }
+            // room for all ranges, extrapolated from the bases done so far
+            const double done = double(ranges[r].second), all = double(J.n_bytes);
+            const size_t hint = size_t(double(roff.back() + nr) * all / done * 1.05) + 1;
+            SuperRec* sk_all = (SuperRec*)P.sk_all.ensure_keep<SuperRec>(roff.back() + nr, roff.back() * sizeof(SuperRec),
+                                                                       st, hint);
+            if (nr) CDBG_HIP(hipMemcpyAsync(sk_all + roff.back(), rec, nr * sizeof(SuperRec), hipMemcpyDeviceToDevice, st));
+            uint64_t* boff = P.bucket_off.ensure<uint64_t>(B + 1);
+            { KSCOPE("k_bucket_offsets", st); k_bucket_offsets<<<blocks_for(std::max<uint64_t>(nr, B + 1), 256), 256, 0, st>>>(nr, key, J.B, boff); }
+            h_off.emplace_back(B + 1);
+            CDBG_HIP(hipMemcpyAsync(h_off.back().data(), boff, (B + 1) * 8, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipStreamSynchronize(st));
+            roff.push_back(roff.back() + nr);
+        }
+        n = roff.back();
+        if (n >= (1ull << 32)) throw Error("phase 1: more than 2^32 super-k-mers in one job");
+        // bucket b of range r starts after bucket b of every earlier range
+        J.h_bucket_off.assign(B + 1, 0);
+        for (uint64_t b = 0; b < B; ++b) {
+            uint64_t c = 0;
+            for (auto& o : h_off) c += o[b + 1] - o[b];
+            J.h_bucket_off[b + 1] = J.h_bucket_off[b] + c;
+        }
+        uint64_t* b_start = P.b_start.ensure<uint64_t>(n);
+        uint32_t* b_len = P.b_len.ensure<uint32_t>(n);
+        uint8_t* b_flags = P.b_flags.ensure<uint8_t>(n);
+        uint32_t* b_color = P.b_color.ensure<uint32_t>(n);
+        uint32_t* bucket = P.key.ensure<uint32_t>(n);
+        uint64_t* tab = P.range_tab.ensure<uint64_t>(2 * B + 1);
+        std::vector<uint64_t> base(J.h_bucket_off.begin(), J.h_bucket_off.end() - 1);
+        for (size_t r = 0; r < ranges.size(); ++r) {
+            const uint64_t nr = roff[r + 1] - roff[r];
+            if (nr) {
+                // tab = [off_r (B + 1) | base_r (B)]; uploaded from pageable memory: waited for below
+                CDBG_HIP(hipMemcpyAsync(tab, h_off[r].data(), (B + 1) * 8, hipMemcpyHostToDevice, st));
+                CDBG_HIP(hipMemcpyAsync(tab + B + 1, base.data(), B * 8, hipMemcpyHostToDevice, st));
+                KSCOPE("k_merge_supers", st);
+                k_merge_supers<<<blocks_for(nr, 256), 256, 0, st>>>(nr, J.B, (SuperRec*)P.sk_all.p + roff[r], tab,
+                                                                    tab + B + 1, b_start, b_len, b_flags, b_color,
+                                                                    bucket);
+                CDBG_HIP(hipStreamSynchronize(st));
+            }
+            for (uint64_t b = 0; b < B; ++b) base[b] += h_off[r][b + 1] - h_off[r][b];
+        }
+        uint64_t* boff = P.bucket_off.ensure<uint64_t>(B + 1);
+        CDBG_HIP(hipMemcpyAsync(boff, J.h_bucket_off.data(), (B + 1) * 8, hipMemcpyHostToDevice, st));
+        CDBG_HIP(hipGetLastError());
     }
-    uint64_t* b_start = P.b_start.ensure<uint64_t>(n);
-    uint32_t* b_len = P.b_len.ensure<uint32_t>(n);
-    uint8_t* b_flags = P.b_flags.ensure<uint8_t>(n);
-    uint32_t* b_color = P.b_color.ensure<uint32_t>(n);
-    uint64_t* boff = P.bucket_off.ensure<uint64_t>(J.B + 1);
-    if (n) { KSCOPE("k_split_supers", st); k_split_supers<<<blocks_for(n, 256), 256, 0, st>>>(n, rec, b_start, b_len, b_flags, b_color); }
-    if (d2h_u64(too_long, st)) throw Error("phase 1: a super-k-mer is longer than 2^28 bases");
-    { KSCOPE("k_bucket_offsets", st); k_bucket_offsets<<<blocks_for(std::max<uint64_t>(n, uint64_t(J.B) + 1), 256), 256, 0, st>>>(n, key, J.B, boff); }
-    CDBG_HIP(hipGetLastError());
-    J.h_bucket_off.resize(J.B + 1);
-    CDBG_HIP(hipMemcpyAsync(J.h_bucket_off.data(), boff, (J.B + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    J.n_supers = n;
+    if (d2h_u64((const uint64_t*)too_long, st)) throw Error("phase 1: a super-k-mer is longer than 2^28 bases");
     CDBG_HIP(hipStreamSynchronize(st));
 }
 

@@ -130,6 +130,16 @@ CDBG_API int cdbg_gpu_copy_probe(cdbg_gpu_ctx* ctx, uint64_t bytes, int iters, d
 CDBG_API int cdbg_gpu_job_set_chunk_tuples(cdbg_gpu_job* job, uint64_t max_tuples);
 CDBG_API int64_t cdbg_gpu_job_num_chunks(cdbg_gpu_job* job);
 
+/* Phase 1 runs over ranges of whole colours when the job's phase-1 working
+ * set (tile staging, records and their sort buffers) does not fit free HBM:
+ * each range is sorted by bucket on its own and the ranges are merged bucket
+ * by bucket, so the records are those of one pass.  max_bases caps a range
+ * (0, the default: from free HBM on the first run; UINT64_MAX: one range).
+ * The reference streams its input file by file through per-thread buffers
+ * (bucket_ingester.hpp:183-215, bucket_io.hpp:605-647). */
+CDBG_API int cdbg_gpu_job_set_phase1_chunk_bases(cdbg_gpu_job* job, uint64_t max_bases);
+CDBG_API int64_t cdbg_gpu_job_num_phase1_ranges(cdbg_gpu_job* job);
+
 /* Bucket map for phase 1: a super-k-mer whose minimizer hash is h goes to
  * the bucket b with cuts[b-1] <= x < cuts[b], x = the low 32 bits of h >> 1
  * (n_cuts = num_buckets - 1 ascending values below 2^32; cuts = NULL restores

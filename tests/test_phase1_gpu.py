@@ -85,3 +85,33 @@ def test_phase1_window_widths(gpu_ctx, k, m):
     with gpu_ctx.job(data, off, k, m, 64) as job:
         job.phase1()
         _compare(job, ob, 64)
+
+
+@pytest.mark.parametrize("chunk", [1, 25_000, 70_000])
+def test_phase1_colour_ranges(gpu_ctx, chunk):
+    """Phase 1 over ranges of whole colours (forced small ranges: one colour
+    each, or a few) gives the records, bucket order and offsets of one pass,
+    also with empty colours, non-ACGT bytes and a bucket map; phase 2 over
+    them gives the same bytes."""
+    data, off = G.synth(seed=13, num_colors=9, genome_len=20000, snp_rate=0.01, indel_rate=0.001)
+    e_data, e_off = _edge_case_job(np.random.default_rng(6))
+    for d, o, k, m, B, cuts in [(data, off, 31, 19, 64, None), (e_data, e_off, 31, 19, 64, None),
+                                (data, off, 63, 27, 128, G.zipf_bucket_cuts(128, 1.1))]:
+        ob = O.Build(d, o, k, m, B, threads=4, cuts=cuts)
+        with gpu_ctx.job(d, o, k, m, B) as job:
+            if cuts is not None:
+                job.set_bucket_map(cuts)
+            job.set_phase1_chunk_bases(chunk)
+            job.run()
+            if len(d) > 2 * chunk:
+                assert job.num_phase1_ranges() > 1
+            _compare(job, ob, B)
+            assert np.array_equal(job.frags(), ob.frags())
+            gw, go = job.color_sets()
+            ow, oo = ob.color_sets()
+            assert np.array_equal(gw, ow) and np.array_equal(go, oo)
+            # the same job in one pass, then in ranges again
+            job.set_phase1_chunk_bases((1 << 64) - 1)
+            job.phase1()
+            assert job.num_phase1_ranges() == 1
+            _compare(job, ob, B)
