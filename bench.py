@@ -224,6 +224,50 @@ def batched_leg(G, ctx, steps, warmup, threads):
                         f"Zipf({c['zipf']}) shares of the minimizer space"}
 
 
+# Large resident build: colours of 5 Mbp genomes at 1.5% divergence (1.5%
+# SNPs, 0.15% indels of 1-10 bp), k=31, m=21, 2^14 buckets; phase 1 runs in
+# colour ranges and phase 2 in bucket chunks, both sized from free HBM.
+LARGE = dict(genome=5_000_000, k=31, m=21, buckets=1 << 14, snp_rate=0.015, indel_rate=0.0015)
+
+
+def large_leg(G, ctx, colors, steps, threads):
+    """Phases 1+2 of the large resident build at `colors` colours: distinct
+    k-mers/s, phase-1 colour ranges, phase-2 chunks, device memory."""
+    c = LARGE
+    t_gen = time.perf_counter()
+    data, off = G.synth(seed=4, num_colors=colors, genome_len=c["genome"], snp_rate=c["snp_rate"],
+                        indel_rate=c["indel_rate"], indel_max=10, threads=threads)
+    print(f"bench: large workload {colors} colours, {len(data) / 1e9:.2f} GB in "
+          f"{time.perf_counter() - t_gen:.1f} s", file=sys.stderr, flush=True)
+    with ctx.job(data, off, c["k"], c["m"], c["buckets"]) as job:
+        del data
+        job.phase1()
+        print(f"bench: large phase 1 done: {job.num_phase1_ranges()} colour ranges, "
+              f"{G.lib().cdbg_gpu_device_bytes() / 2**30:.1f} GiB on device", file=sys.stderr, flush=True)
+        job.phase2()
+        peak = G.lib().cdbg_gpu_device_bytes()
+        print(f"bench: large warmup done ({peak / 2**30:.1f} GiB on device)", file=sys.stderr, flush=True)
+        job.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            job.run()
+        elapsed = time.perf_counter() - t0
+        s = job.stats()
+        kt = job.kernel_times()
+        top = sorted(kt.items(), key=lambda x: -x[1][1])[:12]
+        res = {"metric": "distinct k-mers/s (phases 1+2)", "value": round(s["num_kmers"] * steps / elapsed, 1),
+               "top_kernels_ms_per_step": {n: round(ms / steps, 1) for n, (c, ms) in top},
+               "ms_per_step": round(1e3 * elapsed / steps, 1), "steps": steps,
+               "ms_phase1": round(s["ms_phase1"], 1), "ms_phase2": round(s["ms_phase2"], 1),
+               "phase1_ranges": job.num_phase1_ranges(), "phase2_chunks": job.num_chunks(),
+               "num_bases": s["num_bases"], "num_kmers": s["num_kmers"], "num_occ": s["num_occ"],
+               "device_gib_after_run": round(peak / 2**30, 1),
+               "workload": f"{colors} colours x {c['genome'] / 1e6:g} Mbp, 1.5% SNP, 0.15% indel; k={c['k']} "
+                           f"m={c['m']} B={c['buckets']}; input resident in HBM"}
+    print(f"bench: large {json.dumps(res)}", file=sys.stderr, flush=True)
+    return res
+
+
 def serve_workload(n, rate, seed):
     """Build requests of the serving configuration: colours log-uniform in
     [2, 200], genomes U[0.5, 5] Mbp at U[0.5, 3]% divergence (SNPs; indels at a
@@ -430,6 +474,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the two profiler runs that measure traffic")
     ap.add_argument("--file-colors", type=int, default=100,
                     help="colours of the gzipped-FASTA file-path leg (0: skip it)")
+    ap.add_argument("--large-colors", type=int, default=4000,
+                    help="colours of the large resident build leg (0: skip it)")
+    ap.add_argument("--large-only", action="store_true", help="run only the large resident build leg")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -444,6 +491,11 @@ def main():
         sys.exit(2)
 
     G = load_pkg()
+    if args.large_only:
+        ctx = G.Context(0)
+        print(json.dumps({"large": large_leg(G, ctx, args.large_colors or 4000, args.steps, args.cpu_cores)}))
+        ctx.close()
+        return
     if args.copy_probe_only:
         ctx = G.Context(0)
         print(json.dumps({"copy_probe_bytes": args.copy_probe_only,
@@ -487,27 +539,42 @@ def main():
 
     job.close()
     if not args.no_traffic:
-        tr = measure_traffic(args, roof["kernel"], roof["launches_per_step"])
+        try:
+            tr = measure_traffic(args, roof["kernel"], roof["launches_per_step"])
+        except Exception as e:  # noqa: BLE001
+            print(f"bench: traffic measurement failed: {e}", file=sys.stderr, flush=True)
+            tr = None
         if tr:
             roof["traffic"] = tr["traffic"]
             roof["traffic_fetch"] = tr["fetch_bytes"]
             roof["traffic_write"] = tr["write_bytes"]
             roof["traffic_over_model"] = round(tr["traffic"] / roof["bytes_per_launch"], 3)
         print(f"bench: traffic of {roof['kernel']}: {tr}", file=sys.stderr, flush=True)
+    def leg(fn, *a):
+        """A secondary leg; its failure is reported in its place, not raised."""
+        try:
+            return fn(*a)
+        except Exception as e:  # noqa: BLE001
+            print(f"bench: {fn.__name__} failed: {e}", file=sys.stderr, flush=True)
+            return {"error": str(e)}
+
     files = None
     if args.file_colors:
-        files = file_path_leg(G, ctx, min(args.file_colors, args.colors), args.genome, args.k, args.m,
-                              args.buckets, args.cpu_cores)
+        files = leg(file_path_leg, G, ctx, min(args.file_colors, args.colors), args.genome, args.k, args.m,
+                    args.buckets, args.cpu_cores)
     batched = None
     if not args.no_batched:
-        batched = batched_leg(G, ctx, args.steps, args.warmup, args.cpu_cores)
+        batched = leg(batched_leg, G, ctx, args.steps, args.warmup, args.cpu_cores)
     served = None
     if args.serve_jobs:
-        served = served_leg(G, ctx, args.serve_jobs, args.serve_slots, args.serve_rate, args.cpu_cores)
+        served = leg(served_leg, G, ctx, args.serve_jobs, args.serve_slots, args.serve_rate, args.cpu_cores)
+    large = None
+    if args.large_colors:
+        large = leg(large_leg, G, ctx, args.large_colors, 1, args.cpu_cores)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(G, args.cpu_sample_colors, args.genome, args.k, args.m, args.buckets,
-                           args.cpu_cores)
+        cpu = leg(cpu_baseline, G, args.cpu_sample_colors, args.genome, args.k, args.m, args.buckets,
+                  args.cpu_cores)
     out = {
         "metric": "distinct k-mers/s (phases 1+2)",
         "value": round(value, 1),
@@ -532,6 +599,7 @@ def main():
         "batched": batched,
         "served": served,
         "file_path": files,
+        "large": large,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

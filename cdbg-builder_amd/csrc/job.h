@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -23,7 +24,8 @@ struct Phase1Bufs {
     DevBuf bucket_off;                 // B+1
     DevBuf rank0;                      // per record: stream rank of its first k-mer (bucket files only)
     DevBuf cuts;                       // bucket map (B - 1 cuts), when one is set
-    DevBuf rkey_t, sk_all, range_tab;  // colour ranges: sort buffer, every range's sorted records, merge table
+    DevBuf rkey_t, range_tab;          // colour ranges: sort buffer, merge table
+    std::vector<std::unique_ptr<DevBuf>> range_rec;  // colour ranges: every range's records, sorted by bucket
 };
 
 struct Phase2Bufs;  // defined in phase2.hip
@@ -70,6 +72,7 @@ struct Job {
     uint64_t p1_chunk_bases = 0;  // phase-1 colour range limit in bases (0 = from free HBM, ~0 = one range)
     uint64_t p1_auto_bases = 0;   // limit derived from free HBM on the first run, then kept (~0: one range)
     uint32_t p1_ranges = 0;       // colour ranges used by the last phase-1 run
+    bool frags_in_host = false;   // the last phase-2 run kept its fragment stream in host memory
     ~Job();
 };
 

@@ -536,6 +536,9 @@ void phase1_core(Job& J) {
     const int dbits = bbits > 10 ? (bbits + 1) / 2 : bbits;
     const auto ranges = phase1_ranges(J);
     J.p1_ranges = uint32_t(ranges.size());
+    // a job too large for one pass gives its phase-2 buffers (of an earlier
+    // run) back before phase 1 takes its room: the results are being redone
+    if (ranges.size() > 1) free_phase2(J);
     uint64_t n = 0;
     if (ranges.size() == 1) {
         // Bucket scatter: stable sort of the records by bucket.
@@ -566,9 +569,12 @@ void phase1_core(Job& J) {
         J.h_bucket_off.resize(J.B + 1);
         CDBG_HIP(hipMemcpyAsync(J.h_bucket_off.data(), boff, (J.B + 1) * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     } else {
-        // every range sorted by bucket on its own and kept (sk_all), with its
-        // bucket offsets on the host
+        // every range sorted by bucket on its own and kept in a buffer of its
+        // own (blocks of a range's size are what the block cache reuses for
+        // phase-2 chunks and the next run), with its bucket offsets on the host
         const uint64_t B = J.B;
+        P.range_rec.clear();
+        for (size_t r = 0; r < ranges.size(); ++r) P.range_rec.emplace_back(new DevBuf());
         std::vector<uint64_t> roff{0};
         std::vector<std::vector<uint64_t>> h_off;
         for (size_t r = 0; r < ranges.size(); ++r) {
@@ -586,12 +592,10 @@ void phase1_core(Job& J) {
                 rec = rec_t;
                 key = key_t;
             }
-            // room for all ranges, extrapolated from the bases done so far
-            const double done = double(ranges[r].second), all = double(J.n_bytes);
-            const size_t hint = size_t(double(roff.back() + nr) * all / done * 1.05) + 1;
-            SuperRec* sk_all = (SuperRec*)P.sk_all.ensure_keep<SuperRec>(roff.back() + nr, roff.back() * sizeof(SuperRec),
-                                                                       st, hint);
-            if (nr) CDBG_HIP(hipMemcpyAsync(sk_all + roff.back(), rec, nr * sizeof(SuperRec), hipMemcpyDeviceToDevice, st));
+            if (nr) {
+                SuperRec* kept = (SuperRec*)P.range_rec[r]->ensure<SuperRec>(nr);
+                CDBG_HIP(hipMemcpyAsync(kept, rec, nr * sizeof(SuperRec), hipMemcpyDeviceToDevice, st));
+            }
             uint64_t* boff = P.bucket_off.ensure<uint64_t>(B + 1);
             { KSCOPE("k_bucket_offsets", st); k_bucket_offsets<<<blocks_for(std::max<uint64_t>(nr, B + 1), 256), 256, 0, st>>>(nr, key, J.B, boff); }
             h_off.emplace_back(B + 1);
@@ -622,7 +626,7 @@ void phase1_core(Job& J) {
                 CDBG_HIP(hipMemcpyAsync(tab, h_off[r].data(), (B + 1) * 8, hipMemcpyHostToDevice, st));
                 CDBG_HIP(hipMemcpyAsync(tab + B + 1, base.data(), B * 8, hipMemcpyHostToDevice, st));
                 KSCOPE("k_merge_supers", st);
-                k_merge_supers<<<blocks_for(nr, 256), 256, 0, st>>>(nr, J.B, (SuperRec*)P.sk_all.p + roff[r], tab,
+                k_merge_supers<<<blocks_for(nr, 256), 256, 0, st>>>(nr, J.B, P.range_rec[r]->as<SuperRec>(), tab,
                                                                     tab + B + 1, b_start, b_len, b_flags, b_color,
                                                                     bucket);
                 CDBG_HIP(hipStreamSynchronize(st));
@@ -632,6 +636,11 @@ void phase1_core(Job& J) {
         uint64_t* boff = P.bucket_off.ensure<uint64_t>(B + 1);
         CDBG_HIP(hipMemcpyAsync(boff, J.h_bucket_off.data(), (B + 1) * 8, hipMemcpyHostToDevice, st));
         CDBG_HIP(hipGetLastError());
+        CDBG_HIP(hipStreamSynchronize(st));
+        // a job this large needs the room in phase 2: the per-range buffers
+        // and the kept ranges go back to the cache
+        for (DevBuf* b : {&P.st_s, &P.st_e, &P.sk_rec, &P.sk_rec_t, &P.key_t, &P.rkey_t}) b->release();
+        P.range_rec.clear();
     }
     J.n_supers = n;
     if (d2h_u64((const uint64_t*)too_long, st)) throw Error("phase 1: a super-k-mer is longer than 2^28 bases");

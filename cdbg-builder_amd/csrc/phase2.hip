@@ -20,8 +20,8 @@
 // seen in a chunk sorts after all sets of earlier chunks: its global id and
 // its place in the colour-set bit stream are final when the chunk ends, as in
 // the reference's streaming dictionary.  The singleton table, the multi-set
-// hash table with its colour-list store, the colour-set stream and the
-// fragment stream persist across chunks.
+// hash table (128-bit content hashes, no colour lists), the colour-set stream
+// and the fragment stream persist across chunks.
 //
 // Kernels per chunk (tuple = one k-mer occurrence):
 //   k_expand<K>               thread per tuple: canonical k-mer straight from
@@ -31,10 +31,12 @@
 //   k_tile_heads / k_nodes    node per run of equal keys, numbered per tile:
 //                             flags, primary, min rank, owned colours
 //   k_set_*                   colour-set interning: singleton table keyed by
-//                             colour, content-hash table for larger sets with
-//                             an exact list compare against the stored list;
-//                             new sets sorted by key, ids assigned, hybrid
-//                             encoded (hybrid_color_sets.hpp:41-96)
+//                             colour, content-hash table for larger sets keyed
+//                             by 64 bits and checked on the other 64 of a
+//                             128-bit content hash (the reference's rule:
+//                             dedup on the hash alone); new sets sorted by key,
+//                             ids assigned, hybrid encoded
+//                             (hybrid_color_sets.hpp:41-96)
 //   k_dir_build<K>            per-bucket directory over the sorted node keys
 //   k_masks<K> / k_links<K>   8-neighbour ext masks (local_ext_mask :260-271)
 //                             and clean links (walk_step :317-336), kept with
@@ -128,6 +130,12 @@ __device__ __forceinline__ void block_add(unsigned long long* dst, uint64_t v) {
 __global__ void k_count_kmers(uint64_t n, const uint32_t* __restrict__ len, uint32_t k, uint32_t* __restrict__ cnt) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) cnt[i] = len[i] - k + 1;
+}
+
+// v[i] += add for i < n
+__global__ void k_add_u64(uint64_t* __restrict__ v, uint64_t n, uint64_t add) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] += add;
 }
 
 __global__ void k_bucket_tuple_off(uint32_t B, const uint64_t* __restrict__ boff, const uint64_t* __restrict__ tup_off,
@@ -631,12 +639,19 @@ __global__ void k_node_off(uint32_t nb, const uint64_t* __restrict__ lbtup, cons
 constexpr uint32_t BIG_RUN = 32;
 // counters: 0 primary nodes, 1 multi-colour primary nodes, 2 total integers,
 // 3 nodes placed by the walk, 4 fragments, 5 walk errors, 6 colour-set
-// collisions, 7 colours of multi-colour nodes, 8 big nodes
+// collisions, 7 colours of multi-colour nodes, 8 big nodes, 10 paths outside
+// the end walk, 12 fragment ranks out of range, 13 new multi-colour sets
 constexpr int N_COUNTERS = 16;
 
 // Colour-set hash: a sum of mixed terms over the sorted unique list, so a
 // wave can add its lanes' terms in any order and get the thread's value.
 __device__ __forceinline__ uint64_t set_term(uint32_t c) { return mix64(uint64_t(c) + 0x9E3779B97F4A7C15ull); }
+// a second, independent term: the two sums together are the set's 128-bit
+// content hash (the reference dedups on a 128-bit content hash alone,
+// streaming_color_set_dict.hpp:329-341)
+__device__ __forceinline__ uint64_t set_term2(uint32_t c) {
+    return mix64((uint64_t(c) * 0xD6E8FEB86659FD93ull) ^ 0x5851F42D4C957F2Dull);
+}
 __device__ __forceinline__ uint64_t set_hash_fin(uint64_t sum, uint32_t n) {
     return mix64(sum ^ (uint64_t(n) * 0xC2B2AE3D27D4EB4Full)) | 1ull;  // 0 marks an empty slot
 }
@@ -661,7 +676,8 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
                                                const uint64_t* __restrict__ vals, K* __restrict__ node_key,
                                                uint8_t* __restrict__ node_meta, uint32_t* __restrict__ node_rank,
                                                uint32_t* __restrict__ node_ncol, uint64_t* __restrict__ node_hash,
-                                               uint32_t* __restrict__ colors, uint8_t* __restrict__ big_flag) {
+                                               uint64_t* __restrict__ node_hash2, uint32_t* __restrict__ colors,
+                                               uint8_t* __restrict__ big_flag) {
     __shared__ uint64_t s_val[NT_TILE + BIG_RUN];
     __shared__ uint32_t s_list[NT_TILE];
     __shared__ uint32_t s_wn[TPB / 64 + 1];
@@ -735,7 +751,7 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
         big_flag[n] = is_big;
         if (is_big) continue;
         uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
-        uint64_t sum = 0;
+        uint64_t sum = 0, sum2 = 0;
         bool primary = false;
         for (uint32_t t = t0; t < t1; ++t) {
             const uint64_t v = s_val[t - T0];
@@ -748,6 +764,7 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
                 if (c != last) {
                     colors[t0 + ncol] = c;
                     sum += set_term(c);
+                    sum2 += set_term2(c);
                     ++ncol;
                     last = c;
                 }
@@ -757,6 +774,7 @@ __global__ __launch_bounds__(TPB) void k_nodes(uint64_t nt, const uint8_t* __res
         node_rank[n] = rank;
         node_ncol[n] = ncol;
         node_hash[n] = set_hash_fin(sum, ncol);
+        node_hash2[n] = sum2;
     }
 }
 
@@ -769,7 +787,8 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ node_start,
                                                    const uint64_t* __restrict__ vals, uint8_t* __restrict__ node_meta,
                                                    uint32_t* __restrict__ node_rank, uint32_t* __restrict__ node_ncol,
-                                                   uint64_t* __restrict__ node_hash, uint32_t* __restrict__ colors) {
+                                                   uint64_t* __restrict__ node_hash, uint64_t* __restrict__ node_hash2,
+                                                   uint32_t* __restrict__ colors) {
     const int lane = threadIdx.x & 63;
     const uint64_t lt = lanes_below(lane);
     const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
@@ -777,7 +796,7 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
         const uint32_t n = big[i];
         const uint32_t t0 = node_start[n], t1 = node_start[n + 1];
         uint32_t fl = 0, rank = 0xFFFFFFFFu, ncol = 0, last = 0xFFFFFFFFu;
-        uint64_t sum = 0;
+        uint64_t sum = 0, sum2 = 0;
         bool primary = false;
         for (uint32_t base = t0; base < t1; base += 64 * BIG_UNROLL) {
             uint64_t v[BIG_UNROLL];
@@ -806,6 +825,7 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
                 if (fresh) {
                     colors[t0 + ncol + __popcll(fm & lt)] = c;
                     sum += set_term(c);
+                    sum2 += set_term2(c);
                 }
                 ncol += __popcll(fm);
                 if (om) {
@@ -820,12 +840,14 @@ __global__ __launch_bounds__(TPB) void k_nodes_big(const uint32_t* __restrict__ 
             const uint32_t r = __shfl_xor(rank, o, 64);
             rank = r < rank ? r : rank;
             sum += __shfl_xor(sum, o, 64);
+            sum2 += __shfl_xor(sum2, o, 64);
         }
         if (lane == 0) {
             node_meta[n] = uint8_t(fl | (primary ? 4u : 0u));
             node_rank[n] = rank;
             node_ncol[n] = ncol;
             node_hash[n] = set_hash_fin(sum, ncol);
+            node_hash2[n] = sum2;
         }
     }
 }
@@ -861,17 +883,17 @@ struct SetTables {
     uint32_t* single_cid;            // [num_colors]
     unsigned long long* mt_hk;       // [2 cap] per slot: hash (0 = empty), min key; one line per probe
     uint32_t* mt_cid;                // [cap]
-    uint64_t* mt_loff;               // [cap] colour list in `store`, or (IN_CHUNK | offset) in cur_colors
+    uint64_t* mt_h2;                 // [cap] the second half of the set's 128-bit content hash
+    uint64_t* mt_loff;               // [cap] colour list of a set created in this chunk, in cur_colors
     uint32_t* mt_llen;               // [cap]
     uint64_t cap;
-    uint32_t* store;
     const uint32_t* cur_colors;      // node colour lists of the chunk being processed
 };
-constexpr uint64_t IN_CHUNK = uint64_t(1) << 63;
 
+// The colour list of a set created in the chunk being processed (the list
+// of the node that created it); sets of earlier chunks keep no list.
 __device__ __forceinline__ const uint32_t* slot_list(const SetTables& T, uint64_t s) {
-    const uint64_t o = T.mt_loff[s];
-    return (o & IN_CHUNK) ? T.cur_colors + (o & ~IN_CHUNK) : T.store + o;
+    return T.cur_colors + T.mt_loff[s];
 }
 
 // Colour lists live at colors[coff[n]] (coff = the node's first occurrence).
@@ -879,7 +901,8 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
                              const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
                              const uint64_t* __restrict__ node_hash, const uint32_t* __restrict__ rank,
                              const uint32_t* __restrict__ node_bucket, uint32_t b0, SetTables T,
-                             uint32_t* __restrict__ node_slot, uint8_t* __restrict__ created) {
+                             const uint64_t* __restrict__ node_hash2, uint32_t* __restrict__ node_slot,
+                             uint8_t* __restrict__ created) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (n >= n_nodes) return;
     if (!(meta[n] & 4u)) {
@@ -906,8 +929,9 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
         if (cur == 0ull) {
             cur = atomicCAS(&T.mt_hk[2 * s], 0ull, h);
             if (cur == 0ull) {  // this thread created the slot: its list stands for the set
-                T.mt_loff[s] = IN_CHUNK | coff[n];
+                T.mt_loff[s] = coff[n];
                 T.mt_llen[s] = nc;
+                T.mt_h2[s] = node_hash2[n];
                 cur = h;
                 made = true;
             }
@@ -923,43 +947,21 @@ __global__ void k_set_insert(uint32_t n_nodes, const uint8_t* __restrict__ meta,
     created[n] = 0;  // table full (cannot happen at the load kept below 1/2)
 }
 
-// Exact check of every multi-colour node's list against the list stored with
-// its hash slot; nodes with long runs are checked by k_set_verify_big.
+// Every multi-colour primary node has the set of its hash slot: the second
+// half of the 128-bit content hash and the size agree (a mismatch is a 64-bit
+// hash collision between two sets, reported as an error).
 __global__ void k_set_verify(uint32_t n_nodes, const uint8_t* __restrict__ meta, const uint32_t* __restrict__ ncol,
-                             const uint32_t* __restrict__ coff, const uint32_t* __restrict__ colors,
-                             const uint32_t* __restrict__ node_slot, SetTables T, unsigned long long* __restrict__ err) {
+                             const uint64_t* __restrict__ node_hash2, const uint32_t* __restrict__ node_slot,
+                             SetTables T, unsigned long long* __restrict__ err) {
     const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2 || coff[n + 1] - coff[n] > BIG_RUN) return;
+    if (n >= n_nodes || !(meta[n] & 4u) || ncol[n] < 2) return;
     const uint32_t s = node_slot[n];
-    const uint32_t* ref = slot_list(T, s);
-    bool same = T.mt_llen[s] == ncol[n];
-    for (uint32_t i = 0; same && i < ncol[n]; ++i) same = colors[coff[n] + i] == ref[i];
-    if (!same) atomicAdd(err, 1ull);
-}
-
-__global__ __launch_bounds__(TPB) void k_set_verify_big(const uint32_t* __restrict__ big, uint64_t nbig,
-                                                        const uint8_t* __restrict__ meta,
-                                                        const uint32_t* __restrict__ ncol,
-                                                        const uint32_t* __restrict__ coff,
-                                                        const uint32_t* __restrict__ colors,
-                                                        const uint32_t* __restrict__ node_slot, SetTables T,
-                                                        unsigned long long* __restrict__ err) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
-    if (i >= nbig) return;
-    const uint32_t n = big[i];
-    if (!(meta[n] & 4u) || ncol[n] < 2) return;
-    const uint32_t s = node_slot[n], nc = ncol[n];
-    const uint32_t* ref = slot_list(T, s);
-    const uint32_t* own = colors + coff[n];
-    bool diff = T.mt_llen[s] != nc;
-    for (uint32_t j = uint32_t(lane); !diff && j < nc; j += 64) diff = own[j] != ref[j];
-    if (__ballot(diff) && lane == 0) atomicAdd(err, 1ull);
+    if (T.mt_h2[s] != node_hash2[n] || T.mt_llen[s] != ncol[n]) atomicAdd(err, 1ull);
 }
 
 __global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restrict__ ohk,
-                            const uint32_t* __restrict__ oc,
-                            const uint64_t* __restrict__ oo, const uint32_t* __restrict__ ol, SetTables T) {
+                            const uint32_t* __restrict__ oc, const uint64_t* __restrict__ oh2,
+                            const uint32_t* __restrict__ ol, SetTables T) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= old_cap || ohk[2 * i] == 0ull) return;
     const unsigned long long h = ohk[2 * i];
@@ -968,7 +970,7 @@ __global__ void k_mt_rehash(uint64_t old_cap, const unsigned long long* __restri
         if (atomicCAS(&T.mt_hk[2 * s], 0ull, h) == 0ull) {
             T.mt_hk[2 * s + 1] = ohk[2 * i + 1];
             T.mt_cid[s] = oc[i];
-            T.mt_loff[s] = oo[i];
+            T.mt_h2[s] = oh2[i];
             T.mt_llen[s] = ol[i];
             return;
         }
@@ -1019,10 +1021,10 @@ __device__ __forceinline__ SetColors set_list(uint64_t desc, const SetTables& T)
 }
 
 __global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t* __restrict__ set_desc, SetTables T,
-                              uint32_t num_colors, uint64_t* __restrict__ bits, uint64_t* __restrict__ keep,
+                              uint32_t num_colors, uint64_t* __restrict__ bits,
                               unsigned long long* __restrict__ counters) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    uint64_t ints = 0;
+    uint64_t ints = 0, multi = 0;
     if (i < n_new) {
         const uint64_t d = set_desc[i];
         if (d >> 63)
@@ -1051,10 +1053,11 @@ __global__ void k_set_cid_len(uint64_t n_new, uint64_t cid_base, const uint64_t*
             }
         }
         bits[i] = L;
-        keep[i] = (d >> 63) ? n : 0;  // colours to keep in the store
         ints = n;
+        multi = d >> 63;
     }
     block_add(&counters[2], ints);
+    block_add(&counters[13], multi);
 }
 
 // ORs the low `len` (<= 64) bits of v in at bit `pos` (words zeroed beforehand).
@@ -1215,24 +1218,6 @@ __global__ __launch_bounds__(TPB) void k_set_encode(uint64_t n_new, uint64_t cid
             prev = top;
         }
     }
-}
-
-// Copies the lists of the chunk's new multi-colour sets into the job-wide
-// store (STORE_LANES lanes per set) and points their slots at the copies.
-constexpr int STORE_LANES = 16;  // lanes per set (most lists are short)
-__global__ __launch_bounds__(TPB) void k_store_new(uint64_t n_new, const uint64_t* __restrict__ set_desc,
-                                                   const uint64_t* __restrict__ keep_off, uint64_t base, SetTables T) {
-    const int lane = threadIdx.x & (STORE_LANES - 1);
-    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) / STORE_LANES;
-    if (i >= n_new) return;
-    const uint64_t d = set_desc[i];
-    if (!(d >> 63)) return;
-    const uint64_t s = d & 0xFFFFFFFFFFFFull;
-    const uint32_t* src = slot_list(T, s);
-    const uint32_t n = T.mt_llen[s];
-    uint32_t* dst = T.store + base + keep_off[i];
-    for (uint32_t j = uint32_t(lane); j < n; j += STORE_LANES) dst[j] = src[j];
-    if (lane == 0) T.mt_loff[s] = base + keep_off[i];
 }
 
 // Everything the walk reads of a node, in one record: a path position
@@ -2200,15 +2185,14 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
 struct Phase2Bufs {
     // job-wide
     DevBuf kcnt, tup_off, btup, total, lbtup;
-    DevBuf single_min, single_cid, mt_hk, mt_cid, mt_loff, mt_llen, store, keep, keep_off;
-    uint64_t store_used = 0;  // colours held in `store`
+    DevBuf single_min, single_cid, mt_hk, mt_cid, mt_h2, mt_loff, mt_llen;
     DevBuf set_off, cs_words, frag_bytes;
     uint64_t mt_cap = 0, mt_count = 0;
     // per chunk (reused)
-    DevBuf keys, vals, keys_t, vals_t, block_super;
+    DevBuf keys, vals, keys_t, vals_t, block_super, ckcnt, ctup_off;
     DevBuf cell_start, cell_base, cell_dbits, cell_mid, cell_mid2, cell_over, cell_novf, cell_oflag, cell_oscan, fb_start, fb_pre, fb_k, fb_v, fb_kt, fb_vt;
     DevBuf heads, tile_heads, node_start, node_off;
-    DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, big, big_flag, big_cnt, colors, counters;
+    DevBuf node_key, node_meta, node_rank, node_ncol, node_hash, node_hash2, big, big_flag, big_cnt, colors, counters;
     DevBuf node_slot, set_fl, set_pos, set_key, set_desc, set_key_t, set_desc_t, set_bits, set_boff;
     DevBuf node_bucket, dir, filt, node_mask, nb, link, visited;
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fcnt, fkey;
@@ -2216,6 +2200,25 @@ struct Phase2Bufs {
     DevBuf node_rec, eflag, ecnt, ends;
     DevBuf fkey_t, forder, forder_t, fdval, fpsize, fpboff, fsize, fboff;
     RadixScratch rs2;
+    // bytes of the per-chunk buffers (what a new chunk reuses)
+    uint64_t chunk_bytes() const {
+        uint64_t b = 0;
+        for (const DevBuf* x : {&keys, &vals, &keys_t, &vals_t, &block_super, &ckcnt, &ctup_off, &cell_start, &cell_mid,
+                                &cell_mid2, &cell_over, &cell_oflag, &cell_oscan, &fb_k, &fb_v, &fb_kt, &fb_vt, &heads,
+                                &tile_heads, &node_start, &node_key, &node_meta, &node_rank, &node_ncol, &node_hash,
+                                &node_hash2, &big, &big_flag, &colors, &node_slot, &set_fl, &set_pos, &set_key,
+                                &set_desc, &set_key_t, &set_desc_t, &set_bits, &set_boff, &node_bucket, &dir, &filt,
+                                &node_mask, &nb, &link, &visited, &frag_n, &frag_open, &frag_key, &frag_dir, &fflag,
+                                &fkey, &comp_ex, &comp_len, &comp_nodes, &comp_off, &other_x, &other_ex, &prc, &pinfo,
+                                &prank, &pkey, &pcid, &node_rec, &eflag, &ends, &fkey_t, &forder, &forder_t, &fdval,
+                                &fpsize, &fpboff, &fsize, &fboff})
+            b += x->cap;
+        return b;
+    }
+    // fragment stream kept in host memory, chunk by chunk, when the job's
+    // projected stream does not fit beside its working set in HBM
+    bool spill = false;
+    PinnedBuf h_frags;
     // results
     uint64_t n_tuples = 0, n_nodes = 0, n_sets = 0, n_frags = 0, frag_total = 0, cs_bits = 0;
     uint64_t n_primary = 0, total_integers = 0, words_zeroed = 0;
@@ -2241,25 +2244,28 @@ static SetTables set_tables(Phase2Bufs& P) {
     T.single_cid = P.single_cid.as<uint32_t>();
     T.mt_hk = P.mt_hk.as<unsigned long long>();
     T.mt_cid = P.mt_cid.as<uint32_t>();
+    T.mt_h2 = P.mt_h2.as<uint64_t>();
     T.mt_loff = P.mt_loff.as<uint64_t>();
     T.mt_llen = P.mt_llen.as<uint32_t>();
     T.cap = P.mt_cap;
-    T.store = P.store.as<uint32_t>();
     T.cur_colors = nullptr;
     return T;
 }
 
 // (Re)allocates the colour-set hash table with `cap` slots, keeping entries.
+// The list offsets belong to the sets of the chunk about to run, which has
+// not inserted any yet: they are not carried over (the sizes are).
 static void mt_resize(Phase2Bufs& P, uint64_t cap, hipStream_t st) {
-    DevBuf ohk, oc, oo, ol;
+    DevBuf ohk, oc, oh2, ol;
     const uint64_t old_cap = P.mt_cap;
     ohk.swap(P.mt_hk);
     oc.swap(P.mt_cid);
-    oo.swap(P.mt_loff);
+    oh2.swap(P.mt_h2);
     ol.swap(P.mt_llen);
     P.mt_cap = cap;
     P.mt_hk.ensure<unsigned long long>(2 * cap);
     P.mt_cid.ensure<uint32_t>(cap);
+    P.mt_h2.ensure<uint64_t>(cap);
     P.mt_loff.ensure<uint64_t>(cap);
     P.mt_llen.ensure<uint32_t>(cap);
     { KSCOPE("k_fill2", st); k_fill2<unsigned long long><<<blocks_for(cap), TPB, 0, st>>>(P.mt_hk.as<unsigned long long>(), cap, 0ull, ~0ull); }
@@ -2267,7 +2273,7 @@ static void mt_resize(Phase2Bufs& P, uint64_t cap, hipStream_t st) {
     if (old_cap) {
         KSCOPE("k_mt_rehash", st);
         k_mt_rehash<<<blocks_for(old_cap), TPB, 0, st>>>(old_cap, ohk.as<unsigned long long>(), oc.as<uint32_t>(),
-                                                          oo.as<uint64_t>(), ol.as<uint32_t>(), set_tables(P));
+                                                          oh2.as<uint64_t>(), ol.as<uint32_t>(), set_tables(P));
     }
     CDBG_HIP(hipStreamSynchronize(st));
 }
@@ -2298,17 +2304,30 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     CDBG_HIP(hipMemcpyAsync(lbtup, lb.data(), (nb + 1) * 8, hipMemcpyHostToDevice, st));
     CDBG_HIP(hipStreamSynchronize(st));
 
+    // tuple offsets of the chunk's records: the job's array, or (jobs of
+    // several chunks, which drop it) the chunk's own, indexed by record
+    const uint64_t* tup_off = P.tup_off.as<uint64_t>();
+    if (!tup_off) {
+        const uint64_t nc = s1 - s0;
+        uint32_t* ck = P.ckcnt.ensure<uint32_t>(nc);
+        uint64_t* co = P.ctup_off.ensure<uint64_t>(nc + 1);
+        { KSCOPE("k_count_kmers", st); k_count_kmers<<<blocks_for(nc), TPB, 0, st>>>(nc, Q.b_len.as<uint32_t>() + s0, k, ck); }
+        exclusive_scan<uint32_t, uint64_t>(ck, co, nc, co + nc, J.rs.scan_tmp, st);
+        { KSCOPE("k_add_u64", st); k_add_u64<<<blocks_for(nc + 1), TPB, 0, st>>>(co, nc + 1, t0); }
+        tup_off = co - s0;
+    }
+
     // --- expand
     K* keys = P.keys.ensure<K>(nt);
     uint64_t* vals = P.vals.ensure<uint64_t>(nt);
     K* keys_t = P.keys_t.ensure<K>(nt);
     uint64_t* vals_t = P.vals_t.ensure<uint64_t>(nt);
-    ExpArgs ea{nt, t0, s0, s1, P.tup_off.as<uint64_t>(), P.btup.as<uint64_t>(), Q.key.as<uint32_t>(),
+    ExpArgs ea{nt, t0, s0, s1, tup_off, P.btup.as<uint64_t>(), Q.key.as<uint32_t>(),
                Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(),
                Q.packed.as<uint32_t>(), k, J.from_files ? Q.rank0.as<uint32_t>() : nullptr};
     const uint64_t n_eblocks = (nt + EXP_TILE - 1) / EXP_TILE;
     uint32_t* block_super = P.block_super.ensure<uint32_t>(n_eblocks + 1);
-    { KSCOPE("k_block_super", st); k_block_super<<<blocks_for(s1 - s0), TPB, 0, st>>>(s0, s1, t0, n_eblocks, P.tup_off.as<uint64_t>(), block_super); }
+    { KSCOPE("k_block_super", st); k_block_super<<<blocks_for(s1 - s0), TPB, 0, st>>>(s0, s1, t0, n_eblocks, tup_off, block_super); }
     { KSCOPE("k_expand", st); k_expand<K><<<uint32_t(n_eblocks), TPB, 0, st>>>(ea, block_super, keys, vals); }
 
     // --- sort tuples by k-mer inside every bucket (stable): the top D_max
@@ -2442,20 +2461,21 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint32_t* rank = P.node_rank.ensure<uint32_t>(nn);
     uint32_t* ncol = P.node_ncol.ensure<uint32_t>(nn);
     uint64_t* nhash = P.node_hash.ensure<uint64_t>(nn);
+    uint64_t* nhash2 = P.node_hash2.ensure<uint64_t>(nn);
     uint32_t* big = P.big.ensure<uint32_t>(nn);
     uint32_t* colors = P.colors.ensure<uint32_t>(nt);  // node lists in place at their first occurrence
     const uint32_t* coff = node_start;
     unsigned long long* cnt = P.counters.ensure<unsigned long long>(N_COUNTERS);
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(cnt, N_COUNTERS, 0ull); }
     uint8_t* big_flag = P.big_flag.ensure<uint8_t>(nn);
-    { KSCOPE("k_nodes", st); k_nodes<K><<<uint32_t(n_ntiles), TPB, 0, st>>>(nt, heads, tile_base, lbtup, nb, node_bucket, node_start, keys, vals, node_key, meta, rank, ncol, nhash, colors, big_flag); }
+    { KSCOPE("k_nodes", st); k_nodes<K><<<uint32_t(n_ntiles), TPB, 0, st>>>(nt, heads, tile_base, lbtup, nb, node_bucket, node_start, keys, vals, node_key, meta, rank, ncol, nhash, nhash2, colors, big_flag); }
     { KSCOPE("k_set_u32", st); k_set_u32<<<1, 64, 0, st>>>(node_start + nn, uint32_t(nt)); }
     { KSCOPE("k_node_off", st); k_node_off<<<blocks_for(nb + 1), TPB, 0, st>>>(nb, lbtup, node_start, nn, node_off); }
     uint32_t* d_nbig = (uint32_t*)(dtot + 2);
     compact_flags<uint8_t>(big_flag, nn, big, d_nbig, P.big_cnt, J.rs.scan_tmp, st);
     const uint64_t nbig = d2h<uint32_t>(d_nbig, st);
     unsigned long long h_cnt[N_COUNTERS];
-    if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, colors); }
+    if (nbig) { KSCOPE("k_nodes_big", st); k_nodes_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, node_start, vals, meta, rank, ncol, nhash, nhash2, colors); }
     { KSCOPE("k_count_nodes", st); k_count_nodes<<<4096, TPB, 0, st>>>(nn, meta, ncol, cnt); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
@@ -2469,9 +2489,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     T.cur_colors = colors;
     uint32_t* node_slot = P.node_slot.ensure<uint32_t>(nn);
     uint8_t* created = P.set_fl.ensure<uint8_t>(nn);  // 1: the node created its set in this chunk
-    { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, nhash, rank, node_bucket, b0, T, node_slot, created); }
-    { KSCOPE("k_set_verify", st); k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
-    if (nbig) { KSCOPE("k_set_verify_big", st); k_set_verify_big<<<blocks_for(nbig, TPB / 64), TPB, 0, st>>>(big, nbig, meta, ncol, coff, colors, node_slot, T, cnt + 6); }
+    { KSCOPE("k_set_insert", st); k_set_insert<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, coff, colors, nhash, rank, node_bucket, b0, T, nhash2, node_slot, created); }
+    { KSCOPE("k_set_verify", st); k_set_verify<<<blocks_for(nn), TPB, 0, st>>>(nn, meta, ncol, nhash2, node_slot, T, cnt + 6); }
     const uint32_t NC = J.num_colors;
     uint32_t* set_list = P.set_pos.ensure<uint32_t>(nn);
     uint32_t* d_new = (uint32_t*)(dtot + 3);
@@ -2489,14 +2508,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     const uint64_t cid_base = P.n_sets;
     uint64_t* set_bits = P.set_bits.ensure<uint64_t>(n_new);
     uint64_t* set_boff = P.set_boff.ensure<uint64_t>(n_new + 1);
-    uint64_t* keep = P.keep.ensure<uint64_t>(n_new);
-    uint64_t* keep_off = P.keep_off.ensure<uint64_t>(n_new + 1);
-    if (n_new) { KSCOPE("k_set_cid_len", st); k_set_cid_len<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, set_desc, T, NC, set_bits, keep, cnt); }
-    exclusive_scan<uint64_t, uint64_t>(keep, keep_off, n_new, keep_off + n_new, J.rs.scan_tmp, st);
-    const uint64_t kept = d2h<uint64_t>(keep_off + n_new, st);
-    P.store.ensure_keep<uint32_t>(P.store_used + kept + 1, P.store_used * 4, st, grown(P.store_used + kept + 1));
-    T = set_tables(P);
-    T.cur_colors = colors;
+    if (n_new) { KSCOPE("k_set_cid_len", st); k_set_cid_len<<<blocks_for(n_new), TPB, 0, st>>>(n_new, cid_base, set_desc, T, NC, set_bits, cnt); }
     exclusive_scan<uint64_t, uint64_t>(set_bits, set_boff, n_new, set_boff + n_new, J.rs.scan_tmp, st);
     const uint64_t new_bits = d2h<uint64_t>(set_boff + n_new, st);
     const uint64_t bit_base = P.cs_bits;
@@ -2510,11 +2522,9 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     }
     uint64_t* set_off = P.set_off.ensure_keep<uint64_t>(cid_base + n_new, cid_base * 8, st, grown(cid_base + n_new));
     if (n_new) { KSCOPE("k_set_encode", st); k_set_encode<<<blocks_for(n_new, TPB / 64), TPB, 0, st>>>(n_new, cid_base, bit_base, set_desc, set_boff, T, NC, words, set_off); }
-    if (n_new) { KSCOPE("k_store_new", st); k_store_new<<<blocks_for(n_new, TPB / STORE_LANES), TPB, 0, st>>>(n_new, set_desc, keep_off, P.store_used, T); }
-    P.store_used += kept;
     P.n_sets += n_new;
     P.cs_bits += new_bits;
-    P.mt_count += n_new;  // upper bound of occupied slots (singletons included): sizes the table
+    // P.mt_count (the multi-colour sets in the table) grows at the chunk's end
 
     // --- per-bucket directory over the sorted keys, masks, links
     uint32_t* dir = P.dir.ensure<uint32_t>(uint64_t(nn) + nb);
@@ -2631,18 +2641,39 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     exclusive_scan<uint64_t, uint64_t>(fsize, fboff, nf, fboff + nf, J.rs.scan_tmp, st);
     if (nf) { KSCOPE("k_frag_pboff", st); k_frag_pboff<<<blocks_for(nf), TPB, 0, st>>>(nf, dval, fboff, pboff); }
     const uint64_t fbytes = d2h<uint64_t>(fboff + nf, st);
-    uint8_t* fout = P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st, grown(P.frag_total + fbytes));
-    if (nf) { KSCOPE("k_frag_write", st); k_frag_write_pos<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout + P.frag_total, pos_cap); }
+    if (P.frag_total == 0 && t_done < t_all && !P.spill) {
+        // the first chunk decides where the job's fragment stream lives: in
+        // HBM only when its size, extrapolated from this chunk, leaves room
+        // for the chunks' working set; else in host memory, chunk by chunk
+        size_t free_b = 0, total_b = 0;
+        CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
+        const double projected = double(fbytes) * double(t_all) / double(t_done);
+        const double room = double(free_b) + double(g_block_cache.cached.load()) + double(P.frag_bytes.cap);
+        const char* force = getenv("CDBG_FRAG_SPILL");  // testing: keep the stream in host memory
+        P.spill = projected > 0.35 * room || (force && *force == '1');
+    }
+    // the chunk's records go after the stream's (at its start when the
+    // stream is kept in host memory)
+    uint8_t* fout = P.spill ? P.frag_bytes.ensure<uint8_t>(fbytes)
+                            : P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st,
+                                                                grown(P.frag_total + fbytes)) + P.frag_total;
+    if (nf) { KSCOPE("k_frag_write", st); k_frag_write_pos<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout, pos_cap); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     if (h_cnt[12]) throw Error("phase 2: fragment seed rank outside its bucket's tuples");
+    if (P.spill && fbytes) {
+        uint8_t* h = P.h_frags.ensure_keep(P.frag_total + fbytes, P.frag_total, grown(P.frag_total + fbytes));
+        CDBG_HIP(hipMemcpyAsync(h + P.frag_total, fout, fbytes, hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+    }
     P.frag_total += fbytes;
     P.n_frags += nf;
     P.n_nodes += nn;
     P.n_tuples += nt;
     P.n_primary += n_primary;
     P.total_integers += h_cnt[2];
+    P.mt_count += h_cnt[13];  // multi-colour sets in the table: sizes it
 }
 
 template <class K>
@@ -2669,7 +2700,6 @@ static void phase2_impl(Job& J) {
     auto* single_cid = P.single_cid.ensure<uint32_t>(NC);
     { KSCOPE("k_fill", st); k_fill<unsigned long long><<<blocks_for(NC), TPB, 0, st>>>(single_min, NC, ~0ull); }
     { KSCOPE("k_fill", st); k_fill<uint32_t><<<blocks_for(NC), TPB, 0, st>>>(single_cid, NC, NONE32); }
-    P.store_used = 0;
     if (P.mt_cap == 0) {
         mt_resize(P, 1024, st);
     } else {  // clear, keeping the capacity of the previous run
@@ -2703,13 +2733,40 @@ static void phase2_impl(Job& J) {
         J.auto_chunk = budget;  // later runs of this job plan the same chunks
     }
     uint32_t nchunks = 0;
+    P.spill = false;  // the host stream keeps its capacity for the next run
+    {
+        // a job of several chunks computes its tuple offsets chunk by chunk
+        uint32_t b1 = 1;
+        while (b1 < B && h_btup[b1 + 1] <= budget) ++b1;
+        if (b1 < B) {
+            P.kcnt.release();
+            P.tup_off.release();
+        }
+    }
+    const bool adaptive = J.chunk_tuples == 0;
     for (uint32_t b0 = 0; b0 < B;) {
+        // the chunk also fits what the job-wide structures (colour-set table
+        // and streams) leave free now: they grow from chunk to chunk
+        uint64_t cap = budget;
+        if (adaptive && b0 > 0) {
+            size_t free_b = 0, total_b = 0;
+            CDBG_HIP(hipMemGetInfo(&free_b, &total_b));
+            cap = std::min(cap, chunk_budget(uint64_t(free_b) + g_block_cache.cached.load() + P.chunk_bytes(), k));
+        }
         uint32_t b1 = b0 + 1;
-        while (b1 < B && h_btup[b1 + 1] - h_btup[b0] <= budget) ++b1;
-        run_chunk<K>(J, P, b0, b1, h_btup);
+        while (b1 < B && h_btup[b1 + 1] - h_btup[b0] <= cap) ++b1;
+        try {
+            run_chunk<K>(J, P, b0, b1, h_btup);
+        } catch (Error& e) {
+            throw Error(std::string(e.what()) + " [phase-2 chunk " + std::to_string(nchunks + 1) + ": buckets " +
+                        std::to_string(b0) + ".." + std::to_string(b1 - 1) + ", " +
+                        std::to_string(h_btup[b1] - h_btup[b0]) + " of " + std::to_string(h_btup[B]) +
+                        " occurrences, budget " + std::to_string(budget) + "]");
+        }
         ++nchunks;
         b0 = b1;
     }
+    J.frags_in_host = P.spill;
     J.n_chunks = nchunks;
 
     CDBG_HIP(hipEventRecord(e1, st));
@@ -2748,7 +2805,9 @@ void copy_phase2_results_device(Job& J, void* frags, void* words, void* offsets,
     if (!J.phase2_done) throw Error("phase 2 has not run");
     Phase2Bufs& P = *J.p2;
     CDBG_HIP(hipStreamSynchronize(J.st));
-    if (frags && P.frag_total) CDBG_HIP(hipMemcpyAsync(frags, P.frag_bytes.p, P.frag_total, hipMemcpyDeviceToDevice, st));
+    if (frags && P.frag_total)
+        CDBG_HIP(hipMemcpyAsync(frags, P.spill ? (const void*)P.h_frags.p : P.frag_bytes.p, P.frag_total,
+                                P.spill ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, st));
     const uint64_t nw = (P.cs_bits + 63) / 64;
     if (words && nw) CDBG_HIP(hipMemcpyAsync(words, P.cs_words.p, nw * 8, hipMemcpyDeviceToDevice, st));
     if (offsets && P.n_sets) CDBG_HIP(hipMemcpyAsync(offsets, P.set_off.p, P.n_sets * 8, hipMemcpyDeviceToDevice, st));
@@ -2759,7 +2818,12 @@ void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offs
     AllocStream alloc_on_job_stream(J.st);
     if (!J.phase2_done) throw Error("phase 2 has not run");
     Phase2Bufs& P = *J.p2;
-    if (frags && P.frag_total) CDBG_HIP(hipMemcpy(frags, P.frag_bytes.p, P.frag_total, hipMemcpyDeviceToHost));
+    if (frags && P.frag_total) {
+        if (P.spill)
+            std::memcpy(frags, P.h_frags.p, P.frag_total);
+        else
+            CDBG_HIP(hipMemcpy(frags, P.frag_bytes.p, P.frag_total, hipMemcpyDeviceToHost));
+    }
     const uint64_t nw = (P.cs_bits + 63) / 64;
     if (words && nw) CDBG_HIP(hipMemcpy(words, P.cs_words.p, nw * 8, hipMemcpyDeviceToHost));
     if (offsets && P.n_sets) CDBG_HIP(hipMemcpy(offsets, P.set_off.p, P.n_sets * 8, hipMemcpyDeviceToHost));

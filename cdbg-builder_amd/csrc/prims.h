@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <chrono>
 #include <mutex>
@@ -121,7 +122,14 @@ struct BlockCache {
             if (hipMalloc(&p, bytes) != hipSuccess) {
                 (void)hipGetLastError();
                 trim();
-                CDBG_HIP(hipMalloc(&p, bytes));
+                if (hipMalloc(&p, bytes) != hipSuccess) {
+                    (void)hipGetLastError();
+                    size_t f = 0, t = 0;
+                    (void)hipMemGetInfo(&f, &t);
+                    throw Error("device allocation of " + std::to_string(bytes) + " bytes: out of memory (" +
+                                std::to_string(g_device_bytes.load()) + " bytes held by device buffers, " +
+                                std::to_string(f) + " free)");
+                }
             }
         }
         ++n_malloc;
@@ -280,6 +288,35 @@ struct DevBuf {
         CDBG_HIP(hipStreamSynchronize(st));
         swap(nb);
         return (T*)p;
+    }
+};
+
+// Pinned host buffer that grows keeping its contents (hipHostMalloc: device
+// copies into it run at full link speed, without staging).
+struct PinnedBuf {
+    uint8_t* p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    // at least n bytes, the first `keep` kept; a growth takes at least
+    // `hint` bytes and 1.5x the old capacity
+    uint8_t* ensure_keep(size_t n, size_t keep, size_t hint = 0) {
+        if (n <= cap) return p;
+        const size_t want = std::max(std::max(n, cap + cap / 2), hint);
+        void* q = nullptr;
+        CDBG_HIP(hipHostMalloc(&q, want, hipHostMallocDefault));
+        if (keep && p) std::memcpy(q, p, std::min(keep, cap));
+        release();
+        p = (uint8_t*)q;
+        cap = want;
+        return p;
     }
 };
 

@@ -182,3 +182,13 @@ def test_phase2_skewed_bucket_map(gpu_ctx, k, m, B, s, chunk):
     map, also in several chunks."""
     data, off = G.synth(seed=12, num_colors=6, genome_len=30000, snp_rate=0.01, indel_rate=0.001)
     _assert_same(*_run_both(gpu_ctx, data, off, k, m, B, chunk=chunk, cuts=G.zipf_bucket_cuts(B, s)))
+
+
+@pytest.mark.parametrize("k,m,B,chunk", [(31, 19, 64, 20000), (63, 27, 97, 30000)])
+def test_phase2_fragment_stream_in_host_memory(gpu_ctx, k, m, B, chunk, monkeypatch):
+    """A job whose fragment stream would not fit HBM keeps it in host memory
+    chunk by chunk (forced here after the first chunk): the same bytes, from
+    the job and from its files."""
+    monkeypatch.setenv("CDBG_FRAG_SPILL", "1")
+    data, off = G.synth(seed=8, num_colors=7, genome_len=25000, snp_rate=0.015, indel_rate=0.001)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, B, chunk=chunk))
