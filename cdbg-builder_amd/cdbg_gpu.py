@@ -131,6 +131,8 @@ _SIGS = {
     "cdbg_seqs_free": (None, [C.c_void_p]),
     "cdbg_lz4_compress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
     "cdbg_lz4_decompress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
+    "cdbg_gpu_lz4_compress_frames": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, U8P, C.c_uint64, U64P]),
+    "cdbg_gpu_lz4_decompress_frames": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, U8P, U64P]),
     "cdbg_synth_create": (C.c_void_p, [C.POINTER(SynthParams), C.c_uint32]),
     "cdbg_synth_total_bytes": (C.c_uint64, [C.c_void_p]),
     "cdbg_synth_bytes": (U8P, [C.c_void_p]),
@@ -342,6 +344,33 @@ class Context:
 
     def server(self, slots=4, hbm_budget=0, keep_results=True, borrow_inputs=True):
         return Server(self, slots, hbm_budget, keep_results, borrow_inputs)
+
+    def lz4_compress_frames(self, frames):
+        """LZ4 blocks of the byte strings `frames` (each <= 64 KiB), compressed
+        on the device, one lane per frame."""
+        frames = [bytes(f) for f in frames]
+        src = np.frombuffer(b"".join(frames) + bytes(8), np.uint8).copy()
+        off = np.zeros(len(frames) + 1, np.uint64)
+        off[1:] = np.cumsum([len(f) for f in frames], dtype=np.uint64)
+        cap = int(sum(len(f) + len(f) // 255 + 16 for f in frames)) + 1
+        dst = np.zeros(cap, np.uint8)
+        coff = np.zeros(len(frames) + 1, np.uint64)
+        _check(lib().cdbg_gpu_lz4_compress_frames(self.h, _ptr(src, U8P), _ptr(off, U64P), len(frames),
+                                                  _ptr(dst, U8P), cap, _ptr(coff, U64P)))
+        return [dst[int(coff[i]):int(coff[i + 1])].tobytes() for i in range(len(frames))]
+
+    def lz4_decompress_frames(self, blocks, sizes):
+        """Device decode of LZ4 blocks; block i must decode to sizes[i] bytes."""
+        blocks = [bytes(b) for b in blocks]
+        src = np.frombuffer(b"".join(blocks) + bytes(8), np.uint8).copy()
+        coff = np.zeros(len(blocks) + 1, np.uint64)
+        coff[1:] = np.cumsum([len(b) for b in blocks], dtype=np.uint64)
+        doff = np.zeros(len(blocks) + 1, np.uint64)
+        doff[1:] = np.cumsum(sizes, dtype=np.uint64)
+        dst = np.zeros(int(doff[-1]) + 1, np.uint8)
+        _check(lib().cdbg_gpu_lz4_decompress_frames(self.h, _ptr(src, U8P), _ptr(coff, U64P), len(blocks),
+                                                    _ptr(dst, U8P), _ptr(doff, U64P)))
+        return [dst[int(doff[i]):int(doff[i + 1])].tobytes() for i in range(len(blocks))]
 
     def bucket_write(self, files, k, m, num_buckets, tmp_dir, threads=8):
         unc = np.zeros(num_buckets, np.uint64)

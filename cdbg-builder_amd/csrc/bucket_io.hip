@@ -5,19 +5,37 @@
 //   decompressed stream = records
 //     varint(n_colors) varint(c0) varint(c1-c0)... varint((len<<4)|flags)
 //     ceil(len/4) bytes, base i in bits 2(i%4) of byte i/4.
-// Phase 1 writes one record per super-k-mer occurrence (one colour), in
-// colour-major position order inside each bucket: a valid bucket stream
-// (the reference's online compaction only merges identical records, which
-// "is purely a memory bound, never a correctness one", algorithm.md:264-266).
-// Records are serialized on the device; LZ4 framing and file writes run on
-// host threads.
 //
-// Reading a bucket file (bucket_reader, bucket_io.hpp:651-720) builds the
-// phase-2 input: the records' packed bases, and one (record, colour) pair
-// per colour of every record with len >= k (load_bucket skips shorter ones,
-// bucket_walker.hpp:179), ordered colour-major inside the bucket.  Each pair
-// carries the stream rank of its record's first k-mer, so first-occurrence
-// order is that of the file.
+// Writing (the compacting writer, bucket_io.hpp:127-277), all on the device:
+//   * records with equal bases, length and ownership bits in one bucket merge
+//     into one record carrying the sorted unique colours of all of them and
+//     the AND of their run-begin/run-end bits (:145-160, 221-240); records
+//     keep the order of their first occurrence (the insertion order of the
+//     reference's map).  The reference merges inside a spill window and on
+//     the bases alone; merging over the whole bucket only merges more
+//     (dedup "is purely a memory bound, never a correctness one",
+//     algorithm.md:264-266), and keying on the ownership bits as well keeps
+//     the phase-2 result the same as over unmerged records (a one-k-mer
+//     record met at a run start and at a run end of one bucket carries
+//     different ownership bits, and AND-ing them would drop a colour).
+//     Grouping: a 64-bit hash of (bucket, length, ownership, bases) in an
+//     open-addressing table whose slot keeps the group's first record;
+//     every record is compared base by base with that record, and one that
+//     differs (a hash collision) stays a record of its own;
+//   * the stream is cut into frames of at most 64 KiB inside each bucket
+//     (the reference's spill size, builder.hpp:525), LZ4-compressed one lane
+//     per frame, packed with their headers into the files' image, and the
+//     host only writes each file's bytes.
+//
+// Reading (bucket_reader, bucket_io.hpp:651-720): host threads read the
+// files; frames are LZ4-decoded on the device into one stream; one thread
+// per bucket walks its records (stopping at the first malformed one, like
+// read_super_kmer, super_kmer.hpp:121-143) and emits one (record, colour)
+// pair per colour of every record with len >= k (load_bucket skips shorter
+// ones, bucket_walker.hpp:179); pairs are stably sorted by colour inside
+// each bucket.  Each pair carries the stream rank of its record's first
+// k-mer, so first-occurrence order is that of the file.  The decoded stream
+// itself holds the 2-bit bases phase 2 reads.
 //
 // Phase-2 files: frag_unitigs.bin (unitig_spill.hpp:491-522), the colour-set
 // stream <out>.color_sets as the reference leaves it at the end of phase 2 (a
@@ -36,16 +54,18 @@
 
 #include "job.h"
 #include "kmer_ops.h"
-#include "lz4_block.h"
+#include "lz4_device.h"
 
 namespace cdbg_amd {
 
 namespace {
 
 constexpr int TPB = 256;
-constexpr size_t FRAME_BYTES = 256 * 1024;
+constexpr int WALK_TPB = 64;  // bucket walkers: one thread per bucket
 
-inline uint32_t blocks_for(uint64_t n) { return uint32_t(std::max<uint64_t>(1, (n + TPB - 1) / TPB)); }
+inline uint32_t blocks_for(uint64_t n, int tpb = TPB) {
+    return uint32_t(std::max<uint64_t>(1, (n + tpb - 1) / tpb));
+}
 
 __host__ __device__ __forceinline__ uint32_t varint_len(uint64_t v) {
     uint32_t n = 1;
@@ -64,14 +84,22 @@ __device__ __forceinline__ uint8_t* put_varint(uint8_t* p, uint64_t v) {
     return p;
 }
 
-__global__ void k_record_size(uint64_t n, const uint32_t* __restrict__ len, const uint8_t* __restrict__ flags,
-                              const uint32_t* __restrict__ color, uint64_t* __restrict__ size) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t lf = (uint64_t(len[i]) << 4) | (flags[i] & 15u);
-    size[i] = varint_len(1) + varint_len(color[i]) + varint_len(lf) + (uint64_t(len[i]) + 3) / 4;
+// LEB128 read bounded by `end` (varint_read, super_kmer.hpp:71-82): at most
+// ten bytes, a truncated value is what was read so far.
+__device__ __forceinline__ uint64_t get_varint(const uint8_t* __restrict__ d, uint64_t end, uint64_t& q) {
+    uint64_t v = 0;
+    uint32_t sh = 0;
+    while (q < end) {
+        const uint32_t b = d[q++];
+        v |= uint64_t(b & 0x7f) << sh;
+        if (!(b & 0x80)) return v;
+        sh += 7;
+        if (sh >= 64) break;
+    }
+    return v;
 }
 
+// 32 bases from base p of a 2-bit stream (16 bases per u32, first base low).
 __device__ __forceinline__ uint64_t window64(const uint32_t* __restrict__ packed, uint64_t p) {
     const uint64_t w = p >> 4;
     const uint32_t sh = 2 * uint32_t(p & 15);
@@ -81,149 +109,289 @@ __device__ __forceinline__ uint64_t window64(const uint32_t* __restrict__ packed
     return v;
 }
 
-__global__ void k_record_write(uint64_t n, const uint64_t* __restrict__ start, const uint32_t* __restrict__ len,
-                               const uint8_t* __restrict__ flags, const uint32_t* __restrict__ color,
-                               const uint32_t* __restrict__ packed, const uint64_t* __restrict__ off,
-                               uint8_t* __restrict__ out) {
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// Largest b with off[b] <= i (off ascending, off[0] = 0, B + 1 entries).
+__device__ __forceinline__ uint32_t segment_of(const uint64_t* __restrict__ off, uint32_t B, uint64_t i) {
+    uint32_t lo = 0, hi = B;  // off[lo] <= i < off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+constexpr uint8_t OWN_BITS = 0xC;  // SK_FLAG_OWNS_FIRST | SK_FLAG_OWNS_LAST (super_kmer.hpp:96-97)
+
+// ---- writer: merge identical records --------------------------------------
+
+struct RecArgs {
+    uint64_t n;
+    uint32_t B;
+    const uint64_t* boff;  // records of bucket b: [boff[b], boff[b+1])
+    const uint64_t* start;
+    const uint32_t* len;
+    const uint8_t* flags;
+    const uint32_t* color;
+    const uint32_t* packed;
+};
+
+__global__ void k_dedup_insert(RecArgs a, uint64_t* __restrict__ tab_key, uint32_t* __restrict__ tab_first,
+                               uint64_t mask, uint32_t* __restrict__ rslot, uint32_t* __restrict__ fl_and) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint8_t* p = out + off[i];
-    p = put_varint(p, 1);
-    p = put_varint(p, color[i]);
-    const uint32_t L = len[i];
-    p = put_varint(p, (uint64_t(L) << 4) | (flags[i] & 15u));
-    const uint64_t s = start[i];
+    if (i >= a.n) return;
+    const uint32_t b = segment_of(a.boff, a.B, i);
+    const uint32_t L = a.len[i];
+    const uint64_t s = a.start[i];
+    uint64_t h = mix64((uint64_t(b) << 36) ^ (uint64_t(L) << 4) ^ (a.flags[i] & OWN_BITS) ^ 0x9e3779b97f4a7c15ull);
+    for (uint32_t j = 0; j < L; j += 32) {
+        uint64_t w = window64(a.packed, s + j);
+        if (L - j < 32) w &= (uint64_t(1) << (2 * (L - j))) - 1;
+        h = mix64(h + w * 0xff51afd7ed558ccdull + j);
+    }
+    if (h == 0) h = 1;  // 0 marks an empty slot
+    uint64_t slot = h & mask;
+    for (;;) {
+        const uint64_t cur = tab_key[slot];
+        if (cur == h) break;
+        if (cur == 0) {
+            const uint64_t prev = atomicCAS((unsigned long long*)&tab_key[slot], 0ull, (unsigned long long)h);
+            if (prev == 0 || prev == h) break;
+        }
+        slot = (slot + 1) & mask;
+    }
+    atomicMin(&tab_first[slot], uint32_t(i));
+    rslot[i] = uint32_t(slot);
+    fl_and[i] = 15;
+}
+
+// grp[i] = the first record of i's group (i itself when it leads, or when
+// the record in its slot differs from it); lkey[i] = grp[i] local to the
+// bucket, the sort key that gathers each group's colours.
+__global__ void k_dedup_group(RecArgs a, const uint32_t* __restrict__ tab_first, const uint32_t* __restrict__ rslot,
+                              uint32_t* __restrict__ grp, uint32_t* __restrict__ lkey, uint32_t* __restrict__ fl_and) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const uint32_t b = segment_of(a.boff, a.B, i);
+    uint64_t g = tab_first[rslot[i]];
+    if (g != i) {
+        bool same = g >= a.boff[b] && a.len[g] == a.len[i] && ((a.flags[g] ^ a.flags[i]) & OWN_BITS) == 0;
+        const uint32_t L = a.len[i];
+        for (uint32_t j = 0; same && j < L; j += 32) {
+            uint64_t x = window64(a.packed, a.start[g] + j) ^ window64(a.packed, a.start[i] + j);
+            if (L - j < 32) x &= (uint64_t(1) << (2 * (L - j))) - 1;
+            same = x == 0;
+        }
+        if (!same) g = i;
+    }
+    grp[i] = uint32_t(g);
+    lkey[i] = uint32_t(g - a.boff[b]);
+    atomicAnd(&fl_and[g], uint32_t(a.flags[i]));
+}
+
+// After the stable sort by lkey: each group's members are a run of sorted
+// positions, in record order.  gspan[g] = (first, end) of group g's run.
+__global__ void k_group_span(uint64_t n, uint32_t B, const uint64_t* __restrict__ boff,
+                             const uint32_t* __restrict__ sk, uint32_t* __restrict__ gbeg,
+                             uint32_t* __restrict__ gend) {
+    const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t b = segment_of(boff, B, p);
+    const uint64_t g = boff[b] + sk[p];
+    if (p == boff[b] || sk[p - 1] != sk[p]) gbeg[g] = uint32_t(p);
+    if (p + 1 == boff[b + 1] || sk[p + 1] != sk[p]) gend[g] = uint32_t(p + 1);
+}
+
+// Record bytes of every group leader (0 for other records).  Colours of a
+// group arrive in record order, which is colour order inside a bucket;
+// *bad is set when they do not.
+__global__ void k_dedup_size(RecArgs a, const uint32_t* __restrict__ grp, const uint32_t* __restrict__ gbeg,
+                             const uint32_t* __restrict__ gend, const uint32_t* __restrict__ scol,
+                             const uint32_t* __restrict__ fl_and, uint64_t* __restrict__ size,
+                             uint32_t* __restrict__ bad) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    if (grp[i] != i) {
+        size[i] = 0;
+        return;
+    }
+    uint64_t nc = 0, bytes = 0;
+    uint32_t prev = 0;
+    for (uint32_t p = gbeg[i]; p < gend[i]; ++p) {
+        const uint32_t c = scol[p];
+        if (nc && c < prev) *bad = 1;
+        if (nc == 0 || c != prev) {
+            bytes += varint_len(nc ? c - prev : c);
+            ++nc;
+            prev = c;
+        }
+    }
+    const uint64_t lf = (uint64_t(a.len[i]) << 4) | (fl_and[i] & 15u);
+    size[i] = varint_len(nc) + bytes + varint_len(lf) + (uint64_t(a.len[i]) + 3) / 4;
+}
+
+__global__ void k_dedup_write(RecArgs a, const uint32_t* __restrict__ grp, const uint32_t* __restrict__ gbeg,
+                              const uint32_t* __restrict__ gend, const uint32_t* __restrict__ scol,
+                              const uint32_t* __restrict__ fl_and, const uint64_t* __restrict__ off,
+                              uint8_t* __restrict__ out) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n || grp[i] != i) return;
+    const uint32_t g0 = gbeg[i], g1 = gend[i];
+    uint64_t nc = 0;
+    for (uint32_t p = g0; p < g1; ++p)
+        nc += (p == g0 || scol[p] != scol[p - 1]);
+    uint8_t* p8 = put_varint(out + off[i], nc);
+    for (uint32_t p = g0; p < g1; ++p)
+        if (p == g0) p8 = put_varint(p8, scol[p]);
+        else if (scol[p] != scol[p - 1]) p8 = put_varint(p8, scol[p] - scol[p - 1]);
+    const uint32_t L = a.len[i];
+    p8 = put_varint(p8, (uint64_t(L) << 4) | (fl_and[i] & 15u));
+    const uint64_t s = a.start[i];
     const uint32_t nb = (L + 3) / 4;
     for (uint32_t j = 0; j < nb; ++j) {
-        uint32_t b = uint32_t(window64(packed, s + 4ull * j)) & 0xffu;
+        uint32_t v = uint32_t(window64(a.packed, s + 4ull * j)) & 0xffu;
         const uint32_t rem = L - 4 * j;
-        if (rem < 4) b &= (1u << (2 * rem)) - 1;  // unused base slots stay zero
-        p[j] = uint8_t(b);
+        if (rem < 4) v &= (1u << (2 * rem)) - 1;  // unused base slots stay zero
+        p8[j] = uint8_t(v);
     }
+}
+
+__global__ void k_gather_u64(const uint64_t* __restrict__ idx, uint64_t n, const uint64_t* __restrict__ src,
+                             uint64_t* __restrict__ dst) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[idx[i]];
+}
+
+// ---- reader: walk each bucket's records -----------------------------------
+
+enum : uint32_t { WALK_OK = 0, WALK_RANK = 1, WALK_COLOR = 2 };
+
+struct WalkCounts {
+    uint64_t read;   // records parsed
+    uint64_t kept;   // records with len >= k
+    uint64_t pairs;  // colours of kept records
+    uint32_t err, err_color;
+};
+
+// Calls rec(colour_pos, n_colors, len, flags, bases_pos) for every record of
+// d[q, end) up to the first malformed one (read_super_kmer,
+// super_kmer.hpp:121-143); stops early when rec returns false.
+template <class F>
+__device__ __forceinline__ void walk_records(const uint8_t* __restrict__ d, uint64_t q, uint64_t end, F&& rec) {
+    while (q < end) {
+        const uint64_t nc = get_varint(d, end, q);
+        const uint64_t cpos = q;
+        uint64_t got = 0;
+        for (; got < nc && q < end; ++got) (void)get_varint(d, end, q);
+        if (q >= end || got != nc) return;
+        const uint64_t lf = get_varint(d, end, q);
+        const uint64_t len = lf >> 4;
+        const uint64_t nbytes = (len + 3) / 4;
+        if (q + nbytes > end) return;
+        if (!rec(cpos, nc, len, uint8_t(lf & 15u), q)) return;
+        q += nbytes;
+    }
+}
+
+__global__ __launch_bounds__(WALK_TPB) void k_walk_count(const uint8_t* __restrict__ d,
+                                                         const uint64_t* __restrict__ dbo, uint32_t B, uint32_t k,
+                                                         uint32_t num_colors, WalkCounts* __restrict__ wc) {
+    const uint32_t b = blockIdx.x * WALK_TPB + threadIdx.x;
+    if (b >= B) return;
+    const uint64_t end = dbo[b + 1];
+    WalkCounts c{0, 0, 0, WALK_OK, 0};
+    uint64_t rank = 0;
+    walk_records(d, dbo[b], end, [&](uint64_t cpos, uint64_t nc, uint64_t len, uint8_t, uint64_t) {
+        ++c.read;
+        if (len < k) return true;
+        ++c.kept;
+        c.pairs += nc;
+        rank += len - k + 1;
+        if (rank >= (uint64_t(1) << 32)) {
+            c.err = WALK_RANK;
+            return false;
+        }
+        uint32_t col = 0;
+        for (uint64_t j = 0; j < nc; ++j) {
+            col += uint32_t(get_varint(d, end, cpos));
+            if (col >= num_colors) {
+                c.err = WALK_COLOR;
+                c.err_color = col;
+                return false;
+            }
+        }
+        return true;
+    });
+    wc[b] = c;
+}
+
+struct WalkOut {
+    uint64_t* rec_byte;   // bases of record r: byte offset in the decoded stream
+    uint32_t* rec_len;
+    uint8_t* rec_flags;
+    uint32_t* rec_rank0;
+    uint32_t* rec_bucket;
+    uint32_t* pair_color;
+    uint32_t* pair_rec;
+};
+
+__global__ __launch_bounds__(WALK_TPB) void k_walk_emit(const uint8_t* __restrict__ d, const uint64_t* __restrict__ dbo,
+                                                        uint32_t B, uint32_t k, const uint64_t* __restrict__ rec_base,
+                                                        const uint64_t* __restrict__ pair_base, WalkOut o) {
+    const uint32_t b = blockIdx.x * WALK_TPB + threadIdx.x;
+    if (b >= B) return;
+    const uint64_t end = dbo[b + 1];
+    uint64_t r = rec_base[b], p = pair_base[b];
+    const uint64_t r_end = rec_base[b + 1], p_end = pair_base[b + 1];
+    uint64_t rank = 0;
+    walk_records(d, dbo[b], end, [&](uint64_t cpos, uint64_t nc, uint64_t len, uint8_t fl, uint64_t bpos) {
+        if (len < k) return true;
+        if (r >= r_end || p + nc > p_end) return false;  // the counting walk saw no more
+        o.rec_byte[r] = bpos;
+        o.rec_len[r] = uint32_t(len);
+        o.rec_flags[r] = fl;
+        o.rec_rank0[r] = uint32_t(rank);
+        o.rec_bucket[r] = b;
+        rank += len - k + 1;
+        uint32_t col = 0;
+        for (uint64_t j = 0; j < nc; ++j) {
+            col += uint32_t(get_varint(d, end, cpos));
+            o.pair_color[p] = col;
+            o.pair_rec[p] = uint32_t(r);
+            ++p;
+        }
+        ++r;
+        return true;
+    });
+}
+
+__global__ void k_pair_gather(uint64_t n, const uint32_t* __restrict__ pair_rec, const WalkOut o,
+                              uint64_t* __restrict__ start, uint32_t* __restrict__ len, uint8_t* __restrict__ flags,
+                              uint32_t* __restrict__ rank0, uint32_t* __restrict__ bucket) {
+    const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t r = pair_rec[p];
+    start[p] = 4 * o.rec_byte[r];
+    len[p] = o.rec_len[r];
+    flags[p] = o.rec_flags[r];
+    rank0[p] = o.rec_rank0[r];
+    bucket[p] = o.rec_bucket[r];
+}
+
+uint32_t bits_for(uint64_t max_value) {  // bits to hold every value in [0, max_value]
+    uint32_t b = 1;
+    while (b < 64 && (max_value >> b)) ++b;
+    return b;
 }
 
 std::string bucket_path(const std::string& dir, uint32_t b) { return dir + "/bucket_" + std::to_string(b) + ".bin"; }
-
-void write_frames(const std::string& path, const uint8_t* data, size_t n) {
-    FILE* f = std::fopen(path.c_str(), "wb");
-    if (!f) throw Error("cannot open bucket file: " + path);
-    std::vector<uint8_t> comp;
-    bool ok = true;
-    for (size_t o = 0; o < n && ok; o += FRAME_BYTES) {
-        const size_t u = std::min(FRAME_BYTES, n - o);
-        lz4_compress_block(data + o, u, comp);
-        const uint32_t hu = uint32_t(u), hc = uint32_t(comp.size());
-        ok = std::fwrite(&hu, 4, 1, f) == 1 && std::fwrite(&hc, 4, 1, f) == 1 &&
-             std::fwrite(comp.data(), 1, comp.size(), f) == comp.size();
-    }
-    const uint32_t eof = 0;
-    ok = ok && std::fwrite(&eof, 4, 1, f) == 1;
-    ok = (std::fclose(f) == 0) && ok;
-    if (!ok) throw Error("short write to " + path);
-}
-
-std::vector<uint8_t> read_decoded(const std::string& path) {
-    FILE* f = std::fopen(path.c_str(), "rb");
-    if (!f) throw Error("cannot open " + path);
-    std::vector<uint8_t> dec, comp;
-    for (;;) {
-        uint32_t u = 0, c = 0;
-        if (std::fread(&u, 4, 1, f) != 1) {
-            if (std::feof(f)) break;  // a missing EOF marker is tolerated (bucket_io.hpp:675-679)
-            std::fclose(f);
-            throw Error("short read of frame header on " + path);
-        }
-        if (u == 0) break;
-        if (std::fread(&c, 4, 1, f) != 1) {
-            std::fclose(f);
-            throw Error("short read of compressed_size on " + path);
-        }
-        comp.resize(c);
-        if (std::fread(comp.data(), 1, c, f) != c) {
-            std::fclose(f);
-            throw Error("short read of compressed payload on " + path);
-        }
-        const size_t off = dec.size();
-        dec.resize(off + u);
-        size_t got = 0;
-        if (!lz4_decompress_block(comp.data(), c, dec.data() + off, u, got) || got != u) {
-            std::fclose(f);
-            throw Error("LZ4 decode failed on " + path);
-        }
-    }
-    std::fclose(f);
-    return dec;
-}
-
-uint64_t get_varint(const uint8_t* buf, size_t n, size_t& pos) {  // super_kmer.hpp:71-82
-    uint64_t v = 0;
-    unsigned sh = 0;
-    while (pos < n) {
-        const uint8_t b = buf[pos++];
-        v |= uint64_t(b & 0x7f) << sh;
-        if (!(b & 0x80)) return v;
-        sh += 7;
-        if (sh >= 64) break;
-    }
-    return v;
-}
-
-// One bucket's parsed records, ready to concatenate.
-struct ParsedBucket {
-    std::vector<uint8_t> bases;  // packed payloads of records with len >= k
-    std::vector<uint64_t> rec_byte;  // payload byte offset inside `bases`
-    std::vector<uint32_t> rec_len;
-    std::vector<uint8_t> rec_flags;
-    std::vector<uint32_t> rec_rank0;
-    std::vector<uint32_t> pair_rec, pair_color;  // colour-major
-    uint64_t records_read = 0;
-};
-
-void parse_bucket(const std::string& path, uint32_t k, ParsedBucket& P) {
-    const std::vector<uint8_t> d = read_decoded(path);
-    std::vector<std::pair<uint32_t, uint32_t>> pairs;  // (colour, record)
-    std::vector<uint32_t> colors;
-    size_t q = 0;
-    uint64_t rank = 0;
-    // read_super_kmer (super_kmer.hpp:121-143): stop at the first malformed record
-    while (q < d.size()) {
-        const uint64_t nc = get_varint(d.data(), d.size(), q);
-        colors.clear();
-        uint32_t prev = 0;
-        for (uint64_t i = 0; i < nc && q < d.size(); ++i) {
-            prev += uint32_t(get_varint(d.data(), d.size(), q));
-            colors.push_back(prev);
-        }
-        if (q >= d.size() || colors.size() != nc) break;
-        const uint64_t lf = get_varint(d.data(), d.size(), q);
-        const uint64_t len = lf >> 4;
-        const size_t nbytes = size_t((len + 3) / 4);
-        if (q + nbytes > d.size()) break;
-        ++P.records_read;
-        if (len >= k) {
-            const uint32_t r = uint32_t(P.rec_len.size());
-            P.rec_byte.push_back(P.bases.size());
-            P.bases.insert(P.bases.end(), d.begin() + q, d.begin() + q + nbytes);
-            P.rec_len.push_back(uint32_t(len));
-            P.rec_flags.push_back(uint8_t(lf & 15));
-            P.rec_rank0.push_back(uint32_t(rank));
-            rank += len - k + 1;
-            if (rank >= (1ull << 32)) throw Error("bucket holds more than 2^32 k-mer occurrences: " + path);
-            for (uint32_t c : colors) pairs.push_back({c, r});
-        }
-        q += nbytes;
-    }
-    std::stable_sort(pairs.begin(), pairs.end(),
-                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
-                         return a.first < b.first;
-                     });
-    P.pair_rec.resize(pairs.size());
-    P.pair_color.resize(pairs.size());
-    for (size_t i = 0; i < pairs.size(); ++i) {
-        P.pair_color[i] = pairs[i].first;
-        P.pair_rec[i] = pairs[i].second;
-    }
-}
 
 template <class F>
 void parallel_for(uint32_t n, uint32_t threads, F&& f) {
@@ -247,6 +415,18 @@ void parallel_for(uint32_t n, uint32_t threads, F&& f) {
     if (failed) throw Error(err);
 }
 
+// Sorts (keys, vals) stably by the low `bits` key bits inside each segment.
+void sort_in_segments(uint32_t* keys, uint32_t* vals, uint64_t n, const std::vector<uint64_t>& seg_off,
+                      uint32_t bits, RadixScratch& S, hipStream_t st, const char* tag) {
+    if (n == 0) return;
+    SortPlan plan;
+    plan.build(seg_off);
+    DevBuf kt, vt;
+    const int digit = bits <= 8 ? int(bits) : int((bits + 1) / 2) <= 11 ? int((bits + 1) / 2) : 11;
+    segmented_radix_sort<uint32_t, uint32_t>(keys, vals, kt.ensure<uint32_t>(n), vt.ensure<uint32_t>(n), plan, 0,
+                                             int(bits), digit, S, st, tag);
+}
+
 }  // namespace
 
 void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64_t* unc_bytes) {
@@ -254,89 +434,266 @@ void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64
     if (!J.phase1_done || !J.bytes.p) throw Error("bucket files need phase 1 of a sequence job");
     hipStream_t st = J.st;
     Phase1Bufs& Q = J.p1;
+    const uint32_t B = J.B;
     const uint64_t n = J.n_supers;
-    DevBuf d_size, d_off, d_total, d_out, scan_tmp;
+    if (n >= (uint64_t(1) << 32)) throw Error("bucket files: more than 2^32 super-k-mer records in one job");
+    const std::vector<uint64_t>& bo = J.h_bucket_off;
+    uint64_t max_bucket = 1;
+    for (uint32_t b = 0; b < B; ++b) max_bucket = std::max(max_bucket, bo[b + 1] - bo[b]);
+    DevBuf d_boff;
+    uint64_t* boff = d_boff.ensure<uint64_t>(B + 1);
+    CDBG_HIP(hipMemcpyAsync(boff, bo.data(), (B + 1) * 8, hipMemcpyHostToDevice, st));
+    RecArgs a{n, B, boff, Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(),
+              Q.b_color.as<uint32_t>(), Q.packed.as<uint32_t>()};
+    // 1. groups of identical records
+    uint64_t slots = 1024;
+    while (slots < 2 * n) slots <<= 1;
+    DevBuf d_tkey, d_tfirst, d_rslot, d_grp, d_lkey, d_fl, d_col;
+    uint64_t* tkey = d_tkey.ensure<uint64_t>(slots);
+    uint32_t* tfirst = d_tfirst.ensure<uint32_t>(slots);
+    uint32_t* rslot = d_rslot.ensure<uint32_t>(n);
+    uint32_t* grp = d_grp.ensure<uint32_t>(n);
+    uint32_t* lkey = d_lkey.ensure<uint32_t>(n);
+    uint32_t* fl = d_fl.ensure<uint32_t>(n);
+    uint32_t* col = d_col.ensure<uint32_t>(n);
+    CDBG_HIP(hipMemsetAsync(tkey, 0, slots * 8, st));
+    CDBG_HIP(hipMemsetAsync(tfirst, 0xff, slots * 4, st));
+    if (n) {
+        { KSCOPE("k_dedup_insert", st); k_dedup_insert<<<blocks_for(n), TPB, 0, st>>>(a, tkey, tfirst, slots - 1, rslot, fl); }
+        { KSCOPE("k_dedup_group", st); k_dedup_group<<<blocks_for(n), TPB, 0, st>>>(a, tfirst, rslot, grp, lkey, fl); }
+        CDBG_HIP(hipGetLastError());
+        CDBG_HIP(hipMemcpyAsync(col, a.color, n * 4, hipMemcpyDeviceToDevice, st));
+    }
+    d_tkey.release();
+    d_tfirst.release();
+    d_rslot.release();
+    // 2. each group's colours, in record order, as one run
+    sort_in_segments(lkey, col, n, bo, bits_for(max_bucket - 1), J.rs, st, "dedup");
+    DevBuf d_gbeg, d_gend, d_size, d_off, d_bad, scan_tmp;
+    uint32_t* gbeg = d_gbeg.ensure<uint32_t>(n);
+    uint32_t* gend = d_gend.ensure<uint32_t>(n);
     uint64_t* size = d_size.ensure<uint64_t>(n);
     uint64_t* off = d_off.ensure<uint64_t>(n + 1);
-    if (n) k_record_size<<<blocks_for(n), TPB, 0, st>>>(n, Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(), size);
+    uint32_t* bad = d_bad.ensure<uint32_t>(1);
+    CDBG_HIP(hipMemsetAsync(bad, 0, 4, st));
+    if (n) {
+        { KSCOPE("k_group_span", st); k_group_span<<<blocks_for(n), TPB, 0, st>>>(n, B, boff, lkey, gbeg, gend); }
+        { KSCOPE("k_dedup_size", st); k_dedup_size<<<blocks_for(n), TPB, 0, st>>>(a, grp, gbeg, gend, col, fl, size, bad); }
+        CDBG_HIP(hipGetLastError());
+    }
     exclusive_scan<uint64_t, uint64_t>(size, off, n, off + n, scan_tmp, st);
-    uint64_t total = 0;
-    CDBG_HIP(hipMemcpyAsync(&total, off + n, 8, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipStreamSynchronize(st));
-    uint8_t* out = d_out.ensure<uint8_t>(total);
-    if (n) k_record_write<<<blocks_for(n), TPB, 0, st>>>(n, Q.b_start.as<uint64_t>(), Q.b_len.as<uint32_t>(), Q.b_flags.as<uint8_t>(), Q.b_color.as<uint32_t>(), Q.packed.as<uint32_t>(), off, out);
+    // byte range of every bucket in the merged stream
+    std::vector<uint64_t> bb(B + 1);
+    DevBuf d_bb;
+    uint64_t* dbb = d_bb.ensure<uint64_t>(B + 1);
+    k_gather_u64<<<blocks_for(B + 1), TPB, 0, st>>>(boff, B + 1, off, dbb);
     CDBG_HIP(hipGetLastError());
-    std::vector<uint8_t> h(total);
-    std::vector<uint64_t> h_off(n + 1);
-    if (total) CDBG_HIP(hipMemcpyAsync(h.data(), out, total, hipMemcpyDeviceToHost, st));
-    CDBG_HIP(hipMemcpyAsync(h_off.data(), off, (n + 1) * 8, hipMemcpyDeviceToHost, st));
+    uint32_t h_bad = 0;
+    CDBG_HIP(hipMemcpyAsync(bb.data(), dbb, (B + 1) * 8, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipMemcpyAsync(&h_bad, bad, 4, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    if (h_bad) throw Error("bucket files: the records of a bucket are not in colour order");
+    const uint64_t total = bb[B];
+    DevBuf d_out;
+    uint8_t* out = d_out.ensure<uint8_t>(total + 16);
+    if (n) {
+        KSCOPE("k_dedup_write", st);
+        k_dedup_write<<<blocks_for(n), TPB, 0, st>>>(a, grp, gbeg, gend, col, fl, off, out);
+    }
+    CDBG_HIP(hipGetLastError());
+    for (DevBuf* d : {&d_grp, &d_lkey, &d_fl, &d_col, &d_gbeg, &d_gend, &d_size, &d_off}) d->release();
+    // 3. frames of at most LZ_FRAME bytes inside each bucket, compressed
+    std::vector<uint64_t> foff{0}, slot{0};
+    std::vector<uint32_t> bframe(B + 1, 0);
+    for (uint32_t b = 0; b < B; ++b) {
+        bframe[b] = uint32_t(foff.size() - 1);
+        for (uint64_t o = bb[b]; o < bb[b + 1]; o += LZ_FRAME) {
+            const uint64_t e = std::min(o + LZ_FRAME, bb[b + 1]);
+            foff.push_back(e);
+            slot.push_back(slot.back() + lz4_slot(e - o));
+        }
+    }
+    const uint32_t nf = uint32_t(foff.size() - 1);
+    bframe[B] = nf;
+    DevBuf d_ftab, d_slots, d_cs;
+    uint64_t* ftab = d_ftab.ensure<uint64_t>(3 * (uint64_t(nf) + 1));
+    uint64_t* d_foff = ftab;
+    uint64_t* d_slot = ftab + nf + 1;
+    uint64_t* d_ooff = ftab + 2 * (uint64_t(nf) + 1);
+    uint8_t* slots_buf = d_slots.ensure<uint8_t>(slot[nf] + 16);
+    uint32_t* cs = d_cs.ensure<uint32_t>(nf + 1);
+    CDBG_HIP(hipMemcpyAsync(d_foff, foff.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipMemcpyAsync(d_slot, slot.data(), (nf + 1) * 8, hipMemcpyHostToDevice, st));
+    launch_lz4_compress(out, d_foff, nf, slots_buf, d_slot, cs, st);
+    std::vector<uint32_t> h_cs(nf);
+    if (nf) CDBG_HIP(hipMemcpyAsync(h_cs.data(), cs, nf * 4ull, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    // 4. file images: frames with headers, a 4-byte end marker per bucket
+    std::vector<uint64_t> ooff(nf + 1), fbase(B + 1);
+    uint64_t pos = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+        fbase[b] = pos;
+        for (uint32_t f = bframe[b]; f < bframe[b + 1]; ++f) {
+            if (h_cs[f] > slot[f + 1] - slot[f]) throw Error("lz4: a compressed frame overran its slot");
+            ooff[f] = pos;
+            pos += 8 + h_cs[f];
+        }
+        pos += 4;
+    }
+    fbase[B] = pos;
+    DevBuf d_img;
+    uint8_t* img = d_img.ensure<uint8_t>(pos + 16);
+    CDBG_HIP(hipMemcpyAsync(d_ooff, ooff.data(), nf * 8ull, hipMemcpyHostToDevice, st));
+    launch_lz4_pack(slots_buf, d_slot, cs, d_foff, nf, img, d_ooff, st);
+    PinnedBuf h_img;
+    h_img.ensure_keep(pos + 16, 0);
+    if (pos) CDBG_HIP(hipMemcpyAsync(h_img.p, img, pos, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
     std::filesystem::create_directories(dir);
-    const auto& bo = J.h_bucket_off;
-    parallel_for(J.B, threads, [&](uint32_t b) {
-        const uint64_t a = h_off[bo[b]], e = h_off[bo[b + 1]];
-        write_frames(bucket_path(dir, b), h.data() + a, e - a);
-        if (unc_bytes) unc_bytes[b] = e - a;
+    parallel_for(B, threads, [&](uint32_t b) {
+        std::memset(h_img.p + fbase[b + 1] - 4, 0, 4);  // [u32 0]: end of stream
+        const std::string path = bucket_path(dir, b);
+        FILE* f = std::fopen(path.c_str(), "wb");
+        if (!f) throw Error("cannot open bucket file: " + path);
+        const size_t m = fbase[b + 1] - fbase[b];
+        const bool ok = std::fwrite(h_img.p + fbase[b], 1, m, f) == m;
+        if ((std::fclose(f) != 0) || !ok) throw Error("short write to " + path);
+        if (unc_bytes) unc_bytes[b] = bb[b + 1] - bb[b];
     });
 }
 
 void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
     AllocStream alloc_on_job_stream(J.st);
-    const uint32_t B = J.B, k = J.k;
-    std::vector<ParsedBucket> pb(B);
-    parallel_for(B, threads, [&](uint32_t b) { parse_bucket(bucket_path(dir, b), k, pb[b]); });
-    // concatenate in bucket order
-    uint64_t n_bytes = 0, n_rec = 0, n_pair = 0;
-    std::vector<uint64_t> byte_base(B), rec_base(B);
-    for (uint32_t b = 0; b < B; ++b) {
-        byte_base[b] = n_bytes;
-        rec_base[b] = n_rec;
-        n_bytes += pb[b].bases.size();
-        n_rec += pb[b].rec_len.size();
-        n_pair += pb[b].pair_rec.size();
-    }
-    if (n_pair >= (1ull << 32)) throw Error("more than 2^32 (record, colour) pairs in one job");
-    std::vector<uint8_t> bases(n_bytes + 64, 0);
-    std::vector<uint64_t> start(n_pair);
-    std::vector<uint32_t> len(n_pair), color(n_pair), bucket(n_pair), rank0(n_pair);
-    std::vector<uint8_t> flags(n_pair);
-    J.h_bucket_off.assign(B + 1, 0);
-    uint64_t pi = 0;
-    for (uint32_t b = 0; b < B; ++b) {
-        ParsedBucket& P = pb[b];
-        if (!P.bases.empty()) std::memcpy(bases.data() + byte_base[b], P.bases.data(), P.bases.size());
-        J.h_bucket_off[b] = pi;
-        for (size_t i = 0; i < P.pair_rec.size(); ++i, ++pi) {
-            const uint32_t r = P.pair_rec[i];
-            start[pi] = 4 * (byte_base[b] + P.rec_byte[r]);
-            len[pi] = P.rec_len[r];
-            flags[pi] = P.rec_flags[r];
-            rank0[pi] = P.rec_rank0[r];
-            color[pi] = P.pair_color[i];
-            bucket[pi] = b;
-            if (color[pi] >= J.num_colors) throw Error("bucket " + std::to_string(b) + " names colour " +
-                                                       std::to_string(color[pi]) + " >= num_colors");
-        }
-        J.stats.num_supers += P.records_read;
-        ParsedBucket().bases.swap(P.bases);
-    }
-    J.h_bucket_off[B] = pi;
     hipStream_t st = J.st;
-    Phase1Bufs& Q = J.p1;
-    auto up = [&](DevBuf& d, const void* src, size_t bytes) {
-        d.ensure<uint8_t>(bytes);
-        if (bytes) CDBG_HIP(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, st));
+    const uint32_t B = J.B, k = J.k;
+    // 1. every file's bytes into one pinned image
+    std::vector<uint64_t> fbase(B + 1, 0);
+    for (uint32_t b = 0; b < B; ++b) {
+        std::error_code ec;
+        const uint64_t sz = std::filesystem::file_size(bucket_path(dir, b), ec);
+        if (ec) throw Error("cannot open " + bucket_path(dir, b));
+        fbase[b + 1] = fbase[b] + sz;
+    }
+    PinnedBuf img;
+    img.ensure_keep(fbase[B] + 16, 0);
+    // frames of each bucket: (block position in the image, block size, decoded size)
+    struct Frame {
+        uint64_t pos;
+        uint32_t c, u;
     };
-    up(Q.packed, bases.data(), bases.size());
-    up(Q.b_start, start.data(), n_pair * 8);
-    up(Q.b_len, len.data(), n_pair * 4);
-    up(Q.b_flags, flags.data(), n_pair);
-    up(Q.b_color, color.data(), n_pair * 4);
-    up(Q.key, bucket.data(), n_pair * 4);
-    up(Q.rank0, rank0.data(), n_pair * 4);
-    up(Q.bucket_off, J.h_bucket_off.data(), (B + 1) * 8);
-    CDBG_HIP(hipStreamSynchronize(st));  // host vectors are pageable and local
-    J.n_supers = n_pair;
+    std::vector<std::vector<Frame>> frames(B);
+    parallel_for(B, threads, [&](uint32_t b) {
+        const std::string path = bucket_path(dir, b);
+        FILE* f = std::fopen(path.c_str(), "rb");
+        if (!f) throw Error("cannot open " + path);
+        const uint64_t sz = fbase[b + 1] - fbase[b];
+        const bool ok = std::fread(img.p + fbase[b], 1, sz, f) == sz;
+        std::fclose(f);
+        if (!ok) throw Error("short read of " + path);
+        const uint8_t* p = img.p + fbase[b];
+        uint64_t q = 0;
+        for (;;) {
+            uint32_t u = 0, c = 0;
+            if (q + 4 > sz) break;  // a missing end marker is tolerated (bucket_io.hpp:675-679)
+            std::memcpy(&u, p + q, 4);
+            if (u == 0) break;
+            if (q + 8 > sz) throw Error("short read of compressed_size on " + path);
+            std::memcpy(&c, p + q + 4, 4);
+            if (q + 8 + c > sz) throw Error("short read of compressed payload on " + path);
+            frames[b].push_back({fbase[b] + q + 8, c, u});
+            q += 8 + uint64_t(c);
+        }
+    });
+    // 2. decode every frame on the device into one stream
+    std::vector<uint64_t> soff, doff{0}, dbo(B + 1, 0);
+    std::vector<uint32_t> csz, fbucket;
+    for (uint32_t b = 0; b < B; ++b) {
+        for (const Frame& fr : frames[b]) {
+            soff.push_back(fr.pos);
+            csz.push_back(fr.c);
+            doff.push_back(doff.back() + fr.u);
+            fbucket.push_back(b);
+        }
+        dbo[b + 1] = doff.back();
+        std::vector<Frame>().swap(frames[b]);
+    }
+    const uint32_t nf = uint32_t(csz.size());
+    const uint64_t D = doff.back();
+    Phase1Bufs& Q = J.p1;
+    DevBuf d_img, d_ftab;
+    uint8_t* dimg = d_img.ensure<uint8_t>(fbase[B] + 16);
+    if (fbase[B]) CDBG_HIP(hipMemcpyAsync(dimg, img.p, fbase[B], hipMemcpyHostToDevice, st));
+    uint64_t* ftab = d_ftab.ensure<uint64_t>(2 * uint64_t(nf) + 1 + (B + 1) + (nf + 7) / 8 + nf / 2 + 2);
+    uint64_t* d_soff = ftab;
+    uint64_t* d_doff = d_soff + nf;
+    uint64_t* d_dbo = d_doff + nf + 1;
+    uint32_t* d_cs = (uint32_t*)(d_dbo + B + 1);
+    uint8_t* d_bad = (uint8_t*)(d_cs + nf);
+    if (nf) {
+        CDBG_HIP(hipMemcpyAsync(d_soff, soff.data(), nf * 8ull, hipMemcpyHostToDevice, st));
+        CDBG_HIP(hipMemcpyAsync(d_cs, csz.data(), nf * 4ull, hipMemcpyHostToDevice, st));
+    }
+    CDBG_HIP(hipMemcpyAsync(d_doff, doff.data(), (nf + 1) * 8ull, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipMemcpyAsync(d_dbo, dbo.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
+    uint8_t* dec = Q.packed.ensure<uint8_t>(D + 64);
+    CDBG_HIP(hipMemsetAsync(dec + D, 0, 64, st));  // k-mer windows read past the last base
+    launch_lz4_decompress(dimg, d_soff, d_cs, nf, dec, d_doff, d_bad, st);
+    std::vector<uint8_t> bad(nf);
+    if (nf) CDBG_HIP(hipMemcpyAsync(bad.data(), d_bad, nf, hipMemcpyDeviceToHost, st));
+    // 3. count each bucket's records and pairs
+    DevBuf d_wc;
+    WalkCounts* wc = d_wc.ensure<WalkCounts>(B);
+    { KSCOPE("k_walk_count", st); k_walk_count<<<blocks_for(B, WALK_TPB), WALK_TPB, 0, st>>>(dec, d_dbo, B, k, J.num_colors, wc); }
+    CDBG_HIP(hipGetLastError());
+    std::vector<WalkCounts> h_wc(B);
+    CDBG_HIP(hipMemcpyAsync(h_wc.data(), wc, B * sizeof(WalkCounts), hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    for (uint32_t f = 0; f < nf; ++f)
+        if (bad[f]) throw Error("LZ4 decode failed on " + bucket_path(dir, fbucket[f]));
+    std::vector<uint64_t> rec_base(B + 1, 0), pair_base(B + 1, 0);
+    for (uint32_t b = 0; b < B; ++b) {
+        const WalkCounts& c = h_wc[b];
+        if (c.err == WALK_RANK)
+            throw Error("bucket holds more than 2^32 k-mer occurrences: " + bucket_path(dir, b));
+        if (c.err == WALK_COLOR)
+            throw Error("bucket " + std::to_string(b) + " names colour " + std::to_string(c.err_color) +
+                        " >= num_colors");
+        rec_base[b + 1] = rec_base[b] + c.kept;
+        pair_base[b + 1] = pair_base[b] + c.pairs;
+        J.stats.num_supers += c.read;
+    }
+    const uint64_t nr = rec_base[B], np = pair_base[B];
+    if (np >= (1ull << 32)) throw Error("more than 2^32 (record, colour) pairs in one job");
+    // 4. records and pairs
+    DevBuf d_base, d_rbyte, d_rlen, d_rfl, d_rrank, d_rbkt, d_pcol, d_prec;
+    uint64_t* base = d_base.ensure<uint64_t>(2 * (uint64_t(B) + 1));
+    CDBG_HIP(hipMemcpyAsync(base, rec_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipMemcpyAsync(base + B + 1, pair_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
+    WalkOut o{d_rbyte.ensure<uint64_t>(nr), d_rlen.ensure<uint32_t>(nr), d_rfl.ensure<uint8_t>(nr),
+              d_rrank.ensure<uint32_t>(nr), d_rbkt.ensure<uint32_t>(nr), d_pcol.ensure<uint32_t>(np),
+              d_prec.ensure<uint32_t>(np)};
+    { KSCOPE("k_walk_emit", st); k_walk_emit<<<blocks_for(B, WALK_TPB), WALK_TPB, 0, st>>>(dec, d_dbo, B, k, base, base + B + 1, o); }
+    CDBG_HIP(hipGetLastError());
+    d_img.release();
+    // 5. colour-major inside each bucket, record order kept among equal colours
+    sort_in_segments(o.pair_color, o.pair_rec, np, pair_base, bits_for(std::max(1u, J.num_colors) - 1), J.rs, st,
+                     "pairs");
+    uint64_t* start = Q.b_start.ensure<uint64_t>(np);
+    uint32_t* len = Q.b_len.ensure<uint32_t>(np);
+    uint8_t* flags = Q.b_flags.ensure<uint8_t>(np);
+    uint32_t* rank0 = Q.rank0.ensure<uint32_t>(np);
+    uint32_t* bucket = Q.key.ensure<uint32_t>(np);
+    uint32_t* color = Q.b_color.ensure<uint32_t>(np);
+    if (np) {
+        { KSCOPE("k_pair_gather", st); k_pair_gather<<<blocks_for(np), TPB, 0, st>>>(np, o.pair_rec, o, start, len, flags, rank0, bucket); }
+        CDBG_HIP(hipGetLastError());
+        CDBG_HIP(hipMemcpyAsync(color, o.pair_color, np * 4, hipMemcpyDeviceToDevice, st));
+    }
+    uint64_t* bucket_off = Q.bucket_off.ensure<uint64_t>(B + 1);
+    CDBG_HIP(hipMemcpyAsync(bucket_off, pair_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipStreamSynchronize(st));  // host tables are pageable and local
+    J.h_bucket_off = pair_base;
+    J.n_supers = np;
     J.phase1_done = true;
     J.from_files = true;
 }

@@ -104,16 +104,6 @@ __global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ 
 
 using namespace cdbg_amd;
 
-#define CDBG_TRY(...)                           \
-    try {                                       \
-        __VA_ARGS__;                            \
-        return 0;                               \
-    } catch (std::exception & e) {              \
-        return set_error(e.what());             \
-    } catch (...) {                             \
-        return set_error("unknown C++ exception"); \
-    }
-
 extern "C" {
 
 CDBG_API const char* cdbg_gpu_last_error(void) { return last_error(); }

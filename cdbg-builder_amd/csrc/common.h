@@ -26,3 +26,15 @@ struct Error : std::runtime_error {
                                     __FILE__ + ":" + std::to_string(__LINE__) + ")");      \
         }                                                                                   \
     } while (0)
+
+// Body of a C-ABI entry point: 0 on success, else the exception's message
+// becomes the thread's last error and the generic failure code is returned.
+#define CDBG_TRY(...)                              \
+    try {                                          \
+        __VA_ARGS__;                               \
+        return 0;                                  \
+    } catch (std::exception & e) {                 \
+        return ::cdbg_amd::set_error(e.what());    \
+    } catch (...) {                                \
+        return ::cdbg_amd::set_error("unknown C++ exception"); \
+    }

@@ -225,16 +225,6 @@ struct cdbg_gpu_server {
     Server S;
 };
 
-#define CDBG_TRY(...)                              \
-    try {                                          \
-        __VA_ARGS__;                               \
-        return 0;                                  \
-    } catch (std::exception & e) {                 \
-        return set_error(e.what());                \
-    } catch (...) {                                \
-        return set_error("unknown C++ exception"); \
-    }
-
 extern "C" {
 
 CDBG_API uint64_t cdbg_gpu_hbm_estimate(uint64_t bases, uint32_t k, uint32_t m, uint64_t chunk_occurrences) {

@@ -196,7 +196,12 @@ typedef struct {
 
 /* Drop-in phase 1: reads the FASTA/FASTQ files (colour = index in `files`),
  * runs phase 1 on the device and writes <tmp_dir>/bucket_<b>.bin for every
- * bucket (bucket_io.hpp:19-40, records super_kmer.hpp:105-117).
+ * bucket (bucket_io.hpp:19-40, records super_kmer.hpp:105-117).  As the
+ * reference's compacting writer does (bucket_io.hpp:127-277), records with
+ * equal bases (and here equal ownership bits) in one bucket are merged into
+ * one record with the sorted unique union of their colours, in order of
+ * first occurrence; the stream goes out in LZ4 frames of at most 64 KiB,
+ * compressed on the device.
  * bucket_unc_bytes (num_buckets entries, may be NULL) receives each bucket's
  * uncompressed size, as bucket_writer::close records it (bucket_io.hpp:389-415).
  * Replaces ingest_bucketed (bucket_ingester.hpp:183-215). */
@@ -335,6 +340,19 @@ CDBG_API void cdbg_seqs_free(void* h);
  * the output length, or a negative status. */
 CDBG_API int64_t cdbg_lz4_compress(const uint8_t* src, uint64_t n, uint8_t* dst, uint64_t cap);
 CDBG_API int64_t cdbg_lz4_decompress(const uint8_t* src, uint64_t n, uint8_t* dst, uint64_t cap);
+
+/* The same block format on the device, one lane per frame: the codec of the
+ * bucket files' frames (LZ4_compress_default / LZ4_decompress_safe at
+ * bucket_io.hpp:255-265,695-696).  compress: frame f = src[frame_off[f],
+ * frame_off[f+1]) (at most 65536 bytes each) becomes dst[comp_off[f],
+ * comp_off[f+1]) (comp_off has n_frames + 1 entries; dst_cap bytes of room).
+ * decompress: block f = comp[comp_off[f], comp_off[f+1]) must decode to
+ * exactly dst_off[f+1] - dst_off[f] bytes at dst + dst_off[f]; a malformed
+ * block is an error naming the frame. */
+CDBG_API int cdbg_gpu_lz4_compress_frames(cdbg_gpu_ctx* ctx, const uint8_t* src, const uint64_t* frame_off,
+                                          uint32_t n_frames, uint8_t* dst, uint64_t dst_cap, uint64_t* comp_off);
+CDBG_API int cdbg_gpu_lz4_decompress_frames(cdbg_gpu_ctx* ctx, const uint8_t* comp, const uint64_t* comp_off,
+                                            uint32_t n_frames, uint8_t* dst, const uint64_t* dst_off);
 
 /* Synthetic pangenome generator (seeded std::mt19937_64, explicit modulo). */
 typedef struct {

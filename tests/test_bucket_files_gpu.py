@@ -3,7 +3,10 @@
 * cdbg_gpu_bucket_write: FASTA files -> bucket_<b>.bin.  Files are decoded
   with an independent reader (system liblz4 when present + a varint parser
   written from super_kmer.hpp:121-143) and must hold exactly the oracle's
-  super-k-mer records, bucket by bucket, in stream order.
+  super-k-mer records merged as the compacting writer merges them
+  (bucket_io.hpp:145-160,221-240): one record per distinct (bases,
+  ownership bits) of a bucket, in order of first occurrence, with the sorted
+  unique colours of its occurrences and the AND of their flags.
 * cdbg_gpu_bucket_process: bucket files -> frag_unitigs.bin, .color_sets,
   .color_sets.tmp_offsets, byte-identical to the oracle's phase 2 over the
   same files.  Also on bucket files whose records carry several colours and
@@ -44,6 +47,26 @@ def _fasta_job(tmp_path, colors=5, glen=20000, seed=6):
     return paths
 
 
+OWN = 0xC  # SK_FLAG_OWNS_FIRST | SK_FLAG_OWNS_LAST (super_kmer.hpp:96-97)
+
+
+def _merged_bucket(data, sup, lo, hi):
+    """The compacting writer's records for the oracle's super-k-mers
+    [lo, hi) of one bucket (stream order): occurrences with equal bases and
+    ownership bits merge; the record sits at its first occurrence, carries the
+    sorted unique colours and the AND of the flags (bucket_io.hpp:145-160,
+    221-240)."""
+    groups = {}
+    for i in range(lo, hi):
+        s, ln = int(sup["start"][i]), int(sup["len"][i])
+        fl = int(sup["flags"][i])
+        key = (bytes(data[s:s + ln]).upper(), fl & OWN)
+        g = groups.setdefault(key, [set(), 15])
+        g[0].add(int(sup["color"][i]))
+        g[1] &= fl
+    return [(tuple(sorted(c)), fl, key[0]) for key, (c, fl) in groups.items()]
+
+
 @pytest.mark.parametrize("k,m,B", [(31, 19, 64), (63, 27, 32), (21, 11, 7)])
 def test_bucket_write_records(gpu_ctx, tmp_path, k, m, B):
     files = _fasta_job(tmp_path)
@@ -53,15 +76,26 @@ def test_bucket_write_records(gpu_ctx, tmp_path, k, m, B):
     ob = O.Build(data, off, k, m, B)
     sup = ob.supers(1)
     boff = np.concatenate([[0], np.cumsum(np.bincount(sup["bucket"], minlength=B))])
+    merged = 0
     for b in range(B):
-        recs = _records(os.path.join(d, f"bucket_{b}.bin"))
-        want = []
-        for i in range(boff[b], boff[b + 1]):
-            s, ln = int(sup["start"][i]), int(sup["len"][i])
-            want.append(((int(sup["color"][i]),), int(sup["flags"][i]), bytes(data[s:s + ln]).upper()))
+        path = os.path.join(d, f"bucket_{b}.bin")
+        recs = _records(path)
+        want = _merged_bucket(data, sup, boff[b], boff[b + 1])
         assert recs == want, f"bucket {b}"
-        payload = len(_frames(os.path.join(d, f"bucket_{b}.bin")))
+        merged += sum(len(r[0]) > 1 for r in recs)
+        payload = len(_frames(path))
         assert unc[b] == payload
+        raw = open(path, "rb").read()
+        p = 0
+        while True:  # frames of at most 64 KiB, then the end marker
+            u, = struct.unpack_from("<I", raw, p)
+            if u == 0:
+                assert p + 4 == len(raw)
+                break
+            c, = struct.unpack_from("<I", raw, p + 4)
+            assert u <= 65536
+            p += 8 + c
+    assert merged > 0  # five related genomes share super-k-mers
 
 
 def _compare_phase2(gpu_ctx, tmp_path, d, B, k, num_colors):
@@ -126,7 +160,16 @@ def _enc_varint(v):
     return bytes(out)
 
 
-def _write_bucket(path, recs, frame=700):
+def _sys_block(blk):
+    import ctypes
+    cap = len(blk) + len(blk) // 255 + 64
+    buf = ctypes.create_string_buffer(cap)
+    n = SYS.LZ4_compress_default(blk, buf, len(blk), cap)
+    assert n > 0
+    return buf.raw[:n]
+
+
+def _write_bucket(path, recs, frame=700, block=None):
     stream = bytearray()
     for cols, fl, bases in recs:
         stream += _enc_varint(len(cols))
@@ -142,15 +185,18 @@ def _write_bucket(path, recs, frame=700):
     with open(path, "wb") as f:
         for o in range(0, len(stream), frame):
             blk = bytes(stream[o:o + frame])
-            lit = _literal_block(blk)
+            lit = (block or _literal_block)(blk)
             f.write(struct.pack("<II", len(blk), len(lit)) + lit)
         f.write(struct.pack("<I", 0))
 
 
-@pytest.mark.parametrize("k,m,B", [(31, 19, 16), (21, 11, 8)])
-def test_multicolour_records(gpu_ctx, tmp_path, k, m, B):
+@pytest.mark.parametrize("k,m,B,frames", [(31, 19, 16, "literal"), (21, 11, 8, "literal"), (31, 19, 4, "liblz4")])
+def test_multicolour_records(gpu_ctx, tmp_path, k, m, B, frames):
     """Records merged across colours (sorted unique colour lists), shuffled
-    within each bucket, split over literal-only LZ4 frames."""
+    within each bucket, split over literal-only LZ4 frames of 700 bytes, or
+    over frames of up to 200 KB compressed by the system liblz4."""
+    if frames == "liblz4" and SYS is None:
+        pytest.skip("system liblz4 not present")
     files = _fasta_job(tmp_path, colors=7, glen=8000, seed=12)
     data, off = G.load_fasta(files)
     sup = O.Build(data, off, k, m, B).supers(1)
@@ -165,7 +211,10 @@ def test_multicolour_records(gpu_ctx, tmp_path, k, m, B):
             groups.setdefault(key, set()).add(int(sup["color"][i]))
         recs = [(tuple(sorted(cols)), fl, bases) for (bases, fl), cols in groups.items()]
         rng.shuffle(recs)
-        _write_bucket(str(d / f"bucket_{b}.bin"), recs)
+        if frames == "liblz4":
+            _write_bucket(str(d / f"bucket_{b}.bin"), recs, frame=200000, block=_sys_block)
+        else:
+            _write_bucket(str(d / f"bucket_{b}.bin"), recs)
     _compare_phase2(gpu_ctx, tmp_path, str(d), B, k, len(files))
 
 
@@ -190,4 +239,6 @@ def test_failed_profiled_run_then_file_write(gpu_ctx, tmp_path):
         j2.write_bucket_files(d)
     ob = O.Build(data, off, 31, 19, 16)
     sup = ob.supers(1)
-    assert sum(len(_records(os.path.join(d, f"bucket_{b}.bin"))) for b in range(16)) == len(sup["start"])
+    boff = np.concatenate([[0], np.cumsum(np.bincount(sup["bucket"], minlength=16))])
+    for b in range(16):
+        assert _records(os.path.join(d, f"bucket_{b}.bin")) == _merged_bucket(data, sup, boff[b], boff[b + 1])
