@@ -451,6 +451,50 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def bucket_io_leg(G, ctx, job, s, k, B, colors, threads):
+    """Bucket files of the plain workload through the device writer
+    (identical records merged, LZ4 frames compressed on the device, files
+    written by host threads) and the device reader (files read by host
+    threads, frames decoded and records walked on the device), then phase 2
+    over what was read, whose totals must equal the resident run's.  Wall
+    clock; the writer's kernels from the job profiler."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="cdbg_buckets_", dir="/tmp")
+    try:
+        job.phase1()
+        kt0 = job.kernel_times()
+        t0 = time.perf_counter()
+        unc = job.write_bucket_files(d, threads)
+        t_write = time.perf_counter() - t0
+        kt1 = job.kernel_times()
+        wk = {}
+        for name, (cnt, ms) in kt1.items():
+            dm = ms - kt0.get(name, (0, 0.0))[1]
+            if dm > 0.05:
+                wk[name] = round(dm, 2)
+        file_bytes = sum(os.path.getsize(os.path.join(d, f)) for f in os.listdir(d))
+        t0 = time.perf_counter()
+        fj = ctx.job_from_bucket_files(d, B, k, colors, threads)
+        t_read = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        fj.phase2()
+        t_p2 = time.perf_counter() - t0
+        fs = fj.stats()
+        fj.close()
+        same = all(fs[x] == s[x] for x in ("num_kmers", "num_frags", "num_color_sets", "frag_bytes", "cs_bits"))
+        res = {"records_in": s["num_supers"], "records_out": fs["num_supers"],
+               "bytes_uncompressed": int(unc.sum()), "bytes_files": file_bytes,
+               "write_s": round(t_write, 3), "read_s": round(t_read, 3), "phase2_from_files_s": round(t_p2, 3),
+               "write_kernels_ms": dict(sorted(wk.items(), key=lambda x: -x[1])),
+               "phase2_totals_equal_resident": same, "host_threads": threads,
+               "workload": f"the plain workload's {B} buckets; files under /tmp"}
+        print(f"bench: bucket files {json.dumps(res)}", file=sys.stderr, flush=True)
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -477,6 +521,9 @@ def main():
     ap.add_argument("--large-colors", type=int, default=4000,
                     help="colours of the large resident build leg (0: skip it)")
     ap.add_argument("--large-only", action="store_true", help="run only the large resident build leg")
+    ap.add_argument("--no-bucket-files", action="store_true", help="skip the bucket-file write/read leg")
+    ap.add_argument("--bucket-files-only", action="store_true",
+                    help="time the plain workload, then only the bucket-file leg (no traffic runs)")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -537,7 +584,22 @@ def main():
     roof, kernels = roofline(kt, s, args.k, args.steps, chunks, ms_step)
     roof["copy_probe_gbps"] = round(copy_gbps, 1)
 
+    def leg(fn, *a):
+        """A secondary leg; its failure is reported in its place, not raised."""
+        try:
+            return fn(*a)
+        except Exception as e:  # noqa: BLE001
+            print(f"bench: {fn.__name__} failed: {e}", file=sys.stderr, flush=True)
+            return {"error": str(e)}
+
+    bucket_files = None
+    if not args.no_bucket_files:
+        bucket_files = leg(bucket_io_leg, G, ctx, job, s, args.k, args.buckets, args.colors, args.cpu_cores)
     job.close()
+    if args.bucket_files_only:
+        print(json.dumps({"bucket_files": bucket_files}))
+        ctx.close()
+        return
     if not args.no_traffic:
         try:
             tr = measure_traffic(args, roof["kernel"], roof["launches_per_step"])
@@ -550,14 +612,6 @@ def main():
             roof["traffic_write"] = tr["write_bytes"]
             roof["traffic_over_model"] = round(tr["traffic"] / roof["bytes_per_launch"], 3)
         print(f"bench: traffic of {roof['kernel']}: {tr}", file=sys.stderr, flush=True)
-    def leg(fn, *a):
-        """A secondary leg; its failure is reported in its place, not raised."""
-        try:
-            return fn(*a)
-        except Exception as e:  # noqa: BLE001
-            print(f"bench: {fn.__name__} failed: {e}", file=sys.stderr, flush=True)
-            return {"error": str(e)}
-
     files = None
     if args.file_colors:
         files = leg(file_path_leg, G, ctx, min(args.file_colors, args.colors), args.genome, args.k, args.m,
@@ -599,6 +653,7 @@ def main():
         "batched": batched,
         "served": served,
         "file_path": files,
+        "bucket_files": bucket_files,
         "large": large,
         "cpu_baseline": cpu,
     }

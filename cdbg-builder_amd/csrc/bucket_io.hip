@@ -61,7 +61,6 @@ namespace cdbg_amd {
 namespace {
 
 constexpr int TPB = 256;
-constexpr int WALK_TPB = 64;  // bucket walkers: one thread per bucket
 
 inline uint32_t blocks_for(uint64_t n, int tpb = TPB) {
     return uint32_t(std::max<uint64_t>(1, (n + tpb - 1) / tpb));
@@ -270,47 +269,154 @@ __global__ void k_gather_u64(const uint64_t* __restrict__ idx, uint64_t n, const
 }
 
 // ---- reader: walk each bucket's records -----------------------------------
+//
+// One wavefront per bucket.  The bucket's decoded bytes pass through an LDS
+// window; the walk reads 64 bytes at a time (one byte per lane) and finds
+// varint ends with a ballot of the bytes whose high bit is clear, so a
+// record header (count, colour deltas, length) costs a few ballots however
+// many colours it lists.  Every varint of more than ten bytes (only in a
+// malformed stream) is read byte by byte with the ten-byte rule of
+// varint_read.  Kept records (len >= k) are described in a slot range per
+// bucket sized by the smallest kept record; a second kernel compacts the
+// descriptors, a third decodes each record's colours into its pairs.
 
-enum : uint32_t { WALK_OK = 0, WALK_RANK = 1, WALK_COLOR = 2 };
+constexpr uint32_t WALK_WIN = 8192;  // LDS window of one walker, bytes
+
+enum : uint32_t { WALK_OK = 0, WALK_RANK = 1, WALK_SLOTS = 2, WALK_BIG = 3 };
 
 struct WalkCounts {
     uint64_t read;   // records parsed
     uint64_t kept;   // records with len >= k
     uint64_t pairs;  // colours of kept records
-    uint32_t err, err_color;
+    uint32_t err, pad;
 };
 
-// Calls rec(colour_pos, n_colors, len, flags, bases_pos) for every record of
-// d[q, end) up to the first malformed one (read_super_kmer,
-// super_kmer.hpp:121-143); stops early when rec returns false.
-template <class F>
-__device__ __forceinline__ void walk_records(const uint8_t* __restrict__ d, uint64_t q, uint64_t end, F&& rec) {
-    while (q < end) {
-        const uint64_t nc = get_varint(d, end, q);
-        const uint64_t cpos = q;
-        uint64_t got = 0;
-        for (; got < nc && q < end; ++got) (void)get_varint(d, end, q);
-        if (q >= end || got != nc) return;
-        const uint64_t lf = get_varint(d, end, q);
-        const uint64_t len = lf >> 4;
-        const uint64_t nbytes = (len + 3) / 4;
-        if (q + nbytes > end) return;
-        if (!rec(cpos, nc, len, uint8_t(lf & 15u), q)) return;
-        q += nbytes;
-    }
-}
+struct RecDesc {  // kept records in bucket order
+    uint64_t* cpos;  // colour deltas: byte offset in the decoded stream
+    uint64_t* bpos;  // packed bases: byte offset in the decoded stream
+    uint32_t* nc;
+    uint32_t* len;
+    uint8_t* flags;
+    uint32_t* rank0;  // stream rank of the record's first k-mer in its bucket
+    uint64_t* pair0;  // first pair
+};
 
-__global__ __launch_bounds__(WALK_TPB) void k_walk_count(const uint8_t* __restrict__ d,
-                                                         const uint64_t* __restrict__ dbo, uint32_t B, uint32_t k,
-                                                         uint32_t num_colors, WalkCounts* __restrict__ wc) {
-    const uint32_t b = blockIdx.x * WALK_TPB + threadIdx.x;
-    if (b >= B) return;
+// Kept-record descriptor as written by a walker: 8 dwords per slot, stored
+// by lanes 0-7 in one instruction.
+enum : uint32_t { DS_CPOS_LO, DS_CPOS_HI, DS_BDELTA, DS_NC, DS_LEN, DS_RANK0, DS_PAIR0, DS_FLAGS, DS_WORDS };
+
+__global__ __launch_bounds__(64) void k_walk(const uint8_t* __restrict__ d, const uint64_t* __restrict__ dbo,
+                                             uint32_t k, const uint64_t* __restrict__ region,
+                                             uint32_t* __restrict__ desc, WalkCounts* __restrict__ wc) {
+    __shared__ uint32_t win32[WALK_WIN / 4];
+    const uint8_t* win = (const uint8_t*)win32;
+    const uint32_t b = blockIdx.x, lane = threadIdx.x;
     const uint64_t end = dbo[b + 1];
+    uint64_t wbase = 0, wend = 0;  // the window holds bytes [wbase, wend)
+    // after need(q): bytes [q, min(q + 64, end)) are in the window
+    auto need = [&](uint64_t q) {
+        if (q >= wbase && (q + 64 <= wend || wend == end)) return;
+        __syncthreads();  // every lane is done with the old window
+        wbase = q & ~uint64_t(3);
+        wend = min(wbase + WALK_WIN, end);
+        const uint32_t nd = uint32_t((wend - wbase + 3) / 4);  // ends <= 3 bytes past `end`: inside the padding
+        const uint32_t* src = (const uint32_t*)(d + wbase);
+        for (uint32_t i0 = 0; i0 < nd; i0 += 16 * 64) {  // 16 loads in flight per lane
+            uint32_t t[16];
+#pragma unroll
+            for (uint32_t u = 0; u < 16; ++u) {
+                const uint32_t i = i0 + u * 64 + lane;
+                t[u] = i < nd ? src[i] : 0u;
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < 16; ++u) {
+                const uint32_t i = i0 + u * 64 + lane;
+                if (i < nd) win32[i] = t[u];
+            }
+        }
+        __syncthreads();
+    };
+    // byte q + lane (0x80, a continuation byte, past the bucket's end) and
+    // the mask of varint-terminating bytes among the 64
+    auto block = [&](uint64_t q, uint64_t& term) -> uint32_t {
+        need(q);
+        const uint32_t v = q + lane < end ? uint32_t(win[q + lane - wbase]) : 0x80u;
+        term = __ballot(!(v & 0x80u));
+        return v;
+    };
+    // varint_read at q < end (super_kmer.hpp:71-82): up to the terminator,
+    // at most ten bytes, at most to the end
+    auto varint = [&](uint64_t& q) -> uint64_t {
+        uint64_t term;
+        const uint32_t v = block(q, term);
+        const uint64_t avail = end - q;
+        uint32_t nb = term ? uint32_t(__builtin_ctzll(term)) + 1 : 64;
+        if (nb > 10) nb = 10;
+        if (nb > avail) nb = uint32_t(avail);
+        uint64_t x = 0;
+        for (uint32_t j = 0; j < nb; ++j) x |= uint64_t(__builtin_amdgcn_readlane(int(v), int(j)) & 0x7f) << (7 * j);
+        q += nb;
+        return x;
+    };
+    // skips n varints; false when the stream ends first
+    auto skip = [&](uint64_t& q, uint64_t n) -> bool {
+        while (n) {
+            if (q >= end) return false;
+            uint64_t term;
+            (void)block(q, term);
+            const uint64_t avail = end - q;
+            const uint64_t valid = avail >= 64 ? ~0ull : ((1ull << avail) - 1);
+            const uint64_t z = ~term & valid;
+            uint64_t run = z;  // bit a: bytes a..a+9 all continue a varint
+            for (int s = 1; s < 10; ++s) run &= z >> s;
+            const uint32_t cnt = uint32_t(__builtin_popcountll(term));
+            if (cnt == 0) {  // no end in sight: an overlong varint, or the stream's end
+                (void)varint(q);
+                --n;
+                continue;
+            }
+            const uint64_t take = n < cnt ? n : cnt;
+            uint64_t t = term;
+            for (uint64_t i = 1; i < take; ++i) t &= t - 1;
+            const uint32_t pos = uint32_t(__builtin_ctzll(t));  // the take-th end
+            const uint64_t below = pos >= 63 ? ~0ull : ((1ull << (pos + 1)) - 1);
+            if (run & below) {  // an overlong varint among them: one at a time
+                (void)varint(q);
+                --n;
+                continue;
+            }
+            q += pos + 1;
+            n -= take;
+        }
+        return true;
+    };
     WalkCounts c{0, 0, 0, WALK_OK, 0};
-    uint64_t rank = 0;
-    walk_records(d, dbo[b], end, [&](uint64_t cpos, uint64_t nc, uint64_t len, uint8_t, uint64_t) {
+    uint64_t slot = region[b], rank = 0;
+    const uint64_t slot_end = region[b + 1];
+    // one parsed record; false stops the walk
+    auto keep = [&](uint64_t cpos, uint64_t nc, uint64_t len, uint32_t fl, uint64_t bpos) -> bool {
         ++c.read;
         if (len < k) return true;
+        if (slot >= slot_end) {
+            c.err = WALK_SLOTS;
+            return false;
+        }
+        if (nc >= (uint64_t(1) << 32) || len >= (uint64_t(1) << 32) || bpos - cpos >= (uint64_t(1) << 32)) {
+            c.err = WALK_BIG;
+            return false;
+        }
+        if (lane < DS_WORDS) {
+            const uint32_t x = lane == DS_CPOS_LO ? uint32_t(cpos)
+                             : lane == DS_CPOS_HI ? uint32_t(cpos >> 32)
+                             : lane == DS_BDELTA  ? uint32_t(bpos - cpos)
+                             : lane == DS_NC      ? uint32_t(nc)
+                             : lane == DS_LEN     ? uint32_t(len)
+                             : lane == DS_RANK0   ? uint32_t(rank)
+                             : lane == DS_PAIR0   ? uint32_t(c.pairs)
+                                                  : fl;
+            desc[slot * DS_WORDS + lane] = x;
+        }
+        ++slot;
         ++c.kept;
         c.pairs += nc;
         rank += len - k + 1;
@@ -318,71 +424,144 @@ __global__ __launch_bounds__(WALK_TPB) void k_walk_count(const uint8_t* __restri
             c.err = WALK_RANK;
             return false;
         }
-        uint32_t col = 0;
-        for (uint64_t j = 0; j < nc; ++j) {
-            col += uint32_t(get_varint(d, end, cpos));
-            if (col >= num_colors) {
-                c.err = WALK_COLOR;
-                c.err_color = col;
-                return false;
+        return true;
+    };
+    // read_super_kmer (super_kmer.hpp:121-143) at q, one block at a time;
+    // 0: parsed, 1: malformed (the walk stops there)
+    auto general = [&](uint64_t& q) -> int {
+        const uint64_t nc = varint(q);
+        const uint64_t cpos = q;
+        if (!skip(q, nc) || q >= end) return 1;
+        const uint64_t lf = varint(q);
+        const uint64_t len = lf >> 4;
+        const uint64_t nbytes = (len + 3) / 4;
+        if (q + nbytes > end) return 1;
+        if (!keep(cpos, nc, len, uint32_t(lf & 15u), q)) return 1;
+        q += nbytes;
+        return 0;
+    };
+    uint64_t q = dbo[b];
+    bool stop = false;
+    while (!stop && q < end) {
+        // every record whose header (count, colours, length) ends inside the
+        // 64 bytes at q is parsed from this one ballot
+        uint64_t T;
+        const uint32_t v = block(q, T);
+        const uint64_t avail = end - q;
+        const uint64_t valid = avail >= 64 ? ~0ull : ((1ull << avail) - 1);
+        const uint64_t Z = ~T & valid;
+        uint64_t R = Z;  // bit a: bytes a..a+9 all continue a varint
+        for (int s = 1; s < 10; ++s) R &= Z >> s;
+        uint32_t o = 0;
+        bool slow = false;
+        while (o < 64 && o < avail) {
+            const uint64_t T0 = T >> o;
+            if (!T0) {
+                slow = true;
+                break;
             }
+            const uint32_t t0 = o + uint32_t(__builtin_ctzll(T0));  // end of the colour count
+            if (t0 - o >= 10) {
+                slow = true;
+                break;
+            }
+            uint64_t nc = 0;
+            for (uint32_t j = o; j <= t0; ++j) nc |= uint64_t(__builtin_amdgcn_readlane(int(v), int(j)) & 0x7f) << (7 * (j - o));
+            uint64_t rest = t0 >= 63 ? 0 : (T & (~0ull << (t0 + 1)));
+            if (uint64_t(__builtin_popcountll(rest)) < nc + 1) {
+                slow = true;
+                break;
+            }
+            uint32_t t1 = t0;  // end of the last colour
+            for (uint64_t i = 0; i < nc; ++i) {
+                t1 = uint32_t(__builtin_ctzll(rest));
+                rest &= rest - 1;
+            }
+            const uint32_t t2 = uint32_t(__builtin_ctzll(rest));  // end of the length
+            const uint32_t span = t2 - o + 1;
+            if (t2 - t1 > 10 || ((R >> o) & (span >= 64 ? ~0ull : ((1ull << span) - 1)))) {
+                slow = true;  // a varint of more than ten bytes
+                break;
+            }
+            uint64_t lf = 0;
+            for (uint32_t j = t1 + 1; j <= t2; ++j)
+                lf |= uint64_t(__builtin_amdgcn_readlane(int(v), int(j)) & 0x7f) << (7 * (j - t1 - 1));
+            const uint64_t len = lf >> 4;
+            const uint64_t nbytes = (len + 3) / 4;
+            const uint64_t bpos = q + t2 + 1;
+            if (bpos + nbytes > end || !keep(q + t0 + 1, nc, len, uint32_t(lf & 15u), bpos)) {
+                stop = true;
+                break;
+            }
+            const uint64_t next = uint64_t(t2) + 1 + nbytes;
+            if (next >= 64) {
+                o = 64;
+                q += next;
+                break;
+            }
+            o = uint32_t(next);
         }
-        return true;
-    });
-    wc[b] = c;
+        if (stop) break;
+        if (o < 64) q += o;
+        if (slow && q < end) stop = general(q) != 0;
+    }
+    if (lane == 0) wc[b] = c;
 }
 
-struct WalkOut {
-    uint64_t* rec_byte;   // bases of record r: byte offset in the decoded stream
-    uint32_t* rec_len;
-    uint8_t* rec_flags;
-    uint32_t* rec_rank0;
-    uint32_t* rec_bucket;
-    uint32_t* pair_color;
-    uint32_t* pair_rec;
-};
+// Descriptor slots -> records in bucket order (r = rec_base[b] + local).
+__global__ void k_walk_compact(uint64_t n_slots, uint32_t B, const uint64_t* __restrict__ region,
+                               const uint64_t* __restrict__ rec_base, const uint64_t* __restrict__ pair_base,
+                               const uint32_t* __restrict__ desc, RecDesc r, uint32_t* __restrict__ rec_bucket) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    const uint32_t b = segment_of(region, B, i);
+    const uint64_t local = i - region[b];
+    if (local >= rec_base[b + 1] - rec_base[b]) return;
+    const uint64_t j = rec_base[b] + local;
+    const uint32_t* x = desc + i * DS_WORDS;
+    const uint64_t cpos = uint64_t(x[DS_CPOS_LO]) | (uint64_t(x[DS_CPOS_HI]) << 32);
+    r.cpos[j] = cpos;
+    r.bpos[j] = cpos + x[DS_BDELTA];
+    r.nc[j] = x[DS_NC];
+    r.len[j] = x[DS_LEN];
+    r.flags[j] = uint8_t(x[DS_FLAGS]);
+    r.rank0[j] = x[DS_RANK0];
+    r.pair0[j] = pair_base[b] + x[DS_PAIR0];
+    rec_bucket[j] = b;
+}
 
-__global__ __launch_bounds__(WALK_TPB) void k_walk_emit(const uint8_t* __restrict__ d, const uint64_t* __restrict__ dbo,
-                                                        uint32_t B, uint32_t k, const uint64_t* __restrict__ rec_base,
-                                                        const uint64_t* __restrict__ pair_base, WalkOut o) {
-    const uint32_t b = blockIdx.x * WALK_TPB + threadIdx.x;
-    if (b >= B) return;
+// Colours of record j into its pairs; *bad = min (bucket << 32 | colour)
+// over colours >= num_colors.
+__global__ void k_walk_pairs(uint64_t nr, const uint8_t* __restrict__ d, const uint64_t* __restrict__ dbo,
+                             const RecDesc r, const uint32_t* __restrict__ rec_bucket, uint32_t num_colors,
+                             uint32_t* __restrict__ pair_color, uint32_t* __restrict__ pair_rec,
+                             unsigned long long* __restrict__ bad) {
+    const uint64_t j = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (j >= nr) return;
+    const uint32_t b = rec_bucket[j];
     const uint64_t end = dbo[b + 1];
-    uint64_t r = rec_base[b], p = pair_base[b];
-    const uint64_t r_end = rec_base[b + 1], p_end = pair_base[b + 1];
-    uint64_t rank = 0;
-    walk_records(d, dbo[b], end, [&](uint64_t cpos, uint64_t nc, uint64_t len, uint8_t fl, uint64_t bpos) {
-        if (len < k) return true;
-        if (r >= r_end || p + nc > p_end) return false;  // the counting walk saw no more
-        o.rec_byte[r] = bpos;
-        o.rec_len[r] = uint32_t(len);
-        o.rec_flags[r] = fl;
-        o.rec_rank0[r] = uint32_t(rank);
-        o.rec_bucket[r] = b;
-        rank += len - k + 1;
-        uint32_t col = 0;
-        for (uint64_t j = 0; j < nc; ++j) {
-            col += uint32_t(get_varint(d, end, cpos));
-            o.pair_color[p] = col;
-            o.pair_rec[p] = uint32_t(r);
-            ++p;
-        }
-        ++r;
-        return true;
-    });
+    uint64_t q = r.cpos[j], p = r.pair0[j];
+    uint32_t col = 0;
+    for (uint32_t i = 0; i < r.nc[j]; ++i, ++p) {
+        col += uint32_t(get_varint(d, end, q));
+        if (col >= num_colors) atomicMin(bad, (unsigned long long)((uint64_t(b) << 32) | col));
+        pair_color[p] = col;
+        pair_rec[p] = uint32_t(j);
+    }
 }
 
-__global__ void k_pair_gather(uint64_t n, const uint32_t* __restrict__ pair_rec, const WalkOut o,
-                              uint64_t* __restrict__ start, uint32_t* __restrict__ len, uint8_t* __restrict__ flags,
-                              uint32_t* __restrict__ rank0, uint32_t* __restrict__ bucket) {
+__global__ void k_pair_gather(uint64_t n, const uint32_t* __restrict__ pair_rec, const RecDesc r,
+                              const uint32_t* __restrict__ rec_bucket, uint64_t* __restrict__ start,
+                              uint32_t* __restrict__ len, uint8_t* __restrict__ flags, uint32_t* __restrict__ rank0,
+                              uint32_t* __restrict__ bucket) {
     const uint64_t p = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    const uint32_t r = pair_rec[p];
-    start[p] = 4 * o.rec_byte[r];
-    len[p] = o.rec_len[r];
-    flags[p] = o.rec_flags[r];
-    rank0[p] = o.rec_rank0[r];
-    bucket[p] = o.rec_bucket[r];
+    const uint32_t j = pair_rec[p];
+    start[p] = 4 * r.bpos[j];
+    len[p] = r.len[j];
+    flags[p] = r.flags[j];
+    rank0[p] = r.rank0[j];
+    bucket[p] = rec_bucket[j];
 }
 
 uint32_t bits_for(uint64_t max_value) {  // bits to hold every value in [0, max_value]
@@ -560,6 +739,7 @@ void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64
         if ((std::fclose(f) != 0) || !ok) throw Error("short write to " + path);
         if (unc_bytes) unc_bytes[b] = bb[b + 1] - bb[b];
     });
+    if (J.prof.on) J.prof.collect();  // the stream is idle: every launch above has finished
 }
 
 void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
@@ -640,10 +820,21 @@ void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
     launch_lz4_decompress(dimg, d_soff, d_cs, nf, dec, d_doff, d_bad, st);
     std::vector<uint8_t> bad(nf);
     if (nf) CDBG_HIP(hipMemcpyAsync(bad.data(), d_bad, nf, hipMemcpyDeviceToHost, st));
-    // 3. count each bucket's records and pairs
-    DevBuf d_wc;
+    // 3. walk each bucket: kept records into slot ranges sized by the
+    //    smallest kept record (count byte, length byte, ceil(k/4) bases)
+    const uint64_t min_rec = 2 + (k + 3) / 4;
+    std::vector<uint64_t> region(B + 1, 0);
+    for (uint32_t b = 0; b < B; ++b) region[b + 1] = region[b] + (dbo[b + 1] - dbo[b]) / min_rec + 1;
+    const uint64_t ns = region[B];
+    DevBuf d_region, d_wc, d_desc;
+    uint64_t* dreg = d_region.ensure<uint64_t>(3 * (uint64_t(B) + 1));
+    CDBG_HIP(hipMemcpyAsync(dreg, region.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
     WalkCounts* wc = d_wc.ensure<WalkCounts>(B);
-    { KSCOPE("k_walk_count", st); k_walk_count<<<blocks_for(B, WALK_TPB), WALK_TPB, 0, st>>>(dec, d_dbo, B, k, J.num_colors, wc); }
+    uint32_t* desc = d_desc.ensure<uint32_t>(ns * DS_WORDS);
+    if (B) {
+        KSCOPE("k_walk", st);
+        k_walk<<<B, 64, 0, st>>>(dec, d_dbo, k, dreg, desc, wc);
+    }
     CDBG_HIP(hipGetLastError());
     std::vector<WalkCounts> h_wc(B);
     CDBG_HIP(hipMemcpyAsync(h_wc.data(), wc, B * sizeof(WalkCounts), hipMemcpyDeviceToHost, st));
@@ -655,29 +846,46 @@ void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
         const WalkCounts& c = h_wc[b];
         if (c.err == WALK_RANK)
             throw Error("bucket holds more than 2^32 k-mer occurrences: " + bucket_path(dir, b));
-        if (c.err == WALK_COLOR)
-            throw Error("bucket " + std::to_string(b) + " names colour " + std::to_string(c.err_color) +
-                        " >= num_colors");
+        if (c.err == WALK_BIG) throw Error("a record of 4 GiB or more in " + bucket_path(dir, b));
+        if (c.err != WALK_OK) throw Error("bucket walk overran its record slots: " + bucket_path(dir, b));
         rec_base[b + 1] = rec_base[b] + c.kept;
         pair_base[b + 1] = pair_base[b] + c.pairs;
         J.stats.num_supers += c.read;
     }
     const uint64_t nr = rec_base[B], np = pair_base[B];
     if (np >= (1ull << 32)) throw Error("more than 2^32 (record, colour) pairs in one job");
-    // 4. records and pairs
-    DevBuf d_base, d_rbyte, d_rlen, d_rfl, d_rrank, d_rbkt, d_pcol, d_prec;
-    uint64_t* base = d_base.ensure<uint64_t>(2 * (uint64_t(B) + 1));
-    CDBG_HIP(hipMemcpyAsync(base, rec_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
-    CDBG_HIP(hipMemcpyAsync(base + B + 1, pair_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
-    WalkOut o{d_rbyte.ensure<uint64_t>(nr), d_rlen.ensure<uint32_t>(nr), d_rfl.ensure<uint8_t>(nr),
-              d_rrank.ensure<uint32_t>(nr), d_rbkt.ensure<uint32_t>(nr), d_pcol.ensure<uint32_t>(np),
-              d_prec.ensure<uint32_t>(np)};
-    { KSCOPE("k_walk_emit", st); k_walk_emit<<<blocks_for(B, WALK_TPB), WALK_TPB, 0, st>>>(dec, d_dbo, B, k, base, base + B + 1, o); }
+    // 4. records in bucket order, then one pair per colour
+    CDBG_HIP(hipMemcpyAsync(dreg + B + 1, rec_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipMemcpyAsync(dreg + 2 * (B + 1), pair_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));
+    DevBuf r_cpos, r_bpos, r_nc, r_len, r_fl, r_rank, r_pair, r_bkt, d_pcol, d_prec, d_bad2;
+    RecDesc rd{r_cpos.ensure<uint64_t>(nr), r_bpos.ensure<uint64_t>(nr), r_nc.ensure<uint32_t>(nr),
+               r_len.ensure<uint32_t>(nr), r_fl.ensure<uint8_t>(nr), r_rank.ensure<uint32_t>(nr),
+               r_pair.ensure<uint64_t>(nr)};
+    uint32_t* rbkt = r_bkt.ensure<uint32_t>(nr);
+    uint32_t* pcol = d_pcol.ensure<uint32_t>(np);
+    uint32_t* prec = d_prec.ensure<uint32_t>(np);
+    auto* bad2 = d_bad2.ensure<unsigned long long>(1);
+    CDBG_HIP(hipMemsetAsync(bad2, 0xff, 8, st));
+    if (ns) {
+        KSCOPE("k_walk_compact", st);
+        k_walk_compact<<<blocks_for(ns), TPB, 0, st>>>(ns, B, dreg, dreg + B + 1, dreg + 2 * (B + 1), desc, rd, rbkt);
+    }
     CDBG_HIP(hipGetLastError());
+    d_desc.release();
     d_img.release();
+    if (nr) {
+        KSCOPE("k_walk_pairs", st);
+        k_walk_pairs<<<blocks_for(nr), TPB, 0, st>>>(nr, dec, d_dbo, rd, rbkt, J.num_colors, pcol, prec, bad2);
+    }
+    CDBG_HIP(hipGetLastError());
+    unsigned long long h_bad2 = 0;
+    CDBG_HIP(hipMemcpyAsync(&h_bad2, bad2, 8, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    if (h_bad2 != ~0ull)
+        throw Error("bucket " + std::to_string(h_bad2 >> 32) + " names colour " +
+                    std::to_string(h_bad2 & 0xffffffffu) + " >= num_colors");
     // 5. colour-major inside each bucket, record order kept among equal colours
-    sort_in_segments(o.pair_color, o.pair_rec, np, pair_base, bits_for(std::max(1u, J.num_colors) - 1), J.rs, st,
-                     "pairs");
+    sort_in_segments(pcol, prec, np, pair_base, bits_for(std::max(1u, J.num_colors) - 1), J.rs, st, "pairs");
     uint64_t* start = Q.b_start.ensure<uint64_t>(np);
     uint32_t* len = Q.b_len.ensure<uint32_t>(np);
     uint8_t* flags = Q.b_flags.ensure<uint8_t>(np);
@@ -685,9 +893,9 @@ void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
     uint32_t* bucket = Q.key.ensure<uint32_t>(np);
     uint32_t* color = Q.b_color.ensure<uint32_t>(np);
     if (np) {
-        { KSCOPE("k_pair_gather", st); k_pair_gather<<<blocks_for(np), TPB, 0, st>>>(np, o.pair_rec, o, start, len, flags, rank0, bucket); }
+        { KSCOPE("k_pair_gather", st); k_pair_gather<<<blocks_for(np), TPB, 0, st>>>(np, prec, rd, rbkt, start, len, flags, rank0, bucket); }
         CDBG_HIP(hipGetLastError());
-        CDBG_HIP(hipMemcpyAsync(color, o.pair_color, np * 4, hipMemcpyDeviceToDevice, st));
+        CDBG_HIP(hipMemcpyAsync(color, pcol, np * 4, hipMemcpyDeviceToDevice, st));
     }
     uint64_t* bucket_off = Q.bucket_off.ensure<uint64_t>(B + 1);
     CDBG_HIP(hipMemcpyAsync(bucket_off, pair_base.data(), (B + 1) * 8ull, hipMemcpyHostToDevice, st));

@@ -281,6 +281,7 @@ CDBG_API int cdbg_gpu_job_write_bucket_files(cdbg_gpu_job* job, const char* dir,
     CDBG_TRY({
         if (!job || !dir) throw Error("null argument");
         CDBG_HIP(hipSetDevice(job->J.ctx->device));
+        ProfScope prof(&job->J.prof);
         write_bucket_files(job->J, dir, threads, bucket_unc_bytes);
     })
 }

@@ -8,16 +8,14 @@
 // (bucket_io.hpp:19-40, written with LZ4_compress_default :255-265 and read
 // with LZ4_decompress_safe :695-696); any valid block is accepted.
 //
-// Parallel layout: one lane per frame, thousands of frames per launch.
-//   k_lz4_compress   frames of at most LZ_FRAME (64 KiB), so every position
-//                    and match offset fits 16 bits; a 512-entry table of
-//                    positions per lane in LDS (64 lanes x 1 KiB per
-//                    workgroup); greedy parse; the input is read as aligned
-//                    dwords (a two-dword window slides along the scan) and
-//                    the output is gathered into dwords before it is stored.
-//   k_lz4_decompress any frame size; every length, offset and capacity is
-//                    checked, and a malformed frame is flagged, never read or
-//                    written outside its ranges.
+// Parallel layout:
+//   k_lz4_compress   one wavefront per frame of at most LZ_FRAME (64 KiB, so
+//                    every position and match offset fits 16 bits); the frame
+//                    and a 2048-entry hash table live in LDS; matches are
+//                    found 64 positions at a time and taken greedily.
+//   k_lz4_decompress one lane per frame, any frame size; every length,
+//                    offset and capacity is checked, and a malformed frame is
+//                    flagged, never read or written outside its ranges.
 //   k_lz4_pack       one workgroup per frame: header + block into the
 //                    bucket-file image.
 #include <hip/hip_runtime.h>
@@ -32,122 +30,139 @@ namespace cdbg_amd {
 
 namespace {
 
-constexpr int LZ_HASH_LOG = 9;
+constexpr int LZ_HASH_LOG = 11;  // 8 KiB table + 64 KiB frame: two workgroups per CU
 constexpr int LZ_HASH = 1 << LZ_HASH_LOG;
 constexpr int LZ_LANES = 64;
 constexpr uint32_t LZ_MIN_MATCH = 4, LZ_LAST_LITERALS = 5, LZ_MF_LIMIT = 12;
-constexpr uint16_t LZ_EMPTY = 0xFFFF;
+constexpr uint32_t LZ_CAP = 64;  // match bytes each lane measures; longer matches are extended by the wave
 
-// 4 bytes at any address, from the two aligned dwords that cover them.
-__device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
-    const uintptr_t a = uintptr_t(p);
-    const uint32_t* w = (const uint32_t*)(a & ~uintptr_t(3));
-    const uint32_t sh = uint32_t(a & 3) * 8;
-    const uint32_t lo = w[0];
-    return sh ? (lo >> sh) | (w[1] << (32 - sh)) : lo;
+// Frame bytes staged in LDS as dwords; frame byte p sits at LDS byte p + sh.
+__device__ __forceinline__ uint32_t rd32(const uint32_t* fr, uint32_t a) {
+    return __builtin_amdgcn_alignbyte(fr[(a >> 2) + 1], fr[a >> 2], a & 3);
+}
+__device__ __forceinline__ uint32_t rd8(const uint32_t* fr, uint32_t a) { return (fr[a >> 2] >> (8 * (a & 3))) & 0xffu; }
+
+// Length bytes past a token's 15: r / 255 bytes of 255, then r % 255.
+__device__ __forceinline__ void put_run(uint8_t* o, uint32_t& op, uint32_t r, uint32_t lane) {
+    const uint32_t full = r / 255, nb = full + 1;
+    for (uint32_t i = lane; i < nb; i += LZ_LANES) o[op + i] = i < full ? uint8_t(255) : uint8_t(r - 255 * full);
+    op += nb;
 }
 
-// Output gathered into dwords; the slot starts 4-byte aligned and has room
-// for the last partial dword.
-struct Out {
-    uint32_t* o;
-    uint32_t op, acc;
-    __device__ __forceinline__ void put(uint32_t b) {
-        acc |= b << (8 * (op & 3));
-        if ((op & 3) == 3) {
-            o[op >> 2] = acc;
-            acc = 0;
-        }
-        ++op;
-    }
-    __device__ __forceinline__ void len(uint32_t r) {  // length bytes past a token's 15
-        while (r >= 255) {
-            put(255);
-            r -= 255;
-        }
-        put(r);
-    }
-    __device__ __forceinline__ void flush() {
-        if (op & 3) o[op >> 2] = acc;
-    }
-};
-
-// One sequence: the literals s[a, a + nlit), then (mlen > 0) a match of mlen
+// One sequence, written by the whole wave (op is the same in every lane):
+// the literals at frame bytes [a, a + nlit), then (mlen > 0) a match of mlen
 // bytes at distance off.
-__device__ __forceinline__ void emit_seq(Out& w, const uint8_t* s, uint32_t a, uint32_t nlit, uint32_t off,
-                                         uint32_t mlen) {
+__device__ __forceinline__ void emit_seq(uint8_t* o, uint32_t& op, const uint32_t* fr, uint32_t sh, uint32_t a,
+                                         uint32_t nlit, uint32_t off, uint32_t mlen, uint32_t lane) {
     const uint32_t ml = mlen ? mlen - LZ_MIN_MATCH : 0;
-    w.put(((nlit >= 15 ? 15u : nlit) << 4) | (mlen ? (ml >= 15 ? 15u : ml) : 0u));
-    if (nlit >= 15) w.len(nlit - 15);
-    uint32_t i = 0;
-    while (i < nlit) {  // literals a dword at a time
-        const uint32_t v = ld32(s + a + i);
-        const uint32_t c = nlit - i < 4 ? nlit - i : 4;
-        for (uint32_t j = 0; j < c; ++j) w.put((v >> (8 * j)) & 0xffu);
-        i += c;
-    }
+    if (lane == 0) o[op] = uint8_t(((nlit >= 15 ? 15u : nlit) << 4) | (mlen ? (ml >= 15 ? 15u : ml) : 0u));
+    op += 1;
+    if (nlit >= 15) put_run(o, op, nlit - 15, lane);
+    for (uint32_t i = lane; i < nlit; i += LZ_LANES) o[op + i] = uint8_t(rd8(fr, sh + a + i));
+    op += nlit;
     if (!mlen) return;
-    w.put(off & 0xffu);
-    w.put(off >> 8);
-    if (ml >= 15) w.len(ml - 15);
+    if (lane < 2) o[op + lane] = uint8_t(lane ? off >> 8 : off & 0xffu);
+    op += 2;
+    if (ml >= 15) put_run(o, op, ml - 15, lane);
 }
 
+// One wavefront per frame.  Positions are taken 64 at a time: every lane
+// hashes the 4 bytes at its position, reads the last earlier position with
+// that hash from the table (positions of earlier rounds), and measures its
+// match up to LZ_CAP bytes; the wave then takes the matches greedily in
+// position order (a match shadows the positions it covers), extending any
+// that reached LZ_CAP 64 bytes at a time, and writes each sequence.  Every
+// position enters the table, inside matches too.
 __global__ __launch_bounds__(LZ_LANES) void k_lz4_compress(const uint8_t* __restrict__ src,
                                                            const uint64_t* __restrict__ foff, uint32_t nf,
                                                            uint8_t* __restrict__ slots,
                                                            const uint64_t* __restrict__ slot_off,
                                                            uint32_t* __restrict__ csize) {
-    __shared__ uint16_t tab[LZ_HASH][LZ_LANES];  // lane-interleaved: a lane's probes spread over the banks
-    const uint32_t lane = threadIdx.x;
-    for (int i = 0; i < LZ_HASH; ++i) tab[i][lane] = LZ_EMPTY;
-    const uint64_t f = uint64_t(blockIdx.x) * LZ_LANES + lane;
-    if (f >= nf) return;
+    __shared__ uint32_t fr[LZ_FRAME / 4 + 4];
+    __shared__ uint32_t tab[LZ_HASH];  // last position + 1 per hash, 0 when none
+    const uint32_t f = blockIdx.x, lane = threadIdx.x;
     const uint8_t* s = src + foff[f];
     const uint32_t n = uint32_t(foff[f + 1] - foff[f]);
-    Out w{(uint32_t*)(slots + slot_off[f]), 0, 0};
-    uint32_t anchor = 0, ip = 0;
+    const uint32_t sh = uint32_t(uintptr_t(s) & 3);
+    const uint32_t* w = (const uint32_t*)(s - sh);
+    const uint32_t nw = (n + sh + 3) / 4;  // reads end at most 3 bytes past the frame
+    for (uint32_t i0 = 0; i0 < nw + 2; i0 += 16 * LZ_LANES) {  // 16 loads in flight per lane
+        uint32_t t[16];
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u) {
+            const uint32_t i = i0 + u * LZ_LANES + lane;
+            t[u] = i < nw ? w[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 16; ++u) {
+            const uint32_t i = i0 + u * LZ_LANES + lane;
+            if (i < nw + 2) fr[i] = t[u];
+        }
+    }
+    for (uint32_t i = lane; i < uint32_t(LZ_HASH); i += LZ_LANES) tab[i] = 0;
+    __syncthreads();
+    uint8_t* o = slots + slot_off[f];
+    uint32_t op = 0, anchor = 0, cur = 0;
     if (n >= LZ_MF_LIMIT + 1) {
         const uint32_t match_limit = n - LZ_LAST_LITERALS;  // matches end at or before this
         const uint32_t search_end = n - LZ_MF_LIMIT;        // and start before this
-        const uintptr_t a0 = uintptr_t(s);
-        // window: the aligned dwords at wp[0], wp[1], wp = (a0 + ip) & ~3
-        const uint32_t* wp = (const uint32_t*)(a0 & ~uintptr_t(3));
-        uint32_t lo = wp[0], hi = wp[1];
-        while (ip < search_end) {
-            const uint32_t sh = uint32_t((a0 + ip) & 3) * 8;
-            const uint32_t seq = sh ? (lo >> sh) | (hi << (32 - sh)) : lo;
-            const uint32_t h = (seq * 2654435761u) >> (32 - LZ_HASH_LOG);
-            const uint32_t c = tab[h][lane];
-            tab[h][lane] = uint16_t(ip);
-            if (c != LZ_EMPTY && ld32(s + c) == seq) {
-                uint32_t m = LZ_MIN_MATCH;
-                while (ip + m + 4 <= match_limit) {  // extend a dword at a time
-                    const uint32_t x = ld32(s + c + m) ^ ld32(s + ip + m);
-                    if (x) break;
-                    m += 4;
+        for (uint32_t p0 = 0; p0 < search_end; p0 += LZ_LANES) {
+            const uint32_t p = p0 + lane;
+            const bool act = p < search_end;
+            uint32_t seq = 0, h = 0, c = 0;
+            if (act) {
+                seq = rd32(fr, sh + p);
+                h = (seq * 2654435761u) >> (32 - LZ_HASH_LOG);
+                c = tab[h];
+            }
+            __syncthreads();  // this round's reads see only earlier rounds
+            if (act) atomicMax(&tab[h], p + 1);
+            uint32_t L = 0;
+            if (act && c != 0 && rd32(fr, sh + c - 1) == seq) {
+                const uint32_t q = c - 1;
+                bool exact = false;
+                L = LZ_MIN_MATCH;
+                while (L < LZ_CAP && p + L + 4 <= match_limit) {
+                    const uint32_t x = rd32(fr, sh + q + L) ^ rd32(fr, sh + p + L);
+                    if (x) {
+                        L += uint32_t(__builtin_ctz(x)) >> 3;
+                        exact = true;
+                        break;
+                    }
+                    L += 4;
                 }
-                while (ip + m < match_limit && s[c + m] == s[ip + m]) ++m;  // the last few bytes
-                emit_seq(w, s, anchor, ip - anchor, ip - c, m);
-                ip += m;
-                anchor = ip;
-                if (ip < search_end) {
-                    wp = (const uint32_t*)((a0 + ip) & ~uintptr_t(3));
-                    lo = wp[0];
-                    hi = wp[1];
+                if (!exact)
+                    while (L < LZ_CAP && p + L < match_limit && rd8(fr, sh + q + L) == rd8(fr, sh + p + L)) ++L;
+            }
+            const uint64_t M = __ballot(L >= LZ_MIN_MATCH);
+            while (cur < p0 + LZ_LANES) {
+                const uint32_t lo = cur > p0 ? cur - p0 : 0;
+                const uint64_t E = M & (~0ull << lo);
+                if (!E) break;
+                const uint32_t j = uint32_t(__builtin_ctzll(E));
+                const uint32_t mp = p0 + j;
+                uint32_t mL = uint32_t(__builtin_amdgcn_readlane(int(L), int(j)));
+                const uint32_t mq = uint32_t(__builtin_amdgcn_readlane(int(c), int(j))) - 1;
+                if (mL >= LZ_CAP) {
+                    for (;;) {
+                        const uint32_t i = mL + lane;
+                        const bool eq = mp + i < match_limit && rd8(fr, sh + mq + i) == rd8(fr, sh + mp + i);
+                        const uint64_t ne = __ballot(!eq);
+                        if (ne) {
+                            mL += uint32_t(__builtin_ctzll(ne));
+                            break;
+                        }
+                        mL += LZ_LANES;
+                    }
                 }
-            } else {
-                ++ip;
-                if (((a0 + ip) & 3) == 0 && ip < search_end) {
-                    ++wp;
-                    lo = hi;
-                    hi = wp[1];
-                }
+                emit_seq(o, op, fr, sh, anchor, mp - anchor, mp - mq, mL, lane);
+                cur = mp + mL;
+                anchor = cur;
             }
         }
     }
-    emit_seq(w, s, anchor, n - anchor, 0, 0);  // the last literals
-    w.flush();
-    csize[f] = w.op;
+    emit_seq(o, op, fr, sh, anchor, n - anchor, 0, 0, lane);  // the last literals
+    if (lane == 0) csize[f] = op;
 }
 
 __global__ __launch_bounds__(LZ_LANES) void k_lz4_decompress(const uint8_t* __restrict__ src,
@@ -236,7 +251,7 @@ inline uint32_t lz_blocks(uint64_t nf) { return uint32_t((nf + LZ_LANES - 1) / L
 void launch_lz4_compress(const uint8_t* src, const uint64_t* foff, uint32_t nf, uint8_t* slots,
                          const uint64_t* slot_off, uint32_t* csize, hipStream_t st) {
     if (!nf) return;
-    { KSCOPE("k_lz4_compress", st); k_lz4_compress<<<lz_blocks(nf), LZ_LANES, 0, st>>>(src, foff, nf, slots, slot_off, csize); }
+    { KSCOPE("k_lz4_compress", st); k_lz4_compress<<<nf, LZ_LANES, 0, st>>>(src, foff, nf, slots, slot_off, csize); }
     CDBG_HIP(hipGetLastError());
 }
 

@@ -242,3 +242,85 @@ def test_failed_profiled_run_then_file_write(gpu_ctx, tmp_path):
     boff = np.concatenate([[0], np.cumsum(np.bincount(sup["bucket"], minlength=16))])
     for b in range(16):
         assert _records(os.path.join(d, f"bucket_{b}.bin")) == _merged_bucket(data, sup, boff[b], boff[b + 1])
+
+
+def _stream(recs, varint=_enc_varint):
+    out = bytearray()
+    for cols, fl, bases in recs:
+        out += varint(len(cols))
+        prev = 0
+        for c in cols:
+            out += varint(c - prev)
+            prev = c
+        out += varint((len(bases) << 4) | fl)
+        packed = bytearray((len(bases) + 3) // 4)
+        for i, ch in enumerate(bases):
+            packed[i // 4] |= F.ACGT.index(ch) << (2 * (i % 4))
+        out += packed
+    return bytes(out)
+
+
+def _write_stream(path, stream, frame=700):
+    with open(path, "wb") as f:
+        for o in range(0, len(stream), frame):
+            blk = stream[o:o + frame]
+            lit = _literal_block(blk)
+            f.write(struct.pack("<II", len(blk), len(lit)) + lit)
+        f.write(struct.pack("<I", 0))
+
+
+def _padded_varint(width):
+    """LEB128 with redundant continuation bytes, `width` bytes long (at most
+    ten: varint_read takes ten bytes, super_kmer.hpp:71-82)."""
+    def enc(v):
+        out = bytearray()
+        for i in range(width):
+            b = v & 0x7F
+            v >>= 7
+            out.append(b | (0x80 if i + 1 < width else 0))
+        return bytes(out)
+    return enc
+
+
+def test_stream_edge_cases(gpu_ctx, tmp_path):
+    """Streams the reader must walk as read_super_kmer does
+    (super_kmer.hpp:121-143): varints padded to 2..10 bytes, a record of
+    many colours, records shorter than k, a stream cut inside its last record
+    (reading stops there), an empty bucket, a bucket without end marker."""
+    k, m, B, nc = 31, 19, 6, 5
+    files = _fasta_job(tmp_path, colors=nc, glen=6000, seed=31)
+    data, off = G.load_fasta(files)
+    sup = O.Build(data, off, k, m, B).supers(1)
+    boff = np.concatenate([[0], np.cumsum(np.bincount(sup["bucket"], minlength=B))])
+    d = tmp_path / "edge"
+    d.mkdir()
+    for b in range(B):
+        recs = _merged_bucket(data, sup, boff[b], boff[b + 1])
+        path = str(d / f"bucket_{b}.bin")
+        if b == 0:
+            _write_stream(path, _stream(recs, _padded_varint(10)))
+        elif b == 1:
+            s = b"".join(_stream([r], _padded_varint(2 + i % 8)) for i, r in enumerate(recs))
+            _write_stream(path, s, frame=333)
+        elif b == 2:
+            s = _stream(recs + [((0, 1, 2, 3, 4), 3, b"ACGTACGTAC"), ((1,), 0, b"A" * 31)])
+            _write_stream(path, s[:-3])  # the last record loses its last bytes
+        elif b == 3:
+            _write_stream(path, b"")
+        elif b == 4:
+            s = _stream([((0,) * 200 + (1, 2, 3, 4), 12, b"C" * 5)] + recs)  # 204 colours, len < k
+            _write_stream(path, s)
+            with open(path, "r+b") as f:  # drop the end marker
+                f.truncate(os.path.getsize(path) - 4)
+        else:
+            _write_stream(path, _stream(recs), frame=64)
+    _compare_phase2(gpu_ctx, tmp_path, str(d), B, k, nc)
+
+
+def test_colour_out_of_range(gpu_ctx, tmp_path):
+    d = tmp_path / "bad"
+    d.mkdir()
+    _write_stream(str(d / "bucket_0.bin"), _stream([((0, 2), 0, b"ACGT" * 10)]))
+    _write_stream(str(d / "bucket_1.bin"), _stream([((1, 7), 0, b"ACGA" * 10)]))
+    with pytest.raises(RuntimeError, match="bucket 1 names colour 7"):
+        gpu_ctx.job_from_bucket_files(str(d), 2, 31, 4)
