@@ -200,6 +200,17 @@ def batched_leg(G, ctx, steps, warmup, threads):
         job.set_bucket_map(G.zipf_bucket_cuts(c["buckets"], c["zipf"]))
         for _ in range(warmup):
             job.run()
+        # the same workload with the default knobs (one workgroup per cell),
+        # then tuned by the autotuner over a sample of its first buckets
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            job.run()
+        plain_elapsed = time.perf_counter() - t0
+        job.phase1()
+        t0 = time.perf_counter()
+        tune = job.autotune()
+        tune["tuning_s"] = round(time.perf_counter() - t0, 3)
+        job.run()
         job.set_profiling(True)
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -209,12 +220,15 @@ def batched_leg(G, ctx, steps, warmup, threads):
         kt = job.kernel_times()
         chunks = job.num_chunks()
         sizes = np.diff(job.bucket_offsets().astype(np.int64))
+    plain_value = s["num_kmers"] * steps / plain_elapsed
     ms_step = 1e3 * elapsed / steps
     roof, kernels = roofline(kt, s, c["k"], steps, chunks, ms_step)
     value = s["num_kmers"] * steps / elapsed
     print(f"bench: batched {steps} steps in {elapsed:.2f} s, {value / 1e9:.3f} G k-mers/s", file=sys.stderr,
           flush=True)
     return {"metric": "distinct k-mers/s (phases 1+2)", "value": round(value, 1), "ms_per_step": round(ms_step, 3),
+            "default_knobs_value": round(plain_value, 1), "tuned_over_default": round(value / plain_value, 3),
+            "autotune": tune,
             "steps": steps, "ms_phase1": round(s["ms_phase1"], 2), "ms_phase2": round(s["ms_phase2"], 2),
             "chunks": chunks, "num_kmers": s["num_kmers"], "num_occ": s["num_occ"],
             "bucket_supers": {"max": int(sizes.max()), "median": int(np.median(sizes)), "min": int(sizes.min())},
@@ -513,6 +527,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrivals per second")
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel totals to stderr")
     ap.add_argument("--no-batched", action="store_true", help="skip the skewed-bucket (batched) leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the skewed-bucket (batched) leg")
     ap.add_argument("--copy-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two profiler runs that measure traffic")
@@ -541,6 +556,11 @@ def main():
     if args.large_only:
         ctx = G.Context(0)
         print(json.dumps({"large": large_leg(G, ctx, args.large_colors or 4000, args.steps, args.cpu_cores)}))
+        ctx.close()
+        return
+    if args.batched_only:
+        ctx = G.Context(0)
+        print(json.dumps({"batched": batched_leg(G, ctx, args.steps, args.warmup, args.cpu_cores)}))
         ctx.close()
         return
     if args.copy_probe_only:

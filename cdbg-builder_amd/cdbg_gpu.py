@@ -27,6 +27,31 @@ class Stats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Tuning(C.Structure):
+    """Phase-2 performance knobs (cdbg_gpu_tuning); zero fields are defaults."""
+    _fields_ = [("cell_target", C.c_double), ("digit_bits", C.c_uint32), ("persistent", C.c_uint32),
+                ("set_slack", C.c_double), ("graph", C.c_uint32), ("reserved", C.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("best", Tuning), ("trials", C.c_uint32), ("reserved", C.c_uint32),
+                ("sample_occurrences", C.c_uint64), ("default_ms", C.c_double), ("best_ms", C.c_double)]
+
+    def as_dict(self):
+        return {"best": self.best.as_dict(), "trials": self.trials, "sample_occurrences": self.sample_occurrences,
+                "default_ms": self.default_ms, "best_ms": self.best_ms}
+
+
+def tuning(**kw):
+    t = Tuning()
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
 class ColorState(C.Structure):
     _fields_ = [("num_colors", C.c_uint32), ("sparse_threshold", C.c_uint32),
                 ("dense_threshold", C.c_uint32), ("num_color_sets", C.c_uint64),
@@ -131,6 +156,10 @@ _SIGS = {
     "cdbg_seqs_free": (None, [C.c_void_p]),
     "cdbg_lz4_compress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
     "cdbg_lz4_decompress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
+    "cdbg_gpu_job_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "cdbg_gpu_job_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "cdbg_gpu_ctx_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
+    "cdbg_gpu_job_autotune": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(TuneReport)]),
     "cdbg_gpu_lz4_compress_frames": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, U8P, C.c_uint64, U64P]),
     "cdbg_gpu_lz4_decompress_frames": (C.c_int, [C.c_void_p, U8P, U64P, C.c_uint32, U8P, U64P]),
     "cdbg_synth_create": (C.c_void_p, [C.POINTER(SynthParams), C.c_uint32]),
@@ -345,6 +374,10 @@ class Context:
     def server(self, slots=4, hbm_budget=0, keep_results=True, borrow_inputs=True):
         return Server(self, slots, hbm_budget, keep_results, borrow_inputs)
 
+    def set_tuning(self, t=None, **kw):
+        """Default tuning of jobs created on this context from now on."""
+        _check(lib().cdbg_gpu_ctx_set_tuning(self.h, C.byref(t if t is not None else tuning(**kw))))
+
     def lz4_compress_frames(self, frames):
         """LZ4 blocks of the byte strings `frames` (each <= 64 KiB), compressed
         on the device, one lane per frame."""
@@ -462,6 +495,21 @@ class Job:
             return
         c = np.ascontiguousarray(cuts, dtype=np.uint64)
         _check(lib().cdbg_gpu_job_set_bucket_map(self.h, _ptr(c, U64P), len(c)))
+
+    def set_tuning(self, t=None, **kw):
+        _check(lib().cdbg_gpu_job_set_tuning(self.h, C.byref(t if t is not None else tuning(**kw))))
+
+    def get_tuning(self):
+        t = Tuning()
+        _check(lib().cdbg_gpu_job_get_tuning(self.h, C.byref(t)))
+        return t
+
+    def autotune(self, sample_occurrences=0):
+        """Times phase 2 over a sample under candidate knobs and keeps the
+        fastest (phase 1 must have run; run phase 2 again afterwards)."""
+        r = TuneReport()
+        _check(lib().cdbg_gpu_job_autotune(self.h, sample_occurrences, C.byref(r)))
+        return r.as_dict()
 
     def set_profiling(self, on=True):
         _check(lib().cdbg_gpu_job_set_profiling(self.h, 1 if on else 0))

@@ -130,6 +130,7 @@ CDBG_API int cdbg_gpu_job_create(cdbg_gpu_ctx* ctx, const uint8_t* bytes, const 
         if (!ctx || !out) throw Error("null argument");
         CDBG_HIP(hipSetDevice(ctx->device));
         auto* h = new cdbg_gpu_job();
+        h->J.tune = ctx->tune;
         try {
             job_set_input(h->J, ctx, bytes, color_off, num_colors, k, m, num_buckets);
         } catch (...) {
@@ -141,6 +142,38 @@ CDBG_API int cdbg_gpu_job_create(cdbg_gpu_ctx* ctx, const uint8_t* bytes, const 
 }
 
 CDBG_API void cdbg_gpu_job_destroy(cdbg_gpu_job* job) { delete job; }
+
+CDBG_API int cdbg_gpu_job_set_tuning(cdbg_gpu_job* job, const cdbg_gpu_tuning* t) {
+    CDBG_TRY({
+        if (!job || !t) throw Error("null argument");
+        check_tuning(*t);
+        job->J.tune = *t;
+    })
+}
+
+CDBG_API int cdbg_gpu_job_get_tuning(cdbg_gpu_job* job, cdbg_gpu_tuning* t) {
+    CDBG_TRY({
+        if (!job || !t) throw Error("null argument");
+        *t = job->J.tune;
+    })
+}
+
+CDBG_API int cdbg_gpu_ctx_set_tuning(cdbg_gpu_ctx* ctx, const cdbg_gpu_tuning* t) {
+    CDBG_TRY({
+        if (!ctx || !t) throw Error("null argument");
+        check_tuning(*t);
+        ctx->tune = *t;
+    })
+}
+
+CDBG_API int cdbg_gpu_job_autotune(cdbg_gpu_job* job, uint64_t sample_occurrences, cdbg_gpu_tune_report* out) {
+    CDBG_TRY({
+        if (!job) throw Error("null argument");
+        CDBG_HIP(hipSetDevice(job->J.ctx->device));
+        const cdbg_gpu_tune_report r = autotune(job->J, sample_occurrences);
+        if (out) *out = r;
+    })
+}
 
 CDBG_API int cdbg_gpu_job_set_bucket_map(cdbg_gpu_job* job, const uint64_t* cuts, uint32_t n_cuts) {
     CDBG_TRY({
@@ -297,6 +330,7 @@ CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, 
         try {
             Job& J = h->J;
             J.ctx = ctx;
+            J.tune = ctx->tune;
             J.k = k;
             J.B = num_buckets;
             J.num_colors = num_colors;

@@ -73,6 +73,8 @@ struct Job {
     uint64_t p1_auto_bases = 0;   // limit derived from free HBM on the first run, then kept (~0: one range)
     uint32_t p1_ranges = 0;       // colour ranges used by the last phase-1 run
     bool frags_in_host = false;   // the last phase-2 run kept its fragment stream in host memory
+    cdbg_gpu_tuning tune{};       // phase-2 performance knobs (zero fields: defaults)
+    uint32_t p2_max_chunks = 0;   // stop phase 2 after this many chunks (0: all; autotuner trials)
     ~Job();
 };
 
@@ -83,6 +85,8 @@ void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64
 void run_phase1(Job& J);
 void phase1_core(Job& J);  // run_phase1 without timing or allocation scope
 void run_phase2(Job& J);
+void check_tuning(const cdbg_gpu_tuning& t);  // throws on an invalid knob
+cdbg_gpu_tune_report autotune(Job& J, uint64_t sample_occurrences);
 void free_phase2(Job& J);
 // Host copies of the phase-2 results (any pointer may be null).
 void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offsets);
@@ -98,6 +102,7 @@ void write_phase2_files(Job& J, const std::string& frag_path, const std::string&
 
 struct cdbg_gpu_ctx {
     int device = 0;
+    cdbg_gpu_tuning tune{};  // default tuning of jobs created on this context
 };
 
 struct cdbg_gpu_job {

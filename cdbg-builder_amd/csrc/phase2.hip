@@ -52,6 +52,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <memory>
 #include <type_traits>
 #include <cstdlib>
 #include <cstring>
@@ -305,16 +307,23 @@ inline size_t cell_sort_lds() {
            ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
 }
 
-template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
-__global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* __restrict__ vals,
-                                                       const uint32_t* __restrict__ cell_start,
-                                                       const uint32_t* __restrict__ list, uint32_t n_items,
-                                                       uint32_t lo, int end_bit, uint32_t* __restrict__ next,
-                                                       unsigned long long* __restrict__ n_next,
-                                                       uint8_t* __restrict__ heads) {
+struct CellSortArgs {
+    const uint32_t* cell_start;
+    const uint32_t* list;  // cells of this class (null: every cell)
+    uint32_t lo;           // cells of at most lo keys were sorted by a smaller class
+    int end_bit;
+    uint32_t* next;        // cells above CAP, for the next class (null: none)
+    unsigned long long* n_next;
+    uint8_t* heads;
+};
+
+// Sorts the item-th cell of the class's list (every thread of the
+// workgroup calls it; it returns uniformly).
+template <class K, class V, uint32_t CAP, int THREADS, bool PACK>
+__device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restrict__ vals, const CellSortArgs& a,
+                                              uint32_t item, uint64_t* sm64) {
     constexpr int WAVES = THREADS / 64;
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
-    extern __shared__ uint64_t sm64[];
     static_assert(CAP <= 65536 && CAP % 2 == 0, "item indices are 16-bit");
     static_assert(!PACK || (std::is_same<K, uint64_t>::value && CAP <= (1u << PACK_IB)), "packed items");
     // keys and their positions in the cell move through LDS; each value is
@@ -324,13 +333,16 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     uint16_t* s_idx = (uint16_t*)(s_key + CAP);
     uint32_t* wcnt = (uint32_t*)(PACK ? (void*)(s_key + CAP) : (void*)(s_idx + CAP));  // [WAVES][256]
     uint32_t* s_tmp = wcnt + WAVES * 256;
-    if (blockIdx.x >= n_items) return;
-    const uint32_t cell = list ? list[blockIdx.x] : blockIdx.x;
+    const uint32_t* cell_start = a.cell_start;
+    const uint32_t lo = a.lo;
+    const int end_bit = a.end_bit;
+    uint8_t* heads = a.heads;
+    const uint32_t cell = a.list ? a.list[item] : item;
     const uint32_t start = cell_start[cell], count = cell_start[cell + 1] - start;
     if (count == 1 && lo == 0 && threadIdx.x == 0) heads[start] = 1;  // a one-key cell is one node
     if (count <= 1 || count <= lo) return;
     if (count > CAP) {
-        if (next && threadIdx.x == 0) next[atomicAdd(n_next, 1ull)] = cell;
+        if (a.next && threadIdx.x == 0) a.next[atomicAdd(a.n_next, 1ull)] = cell;
         return;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -463,16 +475,70 @@ __global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* 
     }
 }
 
-// Lets a k_cell_sort class use its LDS (once per process).
+// One workgroup per listed cell.
+template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
+__global__ __launch_bounds__(THREADS) void k_cell_sort(K* __restrict__ keys, V* __restrict__ vals, CellSortArgs a,
+                                                       uint32_t n_items) {
+    extern __shared__ uint64_t sm64[];
+    if (blockIdx.x >= n_items) return;
+    cell_sort_one<K, V, CAP, THREADS, PACK>(keys, vals, a, blockIdx.x, sm64);
+}
+
+// Persistent workgroups: each takes CELL_Q cells at a time from a device
+// queue (one atomic per batch) until the class's cells are gone.  The cell
+// count is read from device memory (*n32 or *n64, else n_items), so a class
+// whose list a previous kernel built needs no host read-back, and the stage
+// can be captured in a graph.
+constexpr uint32_t CELL_Q = 4;
+template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
+__global__ __launch_bounds__(THREADS) void k_cell_sort_q(K* __restrict__ keys, V* __restrict__ vals, CellSortArgs a,
+                                                         uint32_t n_items, const uint32_t* __restrict__ n32,
+                                                         const unsigned long long* __restrict__ n64,
+                                                         unsigned long long* __restrict__ queue) {
+    extern __shared__ uint64_t sm64[];
+    __shared__ uint32_t s_base;
+    const uint64_t n = n32 ? uint64_t(*n32) : n64 ? uint64_t(*n64) : uint64_t(n_items);
+    for (;;) {
+        if (threadIdx.x == 0) s_base = uint32_t(atomicAdd(queue, (unsigned long long)CELL_Q));
+        __syncthreads();
+        const uint32_t base = s_base;
+        __syncthreads();  // every thread has read s_base before it is written again
+        if (base >= n) break;
+        for (uint32_t j = 0; j < CELL_Q && base + j < n; ++j) {
+            cell_sort_one<K, V, CAP, THREADS, PACK>(keys, vals, a, base + j, sm64);
+            __syncthreads();  // the cell's last LDS reads are done before the next cell's writes
+        }
+    }
+}
+
+// Lets the k_cell_sort kernels of a class use their LDS (once per process).
 template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 inline void cell_sort_prepare() {
     static const bool done = [] {
         CDBG_HIP(hipFuncSetAttribute((const void*)&k_cell_sort<K, V, CAP, THREADS, PACK>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      int(cell_sort_lds<K, V, CAP, THREADS, PACK>())));
+        CDBG_HIP(hipFuncSetAttribute((const void*)&k_cell_sort_q<K, V, CAP, THREADS, PACK>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(cell_sort_lds<K, V, CAP, THREADS, PACK>())));
         return true;
     }();
     (void)done;
+}
+
+// Workgroups of a persistent class that fit the device at once.
+template <class K, class V, uint32_t CAP, int THREADS, bool PACK>
+inline uint32_t cell_sort_resident() {
+    static const uint32_t n = [] {
+        int dev = 0, per_cu = 0;
+        CDBG_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop{};
+        CDBG_HIP(hipGetDeviceProperties(&prop, dev));
+        CDBG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)&k_cell_sort_q<K, V, CAP, THREADS, PACK>,
+                                                              THREADS, cell_sort_lds<K, V, CAP, THREADS, PACK>()));
+        return uint32_t(std::max(1, per_cu) * std::max(1, prop.multiProcessorCount));
+    }();
+    return n;
 }
 
 // the two LDS classes: (capacity, threads) per key width
@@ -487,21 +553,47 @@ struct CellClasses<Key128> {
 
 // One size class of the cell sort over `grid` cells; packed items when the
 // key is 64-bit and its unsorted bits leave room for the position.
+// Sets up a class's kernels outside any graph capture.
 template <class K, uint32_t CAP, int THREADS>
-static void launch_cell_sort(uint32_t grid, hipStream_t st, K* keys, uint64_t* vals, const uint32_t* cell_start,
-                             const uint32_t* list, uint32_t n_items, uint32_t lo, int end_bit, uint32_t* next,
-                             unsigned long long* n_next, uint8_t* heads) {
+static void cell_sort_warm(int end_bit) {
     if constexpr (std::is_same<K, uint64_t>::value) {
         if (end_bit + PACK_IB <= 64) {
             cell_sort_prepare<K, uint64_t, CAP, THREADS, true>();
-            k_cell_sort<K, uint64_t, CAP, THREADS, true><<<grid, THREADS, cell_sort_lds<K, uint64_t, CAP, THREADS, true>(), st>>>(
-                keys, vals, cell_start, list, n_items, lo, end_bit, next, n_next, heads);
+            (void)cell_sort_resident<K, uint64_t, CAP, THREADS, true>();
             return;
         }
     }
-    cell_sort_prepare<K, uint64_t, CAP, THREADS>();
-    k_cell_sort<K, uint64_t, CAP, THREADS><<<grid, THREADS, cell_sort_lds<K, uint64_t, CAP, THREADS>(), st>>>(
-        keys, vals, cell_start, list, n_items, lo, end_bit, next, n_next, heads);
+    cell_sort_prepare<K, uint64_t, CAP, THREADS, false>();
+    (void)cell_sort_resident<K, uint64_t, CAP, THREADS, false>();
+}
+
+// With `queue` the class runs as persistent workgroups over a device cell
+// count (n32 / n64, else n_items), otherwise one workgroup per cell.
+template <class K, uint32_t CAP, int THREADS, bool PACK>
+static void launch_cell_class(hipStream_t st, K* keys, uint64_t* vals, const CellSortArgs& a, uint32_t n_items,
+                              const uint32_t* n32, const unsigned long long* n64, unsigned long long* queue) {
+    cell_sort_prepare<K, uint64_t, CAP, THREADS, PACK>();
+    const size_t lds = cell_sort_lds<K, uint64_t, CAP, THREADS, PACK>();
+    if (queue) {
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(cell_sort_resident<K, uint64_t, CAP, THREADS, PACK>(),
+                                                                       (n_items + CELL_Q - 1) / CELL_Q));
+        k_cell_sort_q<K, uint64_t, CAP, THREADS, PACK><<<grid, THREADS, lds, st>>>(keys, vals, a, n_items, n32, n64, queue);
+    } else if (n_items) {
+        k_cell_sort<K, uint64_t, CAP, THREADS, PACK><<<n_items, THREADS, lds, st>>>(keys, vals, a, n_items);
+    }
+}
+
+template <class K, uint32_t CAP, int THREADS>
+static void launch_cell_sort(hipStream_t st, K* keys, uint64_t* vals, const CellSortArgs& a, uint32_t n_items,
+                             const uint32_t* n32 = nullptr, const unsigned long long* n64 = nullptr,
+                             unsigned long long* queue = nullptr) {
+    if constexpr (std::is_same<K, uint64_t>::value) {
+        if (a.end_bit + PACK_IB <= 64) {
+            launch_cell_class<K, CAP, THREADS, true>(st, keys, vals, a, n_items, n32, n64, queue);
+            return;
+        }
+    }
+    launch_cell_class<K, CAP, THREADS, false>(st, keys, vals, a, n_items, n32, n64, queue);
 }
 
 // flag[c] = cell c holds more than cap keys
@@ -2182,7 +2274,26 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
+// An instantiated graph of one chunk's stage and the arguments it was
+// captured with.
+struct StageGraph {
+    std::vector<uint64_t> key;
+    hipGraphExec_t exec = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        key.clear();
+    }
+    ~StageGraph() { reset(); }
+};
+
 struct Phase2Bufs {
+    std::map<uint32_t, std::unique_ptr<StageGraph>> graphs;  // cell-sort stage per chunk (first bucket)
+    StageGraph& cell_graph(uint32_t b0) {
+        auto& g = graphs[b0];
+        if (!g) g.reset(new StageGraph());
+        return *g;
+    }
     // job-wide
     DevBuf kcnt, tup_off, btup, total, lbtup;
     DevBuf single_min, single_cid, mt_hk, mt_cid, mt_h2, mt_loff, mt_llen;
@@ -2344,14 +2455,9 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     // the cell count of small buckets; with 64-bit keys D_b >= 2k - 51 when
     // the largest bucket needs it, so the packed cell sort applies to all.
     const int kb = 2 * int(k);
-    // mean keys per cell (CDBG_CELL_TARGET overrides): 1000 for 64-bit keys,
-    // 700 for 128-bit ones (whose 1024-key LDS class the larger cells overflow)
-    static const double cell_target_env = [] {
-        const char* e = std::getenv("CDBG_CELL_TARGET");
-        const double v = e ? std::atof(e) : 0.0;
-        return v >= 16.0 ? v : 0.0;
-    }();
-    const double cell_target = cell_target_env > 0.0 ? cell_target_env : (sizeof(K) == 8 ? 1000.0 : 700.0);
+    // mean keys per cell (the job's tuning; by default 1000 for 64-bit keys,
+    // 700 for 128-bit ones, whose 1024-key LDS class larger cells overflow)
+    const double cell_target = J.tune.cell_target > 0.0 ? J.tune.cell_target : (sizeof(K) == 8 ? 1000.0 : 700.0);
     std::vector<uint8_t> h_dbits(nb);
     int D_max = 0;
     for (uint32_t b = 0; b < nb; ++b) {
@@ -2373,9 +2479,13 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     }
     SortPlan plan;
     plan.build(lb);
-    if (D_max > 0)
+    if (D_max > 0) {
+        // the fewest passes of at most digit_bits bits each, bits spread evenly
+        const int dmax = J.tune.digit_bits ? int(J.tune.digit_bits) : 8;
+        const int passes = (D_max + dmax - 1) / dmax;
         segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D_max, kb,
-                                          D_max <= 8 ? D_max : (D_max + 1) / 2, P.rs2, st, "tuples");
+                                          (D_max + passes - 1) / passes, P.rs2, st, "tuples");
+    }
     const int D = D_min;  // the cells sort bits [0, kb - D_min)
     uint8_t* heads = P.heads.ensure<uint8_t>(nt);  // 1 at the first tuple of every node
     {
@@ -2394,22 +2504,86 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         uint32_t* mid = P.cell_mid.ensure<uint32_t>(n_cells);
         uint32_t* mid2 = P.cell_mid2.ensure<uint32_t>(n_cells);
         uint32_t* over = P.cell_over.ensure<uint32_t>(n_cells);
-        auto* novf = P.cell_novf.ensure<unsigned long long>(3);
-        { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 3, 0ull); }
-        { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C0, 128>(n_cells, st, keys, vals, cell_start, nullptr, n_cells, 0, kb - D, nullptr, novf, heads); }
-        // cells above the tiny class, listed by a scan (many: no shared counter)
+        auto* novf = P.cell_novf.ensure<unsigned long long>(6);  // listed cells of classes 0-2, queue heads 0-2
         uint32_t* oflag = P.cell_oflag.ensure<uint32_t>(n_cells);
         uint32_t* oscan = P.cell_oscan.ensure<uint32_t>(uint64_t(n_cells) + 1);
-        { KSCOPE("k_cell_over", st); k_cell_over<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, cell_start, C0, oflag); }
-        exclusive_scan<uint32_t, uint32_t>(oflag, oscan, n_cells, oscan + n_cells, J.rs.scan_tmp, st);
-        const uint64_t n_mid = d2h<uint32_t>(oscan + n_cells, st);
-        if (n_mid) { KSCOPE("k_compact", st); k_compact<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, oflag, oscan, mid); }
-        if (n_mid) { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C1, 256>(uint32_t(n_mid), st, keys, vals, cell_start, mid, uint32_t(n_mid), C0, kb - D, mid2, novf + 1, heads); }
-        const uint64_t n_mid2 = n_mid ? d2h<unsigned long long>(novf + 1, st) : 0;
-        if (n_mid2 > n_mid) throw Error("phase 2: cell sort listed more cells than exist");
-        if (n_mid2) { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C2, 512>(uint32_t(n_mid2), st, keys, vals, cell_start, mid2, uint32_t(n_mid2), C1, kb - D, over, novf + 2, heads); }
-        const uint64_t n_over = n_mid2 ? d2h<unsigned long long>(novf + 2, st) : 0;
-        if (n_over > n_mid2) throw Error("phase 2: cell sort listed more cells than exist");
+        const CellSortArgs a0{cell_start, nullptr, 0, kb - D, nullptr, novf, heads};
+        const CellSortArgs a1{cell_start, mid, C0, kb - D, mid2, novf + 1, heads};
+        const CellSortArgs a2{cell_start, mid2, C1, kb - D, over, novf + 2, heads};
+        uint64_t n_over = 0;
+        if (!J.tune.persistent) {
+            { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 3, 0ull); }
+            { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C0, 128>(st, keys, vals, a0, n_cells); }
+            // cells above the tiny class, listed by a scan (many: no shared counter)
+            { KSCOPE("k_cell_over", st); k_cell_over<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, cell_start, C0, oflag); }
+            exclusive_scan<uint32_t, uint32_t>(oflag, oscan, n_cells, oscan + n_cells, J.rs.scan_tmp, st);
+            const uint64_t n_mid = d2h<uint32_t>(oscan + n_cells, st);
+            if (n_mid) { KSCOPE("k_compact", st); k_compact<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, oflag, oscan, mid); }
+            if (n_mid) { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C1, 256>(st, keys, vals, a1, uint32_t(n_mid)); }
+            const uint64_t n_mid2 = n_mid ? d2h<unsigned long long>(novf + 1, st) : 0;
+            if (n_mid2 > n_mid) throw Error("phase 2: cell sort listed more cells than exist");
+            if (n_mid2) { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C2, 512>(st, keys, vals, a2, uint32_t(n_mid2)); }
+            n_over = n_mid2 ? d2h<unsigned long long>(novf + 2, st) : 0;
+            if (n_over > n_mid2) throw Error("phase 2: cell sort listed more cells than exist");
+        } else {
+            // persistent classes over device cell counts: no read-back until
+            // the cells too large for LDS are counted; with graphs on, the
+            // stage is one graph launch, kept per chunk and replayed while
+            // its arguments are unchanged (a repeated run of the job)
+            J.rs.scan_tmp.ensure<uint32_t>((uint64_t(n_cells) + SCAN_CHUNK - 1) / SCAN_CHUNK + 2);
+            cell_sort_warm<K, C0, 128>(kb - D);
+            cell_sort_warm<K, C1, 256>(kb - D);
+            cell_sort_warm<K, C2, 512>(kb - D);
+            auto stage = [&]() {
+                { KSCOPE("k_fill", st); k_fill<unsigned long long><<<1, 64, 0, st>>>(novf, 6, 0ull); }
+                { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C0, 128>(st, keys, vals, a0, n_cells, nullptr, nullptr, novf + 3); }
+                { KSCOPE("k_cell_over", st); k_cell_over<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, cell_start, C0, oflag); }
+                exclusive_scan<uint32_t, uint32_t>(oflag, oscan, n_cells, oscan + n_cells, J.rs.scan_tmp, st);
+                { KSCOPE("k_compact", st); k_compact<<<blocks_for(n_cells), TPB, 0, st>>>(n_cells, oflag, oscan, mid); }
+                { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C1, 256>(st, keys, vals, a1, n_cells, oscan + n_cells, nullptr, novf + 4); }
+                { KSCOPE("k_cell_sort", st); launch_cell_sort<K, C2, 512>(st, keys, vals, a2, n_cells, nullptr, novf + 1, novf + 5); }
+            };
+            if (J.tune.graph) {
+                const std::vector<uint64_t> key{uint64_t(b0), n_cells, uint64_t(kb - D), uint64_t(uintptr_t(keys)),
+                                                uint64_t(uintptr_t(vals)), uint64_t(uintptr_t(cell_start)),
+                                                uint64_t(uintptr_t(mid)), uint64_t(uintptr_t(mid2)),
+                                                uint64_t(uintptr_t(over)), uint64_t(uintptr_t(novf)),
+                                                uint64_t(uintptr_t(heads)), uint64_t(uintptr_t(oflag)),
+                                                uint64_t(uintptr_t(oscan)), uint64_t(uintptr_t(J.rs.scan_tmp.p))};
+                StageGraph& g = P.cell_graph(b0);
+                if (!g.exec || g.key != key) {
+                    g.reset();
+                    Profiler* saved = g_prof;  // no per-kernel events inside a capture
+                    g_prof = nullptr;
+                    hipGraph_t graph = nullptr;
+                    CDBG_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                    try {
+                        stage();
+                    } catch (...) {
+                        (void)hipStreamEndCapture(st, &graph);
+                        if (graph) (void)hipGraphDestroy(graph);
+                        g_prof = saved;
+                        throw;
+                    }
+                    CDBG_HIP(hipStreamEndCapture(st, &graph));
+                    g_prof = saved;
+                    const hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    if (e != hipSuccess) {
+                        g.exec = nullptr;
+                        (void)hipGetLastError();
+                        throw Error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                    }
+                    g.key = key;
+                }
+                KSCOPE("k_cell_sort:graph", st);
+                CDBG_HIP(hipGraphLaunch(g.exec, st));
+            } else {
+                stage();
+            }
+            n_over = d2h<unsigned long long>(novf + 2, st);
+            if (n_over > n_cells) throw Error("phase 2: cell sort listed more cells than exist");
+        }
         if (n_over) {
             // cells too large for LDS: gathered, sorted by the global passes, put back
             std::vector<uint32_t> ids(n_over), cs(n_cells + 1);
@@ -2483,7 +2657,8 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
 
     // --- colour sets: intern into the job-wide tables
     uint64_t cap = P.mt_cap;
-    while (cap < 2 * (P.mt_count + n_multi)) cap <<= 1;
+    const double slack = J.tune.set_slack > 0.0 ? J.tune.set_slack : 2.0;  // slots per multi-colour set
+    while (double(cap) < slack * double(P.mt_count + n_multi)) cap <<= 1;
     if (cap != P.mt_cap) mt_resize(P, cap, st);
     SetTables T = set_tables(P);
     T.cur_colors = colors;
@@ -2765,6 +2940,7 @@ static void phase2_impl(Job& J) {
         }
         ++nchunks;
         b0 = b1;
+        if (J.p2_max_chunks && nchunks >= J.p2_max_chunks) break;  // a trial over the first chunks only
     }
     J.frags_in_host = P.spill;
     J.n_chunks = nchunks;

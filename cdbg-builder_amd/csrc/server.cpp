@@ -175,6 +175,7 @@ struct Server {
                     if (attempt > 0) run_alone(me);
                     slots[me].job = std::make_unique<Job>();
                     Job& J = *slots[me].job;
+                    J.tune = ctx->tune;
                     job_set_input(J, ctx, j->in, j->off.data(), j->nc, j->k, j->m, j->B);
                     J.chunk_tuples = j->chunk;
                     run_phase1(J);

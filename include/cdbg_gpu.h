@@ -194,6 +194,36 @@ typedef struct {
     uint64_t num_fragments;
 } cdbg_gpu_color_state;
 
+/* Performance knobs of phase 2 (results never depend on them).  Zero fields
+ * take the defaults.  cell_target: mean k-mer occurrences per LDS sort cell
+ * (default 1000 for k <= 31, 700 above; 16 .. 65536); digit_bits: largest
+ * digit of the global tuple radix passes (default 8; 1 .. 11); set_slack:
+ * colour-set table slots per multi-colour set (default 2; 1.25 .. 16);
+ * persistent: 1 = the cell sorts run as persistent workgroups that take
+ * cells from a device queue, with cell counts read on the device;
+ * graph: 1 (with persistent) = that stage is captured once per chunk in a
+ * hipGraph and replayed while its arguments are unchanged.  No counterpart
+ * in the reference (its knobs are compile-time constants and -t). */
+typedef struct {
+    double cell_target;
+    uint32_t digit_bits;
+    uint32_t persistent;
+    double set_slack;
+    uint32_t graph;
+    uint32_t reserved;
+} cdbg_gpu_tuning;
+
+/* What cdbg_gpu_job_autotune measured: the best tuning, the trials run and
+ * the phase-2 time of the sample with the defaults and with the best. */
+typedef struct {
+    cdbg_gpu_tuning best;
+    uint32_t trials;
+    uint32_t reserved;
+    uint64_t sample_occurrences;
+    double default_ms;
+    double best_ms;
+} cdbg_gpu_tune_report;
+
 /* Drop-in phase 1: reads the FASTA/FASTQ files (colour = index in `files`),
  * runs phase 1 on the device and writes <tmp_dir>/bucket_<b>.bin for every
  * bucket (bucket_io.hpp:19-40, records super_kmer.hpp:105-117).  As the
@@ -216,6 +246,18 @@ CDBG_API int cdbg_gpu_bucket_write(cdbg_gpu_ctx* ctx, const char* const* files, 
 CDBG_API int cdbg_gpu_bucket_process(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
                                      uint32_t num_colors, uint32_t threads, const char* frag_path,
                                      const char* color_sets_path, cdbg_gpu_color_state* state);
+
+/* Tuning of one job, and the default for jobs created later on a context
+ * (served jobs included).  Invalid values are an error. */
+CDBG_API int cdbg_gpu_job_set_tuning(cdbg_gpu_job* job, const cdbg_gpu_tuning* t);
+CDBG_API int cdbg_gpu_job_get_tuning(cdbg_gpu_job* job, cdbg_gpu_tuning* t);
+CDBG_API int cdbg_gpu_ctx_set_tuning(cdbg_gpu_ctx* ctx, const cdbg_gpu_tuning* t);
+/* Autotuner: after phase 1, times phase 2 over a sample of the job's first
+ * buckets (about sample_occurrences k-mer occurrences; 0 = 2^26) under
+ * candidate knobs, one knob at a time (cell size, digit width, persistent
+ * cells, graph capture, table slack), keeps the fastest and sets it on the
+ * job.  Phase 2 must then be run again (the trials leave no result). */
+CDBG_API int cdbg_gpu_job_autotune(cdbg_gpu_job* job, uint64_t sample_occurrences, cdbg_gpu_tune_report* out);
 
 /* Job-level pieces of the two calls above. */
 CDBG_API int cdbg_gpu_job_write_bucket_files(cdbg_gpu_job* job, const char* dir, uint32_t threads,
