@@ -30,10 +30,10 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     """Phase-2 performance knobs (cdbg_gpu_tuning); zero fields are defaults."""
     _fields_ = [("cell_target", C.c_double), ("digit_bits", C.c_uint32), ("persistent", C.c_uint32),
-                ("set_slack", C.c_double), ("graph", C.c_uint32), ("reserved", C.c_uint32)]
+                ("set_slack", C.c_double), ("graph", C.c_uint32), ("gzip_device", C.c_uint32)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class TuneReport(C.Structure):
@@ -156,6 +156,9 @@ _SIGS = {
     "cdbg_seqs_free": (None, [C.c_void_p]),
     "cdbg_lz4_compress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
     "cdbg_lz4_decompress": (C.c_int64, [U8P, C.c_uint64, U8P, C.c_uint64]),
+    "cdbg_gpu_job_create_from_files": (C.c_int, [C.c_void_p, C.POINTER(C.c_char_p), C.c_uint32, C.c_uint32,
+                                               C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "cdbg_gpu_job_input": (C.c_int64, [C.c_void_p, U8P, C.c_uint64, U64P]),
     "cdbg_gpu_job_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "cdbg_gpu_job_get_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
     "cdbg_gpu_ctx_set_tuning": (C.c_int, [C.c_void_p, C.POINTER(Tuning)]),
@@ -363,6 +366,16 @@ class Context:
     def job(self, data, color_off, k, m, num_buckets):
         return Job(self, data, color_off, k, m, num_buckets)
 
+    def job_from_files(self, files, k, m, num_buckets, threads=8):
+        """A job over FASTA/FASTQ files (plain or gzip), decoded on the device
+        when it can be (FASTQ goes to the host reader)."""
+        j = Job.__new__(Job)
+        j.num_buckets = num_buckets
+        j.h = C.c_void_p()
+        _check(lib().cdbg_gpu_job_create_from_files(self.h, _cstrs(files), len(files), k, m, num_buckets, threads,
+                                                    C.byref(j.h)))
+        return j
+
     def job_from_bucket_files(self, d, num_buckets, k, num_colors, threads=8):
         j = Job.__new__(Job)
         j.num_buckets = num_buckets
@@ -495,6 +508,25 @@ class Job:
             return
         c = np.ascontiguousarray(cuts, dtype=np.uint64)
         _check(lib().cdbg_gpu_job_set_bucket_map(self.h, _ptr(c, U64P), len(c)))
+
+    def input(self):
+        """(bytes, colour offsets, decoded on the device?) of the job's input."""
+        n = lib().cdbg_gpu_job_input(self.h, None, 0, None)
+        if n < 0:
+            raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+        on_dev = bool(n >> 62)
+        n &= (1 << 62) - 1
+        buf = np.zeros(max(n, 1), np.uint8)
+        nc = lib().cdbg_gpu_job_input(self.h, _ptr(buf, U8P), n, None)
+        if nc < 0:
+            raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+        return buf[:n], on_dev
+
+    def input_offsets(self, num_colors):
+        off = np.zeros(num_colors + 1, np.uint64)
+        if lib().cdbg_gpu_job_input(self.h, None, 0, _ptr(off, U64P)) < 0:
+            raise RuntimeError(lib().cdbg_gpu_last_error().decode())
+        return off
 
     def set_tuning(self, t=None, **kw):
         _check(lib().cdbg_gpu_job_set_tuning(self.h, C.byref(t if t is not None else tuning(**kw))))

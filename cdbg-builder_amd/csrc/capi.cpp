@@ -54,34 +54,41 @@ StreamOwner::~StreamOwner() {
 
 Job::~Job() { free_phase2(*this); }  // the stream goes last, after every buffer freed on it
 
-void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off, uint32_t num_colors,
-                   uint32_t k, uint32_t m, uint32_t num_buckets) {
-    if (!ctx || !color_off || (!bytes && num_colors)) throw Error("null argument");
+void job_reset(Job& J, cdbg_gpu_ctx* ctx, uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets) {
+    if (!ctx) throw Error("null argument");
     if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
     if (m < 2 || m >= k) throw Error("m must satisfy 2 <= m < k");
     if (num_buckets == 0) throw Error("num_buckets must be >= 1");
     if (num_colors == 0) throw Error("no colours");
-    for (uint32_t c = 0; c < num_colors; ++c)
-        if (color_off[c + 1] < color_off[c]) throw Error("colour offsets must be non-decreasing");
     J.ctx = ctx;
     J.k = k;
     J.m = m;
     J.B = num_buckets;
     J.num_colors = num_colors;
-    J.n_bytes = color_off[num_colors] - color_off[0];
-    // Pad with a non-ACGT byte so tiles and 32-byte groups need no tail case.
-    J.n_pad = ((J.n_bytes + 64) / 64) * 64 + 64;
     J.phase1_done = J.phase2_done = J.from_files = false;
     J.n_supers = 0;
     J.auto_chunk = 0;
     J.n_chunks = 0;
     J.h_bucket_off.clear();
     J.stats = cdbg_gpu_stats{};
+    J.p1_auto_bases = 0;
     if (!J.st) {
         J.st = acquire_stream(ctx->device);
         J.stream_owner.s = J.st;
         J.stream_owner.device = ctx->device;
     }
+}
+
+void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off, uint32_t num_colors,
+                   uint32_t k, uint32_t m, uint32_t num_buckets) {
+    if (!ctx || !color_off || (!bytes && num_colors)) throw Error("null argument");
+    if (num_colors == 0) throw Error("no colours");
+    for (uint32_t c = 0; c < num_colors; ++c)
+        if (color_off[c + 1] < color_off[c]) throw Error("colour offsets must be non-decreasing");
+    job_reset(J, ctx, num_colors, k, m, num_buckets);
+    J.n_bytes = color_off[num_colors] - color_off[0];
+    // Pad with a non-ACGT byte so tiles and 32-byte groups need no tail case.
+    J.n_pad = ((J.n_bytes + 64) / 64) * 64 + 64;
     AllocStream alloc_on_job_stream(J.st);
     uint8_t* d = J.bytes.ensure<uint8_t>(J.n_pad);
     CDBG_HIP(hipMemsetAsync(d + J.n_bytes, '\n', J.n_pad - J.n_bytes, J.st));
@@ -89,7 +96,6 @@ void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64
     std::vector<uint64_t>& off = J.h_color_off;
     off.resize(num_colors + 1);
     for (uint32_t c = 0; c <= num_colors; ++c) off[c] = color_off[c] - color_off[0];
-    J.p1_auto_bases = 0;
     uint64_t* doff = J.color_off.ensure<uint64_t>(num_colors + 1);
     CDBG_HIP(hipMemcpyAsync(doff, off.data(), off.size() * 8, hipMemcpyHostToDevice, J.st));
     CDBG_HIP(hipStreamSynchronize(J.st));  // the host arrays are pageable and the caller's
@@ -139,6 +145,57 @@ CDBG_API int cdbg_gpu_job_create(cdbg_gpu_ctx* ctx, const uint8_t* bytes, const 
         }
         *out = h;
     })
+}
+
+CDBG_API int cdbg_gpu_job_create_from_files(cdbg_gpu_ctx* ctx, const char* const* files, uint32_t n_files, uint32_t k,
+                                            uint32_t m, uint32_t num_buckets, uint32_t threads, cdbg_gpu_job** out) {
+    CDBG_TRY({
+        if (!ctx || !files || !out) throw Error("null argument");
+        CDBG_HIP(hipSetDevice(ctx->device));
+        std::vector<std::string> paths;
+        for (uint32_t i = 0; i < n_files; ++i) {
+            if (!files[i]) throw Error("null file name");
+            paths.emplace_back(files[i]);
+        }
+        auto* h = new cdbg_gpu_job();
+        h->J.tune = ctx->tune;
+        try {
+            Job& J = h->J;
+            job_reset(J, ctx, n_files, k, m, num_buckets);
+            J.input_on_device = load_inputs_device(J, paths, threads);
+            if (!J.input_on_device) {  // FASTQ quality lines: the host reader
+                void* seqs = cdbg_load_fasta(files, n_files, threads);
+                if (!seqs) throw Error(last_error());
+                struct Free {
+                    void* p;
+                    ~Free() { cdbg_seqs_free(p); }
+                } fr{seqs};
+                job_set_input(J, ctx, cdbg_seqs_bytes(seqs), cdbg_seqs_offsets(seqs), n_files, k, m, num_buckets);
+            }
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    })
+}
+
+CDBG_API int64_t cdbg_gpu_job_input(cdbg_gpu_job* job, uint8_t* bytes, uint64_t cap, uint64_t* color_off) {
+    try {
+        if (!job) throw Error("null argument");
+        Job& J = job->J;
+        if (bytes) {
+            if (cap < J.n_bytes) throw Error("input buffer too small");
+            CDBG_HIP(hipSetDevice(J.ctx->device));
+            CDBG_HIP(hipStreamSynchronize(J.st));
+            if (J.n_bytes) CDBG_HIP(hipMemcpy(bytes, J.bytes.p, J.n_bytes, hipMemcpyDeviceToHost));
+        }
+        if (color_off)
+            for (uint32_t c = 0; c <= J.num_colors; ++c) color_off[c] = J.h_color_off[c];
+        return int64_t(J.input_on_device ? J.n_bytes | (uint64_t(1) << 62) : J.n_bytes);
+    } catch (std::exception& e) {
+        return set_error(e.what());
+    }
 }
 
 CDBG_API void cdbg_gpu_job_destroy(cdbg_gpu_job* job) { delete job; }
@@ -358,11 +415,8 @@ CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag
 CDBG_API int cdbg_gpu_bucket_write(cdbg_gpu_ctx* ctx, const char* const* files, uint32_t n_files, uint32_t k,
                                    uint32_t m, uint32_t num_buckets, const char* tmp_dir, uint32_t threads,
                                    uint64_t* bucket_unc_bytes) {
-    void* seqs = cdbg_load_fasta(files, n_files, threads);
-    if (!seqs) return -1;
     cdbg_gpu_job* job = nullptr;
-    int rc = cdbg_gpu_job_create(ctx, cdbg_seqs_bytes(seqs), cdbg_seqs_offsets(seqs), n_files, k, m, num_buckets, &job);
-    cdbg_seqs_free(seqs);
+    int rc = cdbg_gpu_job_create_from_files(ctx, files, n_files, k, m, num_buckets, threads, &job);
     if (rc == 0) rc = cdbg_gpu_job_phase1(job);
     if (rc == 0) rc = cdbg_gpu_job_write_bucket_files(job, tmp_dir, threads, bucket_unc_bytes);
     if (job) cdbg_gpu_job_destroy(job);

@@ -185,12 +185,8 @@ int main(int argc, char** argv) {
         cdbg_gpu_color_state st{};
         cdbg_gpu_stats s{};
         if (c.resident) {
-            void* seqs = cdbg_load_fasta(fp.data(), nc, c.t);
-            if (!seqs) throw std::runtime_error(cdbg_gpu_last_error());
             cdbg_gpu_job* job = nullptr;
-            int rc = cdbg_gpu_job_create(ctx, cdbg_seqs_bytes(seqs), cdbg_seqs_offsets(seqs), nc, c.k, c.m, B, &job);
-            cdbg_seqs_free(seqs);
-            check(rc);
+            check(cdbg_gpu_job_create_from_files(ctx, fp.data(), nc, c.k, c.m, B, c.t, &job));
             check(cdbg_gpu_job_run(job));
             check(cdbg_gpu_job_write_phase2_files(job, frag.c_str(), cs.c_str(), &st));
             check(cdbg_gpu_job_stats(job, &s));

@@ -75,6 +75,7 @@ struct Job {
     bool frags_in_host = false;   // the last phase-2 run kept its fragment stream in host memory
     cdbg_gpu_tuning tune{};       // phase-2 performance knobs (zero fields: defaults)
     uint32_t p2_max_chunks = 0;   // stop phase 2 after this many chunks (0: all; autotuner trials)
+    bool input_on_device = false; // the input was decoded and parsed on the device
     ~Job();
 };
 
@@ -82,6 +83,11 @@ struct Job {
 // colour offsets on the job's stream, keeps the device buffers it already has.
 void job_set_input(Job& J, cdbg_gpu_ctx* ctx, const uint8_t* bytes, const uint64_t* color_off, uint32_t num_colors,
                    uint32_t k, uint32_t m, uint32_t num_buckets);
+// Validates the job parameters and clears its state (no input yet).
+void job_reset(Job& J, cdbg_gpu_ctx* ctx, uint32_t num_colors, uint32_t k, uint32_t m, uint32_t num_buckets);
+// inflate.hip: the files (colour = position) into J's input on the device;
+// false (J's input untouched) when the host reader must take them.
+bool load_inputs_device(Job& J, const std::vector<std::string>& files, uint32_t threads);
 void run_phase1(Job& J);
 void phase1_core(Job& J);  // run_phase1 without timing or allocation scope
 void run_phase2(Job& J);

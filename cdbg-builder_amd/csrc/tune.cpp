@@ -21,6 +21,7 @@ void check_tuning(const cdbg_gpu_tuning& t) {
     if (t.set_slack != 0.0 && !(t.set_slack >= 1.25 && t.set_slack <= 16.0))
         throw Error("tuning: set_slack must be 0 or within [1.25, 16]");
     if (t.persistent > 1 || t.graph > 1) throw Error("tuning: persistent and graph are 0 or 1");
+    if (t.gzip_device > 2) throw Error("tuning: gzip_device is 0, 1 or 2");
 }
 
 cdbg_gpu_tune_report autotune(Job& J, uint64_t sample) {
@@ -38,6 +39,7 @@ cdbg_gpu_tune_report autotune(Job& J, uint64_t sample) {
             J.phase2_done = false;  // the trials leave no result
         }
     } restore{J, J.chunk_tuples, J.p2_max_chunks, J.prof.on};
+    const uint32_t saved_gzip = J.tune.gzip_device;
     J.chunk_tuples = sample;
     J.p2_max_chunks = 1;
     J.prof.on = false;  // the trials stay out of the job's kernel times
@@ -54,6 +56,7 @@ cdbg_gpu_tune_report autotune(Job& J, uint64_t sample) {
         return best;
     };
     cdbg_gpu_tuning cur{};
+    cur.gzip_device = saved_gzip;  // not a phase-2 knob: kept
     J.tune = cur;
     run_phase2(J);  // warm: kernels loaded, buffers cached
     double best = time_of(cur);

@@ -202,15 +202,19 @@ typedef struct {
  * persistent: 1 = the cell sorts run as persistent workgroups that take
  * cells from a device queue, with cell counts read on the device;
  * graph: 1 (with persistent) = that stage is captured once per chunk in a
- * hipGraph and replayed while its arguments are unchanged.  No counterpart
- * in the reference (its knobs are compile-time constants and -t). */
+ * hipGraph and replayed while its arguments are unchanged; gzip_device:
+ * where cdbg_gpu_job_create_from_files inflates gzip files (0 = where a
+ * model of the measured rates says it is faster: one file per wavefront at
+ * ~6 MB/s, two per CU, against ~350 MB/s per host thread; 1 = the device;
+ * 2 = host threads).  No counterpart in the reference (its knobs are compile-time
+ * constants and -t). */
 typedef struct {
     double cell_target;
     uint32_t digit_bits;
     uint32_t persistent;
     double set_slack;
     uint32_t graph;
-    uint32_t reserved;
+    uint32_t gzip_device;
 } cdbg_gpu_tuning;
 
 /* What cdbg_gpu_job_autotune measured: the best tuning, the trials run and
@@ -258,6 +262,22 @@ CDBG_API int cdbg_gpu_ctx_set_tuning(cdbg_gpu_ctx* ctx, const cdbg_gpu_tuning* t
  * cells, graph capture, table slack), keeps the fastest and sets it on the
  * job.  Phase 2 must then be run again (the trials leave no result). */
 CDBG_API int cdbg_gpu_job_autotune(cdbg_gpu_job* job, uint64_t sample_occurrences, cdbg_gpu_tune_report* out);
+
+/* A job over input files (colour = position in `files`), decoded on the
+ * device: gzip members are inflated one wavefront per file and the FASTA
+ * records parsed there with the rules of kseq_read as seq_reader uses them
+ * (seq_reader.hpp:44-77, external/kseq.h:161-201), into the same input
+ * cdbg_gpu_job_create takes.  Inputs with FASTQ quality lines are read by the
+ * host reader (cdbg_load_fasta) instead.  Corrupt gzip data (a bad code,
+ * length, CRC-32 or size field) is an error.  Replaces the reading half of
+ * ingest_bucketed (bucket_ingester.hpp:129-177, seq_reader.hpp:44-77). */
+CDBG_API int cdbg_gpu_job_create_from_files(cdbg_gpu_ctx* ctx, const char* const* files, uint32_t n_files, uint32_t k,
+                                            uint32_t m, uint32_t num_buckets, uint32_t threads, cdbg_gpu_job** out);
+/* The job's input bytes (to `bytes`, cap bytes of room, may be NULL) and its
+ * num_colors + 1 colour offsets (may be NULL).  Returns the input size, with
+ * bit 62 set when the input was decoded on the device; negative on error.
+ * No counterpart in the reference (an inspection aid). */
+CDBG_API int64_t cdbg_gpu_job_input(cdbg_gpu_job* job, uint8_t* bytes, uint64_t cap, uint64_t* color_off);
 
 /* Job-level pieces of the two calls above. */
 CDBG_API int cdbg_gpu_job_write_bucket_files(cdbg_gpu_job* job, const char* dir, uint32_t threads,
