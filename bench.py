@@ -605,7 +605,10 @@ def main():
     roof["copy_probe_gbps"] = round(copy_gbps, 1)
 
     def leg(fn, *a):
-        """A secondary leg; its failure is reported in its place, not raised."""
+        """A secondary leg, started on an empty block cache (what earlier legs
+        cached would otherwise decide its chunking and allocations); its
+        failure is reported in its place, not raised."""
+        G.lib().cdbg_gpu_trim_cache()
         try:
             return fn(*a)
         except Exception as e:  # noqa: BLE001
@@ -616,6 +619,7 @@ def main():
     if not args.no_bucket_files:
         bucket_files = leg(bucket_io_leg, G, ctx, job, s, args.k, args.buckets, args.colors, args.cpu_cores)
     job.close()
+    G.lib().cdbg_gpu_trim_cache()  # the profiler's child processes need the HBM this process cached
     if args.bucket_files_only:
         print(json.dumps({"bucket_files": bucket_files}))
         ctx.close()

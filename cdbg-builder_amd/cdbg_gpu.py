@@ -108,6 +108,7 @@ _SIGS = {
     "cdbg_gpu_job_kernel_times": (C.c_int64, [C.c_void_p, C.c_char_p, C.c_uint64]),
     "cdbg_gpu_device_bytes": (C.c_uint64, []),
     "cdbg_gpu_alloc_stats": (None, [U64P]),
+    "cdbg_gpu_trim_cache": (None, []),
     "cdbg_gpu_inject_oom": (None, [C.c_int64]),
     "cdbg_gpu_copy_probe": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(C.c_double)]),
     "cdbg_gpu_job_set_chunk_tuples": (C.c_int, [C.c_void_p, C.c_uint64]),

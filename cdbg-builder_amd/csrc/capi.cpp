@@ -445,6 +445,8 @@ CDBG_API void cdbg_gpu_alloc_stats(uint64_t out[8]) {
     for (int i = 0; i < 8; ++i) out[i] = v[i];
 }
 
+CDBG_API void cdbg_gpu_trim_cache(void) { g_block_cache.trim(); }
+
 CDBG_API int cdbg_gpu_copy_probe(cdbg_gpu_ctx* ctx, uint64_t bytes, int iters, double* gbps) {
     CDBG_TRY({
         if (!ctx || !gbps || iters < 1) throw Error("bad argument");

@@ -113,6 +113,10 @@ CDBG_API uint64_t cdbg_gpu_device_bytes(void);
  * bytes trimmed, [7] nanoseconds in trims.  No counterpart in the reference
  * (host RAM there is plain malloc). */
 CDBG_API void cdbg_gpu_alloc_stats(uint64_t out[8]);
+/* Hands every cached block back to the driver (waits for the device).  For
+ * a process that is done with its jobs, or that starts child processes which
+ * need the HBM.  No counterpart in the reference. */
+CDBG_API void cdbg_gpu_trim_cache(void);
 /* Testing aid: the device allocation after `n` more from the block cache
  * fails as out of memory (n < 0: none).  No counterpart in the reference. */
 CDBG_API void cdbg_gpu_inject_oom(int64_t n);
