@@ -1601,7 +1601,13 @@ struct WalkArgs {
     uint32_t* pcid;                // the node's colour-set id
     const void* node_rec;          // NodeRec<K> per node
     bool mark_visited;             // nodes outside simple paths exist: keep `visited` exact
+    uint32_t walk_cap;             // links an end follows before its path is left to list ranking
+    uint8_t* long_bucket;          // per chunk bucket: 1 when an end of it stopped at walk_cap
 };
+
+// comp_ex bits beside the exit port
+constexpr uint8_t COMP_RANKED = 2;  // long path: positions come from list ranking
+constexpr uint8_t COMP_LONG = 4;    // the end stopped at walk_cap
 
 
 constexpr uint32_t PATH_BUF = 8;
@@ -1756,14 +1762,16 @@ __device__ void simulate_component(const WalkArgs& a, uint32_t x, uint32_t ex0, 
 // Path ends walk to the other end; the end with the smaller index owns the
 // path.  Paths that never pass a node twice are queued for recording (with
 // their nodes when they have at most PATH_BUF), the others (a node linked to
-// itself turns the walk back) for the general seed loop.  One thread per end
-// of the dense end list; XCD x takes the x-th eighth of it, which keeps a
-// bucket's links in one L2.
+// itself turns the walk back) for the general seed loop.  An end that has
+// followed walk_cap links stops: its path is long and is ranked by pointer
+// jumping instead (k_long_*), so no lane chases a long path.  One thread
+// per end of the dense end list; XCD x takes the x-th eighth of it, which
+// keeps a bucket's links in one L2.
 __global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a, const uint32_t* __restrict__ ends, uint32_t n_ends) {
     const uint32_t nblk = (n_ends + TPB - 1) / TPB;
     const uint32_t blk = tile_of(blockIdx.x, nblk);
     const uint64_t e = uint64_t(blk) * TPB + threadIdx.x;
-    bool owner = false, simple = true;
+    bool owner = false, simple = true, lng = false;
     uint32_t x = 0, ex0 = 0, len = 0;
     uint32_t buf[PATH_BUF];
     if (blk < nblk && e < n_ends) {
@@ -1774,9 +1782,14 @@ __global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a, const uint32_t* _
         buf[0] = cur | (ex0 == PORT_L ? 0x80000000u : 0u);
         len = 1;
         const uint32_t limit = 2 * a.n_nodes + 2;
-        for (uint32_t h = 0; h < limit; ++h) {
+        const uint32_t cap = a.walk_cap < limit ? a.walk_cap : limit;
+        bool reached = false;
+        for (uint32_t h = 0; h < cap; ++h) {
             const uint32_t l = a.link[2 * cur + ex];
-            if (l == NONE32) break;
+            if (l == NONE32) {
+                reached = true;
+                break;
+            }
             const uint32_t y = l >> 1, q = l & 1u;
             if (y == cur) simple = false;  // hairpin: the walk comes back
             cur = y;
@@ -1786,13 +1799,16 @@ __global__ __launch_bounds__(TPB) void k_walk_ends(WalkArgs a, const uint32_t* _
                 if (j == len) buf[j] = cur | (q == PORT_R ? 0x80000000u : 0u);
             ++len;
         }
-        owner = cur >= x;
+        lng = !reached;
+        owner = reached && cur >= x;
     }
     const uint64_t o = wave_alloc(&a.counters[10], owner && !simple);
+    (void)wave_alloc(&a.counters[11], lng);
     if (blk < nblk && e < n_ends) {
         // a path is stored at the index of its owning end (length 0: none)
         a.comp_len[e] = owner && simple ? len : 0u;
-        a.comp_ex[e] = uint8_t(ex0);
+        a.comp_ex[e] = uint8_t(ex0 | (lng ? COMP_LONG : 0u));
+        if (lng) a.long_bucket[a.node_bucket[x]] = 1;
         if (owner && simple && len <= PATH_BUF)
 #pragma unroll
             for (uint32_t j = 0; j < PATH_BUF; ++j)
@@ -1859,13 +1875,18 @@ __global__ __launch_bounds__(TPB) void k_walk_record(WalkArgs a, uint64_t n_comp
         }
         if (a.mark_visited) a.visited[node] = 1;
     };
-    if (L > 0 && L <= PATH_BUF) {
+    // A ranked path's positions are written by k_long_place, which runs
+    // after this kernel (a staged wave also stores the slots of such a
+    // path).  k_walk_ends buffers the nodes of the short paths it walked to
+    // the end only, so a ranked path of at most PATH_BUF nodes has none.
+    const uint32_t cex = L > 0 ? a.comp_ex[c] : 0u;
+    if (L > 0 && L <= PATH_BUF && !(cex & COMP_RANKED)) {
         for (uint32_t i = 0; i < L; ++i) {
             const uint32_t v = a.comp_nodes[c * PATH_BUF + i], node = v & 0x7FFFFFFFu;
             put(i, rec[node], node, uint8_t(v >> 31));
         }
-    } else if (L > 0) {
-        uint32_t cur = a.comp_x[c], ex = a.comp_ex[c];
+    } else if (L > PATH_BUF && !(cex & COMP_RANKED)) {
+        uint32_t cur = a.comp_x[c], ex = cex & 1u;
         uint8_t o = ex == PORT_L ? 1 : 0;
         for (uint32_t i = 0;; ++i) {
             const NodeRec<K> r = rec[cur];
@@ -1932,7 +1953,7 @@ __global__ __launch_bounds__(TPB) void k_walk_frags(WalkArgs a, uint64_t n_comp)
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
-    if (L == 0) return;
+    if (L == 0 || (a.comp_ex[c] & COMP_RANKED)) return;  // ranked paths: k_walk_frags_long
     uint8_t* rc = staged ? &s_rc[wave][off - base] : a.prc + off;
     const uint16_t* info = staged ? &s_info[wave][off - base] : a.pinfo + off;
     const uint32_t* rk = staged ? &s_rk[wave][off - base] : a.prank + off;
@@ -2041,6 +2062,238 @@ __global__ void k_walk_cycles(WalkArgs a) {
         }
     }
     simulate_component<K>(a, uint32_t(x), PORT_R, true, simple ? len : 0, seed);
+}
+
+// ---------------------------------------------------------------------------
+// List ranking of long paths (emit_from_seed's walk, bucket_walker.hpp:451-566,
+// for paths longer than walk_cap links).
+//
+// The nodes U of the buckets that hold a long path get two half-edge states
+// each, state 2u + p = "leaving U[u] by port p".  A linked port's successor
+// is the neighbour's state leaving by its other port (distance 1); a port
+// without a link is terminal.  Links are symmetric, so every state has at
+// most one predecessor: the states form simple chains, one per path and
+// direction, and rings (cycles, handled by the cycle loop).  Pointer jumping
+// (next = next[next], dist += dist[next]) gives every chain state its
+// terminal and its distance in ceil(log2(2|U|)) rounds, so the owning end
+// learns its path's length and every node its position and orientation on
+// its path, each in one parallel pass.  A hairpin link (a node linked to
+// itself) is or-ed along the chain: such a path goes to the general loop.
+// ---------------------------------------------------------------------------
+constexpr uint32_t ST_TERM = 0x80000000u, ST_HAIR = 0x40000000u, ST_DIST = 0x3FFFFFFFu;
+
+__global__ void k_long_nodes(uint32_t n_nodes, const uint32_t* __restrict__ node_bucket,
+                             const uint8_t* __restrict__ long_bucket, uint8_t* __restrict__ nflag) {
+    const uint64_t n = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (n < n_nodes) nflag[n] = long_bucket[node_bucket[n]];
+}
+
+__global__ void k_long_index(uint32_t nu, const uint32_t* __restrict__ U, uint32_t* __restrict__ uidx,
+                             uint32_t* __restrict__ own_e) {
+    const uint64_t u = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    uidx[U[u]] = uint32_t(u);
+    own_e[u] = NONE32;
+}
+
+__global__ void k_long_states(uint32_t nu, const uint32_t* __restrict__ U, const uint32_t* __restrict__ uidx,
+                              const uint32_t* __restrict__ link, uint2* __restrict__ st,
+                              unsigned long long* __restrict__ err) {
+    const uint64_t u = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const uint32_t v = U[u];
+#pragma unroll
+    for (uint32_t p = 0; p < 2; ++p) {
+        const uint32_t l = link[2 * v + p], s = uint32_t(2 * u + p);
+        const uint32_t y = l >> 1, q = l & 1u;
+        // a neighbour lies in the node's bucket, so it is in U
+        const uint32_t uy = l == NONE32 ? NONE32 : uidx[y];
+        const bool in_u = uy < nu && U[uy] == y;
+        if (l != NONE32 && !in_u) atomicAdd(err, 1ull);
+        if (l == NONE32 || !in_u)
+            st[s] = make_uint2(s, ST_TERM);
+        else
+            st[s] = make_uint2(2 * uy + (1u - q), 1u | (y == v ? ST_HAIR : 0u));
+    }
+}
+
+__global__ void k_long_jump(uint64_t ns, const uint2* __restrict__ A, uint2* __restrict__ Bn) {
+    const uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= ns) return;
+    const uint2 a = A[s];
+    if (a.y & ST_TERM) {
+        Bn[s] = a;
+        return;
+    }
+    const uint2 b = A[a.x];
+    uint32_t d = (a.y & ST_DIST) + (b.y & ST_DIST);
+    if (d > ST_DIST) d = ST_DIST;  // rings only: their distances are not used
+    Bn[s] = make_uint2(b.x, d | ((a.y | b.y) & ST_HAIR));
+}
+
+// Long ends: the owning end (the smaller index, as in k_walk_ends) takes the
+// path's length; hairpin paths are queued for the general loop.  Every lane
+// of a wave calls wave_alloc.
+__global__ __launch_bounds__(TPB) void k_long_ends(WalkArgs a, const uint32_t* __restrict__ ends, uint32_t n_ends,
+                                                   uint32_t nu, const uint32_t* __restrict__ U,
+                                                   const uint32_t* __restrict__ uidx,
+                                                   const uint2* __restrict__ st, uint32_t* __restrict__ own_e,
+                                                   uint32_t* __restrict__ long_list) {
+    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool ranked = false, other = false;
+    uint32_t x = 0, ex = 0, u = 0;
+    if (e < n_ends) {
+        ex = a.comp_ex[e];
+        if (ex & COMP_LONG) {
+            x = ends[e];
+            u = uidx[x];  // the end's bucket is flagged, so the end is in U
+            const bool in_u = u < nu && U[u] == x;
+            const uint2 s = in_u ? st[2 * u + (ex & 1u)] : make_uint2(0u, 0u);
+            if (!in_u || !(st[s.x].y & ST_TERM)) {
+                atomicAdd(&a.counters[5], 1ull);  // a path end cannot lie on a ring
+            } else if (U[s.x >> 1] >= x) {
+                if (s.y & ST_HAIR) {
+                    other = true;
+                } else {
+                    ranked = true;
+                    a.comp_len[e] = (s.y & ST_DIST) + 1;
+                    own_e[u] = uint32_t(e);
+                }
+            }
+            a.comp_ex[e] = uint8_t((ex & 1u) | (ranked ? COMP_RANKED : 0u));
+        }
+    }
+    const uint64_t o = wave_alloc(&a.counters[10], other);
+    const uint64_t r = wave_alloc(&a.counters[14], ranked);
+    if (other) {
+        a.other_x[o] = x;
+        a.other_ex[o] = uint8_t(ex & 1u);
+    }
+    if (ranked) long_list[r] = uint32_t(e);
+}
+
+// Every node of a ranked path writes its own position: its distance from
+// the owning end, entered by L (orientation 0) when the owner lies behind
+// its L port, by R (orientation 1) otherwise.
+template <class K>
+__global__ void k_long_place(WalkArgs a, uint32_t nu, const uint32_t* __restrict__ U, const uint2* __restrict__ st,
+                             const uint32_t* __restrict__ own_e) {
+    const uint64_t u = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const uint2 sl = st[2 * u + PORT_L], sr = st[2 * u + PORT_R];
+    if (!(st[sl.x].y & ST_TERM) || !(st[sr.x].y & ST_TERM)) return;  // ring
+    const uint32_t zl = U[sl.x >> 1], zr = U[sr.x >> 1];
+    if (zl == zr) return;  // one-node path, or a path turned back by a hairpin
+    const bool via_l = zl < zr;
+    const uint32_t e = own_e[via_l ? (sl.x >> 1) : (sr.x >> 1)];
+    if (e == NONE32) return;  // a short path of the bucket: already recorded
+    const uint32_t i = via_l ? (sl.y & ST_DIST) : (sr.y & ST_DIST);
+    const uint32_t v = U[u];
+    const uint64_t pos = uint64_t(a.comp_off[e]) + i;
+    if (i >= a.comp_len[e] || pos >= a.n_nodes) {
+        atomicAdd(&a.counters[5], 1ull);
+        return;
+    }
+    put_position<K>(a, pos, v, via_l ? 0 : 1);
+    if (a.mark_visited) a.visited[v] = 1;
+}
+
+// The seed loop of k_walk_frags on a ranked path, one wave per path: the
+// smallest unplaced rank by a wave reduction, each side's growth 64
+// positions per step (the first placed position or path end stops before,
+// the first contig break stops after).  Placement marks live in prc; the
+// fences make one lane's marks visible to the others before they are read.
+__global__ __launch_bounds__(TPB) void k_walk_frags_long(WalkArgs a, const uint32_t* __restrict__ list,
+                                                         uint32_t n_list) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t w = uint64_t(blockIdx.x) * (TPB / 64) + (threadIdx.x >> 6);
+    if (w >= n_list) return;  // the whole wave
+    const uint32_t c = list[w];
+    const uint32_t L = a.comp_len[c];
+    const uint64_t off = a.comp_off[c];
+    uint8_t* rc = a.prc + off;
+    const uint16_t* info = a.pinfo + off;
+    const uint32_t* rk = a.prank + off;
+    const uint32_t b = a.b0 + a.node_bucket[a.comp_x[c]];
+    uint32_t placed = 0;
+    while (placed < L) {
+        uint64_t best = ~0ull;  // rank << 32 | position: the smallest rank, then the first position
+        for (uint32_t i = uint32_t(lane); i < L; i += 64)
+            if (!(rc[i] & 2u)) {
+                const uint64_t key = (uint64_t(rk[i]) << 32) | i;
+                best = key < best ? key : best;
+            }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const uint64_t y = __shfl_xor(best, o, 64);
+            best = y < best ? y : best;
+        }
+        if (best == ~0ull) {  // cannot happen while positions are unplaced
+            if (lane == 0) atomicAdd(&a.counters[5], 1ull);
+            return;
+        }
+        const int64_t p = int64_t(uint32_t(best));
+        const uint32_t o = rc[p] & 1u;
+        const int64_t fwd = o ? -1 : 1;
+        const uint32_t sflags = info[p] & 3u, ms = info[p] >> 8;
+        int64_t lo = p, hi = p;
+        bool open_l = false, open_r = false;
+        for (int side = 0; side < 2; ++side) {
+            const int64_t d = side == 0 ? -fwd : fwd;
+            bool open = false;
+            if (sflags == (side == 0 ? BOUND_LEFT : BOUND_RIGHT)) {
+                open = __popc(side == 0 ? (ms >> 4) : (ms & 0xfu)) <= 1;
+            } else {
+            for (int64_t step = 1;; step += 64) {
+                const int64_t i = p + d * (step + lane);
+                const bool inb = i >= 0 && i < int64_t(L);
+                const uint32_t r = inb ? rc[i] : 2u;
+                const bool before = !inb || (r & 2u);
+                uint32_t mi = 0;
+                if (!before) mi = info[i];
+                const bool brk = !before && is_break(uint8_t(mi));
+                const uint64_t mb = __ballot(before), mk = __ballot(brk);
+                const int jb = mb ? __ffsll((unsigned long long)mb) - 1 : 64;
+                const int jk = mk ? __ffsll((unsigned long long)mk) - 1 : 64;
+                const int last = jk < jb ? jk : jb - 1;  // last lane whose position joins the fragment
+                if (lane <= last) rc[i] = uint8_t(r | 2u);
+                if (last >= 0) {
+                    const int64_t idx = p + d * (step + last);
+                    if (d < 0)
+                        lo = idx;
+                    else
+                        hi = idx;
+                }
+                if (jk < jb) {
+                    uint32_t ov = 0;
+                    if (lane == jk) {
+                        const uint32_t ex = (d > 0) == ((r & 1u) == 0) ? PORT_R : PORT_L;
+                        ov = __popc(nib(mi >> 8, ex)) <= 1 ? 1u : 0u;
+                    }
+                    open = __shfl(ov, jk, 64) != 0;
+                    break;
+                }
+                if (jb < 64) break;
+            }
+            }
+            if (side == 0)
+                open_l = open;
+            else
+                open_r = open;
+        }
+        if (lane == 0) {
+            rc[p] = uint8_t(rc[p] | 2u);
+            const uint64_t f = off + uint64_t(lo);
+            a.fflag[f] = 1;
+            a.frag_n[f] = uint32_t(hi - lo + 1);
+            a.frag_dir[f] = uint8_t(o);
+            a.frag_open[f] = uint8_t((open_l ? 1u : 0u) | (open_r ? 2u : 0u));
+            a.frag_key[f] = (uint64_t(b) << 32) | uint32_t(best >> 32);
+        }
+        placed += uint32_t(hi - lo + 1);
+        __threadfence();
+        __builtin_amdgcn_wave_barrier();
+    }
 }
 
 // Sort keys of the fragments: forder[d] = the d-th fragment's first position
@@ -2309,6 +2562,7 @@ struct Phase2Bufs {
     DevBuf frag_n, frag_open, frag_key, frag_dir, fflag, fcnt, fkey;
     DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
     DevBuf node_rec, eflag, ecnt, ends;
+    DevBuf lr_bucket, lr_u, lr_uidx, lr_own, lr_st0, lr_st1, lr_list, lr_cnt;  // list ranking of long paths
     DevBuf fkey_t, forder, forder_t, fdval, fpsize, fpboff, fsize, fboff;
     RadixScratch rs2;
     // bytes of the per-chunk buffers (what a new chunk reuses)
@@ -2322,7 +2576,8 @@ struct Phase2Bufs {
                                 &node_mask, &nb, &link, &visited, &frag_n, &frag_open, &frag_key, &frag_dir, &fflag,
                                 &fkey, &comp_ex, &comp_len, &comp_nodes, &comp_off, &other_x, &other_ex, &prc, &pinfo,
                                 &prank, &pkey, &pcid, &node_rec, &eflag, &ends, &fkey_t, &forder, &forder_t, &fdval,
-                                &fpsize, &fpboff, &fsize, &fboff})
+                                &fpsize, &fpboff, &fsize, &fboff, &lr_bucket, &lr_u, &lr_uidx, &lr_own, &lr_st0,
+                                &lr_st1, &lr_list})
             b += x->cap;
         return b;
     }
@@ -2742,6 +2997,11 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     wa.pkey = pkey;
     wa.pcid = pcid;
     wa.node_rec = node_rec;
+    const char* wcap = getenv("CDBG_WALK_CAP");  // testing: rank shorter paths by pointer jumping too
+    wa.walk_cap = wcap && *wcap ? uint32_t(std::max(1l, atol(wcap))) : 1024u;
+    uint8_t* long_bucket = P.lr_bucket.ensure<uint8_t>(nb);
+    { KSCOPE("k_fill", st); k_fill<uint8_t><<<blocks_for(nb), TPB, 0, st>>>(long_bucket, nb, uint8_t(0)); }
+    wa.long_bucket = long_bucket;
     uint32_t* ends = P.ends.ensure<uint32_t>(nn);
     uint32_t* d_count = (uint32_t*)(P.total.ensure<uint64_t>(8) + 2);
     compact_flags<uint8_t>(eflag, nn, ends, d_count, P.ecnt, J.rs.scan_tmp, st);
@@ -2755,16 +3015,51 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     if (n_ends) { KSCOPE("k_walk_ends", st); k_walk_ends<<<xcd_grid(blocks_for(n_ends)), TPB, 0, st>>>(wa, ends, n_ends); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));
-    const uint64_t n_comp = n_ends, n_other = h_cnt[10];
+    const uint64_t n_comp = n_ends;
+    uint64_t n_other = h_cnt[10], n_ranked = 0;
+    uint32_t nu = 0;
+    uint32_t *lr_u = nullptr, *lr_own = nullptr, *long_list = nullptr;
+    uint2* lr_st = nullptr;
+    if (h_cnt[11]) {  // ends stopped at walk_cap: list ranking over their buckets' nodes
+        uint8_t* nflag = P.big_flag.ensure<uint8_t>(nn);  // free after the node reduction
+        { KSCOPE("k_long_nodes", st); k_long_nodes<<<blocks_for(nn), TPB, 0, st>>>(nn, node_bucket, long_bucket, nflag); }
+        lr_u = P.lr_u.ensure<uint32_t>(nn);
+        uint32_t* d_nu = (uint32_t*)(dtot + 4);
+        compact_flags<uint8_t>(nflag, nn, lr_u, d_nu, P.lr_cnt, J.rs.scan_tmp, st);
+        nu = d2h<uint32_t>(d_nu, st);
+        if (2ull * nu >= ST_DIST) throw Error("phase 2: too many nodes in buckets with long paths for list ranking");
+        uint32_t* uidx = P.lr_uidx.ensure<uint32_t>(nn);
+        lr_own = P.lr_own.ensure<uint32_t>(nu);
+        { KSCOPE("k_long_index", st); k_long_index<<<blocks_for(nu), TPB, 0, st>>>(nu, lr_u, uidx, lr_own); }
+        uint2* sa = P.lr_st0.ensure<uint2>(2ull * nu);
+        uint2* sb = P.lr_st1.ensure<uint2>(2ull * nu);
+        { KSCOPE("k_long_states", st); k_long_states<<<blocks_for(nu), TPB, 0, st>>>(nu, lr_u, uidx, link, sa, cnt + 5); }
+        const int rounds = bits_for(2ull * nu) + 1;  // 2^rounds > the longest chain of states
+        for (int r = 0; r < rounds; ++r) {
+            { KSCOPE("k_long_jump", st); k_long_jump<<<blocks_for(2ull * nu), TPB, 0, st>>>(2ull * nu, sa, sb); }
+            std::swap(sa, sb);
+        }
+        lr_st = sa;
+        long_list = P.lr_list.ensure<uint32_t>(h_cnt[11]);
+        { KSCOPE("k_long_ends", st); k_long_ends<<<blocks_for(n_ends), TPB, 0, st>>>(wa, ends, n_ends, nu, lr_u, uidx, lr_st, lr_own, long_list); }
+        CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+        if (h_cnt[5]) throw Error("phase 2: list ranking left a path end unresolved");
+        n_other = h_cnt[10];
+        n_ranked = h_cnt[14];
+    }
     exclusive_scan<uint32_t, uint32_t>(wa.comp_len, comp_off, n_comp, comp_off + n_comp, J.rs.scan_tmp, st);
     const uint32_t path_nodes = n_comp ? d2h<uint32_t>(comp_off + n_comp, st) : 0u;
+    if (path_nodes > nn) throw Error("phase 2: paths hold more positions than the chunk has nodes");
     // nodes outside simple paths (hairpin paths, cycles) need the visited marks
     wa.mark_visited = path_nodes < nn;
     if (wa.mark_visited) { KSCOPE("k_fill", st); k_fill<uint8_t><<<blocks_for(nn), TPB, 0, st>>>(visited, nn, uint8_t(0)); }
     // the general seed loop places its nodes after the recorded paths
     { KSCOPE("k_set_u64", st); k_set_u64<<<1, 64, 0, st>>>(cnt + 3, path_nodes); }
     if (n_comp) { KSCOPE("k_walk_record", st); k_walk_record<K><<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
+    if (n_ranked) { KSCOPE("k_long_place", st); k_long_place<K><<<blocks_for(nu), TPB, 0, st>>>(wa, nu, lr_u, lr_st, lr_own); }
     if (n_comp) { KSCOPE("k_walk_frags", st); k_walk_frags<<<blocks_for(n_comp), TPB, 0, st>>>(wa, n_comp); }
+    if (n_ranked) { KSCOPE("k_walk_frags_long", st); k_walk_frags_long<<<blocks_for(n_ranked, TPB / 64), TPB, 0, st>>>(wa, long_list, uint32_t(n_ranked)); }
     if (n_other) { KSCOPE("k_walk_other", st); k_walk_other<K><<<blocks_for(n_other), TPB, 0, st>>>(wa, n_other); }
     if (wa.mark_visited) { KSCOPE("k_walk_cycles", st); k_walk_cycles<K><<<blocks_for(nn), TPB, 0, st>>>(wa); }
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
