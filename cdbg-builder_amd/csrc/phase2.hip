@@ -2427,17 +2427,189 @@ constexpr uint32_t fw_pos() { return sizeof(K) == 8 ? 1024 : 512; }
 // Fragments in position order (d = the d-th fragment start, ascending): the
 // record size of each (read from consecutive positions) and d itself, the
 // value the output sort carries.
+// Fragments of more than LONG_FRAG k-mers (a unitig that spans a bucket's
+// whole graph, with few buckets) are listed instead and sized and written by
+// a block each (k_frag_psize_long, k_frag_write_long).
+constexpr uint32_t LONG_FRAG = 4096;
+
 __global__ void k_frag_psize(uint64_t nf, const uint32_t* __restrict__ fpos, const uint32_t* __restrict__ frag_n,
                              const uint32_t* __restrict__ pcid, uint32_t k, uint32_t* __restrict__ psize,
-                             uint32_t* __restrict__ dval) {
+                             uint32_t* __restrict__ dval, uint32_t* __restrict__ long_list,
+                             unsigned long long* __restrict__ n_long) {
     const uint64_t d = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (d >= nf) return;
-    const uint64_t o = fpos[d];
-    const uint32_t n = frag_n[o];
+    const bool live = d < nf;
+    const uint64_t o = live ? fpos[d] : 0ull;
+    const uint32_t n = live ? frag_n[o] : 0u;
+    const bool lng = n > LONG_FRAG;
+    const uint64_t li = wave_alloc(n_long, lng);  // every lane of the wave calls it
+    if (!live) return;
+    dval[d] = uint32_t(d);
+    if (lng) {
+        long_list[li] = uint32_t(d);
+        return;
+    }
     uint32_t runs = 1;  // run count is the same read either way
     for (uint32_t j = 1; j < n; ++j) runs += pcid[o + j] != pcid[o + j - 1];
     psize[d] = 1 + 4 + 12 * runs + 4 + (k + n - 1);
-    dval[d] = uint32_t(d);
+}
+
+__global__ __launch_bounds__(TPB) void k_frag_psize_long(const uint32_t* __restrict__ long_list,
+                                                         const unsigned long long* __restrict__ n_long,
+                                                         const uint32_t* __restrict__ fpos,
+                                                         const uint32_t* __restrict__ frag_n,
+                                                         const uint32_t* __restrict__ pcid, uint32_t k,
+                                                         uint32_t* __restrict__ psize) {
+    __shared__ uint32_t s_tmp[TPB / 64 + 1];
+    const uint64_t nl = *n_long;
+    for (uint64_t li = blockIdx.x; li < nl; li += gridDim.x) {  // the same trip count for the whole block
+        const uint32_t d = long_list[li];
+        const uint64_t o = fpos[d];
+        const uint32_t n = frag_n[o];
+        uint32_t c = 0;
+        for (uint32_t j = 1 + threadIdx.x; j < n; j += TPB) c += pcid[o + j] != pcid[o + j - 1];
+        uint32_t changes;
+        (void)block_exclusive_scan<uint32_t>(c, s_tmp, changes);
+        if (threadIdx.x == 0) psize[d] = 1 + 4 + 12 * (changes + 1) + 4 + (k + n - 1);
+    }
+}
+
+// Block-wide exclusive max over threads (identity -1); total = the block's max.
+__device__ inline int64_t block_exclusive_max(int64_t v, int64_t* s_tmp, int64_t& total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x = y > x ? y : x;
+    }
+    int64_t ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = -1;
+    if (lane == 63) s_tmp[wave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = -1;
+        for (int w = 0; w < nw; ++w) {
+            const int64_t t = s_tmp[w];
+            s_tmp[w] = run;
+            run = t > run ? t : run;
+        }
+        s_tmp[nw] = run;
+    }
+    __syncthreads();
+    const int64_t before = s_tmp[wave];
+    total = s_tmp[nw];
+    __syncthreads();
+    return before > ex ? before : ex;
+}
+
+// One block per long fragment: the sequence (first k-mer whole, then the
+// last base of each following k-mer) by all threads, the colour runs in
+// tiles of TPB * LF_ITEMS k-mers: a thread's run index and the start of its
+// current run come from a block sum of run starts and a block max of start
+// positions, carried from tile to tile.
+constexpr uint32_t LF_ITEMS = 8;
+template <class K>
+__device__ void write_long_one(uint32_t d, const uint32_t* __restrict__ fpos, const uint32_t* __restrict__ frag_n,
+                               const uint8_t* __restrict__ frag_open, const uint8_t* __restrict__ frag_dir,
+                               const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
+                               const uint32_t* __restrict__ pcid, uint32_t k, const uint32_t* __restrict__ psize,
+                               const uint64_t* __restrict__ pboff, uint8_t* __restrict__ out, uint32_t* s_sum,
+                               int64_t* s_max);
+template <class K>
+__global__ __launch_bounds__(TPB) void k_frag_write_long(const uint32_t* __restrict__ long_list,
+                                                         const unsigned long long* __restrict__ n_long,
+                                                         const uint32_t* __restrict__ fpos,
+                                                         const uint32_t* __restrict__ frag_n,
+                                                         const uint8_t* __restrict__ frag_open,
+                                                         const uint8_t* __restrict__ frag_dir,
+                                                         const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
+                                                         const uint32_t* __restrict__ pcid, uint32_t k,
+                                                         const uint32_t* __restrict__ psize,
+                                                         const uint64_t* __restrict__ pboff,
+                                                         uint8_t* __restrict__ out) {
+    __shared__ uint32_t s_sum[TPB / 64 + 1];
+    __shared__ int64_t s_max[TPB / 64 + 1];
+    const uint64_t nl = *n_long;
+    for (uint64_t li = blockIdx.x; li < nl; li += gridDim.x)  // the same trip count for the whole block
+        write_long_one<K>(long_list[li], fpos, frag_n, frag_open, frag_dir, frc, pkey, pcid, k, psize, pboff, out,
+                          s_sum, s_max);
+}
+
+template <class K>
+__device__ void write_long_one(uint32_t d, const uint32_t* __restrict__ fpos, const uint32_t* __restrict__ frag_n,
+                               const uint8_t* __restrict__ frag_open, const uint8_t* __restrict__ frag_dir,
+                               const uint8_t* __restrict__ frc, const K* __restrict__ pkey,
+                               const uint32_t* __restrict__ pcid, uint32_t k, const uint32_t* __restrict__ psize,
+                               const uint64_t* __restrict__ pboff, uint8_t* __restrict__ out, uint32_t* s_sum,
+                               int64_t* s_max) {
+    using O = KOps<K>;
+    const uint64_t o = fpos[d];
+    const uint32_t n = frag_n[o], rev = frag_dir[o];
+    const uint32_t seq_len = k + n - 1;
+    const uint32_t runs = (psize[d] - (1 + 4 + 4 + seq_len)) / 12;
+    uint8_t* p = out + pboff[d];
+    uint8_t* pr = p + 5;
+    uint8_t* ps = pr + 12 * uint64_t(runs) + 4;
+    const char ACGT[4] = {'A', 'C', 'G', 'T'};
+    auto at = [&](uint32_t j) { return o + (rev ? n - 1 - j : j); };
+    if (threadIdx.x == 0) {
+        p[0] = frag_open[o];
+        put_bytes(p + 1, runs, 4);
+        put_bytes(ps - 4, seq_len, 4);
+    }
+    if (threadIdx.x < k) {
+        const uint64_t a0 = at(0);
+        const K key = pkey[a0];
+        const K km = ((frc[a0] & 1u) ^ rev) ? O::rc(key, k) : key;
+        ps[threadIdx.x] = ACGT[O::base(km, threadIdx.x, k)];
+    }
+    for (uint32_t j = 1 + threadIdx.x; j < n; j += TPB) {
+        const uint64_t a = at(j);
+        const K key = pkey[a];
+        const K km = ((frc[a] & 1u) ^ rev) ? O::rc(key, k) : key;
+        ps[k + j - 1] = ACGT[O::base(km, k - 1, k)];
+    }
+    uint32_t carry_runs = 0;    // runs started before the tile
+    int64_t carry_start = -1;   // the last run start before the tile
+    for (uint32_t t0 = 0; t0 < n; t0 += TPB * LF_ITEMS) {
+        const uint32_t j0 = t0 + threadIdx.x * LF_ITEMS;
+        uint32_t c[LF_ITEMS + 2];  // cids of j0 - 1 .. j0 + LF_ITEMS
+#pragma unroll
+        for (uint32_t i = 0; i < LF_ITEMS + 2; ++i) {
+            const int64_t j = int64_t(j0) + int64_t(i) - 1;
+            c[i] = (j >= 0 && j < int64_t(n)) ? pcid[at(uint32_t(j))] : NONE32;
+        }
+        uint32_t nstart = 0;
+        int64_t last = -1;
+#pragma unroll
+        for (uint32_t i = 0; i < LF_ITEMS; ++i) {
+            const uint32_t j = j0 + i;
+            if (j < n && (j == 0 || c[i + 1] != c[i])) {
+                ++nstart;
+                last = j;
+            }
+        }
+        uint32_t tile_runs;
+        const uint32_t ex = block_exclusive_scan<uint32_t>(nstart, s_sum, tile_runs) + carry_runs;
+        int64_t tile_max;
+        const int64_t mx = block_exclusive_max(last, s_max, tile_max);
+        int64_t r = int64_t(ex) - 1;  // index of the run the thread's first k-mer continues
+        int64_t cur = mx > carry_start ? mx : carry_start;
+#pragma unroll
+        for (uint32_t i = 0; i < LF_ITEMS; ++i) {
+            const uint32_t j = j0 + i;
+            if (j >= n) break;
+            const uint32_t cid = c[i + 1];
+            if (j == 0 || cid != c[i]) {
+                ++r;
+                cur = j;
+                put_bytes(pr + 12 * r, cid == NONE32 ? ~0ull : uint64_t(cid), 8);
+            }
+            if (j + 1 == n || c[i + 2] != cid) put_bytes(pr + 12 * r + 8, uint32_t(int64_t(j) - cur + 1), 4);
+        }
+        carry_runs += tile_runs;
+        carry_start = tile_max > carry_start ? tile_max : carry_start;
+    }
 }
 
 // osize[i] = size of the i-th fragment in output order
@@ -2502,10 +2674,10 @@ __global__ __launch_bounds__(TPB) void k_frag_write_pos(uint64_t nf, const uint3
         const uint32_t n = frag_n[f], rev = frag_dir[f];
         if (pstaged)
             write_record_lds<K>(p, frag_open[f], f - p0, n, rev, s_cid, s_pb, pkey[f + (rev ? n - 1 : 0)], k);
-        else
+        else if (n <= LONG_FRAG)  // (a block with a long fragment is never staged)
             write_record<K>(p, frag_open[f], f, n, rev, frc, pkey, pcid, k);
         s_loff[threadIdx.x] = loff;
-        s_size[threadIdx.x] = sz;
+        s_size[threadIdx.x] = n <= LONG_FRAG ? sz : 0u;  // a long one is written by k_frag_write_long
         s_dst[threadIdx.x] = dst;
     } else {
         s_size[threadIdx.x] = 0;
@@ -2563,6 +2735,7 @@ struct Phase2Bufs {
     DevBuf comp_ex, comp_len, comp_nodes, comp_off, other_x, other_ex, prc, pinfo, prank, pkey, pcid;
     DevBuf node_rec, eflag, ecnt, ends;
     DevBuf lr_bucket, lr_u, lr_uidx, lr_own, lr_st0, lr_st1, lr_list, lr_cnt;  // list ranking of long paths
+    DevBuf flong;                                                            // fragments of > LONG_FRAG k-mers
     DevBuf fkey_t, forder, forder_t, fdval, fpsize, fpboff, fsize, fboff;
     RadixScratch rs2;
     // bytes of the per-chunk buffers (what a new chunk reuses)
@@ -2577,7 +2750,7 @@ struct Phase2Bufs {
                                 &fkey, &comp_ex, &comp_len, &comp_nodes, &comp_off, &other_x, &other_ex, &prc, &pinfo,
                                 &prank, &pkey, &pcid, &node_rec, &eflag, &ends, &fkey_t, &forder, &forder_t, &fdval,
                                 &fpsize, &fpboff, &fsize, &fboff, &lr_bucket, &lr_u, &lr_uidx, &lr_own, &lr_st0,
-                                &lr_st1, &lr_list})
+                                &lr_st1, &lr_list, &flong})
             b += x->cap;
         return b;
     }
@@ -3080,7 +3253,13 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     uint32_t* dval = P.fdval.ensure<uint32_t>(nf);  // sorted: output rank -> position-order index
     uint32_t* dval_t = P.forder_t.ensure<uint32_t>(nf);
     uint32_t* psize = P.fpsize.ensure<uint32_t>(nf);
-    if (nf) { KSCOPE("k_frag_psize", st); k_frag_psize<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, pcid, k, psize, dval); }
+    // fragments of more than LONG_FRAG k-mers: a block each, the blocks
+    // striding over the device-side list (no host read of its length)
+    const uint32_t max_long = uint32_t(uint64_t(nn) / (LONG_FRAG + 1) + 1);
+    const uint32_t long_grid = std::min<uint32_t>(max_long, 1024);
+    uint32_t* long_frags = P.flong.ensure<uint32_t>(max_long);
+    if (nf) { KSCOPE("k_frag_psize", st); k_frag_psize<<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, pcid, k, psize, dval, long_frags, cnt + 15); }
+    if (nf) { KSCOPE("k_frag_psize_long", st); k_frag_psize_long<<<long_grid, TPB, 0, st>>>(long_frags, cnt + 15, forder, wa.frag_n, pcid, k, psize); }
     uint64_t max_btup = 0;
     for (uint32_t b = b0; b < b1; ++b) max_btup = std::max<uint64_t>(max_btup, h_btup[b + 1] - h_btup[b]);
     const int rbits = bits_for(max_btup), lbits = bits_for(b1 - b0);
@@ -3128,6 +3307,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
                             : P.frag_bytes.ensure_keep<uint8_t>(P.frag_total + fbytes, P.frag_total, st,
                                                                 grown(P.frag_total + fbytes)) + P.frag_total;
     if (nf) { KSCOPE("k_frag_write", st); k_frag_write_pos<K><<<blocks_for(nf), TPB, 0, st>>>(nf, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout, pos_cap); }
+    if (nf) { KSCOPE("k_frag_write_long", st); k_frag_write_long<K><<<long_grid, TPB, 0, st>>>(long_frags, cnt + 15, forder, wa.frag_n, wa.frag_open, wa.frag_dir, prc, pkey, pcid, k, psize, pboff, fout); }
     CDBG_HIP(hipGetLastError());
     CDBG_HIP(hipMemcpyAsync(h_cnt, cnt, sizeof h_cnt, hipMemcpyDeviceToHost, st));
     CDBG_HIP(hipStreamSynchronize(st));

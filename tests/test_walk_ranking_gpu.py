@@ -104,3 +104,48 @@ def test_forced_ranking_chunked_skewed(gpu_ctx, monkeypatch):
     monkeypatch.setenv("CDBG_WALK_CAP", "2")
     data, off = G.synth(seed=12, num_colors=6, genome_len=30000, snp_rate=0.01, indel_rate=0.001)
     _assert_same(*_run_both(gpu_ctx, data, off, 31, 21, 1024, chunk=40000, cuts=G.zipf_bucket_cuts(1024, 1.3)))
+
+
+def _kernels(ctx, data, off, k, m, B, cuts=None):
+    with ctx.job(data, off, k, m, B) as job:
+        if cuts is not None:
+            job.set_bucket_map(cuts)
+        job.set_profiling(True)
+        job.run()
+        return set(job.kernel_times())
+
+
+@pytest.mark.parametrize("k,m", [(31, 19), (63, 27)])
+def test_long_fragment_colour_runs(gpu_ctx, k, m):
+    """One bucket; colours that cover different stretches of one genome make
+    a fragment of ~60,000 k-mers with several colour runs (records of more
+    than 4096 k-mers are sized and written by a block each)."""
+    rng = np.random.default_rng(17)
+    g = ACGT[rng.integers(0, 4, 60000)].tobytes()
+    parts = [g + b"\n", g[:30000] + b"\n", g[20000:50000] + b"\n", g[:200000] + b"\n"]
+    data = np.frombuffer(b"".join(parts), np.uint8).copy()
+    off = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.uint64)
+    _assert_same(*_run_both(gpu_ctx, data, off, k, m, 1))
+    assert "k_frag_write_long" in _kernels(gpu_ctx, data, off, k, m, 1)
+
+
+def test_long_fragments_foreign_runs(gpu_ctx):
+    """Two buckets, one holding all but 1/4096 of the minimizers: paths and
+    fragments of up to ~5,800 k-mers whose fragments end on foreign boundary
+    k-mers (COLOR_RUN_FOREIGN runs)."""
+    data, off = G.synth(seed=19, num_colors=3, genome_len=100000, snp_rate=0.0002, indel_rate=0.00005)
+    cuts = G.zipf_bucket_cuts(2, 12.0)
+    _assert_same(*_run_both(gpu_ctx, data, off, 31, 19, 2, cuts=cuts))
+
+
+@pytest.mark.parametrize("k,m", [(31, 21), (63, 27)])
+def test_one_fragment_whole_genome(gpu_ctx, k, m):
+    """A single 200 kbp colour in one bucket: one path and one fragment of
+    ~200,000 k-mers."""
+    rng = np.random.default_rng(23)
+    g = ACGT[rng.integers(0, 4, 200000)].tobytes() + b"\n"
+    data = np.frombuffer(g, np.uint8).copy()
+    off = np.array([0, len(g)], np.uint64)
+    ob, s, gf, gw, go = _run_both(gpu_ctx, data, off, k, m, 1)
+    assert s["num_frags"] == 1
+    _assert_same(ob, s, gf, gw, go)
