@@ -125,6 +125,9 @@ _SIGS = {
                                         C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32, U64P]),
     "cdbg_gpu_bucket_process": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32,
                                           C.c_uint32, C.c_char_p, C.c_char_p, C.POINTER(ColorState)]),
+    "cdbg_gpu_bucket_process_ex": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                             C.c_uint32, C.c_char_p, C.c_char_p, C.POINTER(ColorState),
+                                             C.POINTER(C.c_uint64), C.c_uint64, C.c_int]),
     "cdbg_gpu_job_write_bucket_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, U64P]),
     "cdbg_gpu_job_from_bucket_files": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32,
                                                  C.c_uint32, C.c_uint32, C.POINTER(C.c_void_p)]),
@@ -431,6 +434,18 @@ class Context:
                                              threads, os.fsencode(frag_path), os.fsencode(cs_path),
                                              C.byref(st)))
         return st.as_dict()
+
+    def bucket_process_ex(self, tmp_dir, num_buckets, k, num_colors, frag_path, cs_path, threads=8,
+                          mem_budget_bytes=0, delete_consumed=False):
+        """bucket_process with process_buckets' progress counter, memory
+        budget and delete_consumed_buckets; returns (colour state, buckets done)."""
+        st = ColorState()
+        done = C.c_uint64(0)
+        _check(lib().cdbg_gpu_bucket_process_ex(self.h, os.fsencode(tmp_dir), num_buckets, k, num_colors,
+                                                threads, os.fsencode(frag_path), os.fsencode(cs_path),
+                                                C.byref(st), C.byref(done), mem_budget_bytes,
+                                                1 if delete_consumed else 0))
+        return st.as_dict(), done.value
 
 
 class Job:

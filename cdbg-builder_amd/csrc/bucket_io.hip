@@ -742,7 +742,10 @@ void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64
     if (J.prof.on) J.prof.collect();  // the stream is idle: every launch above has finished
 }
 
-void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
+// With delete_consumed the bucket files are removed once the whole load has
+// succeeded (process_buckets' delete_consumed_buckets,
+// bucket_walker.hpp:684-691: scratch does not keep what phase 2 has read).
+void load_bucket_files(Job& J, const std::string& dir, uint32_t threads, bool delete_consumed) {
     AllocStream alloc_on_job_stream(J.st);
     hipStream_t st = J.st;
     const uint32_t B = J.B, k = J.k;
@@ -904,6 +907,11 @@ void load_bucket_files(Job& J, const std::string& dir, uint32_t threads) {
     J.n_supers = np;
     J.phase1_done = true;
     J.from_files = true;
+    if (delete_consumed)
+        for (uint32_t b = 0; b < B; ++b) {
+            std::error_code ec;
+            std::filesystem::remove(bucket_path(dir, b), ec);
+        }
 }
 
 void write_phase2_files(Job& J, const std::string& frag_path, const std::string& cs_path,

@@ -11,6 +11,7 @@
 
 #include "cdbg_gpu.h"
 #include "common.h"
+#include "hbm_model.h"
 #include "job.h"
 
 namespace cdbg_amd {
@@ -376,8 +377,8 @@ CDBG_API int cdbg_gpu_job_write_bucket_files(cdbg_gpu_job* job, const char* dir,
     })
 }
 
-CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, uint32_t num_buckets, uint32_t k,
-                                            uint32_t num_colors, uint32_t threads, cdbg_gpu_job** out) {
+static int job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, uint32_t num_buckets, uint32_t k,
+                                 uint32_t num_colors, uint32_t threads, bool delete_consumed, cdbg_gpu_job** out) {
     CDBG_TRY({
         if (!ctx || !dir || !out) throw Error("null argument");
         if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
@@ -394,13 +395,18 @@ CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, 
             J.st = acquire_stream(ctx->device);
             J.stream_owner.s = J.st;
             J.stream_owner.device = ctx->device;
-            load_bucket_files(J, dir, threads);
+            load_bucket_files(J, dir, threads, delete_consumed);
         } catch (...) {
             delete h;
             throw;
         }
         *out = h;
     })
+}
+
+CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, uint32_t num_buckets, uint32_t k,
+                                            uint32_t num_colors, uint32_t threads, cdbg_gpu_job** out) {
+    return job_from_bucket_files(ctx, dir, num_buckets, k, num_colors, threads, false, out);
 }
 
 CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag_path, const char* color_sets_path,
@@ -423,15 +429,31 @@ CDBG_API int cdbg_gpu_bucket_write(cdbg_gpu_ctx* ctx, const char* const* files, 
     return rc;
 }
 
-CDBG_API int cdbg_gpu_bucket_process(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
-                                     uint32_t num_colors, uint32_t threads, const char* frag_path,
-                                     const char* color_sets_path, cdbg_gpu_color_state* state) {
+CDBG_API int cdbg_gpu_bucket_process_ex(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
+                                        uint32_t num_colors, uint32_t threads, const char* frag_path,
+                                        const char* color_sets_path, cdbg_gpu_color_state* state, uint64_t* done,
+                                        uint64_t mem_budget_bytes, int delete_consumed_buckets) {
     cdbg_gpu_job* job = nullptr;
-    int rc = cdbg_gpu_job_from_bucket_files(ctx, tmp_dir, num_buckets, k, num_colors, threads, &job);
-    if (rc == 0) rc = cdbg_gpu_job_phase2(job);
+    int rc = job_from_bucket_files(ctx, tmp_dir, num_buckets, k, num_colors, threads, delete_consumed_buckets != 0,
+                                   &job);
+    if (rc == 0) {
+        job->J.progress = done;
+        // the device working set of a phase-2 chunk within the budget (one
+        // bucket at least, as the reference admits one bucket whatever its size)
+        if (mem_budget_bytes)
+            job->J.chunk_tuples = std::max<uint64_t>(1, uint64_t(0.6 * double(mem_budget_bytes) / p2_per_occ(k)));
+        rc = cdbg_gpu_job_phase2(job);
+    }
     if (rc == 0) rc = cdbg_gpu_job_write_phase2_files(job, frag_path, color_sets_path, state);
     if (job) cdbg_gpu_job_destroy(job);
     return rc;
+}
+
+CDBG_API int cdbg_gpu_bucket_process(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
+                                     uint32_t num_colors, uint32_t threads, const char* frag_path,
+                                     const char* color_sets_path, cdbg_gpu_color_state* state) {
+    return cdbg_gpu_bucket_process_ex(ctx, tmp_dir, num_buckets, k, num_colors, threads, frag_path, color_sets_path,
+                                      state, nullptr, 0, 0);
 }
 
 CDBG_API uint64_t cdbg_gpu_device_bytes(void) { return g_device_bytes.load(); }

@@ -198,10 +198,10 @@ int main(int argc, char** argv) {
             uint64_t tot = 0;
             for (uint64_t u : unc) tot += u;
             std::cout << "  bucket-write: " << tot << " uncompressed bucket bytes\n";
-            check(cdbg_gpu_bucket_process(ctx, tmp.c_str(), B, c.k, nc, c.t, frag.c_str(), cs.c_str(), &st));
-            if (!c.keep_tmp)
-                for (uint32_t b = 0; b < B; ++b)
-                    std::filesystem::remove(tmp + "/bucket_" + std::to_string(b) + ".bin");
+            uint64_t done = 0;
+            check(cdbg_gpu_bucket_process_ex(ctx, tmp.c_str(), B, c.k, nc, c.t, frag.c_str(), cs.c_str(), &st, &done,
+                                             0, c.keep_tmp ? 0 : 1));
+            std::cout << "  bucket-process: " << done << " of " << B << " buckets\n";
         }
         std::cout << "  bucket fragments: " << st.num_fragments << "\n"
                   << "  distinct color sets: " << st.num_color_sets << "\n"

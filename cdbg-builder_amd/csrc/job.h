@@ -66,6 +66,7 @@ struct Job {
     cdbg_gpu_stats stats{};
     Profiler prof;  // per-kernel event timing, off unless enabled
     uint64_t chunk_tuples = 0;  // phase-2 chunk budget in k-mer occurrences (0 = from free HBM)
+    uint64_t* progress = nullptr;  // caller's counter of buckets finished by phase 2 (added to atomically)
     uint64_t auto_chunk = 0;    // budget derived from free HBM on the first run, then kept
     uint32_t n_chunks = 0;      // chunks used by the last phase-2 run
     std::vector<uint64_t> h_color_off;  // colour offsets (relative to the first colour), host copy
@@ -100,7 +101,7 @@ void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offs
 void copy_phase2_results_device(Job& J, void* frags, void* words, void* offsets, hipStream_t st);
 // bucket_io.hip
 void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64_t* unc_bytes);
-void load_bucket_files(Job& J, const std::string& dir, uint32_t threads);
+void load_bucket_files(Job& J, const std::string& dir, uint32_t threads, bool delete_consumed = false);
 void write_phase2_files(Job& J, const std::string& frag_path, const std::string& cs_path,
                         cdbg_gpu_color_state* state);
 

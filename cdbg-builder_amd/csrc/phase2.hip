@@ -3414,6 +3414,7 @@ static void phase2_impl(Job& J) {
                         " occurrences, budget " + std::to_string(budget) + "]");
         }
         ++nchunks;
+        if (J.progress) __atomic_fetch_add(J.progress, uint64_t(b1 - b0), __ATOMIC_RELAXED);
         b0 = b1;
         if (J.p2_max_chunks && nchunks >= J.p2_max_chunks) break;  // a trial over the first chunks only
     }

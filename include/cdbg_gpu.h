@@ -254,6 +254,22 @@ CDBG_API int cdbg_gpu_bucket_write(cdbg_gpu_ctx* ctx, const char* const* files, 
 CDBG_API int cdbg_gpu_bucket_process(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
                                      uint32_t num_colors, uint32_t threads, const char* frag_path,
                                      const char* color_sets_path, cdbg_gpu_color_state* state);
+/* cdbg_gpu_bucket_process with the remaining parameters of process_buckets
+ * (bucket_walker.hpp:607-612):
+ *   done              (may be NULL) a counter the call adds the number of
+ *                     finished buckets to, atomically, chunk by chunk, as the
+ *                     reference's std::atomic<uint64_t>* done progress gauge;
+ *   mem_budget_bytes  (0: from free device memory) the device working set of
+ *                     a phase-2 chunk, by the library's per-occurrence model;
+ *                     a chunk holds one bucket at least, as the reference
+ *                     admits one bucket whatever its size (:646-654);
+ *   delete_consumed_buckets  nonzero: the bucket files are removed once they
+ *                     have all been read (:684-691).
+ * The outputs are the same whatever the budget. */
+CDBG_API int cdbg_gpu_bucket_process_ex(cdbg_gpu_ctx* ctx, const char* tmp_dir, uint32_t num_buckets, uint32_t k,
+                                        uint32_t num_colors, uint32_t threads, const char* frag_path,
+                                        const char* color_sets_path, cdbg_gpu_color_state* state, uint64_t* done,
+                                        uint64_t mem_budget_bytes, int delete_consumed_buckets);
 
 /* Tuning of one job, and the default for jobs created later on a context
  * (served jobs included).  Invalid values are an error. */

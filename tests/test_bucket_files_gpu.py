@@ -122,6 +122,26 @@ def test_bucket_process_matches_oracle(gpu_ctx, tmp_path, k, m, B):
     _compare_phase2(gpu_ctx, tmp_path, d, B, k, len(files))
 
 
+@pytest.mark.parametrize("budget", [0, 1 << 20])
+def test_bucket_process_ex(gpu_ctx, tmp_path, budget):
+    """process_buckets' remaining parameters: the progress counter reaches
+    the bucket count, a small device budget (many chunks) leaves the outputs
+    unchanged, and consumed bucket files are deleted."""
+    files = _fasta_job(tmp_path, colors=5)
+    d = str(tmp_path / "buckets")
+    B, k = 64, 31
+    gpu_ctx.bucket_write(files, k, 19, B, d)
+    ref = _compare_phase2(gpu_ctx, tmp_path, d, B, k, len(files))
+    want = open(str(tmp_path / "frag_unitigs.bin"), "rb").read()
+    frag, cs = str(tmp_path / "ex_frag.bin"), str(tmp_path / "ex.color_sets")
+    st, done = gpu_ctx.bucket_process_ex(d, B, k, len(files), frag, cs, mem_budget_bytes=budget,
+                                         delete_consumed=True)
+    assert done == B
+    assert st == ref
+    assert open(frag, "rb").read() == want
+    assert not any(os.path.exists(os.path.join(d, f"bucket_{b}.bin")) for b in range(B))
+
+
 def test_file_path_equals_resident_path(gpu_ctx, tmp_path):
     files = _fasta_job(tmp_path, colors=4)
     data, off = G.load_fasta(files)
