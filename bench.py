@@ -332,6 +332,16 @@ def served_leg(G, ctx, n_jobs, slots, rate, threads):
         kmers.append(plain(x))
         service.append(time.perf_counter() - t0)
     plain_tput = sum(kmers) / sum(service)
+    # how busy the plain path keeps the GPU: kernel time (HIP events around
+    # every launch, so an upper bound) over the plain wall time; served
+    # overlap can only fill the rest, so about 1 / busy bounds served / plain
+    kernel_s = 0.0
+    for x in inputs:
+        with ctx.job(*x) as jb:
+            jb.set_profiling(True)
+            jb.run()
+            kernel_s += sum(ms for _, ms in jb.kernel_times().values()) / 1e3
+    plain_busy = kernel_s / sum(service)
     fin, lat_plain = 0.0, []
     for a, sv in zip(arrivals, service):
         fin = max(fin, a) + sv
@@ -389,7 +399,8 @@ def served_leg(G, ctx, n_jobs, slots, rate, threads):
         print(f"bench: served, Poisson arrivals: allocator {alloc_open}", file=sys.stderr, flush=True)
     pct = lambda v, q: round(1e3 * float(np.percentile(v, q)), 1)
     return {"jobs": n_jobs, "slots": slots, "bases": bases, "distinct_kmers": sum(kmers),
-            "plain": {"value": round(plain_tput, 1), "p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99)},
+            "plain": {"value": round(plain_tput, 1), "p50_ms": pct(lat_plain, 50), "p99_ms": pct(lat_plain, 99),
+                      "gpu_busy": round(plain_busy, 3)},
             "served_1_slot": {"value": round(resident_tput, 1)},
             "served": {"value": round(served_tput, 1), "p50_ms": pct(lat_served, 50),
                        "p99_ms": pct(lat_served, 99), "allocator_all_queued": alloc_closed,
@@ -408,8 +419,15 @@ def cpu_baseline(G, sample_colors, genome, k, m, B, cores):
     ob = O.Build(data, off, k, m, B, threads=cores)
     t1, t2 = ob.times()
     nk, nocc = ob.stat("num_kmers"), ob.stat("num_occ")
+    per_thread = nocc / (t1 + t2) / cores
     return {"value": nk / (t1 + t2), "unit": "distinct k-mers/s", "cores": cores, "kind": "port",
             "occurrences_per_s": round(nocc / (t1 + t2), 1), "bases_per_s": round(len(data) / (t1 + t2), 1),
+            "phase_s": [round(t1, 2), round(t2, 2)],
+            # the reference's own published run (README.md:165,183,186): 1.573e11
+            # k-mer occurrences through phases 1+2 in 14,519.9 + 15,609.1 s on
+            # 48 threads, scratch on disk
+            "occurrences_per_s_per_thread": round(per_thread, 1),
+            "reference_published_occurrences_per_s_per_thread": round(1.57311093967e11 / 30129.0 / 48, 1),
             "sample": f"first {sample_colors} of the workload's colours ({len(data)/1e6:.1f} Mbp), "
                       f"phases 1+2 of the oracle, {nk} distinct k-mers ({nocc} k-mer occurrences) "
                       f"in {t1 + t2:.1f} s"}
@@ -528,6 +546,7 @@ def main():
     ap.add_argument("--kernel-times", action="store_true", help="print per-kernel totals to stderr")
     ap.add_argument("--no-batched", action="store_true", help="skip the skewed-bucket (batched) leg")
     ap.add_argument("--batched-only", action="store_true", help="run only the skewed-bucket (batched) leg")
+    ap.add_argument("--served-only", action="store_true", help="run only the serving leg")
     ap.add_argument("--copy-probe-only", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel over BYTES (counter calibration)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the two profiler runs that measure traffic")
@@ -556,6 +575,12 @@ def main():
     if args.large_only:
         ctx = G.Context(0)
         print(json.dumps({"large": large_leg(G, ctx, args.large_colors or 4000, args.steps, args.cpu_cores)}))
+        ctx.close()
+        return
+    if args.served_only:
+        ctx = G.Context(0)
+        print(json.dumps({"served": served_leg(G, ctx, args.serve_jobs or 40, args.serve_slots, args.serve_rate,
+                                               args.cpu_cores)}))
         ctx.close()
         return
     if args.batched_only:
