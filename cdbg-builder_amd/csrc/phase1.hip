@@ -103,7 +103,11 @@ struct SkArgs {
 // the high bits of a window minimum are not; every minimizer still has one
 // bucket, so all super-k-mers of a minimizer stay together).
 __device__ __forceinline__ uint32_t bucket_of(uint64_t h, const SkArgs& a) {
-    if (!a.cuts) return uint32_t((h >> 1) % a.B);
+    if (!a.cuts) {
+        // B from -b is a power of two: a mask instead of a 64-bit division
+        if ((a.B & (a.B - 1)) == 0) return uint32_t(h >> 1) & (a.B - 1);
+        return uint32_t((h >> 1) % a.B);
+    }
     const uint64_t x = (h >> 1) & 0xFFFFFFFFull;
     uint32_t lo = 0, hi = a.B - 1;  // first cut > x
     while (lo < hi) {
@@ -191,10 +195,15 @@ __global__ __launch_bounds__(SK_THREADS) void k_superkmers(SkArgs a) {
         return (s_pk[lp >> 4] >> (2 * (lp & 15))) & 3u;
     };
 
-    // 1. canonical m-mer hashes for positions p0-1 .. p0-1+SK_TILE+W
+    // 1. canonical m-mer hashes for positions p0-1 .. p0-1+SK_TILE+W: one run
+    //    per thread, rolled from one hash computed whole (runs of
+    //    ceil(nq / SK_THREADS) positions, so the W + 1 positions past the
+    //    tile leave no thread a second run)
     const int nq = SK_TILE + int(W) + 1;
-    for (int q0 = threadIdx.x * 16; q0 < nq; q0 += SK_THREADS * 16) {
-        const int cnt = min(16, nq - q0);
+    const int rlen = (nq + SK_THREADS - 1) / SK_THREADS;
+    if (int(threadIdx.x) * rlen < nq) {  // the last threads may have no run
+        const int q0 = int(threadIdx.x) * rlen;
+        const int cnt = min(rlen, nq - q0);
         const int64_t pos = p0 - 1 + q0;
         uint64_t f = 0, r = 0;
         for (uint32_t i = 0; i < m; ++i) {
