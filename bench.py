@@ -220,6 +220,20 @@ def batched_leg(G, ctx, steps, warmup, threads):
         kt = job.kernel_times()
         chunks = job.num_chunks()
         sizes = np.diff(job.bucket_offsets().astype(np.int64))
+        # the same buckets one at a time: a chunk per bucket, so every kernel
+        # is launched (and every count read back) per bucket, as a GPU port
+        # of the reference's per-bucket scheduler (bucket_walker.hpp:643-779)
+        # would run them
+        job.set_profiling(False)
+        job.set_chunk_tuples(1)
+        t0 = time.perf_counter()
+        job.run()
+        pb_elapsed = time.perf_counter() - t0
+        pb_chunks = job.num_chunks()
+        if job.stats()["num_kmers"] != s["num_kmers"]:
+            raise RuntimeError("per-bucket run differs from the batched run")
+        job.set_chunk_tuples(0)
+    pb_value = s["num_kmers"] / pb_elapsed
     plain_value = s["num_kmers"] * steps / plain_elapsed
     ms_step = 1e3 * elapsed / steps
     roof, kernels = roofline(kt, s, c["k"], steps, chunks, ms_step)
@@ -228,6 +242,9 @@ def batched_leg(G, ctx, steps, warmup, threads):
           flush=True)
     return {"metric": "distinct k-mers/s (phases 1+2)", "value": round(value, 1), "ms_per_step": round(ms_step, 3),
             "default_knobs_value": round(plain_value, 1), "tuned_over_default": round(value / plain_value, 3),
+            "per_bucket": {"value": round(pb_value, 1), "ms_per_step": round(1e3 * pb_elapsed, 1),
+                           "chunks": pb_chunks},
+            "batched_over_per_bucket": round(value / pb_value, 3),
             "autotune": tune,
             "steps": steps, "ms_phase1": round(s["ms_phase1"], 2), "ms_phase2": round(s["ms_phase2"], 2),
             "chunks": chunks, "num_kmers": s["num_kmers"], "num_occ": s["num_occ"],

@@ -2908,8 +2908,11 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
     SortPlan plan;
     plan.build(lb);
     if (D_max > 0) {
-        // the fewest passes of at most digit_bits bits each, bits spread evenly
-        const int dmax = J.tune.digit_bits ? int(J.tune.digit_bits) : 8;
+        // the fewest passes of at most digit_bits bits each, bits spread
+        // evenly; a pass's tile and counters must fit the LDS (128-bit keys
+        // allow 10 bits, 64-bit keys 11)
+        int dmax = J.tune.digit_bits ? int(J.tune.digit_bits) : 8;
+        while (dmax > 1 && radix_scatter_lds<K, uint64_t>(dmax) > 160 * 1024) --dmax;
         const int passes = (D_max + dmax - 1) / dmax;
         segmented_radix_sort<K, uint64_t>(keys, vals, keys_t, vals_t, plan, kb - D_max, kb,
                                           (D_max + passes - 1) / passes, P.rs2, st, "tuples");

@@ -92,3 +92,21 @@ def test_context_tuning_is_inherited(gpu_ctx):
         gpu_ctx.set_tuning()
     with gpu_ctx.job(data, off, 31, 19, 8) as job:
         assert job.get_tuning().persistent == 0
+
+
+@pytest.mark.parametrize("chunk", [0, 1])
+def test_widest_digit_on_128bit_keys(gpu_ctx, chunk):
+    """One bucket of ~5e5 128-bit keys sorts its top 11 bits in one global
+    pass; with 11-bit digits that pass would need more LDS than a workgroup
+    has, so the digit narrows to 10 bits (as with one bucket per chunk after
+    the tuner chose 11 bits on a multi-bucket sample)."""
+    data, off = G.synth(seed=29, num_colors=5, genome_len=100000, snp_rate=0.01, indel_rate=0.001)
+    k, m, B = 63, 27, 1
+    want = _expected(data, off, k, m, B)
+    with gpu_ctx.job(data, off, k, m, B) as job:
+        job.set_tuning(digit_bits=11)
+        if chunk:
+            job.set_chunk_tuples(chunk)
+        job.run()
+        assert job.stats()["num_occ"] > 358 * 1024  # top 11 bits of the bucket take the global pass
+        _check(job, want)
