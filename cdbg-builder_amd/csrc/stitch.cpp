@@ -181,6 +181,7 @@ struct Stitcher {
     FragView frag(uint64_t i) const { return view_at(buf, off[i]); }
 
     void index() {
+        off.reserve(n_bytes / 48 + 16);  // records average well above 48 bytes
         uint64_t p = 0;
         while (p < n_bytes) {
             if (n_bytes - p < 9) throw Error("stitch: truncated fragment record");
@@ -258,6 +259,7 @@ struct Stitcher {
                 End* a = ends.get() + boff[h];
                 End* z = ends.get() + boff[h + 1];
                 std::sort(a, z, [](const End& x, const End& y) { return x.key < y.key || (x.key == y.key && x.id < y.id); });
+                uint64_t nj = 0, nu = 0;  // this bucket's joins and unmatched ends (one atomic add each)
                 for (End* p = a; p < z;) {
                     End* q = p + 1;
                     while (q < z && q->key == p->key) ++q;
@@ -271,12 +273,14 @@ struct Stitcher {
                             partner[p[0].id >> 1] = p[1].id >> 1;
                             partner[p[1].id >> 1] = p[0].id >> 1;
                             paired = true;
-                            joins.fetch_add(1);
+                            ++nj;
                         }
                     }
-                    if (!paired) unmatched.fetch_add(uint64_t(q - p));
+                    if (!paired) nu += uint64_t(q - p);
                     p = q;
                 }
+                joins.fetch_add(nj);
+                unmatched.fetch_add(nu);
             }
         }, 1);
         st.joins = joins;
