@@ -144,6 +144,9 @@ _SIGS = {
     "cdbg_gpu_server_destroy": (None, [C.c_void_p]),
     "cdbg_stitch": (C.c_void_p, [U8P, C.c_uint64, C.c_uint32, C.c_uint32]),
     "cdbg_stitch_file": (C.c_void_p, [C.c_char_p, C.c_uint32, C.c_uint32]),
+    "cdbg_gpu_stitch": (C.c_void_p, [C.c_void_p, U8P, C.c_uint64, C.c_uint32, C.c_uint32]),
+    "cdbg_gpu_stitch_file": (C.c_void_p, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32]),
+    "cdbg_frag_offsets": (C.c_int, [U8P, C.c_uint64, C.c_uint32, U64P, C.c_uint64, U64P]),
     "cdbg_unitigs_stats": (C.c_int, [C.c_void_p, C.POINTER(StitchStats)]),
     "cdbg_unitigs_bases": (C.c_uint64, [C.c_void_p]),
     "cdbg_unitigs_export": (C.c_int, [C.c_void_p, C.c_char_p, U64P, U64P]),
@@ -320,6 +323,21 @@ def stitch_file(path, k, threads=8):
     return Unitigs(lib().cdbg_stitch_file(os.fsencode(path), k, threads))
 
 
+def _u8(frags):
+    return np.ascontiguousarray(np.frombuffer(bytes(frags), np.uint8) if isinstance(frags, (bytes, bytearray))
+                                else frags, dtype=np.uint8)
+
+
+def frag_offsets(frags, threads=8):
+    """Start of every record of a frag_unitigs.bin byte stream (u64 array)."""
+    a = _u8(frags)
+    cnt = C.c_uint64()
+    cap = len(a) // 9 + 1  # a record takes at least 9 bytes
+    out = np.zeros(cap, np.uint64)
+    _check(lib().cdbg_frag_offsets(_ptr(a, U8P), len(a), threads, _ptr(out, U64P), cap, C.byref(cnt)))
+    return out[:cnt.value]
+
+
 def finalize_color_sets(cs_path, state):
     """Finish <out>.color_sets from phase 2's files and colour state (a dict
     from bucket_process / write_phase2_files)."""
@@ -394,6 +412,15 @@ class Context:
     def set_tuning(self, t=None, **kw):
         """Default tuning of jobs created on this context from now on."""
         _check(lib().cdbg_gpu_ctx_set_tuning(self.h, C.byref(t if t is not None else tuning(**kw))))
+
+    def stitch(self, frags, k, threads=8):
+        """Phase 3 on the device over a frag_unitigs.bin byte stream: the same
+        unitigs, order and statistics as stitch()."""
+        a = _u8(frags)
+        return Unitigs(lib().cdbg_gpu_stitch(self.h, _ptr(a, U8P), len(a), k, threads))
+
+    def stitch_file(self, path, k, threads=8):
+        return Unitigs(lib().cdbg_gpu_stitch_file(self.h, os.fsencode(path), k, threads))
 
     def lz4_compress_frames(self, frames):
         """LZ4 blocks of the byte strings `frames` (each <= 64 KiB), compressed

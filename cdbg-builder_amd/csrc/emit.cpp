@@ -269,23 +269,26 @@ CDBG_API void* cdbg_stitch(const uint8_t* frags, uint64_t n_bytes, uint32_t k, u
 CDBG_API void* cdbg_stitch_file(const char* frag_path, uint32_t k, uint32_t threads) {
     try {
         if (!frag_path) throw Error("null argument");
-        std::FILE* f = std::fopen(frag_path, "rb");
-        if (!f) throw Error(std::string("cannot open ") + frag_path);
-        std::vector<uint8_t> buf;
-        std::error_code ec;
-        const uint64_t n = std::filesystem::file_size(frag_path, ec);
-        if (ec) {
-            std::fclose(f);
-            throw Error(std::string("cannot stat ") + frag_path);
-        }
-        buf.resize(n);
-        const size_t got = n ? std::fread(buf.data(), 1, n, f) : 0;
-        std::fclose(f);
-        if (got != n) throw Error(std::string("short read of ") + frag_path);
-        return stitch_fragments(buf.data(), n, k, threads);
+        uint64_t n = 0;
+        const std::unique_ptr<uint8_t[]> buf = read_file(frag_path, threads, n);
+        return stitch_fragments(buf.get(), n, k, threads);
     } catch (std::exception& e) {
         set_error(e.what());
         return nullptr;
+    }
+}
+
+CDBG_API int cdbg_frag_offsets(const uint8_t* frags, uint64_t n_bytes, uint32_t threads, uint64_t* out, uint64_t cap,
+                               uint64_t* count) {
+    try {
+        if ((!frags && n_bytes) || !count || (!out && cap)) throw Error("null argument");
+        const std::vector<uint64_t> off = fragment_offsets(frags, n_bytes, threads);
+        *count = off.size();
+        if (off.size() > cap) throw Error("cdbg_frag_offsets: output capacity too small");
+        if (!off.empty()) std::memcpy(out, off.data(), off.size() * 8);
+        return 0;
+    } catch (std::exception& e) {
+        return set_error(e.what());
     }
 }
 

@@ -24,6 +24,10 @@
 // id, then chain order.
 #include "stitch.h"
 
+#include <fcntl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <atomic>
 #include <cstring>
@@ -155,6 +159,34 @@ void parallel_for(uint64_t n, uint32_t threads, F&& fn, uint64_t grain = 4096) {
     if (failed) throw Error(err);
 }
 
+inline bool is_base(uint8_t c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; }
+
+// Size of the record at p, 0 when it does not fit in [p, n).
+inline uint64_t record_size(const uint8_t* buf, uint64_t n, uint64_t p) {
+    if (n - p < 9) return 0;
+    const uint64_t nr = rd32(buf + p + 1);
+    if (n - p < 9 + 12 * nr) return 0;
+    const uint64_t len = rd32(buf + p + 5 + 12 * nr);
+    if (n - p - 9 - 12 * nr < len) return 0;
+    return 9 + 12 * nr + len;
+}
+
+// Whether p looks like a record start inside a stream: an open-flag byte
+// (0-3) after the last base of a sequence, opening a chain of a few records
+// that fit, each with a flag byte and ACGT at both ends of its sequence.
+inline bool plausible_start(const uint8_t* buf, uint64_t n, uint64_t p) {
+    if (p == 0 || buf[p] > 3 || !is_base(buf[p - 1])) return false;
+    for (int r = 0; r < 4 && p < n; ++r) {
+        if (buf[p] > 3) return false;
+        const uint64_t s = record_size(buf, n, p);
+        if (!s) return false;
+        const uint64_t seq = p + 9 + 12 * uint64_t(rd32(buf + p + 1));  // s = seq - p + len
+        if (seq < p + s && (!is_base(buf[seq]) || !is_base(buf[p + s - 1]))) return false;
+        p += s;
+    }
+    return true;
+}
+
 // A tig under construction: bases and the colour-set run of every k-mer.
 struct Tig {
     std::string seq;
@@ -181,17 +213,7 @@ struct Stitcher {
     FragView frag(uint64_t i) const { return view_at(buf, off[i]); }
 
     void index() {
-        off.reserve(n_bytes / 48 + 16);  // records average well above 48 bytes
-        uint64_t p = 0;
-        while (p < n_bytes) {
-            if (n_bytes - p < 9) throw Error("stitch: truncated fragment record");
-            const uint64_t nr = rd32(buf + p + 1);
-            if (n_bytes - p < 9 + 12 * nr) throw Error("stitch: truncated fragment record");
-            const uint64_t len = rd32(buf + p + 5 + 12 * nr);
-            if (n_bytes - p - 9 - 12 * nr < len) throw Error("stitch: truncated fragment record");
-            off.push_back(p);
-            p += 9 + 12 * nr + len;
-        }
+        off = fragment_offsets(buf, n_bytes, threads);
         st.num_fragments = off.size();
         // every record's runs cover its k-mers
         parallel_for(off.size(), threads, [&](uint64_t b, uint64_t e, uint32_t) {
@@ -421,6 +443,134 @@ void cut_cycle(Tig& t, uint32_t k, UnitigList& out, cdbg_stitch_stats& st) {
 }
 
 }  // namespace
+
+std::vector<uint64_t> fragment_offsets(const uint8_t* buf, uint64_t n, uint32_t threads) {
+    // thread t guesses the first record start at or after n t / T and follows
+    // the chain of records from there past the end of its range (ranges of
+    // at least 64 KiB)
+    const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads, n >> 16));
+    struct Part {
+        uint64_t lo, hi, guess = ~0ull, exit = 0;
+        bool failed = false;
+        std::vector<uint64_t> starts;
+    };
+    std::vector<Part> parts(T);
+    for (uint32_t t = 0; t < T; ++t) {
+        parts[t].lo = n * t / T;
+        parts[t].hi = n * (t + 1) / T;
+    }
+    parallel_for(T, T, [&](uint64_t b, uint64_t e, uint32_t) {
+        for (uint64_t t = b; t < e; ++t) {
+            Part& P = parts[t];
+            uint64_t p = P.lo;
+            if (t > 0)
+                while (p < P.hi && !plausible_start(buf, n, p)) ++p;
+            if (p >= P.hi && t > 0) continue;  // no start guessed in this range
+            P.guess = p;
+            P.starts.reserve((P.hi - P.lo) / 48 + 16);  // records average well above 48 bytes
+            while (p < P.hi) {
+                const uint64_t s = record_size(buf, n, p);
+                if (!s) {
+                    P.failed = true;
+                    break;
+                }
+                P.starts.push_back(p);
+                p += s;
+            }
+            P.exit = p;
+        }
+    }, 1);
+    // the true chain enters range t at e: the guess is kept when it is e, the
+    // range is parsed again from e otherwise (or when its chain broke)
+    std::vector<uint64_t> off;
+    uint64_t total = 0;
+    for (auto& P : parts) total += P.starts.size();
+    off.reserve(total + 16);
+    uint64_t e = 0;
+    for (uint32_t t = 0; t < T; ++t) {
+        Part& P = parts[t];
+        if (e >= P.hi) continue;  // a record spans the whole range
+        if (P.guess == e && !P.failed) {
+            off.insert(off.end(), P.starts.begin(), P.starts.end());
+            e = P.exit;
+        } else {
+            while (e < P.hi) {
+                const uint64_t s = record_size(buf, n, e);
+                if (!s) throw Error("stitch: truncated fragment record");
+                off.push_back(e);
+                e += s;
+            }
+        }
+        std::vector<uint64_t>().swap(P.starts);
+    }
+    return off;
+}
+
+std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_t& n) {
+    const int fd = ::open(path, O_RDONLY);
+    if (fd < 0) throw Error(std::string("cannot open ") + path);
+    struct stat sb;
+    if (::fstat(fd, &sb) != 0) {
+        ::close(fd);
+        throw Error(std::string("cannot stat ") + path);
+    }
+    n = uint64_t(sb.st_size);
+    std::unique_ptr<uint8_t[]> buf(new uint8_t[n + 1]);
+    // ranges of at least 64 MiB per thread
+    const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads ? threads : 1, n >> 26));
+    std::atomic<bool> short_read{false};
+    parallel_for(T, T, [&](uint64_t b, uint64_t e, uint32_t) {
+        for (uint64_t t = b; t < e; ++t) {
+            uint64_t p = n * t / T;
+            const uint64_t end = n * (t + 1) / T;
+            while (p < end) {
+                const ssize_t got = ::pread(fd, buf.get() + p, size_t(std::min<uint64_t>(end - p, 1ull << 30)), off_t(p));
+                if (got <= 0) {
+                    short_read = true;
+                    break;
+                }
+                p += uint64_t(got);
+            }
+        }
+    }, 1);
+    ::close(fd);
+    if (short_read) throw Error(std::string("short read of ") + path);
+    return buf;
+}
+
+void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& off, uint32_t k,
+                   const std::vector<uint32_t>& frags, const std::vector<uint32_t>& partner, UnitigList& out,
+                   cdbg_stitch_stats& st) {
+    const uint64_t nc = frags.size();
+    if (partner.size() != 2 * nc) throw Error("stitch: cycle partners do not match the cycle fragments");
+    Stitcher S{buf, n, k, 1, {}, {}, {}, {}};
+    S.off.resize(nc);
+    S.partner.resize(2 * nc);
+    auto local = [&](uint32_t f) -> uint64_t {
+        const auto it = std::lower_bound(frags.begin(), frags.end(), f);
+        if (it == frags.end() || *it != f) throw Error("stitch: an unvisited fragment is not on a cycle");
+        return uint64_t(it - frags.begin());
+    };
+    for (uint64_t l = 0; l < nc; ++l) {
+        if (frags[l] >= off.size() || (l && frags[l] <= frags[l - 1]))
+            throw Error("stitch: cycle fragments out of range or order");
+        S.off[l] = off[frags[l]];
+        for (uint32_t s = 0; s < 2; ++s) {
+            const uint32_t p = partner[2 * l + s];
+            S.partner[2 * l + s] = p == ~0u ? NONE : (local(p >> 1) << 1) | (p & 1u);
+        }
+    }
+    S.used.assign(nc, 0);
+    Tig tig;
+    for (uint64_t l = 0; l < nc; ++l) {
+        if (S.used[l]) continue;
+        if (!S.walk(l, 0, tig, S.used)) throw Error("stitch: an unvisited fragment is not on a cycle");
+        ++st.cycles;
+        cut_cycle(tig, k, out, st);
+    }
+    std::stable_sort(out.items.begin(), out.items.end(),
+                     [](const UnitigList::Item& a, const UnitigList::Item& b) { return a.cid < b.cid; });
+}
 
 void Unitigs::build_order() {
     if (!order.empty() || lists.empty()) return;

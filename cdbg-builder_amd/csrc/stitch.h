@@ -4,6 +4,7 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -40,5 +41,25 @@ struct Unitigs {
 // Stitches the fragment records in buf[0, n) (the frag_unitigs.bin byte
 // stream) with `threads` host threads.
 Unitigs* stitch_fragments(const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads);
+
+// Start of every record of the frag_unitigs.bin stream buf[0, n), parsed by
+// `threads` threads; throws on a record that does not fit.
+std::vector<uint64_t> fragment_offsets(const uint8_t* buf, uint64_t n, uint32_t threads);
+
+// The tigs that close on themselves, over the fragments `frags` (ascending
+// fragment numbers, records at off[]) whose sides pair up among themselves:
+// partner[2 l + s] is the (fragment << 1 | side) joined to side s of frags[l],
+// or ~0u.  Walked in fragment order as the host stitch walks them; the
+// unitigs go to `out`, sorted stably by colour-set id.
+void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& off, uint32_t k,
+                   const std::vector<uint32_t>& frags, const std::vector<uint32_t>& partner, UnitigList& out,
+                   cdbg_stitch_stats& st);
+
+// The whole file at `path`, read by up to `threads` threads; n = its size.
+std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_t& n);
+
+// Phase 3 on the device (stitch_gpu.hip): the same unitigs, in the same
+// order, as stitch_fragments.
+Unitigs* stitch_fragments_gpu(int device, const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads);
 
 }  // namespace cdbg_amd

@@ -1,0 +1,925 @@
+// Phase 3 (stitch) on the device: the unitigs of the host stitch
+// (stitch.cpp), in the same order, from sorts, scans and list ranking.
+//
+// Semantics (stitch_extmem.hpp:112-355, the same as the host stitch): open
+// fragment sides are keyed on their canonical boundary k-mer; two sides with
+// one key and compatible orientation join (ext_pair_compatible :296-300);
+// the shared k-mer of a join is kept once with the colour of the side that
+// owns it (ext_concat_runs :170-226); tigs are cut at colour-set changes and
+// foreign placeholder runs at their ends are dropped (ext_split_monochromatic
+// :112-151).
+//
+// Method.  A fragment side s = fragment << 1 | side; entering fragment i at
+// side s and leaving it by s ^ 1 continues into partner[s ^ 1], so
+// next(s) = partner[s ^ 1] is injective and its graph is paths and cycles.
+//   1. one thread per fragment checks the record (length, run coverage),
+//      keeps its first/last colour ids and emits the canonical key of every
+//      open side;
+//   2. the keys are radix sorted; equal neighbours pair up;
+//   3. pointer jumping over next() gives every side the last side of its
+//      path and its distance to it, until a round resolves no side (what is
+//      left lies on cycles);
+//   4. a fragment is walked in the direction whose free start side is the
+//      smaller of the path's two free sides (the host stitch's rule); its
+//      position is the reverse direction's distance, chains are ordered by
+//      their start side, and a scan of chain lengths places every fragment
+//      at its slot in tig order;
+//   5. per slot: the k-mers, bases and colour runs it adds to its tig (the
+//      first k-mer of a joined fragment is the previous one's last; that
+//      shared k-mer takes the first side's colour unless it is foreign);
+//      scans give the offsets;
+//   6. runs of k-mers with one colour set across a tig are the unitigs;
+//      foreign ones at tig ends are dropped, inside a tig they are an error;
+//   7. a stable radix sort by colour-set id gives the emit order, and two
+//      copies (fragments -> tig sequences -> unitigs) give the bases.
+// Fragments on cycles go back to the host stitch (stitch_cycles), which
+// walks them in fragment order exactly as it would have.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <fcntl.h>
+#include <memory>
+#include <string>
+#include <thread>
+#include <unistd.h>
+#include <vector>
+
+#include "cdbg_gpu.h"
+#include "common.h"
+#include "job.h"
+#include "kmer_ops.h"
+#include "prims.h"
+#include "stitch.h"
+
+namespace cdbg_amd {
+
+namespace {
+
+constexpr int ST_TPB = 256;
+constexpr uint32_t NONE32 = ~0u;
+constexpr uint32_t DONE = 1u << 31;  // jump state: .x is the last side of the path
+constexpr uint32_t PIECE = 4096;     // bytes one wave copies per work item
+
+// error bits, in the order the host stitch meets them
+enum : uint32_t {
+    E_SHORT = 1,
+    E_COVER = 2,
+    E_BASE = 4,
+    E_GROUP = 8,
+    E_REENTER = 16,
+    E_FOREIGN = 32,
+    E_INTERNAL = 64,
+};
+
+inline uint32_t st_blocks(uint64_t n, uint32_t t = ST_TPB) {
+    const uint64_t b = (n + t - 1) / t;
+    if (b >= (1ull << 31)) throw Error("stitch: launch too large");
+    return uint32_t(b ? b : 1);
+}
+
+__device__ __forceinline__ uint32_t ld32(const uint8_t* p) {
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+__device__ __forceinline__ uint64_t ld64(const uint8_t* p) {
+    uint64_t v;
+    __builtin_memcpy(&v, p, 8);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t st_lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// wave-aggregated atomic add of one lane's v
+__device__ __forceinline__ void wave_add(unsigned long long* ctr, uint64_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+}
+
+__device__ __forceinline__ uint64_t wave_slot(unsigned long long* ctr, bool want) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t m = __ballot(want);
+    if (!m) return 0;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    unsigned long long base = 0;
+    if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
+    base = __shfl(base, leader, 64);
+    return base + __popcll(m & st_lanes_below(lane));
+}
+
+__device__ __forceinline__ int code_of(uint8_t c) {
+    return c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : -1;
+}
+__device__ __forceinline__ uint8_t comp_of(uint8_t c) {
+    return c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
+}
+
+template <class K>
+__device__ __forceinline__ K push_base(const K& x, uint32_t c);
+template <>
+__device__ __forceinline__ uint64_t push_base<uint64_t>(const uint64_t& x, uint32_t c) {
+    return (x << 2) | c;
+}
+template <>
+__device__ __forceinline__ Key128 push_base<Key128>(const Key128& x, uint32_t c) {
+    return Key128{(x.lo << 2) | c, (x.hi << 2) | (x.lo >> 62)};
+}
+__device__ __forceinline__ bool key_le(uint64_t a, uint64_t b) { return a <= b; }
+__device__ __forceinline__ bool key_le(const Key128& a, const Key128& b) { return !(b < a); }
+
+// 1. Per fragment: checks, colour ids, keys of the open sides.
+template <class K>
+__global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t* __restrict__ buf,
+                                                     const uint64_t* __restrict__ off, uint32_t k,
+                                                     uint32_t* __restrict__ f_len, uint32_t* __restrict__ f_nr,
+                                                     uint64_t* __restrict__ f_cid, K* __restrict__ ekey,
+                                                     uint32_t* __restrict__ eid, unsigned long long* __restrict__ ctr,
+                                                     uint32_t* __restrict__ err) {
+    const uint64_t i = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    bool live = i < nf;
+    uint8_t open = 0;
+    uint32_t len = 0, nr = 0;
+    const uint8_t* rec = nullptr;
+    if (live) {
+        rec = buf + off[i];
+        open = rec[0];
+        nr = ld32(rec + 1);
+        len = ld32(rec + 5 + 12 * uint64_t(nr));
+        f_len[i] = len;
+        f_nr[i] = nr;
+        uint32_t bad = 0;
+        if (len < k) {
+            bad = E_SHORT;
+        } else {
+            // colour ids: first and last run, first and last run with k-mers
+            uint64_t sum = 0, c0 = 0, cl = 0, p0 = COLOR_FOREIGN, pl = COLOR_FOREIGN;
+            bool any = false;
+            for (uint32_t r = 0; r < nr; ++r) {
+                const uint64_t cid = ld64(rec + 5 + 12 * uint64_t(r));
+                const uint32_t n = ld32(rec + 13 + 12 * uint64_t(r));
+                if (r == 0) c0 = cid;
+                cl = cid;
+                if (n) {
+                    if (!any) p0 = cid;
+                    pl = cid;
+                    any = true;
+                }
+                sum += n;
+            }
+            if (sum != uint64_t(len) - k + 1) bad = E_COVER;
+            f_cid[4 * i + 0] = c0;
+            f_cid[4 * i + 1] = cl;
+            f_cid[4 * i + 2] = p0;
+            f_cid[4 * i + 3] = pl;
+        }
+        if (bad) {
+            atomicOr(err, bad);
+            open = 0;
+        }
+    }
+    // open sides: canonical boundary k-mer, and whether the forward word is it
+    for (uint32_t side = 0; side < 2; ++side) {
+        const bool want = live && (open & (side ? 2u : 1u));
+        K key{};
+        bool fwd = false;
+        if (want) {
+            const uint8_t* s = rec + 9 + 12 * uint64_t(nr) + (side ? len - k : 0);
+            K f{}, r{};
+            int bad = 0;
+            for (uint32_t j = 0; j < k; ++j) {
+                const int c = code_of(s[j]), d = code_of(s[k - 1 - j]);
+                bad |= c | d;
+                f = push_base<K>(f, uint32_t(c & 3));
+                r = push_base<K>(r, uint32_t(3 - (d & 3)));
+            }
+            if (bad < 0) atomicOr(err, E_BASE);
+            fwd = key_le(f, r);
+            key = fwd ? f : r;
+        }
+        const uint64_t at = wave_slot(&ctr[0], want);
+        if (want) {
+            ekey[at] = key;
+            eid[at] = uint32_t(i << 2) | (side << 1) | uint32_t(fwd);
+        }
+    }
+}
+
+template <class K>
+__device__ __forceinline__ bool key_eq(const K& a, const K& b) {
+    return a == b;
+}
+
+// 2. Equal keys pair up (counters: 1 joins, 2 unmatched ends).
+template <class K>
+__global__ __launch_bounds__(ST_TPB) void k_st_pair(uint64_t ne, const K* __restrict__ key,
+                                                    const uint32_t* __restrict__ id, uint32_t* __restrict__ partner,
+                                                    unsigned long long* __restrict__ ctr, uint32_t* __restrict__ err) {
+    const uint64_t j = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    uint64_t joins = 0, unmatched = 0;
+    if (j < ne && (j == 0 || !key_eq(key[j], key[j - 1]))) {
+        const K x = key[j];
+        const bool two = j + 1 < ne && key_eq(key[j + 1], x);
+        if (two && j + 2 < ne && key_eq(key[j + 2], x)) atomicOr(err, E_GROUP);
+        bool paired = false;
+        if (two) {
+            const uint32_t a = id[j], b = id[j + 1];
+            const uint32_t sa = (a >> 1) & 1u, sb = (b >> 1) & 1u, fa = a & 1u, fb = b & 1u;
+            const bool ok = sa != sb ? fa == fb : fa != fb;
+            if (ok && (a >> 1) != (b >> 1)) {
+                partner[a >> 1] = b >> 1;
+                partner[b >> 1] = a >> 1;
+                paired = true;
+                joins = 1;
+            }
+        }
+        if (!paired) unmatched = two ? 2 : 1;
+    }
+    wave_add(&ctr[1], joins);
+    wave_add(&ctr[2], unmatched);
+}
+
+// 3. Pointer jumping over next(s) = partner[s ^ 1].
+__global__ __launch_bounds__(ST_TPB) void k_st_jump_init(uint64_t ns, const uint32_t* __restrict__ partner,
+                                                         uint2* __restrict__ S) {
+    const uint64_t s = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (s >= ns) return;
+    const uint32_t nx = partner[s ^ 1];
+    S[s] = nx == NONE32 ? make_uint2(uint32_t(s) | DONE, 0u) : make_uint2(nx, 1u);
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_jump(uint64_t ns, const uint2* __restrict__ A, uint2* __restrict__ B,
+                                                    unsigned long long* __restrict__ open_ctr) {
+    const uint64_t s = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    bool open = false;
+    if (s < ns) {
+        uint2 v = A[s];
+        if (!(v.x & DONE)) {
+            const uint2 w = A[v.x];
+            v.x = w.x;
+            v.y += w.y;
+            open = !(w.x & DONE);
+        }
+        B[s] = v;
+    }
+    const uint64_t m = __ballot(open);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(open_ctr, (unsigned long long)__popcll(m));
+}
+
+// 4. The walked direction of every fragment and its place.
+__global__ __launch_bounds__(ST_TPB) void k_st_choose(uint64_t nf, const uint2* __restrict__ S,
+                                                      uint32_t* __restrict__ ch_side, uint32_t* __restrict__ ch_start,
+                                                      uint32_t* __restrict__ ch_pos, uint32_t* __restrict__ chain_len,
+                                                      uint8_t* __restrict__ on_cycle, uint32_t* __restrict__ err) {
+    const uint64_t i = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (i >= nf) return;
+    const uint2 s0 = S[2 * i], s1 = S[2 * i + 1];
+    if (!(s0.x & DONE) || !(s1.x & DONE)) {
+        // a side on a cycle: so is the other (the reverse direction of a cycle is one)
+        if ((s0.x & DONE) || (s1.x & DONE)) atomicOr(err, E_INTERNAL);
+        on_cycle[i] = 1;
+        ch_side[i] = NONE32;
+        return;
+    }
+    on_cycle[i] = 0;
+    const uint32_t end0 = (s0.x & ~DONE) ^ 1u, end1 = (s1.x & ~DONE) ^ 1u;  // free exit side of each direction
+    if (end0 == end1) {
+        atomicOr(err, E_REENTER);  // both directions of one path
+        ch_side[i] = NONE32;
+        return;
+    }
+    // entering at side c, the path starts at the other direction's free end
+    const bool take0 = end1 < end0;
+    const uint32_t c = uint32_t(2 * i) + (take0 ? 0u : 1u);
+    const uint2 sc = take0 ? s0 : s1, so = take0 ? s1 : s0;
+    const uint32_t start = take0 ? end1 : end0;
+    ch_side[i] = c;
+    ch_start[i] = start;
+    ch_pos[i] = so.y;
+    if (so.y == 0) {
+        if (start != c) atomicOr(err, E_INTERNAL);
+        chain_len[c] = sc.y + 1;
+    }
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_place(uint64_t nf, const uint32_t* __restrict__ ch_side,
+                                                     const uint32_t* __restrict__ ch_start,
+                                                     const uint32_t* __restrict__ ch_pos,
+                                                     const uint32_t* __restrict__ chain_off, uint64_t n_lin,
+                                                     uint32_t* __restrict__ slot_side, uint32_t* __restrict__ err) {
+    const uint64_t i = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (i >= nf || ch_side[i] == NONE32) return;
+    const uint64_t slot = uint64_t(chain_off[ch_start[i]]) + ch_pos[i];
+    if (slot >= n_lin) {
+        atomicOr(err, E_INTERNAL);
+        return;
+    }
+    slot_side[slot] = ch_side[i];
+}
+
+// 5. What each slot adds to its tig: k-mers, bases, run entries (nr + 1),
+// and the tig it belongs to (flag of a tig's first slot, scanned).
+__global__ __launch_bounds__(ST_TPB) void k_st_slot_sizes(uint64_t n_lin, const uint32_t* __restrict__ slot_side,
+                                                          const uint32_t* __restrict__ ch_pos,
+                                                          const uint2* __restrict__ S, const uint32_t* __restrict__ f_len,
+                                                          const uint32_t* __restrict__ f_nr, uint32_t k,
+                                                          uint64_t* __restrict__ s_kmers, uint64_t* __restrict__ s_bases,
+                                                          uint64_t* __restrict__ s_runs, uint32_t* __restrict__ s_first,
+                                                          uint32_t* __restrict__ err) {
+    const uint64_t j = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (j >= n_lin) return;
+    const uint32_t s = slot_side[j];
+    if (s == NONE32) {
+        atomicOr(err, E_INTERNAL);
+        s_kmers[j] = s_bases[j] = s_runs[j] = 0;
+        s_first[j] = 0;
+        return;
+    }
+    const uint32_t i = s >> 1;
+    const bool joined = ch_pos[i] > 0, last = S[s].y == 0;
+    const uint64_t len = f_len[i], nk = len - k + 1;
+    const uint64_t m = nk - (joined ? 1 : 0);
+    // a joined fragment that is not the last of its tig adds a k-mer of its own
+    if (joined && !last && m == 0) atomicOr(err, E_INTERNAL);
+    s_kmers[j] = m;
+    s_bases[j] = joined ? len - k : len;
+    s_runs[j] = uint64_t(f_nr[i]) + 1;
+    s_first[j] = joined ? 0u : 1u;
+}
+
+// Run entries of a slot in walk order: the fragment's runs (reversed when it
+// is entered at its right side), less its first k-mer when joined, and its
+// last k-mer recoloured by the next join when that differs; the unused
+// entries carry no k-mers.
+__global__ __launch_bounds__(ST_TPB) void k_st_runs(uint64_t n_lin, const uint32_t* __restrict__ slot_side,
+                                                    const uint32_t* __restrict__ ch_pos, const uint2* __restrict__ S,
+                                                    const uint8_t* __restrict__ buf, const uint64_t* __restrict__ off,
+                                                    const uint32_t* __restrict__ f_nr, const uint64_t* __restrict__ f_cid,
+                                                    const uint64_t* __restrict__ r_off, const uint32_t* __restrict__ s_tig,
+                                                    uint64_t n_ent, uint64_t* __restrict__ e_cid,
+                                                    uint32_t* __restrict__ e_n, uint32_t* __restrict__ e_tig,
+                                                    uint32_t* __restrict__ err) {
+    const uint64_t j = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (j >= n_lin) return;
+    const uint32_t s = slot_side[j];
+    const uint32_t i = s >> 1;
+    const bool rev = s & 1u, joined = ch_pos[i] > 0, last = S[s].y == 0;
+    const uint32_t nr = f_nr[i];
+    const uint8_t* runs = buf + off[i] + 5;
+    const uint64_t base = r_off[j];
+    if (base + nr >= n_ent) {  // the slot's nr + 1 entries
+        atomicOr(err, E_INTERNAL);
+        return;
+    }
+    // s_tig: tig starts before slot j (exclusive count); a joined slot's own
+    // tig started before it
+    const uint32_t tig = joined ? s_tig[j] - 1u : s_tig[j];
+    uint32_t skip = joined ? 1u : 0u;
+    int64_t last_pos = -1;  // the last entry with k-mers
+    for (uint32_t x = 0; x < nr; ++x) {
+        const uint32_t r = rev ? nr - 1 - x : x;
+        const uint64_t cid = ld64(runs + 12 * uint64_t(r));
+        const uint32_t n = ld32(runs + 12 * uint64_t(r) + 8);
+        const uint32_t take = n > skip ? n - skip : 0u;
+        skip = n > skip ? 0u : skip - n;
+        e_cid[base + x] = cid;
+        e_n[base + x] = take;
+        e_tig[base + x] = tig;
+        if (take) last_pos = int64_t(x);
+    }
+    uint64_t extra_cid = 0;
+    uint32_t extra_n = 0;
+    if (!last && last_pos >= 0) {
+        // the shared k-mer with the next fragment: this side's colour unless
+        // foreign, else the next fragment's first (walk order) run's
+        const uint32_t t = slot_side[j + 1];
+        const uint64_t mine = f_cid[4 * uint64_t(i) + (rev ? 2 : 3)];
+        const uint64_t theirs = f_cid[4 * uint64_t(t >> 1) + ((t & 1u) ? 1 : 0)];
+        const uint64_t real = mine != COLOR_FOREIGN ? mine : theirs;
+        if (real != mine) {
+            e_n[base + uint64_t(last_pos)] -= 1;
+            extra_cid = real;
+            extra_n = 1;
+        }
+    }
+    e_cid[base + nr] = extra_cid;
+    e_n[base + nr] = extra_n;
+    e_tig[base + nr] = tig;
+}
+
+// 6. Unitig heads among the entries with k-mers (compacted, in tig order):
+// the first entry, a new tig or a new colour id.
+__global__ __launch_bounds__(ST_TPB) void k_st_heads(uint64_t nk_e, const uint32_t* __restrict__ list,
+                                                     const uint64_t* __restrict__ e_cid,
+                                                     const uint32_t* __restrict__ e_tig, uint8_t* __restrict__ head) {
+    const uint64_t q = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (q >= nk_e) return;
+    const uint32_t a = list[q];
+    bool h = q == 0;
+    if (!h) {
+        const uint32_t b = list[q - 1];
+        h = e_tig[a] != e_tig[b] || e_cid[a] != e_cid[b];
+    }
+    head[q] = h ? 1 : 0;
+}
+
+// Unitig candidates (one per head): colour id, first k-mer, k-mers, tig;
+// foreign ones at a tig's ends are dropped, inside a tig they are an error.
+__global__ __launch_bounds__(ST_TPB) void k_st_unitigs(uint64_t n_cand, uint64_t nk_e, const uint32_t* __restrict__ hlist,
+                                                       const uint32_t* __restrict__ list,
+                                                       const uint64_t* __restrict__ e_cid,
+                                                       const uint32_t* __restrict__ e_tig,
+                                                       const uint64_t* __restrict__ e_x, uint64_t total_kmers,
+                                                       uint64_t* __restrict__ u_cid, uint64_t* __restrict__ u_x,
+                                                       uint64_t* __restrict__ u_n, uint32_t* __restrict__ u_tig,
+                                                       uint8_t* __restrict__ keep,
+                                                       unsigned long long* __restrict__ ctr,
+                                                       uint32_t* __restrict__ err) {
+    const uint64_t u = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    uint64_t dropped = 0, kept_kmers = 0;
+    if (u < n_cand) {
+        const uint64_t q = hlist[u];
+        const uint32_t a = list[q];
+        const uint64_t x = e_x[q];
+        const uint64_t x1 = u + 1 < n_cand ? e_x[hlist[u + 1]] : total_kmers;
+        const uint64_t cid = e_cid[a];
+        const uint32_t tig = e_tig[a];
+        const bool first = u == 0 || e_tig[list[hlist[u - 1]]] != tig;
+        const bool lastu = u + 1 == n_cand || e_tig[list[hlist[u + 1]]] != tig;
+        bool k_ = true;
+        if (cid == COLOR_FOREIGN) {
+            k_ = false;
+            if (first || lastu)
+                dropped = x1 - x;
+            else
+                atomicOr(err, E_FOREIGN);
+        } else {
+            kept_kmers = x1 - x;
+        }
+        u_cid[u] = cid;
+        u_x[u] = x;
+        u_n[u] = x1 - x;
+        u_tig[u] = tig;
+        keep[u] = k_ ? 1 : 0;
+    }
+    wave_add(&ctr[3], dropped);
+    wave_add(&ctr[4], kept_kmers);
+}
+
+// 7. Emit order: kept unitigs (tig order) with their colour ids as sort keys.
+__global__ __launch_bounds__(ST_TPB) void k_st_keys(uint64_t nu, const uint32_t* __restrict__ klist,
+                                                    const uint64_t* __restrict__ u_cid, uint64_t* __restrict__ key,
+                                                    uint32_t* __restrict__ val, unsigned long long* __restrict__ cmax) {
+    const uint64_t t = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    uint64_t c = 0;
+    if (t < nu) {
+        c = u_cid[klist[t]];
+        key[t] = c;
+        val[t] = klist[t];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t y = __shfl_xor(c, o, 64);
+        c = y > c ? y : c;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(cmax, (unsigned long long)c);
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_out_len(uint64_t nu, const uint32_t* __restrict__ ord,
+                                                       const uint64_t* __restrict__ u_n, uint32_t k,
+                                                       uint64_t* __restrict__ olen) {
+    const uint64_t t = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (t < nu) olen[t] = u_n[ord[t]] + k - 1;
+}
+
+// The host's unitig items {cid, offset, length} in emit order.
+__global__ __launch_bounds__(ST_TPB) void k_st_items(uint64_t nu, const uint32_t* __restrict__ ord,
+                                                     const uint64_t* __restrict__ u_cid, const uint64_t* __restrict__ u_n,
+                                                     const uint64_t* __restrict__ ooff, uint32_t k,
+                                                     uint64_t* __restrict__ items) {
+    const uint64_t t = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (t >= nu) return;
+    const uint32_t u = ord[t];
+    items[3 * t + 0] = u_cid[u];
+    items[3 * t + 1] = ooff[t];
+    items[3 * t + 2] = u_n[u] + k - 1;
+}
+
+// Work items of PIECE bytes: item_of[piece] = the element it copies from.
+__global__ __launch_bounds__(ST_TPB) void k_st_pieces(uint64_t n, const uint64_t* __restrict__ bytes_scan,
+                                                      uint64_t total, const uint64_t* __restrict__ piece_off,
+                                                      uint32_t* __restrict__ item_of) {
+    const uint64_t e = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t b = (e + 1 < n ? bytes_scan[e + 1] : total) - bytes_scan[e];
+    const uint64_t p0 = piece_off[e], np = (b + PIECE - 1) / PIECE;
+    for (uint64_t p = 0; p < np; ++p) item_of[p0 + p] = uint32_t(e);
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_piece_count(uint64_t n, const uint64_t* __restrict__ bytes_scan,
+                                                           uint64_t total, uint64_t* __restrict__ np) {
+    const uint64_t e = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (e >= n) return;
+    const uint64_t b = (e + 1 < n ? bytes_scan[e + 1] : total) - bytes_scan[e];
+    np[e] = (b + PIECE - 1) / PIECE;
+}
+
+// Tig sequences: slot j's bases (reverse complement when entered at the
+// right side; without the shared k bases when joined) at s_boff[j].
+__global__ __launch_bounds__(ST_TPB) void k_st_fill_tigs(uint64_t n_pieces, const uint32_t* __restrict__ item_of,
+                                                         const uint64_t* __restrict__ piece_off,
+                                                         const uint32_t* __restrict__ slot_side,
+                                                         const uint32_t* __restrict__ ch_pos,
+                                                         const uint8_t* __restrict__ buf,
+                                                         const uint64_t* __restrict__ off,
+                                                         const uint32_t* __restrict__ f_nr,
+                                                         const uint32_t* __restrict__ f_len,
+                                                         const uint64_t* __restrict__ s_boff, uint32_t k,
+                                                         uint64_t total_tig, uint8_t* __restrict__ tig,
+                                                         uint32_t* __restrict__ err) {
+    const uint64_t w = (uint64_t(blockIdx.x) * ST_TPB + threadIdx.x) >> 6;
+    if (w >= n_pieces) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t j = item_of[w];
+    const uint32_t s = slot_side[j], i = s >> 1;
+    const bool rev = s & 1u;
+    const uint32_t len = f_len[i];
+    const uint32_t from = ch_pos[i] > 0 ? k : 0u;
+    const uint8_t* seq = buf + off[i] + 9 + 12 * uint64_t(f_nr[i]);
+    const uint64_t q0 = (w - piece_off[j]) * PIECE;
+    const uint64_t q1 = uint64_t(len - from) < q0 + PIECE ? uint64_t(len - from) : q0 + PIECE;
+    if (len < from || s_boff[j] + (len - from) > total_tig) {
+        if (lane == 0) atomicOr(err, E_INTERNAL);
+        return;
+    }
+    uint8_t* dst = tig + s_boff[j];
+    for (uint64_t q = q0 + lane; q < q1; q += 64) {
+        const uint64_t t = from + q;
+        dst[q] = rev ? comp_of(seq[len - 1 - t]) : seq[t];
+    }
+}
+
+// Unitig bases, in emit order, from the tig sequences.
+__global__ __launch_bounds__(ST_TPB) void k_st_copy_out(uint64_t n_pieces, const uint32_t* __restrict__ item_of,
+                                                        const uint64_t* __restrict__ piece_off, uint64_t nu,
+                                                        const uint32_t* __restrict__ ord,
+                                                        const uint64_t* __restrict__ u_x,
+                                                        const uint32_t* __restrict__ u_tig,
+                                                        const uint64_t* __restrict__ ooff, uint64_t total_out,
+                                                        uint32_t k, const uint8_t* __restrict__ tig,
+                                                        uint64_t total_tig, uint8_t* __restrict__ out,
+                                                        uint32_t* __restrict__ err) {
+    const uint64_t w = (uint64_t(blockIdx.x) * ST_TPB + threadIdx.x) >> 6;
+    if (w >= n_pieces) return;
+    const int lane = threadIdx.x & 63;
+    const uint32_t t = item_of[w];
+    const uint32_t u = ord[t];
+    const uint64_t from = u_x[u] + uint64_t(k - 1) * u_tig[u];  // the tig base of the unitig's first k-mer
+    const uint64_t o0 = ooff[t];
+    const uint64_t len = (t + 1 < nu ? ooff[t + 1] : total_out) - o0;
+    if (from > total_tig || len > total_tig - from || o0 + len > total_out) {
+        if (lane == 0) atomicOr(err, E_INTERNAL);
+        return;
+    }
+    const uint8_t* src = tig + from;
+    const uint64_t q0 = (w - piece_off[t]) * PIECE;
+    const uint64_t q1 = len < q0 + PIECE ? len : q0 + PIECE;
+    for (uint64_t q = q0 + lane; q < q1; q += 64) out[o0 + q] = src[q];
+}
+
+__global__ void k_st_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_gather_n(uint64_t n, const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ e_n, uint64_t* __restrict__ g) {
+    const uint64_t q = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (q < n) g[q] = e_n[list[q]];
+}
+
+// The partners of both sides of every fragment on a cycle.
+__global__ __launch_bounds__(ST_TPB) void k_st_cycle_partners(uint64_t nc, const uint32_t* __restrict__ list,
+                                                              const uint32_t* __restrict__ partner,
+                                                              uint32_t* __restrict__ out) {
+    const uint64_t q = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (q >= nc) return;
+    const uint64_t i = list[q];
+    out[2 * q] = partner[2 * i];
+    out[2 * q + 1] = partner[2 * i + 1];
+}
+
+template <class T>
+T st_d2h(const void* d, hipStream_t st) {
+    T v;
+    CDBG_HIP(hipMemcpyAsync(&v, d, sizeof(T), hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    return v;
+}
+
+void throw_on(uint32_t e) {
+    if (e & E_SHORT) throw Error("stitch: a fragment is shorter than k");
+    if (e & E_COVER) throw Error("stitch: colour runs do not cover a fragment's k-mers");
+    if (e & E_BASE) throw Error("stitch: non-ACGT base in a fragment sequence");
+    if (e & E_GROUP) throw Error("stitch: more than two open fragment ends share a boundary k-mer");
+    if (e & E_REENTER) throw Error("stitch: a chain re-enters a fragment");
+    if (e & E_FOREIGN) throw Error("stitch: a foreign k-mer inside a tig (its owner never joined)");
+    if (e) throw Error("stitch: inconsistent fragment chains on the device");
+}
+
+int st_bits_for(uint64_t v) {
+    int b = 0;
+    while (b < 64 && (v >> b)) ++b;
+    return b;
+}
+
+// Work items of PIECE bytes over elements whose byte offsets are `scan`
+// (exclusive, `total` in all): item_of[p] = element of piece p, piece_off[e]
+// = first piece of element e.  Returns the number of pieces.
+uint64_t make_pieces(uint64_t n, const uint64_t* scan, uint64_t total, DevBuf& np, DevBuf& poff, DevBuf& items,
+                     DevBuf& tot, DevBuf& tmp, hipStream_t st) {
+    uint64_t* d_np = np.ensure<uint64_t>(n);
+    uint64_t* d_po = poff.ensure<uint64_t>(n);
+    uint64_t* d_tot = tot.ensure<uint64_t>(1);
+    { KSCOPE("k_st_piece_count", st); k_st_piece_count<<<st_blocks(n), ST_TPB, 0, st>>>(n, scan, total, d_np); }
+    exclusive_scan<uint64_t, uint64_t>(d_np, d_po, n, d_tot, tmp, st);
+    const uint64_t P = st_d2h<uint64_t>(d_tot, st);
+    if (P >= (1ull << 32) || P * 64 / ST_TPB >= (1ull << 31)) throw Error("stitch: too many copy pieces");
+    uint32_t* d_items = items.ensure<uint32_t>(P + 1);
+    { KSCOPE("k_st_pieces", st); k_st_pieces<<<st_blocks(n), ST_TPB, 0, st>>>(n, scan, total, d_po, d_items); }
+    return P;
+}
+
+template <class K>
+Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, const std::vector<uint64_t>& off, uint32_t k) {
+    std::unique_ptr<Unitigs> U(new Unitigs());
+    U->k = k;
+    U->lists.resize(2);  // the linear tigs' unitigs, then the cycles'
+    cdbg_stitch_stats& S = U->stats;
+    const uint64_t nf = off.size();
+    S.num_fragments = nf;
+    if (nf == 0) return U.release();
+    if (nf >= (1ull << 30)) throw Error("stitch: 2^30 or more fragments for the device stitch");
+    const uint64_t ns = 2 * nf;
+    DevBuf b_buf, b_off, b_len, b_nr, b_cid, b_ekey, b_eid, b_ekey_t, b_eid_t, b_ctr, b_err, b_partner, b_A, b_B;
+    DevBuf b_chs, b_chst, b_chp, b_clen, b_coff, b_cyc, b_tot, b_tmp, b_cnt, b_slot;
+    RadixScratch rs;
+    uint8_t* buf = b_buf.ensure<uint8_t>(n + 16);
+    uint64_t* d_off = b_off.ensure<uint64_t>(nf);
+    CDBG_HIP(hipMemcpyAsync(buf, hbuf, n, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipMemcpyAsync(d_off, off.data(), nf * 8, hipMemcpyHostToDevice, st));
+    unsigned long long* ctr = b_ctr.ensure<unsigned long long>(8);
+    uint32_t* err = b_err.ensure<uint32_t>(1);
+    CDBG_HIP(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), st));
+    CDBG_HIP(hipMemsetAsync(err, 0, 4, st));
+    auto check = [&]() { throw_on(st_d2h<uint32_t>(err, st)); };
+
+    // 1-2. keys of the open sides, sorted; equal keys pair up
+    uint32_t* f_len = b_len.ensure<uint32_t>(nf);
+    uint32_t* f_nr = b_nr.ensure<uint32_t>(nf);
+    uint64_t* f_cid = b_cid.ensure<uint64_t>(4 * nf);
+    K* ekey = b_ekey.ensure<K>(ns);
+    uint32_t* eid = b_eid.ensure<uint32_t>(ns);
+    { KSCOPE("k_st_frags", st); k_st_frags<K><<<st_blocks(nf), ST_TPB, 0, st>>>(nf, buf, d_off, k, f_len, f_nr, f_cid, ekey, eid, ctr, err); }
+    check();
+    const uint64_t ne = st_d2h<unsigned long long>(ctr, st);
+    S.open_ends = ne;
+    uint32_t* partner = b_partner.ensure<uint32_t>(ns);
+    { KSCOPE("k_st_fill32", st); k_st_fill32<<<st_blocks(ns), ST_TPB, 0, st>>>(partner, ns, NONE32); }
+    if (ne) {
+        SortPlan plan;
+        plan.build({0, ne});
+        segmented_radix_sort<K, uint32_t>(ekey, eid, b_ekey_t.ensure<K>(ne), b_eid_t.ensure<uint32_t>(ne), plan, 0,
+                                          int(2 * k), 8, rs, st, "stitch_ends");
+        { KSCOPE("k_st_pair", st); k_st_pair<K><<<st_blocks(ne), ST_TPB, 0, st>>>(ne, ekey, eid, partner, ctr, err); }
+        check();
+    }
+    b_ekey_t.release();
+    b_eid_t.release();
+
+    // 3. list ranking over next(s) = partner[s ^ 1]
+    uint2* A = b_A.ensure<uint2>(ns);
+    uint2* B = b_B.ensure<uint2>(ns);
+    { KSCOPE("k_st_jump_init", st); k_st_jump_init<<<st_blocks(ns), ST_TPB, 0, st>>>(ns, partner, A); }
+    uint64_t prev = ~0ull;
+    for (int r = 0; r < 64; ++r) {
+        CDBG_HIP(hipMemsetAsync(ctr + 5, 0, sizeof(unsigned long long), st));
+        { KSCOPE("k_st_jump", st); k_st_jump<<<st_blocks(ns), ST_TPB, 0, st>>>(ns, A, B, ctr + 5); }
+        std::swap(A, B);
+        const uint64_t open = st_d2h<unsigned long long>(ctr + 5, st);
+        // a round that resolves no side leaves only sides on cycles
+        if (open == 0 || open == prev) break;
+        prev = open;
+    }
+
+    // 4. walked direction, chain starts and lengths, slots in tig order
+    uint32_t* ch_side = b_chs.ensure<uint32_t>(nf);
+    uint32_t* ch_start = b_chst.ensure<uint32_t>(nf);
+    uint32_t* ch_pos = b_chp.ensure<uint32_t>(nf);
+    uint32_t* chain_len = b_clen.ensure<uint32_t>(ns);
+    uint8_t* on_cycle = b_cyc.ensure<uint8_t>(nf);
+    { KSCOPE("k_st_fill32", st); k_st_fill32<<<st_blocks(ns), ST_TPB, 0, st>>>(chain_len, ns, 0u); }
+    { KSCOPE("k_st_choose", st); k_st_choose<<<st_blocks(nf), ST_TPB, 0, st>>>(nf, A, ch_side, ch_start, ch_pos, chain_len, on_cycle, err); }
+    check();
+    uint32_t* chain_off = b_coff.ensure<uint32_t>(ns);
+    uint32_t* d_tot32 = b_tot.ensure<uint32_t>(2);
+    exclusive_scan<uint32_t, uint32_t>(chain_len, chain_off, ns, d_tot32, b_tmp, st);
+    const uint64_t n_lin = st_d2h<uint32_t>(d_tot32, st);
+    if (n_lin > nf) throw_on(E_INTERNAL);
+    b_clen.release();
+
+    // host: the fragments on cycles
+    std::vector<uint32_t> cyc_frags, cyc_partner;
+    if (n_lin < nf) {
+        DevBuf b_list, b_pp;
+        uint32_t* list = b_list.ensure<uint32_t>(nf);
+        compact_flags<uint8_t>(on_cycle, nf, list, d_tot32, b_cnt, b_tmp, st);
+        const uint64_t nc = st_d2h<uint32_t>(d_tot32, st);
+        if (nc != nf - n_lin) throw_on(E_INTERNAL);
+        uint32_t* pp = b_pp.ensure<uint32_t>(2 * nc);
+        { KSCOPE("k_st_cycle_partners", st); k_st_cycle_partners<<<st_blocks(nc), ST_TPB, 0, st>>>(nc, list, partner, pp); }
+        cyc_frags.resize(nc);
+        cyc_partner.resize(2 * nc);
+        CDBG_HIP(hipMemcpyAsync(cyc_frags.data(), list, nc * 4, hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipMemcpyAsync(cyc_partner.data(), pp, 2 * nc * 4, hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+    }
+    b_partner.release();
+
+    if (n_lin) {
+        uint32_t* slot_side = b_slot.ensure<uint32_t>(n_lin);
+        { KSCOPE("k_st_fill32", st); k_st_fill32<<<st_blocks(n_lin), ST_TPB, 0, st>>>(slot_side, n_lin, NONE32); }
+        { KSCOPE("k_st_place", st); k_st_place<<<st_blocks(nf), ST_TPB, 0, st>>>(nf, ch_side, ch_start, ch_pos, chain_off, n_lin, slot_side, err); }
+
+        // 5. what each slot adds; offsets
+        DevBuf b_sk, b_sb, b_sr, b_sf, b_sko, b_sbo, b_sro, b_sfo, b_t64;
+        uint64_t* s_kmers = b_sk.ensure<uint64_t>(n_lin);
+        uint64_t* s_bases = b_sb.ensure<uint64_t>(n_lin);
+        uint64_t* s_runs = b_sr.ensure<uint64_t>(n_lin);
+        uint32_t* s_first = b_sf.ensure<uint32_t>(n_lin);
+        { KSCOPE("k_st_slot_sizes", st); k_st_slot_sizes<<<st_blocks(n_lin), ST_TPB, 0, st>>>(n_lin, slot_side, ch_pos, A, f_len, f_nr, k, s_kmers, s_bases, s_runs, s_first, err); }
+        check();
+        uint64_t* t64 = b_t64.ensure<uint64_t>(4);
+        uint64_t* s_boff = b_sbo.ensure<uint64_t>(n_lin);
+        uint64_t* r_off = b_sro.ensure<uint64_t>(n_lin);
+        uint32_t* s_tig = b_sfo.ensure<uint32_t>(n_lin);
+        exclusive_scan<uint64_t, uint64_t>(s_bases, s_boff, n_lin, t64 + 0, b_tmp, st);
+        exclusive_scan<uint64_t, uint64_t>(s_runs, r_off, n_lin, t64 + 1, b_tmp, st);
+        exclusive_scan<uint32_t, uint32_t>(s_first, s_tig, n_lin, d_tot32, b_tmp, st);
+        uint64_t h64[2];
+        CDBG_HIP(hipMemcpyAsync(h64, t64, 16, hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+        const uint64_t total_tig = h64[0], n_ent = h64[1];
+        if (n_ent >= (1ull << 32)) throw Error("stitch: 2^32 or more colour runs for the device stitch");
+        b_sk.release();
+        b_sr.release();
+
+        // run entries in tig order; those with k-mers, their first k-mers
+        DevBuf b_ec, b_en, b_et, b_list, b_g, b_x, b_head, b_hlist;
+        uint64_t* e_cid = b_ec.ensure<uint64_t>(n_ent);
+        uint32_t* e_n = b_en.ensure<uint32_t>(n_ent);
+        uint32_t* e_tig = b_et.ensure<uint32_t>(n_ent);
+        { KSCOPE("k_st_runs", st); k_st_runs<<<st_blocks(n_lin), ST_TPB, 0, st>>>(n_lin, slot_side, ch_pos, A, buf, d_off, f_nr, f_cid, r_off, s_tig, n_ent, e_cid, e_n, e_tig, err); }
+        check();
+        uint32_t* list = b_list.ensure<uint32_t>(n_ent);
+        compact_flags<uint32_t>(e_n, n_ent, list, d_tot32, b_cnt, b_tmp, st);
+        const uint64_t nk_e = st_d2h<uint32_t>(d_tot32, st);
+        if (nk_e == 0) throw_on(E_INTERNAL);
+        uint64_t* g = b_g.ensure<uint64_t>(nk_e);
+        uint64_t* e_x = b_x.ensure<uint64_t>(nk_e);
+        { KSCOPE("k_st_gather_n", st); k_st_gather_n<<<st_blocks(nk_e), ST_TPB, 0, st>>>(nk_e, list, e_n, g); }
+        exclusive_scan<uint64_t, uint64_t>(g, e_x, nk_e, t64 + 2, b_tmp, st);
+        const uint64_t total_kmers = st_d2h<uint64_t>(t64 + 2, st);
+        b_g.release();
+        uint8_t* head = b_head.ensure<uint8_t>(nk_e);
+        { KSCOPE("k_st_heads", st); k_st_heads<<<st_blocks(nk_e), ST_TPB, 0, st>>>(nk_e, list, e_cid, e_tig, head); }
+        uint32_t* hlist = b_hlist.ensure<uint32_t>(nk_e);
+        compact_flags<uint8_t>(head, nk_e, hlist, d_tot32, b_cnt, b_tmp, st);
+        const uint64_t n_cand = st_d2h<uint32_t>(d_tot32, st);
+
+        // 6. unitigs of the linear tigs
+        DevBuf b_uc, b_ux, b_un, b_ut, b_keep, b_klist;
+        uint64_t* u_cid = b_uc.ensure<uint64_t>(n_cand);
+        uint64_t* u_x = b_ux.ensure<uint64_t>(n_cand);
+        uint64_t* u_n = b_un.ensure<uint64_t>(n_cand);
+        uint32_t* u_tig = b_ut.ensure<uint32_t>(n_cand);
+        uint8_t* keep = b_keep.ensure<uint8_t>(n_cand);
+        { KSCOPE("k_st_unitigs", st); k_st_unitigs<<<st_blocks(n_cand), ST_TPB, 0, st>>>(n_cand, nk_e, hlist, list, e_cid, e_tig, e_x, total_kmers, u_cid, u_x, u_n, u_tig, keep, ctr, err); }
+        check();
+        b_ec.release();
+        b_en.release();
+        b_et.release();
+        b_list.release();
+        b_x.release();
+        b_head.release();
+        b_hlist.release();
+        uint32_t* klist = b_klist.ensure<uint32_t>(n_cand);
+        compact_flags<uint8_t>(keep, n_cand, klist, d_tot32, b_cnt, b_tmp, st);
+        const uint64_t nu = st_d2h<uint32_t>(d_tot32, st);
+
+        // 7. emit order (stable by colour id), items, bases
+        DevBuf b_key, b_val, b_key_t, b_val_t, b_olen, b_ooff, b_items, b_tig, b_out, b_np, b_po, b_it;
+        UnitigList& L = U->lists[0];
+        if (nu) {
+            uint64_t* key = b_key.ensure<uint64_t>(nu);
+            uint32_t* ord = b_val.ensure<uint32_t>(nu);
+            CDBG_HIP(hipMemsetAsync(ctr + 6, 0, sizeof(unsigned long long), st));
+            { KSCOPE("k_st_keys", st); k_st_keys<<<st_blocks(nu), ST_TPB, 0, st>>>(nu, klist, u_cid, key, ord, ctr + 6); }
+            const int cbits = st_bits_for(st_d2h<unsigned long long>(ctr + 6, st));
+            if (cbits) {
+                SortPlan plan;
+                plan.build({0, nu});
+                segmented_radix_sort<uint64_t, uint32_t>(key, ord, b_key_t.ensure<uint64_t>(nu), b_val_t.ensure<uint32_t>(nu),
+                                                         plan, 0, cbits, 8, rs, st, "stitch_cids");
+            }
+            uint64_t* olen = b_olen.ensure<uint64_t>(nu);
+            uint64_t* ooff = b_ooff.ensure<uint64_t>(nu);
+            { KSCOPE("k_st_out_len", st); k_st_out_len<<<st_blocks(nu), ST_TPB, 0, st>>>(nu, ord, u_n, k, olen); }
+            exclusive_scan<uint64_t, uint64_t>(olen, ooff, nu, t64 + 3, b_tmp, st);
+            const uint64_t total_out = st_d2h<uint64_t>(t64 + 3, st);
+            uint64_t* items = b_items.ensure<uint64_t>(3 * nu);
+            { KSCOPE("k_st_items", st); k_st_items<<<st_blocks(nu), ST_TPB, 0, st>>>(nu, ord, u_cid, u_n, ooff, k, items); }
+
+            // fragments -> tig sequences
+            uint8_t* tig = b_tig.ensure<uint8_t>(total_tig + 16);
+            DevBuf b_ptot;
+            const uint64_t P1 = make_pieces(n_lin, s_boff, total_tig, b_np, b_po, b_it, b_ptot, b_tmp, st);
+            { KSCOPE("k_st_fill_tigs", st); k_st_fill_tigs<<<st_blocks(P1 * 64), ST_TPB, 0, st>>>(P1, b_it.as<uint32_t>(), b_po.as<uint64_t>(), slot_side, ch_pos, buf, d_off, f_nr, f_len, s_boff, k, total_tig, tig, err); }
+            // tig sequences -> unitigs in emit order
+            uint8_t* out = b_out.ensure<uint8_t>(total_out + 16);
+            DevBuf b_np2, b_po2, b_it2;
+            const uint64_t P2 = make_pieces(nu, ooff, total_out, b_np2, b_po2, b_it2, b_ptot, b_tmp, st);
+            { KSCOPE("k_st_copy_out", st); k_st_copy_out<<<st_blocks(P2 * 64), ST_TPB, 0, st>>>(P2, b_it2.as<uint32_t>(), b_po2.as<uint64_t>(), nu, ord, u_x, u_tig, ooff, total_out, k, tig, total_tig, out, err); }
+            check();
+            static_assert(sizeof(UnitigList::Item) == 24, "items are three u64");
+            L.items.resize(nu);
+            L.bases.resize(total_out);
+            CDBG_HIP(hipMemcpyAsync(L.items.data(), items, nu * 24, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipMemcpyAsync(&L.bases[0], out, total_out, hipMemcpyDeviceToHost, st));
+        }
+        uint64_t h[5];
+        CDBG_HIP(hipMemcpyAsync(h, ctr, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+        S.joins = h[1];
+        S.unmatched_ends = h[2];
+        S.foreign_dropped = h[3];
+        S.unitig_kmers = h[4];
+        S.num_unitigs = nu;
+    } else {
+        uint64_t h[3];
+        CDBG_HIP(hipMemcpyAsync(h, ctr, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+        S.joins = h[1];
+        S.unmatched_ends = h[2];
+    }
+    if (!cyc_frags.empty()) {
+        stitch_cycles(hbuf, n, off, k, cyc_frags, cyc_partner, U->lists[1], S);
+        S.num_unitigs += U->lists[1].items.size();
+    }
+    return U.release();
+}
+
+}  // namespace
+
+Unitigs* stitch_fragments_gpu(int device, const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads) {
+    if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
+    if (threads == 0) threads = 1;
+    const std::vector<uint64_t> off = fragment_offsets(buf, n, threads);
+    CDBG_HIP(hipSetDevice(device));
+    StreamOwner own;
+    own.s = acquire_stream(device);
+    own.device = device;
+    AllocStream on_stream(own.s);
+    if (k <= 31) return stitch_device<uint64_t>(own.s, buf, n, off, k);
+    return stitch_device<Key128>(own.s, buf, n, off, k);
+}
+
+}  // namespace cdbg_amd
+
+using namespace cdbg_amd;
+
+extern "C" {
+
+CDBG_API void* cdbg_gpu_stitch(cdbg_gpu_ctx* ctx, const uint8_t* frags, uint64_t n_bytes, uint32_t k,
+                               uint32_t threads) {
+    try {
+        if (!ctx || (!frags && n_bytes)) throw Error("null argument");
+        return stitch_fragments_gpu(ctx->device, frags, n_bytes, k, threads);
+    } catch (std::exception& e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+CDBG_API void* cdbg_gpu_stitch_file(cdbg_gpu_ctx* ctx, const char* frag_path, uint32_t k, uint32_t threads) {
+    try {
+        if (!ctx || !frag_path) throw Error("null argument");
+        uint64_t n = 0;
+        const std::unique_ptr<uint8_t[]> buf = read_file(frag_path, threads, n);
+        return stitch_fragments_gpu(ctx->device, buf.get(), n, k, threads);
+    } catch (std::exception& e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+}  // extern "C"

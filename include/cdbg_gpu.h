@@ -385,6 +385,23 @@ typedef struct {
 } cdbg_stitch_stats;
 CDBG_API void* cdbg_stitch(const uint8_t* frags, uint64_t n_bytes, uint32_t k, uint32_t threads);
 CDBG_API void* cdbg_stitch_file(const char* frag_path, uint32_t k, uint32_t threads);
+/* Phase 3 on the device (the same replacement of
+ * stitch_unitigs_extmem_file_streaming, stitch_extmem.hpp:1012-1129): the
+ * same unitigs, in the same order, with the same statistics and errors as
+ * cdbg_stitch / cdbg_stitch_file.  The host parses the record offsets and
+ * walks tigs that close on themselves (`threads` threads); joins, chain
+ * ranking, unitig cutting, ordering and the bases run on ctx's device.
+ * Returns the same kind of handle. */
+CDBG_API void* cdbg_gpu_stitch(cdbg_gpu_ctx* ctx, const uint8_t* frags, uint64_t n_bytes, uint32_t k,
+                               uint32_t threads);
+CDBG_API void* cdbg_gpu_stitch_file(cdbg_gpu_ctx* ctx, const char* frag_path, uint32_t k, uint32_t threads);
+/* Start of every record of a frag_unitigs.bin stream (frag_record_parse,
+ * unitig_spill.hpp:646-665, applied record after record), parsed by up to
+ * `threads` threads over ranges of at least 64 KiB: *count = the number of
+ * records, the first min(count, cap) offsets go to out; fails on a record
+ * that does not fit the stream or when cap < count. */
+CDBG_API int cdbg_frag_offsets(const uint8_t* frags, uint64_t n_bytes, uint32_t threads, uint64_t* out, uint64_t cap,
+                               uint64_t* count);
 CDBG_API int cdbg_unitigs_stats(void* unitigs, cdbg_stitch_stats* out);
 /* Total bases of the unitigs, and the unitigs themselves in emit order:
  * sequences concatenated into seq (cdbg_unitigs_bases bytes), offsets
