@@ -160,15 +160,15 @@ void emit_fasta(Unitigs& U, const std::string& out) {
     U.build_order();
     File fa(out + ".fa", "wb");
     File u2c(out + ".u2c", "wb");
-    const uint64_t n = U.order.size();
+    const uint64_t n = U.size();
     u2c.pod(n);
     u2c.pod(uint64_t((n + 63) / 64));
     std::vector<char> buf(1 << 20);
     size_t pos = 0;
     uint64_t word = 0, words_out = 0;
     for (uint64_t i = 0; i < n; ++i) {
-        const auto& it = U.lists[U.order[i].first].items[U.order[i].second];
-        const uint64_t next_cid = i + 1 < n ? U.lists[U.order[i + 1].first].items[U.order[i + 1].second].cid : ~it.cid;
+        const auto& it = U.item(i);
+        const uint64_t next_cid = i + 1 < n ? U.item(i + 1).cid : ~it.cid;
         if (next_cid != it.cid) word |= uint64_t(1) << (i & 63);
         if ((i & 63) == 63) {
             u2c.pod(word);
@@ -182,7 +182,7 @@ void emit_fasta(Unitigs& U, const std::string& out) {
         buf[pos++] = '>';
         pos = size_t(put_u64(buf.data() + pos, it.cid) - buf.data());
         buf[pos++] = '\n';
-        const char* s = U.lists[U.order[i].first].bases.data() + it.off;
+        const char* s = U.seq(i);
         uint64_t left = it.len;
         while (left) {
             if (pos == buf.size()) {
@@ -301,7 +301,7 @@ CDBG_API int cdbg_unitigs_stats(void* h, cdbg_stitch_stats* out) {
 CDBG_API uint64_t cdbg_unitigs_bases(void* h) {
     uint64_t b = 0;
     for (auto& l : ((Unitigs*)h)->lists)
-        for (auto& it : l.items) b += it.len;
+        for (uint64_t i = 0; i < l.size(); ++i) b += l.item(i).len;
     return b;
 }
 
@@ -311,15 +311,15 @@ CDBG_API int cdbg_unitigs_export(void* h, char* seq, uint64_t* offsets, uint64_t
         Unitigs& U = *(Unitigs*)h;
         U.build_order();
         uint64_t at = 0;
-        for (uint64_t i = 0; i < U.order.size(); ++i) {
-            const UnitigList& l = U.lists[U.order[i].first];
-            const auto& it = l.items[U.order[i].second];
-            if (seq) std::memcpy(seq + at, l.bases.data() + it.off, it.len);
+        const uint64_t n = U.size();
+        for (uint64_t i = 0; i < n; ++i) {
+            const auto& it = U.item(i);
+            if (seq) std::memcpy(seq + at, U.seq(i), it.len);
             if (offsets) offsets[i] = at;
             if (cids) cids[i] = it.cid;
             at += it.len;
         }
-        if (offsets) offsets[U.order.size()] = at;
+        if (offsets) offsets[n] = at;
         return 0;
     } catch (std::exception& e) {
         return set_error(e.what());

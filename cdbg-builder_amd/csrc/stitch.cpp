@@ -506,16 +506,15 @@ std::vector<uint64_t> fragment_offsets(const uint8_t* buf, uint64_t n, uint32_t 
     return off;
 }
 
-std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_t& n) {
+uint64_t file_size_of(const char* path) {
+    struct stat sb;
+    if (::stat(path, &sb) != 0) throw Error(std::string("cannot stat ") + path);
+    return uint64_t(sb.st_size);
+}
+
+void read_file_into(const char* path, uint8_t* dst, uint64_t n, uint32_t threads) {
     const int fd = ::open(path, O_RDONLY);
     if (fd < 0) throw Error(std::string("cannot open ") + path);
-    struct stat sb;
-    if (::fstat(fd, &sb) != 0) {
-        ::close(fd);
-        throw Error(std::string("cannot stat ") + path);
-    }
-    n = uint64_t(sb.st_size);
-    std::unique_ptr<uint8_t[]> buf(new uint8_t[n + 1]);
     // ranges of at least 64 MiB per thread
     const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads ? threads : 1, n >> 26));
     std::atomic<bool> short_read{false};
@@ -524,7 +523,7 @@ std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_
             uint64_t p = n * t / T;
             const uint64_t end = n * (t + 1) / T;
             while (p < end) {
-                const ssize_t got = ::pread(fd, buf.get() + p, size_t(std::min<uint64_t>(end - p, 1ull << 30)), off_t(p));
+                const ssize_t got = ::pread(fd, dst + p, size_t(std::min<uint64_t>(end - p, 1ull << 30)), off_t(p));
                 if (got <= 0) {
                     short_read = true;
                     break;
@@ -535,7 +534,19 @@ std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_
     }, 1);
     ::close(fd);
     if (short_read) throw Error(std::string("short read of ") + path);
+}
+
+std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_t& n) {
+    n = file_size_of(path);
+    std::unique_ptr<uint8_t[]> buf(new uint8_t[n + 1]);
+    read_file_into(path, buf.get(), n, threads);
     return buf;
+}
+
+void parallel_copy(void* dst, const void* src, uint64_t n, uint32_t threads) {
+    parallel_for(n, threads, [&](uint64_t b, uint64_t e, uint32_t) {
+        std::memcpy((uint8_t*)dst + b, (const uint8_t*)src + b, e - b);
+    }, uint64_t(1) << 24);
 }
 
 void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& off, uint32_t k,
@@ -572,28 +583,44 @@ void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& 
                      [](const UnitigList::Item& a, const UnitigList::Item& b) { return a.cid < b.cid; });
 }
 
-void Unitigs::build_order() {
-    if (!order.empty() || lists.empty()) return;
+uint64_t Unitigs::size() const {
     uint64_t total = 0;
-    for (auto& l : lists) total += l.items.size();
-    order.reserve(total);
+    for (auto& l : lists) total += l.size();
+    return total;
+}
+
+void Unitigs::build_order() {
+    if (single != -2) return;
+    int nonempty = 0, which = -1;
+    for (uint32_t l = 0; l < lists.size(); ++l)
+        if (lists[l].size()) {
+            ++nonempty;
+            which = int(l);
+        }
+    if (nonempty <= 1) {
+        single = which;
+        return;
+    }
+    single = -1;
+    order.reserve(size());
     // stable merge by cid: the smallest head, ties to the earlier list
     std::vector<uint64_t> pos(lists.size(), 0);
     using H = std::pair<uint64_t, uint32_t>;  // (cid, list)
     std::vector<H> heap;
     auto cmp = [](const H& a, const H& b) { return a.first > b.first || (a.first == b.first && a.second > b.second); };
     for (uint32_t l = 0; l < lists.size(); ++l)
-        if (!lists[l].items.empty()) heap.push_back(H{lists[l].items[0].cid, l});
+        if (lists[l].size()) heap.push_back(H{lists[l].item(0).cid, l});
     std::make_heap(heap.begin(), heap.end(), cmp);
     while (!heap.empty()) {
         std::pop_heap(heap.begin(), heap.end(), cmp);
         const uint32_t l = heap.back().second;
         heap.pop_back();
-        const uint64_t cid = lists[l].items[pos[l]].cid;
+        const UnitigList& L = lists[l];
+        const uint64_t cid = L.item(pos[l]).cid;
         // take this list's whole run of equal cids
-        while (pos[l] < lists[l].items.size() && lists[l].items[pos[l]].cid == cid) order.push_back({l, pos[l]++});
-        if (pos[l] < lists[l].items.size()) {
-            heap.push_back(H{lists[l].items[pos[l]].cid, l});
+        while (pos[l] < L.size() && L.item(pos[l]).cid == cid) order.push_back({l, pos[l]++});
+        if (pos[l] < L.size()) {
+            heap.push_back(H{L.item(pos[l]).cid, l});
             std::push_heap(heap.begin(), heap.end(), cmp);
         }
     }
@@ -659,7 +686,7 @@ Unitigs* stitch_fragments(const uint8_t* buf, uint64_t n, uint32_t k, uint32_t t
         U->stats.unitig_kmers += s.unitig_kmers;
         U->stats.foreign_dropped += s.foreign_dropped;
     }
-    for (auto& l : U->lists) U->stats.num_unitigs += l.items.size();
+    for (auto& l : U->lists) U->stats.num_unitigs += l.size();
     return U;
 }
 

@@ -20,22 +20,46 @@ constexpr uint8_t FRAG_OPEN_LEFT = 1, FRAG_OPEN_RIGHT = 2;
 // earlier list first), i.e. ascending cid, then the order the stitch made
 // them in.
 struct UnitigList {
-    std::string bases;
     struct Item {
         uint64_t cid;
-        uint64_t off;  // into bases
+        uint64_t off;  // into the list's bases
         uint64_t len;
     };
+    std::string bases;
     std::vector<Item> items;
+    // A list the device stitch filled: bases and items live in host memory
+    // that `keep` owns (pinned), and stand in for the two vectors.
+    std::shared_ptr<void> keep;
+    const char* ext_bases = nullptr;
+    const Item* ext_items = nullptr;
+    uint64_t ext_n = 0;
+
+    uint64_t size() const { return ext_items ? ext_n : items.size(); }
+    const Item& item(uint64_t i) const { return ext_items ? ext_items[i] : items[i]; }
+    const char* seq() const { return ext_items ? ext_bases : bases.data(); }
 };
 
 struct Unitigs {
     uint32_t k = 0;
     std::vector<UnitigList> lists;
     cdbg_stitch_stats stats{};
-    // emit order, computed on first use
+    // emit order, computed on first use (left empty when a single list holds
+    // every unitig: that list's order is the emit order)
     std::vector<std::pair<uint32_t, uint64_t>> order;  // (list, item)
+    int single = -2;  // the one non-empty list, -1 none, -2 not computed
     void build_order();
+    uint64_t size() const;
+    std::pair<uint32_t, uint64_t> at(uint64_t i) const {
+        return single >= 0 ? std::pair<uint32_t, uint64_t>(uint32_t(single), i) : order[i];
+    }
+    const UnitigList::Item& item(uint64_t i) const {
+        const auto a = at(i);
+        return lists[a.first].item(a.second);
+    }
+    const char* seq(uint64_t i) const {
+        const auto a = at(i);
+        return lists[a.first].seq() + lists[a.first].item(a.second).off;
+    }
 };
 
 // Stitches the fragment records in buf[0, n) (the frag_unitigs.bin byte
@@ -57,6 +81,11 @@ void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& 
 
 // The whole file at `path`, read by up to `threads` threads; n = its size.
 std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_t& n);
+uint64_t file_size_of(const char* path);
+// Reads the n bytes of the file at `path` into dst with up to `threads` threads.
+void read_file_into(const char* path, uint8_t* dst, uint64_t n, uint32_t threads);
+// memcpy split over up to `threads` threads (16 MiB or more each).
+void parallel_copy(void* dst, const void* src, uint64_t n, uint32_t threads);
 
 // Phase 3 on the device (stitch_gpu.hip): the same unitigs, in the same
 // order, as stitch_fragments.

@@ -60,6 +60,7 @@ constexpr int ST_TPB = 256;
 constexpr uint32_t NONE32 = ~0u;
 constexpr uint32_t DONE = 1u << 31;  // jump state: .x is the last side of the path
 constexpr uint32_t PIECE = 4096;     // bytes one wave copies per work item
+constexpr uint32_t SMALL_COPY = 128; // copies up to this size take one thread, larger ones waves
 
 // error bits, in the order the host stitch meets them
 enum : uint32_t {
@@ -91,21 +92,41 @@ __device__ __forceinline__ uint64_t ld64(const uint8_t* p) {
 
 __device__ __forceinline__ uint64_t st_lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
-// wave-aggregated atomic add of one lane's v
-__device__ __forceinline__ void wave_add(unsigned long long* ctr, uint64_t v) {
+// A counter is N_STRIPE words: a block adds its total to the word of its
+// own stripe, so no address takes more than a few atomics at a time (one
+// word taking an atomic from every wave serialises the launch); the host
+// adds the words up.
+constexpr uint32_t N_STRIPE = 256;
+
+// Sum over the block of each thread's v (every thread of the block calls it).
+__device__ __forceinline__ uint64_t block_sum(uint64_t v) {
+    __shared__ unsigned long long s_part[ST_TPB / 64];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t t = 0;
+    for (int w = 0; w < ST_TPB / 64; ++w) t += s_part[w];
+    __syncthreads();  // s_part is free for the next call
+    return t;
 }
 
-__device__ __forceinline__ uint64_t wave_slot(unsigned long long* ctr, bool want) {
-    const int lane = threadIdx.x & 63;
-    const uint64_t m = __ballot(want);
-    if (!m) return 0;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
-    return base + __popcll(m & st_lanes_below(lane));
+__device__ __forceinline__ uint64_t block_max(uint64_t v) {
+    __shared__ unsigned long long s_part[ST_TPB / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t y = __shfl_xor(v, o, 64);
+        v = y > v ? y : v;
+    }
+    if ((threadIdx.x & 63) == 0) s_part[threadIdx.x >> 6] = v;
+    __syncthreads();
+    uint64_t t = 0;
+    for (int w = 0; w < ST_TPB / 64; ++w) t = s_part[w] > t ? s_part[w] : t;
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void stripe_add(unsigned long long* c, uint64_t v) {
+    const uint64_t t = block_sum(v);
+    if (threadIdx.x == 0 && t) atomicAdd(&c[blockIdx.x % N_STRIPE], (unsigned long long)t);
 }
 
 __device__ __forceinline__ int code_of(uint8_t c) {
@@ -128,22 +149,36 @@ __device__ __forceinline__ Key128 push_base<Key128>(const Key128& x, uint32_t c)
 __device__ __forceinline__ bool key_le(uint64_t a, uint64_t b) { return a <= b; }
 __device__ __forceinline__ bool key_le(const Key128& a, const Key128& b) { return !(b < a); }
 
-// 1. Per fragment: checks, colour ids, keys of the open sides.
+// 1. Open sides per block of fragments (their ends are laid out block by
+// block, in fragment order).
+__global__ __launch_bounds__(ST_TPB) void k_st_count_ends(uint64_t nf, const uint8_t* __restrict__ buf,
+                                                          const uint64_t* __restrict__ off,
+                                                          uint64_t* __restrict__ block_cnt) {
+    const uint64_t i = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    const uint32_t c = i < nf ? uint32_t(__popc(buf[off[i]] & 3u)) : 0u;
+    const uint64_t t = block_sum(c);
+    if (threadIdx.x == 0) block_cnt[blockIdx.x] = t;
+}
+
+// Per fragment: checks, colour ids, keys of the open sides (at the block's
+// base plus the open sides before it in the block).
 template <class K>
 __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t* __restrict__ buf,
                                                      const uint64_t* __restrict__ off, uint32_t k,
+                                                     const uint64_t* __restrict__ block_base,
                                                      uint32_t* __restrict__ f_len, uint32_t* __restrict__ f_nr,
                                                      uint64_t* __restrict__ f_cid, K* __restrict__ ekey,
-                                                     uint32_t* __restrict__ eid, unsigned long long* __restrict__ ctr,
-                                                     uint32_t* __restrict__ err) {
+                                                     uint32_t* __restrict__ eid, uint32_t* __restrict__ err) {
+    __shared__ uint64_t s_tmp[ST_TPB / 64 + 1];
     const uint64_t i = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
-    bool live = i < nf;
+    const bool live = i < nf;
     uint8_t open = 0;
     uint32_t len = 0, nr = 0;
+    bool ok = false;
     const uint8_t* rec = nullptr;
     if (live) {
         rec = buf + off[i];
-        open = rec[0];
+        open = rec[0] & 3u;
         nr = ld32(rec + 1);
         len = ld32(rec + 5 + 12 * uint64_t(nr));
         f_len[i] = len;
@@ -173,17 +208,18 @@ __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t*
             f_cid[4 * i + 2] = p0;
             f_cid[4 * i + 3] = pl;
         }
-        if (bad) {
-            atomicOr(err, bad);
-            open = 0;
-        }
+        if (bad) atomicOr(err, bad);
+        ok = !bad;
     }
+    uint64_t tot;
+    uint64_t at = block_base[blockIdx.x] + block_exclusive_scan<uint64_t>(uint64_t(__popc(open)), s_tmp, tot);
     // open sides: canonical boundary k-mer, and whether the forward word is it
+    // (a record that failed its checks gets placeholder keys: the stitch stops)
     for (uint32_t side = 0; side < 2; ++side) {
-        const bool want = live && (open & (side ? 2u : 1u));
+        if (!(open & (side ? 2u : 1u))) continue;
         K key{};
         bool fwd = false;
-        if (want) {
+        if (ok) {
             const uint8_t* s = rec + 9 + 12 * uint64_t(nr) + (side ? len - k : 0);
             K f{}, r{};
             int bad = 0;
@@ -197,11 +233,9 @@ __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t*
             fwd = key_le(f, r);
             key = fwd ? f : r;
         }
-        const uint64_t at = wave_slot(&ctr[0], want);
-        if (want) {
-            ekey[at] = key;
-            eid[at] = uint32_t(i << 2) | (side << 1) | uint32_t(fwd);
-        }
+        ekey[at] = key;
+        eid[at] = uint32_t(i << 2) | (side << 1) | uint32_t(fwd);
+        ++at;
     }
 }
 
@@ -235,45 +269,54 @@ __global__ __launch_bounds__(ST_TPB) void k_st_pair(uint64_t ne, const K* __rest
         }
         if (!paired) unmatched = two ? 2 : 1;
     }
-    wave_add(&ctr[1], joins);
-    wave_add(&ctr[2], unmatched);
+    stripe_add(ctr + 1 * N_STRIPE, joins);
+    stripe_add(ctr + 2 * N_STRIPE, unmatched);
 }
 
-// 3. Pointer jumping over next(s) = partner[s ^ 1].
+// 3. Pointer jumping over next(s) = partner[s ^ 1].  A side's state is
+// (target side, distance to it), packed in 64 bits; with DONE set in the
+// target, the target is the last side of the path.
+__device__ __forceinline__ uint64_t st_pack(uint32_t x, uint32_t y) { return uint64_t(x) | (uint64_t(y) << 32); }
+__device__ __forceinline__ uint32_t st_x(uint64_t v) { return uint32_t(v); }
+__device__ __forceinline__ uint32_t st_y(uint64_t v) { return uint32_t(v >> 32); }
+
 __global__ __launch_bounds__(ST_TPB) void k_st_jump_init(uint64_t ns, const uint32_t* __restrict__ partner,
-                                                         uint2* __restrict__ S) {
+                                                         uint64_t* __restrict__ S) {
     const uint64_t s = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
     if (s >= ns) return;
     const uint32_t nx = partner[s ^ 1];
-    S[s] = nx == NONE32 ? make_uint2(uint32_t(s) | DONE, 0u) : make_uint2(nx, 1u);
+    S[s] = nx == NONE32 ? st_pack(uint32_t(s) | DONE, 0u) : st_pack(nx, 1u);
 }
 
-__global__ __launch_bounds__(ST_TPB) void k_st_jump(uint64_t ns, const uint2* __restrict__ A, uint2* __restrict__ B,
-                                                    unsigned long long* __restrict__ open_ctr) {
+// One round, in place: every open side takes its target's state.  A state
+// is read and written whole (one 64-bit access), so it is always some
+// (side, distance to it) pair of the path, and a round moves every open
+// side at least twice as far as the round before (the target's state is at
+// least as advanced as at the round's start).  Counts the sides still open.
+__global__ __launch_bounds__(ST_TPB) void k_st_jump(uint64_t ns, uint64_t* S, unsigned long long* __restrict__ open_ctr) {
     const uint64_t s = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
     bool open = false;
     if (s < ns) {
-        uint2 v = A[s];
-        if (!(v.x & DONE)) {
-            const uint2 w = A[v.x];
-            v.x = w.x;
-            v.y += w.y;
-            open = !(w.x & DONE);
+        uint64_t v = __hip_atomic_load(&S[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!(st_x(v) & DONE)) {
+            const uint64_t w = __hip_atomic_load(&S[st_x(v)], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v = st_pack(st_x(w), st_y(v) + st_y(w));
+            __hip_atomic_store(&S[s], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            open = !(st_x(w) & DONE);
         }
-        B[s] = v;
     }
-    const uint64_t m = __ballot(open);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(open_ctr, (unsigned long long)__popcll(m));
+    stripe_add(open_ctr, open ? 1u : 0u);
 }
 
 // 4. The walked direction of every fragment and its place.
-__global__ __launch_bounds__(ST_TPB) void k_st_choose(uint64_t nf, const uint2* __restrict__ S,
+__global__ __launch_bounds__(ST_TPB) void k_st_choose(uint64_t nf, const uint64_t* __restrict__ S,
                                                       uint32_t* __restrict__ ch_side, uint32_t* __restrict__ ch_start,
                                                       uint32_t* __restrict__ ch_pos, uint32_t* __restrict__ chain_len,
                                                       uint8_t* __restrict__ on_cycle, uint32_t* __restrict__ err) {
     const uint64_t i = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
     if (i >= nf) return;
-    const uint2 s0 = S[2 * i], s1 = S[2 * i + 1];
+    const uint64_t v0 = S[2 * i], v1 = S[2 * i + 1];
+    const uint2 s0 = make_uint2(st_x(v0), st_y(v0)), s1 = make_uint2(st_x(v1), st_y(v1));
     if (!(s0.x & DONE) || !(s1.x & DONE)) {
         // a side on a cycle: so is the other (the reverse direction of a cycle is one)
         if ((s0.x & DONE) || (s1.x & DONE)) atomicOr(err, E_INTERNAL);
@@ -321,7 +364,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_place(uint64_t nf, const uint32_t
 // and the tig it belongs to (flag of a tig's first slot, scanned).
 __global__ __launch_bounds__(ST_TPB) void k_st_slot_sizes(uint64_t n_lin, const uint32_t* __restrict__ slot_side,
                                                           const uint32_t* __restrict__ ch_pos,
-                                                          const uint2* __restrict__ S, const uint32_t* __restrict__ f_len,
+                                                          const uint64_t* __restrict__ S, const uint32_t* __restrict__ f_len,
                                                           const uint32_t* __restrict__ f_nr, uint32_t k,
                                                           uint64_t* __restrict__ s_kmers, uint64_t* __restrict__ s_bases,
                                                           uint64_t* __restrict__ s_runs, uint32_t* __restrict__ s_first,
@@ -336,7 +379,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_slot_sizes(uint64_t n_lin, const 
         return;
     }
     const uint32_t i = s >> 1;
-    const bool joined = ch_pos[i] > 0, last = S[s].y == 0;
+    const bool joined = ch_pos[i] > 0, last = st_y(S[s]) == 0;
     const uint64_t len = f_len[i], nk = len - k + 1;
     const uint64_t m = nk - (joined ? 1 : 0);
     // a joined fragment that is not the last of its tig adds a k-mer of its own
@@ -352,7 +395,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_slot_sizes(uint64_t n_lin, const 
 // last k-mer recoloured by the next join when that differs; the unused
 // entries carry no k-mers.
 __global__ __launch_bounds__(ST_TPB) void k_st_runs(uint64_t n_lin, const uint32_t* __restrict__ slot_side,
-                                                    const uint32_t* __restrict__ ch_pos, const uint2* __restrict__ S,
+                                                    const uint32_t* __restrict__ ch_pos, const uint64_t* __restrict__ S,
                                                     const uint8_t* __restrict__ buf, const uint64_t* __restrict__ off,
                                                     const uint32_t* __restrict__ f_nr, const uint64_t* __restrict__ f_cid,
                                                     const uint64_t* __restrict__ r_off, const uint32_t* __restrict__ s_tig,
@@ -363,7 +406,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_runs(uint64_t n_lin, const uint32
     if (j >= n_lin) return;
     const uint32_t s = slot_side[j];
     const uint32_t i = s >> 1;
-    const bool rev = s & 1u, joined = ch_pos[i] > 0, last = S[s].y == 0;
+    const bool rev = s & 1u, joined = ch_pos[i] > 0, last = st_y(S[s]) == 0;
     const uint32_t nr = f_nr[i];
     const uint8_t* runs = buf + off[i] + 5;
     const uint64_t base = r_off[j];
@@ -462,8 +505,8 @@ __global__ __launch_bounds__(ST_TPB) void k_st_unitigs(uint64_t n_cand, uint64_t
         u_tig[u] = tig;
         keep[u] = k_ ? 1 : 0;
     }
-    wave_add(&ctr[3], dropped);
-    wave_add(&ctr[4], kept_kmers);
+    stripe_add(ctr + 3 * N_STRIPE, dropped);
+    stripe_add(ctr + 4 * N_STRIPE, kept_kmers);
 }
 
 // 7. Emit order: kept unitigs (tig order) with their colour ids as sort keys.
@@ -477,11 +520,8 @@ __global__ __launch_bounds__(ST_TPB) void k_st_keys(uint64_t nu, const uint32_t*
         key[t] = c;
         val[t] = klist[t];
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const uint64_t y = __shfl_xor(c, o, 64);
-        c = y > c ? y : c;
-    }
-    if ((threadIdx.x & 63) == 0) atomicMax(cmax, (unsigned long long)c);
+    c = block_max(c);
+    if (threadIdx.x == 0) atomicMax(&cmax[blockIdx.x % N_STRIPE], (unsigned long long)c);
 }
 
 __global__ __launch_bounds__(ST_TPB) void k_st_out_len(uint64_t nu, const uint32_t* __restrict__ ord,
@@ -511,7 +551,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_pieces(uint64_t n, const uint64_t
     const uint64_t e = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
     if (e >= n) return;
     const uint64_t b = (e + 1 < n ? bytes_scan[e + 1] : total) - bytes_scan[e];
-    const uint64_t p0 = piece_off[e], np = (b + PIECE - 1) / PIECE;
+    const uint64_t p0 = piece_off[e], np = b > SMALL_COPY ? (b + PIECE - 1) / PIECE : 0;
     for (uint64_t p = 0; p < np; ++p) item_of[p0 + p] = uint32_t(e);
 }
 
@@ -520,7 +560,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_piece_count(uint64_t n, const uin
     const uint64_t e = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
     if (e >= n) return;
     const uint64_t b = (e + 1 < n ? bytes_scan[e + 1] : total) - bytes_scan[e];
-    np[e] = (b + PIECE - 1) / PIECE;
+    np[e] = b > SMALL_COPY ? (b + PIECE - 1) / PIECE : 0;
 }
 
 // Tig sequences: slot j's bases (reverse complement when entered at the
@@ -586,6 +626,58 @@ __global__ __launch_bounds__(ST_TPB) void k_st_copy_out(uint64_t n_pieces, const
     for (uint64_t q = q0 + lane; q < q1; q += 64) out[o0 + q] = src[q];
 }
 
+// The slots (tig sequences) and unitigs (output) of SMALL_COPY bytes or
+// fewer, one thread each.
+__global__ __launch_bounds__(ST_TPB) void k_st_fill_tigs_small(uint64_t n_lin, const uint32_t* __restrict__ slot_side,
+                                                               const uint32_t* __restrict__ ch_pos,
+                                                               const uint8_t* __restrict__ buf,
+                                                               const uint64_t* __restrict__ off,
+                                                               const uint32_t* __restrict__ f_nr,
+                                                               const uint32_t* __restrict__ f_len,
+                                                               const uint64_t* __restrict__ s_boff, uint32_t k,
+                                                               uint64_t total_tig, uint8_t* __restrict__ tig,
+                                                               uint32_t* __restrict__ err) {
+    const uint64_t j = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (j >= n_lin) return;
+    const uint32_t s = slot_side[j], i = s >> 1;
+    const uint32_t len = f_len[i];
+    const uint32_t from = ch_pos[i] > 0 ? k : 0u;
+    if (len < from || s_boff[j] + (len - from) > total_tig) {
+        atomicOr(err, E_INTERNAL);
+        return;
+    }
+    const uint32_t b = len - from;
+    if (b > SMALL_COPY) return;
+    const uint8_t* seq = buf + off[i] + 9 + 12 * uint64_t(f_nr[i]);
+    uint8_t* dst = tig + s_boff[j];
+    if (s & 1u) {
+        for (uint32_t q = 0; q < b; ++q) dst[q] = comp_of(seq[len - 1 - from - q]);
+    } else {
+        for (uint32_t q = 0; q < b; ++q) dst[q] = seq[from + q];
+    }
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_copy_out_small(uint64_t nu, const uint32_t* __restrict__ ord,
+                                                              const uint64_t* __restrict__ u_x,
+                                                              const uint32_t* __restrict__ u_tig,
+                                                              const uint64_t* __restrict__ ooff, uint64_t total_out,
+                                                              uint32_t k, const uint8_t* __restrict__ tig,
+                                                              uint64_t total_tig, uint8_t* __restrict__ out,
+                                                              uint32_t* __restrict__ err) {
+    const uint64_t t = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (t >= nu) return;
+    const uint32_t u = ord[t];
+    const uint64_t from = u_x[u] + uint64_t(k - 1) * u_tig[u];
+    const uint64_t o0 = ooff[t];
+    const uint64_t len = (t + 1 < nu ? ooff[t + 1] : total_out) - o0;
+    if (from > total_tig || len > total_tig - from || o0 + len > total_out) {
+        atomicOr(err, E_INTERNAL);
+        return;
+    }
+    if (len > SMALL_COPY) return;
+    for (uint32_t q = 0; q < uint32_t(len); ++q) out[o0 + q] = tig[from + q];
+}
+
 __global__ void k_st_fill32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -649,27 +741,66 @@ uint64_t make_pieces(uint64_t n, const uint64_t* scan, uint64_t total, DevBuf& n
     return P;
 }
 
+// Pinned host memory (device copies from and to it run at link speed).
+struct PinnedHost {
+    void* p = nullptr;
+    explicit PinnedHost(size_t bytes) { CDBG_HIP(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault)); }
+    ~PinnedHost() {
+        if (p) (void)hipHostFree(p);
+    }
+    PinnedHost(const PinnedHost&) = delete;
+    PinnedHost& operator=(const PinnedHost&) = delete;
+};
+
 template <class K>
-Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, const std::vector<uint64_t>& off, uint32_t k) {
+Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t k, uint32_t threads) {
     std::unique_ptr<Unitigs> U(new Unitigs());
     U->k = k;
     U->lists.resize(2);  // the linear tigs' unitigs, then the cycles'
     cdbg_stitch_stats& S = U->stats;
+    DevBuf b_buf, b_off, b_len, b_nr, b_cid, b_ekey, b_eid, b_ekey_t, b_eid_t, b_ctr, b_err, b_partner, b_A;
+    DevBuf b_chs, b_chst, b_chp, b_clen, b_coff, b_cyc, b_tot, b_tmp, b_cnt, b_slot;
+    RadixScratch rs;
+    // the stream goes to the device while the host finds the record starts
+    uint8_t* buf = b_buf.ensure<uint8_t>(n + 16);
+    std::vector<uint64_t> off;
+    {
+        std::string perr;
+        std::thread parse([&] {
+            try {
+                off = fragment_offsets(hbuf, n, threads);
+            } catch (std::exception& e) {
+                perr = e.what();
+            }
+        });
+        hipError_t e = n ? hipMemcpyAsync(buf, hbuf, n, hipMemcpyHostToDevice, st) : hipSuccess;
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        parse.join();
+        if (!perr.empty()) throw Error(perr);
+        CDBG_HIP(e);
+    }
     const uint64_t nf = off.size();
     S.num_fragments = nf;
     if (nf == 0) return U.release();
     if (nf >= (1ull << 30)) throw Error("stitch: 2^30 or more fragments for the device stitch");
     const uint64_t ns = 2 * nf;
-    DevBuf b_buf, b_off, b_len, b_nr, b_cid, b_ekey, b_eid, b_ekey_t, b_eid_t, b_ctr, b_err, b_partner, b_A, b_B;
-    DevBuf b_chs, b_chst, b_chp, b_clen, b_coff, b_cyc, b_tot, b_tmp, b_cnt, b_slot;
-    RadixScratch rs;
-    uint8_t* buf = b_buf.ensure<uint8_t>(n + 16);
     uint64_t* d_off = b_off.ensure<uint64_t>(nf);
-    CDBG_HIP(hipMemcpyAsync(buf, hbuf, n, hipMemcpyHostToDevice, st));
-    CDBG_HIP(hipMemcpyAsync(d_off, off.data(), nf * 8, hipMemcpyHostToDevice, st));
-    unsigned long long* ctr = b_ctr.ensure<unsigned long long>(8);
+    PinnedHost p_off(nf * 8);
+    parallel_copy(p_off.p, off.data(), nf * 8, threads);
+    CDBG_HIP(hipMemcpyAsync(d_off, p_off.p, nf * 8, hipMemcpyHostToDevice, st));
+    // counters (N_STRIPE words each): 1 joins, 2 unmatched ends, 3 foreign
+    // k-mers dropped, 4 unitig k-mers, 5 open sides of a jump round, 6 largest id
+    unsigned long long* ctr = b_ctr.ensure<unsigned long long>(8 * N_STRIPE);
     uint32_t* err = b_err.ensure<uint32_t>(1);
-    CDBG_HIP(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), st));
+    CDBG_HIP(hipMemsetAsync(ctr, 0, 8 * N_STRIPE * sizeof(unsigned long long), st));
+    auto stripes = [&](int c, bool max) {
+        std::vector<unsigned long long> h(N_STRIPE);
+        CDBG_HIP(hipMemcpyAsync(h.data(), ctr + uint64_t(c) * N_STRIPE, N_STRIPE * 8, hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+        uint64_t v = 0;
+        for (unsigned long long x : h) v = max ? std::max<uint64_t>(v, x) : v + x;
+        return v;
+    };
     CDBG_HIP(hipMemsetAsync(err, 0, 4, st));
     auto check = [&]() { throw_on(st_d2h<uint32_t>(err, st)); };
 
@@ -679,9 +810,17 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, const st
     uint64_t* f_cid = b_cid.ensure<uint64_t>(4 * nf);
     K* ekey = b_ekey.ensure<K>(ns);
     uint32_t* eid = b_eid.ensure<uint32_t>(ns);
-    { KSCOPE("k_st_frags", st); k_st_frags<K><<<st_blocks(nf), ST_TPB, 0, st>>>(nf, buf, d_off, k, f_len, f_nr, f_cid, ekey, eid, ctr, err); }
+    const uint64_t nbf = st_blocks(nf);
+    DevBuf b_bc, b_bb, b_bt;
+    uint64_t* block_cnt = b_bc.ensure<uint64_t>(nbf);
+    uint64_t* block_base = b_bb.ensure<uint64_t>(nbf);
+    uint64_t* d_ne = b_bt.ensure<uint64_t>(1);
+    { KSCOPE("k_st_count_ends", st); k_st_count_ends<<<uint32_t(nbf), ST_TPB, 0, st>>>(nf, buf, d_off, block_cnt); }
+    exclusive_scan<uint64_t, uint64_t>(block_cnt, block_base, nbf, d_ne, b_tmp, st);
+    { KSCOPE("k_st_frags", st); k_st_frags<K><<<uint32_t(nbf), ST_TPB, 0, st>>>(nf, buf, d_off, k, block_base, f_len, f_nr, f_cid, ekey, eid, err); }
     check();
-    const uint64_t ne = st_d2h<unsigned long long>(ctr, st);
+    const uint64_t ne = st_d2h<uint64_t>(d_ne, st);
+    if (ne > ns) throw_on(E_INTERNAL);
     S.open_ends = ne;
     uint32_t* partner = b_partner.ensure<uint32_t>(ns);
     { KSCOPE("k_st_fill32", st); k_st_fill32<<<st_blocks(ns), ST_TPB, 0, st>>>(partner, ns, NONE32); }
@@ -697,15 +836,13 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, const st
     b_eid_t.release();
 
     // 3. list ranking over next(s) = partner[s ^ 1]
-    uint2* A = b_A.ensure<uint2>(ns);
-    uint2* B = b_B.ensure<uint2>(ns);
+    uint64_t* A = b_A.ensure<uint64_t>(ns);
     { KSCOPE("k_st_jump_init", st); k_st_jump_init<<<st_blocks(ns), ST_TPB, 0, st>>>(ns, partner, A); }
     uint64_t prev = ~0ull;
     for (int r = 0; r < 64; ++r) {
-        CDBG_HIP(hipMemsetAsync(ctr + 5, 0, sizeof(unsigned long long), st));
-        { KSCOPE("k_st_jump", st); k_st_jump<<<st_blocks(ns), ST_TPB, 0, st>>>(ns, A, B, ctr + 5); }
-        std::swap(A, B);
-        const uint64_t open = st_d2h<unsigned long long>(ctr + 5, st);
+        CDBG_HIP(hipMemsetAsync(ctr + 5 * N_STRIPE, 0, N_STRIPE * sizeof(unsigned long long), st));
+        { KSCOPE("k_st_jump", st); k_st_jump<<<st_blocks(ns), ST_TPB, 0, st>>>(ns, A, ctr + 5 * N_STRIPE); }
+        const uint64_t open = stripes(5, false);
         // a round that resolves no side leaves only sides on cycles
         if (open == 0 || open == prev) break;
         prev = open;
@@ -822,9 +959,8 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, const st
         if (nu) {
             uint64_t* key = b_key.ensure<uint64_t>(nu);
             uint32_t* ord = b_val.ensure<uint32_t>(nu);
-            CDBG_HIP(hipMemsetAsync(ctr + 6, 0, sizeof(unsigned long long), st));
-            { KSCOPE("k_st_keys", st); k_st_keys<<<st_blocks(nu), ST_TPB, 0, st>>>(nu, klist, u_cid, key, ord, ctr + 6); }
-            const int cbits = st_bits_for(st_d2h<unsigned long long>(ctr + 6, st));
+            { KSCOPE("k_st_keys", st); k_st_keys<<<st_blocks(nu), ST_TPB, 0, st>>>(nu, klist, u_cid, key, ord, ctr + 6 * N_STRIPE); }
+            const int cbits = st_bits_for(stripes(6, true));
             if (cbits) {
                 SortPlan plan;
                 plan.build({0, nu});
@@ -843,37 +979,35 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, const st
             uint8_t* tig = b_tig.ensure<uint8_t>(total_tig + 16);
             DevBuf b_ptot;
             const uint64_t P1 = make_pieces(n_lin, s_boff, total_tig, b_np, b_po, b_it, b_ptot, b_tmp, st);
-            { KSCOPE("k_st_fill_tigs", st); k_st_fill_tigs<<<st_blocks(P1 * 64), ST_TPB, 0, st>>>(P1, b_it.as<uint32_t>(), b_po.as<uint64_t>(), slot_side, ch_pos, buf, d_off, f_nr, f_len, s_boff, k, total_tig, tig, err); }
+            { KSCOPE("k_st_fill_tigs_small", st); k_st_fill_tigs_small<<<st_blocks(n_lin), ST_TPB, 0, st>>>(n_lin, slot_side, ch_pos, buf, d_off, f_nr, f_len, s_boff, k, total_tig, tig, err); }
+            if (P1) { KSCOPE("k_st_fill_tigs", st); k_st_fill_tigs<<<st_blocks(P1 * 64), ST_TPB, 0, st>>>(P1, b_it.as<uint32_t>(), b_po.as<uint64_t>(), slot_side, ch_pos, buf, d_off, f_nr, f_len, s_boff, k, total_tig, tig, err); }
             // tig sequences -> unitigs in emit order
             uint8_t* out = b_out.ensure<uint8_t>(total_out + 16);
             DevBuf b_np2, b_po2, b_it2;
             const uint64_t P2 = make_pieces(nu, ooff, total_out, b_np2, b_po2, b_it2, b_ptot, b_tmp, st);
-            { KSCOPE("k_st_copy_out", st); k_st_copy_out<<<st_blocks(P2 * 64), ST_TPB, 0, st>>>(P2, b_it2.as<uint32_t>(), b_po2.as<uint64_t>(), nu, ord, u_x, u_tig, ooff, total_out, k, tig, total_tig, out, err); }
+            { KSCOPE("k_st_copy_out_small", st); k_st_copy_out_small<<<st_blocks(nu), ST_TPB, 0, st>>>(nu, ord, u_x, u_tig, ooff, total_out, k, tig, total_tig, out, err); }
+            if (P2) { KSCOPE("k_st_copy_out", st); k_st_copy_out<<<st_blocks(P2 * 64), ST_TPB, 0, st>>>(P2, b_it2.as<uint32_t>(), b_po2.as<uint64_t>(), nu, ord, u_x, u_tig, ooff, total_out, k, tig, total_tig, out, err); }
             check();
             static_assert(sizeof(UnitigList::Item) == 24, "items are three u64");
-            L.items.resize(nu);
-            L.bases.resize(total_out);
-            CDBG_HIP(hipMemcpyAsync(L.items.data(), items, nu * 24, hipMemcpyDeviceToHost, st));
-            CDBG_HIP(hipMemcpyAsync(&L.bases[0], out, total_out, hipMemcpyDeviceToHost, st));
+            // items, then bases, in one pinned block the list keeps
+            void* hp = nullptr;
+            CDBG_HIP(hipHostMalloc(&hp, nu * 24 + total_out + 16, hipHostMallocDefault));
+            L.keep = std::shared_ptr<void>(hp, [](void* q) { (void)hipHostFree(q); });
+            L.ext_items = (const UnitigList::Item*)hp;
+            L.ext_n = nu;
+            L.ext_bases = (const char*)hp + nu * 24;
+            CDBG_HIP(hipMemcpyAsync(hp, items, nu * 24, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipMemcpyAsync((uint8_t*)hp + nu * 24, out, total_out, hipMemcpyDeviceToHost, st));
         }
-        uint64_t h[5];
-        CDBG_HIP(hipMemcpyAsync(h, ctr, 5 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        CDBG_HIP(hipStreamSynchronize(st));
-        S.joins = h[1];
-        S.unmatched_ends = h[2];
-        S.foreign_dropped = h[3];
-        S.unitig_kmers = h[4];
+        S.foreign_dropped = stripes(3, false);
+        S.unitig_kmers = stripes(4, false);
         S.num_unitigs = nu;
-    } else {
-        uint64_t h[3];
-        CDBG_HIP(hipMemcpyAsync(h, ctr, 3 * sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        CDBG_HIP(hipStreamSynchronize(st));
-        S.joins = h[1];
-        S.unmatched_ends = h[2];
     }
+    S.joins = stripes(1, false);
+    S.unmatched_ends = stripes(2, false);
     if (!cyc_frags.empty()) {
         stitch_cycles(hbuf, n, off, k, cyc_frags, cyc_partner, U->lists[1], S);
-        S.num_unitigs += U->lists[1].items.size();
+        S.num_unitigs += U->lists[1].size();
     }
     return U.release();
 }
@@ -883,14 +1017,13 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, const st
 Unitigs* stitch_fragments_gpu(int device, const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads) {
     if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
     if (threads == 0) threads = 1;
-    const std::vector<uint64_t> off = fragment_offsets(buf, n, threads);
     CDBG_HIP(hipSetDevice(device));
     StreamOwner own;
     own.s = acquire_stream(device);
     own.device = device;
     AllocStream on_stream(own.s);
-    if (k <= 31) return stitch_device<uint64_t>(own.s, buf, n, off, k);
-    return stitch_device<Key128>(own.s, buf, n, off, k);
+    if (k <= 31) return stitch_device<uint64_t>(own.s, buf, n, k, threads);
+    return stitch_device<Key128>(own.s, buf, n, k, threads);
 }
 
 }  // namespace cdbg_amd
@@ -913,9 +1046,12 @@ CDBG_API void* cdbg_gpu_stitch(cdbg_gpu_ctx* ctx, const uint8_t* frags, uint64_t
 CDBG_API void* cdbg_gpu_stitch_file(cdbg_gpu_ctx* ctx, const char* frag_path, uint32_t k, uint32_t threads) {
     try {
         if (!ctx || !frag_path) throw Error("null argument");
-        uint64_t n = 0;
-        const std::unique_ptr<uint8_t[]> buf = read_file(frag_path, threads, n);
-        return stitch_fragments_gpu(ctx->device, buf.get(), n, k, threads);
+        // read straight into pinned memory: the upload then runs at link speed
+        const uint64_t n = file_size_of(frag_path);
+        CDBG_HIP(hipSetDevice(ctx->device));
+        PinnedHost buf(n + 16);
+        read_file_into(frag_path, (uint8_t*)buf.p, n, threads);
+        return stitch_fragments_gpu(ctx->device, (const uint8_t*)buf.p, n, k, threads);
     } catch (std::exception& e) {
         set_error(e.what());
         return nullptr;
