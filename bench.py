@@ -455,8 +455,10 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
     per colour, level 6) through cdbg_gpu_bucket_write (inflate + parse,
     upload, phase 1, LZ4 bucket files) and cdbg_gpu_bucket_process (read +
     decompress bucket files, upload, phase 2, frag_unitigs.bin and
-    .color_sets), then the host stitch and emit (.fa, .u2c, finished
-    .color_sets, metadata)."""
+    .color_sets), then the device stitch (cdbg_gpu_stitch_file) and the host
+    emit (.fa, .u2c, finished .color_sets, metadata).  The host stitch of the
+    same file is timed beside it (stitch_host_s) and must give the same
+    statistics."""
     import shutil
     import tempfile
     d = tempfile.mkdtemp(prefix="cdbg_bench_", dir="/tmp")
@@ -477,14 +479,20 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
         st = ctx.bucket_process(tmp, B, k, colors, frag, out + ".color_sets", threads=threads)
         t["bucket_process_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()
-        with G.stitch_file(frag, k, threads=threads) as U:
-            nu = U.stats()["num_unitigs"]
+        with ctx.stitch_file(frag, k, threads=threads) as U:
+            sst = U.stats()
+            nu = sst["num_unitigs"]
             t["stitch_s"] = time.perf_counter() - t0
             t0 = time.perf_counter()
             U.emit_fasta(out)
         G.finalize_color_sets(out + ".color_sets", st)
         G.write_metadata(out, k, st["num_kmers"], colors, nu, st["num_color_sets"])
         t["emit_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        with G.stitch_file(frag, k, threads=threads) as H:
+            t_host = time.perf_counter() - t0
+            if H.stats() != sst:
+                raise RuntimeError("device and host stitch disagree")
         p12 = t["bucket_write_s"] + t["bucket_process_s"]
         res = {"metric": "distinct k-mers/s (phases 1+2, files)", "value": round(st["num_kmers"] / p12, 1),
                "end_to_end_kmers_per_s": round(st["num_kmers"] / sum(t.values()), 1),
@@ -492,6 +500,7 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
                "bucket_bytes_uncompressed": int(unc.sum()), "frag_bytes": os.path.getsize(frag),
                "fa_bytes": os.path.getsize(out + ".fa"), "host_threads": threads,
                **{kk: round(v, 3) for kk, v in t.items()},
+               "stitch_host_s": round(t_host, 3),
                "workload": f"first {colors} colours of the pangenome workload as gzipped FASTA (generated "
                            f"in {t_gen:.1f} s, untimed); k={k} m={m} B={B}; scratch under /tmp"}
         print(f"bench: file path {json.dumps(res)}", file=sys.stderr, flush=True)
@@ -572,6 +581,7 @@ def main():
     ap.add_argument("--large-colors", type=int, default=4000,
                     help="colours of the large resident build leg (0: skip it)")
     ap.add_argument("--large-only", action="store_true", help="run only the large resident build leg")
+    ap.add_argument("--file-only", action="store_true", help="run only the file-path leg")
     ap.add_argument("--no-bucket-files", action="store_true", help="skip the bucket-file write/read leg")
     ap.add_argument("--bucket-files-only", action="store_true",
                     help="time the plain workload, then only the bucket-file leg (no traffic runs)")
@@ -592,6 +602,12 @@ def main():
     if args.large_only:
         ctx = G.Context(0)
         print(json.dumps({"large": large_leg(G, ctx, args.large_colors or 4000, args.steps, args.cpu_cores)}))
+        ctx.close()
+        return
+    if args.file_only:
+        ctx = G.Context(0)
+        print(json.dumps({"files": file_path_leg(G, ctx, args.file_colors or 100, args.genome, args.k, args.m,
+                                                 args.buckets, args.cpu_cores)}))
         ctx.close()
         return
     if args.served_only:

@@ -3,8 +3,9 @@
 // runs on the device and writes the bucket files, phase 2 runs on the device
 // over those files and leaves frag_unitigs.bin in the scratch directory and
 // <out>.color_sets / <out>.color_sets.tmp_offsets as the reference leaves
-// them after bucket-process; the host stitch and emit (phases 3-4) then write
-// <out>.fa, <out>.u2c, the finished <out>.color_sets and <out>.metadata.txt
+// them after bucket-process; the stitch (phase 3, on the device; the host
+// stitch with --host-stitch) and the host emit (phase 4) then write <out>.fa,
+// <out>.u2c, the finished <out>.color_sets and <out>.metadata.txt
 // (builder.hpp:72-79 order).  --phase2-only stops after phase 2 and reports
 // the state the reference's own phases 3-4 need in <out>.phase2.txt.
 //
@@ -34,14 +35,14 @@ struct Config {
     uint32_t k = 0, t = 1, m = 0, b = 0;
     double g = 0, alpha = 2.0, beta = 7.0;
     uint64_t flush = 0, spill = 0;
-    bool verbose = false, keep_tmp = false, resident = false, phase2_only = false;
+    bool verbose = false, keep_tmp = false, resident = false, phase2_only = false, host_stitch = false;
 };
 
 void usage(const char* p) {
     std::fprintf(stderr,
                  "usage: %s -i <filenames_list> -o <out_basename> -k <k> [-t threads] [-m m] [-b buckets_log2]\n"
                  "          [-d tmp_dir] [-g max_ram_gb] [--alpha a] [--beta b] [--flush n] [--spill n]\n"
-                 "          [--verbose] [--keep-tmp] [--resident] [--phase2-only]\n",
+                 "          [--verbose] [--keep-tmp] [--resident] [--phase2-only] [--host-stitch]\n",
                  p);
 }
 
@@ -69,6 +70,7 @@ bool parse(int argc, char** argv, Config& c) {
         else if (a == "--keep-tmp") c.keep_tmp = true;
         else if (a == "--resident") c.resident = true;
         else if (a == "--phase2-only") c.phase2_only = true;
+        else if (a == "--host-stitch") c.host_stitch = true;
         else if (a == "-h" || a == "--help") return false;
         else throw std::runtime_error("unknown option " + a);
     }
@@ -217,9 +219,10 @@ int main(int argc, char** argv) {
                       << ".phase2.txt\n";
             return 0;
         }
+        // phase 3: stitch (on the device unless --host-stitch); phase 4: emit
+        void* U = c.host_stitch ? cdbg_stitch_file(frag.c_str(), c.k, c.t)
+                                : cdbg_gpu_stitch_file(ctx, frag.c_str(), c.k, c.t);
         cdbg_gpu_ctx_destroy(ctx);
-        // phase 3: stitch; phase 4: emit
-        void* U = cdbg_stitch_file(frag.c_str(), c.k, c.t);
         if (!U) throw std::runtime_error(cdbg_gpu_last_error());
         cdbg_stitch_stats ss{};
         int rc = cdbg_unitigs_stats(U, &ss);

@@ -22,13 +22,18 @@
 // u64 width, u64 mask, u64[] words with one spare word).  That library is
 // not in the reference tree, so this layout is restated from its published
 // design, not checked against it.
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <charconv>
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -154,57 +159,114 @@ void save_elias_fano(File& f, const std::vector<uint64_t>& v, uint64_t u) {
 
 char* put_u64(char* p, uint64_t v) { return std::to_chars(p, p + 20, v).ptr; }
 
+// fn(begin, end) over up to `threads` contiguous ranges of [0, n)
+template <class F>
+void parallel_ranges(uint64_t n, uint32_t threads, F&& fn) {
+    const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads ? threads : 1, n));
+    if (T == 1) {
+        fn(uint64_t(0), n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (uint32_t t = 0; t < T; ++t) th.emplace_back([&, t] { fn(n * t / T, n * (t + 1) / T); });
+    for (auto& x : th) x.join();
+}
+
 }  // namespace
 
 void emit_fasta(Unitigs& U, const std::string& out) {
     U.build_order();
-    File fa(out + ".fa", "wb");
-    File u2c(out + ".u2c", "wb");
     const uint64_t n = U.size();
-    u2c.pod(n);
-    u2c.pod(uint64_t((n + 63) / 64));
-    std::vector<char> buf(1 << 20);
-    size_t pos = 0;
-    uint64_t word = 0, words_out = 0;
-    for (uint64_t i = 0; i < n; ++i) {
-        const auto& it = U.item(i);
-        const uint64_t next_cid = i + 1 < n ? U.item(i + 1).cid : ~it.cid;
-        if (next_cid != it.cid) word |= uint64_t(1) << (i & 63);
-        if ((i & 63) == 63) {
-            u2c.pod(word);
-            word = 0;
-            ++words_out;
-        }
-        if (pos + 24 > buf.size()) {
-            fa.write(buf.data(), pos);
-            pos = 0;
-        }
-        buf[pos++] = '>';
-        pos = size_t(put_u64(buf.data() + pos, it.cid) - buf.data());
-        buf[pos++] = '\n';
-        const char* s = U.seq(i);
-        uint64_t left = it.len;
-        while (left) {
-            if (pos == buf.size()) {
-                fa.write(buf.data(), pos);
-                pos = 0;
+    // .u2c: bit i set iff unitig i ends a run of equal ids
+    {
+        File u2c(out + ".u2c", "wb");
+        u2c.pod(n);
+        const uint64_t nw = (n + 63) / 64;
+        u2c.pod(nw);
+        std::vector<uint64_t> words(nw, 0);
+        parallel_ranges(nw, U.threads, [&](uint64_t b, uint64_t e) {
+            for (uint64_t w = b; w < e; ++w) {
+                uint64_t word = 0;
+                const uint64_t i1 = std::min(n, 64 * w + 64);
+                for (uint64_t i = 64 * w; i < i1; ++i)
+                    if (i + 1 == n || U.item(i + 1).cid != U.item(i).cid) word |= uint64_t(1) << (i & 63);
+                words[w] = word;
             }
-            const size_t take = std::min<uint64_t>(left, buf.size() - pos);
-            std::memcpy(buf.data() + pos, s, take);
-            pos += take;
-            s += take;
-            left -= take;
-        }
-        if (pos == buf.size()) {
-            fa.write(buf.data(), pos);
-            pos = 0;
-        }
-        buf[pos++] = '\n';
+        });
+        u2c.write(words.data(), nw * 8);
+        u2c.close();
     }
-    fa.write(buf.data(), pos);
-    if (words_out < (n + 63) / 64) u2c.pod(word);
-    fa.close();
-    u2c.close();
+    // .fa: ">cid\n<seq>\n" per unitig; contiguous ranges of unitigs are sized,
+    // placed by a prefix sum and formatted and written in parallel
+    const std::string fa_path = out + ".fa";
+    const int fd = ::open(fa_path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) throw Error("cannot open " + fa_path);
+    try {
+        const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(U.threads, n / 4096 + 1));
+        std::vector<uint64_t> at(T + 1, 0);
+        auto digits = [](uint64_t v) {
+            uint32_t d = 1;
+            while (v >= 10) {
+                v /= 10;
+                ++d;
+            }
+            return d;
+        };
+        parallel_ranges(T, T, [&](uint64_t b, uint64_t e) {
+            for (uint64_t t = b; t < e; ++t) {
+                uint64_t bytes = 0;
+                for (uint64_t i = n * t / T; i < n * (t + 1) / T; ++i) bytes += 3 + digits(U.item(i).cid) + U.item(i).len;
+                at[t + 1] = bytes;
+            }
+        });
+        for (uint32_t t = 0; t < T; ++t) at[t + 1] += at[t];
+        std::atomic<bool> failed{false};
+        parallel_ranges(T, T, [&](uint64_t b, uint64_t e) {
+            for (uint64_t t = b; t < e; ++t) {
+                std::vector<char> buf(size_t(1) << 22);
+                size_t pos = 0;
+                uint64_t where = at[t];
+                auto flush = [&]() {
+                    size_t done = 0;
+                    while (done < pos) {
+                        const ssize_t w = ::pwrite(fd, buf.data() + done, pos - done, off_t(where + done));
+                        if (w <= 0) {
+                            failed = true;
+                            return;
+                        }
+                        done += size_t(w);
+                    }
+                    where += pos;
+                    pos = 0;
+                };
+                for (uint64_t i = n * t / T; i < n * (t + 1) / T && !failed; ++i) {
+                    const auto& it = U.item(i);
+                    if (pos + 24 > buf.size()) flush();
+                    buf[pos++] = '>';
+                    pos = size_t(put_u64(buf.data() + pos, it.cid) - buf.data());
+                    buf[pos++] = '\n';
+                    const char* sq = U.seq(i);
+                    uint64_t left = it.len;
+                    while (left) {
+                        if (pos == buf.size()) flush();
+                        const size_t take = std::min<uint64_t>(left, buf.size() - pos);
+                        std::memcpy(buf.data() + pos, sq, take);
+                        pos += take;
+                        sq += take;
+                        left -= take;
+                    }
+                    if (pos == buf.size()) flush();
+                    buf[pos++] = '\n';
+                }
+                flush();
+            }
+        });
+        if (failed) throw Error("short write to " + fa_path);
+        if (::close(fd) != 0) throw Error("cannot close " + fa_path);
+    } catch (...) {
+        ::close(fd);
+        throw;
+    }
 }
 
 void finalize_color_sets(const std::string& cs_path, const cdbg_gpu_color_state& s) {

@@ -651,6 +651,7 @@ Unitigs* stitch_fragments(const uint8_t* buf, uint64_t n, uint32_t k, uint32_t t
     }
     auto* U = new Unitigs();
     U->k = k;
+    U->threads = threads;
     const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads, std::max<uint64_t>(starts.size() / 4096, 1)));
     U->lists.resize(T + 1);  // the last list holds the cycles
     S.used.assign(nf, 0);

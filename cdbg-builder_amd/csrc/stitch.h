@@ -41,6 +41,7 @@ struct UnitigList {
 
 struct Unitigs {
     uint32_t k = 0;
+    uint32_t threads = 1;  // host threads of the stitch, used again by emit
     std::vector<UnitigList> lists;
     cdbg_stitch_stats stats{};
     // emit order, computed on first use (left empty when a single list holds

@@ -756,6 +756,7 @@ template <class K>
 Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t k, uint32_t threads) {
     std::unique_ptr<Unitigs> U(new Unitigs());
     U->k = k;
+    U->threads = threads;
     U->lists.resize(2);  // the linear tigs' unitigs, then the cycles'
     cdbg_stitch_stats& S = U->stats;
     DevBuf b_buf, b_off, b_len, b_nr, b_cid, b_ekey, b_eid, b_ekey_t, b_eid_t, b_ctr, b_err, b_partner, b_A;

@@ -4,7 +4,8 @@ rules of builder.hpp:550-649).
 CPU: argument and configuration errors exit with status 1 before any device
 work.  GPU: a run over FASTA files leaves the same phase-2 artifacts as the
 oracle's phase 2 over the bucket files the run kept (--keep-tmp), and the
-resident mode (no bucket files) leaves identical artifacts.
+resident mode (no bucket files) and the host stitch (--host-stitch) leave
+identical deliverables.
 """
 import os
 import subprocess
@@ -158,3 +159,9 @@ def test_cli_end_to_end(tmp_path, k, m, b):
     assert r.returncode == 0, r.stderr
     for ext in (".fa", ".u2c", ".color_sets", ".metadata.txt"):
         assert open(f"{out2}{ext}", "rb").read() == open(f"{out}{ext}", "rb").read(), ext
+    # the host stitch (--host-stitch) writes the same deliverables as the device stitch
+    d3, out3 = tmp_path / "scratch3", tmp_path / "hst"
+    r = _run("-i", lst, "-o", out3, "-k", k, "-m", m, "-b", b, "-d", d3, "--host-stitch")
+    assert r.returncode == 0, r.stderr
+    for ext in (".fa", ".u2c", ".color_sets", ".metadata.txt"):
+        assert open(f"{out3}{ext}", "rb").read() == open(f"{out}{ext}", "rb").read(), ext

@@ -107,6 +107,17 @@ def test_stitch_equals_test_side_stitch():
     assert st["num_fragments"] == ob.stat("num_frags")
 
 
+def test_emit_is_independent_of_the_thread_count(tmp_path):
+    data, off = G.synth(seed=19, num_colors=10, genome_len=30000, snp_rate=0.01, indel_rate=0.001)
+    f = O.Build(data, off, 31, 19, 64, threads=4).frags()
+    outs = []
+    for th in (1, 3, 16):
+        with G.stitch(f, 31, threads=th) as U:
+            U.emit_fasta(str(tmp_path / f"e{th}"))
+        outs.append(((tmp_path / f"e{th}.fa").read_bytes(), (tmp_path / f"e{th}.u2c").read_bytes()))
+    assert outs[0] == outs[1] == outs[2] and len(outs[0][0]) > 100000
+
+
 def test_stitch_rejects_truncated_input():
     data, off = G.synth(seed=2, num_colors=2, genome_len=3000)
     f = O.Build(data, off, 21, 11, 8).frags()
