@@ -50,9 +50,12 @@
 #include <cstdio>
 #include <cstring>
 #include <filesystem>
+#include <mutex>
 #include <thread>
+#include <vector>
 
 #include "job.h"
+#include "stitch.h"
 #include "kmer_ops.h"
 #include "lz4_device.h"
 
@@ -914,22 +917,119 @@ void load_bucket_files(Job& J, const std::string& dir, uint32_t threads, bool de
         }
 }
 
+namespace {
+
+// Pinned memory is mapped for the host on first touch, a fault per page:
+// a fresh 3.5 GB block the device filled took seconds to write out, its
+// pages faulting one by one under the file's write lock.  Device bytes are
+// written through two 64 MiB pinned buffers kept for the process (the copy
+// of one chunk overlaps the write of the other); bytes already in pinned
+// host memory have their pages touched by `threads` threads first.
+constexpr uint64_t STAGE = uint64_t(64) << 20;
+
+struct StagePool {
+    std::mutex mu;
+    std::vector<uint8_t*> free;
+    uint8_t* take() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!free.empty()) {
+                uint8_t* p = free.back();
+                free.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        CDBG_HIP(hipHostMalloc(&p, STAGE, hipHostMallocDefault));
+        return (uint8_t*)p;
+    }
+    void give(uint8_t* p) {
+        std::lock_guard<std::mutex> lk(mu);
+        free.push_back(p);
+    }
+};
+StagePool g_stage;
+
+void write_device_stream(const std::string& path, const uint8_t* d, uint64_t n, hipStream_t st) {
+    struct Bufs {
+        uint8_t* b[2];
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        ~Bufs() {
+            for (int i = 0; i < 2; ++i) {
+                if (ev[i]) (void)hipEventDestroy(ev[i]);
+                if (b[i]) g_stage.give(b[i]);
+            }
+        }
+    } B{{nullptr, nullptr}};
+    for (int i = 0; i < 2; ++i) {
+        B.b[i] = g_stage.take();
+        CDBG_HIP(hipEventCreateWithFlags(&B.ev[i], hipEventDisableTiming));
+    }
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw Error("cannot open " + path);
+    bool ok = true;
+    try {
+        const uint64_t nc = (n + STAGE - 1) / STAGE;
+        auto issue = [&](uint64_t c) {
+            const uint64_t len = std::min<uint64_t>(STAGE, n - c * STAGE);
+            CDBG_HIP(hipMemcpyAsync(B.b[c & 1], d + c * STAGE, len, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipEventRecord(B.ev[c & 1], st));
+        };
+        if (nc) issue(0);
+        for (uint64_t c = 0; c < nc && ok; ++c) {
+            if (c + 1 < nc) issue(c + 1);  // its buffer was written out last round
+            CDBG_HIP(hipEventSynchronize(B.ev[c & 1]));
+            const uint64_t len = std::min<uint64_t>(STAGE, n - c * STAGE);
+            ok = std::fwrite(B.b[c & 1], 1, len, f) == len;
+        }
+        CDBG_HIP(hipStreamSynchronize(st));
+    } catch (...) {
+        std::fclose(f);
+        throw;
+    }
+    if (std::fclose(f) != 0 || !ok) throw Error("short write to " + path);
+}
+
+void write_host_stream(const std::string& path, const uint8_t* h, uint64_t n, uint32_t threads) {
+    const uint64_t pages = (n + 4095) / 4096;
+    const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads, pages / 1024 + 1));
+    std::vector<std::thread> th;
+    std::atomic<uint64_t> sink{0};
+    for (uint32_t t = 0; t < T; ++t)
+        th.emplace_back([&, t] {
+            uint64_t acc = 0;
+            for (uint64_t p = pages * t / T; p < pages * (t + 1) / T; ++p) acc += ((const volatile uint8_t*)h)[p * 4096];
+            sink += acc;
+        });
+    for (auto& x : th) x.join();
+    write_file_parallel(path.c_str(), h, n, 1);
+}
+
+}  // namespace
+
 void write_phase2_files(Job& J, const std::string& frag_path, const std::string& cs_path,
                         cdbg_gpu_color_state* state) {
     AllocStream alloc_on_job_stream(J.st);
     if (!J.phase2_done) throw Error("phase 2 has not run");
     const cdbg_gpu_stats& s = J.stats;
-    std::vector<uint8_t> frags(s.frag_bytes);
     const uint64_t nw = (s.cs_bits + 63) / 64;
     std::vector<uint64_t> words(nw), offs(s.num_color_sets);
-    copy_phase2_results(J, frags.data(), words.data(), offs.data());
+    copy_phase2_results(J, nullptr, words.data(), offs.data());
     auto write_all = [](const std::string& path, const void* p, size_t n, const char* mode) {
         FILE* f = std::fopen(path.c_str(), mode);
         if (!f) throw Error("cannot open " + path);
         const bool ok = (n == 0 || std::fwrite(p, 1, n, f) == n);
         if (std::fclose(f) != 0 || !ok) throw Error("short write to " + path);
     };
-    write_all(frag_path, frags.data(), frags.size(), "wb");
+    {
+        bool on_device = false;
+        uint64_t n = 0;
+        const uint8_t* src = phase2_frags_where(J, on_device, n);
+        if (on_device)
+            write_device_stream(frag_path, src, n, J.st);
+        else
+            write_host_stream(frag_path, src, n, J.io_threads);
+    }
     // colour-set stream as left by phase 2: zero header + complete words
     std::vector<uint8_t> cs(28, 0);
     const uint64_t full = s.cs_bits / 64;

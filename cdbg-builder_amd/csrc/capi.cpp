@@ -163,6 +163,7 @@ CDBG_API int cdbg_gpu_job_create_from_files(cdbg_gpu_ctx* ctx, const char* const
         try {
             Job& J = h->J;
             job_reset(J, ctx, n_files, k, m, num_buckets);
+            J.io_threads = std::max<uint32_t>(1, threads);
             J.input_on_device = load_inputs_device(J, paths, threads);
             if (!J.input_on_device) {  // FASTQ quality lines: the host reader
                 void* seqs = cdbg_load_fasta(files, n_files, threads);
@@ -395,6 +396,7 @@ static int job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, uint32_t nu
             J.st = acquire_stream(ctx->device);
             J.stream_owner.s = J.st;
             J.stream_owner.device = ctx->device;
+            J.io_threads = std::max<uint32_t>(1, threads);
             load_bucket_files(J, dir, threads, delete_consumed);
         } catch (...) {
             delete h;

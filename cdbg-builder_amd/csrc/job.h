@@ -77,6 +77,7 @@ struct Job {
     cdbg_gpu_tuning tune{};       // phase-2 performance knobs (zero fields: defaults)
     uint32_t p2_max_chunks = 0;   // stop phase 2 after this many chunks (0: all; autotuner trials)
     bool input_on_device = false; // the input was decoded and parsed on the device
+    uint32_t io_threads = 1;      // host threads for the job's file reads and writes
     ~Job();
 };
 
@@ -97,6 +98,9 @@ cdbg_gpu_tune_report autotune(Job& J, uint64_t sample_occurrences);
 void free_phase2(Job& J);
 // Host copies of the phase-2 results (any pointer may be null).
 void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offsets);
+// Where the fragment stream of the last phase-2 run is (n bytes): on the
+// device, or in phase 2's own pinned host copy.
+const uint8_t* phase2_frags_where(Job& J, bool& on_device, uint64_t& n);
 // Device copies of the same results on stream st (waits for the job first).
 void copy_phase2_results_device(Job& J, void* frags, void* words, void* offsets, hipStream_t st);
 // bucket_io.hip

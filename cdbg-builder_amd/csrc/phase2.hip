@@ -3484,6 +3484,15 @@ void copy_phase2_results(Job& J, uint8_t* frags, uint64_t* words, uint64_t* offs
     if (offsets && P.n_sets) CDBG_HIP(hipMemcpy(offsets, P.set_off.p, P.n_sets * 8, hipMemcpyDeviceToHost));
 }
 
+const uint8_t* phase2_frags_where(Job& J, bool& on_device, uint64_t& n) {
+    if (!J.phase2_done) throw Error("phase 2 has not run");
+    Phase2Bufs& P = *J.p2;
+    CDBG_HIP(hipStreamSynchronize(J.st));
+    n = P.frag_total;
+    on_device = !P.spill;
+    return P.spill ? P.h_frags.p : (const uint8_t*)P.frag_bytes.p;
+}
+
 }  // namespace cdbg_amd
 
 using namespace cdbg_amd;

@@ -536,6 +536,31 @@ void read_file_into(const char* path, uint8_t* dst, uint64_t n, uint32_t threads
     if (short_read) throw Error(std::string("short read of ") + path);
 }
 
+void write_file_parallel(const char* path, const void* src, uint64_t n, uint32_t threads) {
+    const int fd = ::open(path, O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd < 0) throw Error(std::string("cannot open ") + path);
+    // ranges of at least 64 MiB per thread, written at their offsets
+    const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads ? threads : 1, n >> 26));
+    std::atomic<bool> failed{false};
+    parallel_for(T, T, [&](uint64_t b, uint64_t e, uint32_t) {
+        for (uint64_t t = b; t < e; ++t) {
+            uint64_t p = n * t / T;
+            const uint64_t end = n * (t + 1) / T;
+            while (p < end && !failed) {
+                const ssize_t w = ::pwrite(fd, (const uint8_t*)src + p, size_t(std::min<uint64_t>(end - p, 1ull << 30)),
+                                           off_t(p));
+                if (w <= 0) {
+                    failed = true;
+                    break;
+                }
+                p += uint64_t(w);
+            }
+        }
+    }, 1);
+    const bool closed = ::close(fd) == 0;
+    if (failed || !closed) throw Error(std::string("short write to ") + path);
+}
+
 std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_t& n) {
     n = file_size_of(path);
     std::unique_ptr<uint8_t[]> buf(new uint8_t[n + 1]);

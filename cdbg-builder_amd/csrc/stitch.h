@@ -85,6 +85,9 @@ std::unique_ptr<uint8_t[]> read_file(const char* path, uint32_t threads, uint64_
 uint64_t file_size_of(const char* path);
 // Reads the n bytes of the file at `path` into dst with up to `threads` threads.
 void read_file_into(const char* path, uint8_t* dst, uint64_t n, uint32_t threads);
+// Writes n bytes from src as the file at `path` (created or truncated) with
+// up to `threads` threads, each writing its range at its offset.
+void write_file_parallel(const char* path, const void* src, uint64_t n, uint32_t threads);
 // memcpy split over up to `threads` threads (16 MiB or more each).
 void parallel_copy(void* dst, const void* src, uint64_t n, uint32_t threads);
 
