@@ -574,11 +574,12 @@ void parallel_copy(void* dst, const void* src, uint64_t n, uint32_t threads) {
     }, uint64_t(1) << 24);
 }
 
-void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& off, uint32_t k,
+void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& rec, uint32_t k,
                    const std::vector<uint32_t>& frags, const std::vector<uint32_t>& partner, UnitigList& out,
                    cdbg_stitch_stats& st) {
     const uint64_t nc = frags.size();
-    if (partner.size() != 2 * nc) throw Error("stitch: cycle partners do not match the cycle fragments");
+    if (partner.size() != 2 * nc || rec.size() != nc)
+        throw Error("stitch: cycle partners or records do not match the cycle fragments");
     Stitcher S{buf, n, k, 1, {}, {}, {}, {}};
     S.off.resize(nc);
     S.partner.resize(2 * nc);
@@ -588,9 +589,9 @@ void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& 
         return uint64_t(it - frags.begin());
     };
     for (uint64_t l = 0; l < nc; ++l) {
-        if (frags[l] >= off.size() || (l && frags[l] <= frags[l - 1]))
-            throw Error("stitch: cycle fragments out of range or order");
-        S.off[l] = off[frags[l]];
+        if ((l && frags[l] <= frags[l - 1]) || rec[l] >= n || record_size(buf, n, rec[l]) == 0)
+            throw Error("stitch: cycle fragments out of order or their records out of range");
+        S.off[l] = rec[l];
         for (uint32_t s = 0; s < 2; ++s) {
             const uint32_t p = partner[2 * l + s];
             S.partner[2 * l + s] = p == ~0u ? NONE : (local(p >> 1) << 1) | (p & 1u);

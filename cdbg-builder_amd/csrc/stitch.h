@@ -72,11 +72,11 @@ Unitigs* stitch_fragments(const uint8_t* buf, uint64_t n, uint32_t k, uint32_t t
 std::vector<uint64_t> fragment_offsets(const uint8_t* buf, uint64_t n, uint32_t threads);
 
 // The tigs that close on themselves, over the fragments `frags` (ascending
-// fragment numbers, records at off[]) whose sides pair up among themselves:
-// partner[2 l + s] is the (fragment << 1 | side) joined to side s of frags[l],
-// or ~0u.  Walked in fragment order as the host stitch walks them; the
-// unitigs go to `out`, sorted stably by colour-set id.
-void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& off, uint32_t k,
+// fragment numbers; the record of frags[l] at buf + rec[l]) whose sides pair
+// up among themselves: partner[2 l + s] is the (fragment << 1 | side) joined
+// to side s of frags[l], or ~0u.  Walked in fragment order as the host stitch
+// walks them; the unitigs go to `out`, sorted stably by colour-set id.
+void stitch_cycles(const uint8_t* buf, uint64_t n, const std::vector<uint64_t>& rec, uint32_t k,
                    const std::vector<uint32_t>& frags, const std::vector<uint32_t>& partner, UnitigList& out,
                    cdbg_stitch_stats& st);
 
@@ -94,5 +94,7 @@ void parallel_copy(void* dst, const void* src, uint64_t n, uint32_t threads);
 // Phase 3 on the device (stitch_gpu.hip): the same unitigs, in the same
 // order, as stitch_fragments.
 Unitigs* stitch_fragments_gpu(int device, const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads);
+// The same over the file at `path`, streamed to the device.
+Unitigs* stitch_file_gpu(int device, const char* path, uint32_t k, uint32_t threads);
 
 }  // namespace cdbg_amd

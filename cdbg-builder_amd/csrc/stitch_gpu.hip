@@ -9,7 +9,9 @@
 // foreign placeholder runs at their ends are dropped (ext_split_monochromatic
 // :112-151).
 //
-// Method.  A fragment side s = fragment << 1 | side; entering fragment i at
+// Method.  The stream is on the device (a file goes through pinned staging
+// buffers); its record starts are found there (see k_st_parse_guess).  A
+// fragment side s = fragment << 1 | side; entering fragment i at
 // side s and leaving it by s ^ 1 continues into partner[s ^ 1], so
 // next(s) = partner[s ^ 1] is injective and its graph is paths and cycles.
 //   1. one thread per fragment checks the record (length, run coverage),
@@ -32,14 +34,16 @@
 //      foreign ones at tig ends are dropped, inside a tig they are an error;
 //   7. a stable radix sort by colour-set id gives the emit order, and two
 //      copies (fragments -> tig sequences -> unitigs) give the bases.
-// Fragments on cycles go back to the host stitch (stitch_cycles), which
-// walks them in fragment order exactly as it would have.
+// Fragments on cycles go back to the host stitch (stitch_cycles) with their
+// records, and are walked in fragment order exactly as it would have.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <fcntl.h>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <unistd.h>
@@ -741,58 +745,219 @@ uint64_t make_pieces(uint64_t n, const uint64_t* scan, uint64_t total, DevBuf& n
     return P;
 }
 
-// Pinned host memory (device copies from and to it run at link speed).
-struct PinnedHost {
-    void* p = nullptr;
-    explicit PinnedHost(size_t bytes) { CDBG_HIP(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault)); }
-    ~PinnedHost() {
-        if (p) (void)hipHostFree(p);
-    }
-    PinnedHost(const PinnedHost&) = delete;
-    PinnedHost& operator=(const PinnedHost&) = delete;
-};
+// Record starts on the device.  The stream is cut into PARSE_CHUNK-byte
+// chunks; a thread per chunk guesses the chunk's first record (an open-flag
+// byte after a base, opening four records that fit with ACGT at both ends
+// of their sequences; chunk 0 starts at 0) and follows the chain of records
+// past the chunk's end.  The host then follows the true chain chunk by
+// chunk: a chunk whose guess is where the chain enters it keeps its count,
+// any other is walked again from the entry (one thread), and a second pass
+// writes every chunk's starts at its place.  Same starts and the same error
+// as one sequential parse.
+constexpr uint64_t PARSE_CHUNK = uint64_t(64) << 10;
+constexpr uint64_t NO_START = ~0ull;
 
+__device__ __forceinline__ bool d_is_base(uint8_t c) { return c == 'A' || c == 'C' || c == 'G' || c == 'T'; }
+
+// size of the record at p, 0 when it does not fit in [p, n)
+__device__ __forceinline__ uint64_t d_record_size(const uint8_t* __restrict__ buf, uint64_t n, uint64_t p) {
+    if (p >= n || n - p < 9) return 0;
+    const uint64_t nr = ld32(buf + p + 1);
+    if (n - p < 9 + 12 * nr) return 0;
+    const uint64_t len = ld32(buf + p + 5 + 12 * nr);
+    if (n - p - 9 - 12 * nr < len) return 0;
+    return 9 + 12 * nr + len;
+}
+
+__device__ bool d_plausible(const uint8_t* __restrict__ buf, uint64_t n, uint64_t p) {
+    if (p == 0 || p >= n || buf[p] > 3 || !d_is_base(buf[p - 1])) return false;
+    for (int r = 0; r < 4 && p < n; ++r) {
+        if (buf[p] > 3) return false;
+        const uint64_t s = d_record_size(buf, n, p);
+        if (!s) return false;
+        const uint64_t seq = p + 9 + 12 * uint64_t(ld32(buf + p + 1));
+        if (seq < p + s && (!d_is_base(buf[seq]) || !d_is_base(buf[p + s - 1]))) return false;
+        p += s;
+    }
+    return true;
+}
+
+// chain from `from` while it is below hi: records, where it leaves, whether
+// a record did not fit
+__device__ void d_walk(const uint8_t* __restrict__ buf, uint64_t n, uint64_t from, uint64_t hi, uint64_t& cnt,
+                       uint64_t& exitp, uint8_t& bad) {
+    uint64_t p = from, m = 0;
+    bad = 0;
+    while (p < hi) {
+        const uint64_t s = d_record_size(buf, n, p);
+        if (!s) {
+            bad = 1;
+            break;
+        }
+        ++m;
+        p += s;
+    }
+    cnt = m;
+    exitp = p;
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_parse_guess(uint64_t nc, const uint8_t* __restrict__ buf, uint64_t n,
+                                                           uint64_t* __restrict__ guess, uint64_t* __restrict__ cnt,
+                                                           uint64_t* __restrict__ exitp, uint8_t* __restrict__ bad) {
+    const uint64_t c = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (c >= nc) return;
+    const uint64_t lo = c * PARSE_CHUNK, hi = n - lo < PARSE_CHUNK ? n : lo + PARSE_CHUNK;
+    uint64_t p = lo;
+    if (c > 0)
+        while (p < hi && !d_plausible(buf, n, p)) ++p;
+    if (p >= hi) {
+        guess[c] = NO_START;
+        cnt[c] = 0;
+        exitp[c] = 0;
+        bad[c] = 0;
+        return;
+    }
+    guess[c] = p;
+    uint64_t m, e;
+    uint8_t b;
+    d_walk(buf, n, p, hi, m, e, b);
+    cnt[c] = m;
+    exitp[c] = e;
+    bad[c] = b;
+}
+
+// one chunk walked again from where the true chain enters it
+__global__ void k_st_parse_walk(const uint8_t* __restrict__ buf, uint64_t n, uint64_t from, uint64_t hi,
+                                uint64_t* __restrict__ out) {
+    if (threadIdx.x || blockIdx.x) return;
+    uint64_t m, e;
+    uint8_t b;
+    d_walk(buf, n, from, hi, m, e, b);
+    out[0] = m;
+    out[1] = e;
+    out[2] = b;
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_parse_write(uint64_t nc, const uint8_t* __restrict__ buf, uint64_t n,
+                                                           const uint64_t* __restrict__ entry,
+                                                           const uint64_t* __restrict__ base, uint64_t nf,
+                                                           uint64_t* __restrict__ off, uint32_t* __restrict__ err) {
+    const uint64_t c = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (c >= nc || entry[c] == NO_START) return;
+    const uint64_t lo = c * PARSE_CHUNK, hi = n - lo < PARSE_CHUNK ? n : lo + PARSE_CHUNK;
+    uint64_t p = entry[c], j = base[c];
+    while (p < hi) {
+        const uint64_t s = d_record_size(buf, n, p);
+        if (!s || j >= nf) {
+            atomicOr(err, E_INTERNAL);
+            return;
+        }
+        off[j++] = p;
+        p += s;
+    }
+}
+
+// Records of the fragments on cycles, gathered into one block (a wave each).
+__global__ __launch_bounds__(ST_TPB) void k_st_gather_records(uint64_t nc, const uint32_t* __restrict__ list,
+                                                              const uint64_t* __restrict__ off, uint64_t nf, uint64_t n,
+                                                              const uint64_t* __restrict__ dst_off,
+                                                              const uint8_t* __restrict__ buf,
+                                                              uint8_t* __restrict__ dst) {
+    const uint64_t w = (uint64_t(blockIdx.x) * ST_TPB + threadIdx.x) >> 6;
+    if (w >= nc) return;
+    const uint32_t f = list[w];
+    const uint64_t a = off[f], b = f + 1 < nf ? off[f + 1] : n;
+    for (uint64_t q = (threadIdx.x & 63); q < b - a; q += 64) dst[dst_off[w] + q] = buf[a + q];
+}
+
+__global__ __launch_bounds__(ST_TPB) void k_st_record_sizes(uint64_t nc, const uint32_t* __restrict__ list,
+                                                            const uint64_t* __restrict__ off, uint64_t nf, uint64_t n,
+                                                            uint64_t* __restrict__ size) {
+    const uint64_t l = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
+    if (l >= nc) return;
+    const uint32_t f = list[l];
+    size[l] = (f + 1 < nf ? off[f + 1] : n) - off[f];
+}
+
+
+// Record starts of the device stream buf[0, n) into b_off; returns their
+// number.
+uint64_t device_offsets(const uint8_t* buf, uint64_t n, DevBuf& b_off, uint32_t* err, hipStream_t st) {
+    if (n == 0) return 0;
+    const uint64_t nc = (n + PARSE_CHUNK - 1) / PARSE_CHUNK;
+    DevBuf b_g, b_c, b_e, b_b, b_w, b_entry, b_base;
+    uint64_t* g = b_g.ensure<uint64_t>(nc);
+    uint64_t* c = b_c.ensure<uint64_t>(nc);
+    uint64_t* e = b_e.ensure<uint64_t>(nc);
+    uint8_t* bad = b_b.ensure<uint8_t>(nc);
+    { KSCOPE("k_st_parse_guess", st); k_st_parse_guess<<<st_blocks(nc), ST_TPB, 0, st>>>(nc, buf, n, g, c, e, bad); }
+    std::vector<uint64_t> hg(nc), hc(nc), he(nc), entry(nc, NO_START), base(nc, 0);
+    std::vector<uint8_t> hb(nc);
+    CDBG_HIP(hipMemcpyAsync(hg.data(), g, nc * 8, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipMemcpyAsync(hc.data(), c, nc * 8, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipMemcpyAsync(he.data(), e, nc * 8, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipMemcpyAsync(hb.data(), bad, nc, hipMemcpyDeviceToHost, st));
+    CDBG_HIP(hipStreamSynchronize(st));
+    // the true chain, chunk by chunk
+    uint64_t at = 0, nf = 0;
+    uint64_t* w = b_w.ensure<uint64_t>(3);
+    for (uint64_t ch = 0; ch < nc; ++ch) {
+        const uint64_t lo = ch * PARSE_CHUNK, hi = std::min(n, lo + PARSE_CHUNK);
+        base[ch] = nf;
+        if (at >= hi) continue;  // a record spans the whole chunk
+        uint64_t m, ex, bd;
+        if (hg[ch] == at && !hb[ch]) {
+            m = hc[ch];
+            ex = he[ch];
+            bd = 0;
+        } else {
+            { KSCOPE("k_st_parse_walk", st); k_st_parse_walk<<<1, 64, 0, st>>>(buf, n, at, hi, w); }
+            uint64_t r[3];
+            CDBG_HIP(hipMemcpyAsync(r, w, 24, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipStreamSynchronize(st));
+            m = r[0];
+            ex = r[1];
+            bd = r[2];
+        }
+        if (bd || ex < hi) throw Error("stitch: truncated fragment record");
+        entry[ch] = at;
+        nf += m;
+        at = ex;
+    }
+    if (at != n) throw Error("stitch: truncated fragment record");
+    if (nf >= (1ull << 30)) throw Error("stitch: 2^30 or more fragments for the device stitch");
+    uint64_t* d_entry = b_entry.ensure<uint64_t>(nc);
+    uint64_t* d_base = b_base.ensure<uint64_t>(nc);
+    CDBG_HIP(hipMemcpyAsync(d_entry, entry.data(), nc * 8, hipMemcpyHostToDevice, st));
+    CDBG_HIP(hipMemcpyAsync(d_base, base.data(), nc * 8, hipMemcpyHostToDevice, st));
+    uint64_t* off = b_off.ensure<uint64_t>(nf);
+    { KSCOPE("k_st_parse_write", st); k_st_parse_write<<<st_blocks(nc), ST_TPB, 0, st>>>(nc, buf, n, d_entry, d_base, nf, off, err); }
+    CDBG_HIP(hipStreamSynchronize(st));  // the host tables are local
+    return nf;
+}
+
+// The stitch of the fragment stream buf[0, n) already on the device.
 template <class K>
-Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t k, uint32_t threads) {
+Unitigs* stitch_device(hipStream_t st, const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads) {
     std::unique_ptr<Unitigs> U(new Unitigs());
     U->k = k;
     U->threads = threads;
     U->lists.resize(2);  // the linear tigs' unitigs, then the cycles'
     cdbg_stitch_stats& S = U->stats;
-    DevBuf b_buf, b_off, b_len, b_nr, b_cid, b_ekey, b_eid, b_ekey_t, b_eid_t, b_ctr, b_err, b_partner, b_A;
+    DevBuf b_off, b_len, b_nr, b_cid, b_ekey, b_eid, b_ekey_t, b_eid_t, b_ctr, b_err, b_partner, b_A;
     DevBuf b_chs, b_chst, b_chp, b_clen, b_coff, b_cyc, b_tot, b_tmp, b_cnt, b_slot;
     RadixScratch rs;
-    // the stream goes to the device while the host finds the record starts
-    uint8_t* buf = b_buf.ensure<uint8_t>(n + 16);
-    std::vector<uint64_t> off;
-    {
-        std::string perr;
-        std::thread parse([&] {
-            try {
-                off = fragment_offsets(hbuf, n, threads);
-            } catch (std::exception& e) {
-                perr = e.what();
-            }
-        });
-        hipError_t e = n ? hipMemcpyAsync(buf, hbuf, n, hipMemcpyHostToDevice, st) : hipSuccess;
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        parse.join();
-        if (!perr.empty()) throw Error(perr);
-        CDBG_HIP(e);
-    }
-    const uint64_t nf = off.size();
+    uint32_t* err = b_err.ensure<uint32_t>(1);
+    CDBG_HIP(hipMemsetAsync(err, 0, 4, st));
+    const uint64_t nf = device_offsets(buf, n, b_off, err, st);
+    throw_on(st_d2h<uint32_t>(err, st));
     S.num_fragments = nf;
     if (nf == 0) return U.release();
-    if (nf >= (1ull << 30)) throw Error("stitch: 2^30 or more fragments for the device stitch");
     const uint64_t ns = 2 * nf;
-    uint64_t* d_off = b_off.ensure<uint64_t>(nf);
-    PinnedHost p_off(nf * 8);
-    parallel_copy(p_off.p, off.data(), nf * 8, threads);
-    CDBG_HIP(hipMemcpyAsync(d_off, p_off.p, nf * 8, hipMemcpyHostToDevice, st));
+    const uint64_t* d_off = b_off.as<uint64_t>();
     // counters (N_STRIPE words each): 1 joins, 2 unmatched ends, 3 foreign
     // k-mers dropped, 4 unitig k-mers, 5 open sides of a jump round, 6 largest id
     unsigned long long* ctr = b_ctr.ensure<unsigned long long>(8 * N_STRIPE);
-    uint32_t* err = b_err.ensure<uint32_t>(1);
     CDBG_HIP(hipMemsetAsync(ctr, 0, 8 * N_STRIPE * sizeof(unsigned long long), st));
     auto stripes = [&](int c, bool max) {
         std::vector<unsigned long long> h(N_STRIPE);
@@ -802,7 +967,6 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t
         for (unsigned long long x : h) v = max ? std::max<uint64_t>(v, x) : v + x;
         return v;
     };
-    CDBG_HIP(hipMemsetAsync(err, 0, 4, st));
     auto check = [&]() { throw_on(st_d2h<uint32_t>(err, st)); };
 
     // 1-2. keys of the open sides, sorted; equal keys pair up
@@ -865,8 +1029,10 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t
     if (n_lin > nf) throw_on(E_INTERNAL);
     b_clen.release();
 
-    // host: the fragments on cycles
+    // host: the fragments on cycles (numbers, partners, their records)
     std::vector<uint32_t> cyc_frags, cyc_partner;
+    std::vector<uint64_t> cyc_rec;
+    std::vector<uint8_t> cyc_bytes;
     if (n_lin < nf) {
         DevBuf b_list, b_pp;
         uint32_t* list = b_list.ensure<uint32_t>(nf);
@@ -879,6 +1045,25 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t
         cyc_partner.resize(2 * nc);
         CDBG_HIP(hipMemcpyAsync(cyc_frags.data(), list, nc * 4, hipMemcpyDeviceToHost, st));
         CDBG_HIP(hipMemcpyAsync(cyc_partner.data(), pp, 2 * nc * 4, hipMemcpyDeviceToHost, st));
+        // their records, gathered into one block for the host walk
+        DevBuf b_sz, b_do, b_recs;
+        uint64_t* sz = b_sz.ensure<uint64_t>(nc);
+        { KSCOPE("k_st_record_sizes", st); k_st_record_sizes<<<st_blocks(nc), ST_TPB, 0, st>>>(nc, list, d_off, nf, n, sz); }
+        std::vector<uint64_t> hsz(nc);
+        CDBG_HIP(hipMemcpyAsync(hsz.data(), sz, nc * 8, hipMemcpyDeviceToHost, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+        cyc_rec.resize(nc);
+        uint64_t total = 0;
+        for (uint64_t l = 0; l < nc; ++l) {
+            cyc_rec[l] = total;
+            total += hsz[l];
+        }
+        uint64_t* dof = b_do.ensure<uint64_t>(nc);
+        CDBG_HIP(hipMemcpyAsync(dof, cyc_rec.data(), nc * 8, hipMemcpyHostToDevice, st));
+        uint8_t* recs = b_recs.ensure<uint8_t>(total + 16);
+        { KSCOPE("k_st_gather_records", st); k_st_gather_records<<<st_blocks(nc * 64), ST_TPB, 0, st>>>(nc, list, d_off, nf, n, dof, buf, recs); }
+        cyc_bytes.resize(total);
+        CDBG_HIP(hipMemcpyAsync(cyc_bytes.data(), recs, total, hipMemcpyDeviceToHost, st));
         CDBG_HIP(hipStreamSynchronize(st));
     }
     b_partner.release();
@@ -1007,7 +1192,7 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t
     S.joins = stripes(1, false);
     S.unmatched_ends = stripes(2, false);
     if (!cyc_frags.empty()) {
-        stitch_cycles(hbuf, n, off, k, cyc_frags, cyc_partner, U->lists[1], S);
+        stitch_cycles(cyc_bytes.data(), cyc_bytes.size(), cyc_rec, k, cyc_frags, cyc_partner, U->lists[1], S);
         S.num_unitigs += U->lists[1].size();
     }
     return U.release();
@@ -1015,7 +1200,92 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* hbuf, uint64_t n, uint32_t
 
 }  // namespace
 
-Unitigs* stitch_fragments_gpu(int device, const uint8_t* buf, uint64_t n, uint32_t k, uint32_t threads) {
+namespace {
+
+// A file goes to the device through two pinned buffers kept for the process
+// (pinned pages are mapped for the host on first touch, so fresh ones would
+// fault page by page): host threads read a chunk into one while the other's
+// copy runs.
+constexpr uint64_t FILE_STAGE = uint64_t(256) << 20;
+
+struct FileStagePool {
+    std::mutex mu;
+    std::vector<uint8_t*> free;
+    uint8_t* take() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!free.empty()) {
+                uint8_t* p = free.back();
+                free.pop_back();
+                return p;
+            }
+        }
+        void* p = nullptr;
+        CDBG_HIP(hipHostMalloc(&p, FILE_STAGE, hipHostMallocDefault));
+        return (uint8_t*)p;
+    }
+    void give(uint8_t* p) {
+        std::lock_guard<std::mutex> lk(mu);
+        free.push_back(p);
+    }
+};
+FileStagePool g_file_stage;
+
+void upload_file(const char* path, uint64_t n, uint8_t* d, hipStream_t st, uint32_t threads) {
+    struct Stage {
+        uint8_t* b[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        int fd = -1;
+        hipStream_t st = nullptr;
+        ~Stage() {
+            if (st) (void)hipStreamSynchronize(st);  // no copy still reads a buffer given back
+            for (int i = 0; i < 2; ++i) {
+                if (ev[i]) (void)hipEventDestroy(ev[i]);
+                if (b[i]) g_file_stage.give(b[i]);
+            }
+            if (fd >= 0) ::close(fd);
+        }
+    } S;
+    S.st = st;
+    S.fd = ::open(path, O_RDONLY);
+    if (S.fd < 0) throw Error(std::string("cannot open ") + path);
+    for (int i = 0; i < 2; ++i) {
+        S.b[i] = g_file_stage.take();
+        CDBG_HIP(hipEventCreateWithFlags(&S.ev[i], hipEventDisableTiming));
+    }
+    const uint64_t nchunks = (n + FILE_STAGE - 1) / FILE_STAGE;
+    for (uint64_t c = 0; c < nchunks; ++c) {
+        const int b = int(c & 1);
+        if (c >= 2) CDBG_HIP(hipEventSynchronize(S.ev[b]));  // this buffer's previous copy is done
+        const uint64_t at = c * FILE_STAGE, len = std::min<uint64_t>(FILE_STAGE, n - at);
+        const uint32_t T = std::max<uint32_t>(1, std::min<uint64_t>(threads, len >> 22));
+        std::atomic<bool> short_read{false};
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < T; ++t)
+            th.emplace_back([&, t] {
+                uint64_t p = len * t / T;
+                const uint64_t e = len * (t + 1) / T;
+                while (p < e) {
+                    const ssize_t got = ::pread(S.fd, S.b[b] + p, size_t(e - p), off_t(at + p));
+                    if (got <= 0) {
+                        short_read = true;
+                        return;
+                    }
+                    p += uint64_t(got);
+                }
+            });
+        for (auto& x : th) x.join();
+        if (short_read) throw Error(std::string("short read of ") + path);
+        CDBG_HIP(hipMemcpyAsync(d + at, S.b[b], len, hipMemcpyHostToDevice, st));
+        CDBG_HIP(hipEventRecord(S.ev[b], st));
+    }
+    CDBG_HIP(hipStreamSynchronize(st));
+}
+
+// Device, stream and allocation scope of one stitch; `load` puts the stream
+// in the device buffer it is given.
+template <class F>
+Unitigs* run_stitch(int device, uint64_t n, uint32_t k, uint32_t threads, F&& load) {
     if (k < 2 || k > 63) throw Error("k must satisfy 2 <= k <= 63");
     if (threads == 0) threads = 1;
     CDBG_HIP(hipSetDevice(device));
@@ -1023,8 +1293,25 @@ Unitigs* stitch_fragments_gpu(int device, const uint8_t* buf, uint64_t n, uint32
     own.s = acquire_stream(device);
     own.device = device;
     AllocStream on_stream(own.s);
+    DevBuf b_buf;  // released (to the block cache, on own.s) before the stream is given back
+    uint8_t* buf = b_buf.ensure<uint8_t>(n + 16);
+    load(buf, own.s);
     if (k <= 31) return stitch_device<uint64_t>(own.s, buf, n, k, threads);
     return stitch_device<Key128>(own.s, buf, n, k, threads);
+}
+
+}  // namespace
+
+Unitigs* stitch_fragments_gpu(int device, const uint8_t* hbuf, uint64_t n, uint32_t k, uint32_t threads) {
+    return run_stitch(device, n, k, threads, [&](uint8_t* buf, hipStream_t st) {
+        if (n) CDBG_HIP(hipMemcpyAsync(buf, hbuf, n, hipMemcpyHostToDevice, st));
+        CDBG_HIP(hipStreamSynchronize(st));
+    });
+}
+
+Unitigs* stitch_file_gpu(int device, const char* path, uint32_t k, uint32_t threads) {
+    const uint64_t n = file_size_of(path);
+    return run_stitch(device, n, k, threads, [&](uint8_t* buf, hipStream_t st) { upload_file(path, n, buf, st, threads); });
 }
 
 }  // namespace cdbg_amd
@@ -1047,12 +1334,7 @@ CDBG_API void* cdbg_gpu_stitch(cdbg_gpu_ctx* ctx, const uint8_t* frags, uint64_t
 CDBG_API void* cdbg_gpu_stitch_file(cdbg_gpu_ctx* ctx, const char* frag_path, uint32_t k, uint32_t threads) {
     try {
         if (!ctx || !frag_path) throw Error("null argument");
-        // read straight into pinned memory: the upload then runs at link speed
-        const uint64_t n = file_size_of(frag_path);
-        CDBG_HIP(hipSetDevice(ctx->device));
-        PinnedHost buf(n + 16);
-        read_file_into(frag_path, (uint8_t*)buf.p, n, threads);
-        return stitch_fragments_gpu(ctx->device, (const uint8_t*)buf.p, n, k, threads);
+        return stitch_file_gpu(ctx->device, frag_path, k, threads);
     } catch (std::exception& e) {
         set_error(e.what());
         return nullptr;

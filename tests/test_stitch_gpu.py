@@ -161,10 +161,10 @@ def test_gpu_stitch_equals_host_synthetic_pangenome_and_skew(ctx):
 
 @pytest.mark.gpu
 def test_gpu_stitch_long_fragments(ctx):
-    """Two identical genomes: one colour set, unitigs of tens of kbp, longer
-    than a 4096-byte copy piece; one bucket (whole unitigs in one fragment)
-    and four (chains of fragments)."""
-    data, off = G.synth(seed=3, num_colors=2, genome_len=60000, snp_rate=0.0, indel_rate=0.0)
+    """Two identical genomes: one colour set, unitigs of 150 kbp, longer than
+    a 4096-byte copy piece and than a 64 KiB parse chunk; one bucket (whole
+    unitigs in one fragment) and four (chains of fragments)."""
+    data, off = G.synth(seed=3, num_colors=2, genome_len=150000, snp_rate=0.0, indel_rate=0.0)
     for k, m, B in [(31, 15, 1), (31, 15, 4), (63, 25, 4)]:
         f = _oracle_frags(data, off, k, m, B)
         _same(ctx, f, k)
@@ -188,6 +188,23 @@ def test_gpu_stitch_device_build_pangenome(ctx, tmp_path):
     with G.stitch_file(str(p), 31) as H, ctx.stitch_file(str(p), 31, threads=8) as D:
         assert D.stats() == H.stats()
         assert D.export()[0] == H.export()[0]
+
+
+@pytest.mark.gpu
+def test_gpu_stitch_record_starts_it_cannot_guess(ctx):
+    """Closed fragments whose sequences start with N: no record start looks
+    like one (an open-flag byte before a base), so every parse chunk is
+    walked again from where the true chain enters it."""
+    r = np.random.default_rng(11)
+    k, recs = 11, []
+    for i in range(6000):
+        ln = int(r.integers(k, 600))
+        seq = b"N" + bytes(np.frombuffer(b"ACGT", np.uint8)[r.integers(0, 4, ln - 1)])
+        recs.append(_rec(0, [(i % 7, ln - k + 1)], seq))
+    buf = b"".join(recs)
+    assert len(buf) > 20 * 65536
+    st = _same(ctx, buf, k)
+    assert st["num_fragments"] == 6000 and st["num_unitigs"] == 6000 and st["joins"] == 0
 
 
 @pytest.mark.gpu
