@@ -509,6 +509,37 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def full_build_leg(G, job, s, ms_step, threads):
+    """The rest of a complete build of the plain workload after its timed
+    phases 1+2: the device stitch of the job's fragment stream where it lies
+    in HBM (cdbg_gpu_job_stitch: no frag_unitigs.bin), then the host emit of
+    .fa and .u2c into /tmp.  Wall clock; end to end = one step of phases 1+2
+    plus these two."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp(prefix="cdbg_full_", dir="/tmp")
+    try:
+        t0 = time.perf_counter()
+        with job.stitch(threads=threads) as U:
+            st = U.stats()
+            t_stitch = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            U.emit_fasta(os.path.join(d, "out"))
+            t_emit = time.perf_counter() - t0
+        if st["unitig_kmers"] != s["num_kmers"]:
+            raise RuntimeError("the unitigs do not cover the distinct k-mers")
+        total = ms_step / 1e3 + t_stitch + t_emit
+        res = {"stitch_s": round(t_stitch, 3), "emit_s": round(t_emit, 3), "phases12_s": round(ms_step / 1e3, 3),
+               "end_to_end_s": round(total, 3), "end_to_end_kmers_per_s": round(s["num_kmers"] / total, 1),
+               "num_unitigs": st["num_unitigs"], "joins": st["joins"], "cycles": st["cycles"],
+               "num_fragments": st["num_fragments"], "fa_bytes": os.path.getsize(os.path.join(d, "out.fa")),
+               "host_threads": threads}
+        print(f"bench: full build {json.dumps(res)}", file=sys.stderr, flush=True)
+        return res
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def bucket_io_leg(G, ctx, job, s, k, B, colors, threads):
     """Bucket files of the plain workload through the device writer
     (identical records merged, LZ4 frames compressed on the device, files
@@ -582,6 +613,8 @@ def main():
                     help="colours of the large resident build leg (0: skip it)")
     ap.add_argument("--large-only", action="store_true", help="run only the large resident build leg")
     ap.add_argument("--file-only", action="store_true", help="run only the file-path leg")
+    ap.add_argument("--no-full-build", action="store_true",
+                    help="skip the stitch and emit of the plain workload's fragments")
     ap.add_argument("--no-bucket-files", action="store_true", help="skip the bucket-file write/read leg")
     ap.add_argument("--bucket-files-only", action="store_true",
                     help="time the plain workload, then only the bucket-file leg (no traffic runs)")
@@ -664,15 +697,21 @@ def main():
 
     def leg(fn, *a):
         """A secondary leg, started on an empty block cache (what earlier legs
-        cached would otherwise decide its chunking and allocations); its
-        failure is reported in its place, not raised."""
+        cached would otherwise decide its chunking and allocations) and with
+        the page cache's dirty data written back (the gigabytes earlier legs
+        wrote would otherwise throttle this leg's writes); its failure is
+        reported in its place, not raised."""
         G.lib().cdbg_gpu_trim_cache()
+        os.sync()
         try:
             return fn(*a)
         except Exception as e:  # noqa: BLE001
             print(f"bench: {fn.__name__} failed: {e}", file=sys.stderr, flush=True)
             return {"error": str(e)}
 
+    full = None
+    if not args.no_full_build:
+        full = leg(full_build_leg, G, job, s, ms_step, args.cpu_cores)
     bucket_files = None
     if not args.no_bucket_files:
         bucket_files = leg(bucket_io_leg, G, ctx, job, s, args.k, args.buckets, args.colors, args.cpu_cores)
@@ -734,6 +773,7 @@ def main():
         "chunks": chunks,
         "batched": batched,
         "served": served,
+        "full_build": full,
         "file_path": files,
         "bucket_files": bucket_files,
         "large": large,

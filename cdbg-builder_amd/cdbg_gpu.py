@@ -147,6 +147,7 @@ _SIGS = {
     "cdbg_gpu_stitch": (C.c_void_p, [C.c_void_p, U8P, C.c_uint64, C.c_uint32, C.c_uint32]),
     "cdbg_gpu_stitch_file": (C.c_void_p, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_uint32]),
     "cdbg_frag_offsets": (C.c_int, [U8P, C.c_uint64, C.c_uint32, U64P, C.c_uint64, U64P]),
+    "cdbg_gpu_job_stitch": (C.c_void_p, [C.c_void_p, C.c_uint32]),
     "cdbg_unitigs_stats": (C.c_int, [C.c_void_p, C.POINTER(StitchStats)]),
     "cdbg_unitigs_bases": (C.c_uint64, [C.c_void_p]),
     "cdbg_unitigs_export": (C.c_int, [C.c_void_p, C.c_char_p, U64P, U64P]),
@@ -525,10 +526,15 @@ class Job:
         _check(lib().cdbg_gpu_job_write_bucket_files(self.h, os.fsencode(d), threads, _ptr(unc, U64P)))
         return unc
 
+    def stitch(self, threads=8):
+        """Phase 3 on the device over this job's fragment stream where it lies
+        (no frag_unitigs.bin)."""
+        return Unitigs(lib().cdbg_gpu_job_stitch(self.h, threads))
+
     def write_phase2_files(self, frag_path, cs_path):
         st = ColorState()
-        _check(lib().cdbg_gpu_job_write_phase2_files(self.h, os.fsencode(frag_path), os.fsencode(cs_path),
-                                                     C.byref(st)))
+        _check(lib().cdbg_gpu_job_write_phase2_files(self.h, os.fsencode(frag_path) if frag_path else None,
+                                                     os.fsencode(cs_path), C.byref(st)))
         return st.as_dict()
 
     def set_chunk_tuples(self, n):

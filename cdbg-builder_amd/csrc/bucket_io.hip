@@ -1021,7 +1021,7 @@ void write_phase2_files(Job& J, const std::string& frag_path, const std::string&
         const bool ok = (n == 0 || std::fwrite(p, 1, n, f) == n);
         if (std::fclose(f) != 0 || !ok) throw Error("short write to " + path);
     };
-    {
+    if (!frag_path.empty()) {
         bool on_device = false;
         uint64_t n = 0;
         const uint8_t* src = phase2_frags_where(J, on_device, n);

@@ -414,9 +414,9 @@ CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, 
 CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag_path, const char* color_sets_path,
                                              cdbg_gpu_color_state* state) {
     CDBG_TRY({
-        if (!job || !frag_path || !color_sets_path) throw Error("null argument");
+        if (!job || !color_sets_path) throw Error("null argument");
         CDBG_HIP(hipSetDevice(job->J.ctx->device));
-        write_phase2_files(job->J, frag_path, color_sets_path, state);
+        write_phase2_files(job->J, frag_path ? frag_path : "", color_sets_path, state);
     })
 }
 

@@ -186,13 +186,23 @@ int main(int argc, char** argv) {
         const std::string frag = tmp + "/frag_unitigs.bin", cs = c.out + ".color_sets";
         cdbg_gpu_color_state st{};
         cdbg_gpu_stats s{};
+        void* U = nullptr;  // unitigs stitched from the resident job's fragments
         if (c.resident) {
             cdbg_gpu_job* job = nullptr;
             check(cdbg_gpu_job_create_from_files(ctx, fp.data(), nc, c.k, c.m, B, c.t, &job));
             check(cdbg_gpu_job_run(job));
-            check(cdbg_gpu_job_write_phase2_files(job, frag.c_str(), cs.c_str(), &st));
+            // the fragment stream stays in HBM for the stitch unless a file is wanted
+            const bool frag_file = c.keep_tmp || c.phase2_only || c.host_stitch;
+            check(cdbg_gpu_job_write_phase2_files(job, frag_file ? frag.c_str() : nullptr, cs.c_str(), &st));
             check(cdbg_gpu_job_stats(job, &s));
             std::cout << "  phase 1: " << s.ms_phase1 << " ms, phase 2: " << s.ms_phase2 << " ms (device)\n";
+            if (!c.phase2_only && !c.host_stitch) {
+                U = cdbg_gpu_job_stitch(job, c.t);
+                if (!U) {
+                    cdbg_gpu_job_destroy(job);
+                    throw std::runtime_error(cdbg_gpu_last_error());
+                }
+            }
             cdbg_gpu_job_destroy(job);
         } else {
             std::vector<uint64_t> unc(B);
@@ -220,8 +230,9 @@ int main(int argc, char** argv) {
             return 0;
         }
         // phase 3: stitch (on the device unless --host-stitch); phase 4: emit
-        void* U = c.host_stitch ? cdbg_stitch_file(frag.c_str(), c.k, c.t)
-                                : cdbg_gpu_stitch_file(ctx, frag.c_str(), c.k, c.t);
+        if (!U)
+            U = c.host_stitch ? cdbg_stitch_file(frag.c_str(), c.k, c.t)
+                              : cdbg_gpu_stitch_file(ctx, frag.c_str(), c.k, c.t);
         cdbg_gpu_ctx_destroy(ctx);
         if (!U) throw std::runtime_error(cdbg_gpu_last_error());
         cdbg_stitch_stats ss{};
@@ -233,7 +244,10 @@ int main(int argc, char** argv) {
                   << "\n";
         check(cdbg_finalize_color_sets(cs.c_str(), &st));
         check(cdbg_write_metadata(c.out.c_str(), c.k, st.num_kmers, nc, ss.num_unitigs, st.num_color_sets));
-        if (!c.keep_tmp) std::filesystem::remove(frag);
+        if (!c.keep_tmp) {
+            std::error_code ec;
+            std::filesystem::remove(frag, ec);  // absent when the resident stitch used the HBM copy
+        }
         std::cout << "done. wrote " << c.out << ".fa, " << c.out << ".u2c, " << cs << " and " << c.out
                   << ".metadata.txt\n";
         return 0;

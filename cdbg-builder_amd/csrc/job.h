@@ -106,8 +106,12 @@ void copy_phase2_results_device(Job& J, void* frags, void* words, void* offsets,
 // bucket_io.hip
 void write_bucket_files(Job& J, const std::string& dir, uint32_t threads, uint64_t* unc_bytes);
 void load_bucket_files(Job& J, const std::string& dir, uint32_t threads, bool delete_consumed = false);
+// frag_path empty: the colour-set files only
 void write_phase2_files(Job& J, const std::string& frag_path, const std::string& cs_path,
                         cdbg_gpu_color_state* state);
+// stitch_gpu.hip: phase 3 over the job's fragment stream where it lies
+struct Unitigs;
+Unitigs* stitch_job_gpu(Job& J, uint32_t threads);
 
 }  // namespace cdbg_amd
 

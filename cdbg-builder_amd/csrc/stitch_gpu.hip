@@ -1309,6 +1309,20 @@ Unitigs* stitch_fragments_gpu(int device, const uint8_t* hbuf, uint64_t n, uint3
     });
 }
 
+Unitigs* stitch_job_gpu(Job& J, uint32_t threads) {
+    if (threads == 0) threads = 1;
+    bool on_device = false;
+    uint64_t n = 0;
+    const uint8_t* src = phase2_frags_where(J, on_device, n);
+    if (!on_device)  // phase 2 kept the stream in host memory: it goes up like a caller's buffer
+        return stitch_fragments_gpu(J.ctx->device, src, n, J.k, threads);
+    // in HBM: stitched where it lies, on the job's stream
+    CDBG_HIP(hipSetDevice(J.ctx->device));
+    AllocStream on_stream(J.st);
+    if (J.k <= 31) return stitch_device<uint64_t>(J.st, src, n, J.k, threads);
+    return stitch_device<Key128>(J.st, src, n, J.k, threads);
+}
+
 Unitigs* stitch_file_gpu(int device, const char* path, uint32_t k, uint32_t threads) {
     const uint64_t n = file_size_of(path);
     return run_stitch(device, n, k, threads, [&](uint8_t* buf, hipStream_t st) { upload_file(path, n, buf, st, threads); });
@@ -1325,6 +1339,16 @@ CDBG_API void* cdbg_gpu_stitch(cdbg_gpu_ctx* ctx, const uint8_t* frags, uint64_t
     try {
         if (!ctx || (!frags && n_bytes)) throw Error("null argument");
         return stitch_fragments_gpu(ctx->device, frags, n_bytes, k, threads);
+    } catch (std::exception& e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+CDBG_API void* cdbg_gpu_job_stitch(cdbg_gpu_job* job, uint32_t threads) {
+    try {
+        if (!job) throw Error("null argument");
+        return stitch_job_gpu(job->J, threads);
     } catch (std::exception& e) {
         set_error(e.what());
         return nullptr;

@@ -304,6 +304,8 @@ CDBG_API int cdbg_gpu_job_write_bucket_files(cdbg_gpu_job* job, const char* dir,
                                              uint64_t* bucket_unc_bytes);
 CDBG_API int cdbg_gpu_job_from_bucket_files(cdbg_gpu_ctx* ctx, const char* dir, uint32_t num_buckets, uint32_t k,
                                             uint32_t num_colors, uint32_t threads, cdbg_gpu_job** out);
+/* frag_path may be NULL: the colour-set files and state only (the fragment
+ * stream then goes to cdbg_gpu_job_stitch). */
 CDBG_API int cdbg_gpu_job_write_phase2_files(cdbg_gpu_job* job, const char* frag_path, const char* color_sets_path,
                                              cdbg_gpu_color_state* state);
 
@@ -395,6 +397,12 @@ CDBG_API void* cdbg_stitch_file(const char* frag_path, uint32_t k, uint32_t thre
 CDBG_API void* cdbg_gpu_stitch(cdbg_gpu_ctx* ctx, const uint8_t* frags, uint64_t n_bytes, uint32_t k,
                                uint32_t threads);
 CDBG_API void* cdbg_gpu_stitch_file(cdbg_gpu_ctx* ctx, const char* frag_path, uint32_t k, uint32_t threads);
+/* The same stitch over a job's phase-2 fragment stream where it lies (in HBM,
+ * or in host memory when phase 2 kept it there): no frag_unitigs.bin is
+ * written or read.  No counterpart in the reference, whose phases 2 and 3
+ * meet at that file (builder.hpp:239-394); for callers of the resident job
+ * API that want unitigs. */
+CDBG_API void* cdbg_gpu_job_stitch(cdbg_gpu_job* job, uint32_t threads);
 /* Start of every record of a frag_unitigs.bin stream (frag_record_parse,
  * unitig_spill.hpp:646-665, applied record after record), parsed by up to
  * `threads` threads over ranges of at least 64 KiB: *count = the number of

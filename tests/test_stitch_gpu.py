@@ -191,6 +191,29 @@ def test_gpu_stitch_device_build_pangenome(ctx, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("spill", [False, True])
+def test_gpu_job_stitch(ctx, monkeypatch, tmp_path, spill):
+    """The stitch of a resident job's fragment stream where it lies (HBM, or
+    host memory when phase 2 keeps it there) equals the host stitch of the
+    same stream; the job can still write its colour-set files without the
+    fragment file."""
+    if spill:
+        monkeypatch.setenv("CDBG_FRAG_SPILL", "1")
+    data, off = G.synth(seed=23, num_colors=12, genome_len=120_000, snp_rate=0.01, indel_rate=0.001)
+    with ctx.job(data, off, 31, 19, 64) as j:
+        j.run()
+        frags = j.frags()
+        with j.stitch(threads=8) as D:
+            got, gst = D.unitigs(), D.stats()
+        state = j.write_phase2_files(None, str(tmp_path / "o.color_sets"))
+    with G.stitch(frags, 31, threads=8) as H:
+        assert H.stats() == gst
+        assert H.unitigs() == got
+    assert gst["num_fragments"] == state["num_fragments"] > 1000
+    assert (tmp_path / "o.color_sets").exists() and not (tmp_path / "frag_unitigs.bin").exists()
+
+
+@pytest.mark.gpu
 def test_gpu_stitch_record_starts_it_cannot_guess(ctx):
     """Closed fragments whose sequences start with N: no record start looks
     like one (an open-flag byte before a base), so every parse chunk is
