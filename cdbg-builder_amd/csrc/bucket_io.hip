@@ -44,6 +44,7 @@
 // (one u64 bit offset per class, :170-175).  The trailing partial word and
 // the totals are returned for the host's finalize step.
 #include <hip/hip_runtime.h>
+#include <fcntl.h>
 
 #include <algorithm>
 #include <atomic>
@@ -967,6 +968,7 @@ void write_device_stream(const std::string& path, const uint8_t* d, uint64_t n, 
     }
     FILE* f = std::fopen(path.c_str(), "wb");
     if (!f) throw Error("cannot open " + path);
+    if (n) (void)posix_fallocate(fileno(f), 0, off_t(n));  // the extent up front (best effort)
     bool ok = true;
     try {
         const uint64_t nc = (n + STAGE - 1) / STAGE;

@@ -156,6 +156,9 @@ void check(int rc) {
 
 }  // namespace
 
+// the device stitch numbers fragment sides with 32 bits
+static bool too_many(uint64_t fragments) { return fragments >= (uint64_t(1) << 30); }
+
 int main(int argc, char** argv) {
     Config c;
     try {
@@ -191,12 +194,12 @@ int main(int argc, char** argv) {
             cdbg_gpu_job* job = nullptr;
             check(cdbg_gpu_job_create_from_files(ctx, fp.data(), nc, c.k, c.m, B, c.t, &job));
             check(cdbg_gpu_job_run(job));
-            // the fragment stream stays in HBM for the stitch unless a file is wanted
-            const bool frag_file = c.keep_tmp || c.phase2_only || c.host_stitch;
-            check(cdbg_gpu_job_write_phase2_files(job, frag_file ? frag.c_str() : nullptr, cs.c_str(), &st));
             check(cdbg_gpu_job_stats(job, &s));
+            // the fragment stream stays in HBM for the stitch unless a file is wanted
+            const bool frag_file = c.keep_tmp || c.phase2_only || c.host_stitch || too_many(s.num_frags);
+            check(cdbg_gpu_job_write_phase2_files(job, frag_file ? frag.c_str() : nullptr, cs.c_str(), &st));
             std::cout << "  phase 1: " << s.ms_phase1 << " ms, phase 2: " << s.ms_phase2 << " ms (device)\n";
-            if (!c.phase2_only && !c.host_stitch) {
+            if (!c.phase2_only && !c.host_stitch && !too_many(s.num_frags)) {
                 U = cdbg_gpu_job_stitch(job, c.t);
                 if (!U) {
                     cdbg_gpu_job_destroy(job);
@@ -230,9 +233,12 @@ int main(int argc, char** argv) {
             return 0;
         }
         // phase 3: stitch (on the device unless --host-stitch); phase 4: emit
+        const bool host = c.host_stitch || too_many(st.num_fragments);
+        if (host && !c.host_stitch)
+            std::cout << "  stitch: " << st.num_fragments
+                      << " fragments, more than the device stitch indexes (2^30): stitching on the host\n";
         if (!U)
-            U = c.host_stitch ? cdbg_stitch_file(frag.c_str(), c.k, c.t)
-                              : cdbg_gpu_stitch_file(ctx, frag.c_str(), c.k, c.t);
+            U = host ? cdbg_stitch_file(frag.c_str(), c.k, c.t) : cdbg_gpu_stitch_file(ctx, frag.c_str(), c.k, c.t);
         cdbg_gpu_ctx_destroy(ctx);
         if (!U) throw std::runtime_error(cdbg_gpu_last_error());
         cdbg_stitch_stats ss{};
