@@ -150,8 +150,51 @@ template <>
 __device__ __forceinline__ Key128 push_base<Key128>(const Key128& x, uint32_t c) {
     return Key128{(x.lo << 2) | c, (x.hi << 2) | (x.lo >> 62)};
 }
+// four bases (8 bits of codes) at once
+template <class K>
+__device__ __forceinline__ K push_4(const K& x, uint32_t c8);
+template <>
+__device__ __forceinline__ uint64_t push_4<uint64_t>(const uint64_t& x, uint32_t c8) {
+    return (x << 8) | c8;
+}
+template <>
+__device__ __forceinline__ Key128 push_4<Key128>(const Key128& x, uint32_t c8) {
+    return Key128{(x.lo << 8) | c8, (x.hi << 8) | (x.lo >> 56)};
+}
+
+// 2-bit codes of the four ASCII bytes of w (first byte lowest), first base
+// in the highest bits; *bad |= 1 when a byte is not A, C, G or T.
+__device__ __forceinline__ uint32_t codes4(uint32_t w, uint32_t& bad) {
+    uint32_t c8 = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const uint32_t b = (w >> (8 * i)) & 0xFFu;
+        bad |= uint32_t(b != 'A' && b != 'C' && b != 'G' && b != 'T');
+        c8 = (c8 << 2) | (((b >> 1) ^ (b >> 2)) & 3u);  // A 0, C 1, G 2, T 3
+    }
+    return c8;
+}
+
 __device__ __forceinline__ bool key_le(uint64_t a, uint64_t b) { return a <= b; }
 __device__ __forceinline__ bool key_le(const Key128& a, const Key128& b) { return !(b < a); }
+
+// The canonical k-mer of s[0, k) and whether the forward word is it: the
+// bytes are read once, four per load where whole words lie inside the
+// k-mer; the reverse complement comes from the forward word.
+template <class K>
+__device__ __forceinline__ K canonical_at(const uint8_t* __restrict__ s, uint32_t k, bool& fwd, uint32_t& bad) {
+    K f{};
+    uint32_t j = 0;
+    for (; j + 4 <= k; j += 4) f = push_4<K>(f, codes4(ld32(s + j), bad));
+    for (; j < k; ++j) {
+        const uint32_t b = s[j];
+        bad |= uint32_t(b != 'A' && b != 'C' && b != 'G' && b != 'T');
+        f = push_base<K>(f, ((b >> 1) ^ (b >> 2)) & 3u);
+    }
+    const K r = KOps<K>::rc(f, k);
+    fwd = key_le(f, r);
+    return fwd ? f : r;
+}
 
 // 1. Open sides per block of fragments (their ends are laid out block by
 // block, in fragment order).
@@ -171,7 +214,8 @@ __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t*
                                                      const uint64_t* __restrict__ off, uint32_t k,
                                                      const uint64_t* __restrict__ block_base,
                                                      uint32_t* __restrict__ f_len, uint32_t* __restrict__ f_nr,
-                                                     uint64_t* __restrict__ f_cid, K* __restrict__ ekey,
+                                                     uint64_t* __restrict__ f_cid, uint2* __restrict__ f_pos,
+                                                     K* __restrict__ ekey,
                                                      uint32_t* __restrict__ eid, uint32_t* __restrict__ err) {
     __shared__ uint64_t s_tmp[ST_TPB / 64 + 1];
     const uint64_t i = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
@@ -193,6 +237,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t*
         } else {
             // colour ids: first and last run, first and last run with k-mers
             uint64_t sum = 0, c0 = 0, cl = 0, p0 = COLOR_FOREIGN, pl = COLOR_FOREIGN;
+            uint32_t i0 = 0, il = 0;  // runs of the first and last colour with k-mers
             bool any = false;
             for (uint32_t r = 0; r < nr; ++r) {
                 const uint64_t cid = ld64(rec + 5 + 12 * uint64_t(r));
@@ -200,8 +245,12 @@ __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t*
                 if (r == 0) c0 = cid;
                 cl = cid;
                 if (n) {
-                    if (!any) p0 = cid;
+                    if (!any) {
+                        p0 = cid;
+                        i0 = r;
+                    }
                     pl = cid;
+                    il = r;
                     any = true;
                 }
                 sum += n;
@@ -211,6 +260,7 @@ __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t*
             f_cid[4 * i + 1] = cl;
             f_cid[4 * i + 2] = p0;
             f_cid[4 * i + 3] = pl;
+            f_pos[i] = make_uint2(i0, il);
         }
         if (bad) atomicOr(err, bad);
         ok = !bad;
@@ -224,18 +274,9 @@ __global__ __launch_bounds__(ST_TPB) void k_st_frags(uint64_t nf, const uint8_t*
         K key{};
         bool fwd = false;
         if (ok) {
-            const uint8_t* s = rec + 9 + 12 * uint64_t(nr) + (side ? len - k : 0);
-            K f{}, r{};
-            int bad = 0;
-            for (uint32_t j = 0; j < k; ++j) {
-                const int c = code_of(s[j]), d = code_of(s[k - 1 - j]);
-                bad |= c | d;
-                f = push_base<K>(f, uint32_t(c & 3));
-                r = push_base<K>(r, uint32_t(3 - (d & 3)));
-            }
-            if (bad < 0) atomicOr(err, E_BASE);
-            fwd = key_le(f, r);
-            key = fwd ? f : r;
+            uint32_t bad = 0;
+            key = canonical_at<K>(rec + 9 + 12 * uint64_t(nr) + (side ? len - k : 0), k, fwd, bad);
+            if (bad) atomicOr(err, E_BASE);
         }
         ekey[at] = key;
         eid[at] = uint32_t(i << 2) | (side << 1) | uint32_t(fwd);
@@ -395,63 +436,91 @@ __global__ __launch_bounds__(ST_TPB) void k_st_slot_sizes(uint64_t n_lin, const 
 }
 
 // Run entries of a slot in walk order: the fragment's runs (reversed when it
-// is entered at its right side), less its first k-mer when joined, and its
-// last k-mer recoloured by the next join when that differs; the unused
-// entries carry no k-mers.
-__global__ __launch_bounds__(ST_TPB) void k_st_runs(uint64_t n_lin, const uint32_t* __restrict__ slot_side,
+// is entered at its right side), less its first k-mer when joined (taken
+// from its first run with k-mers), and its last k-mer recoloured by the next
+// join when that differs (one k-mer less in its last run with k-mers, and the
+// slot's extra entry holds the new colour); the extra entry is empty
+// otherwise.  One thread per entry, its slot found among the block's slots
+// in LDS (every slot has at least one entry), so consecutive entries are
+// written by consecutive lanes.
+__global__ __launch_bounds__(ST_TPB) void k_st_runs(uint64_t n_ent, uint64_t n_lin, const uint64_t* __restrict__ r_off,
+                                                    const uint32_t* __restrict__ slot_side,
                                                     const uint32_t* __restrict__ ch_pos, const uint64_t* __restrict__ S,
                                                     const uint8_t* __restrict__ buf, const uint64_t* __restrict__ off,
                                                     const uint32_t* __restrict__ f_nr, const uint64_t* __restrict__ f_cid,
-                                                    const uint64_t* __restrict__ r_off, const uint32_t* __restrict__ s_tig,
-                                                    uint64_t n_ent, uint64_t* __restrict__ e_cid,
-                                                    uint32_t* __restrict__ e_n, uint32_t* __restrict__ e_tig,
-                                                    uint32_t* __restrict__ err) {
-    const uint64_t j = uint64_t(blockIdx.x) * ST_TPB + threadIdx.x;
-    if (j >= n_lin) return;
+                                                    const uint2* __restrict__ f_pos, const uint32_t* __restrict__ s_tig,
+                                                    uint64_t* __restrict__ e_cid, uint32_t* __restrict__ e_n,
+                                                    uint32_t* __restrict__ e_tig, uint32_t* __restrict__ err) {
+    __shared__ uint64_t s_off[ST_TPB + 2];
+    __shared__ uint64_t s_j0;
+    const uint64_t t0 = uint64_t(blockIdx.x) * ST_TPB;
+    if (threadIdx.x == 0) {  // the slot holding entry t0: the last with r_off <= t0
+        uint64_t lo = 0, hi = n_lin;
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) >> 1;
+            if (r_off[mid] <= t0)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        s_j0 = lo;
+    }
+    __syncthreads();
+    const uint64_t j0 = s_j0;
+    for (uint32_t q = threadIdx.x; q < ST_TPB + 2; q += ST_TPB) s_off[q] = j0 + q < n_lin ? r_off[j0 + q] : n_ent;
+    __syncthreads();
+    const uint64_t t = t0 + threadIdx.x;
+    if (t >= n_ent) return;
+    uint32_t lo = 0, hi = ST_TPB + 1;  // s_off[lo] <= t < s_off[hi]
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= t)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint64_t j = j0 + lo;
+    if (j >= n_lin || s_off[lo] > t) {
+        atomicOr(err, E_INTERNAL);
+        return;
+    }
     const uint32_t s = slot_side[j];
     const uint32_t i = s >> 1;
     const bool rev = s & 1u, joined = ch_pos[i] > 0, last = st_y(S[s]) == 0;
     const uint32_t nr = f_nr[i];
-    const uint8_t* runs = buf + off[i] + 5;
-    const uint64_t base = r_off[j];
-    if (base + nr >= n_ent) {  // the slot's nr + 1 entries
+    const uint64_t x = t - s_off[lo];  // entry of the slot, walk order; nr is the extra one
+    if (x > nr) {
         atomicOr(err, E_INTERNAL);
         return;
     }
-    // s_tig: tig starts before slot j (exclusive count); a joined slot's own
-    // tig started before it
     const uint32_t tig = joined ? s_tig[j] - 1u : s_tig[j];
-    uint32_t skip = joined ? 1u : 0u;
-    int64_t last_pos = -1;  // the last entry with k-mers
-    for (uint32_t x = 0; x < nr; ++x) {
-        const uint32_t r = rev ? nr - 1 - x : x;
-        const uint64_t cid = ld64(runs + 12 * uint64_t(r));
-        const uint32_t n = ld32(runs + 12 * uint64_t(r) + 8);
-        const uint32_t take = n > skip ? n - skip : 0u;
-        skip = n > skip ? 0u : skip - n;
-        e_cid[base + x] = cid;
-        e_n[base + x] = take;
-        e_tig[base + x] = tig;
-        if (take) last_pos = int64_t(x);
-    }
-    uint64_t extra_cid = 0;
-    uint32_t extra_n = 0;
-    if (!last && last_pos >= 0) {
-        // the shared k-mer with the next fragment: this side's colour unless
-        // foreign, else the next fragment's first (walk order) run's
-        const uint32_t t = slot_side[j + 1];
+    // the shared k-mer with the next fragment: this side's colour unless
+    // foreign, else the next fragment's first (walk order) run's
+    bool recolour = false;
+    uint64_t real = 0;
+    if (!last) {
+        const uint32_t nx = slot_side[j + 1];
         const uint64_t mine = f_cid[4 * uint64_t(i) + (rev ? 2 : 3)];
-        const uint64_t theirs = f_cid[4 * uint64_t(t >> 1) + ((t & 1u) ? 1 : 0)];
-        const uint64_t real = mine != COLOR_FOREIGN ? mine : theirs;
-        if (real != mine) {
-            e_n[base + uint64_t(last_pos)] -= 1;
-            extra_cid = real;
-            extra_n = 1;
-        }
+        const uint64_t theirs = f_cid[4 * uint64_t(nx >> 1) + ((nx & 1u) ? 1 : 0)];
+        real = mine != COLOR_FOREIGN ? mine : theirs;
+        recolour = real != mine;
     }
-    e_cid[base + nr] = extra_cid;
-    e_n[base + nr] = extra_n;
-    e_tig[base + nr] = tig;
+    if (x == nr) {
+        e_cid[t] = recolour ? real : 0;
+        e_n[t] = recolour ? 1u : 0u;
+        e_tig[t] = tig;
+        return;
+    }
+    const uint2 pos = f_pos[i];
+    const uint32_t first_pos = rev ? pos.y : pos.x, last_pos = rev ? pos.x : pos.y;  // stored run indices
+    const uint32_t r = rev ? nr - 1 - uint32_t(x) : uint32_t(x);
+    const uint8_t* run = buf + off[i] + 5 + 12 * uint64_t(r);
+    uint32_t take = ld32(run + 8);
+    if (joined && r == first_pos) --take;   // the shared first k-mer (the run has k-mers)
+    if (recolour && r == last_pos) --take;  // the recoloured last k-mer
+    e_cid[t] = ld64(run);
+    e_n[t] = take;
+    e_tig[t] = tig;
 }
 
 // 6. Unitig heads among the entries with k-mers (compacted, in tig order):
@@ -973,6 +1042,8 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* buf, uint64_t n, uint32_t 
     uint32_t* f_len = b_len.ensure<uint32_t>(nf);
     uint32_t* f_nr = b_nr.ensure<uint32_t>(nf);
     uint64_t* f_cid = b_cid.ensure<uint64_t>(4 * nf);
+    DevBuf b_fpos;
+    uint2* f_pos = b_fpos.ensure<uint2>(nf);
     K* ekey = b_ekey.ensure<K>(ns);
     uint32_t* eid = b_eid.ensure<uint32_t>(ns);
     const uint64_t nbf = st_blocks(nf);
@@ -982,7 +1053,7 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* buf, uint64_t n, uint32_t 
     uint64_t* d_ne = b_bt.ensure<uint64_t>(1);
     { KSCOPE("k_st_count_ends", st); k_st_count_ends<<<uint32_t(nbf), ST_TPB, 0, st>>>(nf, buf, d_off, block_cnt); }
     exclusive_scan<uint64_t, uint64_t>(block_cnt, block_base, nbf, d_ne, b_tmp, st);
-    { KSCOPE("k_st_frags", st); k_st_frags<K><<<uint32_t(nbf), ST_TPB, 0, st>>>(nf, buf, d_off, k, block_base, f_len, f_nr, f_cid, ekey, eid, err); }
+    { KSCOPE("k_st_frags", st); k_st_frags<K><<<uint32_t(nbf), ST_TPB, 0, st>>>(nf, buf, d_off, k, block_base, f_len, f_nr, f_cid, f_pos, ekey, eid, err); }
     check();
     const uint64_t ne = st_d2h<uint64_t>(d_ne, st);
     if (ne > ns) throw_on(E_INTERNAL);
@@ -1101,7 +1172,7 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* buf, uint64_t n, uint32_t 
         uint64_t* e_cid = b_ec.ensure<uint64_t>(n_ent);
         uint32_t* e_n = b_en.ensure<uint32_t>(n_ent);
         uint32_t* e_tig = b_et.ensure<uint32_t>(n_ent);
-        { KSCOPE("k_st_runs", st); k_st_runs<<<st_blocks(n_lin), ST_TPB, 0, st>>>(n_lin, slot_side, ch_pos, A, buf, d_off, f_nr, f_cid, r_off, s_tig, n_ent, e_cid, e_n, e_tig, err); }
+        { KSCOPE("k_st_runs", st); k_st_runs<<<st_blocks(n_ent), ST_TPB, 0, st>>>(n_ent, n_lin, r_off, slot_side, ch_pos, A, buf, d_off, f_nr, f_cid, f_pos, s_tig, e_cid, e_n, e_tig, err); }
         check();
         uint32_t* list = b_list.ensure<uint32_t>(n_ent);
         compact_flags<uint32_t>(e_n, n_ent, list, d_tot32, b_cnt, b_tmp, st);
