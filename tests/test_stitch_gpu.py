@@ -120,9 +120,11 @@ def _oracle_frags(data, off, k, m, B, cuts=None):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k,m,B,seed", [(15, 7, 4, 1), (21, 11, 8, 3), (31, 19, 64, 4), (32, 19, 16, 5),
-                                        (33, 21, 8, 6), (63, 27, 16, 7)])
+@pytest.mark.parametrize("k,m,B,seed", [(5, 2, 2, 9), (7, 3, 4, 8), (15, 7, 4, 1), (21, 11, 8, 3), (31, 19, 64, 4),
+                                        (32, 19, 16, 5), (33, 21, 8, 6), (63, 27, 16, 7)])
 def test_gpu_stitch_equals_host_mutated_genomes(ctx, k, m, B, seed):
+    """k = 5 and 7 make dense graphs (every canonical 5-mer occurs); k = 31 is
+    the widest 64-bit key, 32 the narrowest 128-bit one."""
     base = bytes(np.frombuffer(b"ACGT", np.uint8)[np.random.default_rng(seed).integers(0, 4, 3000)])
     data, off = _job([[g] for g in _mutated(base, 4, 0.02, 0.003, seed + 100)])
     st = _same(ctx, _oracle_frags(data, off, k, m, B), k)
