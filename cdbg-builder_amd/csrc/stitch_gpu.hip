@@ -94,8 +94,6 @@ __device__ __forceinline__ uint64_t ld64(const uint8_t* p) {
     return v;
 }
 
-__device__ __forceinline__ uint64_t st_lanes_below(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
-
 // A counter is N_STRIPE words: a block adds its total to the word of its
 // own stripe, so no address takes more than a few atomics at a time (one
 // word taking an atomic from every wave serialises the launch); the host
@@ -133,9 +131,6 @@ __device__ __forceinline__ void stripe_add(unsigned long long* c, uint64_t v) {
     if (threadIdx.x == 0 && t) atomicAdd(&c[blockIdx.x % N_STRIPE], (unsigned long long)t);
 }
 
-__device__ __forceinline__ int code_of(uint8_t c) {
-    return c == 'A' ? 0 : c == 'C' ? 1 : c == 'G' ? 2 : c == 'T' ? 3 : -1;
-}
 __device__ __forceinline__ uint8_t comp_of(uint8_t c) {
     return c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
 }
