@@ -955,13 +955,16 @@ void write_device_stream(const std::string& path, const uint8_t* d, uint64_t n, 
     struct Bufs {
         uint8_t* b[2];
         hipEvent_t ev[2] = {nullptr, nullptr};
+        hipStream_t st = nullptr;
         ~Bufs() {
+            if (st) (void)hipStreamSynchronize(st);  // no copy still writes a buffer given back (error paths)
             for (int i = 0; i < 2; ++i) {
                 if (ev[i]) (void)hipEventDestroy(ev[i]);
                 if (b[i]) g_stage.give(b[i]);
             }
         }
     } B{{nullptr, nullptr}};
+    B.st = st;
     for (int i = 0; i < 2; ++i) {
         B.b[i] = g_stage.take();
         CDBG_HIP(hipEventCreateWithFlags(&B.ev[i], hipEventDisableTiming));
