@@ -1250,6 +1250,7 @@ Unitigs* stitch_device(hipStream_t st, const uint8_t* buf, uint64_t n, uint32_t 
             L.ext_bases = (const char*)hp + nu * 24;
             CDBG_HIP(hipMemcpyAsync(hp, items, nu * 24, hipMemcpyDeviceToHost, st));
             CDBG_HIP(hipMemcpyAsync((uint8_t*)hp + nu * 24, out, total_out, hipMemcpyDeviceToHost, st));
+            CDBG_HIP(hipStreamSynchronize(st));  // the block is complete before anything can free it
         }
         S.foreign_dropped = stripes(3, false);
         S.unitig_kmers = stripes(4, false);
