@@ -458,7 +458,7 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
     .color_sets), then the device stitch (cdbg_gpu_stitch_file) and the host
     emit (.fa, .u2c, finished .color_sets, metadata).  The host stitch of the
     same file is timed beside it (stitch_host_s) and must give the same
-    statistics."""
+    statistics and, emitted, byte-identical .fa and .u2c."""
     import shutil
     import tempfile
     d = tempfile.mkdtemp(prefix="cdbg_bench_", dir="/tmp")
@@ -493,6 +493,11 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
             t_host = time.perf_counter() - t0
             if H.stats() != sst:
                 raise RuntimeError("device and host stitch disagree")
+            H.emit_fasta(out + ".host")
+        import filecmp
+        same_files = all(filecmp.cmp(out + ext, out + ".host" + ext, shallow=False) for ext in (".fa", ".u2c"))
+        if not same_files:
+            raise RuntimeError("device and host stitch wrote different .fa / .u2c")
         p12 = t["bucket_write_s"] + t["bucket_process_s"]
         res = {"metric": "distinct k-mers/s (phases 1+2, files)", "value": round(st["num_kmers"] / p12, 1),
                "end_to_end_kmers_per_s": round(st["num_kmers"] / sum(t.values()), 1),
@@ -500,7 +505,7 @@ def file_path_leg(G, ctx, colors, genome, k, m, B, threads):
                "bucket_bytes_uncompressed": int(unc.sum()), "frag_bytes": os.path.getsize(frag),
                "fa_bytes": os.path.getsize(out + ".fa"), "host_threads": threads,
                **{kk: round(v, 3) for kk, v in t.items()},
-               "stitch_host_s": round(t_host, 3),
+               "stitch_host_s": round(t_host, 3), "fa_u2c_identical_to_host_stitch": same_files,
                "workload": f"first {colors} colours of the pangenome workload as gzipped FASTA (generated "
                            f"in {t_gen:.1f} s, untimed); k={k} m={m} B={B}; scratch under /tmp"}
         print(f"bench: file path {json.dumps(res)}", file=sys.stderr, flush=True)
