@@ -190,6 +190,7 @@ int main(int argc, char** argv) {
         cdbg_gpu_color_state st{};
         cdbg_gpu_stats s{};
         void* U = nullptr;  // unitigs stitched from the resident job's fragments
+        bool device_failed = false;
         if (c.resident) {
             cdbg_gpu_job* job = nullptr;
             check(cdbg_gpu_job_create_from_files(ctx, fp.data(), nc, c.k, c.m, B, c.t, &job));
@@ -201,9 +202,12 @@ int main(int argc, char** argv) {
             std::cout << "  phase 1: " << s.ms_phase1 << " ms, phase 2: " << s.ms_phase2 << " ms (device)\n";
             if (!c.phase2_only && !c.host_stitch && !too_many(s.num_frags)) {
                 U = cdbg_gpu_job_stitch(job, c.t);
-                if (!U) {
-                    cdbg_gpu_job_destroy(job);
-                    throw std::runtime_error(cdbg_gpu_last_error());
+                if (!U) {  // the host stitch needs the stream as a file
+                    std::cerr << "warning: device stitch failed (" << cdbg_gpu_last_error()
+                              << "); stitching on the host\n";
+                    device_failed = true;
+                    if (!frag_file)
+                        check(cdbg_gpu_job_write_phase2_files(job, frag.c_str(), cs.c_str(), &st));
                 }
             }
             cdbg_gpu_job_destroy(job);
@@ -233,12 +237,19 @@ int main(int argc, char** argv) {
             return 0;
         }
         // phase 3: stitch (on the device unless --host-stitch); phase 4: emit
-        const bool host = c.host_stitch || too_many(st.num_fragments);
+        const bool host = c.host_stitch || too_many(st.num_fragments) || device_failed;
         if (host && !c.host_stitch)
             std::cout << "  stitch: " << st.num_fragments
                       << " fragments, more than the device stitch indexes (2^30): stitching on the host\n";
-        if (!U)
-            U = host ? cdbg_stitch_file(frag.c_str(), c.k, c.t) : cdbg_gpu_stitch_file(ctx, frag.c_str(), c.k, c.t);
+        if (!U && !host) {
+            U = cdbg_gpu_stitch_file(ctx, frag.c_str(), c.k, c.t);
+            if (!U) {  // e.g. a stream whose working set does not fit the HBM left
+                std::cerr << "warning: device stitch failed (" << cdbg_gpu_last_error()
+                          << "); stitching on the host\n";
+                cdbg_gpu_trim_cache();
+            }
+        }
+        if (!U) U = cdbg_stitch_file(frag.c_str(), c.k, c.t);
         cdbg_gpu_ctx_destroy(ctx);
         if (!U) throw std::runtime_error(cdbg_gpu_last_error());
         cdbg_stitch_stats ss{};

@@ -233,6 +233,24 @@ def test_gpu_stitch_record_starts_it_cannot_guess(ctx):
 
 
 @pytest.mark.gpu
+def test_gpu_stitch_out_of_memory_is_an_error_and_leaves_nothing_behind(ctx):
+    """A device allocation that fails inside the stitch (injected) is reported
+    as an error; the next stitch on the same context gives the full result."""
+    data, off = G.synth(seed=9, num_colors=5, genome_len=4000, snp_rate=0.01, indel_rate=0.002)
+    f = _oracle_frags(data, off, 21, 9, 16)
+    for n in (0, 3, 12):
+        G.lib().cdbg_gpu_trim_cache()
+        G.lib().cdbg_gpu_inject_oom(n)
+        try:
+            with pytest.raises(RuntimeError, match="out of memory"):
+                with ctx.stitch(f, 21):
+                    pass
+        finally:
+            G.lib().cdbg_gpu_inject_oom(-1)
+    _same(ctx, f, 21)
+
+
+@pytest.mark.gpu
 def test_gpu_stitch_empty_stream(ctx):
     with ctx.stitch(b"", 31) as D:
         st = D.stats()
