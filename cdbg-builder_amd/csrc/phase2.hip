@@ -298,13 +298,93 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
 // with more than `lo` items; cells above CAP go to the next class's list.
 // ---------------------------------------------------------------------------
 // PACK (64-bit keys whose unsorted bits and a 13-bit position fit one word):
-// each item is the word (key bits below end_bit) << 13 | position, so a pass
-// moves 8 bytes per item; the bits above end_bit are the cell's own.
+// each item is the word (key bits below end_bit) << 13 | position, distinct
+// within the cell, so sorting the words is the stable sort of the keys; they
+// are merge-sorted in LDS instead (the bits above end_bit are the cell's own).
 constexpr int PACK_IB = 13;
 template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 inline size_t cell_sort_lds() {
-    return size_t(CAP) * (PACK ? sizeof(uint64_t) : sizeof(K) + sizeof(uint16_t)) +
-           ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
+    if (PACK) return size_t(CAP) * sizeof(uint64_t);  // the words only (merge sort)
+    return size_t(CAP) * (sizeof(K) + sizeof(uint16_t)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
+}
+
+// Sorts the distinct words s[0, count) in LDS ascending (count <= CAP):
+// each thread sorts CAP / THREADS consecutive words in registers by a
+// bitonic network, then runs are merged pairwise, each thread producing
+// CAP / THREADS outputs of a merge after finding where its outputs start
+// by a binary search along the merge path.  The tail up to a multiple of
+// CAP / THREADS is padded with ~0 (equal words are only ~0s, and which copy
+// of a value goes first does not change the values).  Ends with a barrier.
+template <uint32_t CAP, int THREADS>
+__device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
+    constexpr uint32_t IPT = CAP / THREADS;
+    static_assert(IPT >= 2 && (IPT & (IPT - 1)) == 0 && IPT * THREADS == CAP, "items per thread");
+    const uint32_t n_pad = (count + IPT - 1) / IPT * IPT;
+    for (uint32_t i = count + threadIdx.x; i < n_pad; i += THREADS) s[i] = ~0ull;
+    __syncthreads();
+    const uint32_t out0 = threadIdx.x * IPT;
+    const bool act = out0 < n_pad;
+    uint64_t r[IPT];
+    if (act) {
+#pragma unroll
+        for (uint32_t j = 0; j < IPT; ++j) r[j] = s[out0 + j];
+#pragma unroll
+        for (uint32_t k = 2; k <= IPT; k <<= 1)
+#pragma unroll
+            for (uint32_t j = k >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (uint32_t i = 0; i < IPT; ++i) {
+                    const uint32_t l = i ^ j;
+                    if (l > i) {
+                        const uint64_t x = r[i], y = r[l];
+                        const bool sw = ((i & k) == 0) ? (x > y) : (x < y);
+                        r[i] = sw ? y : x;
+                        r[l] = sw ? x : y;
+                    }
+                }
+#pragma unroll
+        for (uint32_t j = 0; j < IPT; ++j) s[out0 + j] = r[j];  // this thread's own words only
+    }
+    __syncthreads();
+    for (uint32_t L = IPT; L < n_pad; L <<= 1) {
+        if (act) {
+            // run A = s[p0, p0 + la), run B = s[p0 + L, p0 + L + lb); this
+            // thread writes outputs d .. d + IPT - 1 of their merge
+            const uint32_t p0 = out0 & ~(2 * L - 1), d = out0 - p0;
+            const uint32_t la = min(L, n_pad - p0);
+            const uint32_t lb = n_pad - p0 > L ? min(L, n_pad - p0 - L) : 0u;
+            const uint64_t* A = s + p0;
+            const uint64_t* Bw = s + p0 + L;
+            uint32_t lo = d > lb ? d - lb : 0u, hi = min(d, la);
+            while (lo < hi) {  // the number of A's words among the first d outputs
+                const uint32_t mid = (lo + hi) >> 1;
+                if (A[mid] < Bw[d - 1 - mid])
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            uint32_t ia = lo, ib = d - lo;
+            uint64_t a = ia < la ? A[ia] : ~0ull, b = ib < lb ? Bw[ib] : ~0ull;
+#pragma unroll
+            for (uint32_t j = 0; j < IPT; ++j) {
+                const bool ta = ib >= lb || (ia < la && a <= b);
+                r[j] = ta ? a : b;
+                ia += ta ? 1u : 0u;
+                ib += ta ? 0u : 1u;
+                const bool ok = ta ? ia < la : ib < lb;
+                const uint64_t* p = ta ? A + ia : Bw + ib;
+                const uint64_t v = ok ? *p : ~0ull;
+                a = ta ? v : a;
+                b = ta ? b : v;
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (uint32_t j = 0; j < IPT; ++j) s[out0 + j] = r[j];
+        }
+        __syncthreads();
+    }
 }
 
 struct CellSortArgs {
@@ -354,88 +434,97 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     uint32_t dr[RMAX];
     const uint64_t low = PACK ? (uint64_t(1) << end_bit) - 1 : 0;  // end_bit < 64 when packed
     uint64_t hi = 0;  // packed: the cell's bits above end_bit
+    if constexpr (PACK) {
+        // distinct words (the position breaks ties): merge-sorted in LDS,
+        // then read in the layout the epilogue uses
+        hi = keys[start] & ~low;
+        for (uint32_t i = threadIdx.x; i < count; i += THREADS) s_key[i] = ((keys[start + i] & low) << PACK_IB) | i;
+        if (end_bit > 0)
+            merge_sort_words<CAP, THREADS>(s_key, count);
+        else
+            __syncthreads();
 #pragma unroll
-    for (int r = 0; r < RMAX; ++r) {
-        const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
-        if (uint32_t(r) < rounds && i < count) {
-            if constexpr (PACK) {
-                const uint64_t kk = keys[start + i];
-                hi = kk & ~low;
-                key[r] = ((kk & low) << PACK_IB) | i;
-            } else {
+        for (int r = 0; r < RMAX; ++r) {
+            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+            if (uint32_t(r) < rounds && i < count) key[r] = s_key[i];
+        }
+    } else {
+#pragma unroll
+        for (int r = 0; r < RMAX; ++r) {
+            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+            if (uint32_t(r) < rounds && i < count) {
                 key[r] = keys[start + i];
                 idx[r] = i;
             }
         }
-    }
-    const int sh0 = PACK ? PACK_IB : 0;  // digits of the packed word start above the position
-    uint32_t* my = wcnt + wave * 256;
-    for (int sh = 0; sh < end_bit; sh += 8) {
-        const int bits = (end_bit - sh) < 8 ? (end_bit - sh) : 8;
-        const uint32_t mask = (1u << bits) - 1;
-        for (uint32_t b = threadIdx.x; b < WAVES * 256; b += THREADS) wcnt[b] = 0;
-        __syncthreads();
+        uint32_t* my = wcnt + wave * 256;
+        for (int sh = 0; sh < end_bit; sh += 8) {
+            const int bits = (end_bit - sh) < 8 ? (end_bit - sh) : 8;
+            const uint32_t mask = (1u << bits) - 1;
+            for (uint32_t b = threadIdx.x; b < WAVES * 256; b += THREADS) wcnt[b] = 0;
+            __syncthreads();
 #pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-            if (uint32_t(r) >= rounds) break;
-            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
-            const bool active = i < count;
-            const uint32_t dig = active ? digit_of(key[r], sh + sh0, mask) : 0u;
-            uint64_t peers = __ballot(active);
-            for (int b = 0; b < bits; ++b) {
-                const bool bit = (dig >> b) & 1u;
-                const uint64_t m = __ballot(bit);
-                peers &= bit ? m : ~m;
+            for (int r = 0; r < RMAX; ++r) {
+                if (uint32_t(r) >= rounds) break;
+                const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+                const bool active = i < count;
+                const uint32_t dig = active ? digit_of(key[r], sh, mask) : 0u;
+                uint64_t peers = __ballot(active);
+                for (int b = 0; b < bits; ++b) {
+                    const bool bit = (dig >> b) & 1u;
+                    const uint64_t m = __ballot(bit);
+                    peers &= bit ? m : ~m;
+                }
+                const uint32_t rk = __popcll(peers & lt);
+                uint32_t before = 0;
+                if (active) before = my[dig];
+                if (active && rk == 0) my[dig] = before + __popcll(peers);
+                dr[r] = (dig << 16) | (before + rk);
             }
-            const uint32_t rk = __popcll(peers & lt);
-            uint32_t before = 0;
-            if (active) before = my[dig];
-            if (active && rk == 0) my[dig] = before + __popcll(peers);
-            dr[r] = (dig << 16) | (before + rk);
-        }
-        __syncthreads();
-        {
-            // digit-major, wave-minor bases; thread t owns digits DPT t .. DPT t + DPT - 1
-            constexpr uint32_t DPT = THREADS >= 256 ? 1u : 256u / THREADS;
-            uint32_t tot = 0;
+            __syncthreads();
+            {
+                // digit-major, wave-minor bases; thread t owns digits DPT t .. DPT t + DPT - 1
+                constexpr uint32_t DPT = THREADS >= 256 ? 1u : 256u / THREADS;
+                uint32_t tot = 0;
 #pragma unroll
-            for (uint32_t j = 0; j < DPT; ++j) {
-                const uint32_t d = threadIdx.x * DPT + j;
-                if (d < 256)
-                    for (int w = 0; w < WAVES; ++w) tot += wcnt[w * 256 + d];
+                for (uint32_t j = 0; j < DPT; ++j) {
+                    const uint32_t d = threadIdx.x * DPT + j;
+                    if (d < 256)
+                        for (int w = 0; w < WAVES; ++w) tot += wcnt[w * 256 + d];
+                }
+                uint32_t all;
+                uint32_t run = block_exclusive_scan<uint32_t>(tot, s_tmp, all);
+#pragma unroll
+                for (uint32_t j = 0; j < DPT; ++j) {
+                    const uint32_t d = threadIdx.x * DPT + j;
+                    if (d < 256)
+                        for (int w = 0; w < WAVES; ++w) {
+                            const uint32_t c = wcnt[w * 256 + d];
+                            wcnt[w * 256 + d] = run;
+                            run += c;
+                        }
+                }
             }
-            uint32_t all;
-            uint32_t run = block_exclusive_scan<uint32_t>(tot, s_tmp, all);
+            __syncthreads();
 #pragma unroll
-            for (uint32_t j = 0; j < DPT; ++j) {
-                const uint32_t d = threadIdx.x * DPT + j;
-                if (d < 256)
-                    for (int w = 0; w < WAVES; ++w) {
-                        const uint32_t c = wcnt[w * 256 + d];
-                        wcnt[w * 256 + d] = run;
-                        run += c;
-                    }
+            for (int r = 0; r < RMAX; ++r) {
+                if (uint32_t(r) >= rounds) break;
+                const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+                if (i < count) {
+                    const uint32_t pos = my[dr[r] >> 16] + (dr[r] & 0xFFFFu);
+                    s_key[pos] = key[r];
+                    s_idx[pos] = uint16_t(idx[r]);
+                }
             }
-        }
-        __syncthreads();
+            __syncthreads();
 #pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-            if (uint32_t(r) >= rounds) break;
-            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
-            if (i < count) {
-                const uint32_t pos = my[dr[r] >> 16] + (dr[r] & 0xFFFFu);
-                s_key[pos] = key[r];
-                if constexpr (!PACK) s_idx[pos] = uint16_t(idx[r]);
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < RMAX; ++r) {
-            if (uint32_t(r) >= rounds) break;
-            const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
-            if (i < count) {
-                key[r] = s_key[i];
-                if constexpr (!PACK) idx[r] = s_idx[i];
+            for (int r = 0; r < RMAX; ++r) {
+                if (uint32_t(r) >= rounds) break;
+                const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
+                if (i < count) {
+                    key[r] = s_key[i];
+                    idx[r] = s_idx[i];
+                }
             }
         }
     }
@@ -444,7 +533,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     for (int r = 0; r < RMAX; ++r) {
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (uint32_t(r) < rounds && i < count) {
-            if constexpr (PACK) idx[r] = uint32_t(key[r]) & ((1u << PACK_IB) - 1);
+            if constexpr (PACK) idx[r] = min(uint32_t(key[r]) & ((1u << PACK_IB) - 1), count - 1);  // in the cell whatever LDS held
             val[r] = vals[start + idx[r]];
         }
     }
