@@ -304,30 +304,39 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
 constexpr int PACK_IB = 13;
 template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 inline size_t cell_sort_lds() {
-    if (PACK) return size_t(CAP) * sizeof(uint64_t);  // the words only (merge sort)
+    if (PACK) return size_t(CAP + THREADS) * sizeof(uint64_t);  // the words, spread (mslot)
     return size_t(CAP) * (sizeof(K) + sizeof(uint16_t)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
 }
 
-// Sorts the distinct words s[0, count) in LDS ascending (count <= CAP):
-// each thread sorts CAP / THREADS consecutive words in registers by a
-// bitonic network, then runs are merged pairwise, each thread producing
-// CAP / THREADS outputs of a merge after finding where its outputs start
-// by a binary search along the merge path.  The tail up to a multiple of
-// CAP / THREADS is padded with ~0 (equal words are only ~0s, and which copy
-// of a value goes first does not change the values).  Ends with a barrier.
+// LDS slot of word i of a merge-sorted cell: one spare word after every
+// `IPT` words, so that a wave whose lanes each touch their own IPT
+// consecutive words spreads them over the banks instead of landing on one.
+template <uint32_t IPT>
+__device__ __forceinline__ uint32_t mslot(uint32_t i) {
+    return i + i / IPT;
+}
+
+// Sorts the distinct words s[mslot(0 .. count - 1)] in LDS ascending
+// (count <= CAP): each thread sorts IPT = CAP / THREADS consecutive words in
+// registers by a bitonic network, then runs are merged pairwise, each thread
+// producing IPT outputs of a merge after finding where its outputs start by
+// a binary search along the merge path.  The tail up to a multiple of IPT is
+// padded with ~0 (equal words are only ~0s, and which copy of a value goes
+// first does not change the values).  Ends with a barrier.
 template <uint32_t CAP, int THREADS>
 __device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
     constexpr uint32_t IPT = CAP / THREADS;
     static_assert(IPT >= 2 && (IPT & (IPT - 1)) == 0 && IPT * THREADS == CAP, "items per thread");
     const uint32_t n_pad = (count + IPT - 1) / IPT * IPT;
-    for (uint32_t i = count + threadIdx.x; i < n_pad; i += THREADS) s[i] = ~0ull;
+    for (uint32_t i = count + threadIdx.x; i < n_pad; i += THREADS) s[mslot<IPT>(i)] = ~0ull;
     __syncthreads();
     const uint32_t out0 = threadIdx.x * IPT;
     const bool act = out0 < n_pad;
+    uint64_t* const mine = s + mslot<IPT>(out0);  // this thread's IPT words, contiguous
     uint64_t r[IPT];
     if (act) {
 #pragma unroll
-        for (uint32_t j = 0; j < IPT; ++j) r[j] = s[out0 + j];
+        for (uint32_t j = 0; j < IPT; ++j) r[j] = mine[j];
 #pragma unroll
         for (uint32_t k = 2; k <= IPT; k <<= 1)
 #pragma unroll
@@ -343,28 +352,28 @@ __device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
                     }
                 }
 #pragma unroll
-        for (uint32_t j = 0; j < IPT; ++j) s[out0 + j] = r[j];  // this thread's own words only
+        for (uint32_t j = 0; j < IPT; ++j) mine[j] = r[j];  // this thread's own words only
     }
     __syncthreads();
     for (uint32_t L = IPT; L < n_pad; L <<= 1) {
         if (act) {
-            // run A = s[p0, p0 + la), run B = s[p0 + L, p0 + L + lb); this
-            // thread writes outputs d .. d + IPT - 1 of their merge
+            // run A = words [p0, p0 + la), run B = words [p0 + L, p0 + L + lb);
+            // this thread writes outputs d .. d + IPT - 1 of their merge
             const uint32_t p0 = out0 & ~(2 * L - 1), d = out0 - p0;
             const uint32_t la = min(L, n_pad - p0);
             const uint32_t lb = n_pad - p0 > L ? min(L, n_pad - p0 - L) : 0u;
-            const uint64_t* A = s + p0;
-            const uint64_t* Bw = s + p0 + L;
+            const uint32_t pb = p0 + L;
             uint32_t lo = d > lb ? d - lb : 0u, hi = min(d, la);
             while (lo < hi) {  // the number of A's words among the first d outputs
                 const uint32_t mid = (lo + hi) >> 1;
-                if (A[mid] < Bw[d - 1 - mid])
+                if (s[mslot<IPT>(p0 + mid)] < s[mslot<IPT>(pb + d - 1 - mid)])
                     lo = mid + 1;
                 else
                     hi = mid;
             }
             uint32_t ia = lo, ib = d - lo;
-            uint64_t a = ia < la ? A[ia] : ~0ull, b = ib < lb ? Bw[ib] : ~0ull;
+            uint64_t a = ia < la ? s[mslot<IPT>(p0 + ia)] : ~0ull;
+            uint64_t b = ib < lb ? s[mslot<IPT>(pb + ib)] : ~0ull;
 #pragma unroll
             for (uint32_t j = 0; j < IPT; ++j) {
                 const bool ta = ib >= lb || (ia < la && a <= b);
@@ -372,8 +381,7 @@ __device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
                 ia += ta ? 1u : 0u;
                 ib += ta ? 0u : 1u;
                 const bool ok = ta ? ia < la : ib < lb;
-                const uint64_t* p = ta ? A + ia : Bw + ib;
-                const uint64_t v = ok ? *p : ~0ull;
+                const uint64_t v = ok ? s[mslot<IPT>(ta ? p0 + ia : pb + ib)] : ~0ull;
                 a = ta ? v : a;
                 b = ta ? b : v;
             }
@@ -381,7 +389,7 @@ __device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
         __syncthreads();
         if (act) {
 #pragma unroll
-            for (uint32_t j = 0; j < IPT; ++j) s[out0 + j] = r[j];
+            for (uint32_t j = 0; j < IPT; ++j) mine[j] = r[j];
         }
         __syncthreads();
     }
@@ -404,6 +412,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
                                               uint32_t item, uint64_t* sm64) {
     constexpr int WAVES = THREADS / 64;
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
+    constexpr uint32_t IPT = CAP / THREADS;  // packed: words per thread of the merge sort
     static_assert(CAP <= 65536 && CAP % 2 == 0, "item indices are 16-bit");
     static_assert(!PACK || (std::is_same<K, uint64_t>::value && CAP <= (1u << PACK_IB)), "packed items");
     // keys and their positions in the cell move through LDS; each value is
@@ -438,7 +447,8 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
         // distinct words (the position breaks ties): merge-sorted in LDS,
         // then read in the layout the epilogue uses
         hi = keys[start] & ~low;
-        for (uint32_t i = threadIdx.x; i < count; i += THREADS) s_key[i] = ((keys[start + i] & low) << PACK_IB) | i;
+        for (uint32_t i = threadIdx.x; i < count; i += THREADS)
+            s_key[mslot<IPT>(i)] = ((keys[start + i] & low) << PACK_IB) | i;
         if (end_bit > 0)
             merge_sort_words<CAP, THREADS>(s_key, count);
         else
@@ -446,7 +456,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
 #pragma unroll
         for (int r = 0; r < RMAX; ++r) {
             const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
-            if (uint32_t(r) < rounds && i < count) key[r] = s_key[i];
+            if (uint32_t(r) < rounds && i < count) key[r] = s_key[mslot<IPT>(i)];
         }
     } else {
 #pragma unroll
@@ -557,7 +567,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     for (uint32_t i = threadIdx.x; i < count; i += THREADS) {
         bool differs;
         if constexpr (PACK)
-            differs = (s_key[i] >> PACK_IB) != (s_key[i - (i > 0)] >> PACK_IB);
+            differs = (s_key[mslot<IPT>(i)] >> PACK_IB) != (s_key[mslot<IPT>(i - (i > 0))] >> PACK_IB);
         else
             differs = !KOps<K>::eq(s_key[i], s_key[i - (i > 0)]);
         heads[start + i] = (i == 0 || (sorted_in_lds && differs)) ? 1 : 0;
@@ -633,7 +643,7 @@ inline uint32_t cell_sort_resident() {
 // the two LDS classes: (capacity, threads) per key width
 template <class K>
 struct CellClasses {
-    static constexpr uint32_t TINY = 512, SMALL = 2048, LARGE = 8192;
+    static constexpr uint32_t TINY = 1024, SMALL = 2048, LARGE = 8192;
 };
 template <>
 struct CellClasses<Key128> {
