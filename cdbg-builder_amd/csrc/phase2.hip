@@ -297,16 +297,54 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
 // THREADS threads and capacity CAP takes the listed cells (or all cells)
 // with more than `lo` items; cells above CAP go to the next class's list.
 // ---------------------------------------------------------------------------
-// PACK (64-bit keys whose unsorted bits and a 13-bit position fit one word):
+// PACK (keys whose unsorted bits and a 13-bit position fit one key word):
 // each item is the word (key bits below end_bit) << 13 | position, distinct
 // within the cell, so sorting the words is the stable sort of the keys; they
 // are merge-sorted in LDS instead (the bits above end_bit are the cell's own).
 constexpr int PACK_IB = 13;
 template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 inline size_t cell_sort_lds() {
-    if (PACK) return size_t(CAP + THREADS) * sizeof(uint64_t);  // the words, spread (mslot)
+    if (PACK) return size_t(CAP + THREADS) * sizeof(K);  // the words, spread (mslot)
     return size_t(CAP) * (sizeof(K) + sizeof(uint16_t)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
 }
+// whether a cell sort over the bits below end_bit packs its items
+template <class K>
+inline bool cell_packed(int end_bit) {
+    return end_bit + PACK_IB <= int(8 * sizeof(K));
+}
+
+// Packed words of 64- and 128-bit keys: order by <, selection word by word.
+__device__ __forceinline__ bool w_lt(uint64_t a, uint64_t b) { return a < b; }
+__device__ __forceinline__ bool w_lt(const Key128& a, const Key128& b) { return a < b; }
+__device__ __forceinline__ uint64_t w_sel(bool c, uint64_t a, uint64_t b) { return c ? a : b; }
+__device__ __forceinline__ Key128 w_sel(bool c, const Key128& a, const Key128& b) {
+    return KOps<Key128>::select(c, a, b);
+}
+template <class K>
+__device__ __forceinline__ K w_max();
+template <>
+__device__ __forceinline__ uint64_t w_max<uint64_t>() { return ~0ull; }
+template <>
+__device__ __forceinline__ Key128 w_max<Key128>() { return Key128{~0ull, ~0ull}; }
+// bits [0, b) set, b < 128
+__device__ __forceinline__ uint64_t w_low(uint64_t, int b) { return b >= 64 ? ~0ull : (uint64_t(1) << b) - 1; }
+__device__ __forceinline__ Key128 w_low(const Key128&, int b) {
+    return b >= 64 ? Key128{~0ull, (uint64_t(1) << (b - 64)) - 1} : Key128{(uint64_t(1) << b) - 1, 0};
+}
+__device__ __forceinline__ uint64_t w_pack(uint64_t key, uint64_t low, uint32_t i) { return ((key & low) << PACK_IB) | i; }
+__device__ __forceinline__ Key128 w_pack(const Key128& key, const Key128& low, uint32_t i) {
+    Key128 w = KOps<Key128>::shl(Key128{key.lo & low.lo, key.hi & low.hi}, PACK_IB);
+    w.lo |= i;
+    return w;
+}
+__device__ __forceinline__ uint64_t w_key_bits(uint64_t w) { return w >> PACK_IB; }
+__device__ __forceinline__ Key128 w_key_bits(const Key128& w) { return KOps<Key128>::shr(w, PACK_IB); }
+__device__ __forceinline__ uint32_t w_pos(uint64_t w) { return uint32_t(w) & ((1u << PACK_IB) - 1); }
+__device__ __forceinline__ uint32_t w_pos(const Key128& w) { return uint32_t(w.lo) & ((1u << PACK_IB) - 1); }
+__device__ __forceinline__ uint64_t w_or(uint64_t a, uint64_t b) { return a | b; }
+__device__ __forceinline__ Key128 w_or(const Key128& a, const Key128& b) { return Key128{a.lo | b.lo, a.hi | b.hi}; }
+__device__ __forceinline__ uint64_t w_andn(uint64_t a, uint64_t m) { return a & ~m; }
+__device__ __forceinline__ Key128 w_andn(const Key128& a, const Key128& m) { return Key128{a.lo & ~m.lo, a.hi & ~m.hi}; }
 
 // LDS slot of word i of a merge-sorted cell: one spare word after every
 // `IPT` words, so that a wave whose lanes each touch their own IPT
@@ -323,17 +361,18 @@ __device__ __forceinline__ uint32_t mslot(uint32_t i) {
 // a binary search along the merge path.  The tail up to a multiple of IPT is
 // padded with ~0 (equal words are only ~0s, and which copy of a value goes
 // first does not change the values).  Ends with a barrier.
-template <uint32_t CAP, int THREADS>
-__device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
+template <class W, uint32_t CAP, int THREADS>
+__device__ __forceinline__ void merge_sort_words(W* s, uint32_t count) {
     constexpr uint32_t IPT = CAP / THREADS;
     static_assert(IPT >= 2 && (IPT & (IPT - 1)) == 0 && IPT * THREADS == CAP, "items per thread");
+    const W top = w_max<W>();
     const uint32_t n_pad = (count + IPT - 1) / IPT * IPT;
-    for (uint32_t i = count + threadIdx.x; i < n_pad; i += THREADS) s[mslot<IPT>(i)] = ~0ull;
+    for (uint32_t i = count + threadIdx.x; i < n_pad; i += THREADS) s[mslot<IPT>(i)] = top;
     __syncthreads();
     const uint32_t out0 = threadIdx.x * IPT;
     const bool act = out0 < n_pad;
-    uint64_t* const mine = s + mslot<IPT>(out0);  // this thread's IPT words, contiguous
-    uint64_t r[IPT];
+    W* const mine = s + mslot<IPT>(out0);  // this thread's IPT words, contiguous
+    W r[IPT];
     if (act) {
 #pragma unroll
         for (uint32_t j = 0; j < IPT; ++j) r[j] = mine[j];
@@ -345,10 +384,10 @@ __device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
                 for (uint32_t i = 0; i < IPT; ++i) {
                     const uint32_t l = i ^ j;
                     if (l > i) {
-                        const uint64_t x = r[i], y = r[l];
-                        const bool sw = ((i & k) == 0) ? (x > y) : (x < y);
-                        r[i] = sw ? y : x;
-                        r[l] = sw ? x : y;
+                        const W x = r[i], y = r[l];
+                        const bool sw = ((i & k) == 0) ? w_lt(y, x) : w_lt(x, y);
+                        r[i] = w_sel(sw, y, x);
+                        r[l] = w_sel(sw, x, y);
                     }
                 }
 #pragma unroll
@@ -366,24 +405,24 @@ __device__ __forceinline__ void merge_sort_words(uint64_t* s, uint32_t count) {
             uint32_t lo = d > lb ? d - lb : 0u, hi = min(d, la);
             while (lo < hi) {  // the number of A's words among the first d outputs
                 const uint32_t mid = (lo + hi) >> 1;
-                if (s[mslot<IPT>(p0 + mid)] < s[mslot<IPT>(pb + d - 1 - mid)])
+                if (w_lt(s[mslot<IPT>(p0 + mid)], s[mslot<IPT>(pb + d - 1 - mid)]))
                     lo = mid + 1;
                 else
                     hi = mid;
             }
             uint32_t ia = lo, ib = d - lo;
-            uint64_t a = ia < la ? s[mslot<IPT>(p0 + ia)] : ~0ull;
-            uint64_t b = ib < lb ? s[mslot<IPT>(pb + ib)] : ~0ull;
+            W a = w_sel(ia < la, s[mslot<IPT>(ia < la ? p0 + ia : 0u)], top);
+            W b = w_sel(ib < lb, s[mslot<IPT>(ib < lb ? pb + ib : 0u)], top);
 #pragma unroll
             for (uint32_t j = 0; j < IPT; ++j) {
-                const bool ta = ib >= lb || (ia < la && a <= b);
-                r[j] = ta ? a : b;
+                const bool ta = ib >= lb || (ia < la && !w_lt(b, a));
+                r[j] = w_sel(ta, a, b);
                 ia += ta ? 1u : 0u;
                 ib += ta ? 0u : 1u;
                 const bool ok = ta ? ia < la : ib < lb;
-                const uint64_t v = ok ? s[mslot<IPT>(ta ? p0 + ia : pb + ib)] : ~0ull;
-                a = ta ? v : a;
-                b = ta ? b : v;
+                const W v = w_sel(ok, s[mslot<IPT>(ok ? (ta ? p0 + ia : pb + ib) : 0u)], top);
+                a = w_sel(ta, v, a);
+                b = w_sel(ta, b, v);
             }
         }
         __syncthreads();
@@ -414,10 +453,10 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
     constexpr uint32_t IPT = CAP / THREADS;  // packed: words per thread of the merge sort
     static_assert(CAP <= 65536 && CAP % 2 == 0, "item indices are 16-bit");
-    static_assert(!PACK || (std::is_same<K, uint64_t>::value && CAP <= (1u << PACK_IB)), "packed items");
-    // keys and their positions in the cell move through LDS; each value is
-    // fetched once at the end from its original position
-    using E = typename std::conditional<PACK, uint64_t, K>::type;  // what moves per item
+    static_assert(!PACK || CAP <= (1u << PACK_IB), "packed items");
+    // keys (packed: words) and their positions in the cell move through LDS;
+    // each value is fetched once at the end from its original position
+    using E = K;  // what moves per item
     E* s_key = (E*)sm64;
     uint16_t* s_idx = (uint16_t*)(s_key + CAP);
     uint32_t* wcnt = (uint32_t*)(PACK ? (void*)(s_key + CAP) : (void*)(s_idx + CAP));  // [WAVES][256]
@@ -441,16 +480,15 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     E key[RMAX];
     uint32_t idx[RMAX];
     uint32_t dr[RMAX];
-    const uint64_t low = PACK ? (uint64_t(1) << end_bit) - 1 : 0;  // end_bit < 64 when packed
-    uint64_t hi = 0;  // packed: the cell's bits above end_bit
+    K hi{};  // packed: the cell's bits above end_bit
     if constexpr (PACK) {
         // distinct words (the position breaks ties): merge-sorted in LDS,
         // then read in the layout the epilogue uses
-        hi = keys[start] & ~low;
-        for (uint32_t i = threadIdx.x; i < count; i += THREADS)
-            s_key[mslot<IPT>(i)] = ((keys[start + i] & low) << PACK_IB) | i;
+        const K low = w_low(K{}, end_bit);  // end_bit + 13 fits the word
+        hi = w_andn(keys[start], low);
+        for (uint32_t i = threadIdx.x; i < count; i += THREADS) s_key[mslot<IPT>(i)] = w_pack(keys[start + i], low, i);
         if (end_bit > 0)
-            merge_sort_words<CAP, THREADS>(s_key, count);
+            merge_sort_words<K, CAP, THREADS>(s_key, count);
         else
             __syncthreads();
 #pragma unroll
@@ -543,7 +581,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     for (int r = 0; r < RMAX; ++r) {
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (uint32_t(r) < rounds && i < count) {
-            if constexpr (PACK) idx[r] = min(uint32_t(key[r]) & ((1u << PACK_IB) - 1), count - 1);  // in the cell whatever LDS held
+            if constexpr (PACK) idx[r] = min(w_pos(key[r]), count - 1);  // in the cell whatever LDS held
             val[r] = vals[start + idx[r]];
         }
     }
@@ -554,7 +592,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (i < count) {
             if constexpr (PACK)
-                keys[start + i] = hi | (key[r] >> PACK_IB);
+                keys[start + i] = w_or(hi, w_key_bits(key[r]));
             else
                 keys[start + i] = key[r];
             vals[start + i] = val[r];
@@ -567,7 +605,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     for (uint32_t i = threadIdx.x; i < count; i += THREADS) {
         bool differs;
         if constexpr (PACK)
-            differs = (s_key[mslot<IPT>(i)] >> PACK_IB) != (s_key[mslot<IPT>(i - (i > 0))] >> PACK_IB);
+            differs = !KOps<K>::eq(w_key_bits(s_key[mslot<IPT>(i)]), w_key_bits(s_key[mslot<IPT>(i - (i > 0))]));
         else
             differs = !KOps<K>::eq(s_key[i], s_key[i - (i > 0)]);
         heads[start + i] = (i == 0 || (sorted_in_lds && differs)) ? 1 : 0;
@@ -655,12 +693,10 @@ struct CellClasses<Key128> {
 // Sets up a class's kernels outside any graph capture.
 template <class K, uint32_t CAP, int THREADS>
 static void cell_sort_warm(int end_bit) {
-    if constexpr (std::is_same<K, uint64_t>::value) {
-        if (end_bit + PACK_IB <= 64) {
-            cell_sort_prepare<K, uint64_t, CAP, THREADS, true>();
-            (void)cell_sort_resident<K, uint64_t, CAP, THREADS, true>();
-            return;
-        }
+    if (cell_packed<K>(end_bit)) {
+        cell_sort_prepare<K, uint64_t, CAP, THREADS, true>();
+        (void)cell_sort_resident<K, uint64_t, CAP, THREADS, true>();
+        return;
     }
     cell_sort_prepare<K, uint64_t, CAP, THREADS, false>();
     (void)cell_sort_resident<K, uint64_t, CAP, THREADS, false>();
@@ -686,11 +722,9 @@ template <class K, uint32_t CAP, int THREADS>
 static void launch_cell_sort(hipStream_t st, K* keys, uint64_t* vals, const CellSortArgs& a, uint32_t n_items,
                              const uint32_t* n32 = nullptr, const unsigned long long* n64 = nullptr,
                              unsigned long long* queue = nullptr) {
-    if constexpr (std::is_same<K, uint64_t>::value) {
-        if (a.end_bit + PACK_IB <= 64) {
-            launch_cell_class<K, CAP, THREADS, true>(st, keys, vals, a, n_items, n32, n64, queue);
-            return;
-        }
+    if (cell_packed<K>(a.end_bit)) {
+        launch_cell_class<K, CAP, THREADS, true>(st, keys, vals, a, n_items, n32, n64, queue);
+        return;
     }
     launch_cell_class<K, CAP, THREADS, false>(st, keys, vals, a, n_items, n32, n64, queue);
 }
@@ -2994,7 +3028,7 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         D_max = std::max(D_max, int(h_dbits[b]));
     }
     int D_floor = std::max(D_max - 8, 0);
-    if (std::is_same<K, uint64_t>::value && D_max >= kb - (64 - PACK_IB)) D_floor = std::max(D_floor, kb - (64 - PACK_IB));
+    if (D_max >= kb - int(8 * sizeof(K) - PACK_IB)) D_floor = std::max(D_floor, kb - int(8 * sizeof(K) - PACK_IB));
     int D_min = kb;
     std::vector<uint32_t> h_cbase(nb + 1, 0);
     for (uint32_t b = 0; b < nb; ++b) {
