@@ -137,6 +137,30 @@ def test_phase2_large_buckets_split_into_cells(gpu_ctx, k, m, B):
     _assert_same(*_run_both(gpu_ctx, data, off, k, m, B))
 
 
+@pytest.mark.parametrize("k,m,cell_target,persistent", [
+    (31, 19, 300, 0),   # 11 bits in the global passes: the cells sort 62 - 11 = 51 key bits + 13 position bits
+    (31, 19, 64, 1),    # 13 global bits: 49 key bits
+    (63, 27, 300, 0),   # 126 - 11 = 115 key bits + 13 position bits: a full 128-bit word
+    (63, 27, 64, 1),
+])
+def test_phase2_packed_cells_at_widest_key_bits(gpu_ctx, k, m, cell_target, persistent):
+    """One bucket of ~3.6e5 occurrences with small cells: the cells' unsorted
+    key bits and the 13-bit position fit one key word, so the cells are
+    merge-sorted as packed words, up to the widest case (51 bits of a 64-bit
+    key, 115 of a 128-bit one).  Smaller buckets leave more bits to the cells
+    and take the ranking passes instead."""
+    data, off = G.synth(seed=21, num_colors=6, genome_len=60000, snp_rate=0.01, indel_rate=0.001)
+    ob = O.Build(data, off, k, m, 1, threads=4)
+    with gpu_ctx.job(data, off, k, m, 1) as job:
+        job.set_tuning(cell_target=cell_target, persistent=persistent)
+        job.run()
+        s = job.stats()
+        gf = job.frags()
+        gw, go = job.color_sets()
+    assert 2 ** 10 * cell_target < s["num_occ"] <= 2 ** 11 * cell_target or cell_target == 64
+    _assert_same(ob, s, gf, gw, go)
+
+
 @pytest.mark.parametrize("k,m,B", [(31, 19, 8), (63, 27, 4), (21, 11, 16)])
 def test_phase2_heavy_kmers(gpu_ctx, k, m, B):
     """One k-mer repeated far beyond what a cell holds in LDS (homopolymer
