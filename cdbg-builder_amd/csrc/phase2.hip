@@ -297,20 +297,28 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
 // THREADS threads and capacity CAP takes the listed cells (or all cells)
 // with more than `lo` items; cells above CAP go to the next class's list.
 // ---------------------------------------------------------------------------
-// PACK (keys whose unsorted bits and a 13-bit position fit one key word):
-// each item is the word (key bits below end_bit) << 13 | position, distinct
-// within the cell, so sorting the words is the stable sort of the keys; they
-// are merge-sorted in LDS instead (the bits above end_bit are the cell's own).
+// PACK (keys whose unsorted bits and a position of log2(CAP) <= 13 bits fit
+// one key word): each item is the word (key bits below end_bit) << bits |
+// position, distinct within the cell, so sorting the words is the stable
+// sort of the keys; they are merge-sorted in LDS instead (the bits above
+// end_bit are the cell's own).
 constexpr int PACK_IB = 13;
 template <class K, class V, uint32_t CAP, int THREADS, bool PACK = false>
 inline size_t cell_sort_lds() {
     if (PACK) return size_t(CAP + THREADS) * sizeof(K);  // the words, spread (mslot)
     return size_t(CAP) * (sizeof(K) + sizeof(uint16_t)) + ((THREADS / 64) * 256 + THREADS / 64 + 1) * sizeof(uint32_t);
 }
-// whether a cell sort over the bits below end_bit packs its items
-template <class K>
+// position bits of a packed item in a class of capacity CAP (at most 13)
+template <uint32_t CAP>
+constexpr int pos_bits() {
+    int b = 0;
+    while ((1u << b) < CAP) ++b;
+    return b;
+}
+// whether a class of capacity CAP sorting the bits below end_bit packs its items
+template <class K, uint32_t CAP>
 inline bool cell_packed(int end_bit) {
-    return end_bit + PACK_IB <= int(8 * sizeof(K));
+    return end_bit + pos_bits<CAP>() <= int(8 * sizeof(K));
 }
 
 // Packed words of 64- and 128-bit keys: order by <, selection word by word.
@@ -331,16 +339,22 @@ __device__ __forceinline__ uint64_t w_low(uint64_t, int b) { return b >= 64 ? ~0
 __device__ __forceinline__ Key128 w_low(const Key128&, int b) {
     return b >= 64 ? Key128{~0ull, (uint64_t(1) << (b - 64)) - 1} : Key128{(uint64_t(1) << b) - 1, 0};
 }
-__device__ __forceinline__ uint64_t w_pack(uint64_t key, uint64_t low, uint32_t i) { return ((key & low) << PACK_IB) | i; }
+template <int PB>
+__device__ __forceinline__ uint64_t w_pack(uint64_t key, uint64_t low, uint32_t i) { return ((key & low) << PB) | i; }
+template <int PB>
 __device__ __forceinline__ Key128 w_pack(const Key128& key, const Key128& low, uint32_t i) {
-    Key128 w = KOps<Key128>::shl(Key128{key.lo & low.lo, key.hi & low.hi}, PACK_IB);
+    Key128 w = KOps<Key128>::shl(Key128{key.lo & low.lo, key.hi & low.hi}, PB);
     w.lo |= i;
     return w;
 }
-__device__ __forceinline__ uint64_t w_key_bits(uint64_t w) { return w >> PACK_IB; }
-__device__ __forceinline__ Key128 w_key_bits(const Key128& w) { return KOps<Key128>::shr(w, PACK_IB); }
-__device__ __forceinline__ uint32_t w_pos(uint64_t w) { return uint32_t(w) & ((1u << PACK_IB) - 1); }
-__device__ __forceinline__ uint32_t w_pos(const Key128& w) { return uint32_t(w.lo) & ((1u << PACK_IB) - 1); }
+template <int PB>
+__device__ __forceinline__ uint64_t w_key_bits(uint64_t w) { return w >> PB; }
+template <int PB>
+__device__ __forceinline__ Key128 w_key_bits(const Key128& w) { return KOps<Key128>::shr(w, PB); }
+template <int PB>
+__device__ __forceinline__ uint32_t w_pos(uint64_t w) { return uint32_t(w) & ((1u << PB) - 1); }
+template <int PB>
+__device__ __forceinline__ uint32_t w_pos(const Key128& w) { return uint32_t(w.lo) & ((1u << PB) - 1); }
 __device__ __forceinline__ uint64_t w_or(uint64_t a, uint64_t b) { return a | b; }
 __device__ __forceinline__ Key128 w_or(const Key128& a, const Key128& b) { return Key128{a.lo | b.lo, a.hi | b.hi}; }
 __device__ __forceinline__ uint64_t w_andn(uint64_t a, uint64_t m) { return a & ~m; }
@@ -453,7 +467,8 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     constexpr int RMAX = CAP / THREADS;  // rounds per wave at most
     constexpr uint32_t IPT = CAP / THREADS;  // packed: words per thread of the merge sort
     static_assert(CAP <= 65536 && CAP % 2 == 0, "item indices are 16-bit");
-    static_assert(!PACK || CAP <= (1u << PACK_IB), "packed items");
+    constexpr int PB = pos_bits<CAP>();  // packed: position bits
+    static_assert(!PACK || (PB <= PACK_IB && CAP == (1u << PB)), "packed items");
     // keys (packed: words) and their positions in the cell move through LDS;
     // each value is fetched once at the end from its original position
     using E = K;  // what moves per item
@@ -486,7 +501,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
         // then read in the layout the epilogue uses
         const K low = w_low(K{}, end_bit);  // end_bit + 13 fits the word
         hi = w_andn(keys[start], low);
-        for (uint32_t i = threadIdx.x; i < count; i += THREADS) s_key[mslot<IPT>(i)] = w_pack(keys[start + i], low, i);
+        for (uint32_t i = threadIdx.x; i < count; i += THREADS) s_key[mslot<IPT>(i)] = w_pack<PB>(keys[start + i], low, i);
         if (end_bit > 0)
             merge_sort_words<K, CAP, THREADS>(s_key, count);
         else
@@ -581,7 +596,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     for (int r = 0; r < RMAX; ++r) {
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (uint32_t(r) < rounds && i < count) {
-            if constexpr (PACK) idx[r] = min(w_pos(key[r]), count - 1);  // in the cell whatever LDS held
+            if constexpr (PACK) idx[r] = min(w_pos<PB>(key[r]), count - 1);  // in the cell whatever LDS held
             val[r] = vals[start + idx[r]];
         }
     }
@@ -592,7 +607,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
         const uint32_t i = wbase + uint32_t(r) * 64 + uint32_t(lane);
         if (i < count) {
             if constexpr (PACK)
-                keys[start + i] = w_or(hi, w_key_bits(key[r]));
+                keys[start + i] = w_or(hi, w_key_bits<PB>(key[r]));
             else
                 keys[start + i] = key[r];
             vals[start + i] = val[r];
@@ -605,7 +620,7 @@ __device__ __forceinline__ void cell_sort_one(K* __restrict__ keys, V* __restric
     for (uint32_t i = threadIdx.x; i < count; i += THREADS) {
         bool differs;
         if constexpr (PACK)
-            differs = !KOps<K>::eq(w_key_bits(s_key[mslot<IPT>(i)]), w_key_bits(s_key[mslot<IPT>(i - (i > 0))]));
+            differs = !KOps<K>::eq(w_key_bits<PB>(s_key[mslot<IPT>(i)]), w_key_bits<PB>(s_key[mslot<IPT>(i - (i > 0))]));
         else
             differs = !KOps<K>::eq(s_key[i], s_key[i - (i > 0)]);
         heads[start + i] = (i == 0 || (sorted_in_lds && differs)) ? 1 : 0;
@@ -693,7 +708,7 @@ struct CellClasses<Key128> {
 // Sets up a class's kernels outside any graph capture.
 template <class K, uint32_t CAP, int THREADS>
 static void cell_sort_warm(int end_bit) {
-    if (cell_packed<K>(end_bit)) {
+    if (cell_packed<K, CAP>(end_bit)) {
         cell_sort_prepare<K, uint64_t, CAP, THREADS, true>();
         (void)cell_sort_resident<K, uint64_t, CAP, THREADS, true>();
         return;
@@ -722,7 +737,7 @@ template <class K, uint32_t CAP, int THREADS>
 static void launch_cell_sort(hipStream_t st, K* keys, uint64_t* vals, const CellSortArgs& a, uint32_t n_items,
                              const uint32_t* n32 = nullptr, const unsigned long long* n64 = nullptr,
                              unsigned long long* queue = nullptr) {
-    if (cell_packed<K>(a.end_bit)) {
+    if (cell_packed<K, CAP>(a.end_bit)) {
         launch_cell_class<K, CAP, THREADS, true>(st, keys, vals, a, n_items, n32, n64, queue);
         return;
     }

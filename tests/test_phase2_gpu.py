@@ -138,17 +138,20 @@ def test_phase2_large_buckets_split_into_cells(gpu_ctx, k, m, B):
 
 
 @pytest.mark.parametrize("k,m,cell_target,persistent", [
-    (31, 19, 300, 0),   # 11 bits in the global passes: the cells sort 62 - 11 = 51 key bits + 13 position bits
+    (31, 19, 300, 0),   # 11 bits in the global passes: every class packs (51 key bits + 13 position bits)
     (31, 19, 64, 1),    # 13 global bits: 49 key bits
+    (31, 19, 1200, 0),  # 9 global bits: 53 key bits + 11 position bits fill the 2048-key class's word
+    (31, 19, 2400, 1),  # 8 global bits: 54 + 10 fill the 1024-key class's word; larger cells rank
     (63, 27, 300, 0),   # 126 - 11 = 115 key bits + 13 position bits: a full 128-bit word
     (63, 27, 64, 1),
+    (63, 27, 2400, 0),  # 118 key bits: the 512- and 1024-key classes pack (9 and 10 position bits)
 ])
 def test_phase2_packed_cells_at_widest_key_bits(gpu_ctx, k, m, cell_target, persistent):
-    """One bucket of ~3.6e5 occurrences with small cells: the cells' unsorted
-    key bits and the 13-bit position fit one key word, so the cells are
-    merge-sorted as packed words, up to the widest case (51 bits of a 64-bit
-    key, 115 of a 128-bit one).  Smaller buckets leave more bits to the cells
-    and take the ranking passes instead."""
+    """One bucket of ~3.6e5 occurrences with small cells: a cell's unsorted
+    key bits and its position (log2 of the class's capacity in bits) fit one
+    key word, so the cells are merge-sorted as packed words, up to words
+    filled to the last bit.  Cells whose bits do not fit take the ranking
+    passes."""
     data, off = G.synth(seed=21, num_colors=6, genome_len=60000, snp_rate=0.01, indel_rate=0.001)
     ob = O.Build(data, off, k, m, 1, threads=4)
     with gpu_ctx.job(data, off, k, m, 1) as job:
@@ -157,7 +160,8 @@ def test_phase2_packed_cells_at_widest_key_bits(gpu_ctx, k, m, cell_target, pers
         s = job.stats()
         gf = job.frags()
         gw, go = job.color_sets()
-    assert 2 ** 10 * cell_target < s["num_occ"] <= 2 ** 11 * cell_target or cell_target == 64
+    d = {300: 11, 1200: 9, 2400: 8}.get(cell_target)  # the global bits these sizes give
+    assert d is None or 2 ** (d - 1) * cell_target < s["num_occ"] <= 2 ** d * cell_target
     _assert_same(ob, s, gf, gw, go)
 
 
