@@ -3043,7 +3043,21 @@ static void run_chunk(Job& J, Phase2Bufs& P, uint32_t b0, uint32_t b1, const std
         D_max = std::max(D_max, int(h_dbits[b]));
     }
     int D_floor = std::max(D_max - 8, 0);
-    if (D_max >= kb - int(8 * sizeof(K) - PACK_IB)) D_floor = std::max(D_floor, kb - int(8 * sizeof(K) - PACK_IB));
+    // the cells sort the bits below the shallowest bucket's cut: raise the
+    // shallow buckets (up to the global passes' depth, at no extra pass) so
+    // that the most size classes can pack their items, largest class first
+    {
+        const int wbits = int(8 * sizeof(K));
+        const int pb[3] = {pos_bits<CellClasses<K>::LARGE>(), pos_bits<CellClasses<K>::SMALL>(),
+                           pos_bits<CellClasses<K>::TINY>()};
+        for (int p : pb) {
+            const int need = kb - (wbits - p);
+            if (D_max >= need) {
+                D_floor = std::max(D_floor, need);
+                break;
+            }
+        }
+    }
     int D_min = kb;
     std::vector<uint32_t> h_cbase(nb + 1, 0);
     for (uint32_t b = 0; b < nb; ++b) {
