@@ -290,10 +290,11 @@ __global__ __launch_bounds__(TPB) void k_expand(ExpArgs a, const uint32_t* __res
 // ---------------------------------------------------------------------------
 // Cell sort: the last stage of a stable sort whose top bits are already
 // sorted.  A cell (a run of keys equal above end_bit) is sorted by one
-// workgroup in LDS: LSD passes of 8-bit digits, each ranking the items with
-// the per-wave scheme of k_radix_scatter and placing them in LDS, keys and
-// their original positions held in registers between passes; the values
-// follow once, at the end.  Size classes: a launch of
+// workgroup in LDS: packed words merge-sorted (below), or, where key bits
+// and position do not fit one word, LSD passes of 8-bit digits, each ranking
+// the items with the per-wave scheme of k_radix_scatter and placing them in
+// LDS, keys and their original positions held in registers between passes;
+// the values follow once, at the end.  Size classes: a launch of
 // THREADS threads and capacity CAP takes the listed cells (or all cells)
 // with more than `lo` items; cells above CAP go to the next class's list.
 // ---------------------------------------------------------------------------
